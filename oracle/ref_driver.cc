@@ -1,0 +1,282 @@
+// C-ABI driver around the reference OSCARS C++ core (compiled from the
+// reference sources by oracle/Makefile into oracle/_ref/libref.so).
+//
+// Test infrastructure only: the tests, smoke() and bench.py's cpu_baseline
+// use it as the checker and as the CPU baseline.  Nothing in the product
+// (oscars_amd/) links or loads it.
+//
+// Every entry point mirrors what the reference Python binding does for the
+// same call, so the numbers are the ones a user of `oscars.sr` would get:
+//   set_particle_beam   src/OSCARSSR_Python.cc:3561-3575, 3699-4070
+//   add_bfield_undulator src/OSCARSSR_Python.cc:1309-1400
+//   add_bfield_epu       src/OSCARSSR_Python.cc:1479-1600
+//   add_bfield_file      src/OSCARSSR_Python.cc:526-700
+//   calculate_spectrum   src/OSCARSSR_Python.cc:5653-5928
+//   calculate_flux_rectangle          src/OSCARSSR_Python.cc:7977-8350
+//   calculate_power_density_rectangle src/OSCARSSR_Python.cc:6471-6797
+
+#include "OSCARSSR.h"
+#include "TField3D_IdealUndulator.h"
+#include "TField3D_IdealEPU.h"
+#include "TField3D_Grid.h"
+#include "TSurfacePoints_Rectangle.h"
+#include "TSpectrumContainer.h"
+#include "T3DScalarContainer.h"
+
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+namespace {
+thread_local std::string g_err;
+
+int fail(std::exception const& e) {
+  g_err = e.what();
+  return 1;
+}
+
+TVector3D v3(double const* p) { return TVector3D(p[0], p[1], p[2]); }
+
+TParticleA& current_particle(OSCARSSR* s) {
+  // GetCurrentParticle() returns a const reference to the (non-const) member
+  return const_cast<TParticleA&>(s->GetCurrentParticle());
+}
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+
+void* ref_new() { return new OSCARSSR(); }
+void ref_delete(void* h) { delete static_cast<OSCARSSR*>(h); }
+
+int ref_add_undulator(void* h, double const* bfield, double const* period, int nperiods,
+                      double const* translation, double phase, double taper) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    s->AddMagneticField((TField*) new TField3D_IdealUndulator(v3(bfield), v3(period), nperiods, v3(translation),
+                                                              phase, taper, 0, 0, 0, ""));
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+int ref_add_epu(void* h, double const* bfieldA, double const* bfieldB, double const* period, int nperiods,
+                double const* translation, double phase, double taper) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    s->AddMagneticField((TField*) new TField3D_IdealEPU(v3(bfieldA), v3(bfieldB), v3(period), nperiods,
+                                                        v3(translation), phase, taper, 0, 0, 0, ""));
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+int ref_add_grid_file(void* h, const char* path, double const* rotations, double const* translation) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    s->AddMagneticField(path, "OSCARS", v3(rotations), v3(translation), std::vector<double>(), 0, 0, 0, "");
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+// set_particle_beam(type='electron', energy_GeV=E, x0=..., d0=..., current=I)
+int ref_set_beam(void* h, double energy_GeV, double const* x0, double const* d0, double current) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    s->ClearParticleBeams();
+    TVector3D Direction = v3(d0);
+    TVector3D Horizontal = -Direction.Orthogonal();
+    Horizontal = Horizontal.UnitVector();
+    TParticleBeam* b = &(s->AddParticleBeam("electron", "", v3(x0), Direction, energy_GeV, 0, current, 1));
+    b->SetHorizontalDirection(Horizontal);
+    s->SetCTStartStop(-3, 3);
+    s->SetNewParticle("", "ideal");
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+// Replace the current particle's initial conditions (explicit multi-particle lists).
+int ref_set_particle(void* h, double const* x0, double const* beta0, double t0) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    TParticleA& P = current_particle(s);
+    P.SetInitialParticleConditions(v3(x0), v3(beta0), t0);
+    P.ResetTrajectoryData();
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+int ref_get_particle(void* h, double* out /* x0[3] b0[3] t0 gamma q m qovermgamma current */) {
+  OSCARSSR* s = static_cast<OSCARSSR*>(h);
+  TParticleA const& P = s->GetCurrentParticle();
+  out[0] = P.GetX0().GetX(); out[1] = P.GetX0().GetY(); out[2] = P.GetX0().GetZ();
+  out[3] = P.GetB0().GetX(); out[4] = P.GetB0().GetY(); out[5] = P.GetB0().GetZ();
+  out[6] = P.GetT0();
+  out[7] = P.GetGamma();
+  out[8] = P.GetQ();
+  out[9] = P.GetM();
+  out[10] = P.GetQoverMGamma();
+  out[11] = P.GetCurrent();
+  return 0;
+}
+
+void ref_set_ctstartstop(void* h, double a, double b) { static_cast<OSCARSSR*>(h)->SetCTStartStop(a, b); }
+void ref_set_npoints_trajectory(void* h, long n) { static_cast<OSCARSSR*>(h)->SetNPointsTrajectory((size_t) n); }
+
+int ref_get_bfield(void* h, double const* x, double* out) {
+  TVector3D B = static_cast<OSCARSSR*>(h)->GetB(v3(x), 0);
+  out[0] = B.GetX(); out[1] = B.GetY(); out[2] = B.GetZ();
+  return 0;
+}
+
+int ref_calculate_trajectory(void* h) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    s->CalculateTrajectory(current_particle(s));
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+long ref_trajectory_npoints(void* h) {
+  OSCARSSR* s = static_cast<OSCARSSR*>(h);
+  return (long) current_particle(s).GetTrajectory().GetNPoints();
+}
+
+// out: n rows of [t, x, y, z, bx, by, bz, ax, ay, az]
+int ref_trajectory(void* h, double* out) {
+  OSCARSSR* s = static_cast<OSCARSSR*>(h);
+  TParticleTrajectoryPoints& T = current_particle(s).GetTrajectory();
+  for (size_t i = 0; i != T.GetNPoints(); ++i) {
+    TParticleTrajectoryPoint const& p = T.GetPoint(i);
+    double* o = out + 10 * i;
+    o[0] = T.GetT(i);
+    o[1] = p.GetX().GetX(); o[2] = p.GetX().GetY(); o[3] = p.GetX().GetZ();
+    o[4] = p.GetB().GetX(); o[5] = p.GetB().GetY(); o[6] = p.GetB().GetZ();
+    o[7] = p.GetAoverC().GetX(); o[8] = p.GetAoverC().GetY(); o[9] = p.GetAoverC().GetZ();
+  }
+  return 0;
+}
+
+// Spline second derivatives: n rows of 9 values
+int ref_spline_ypp(void* h, double* out) {
+  OSCARSSR* s = static_cast<OSCARSSR*>(h);
+  TOMATH::TSpline1D3<TParticleTrajectoryPoint> const& S = current_particle(s).GetTrajectoryInterpolated().GetSpline();
+  for (size_t i = 0; i != S.GetNPoints(); ++i) {
+    TParticleTrajectoryPoint const& p = S.GetYPP((int) i);
+    double* o = out + 9 * i;
+    o[0] = p.GetX().GetX(); o[1] = p.GetX().GetY(); o[2] = p.GetX().GetZ();
+    o[3] = p.GetB().GetX(); o[4] = p.GetB().GetY(); o[5] = p.GetB().GetZ();
+    o[6] = p.GetAoverC().GetX(); o[7] = p.GetAoverC().GetY(); o[8] = p.GetAoverC().GetZ();
+  }
+  return 0;
+}
+
+// Level-L samples (2^L rows of 10 values, same layout as ref_trajectory)
+int ref_level(void* h, int level, double* out) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    TParticleTrajectoryPoints const& T = current_particle(s).GetTrajectoryLevel(level);
+    for (size_t i = 0; i != T.GetNPoints(); ++i) {
+      TParticleTrajectoryPoint const& p = T.GetPoint(i);
+      double* o = out + 10 * i;
+      o[0] = T.GetT(i);
+      o[1] = p.GetX().GetX(); o[2] = p.GetX().GetY(); o[3] = p.GetX().GetZ();
+      o[4] = p.GetB().GetX(); o[5] = p.GetB().GetY(); o[6] = p.GetB().GetZ();
+      o[7] = p.GetAoverC().GetX(); o[8] = p.GetAoverC().GetY(); o[9] = p.GetAoverC().GetZ();
+    }
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+static int make_pol(const char* pol, double angle, std::string& Polarization) {
+  Polarization = "all";
+  if (std::strlen(pol) != 0) {
+    Polarization = pol;
+  } else if (angle != 112188979912321) {
+    Polarization = "linear";
+  }
+  return 0;
+}
+
+// calculate_spectrum(obs, energy_range_eV=[e0,e1], npoints=n | energy_points_eV=list)
+int ref_spectrum(void* h, double const* obs, int npoints, double e0, double e1, double const* energies,
+                 const char* pol, double angle, double const* hdir, double const* pdir,
+                 int nthreads, double precision, int max_level, int max_level_extended, int quantity,
+                 double* out_energy, double* out_value, int* out_all_converged) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    TSpectrumContainer C;
+    if (energies) {
+      C.Init(std::vector<double>(energies, energies + npoints));
+    } else {
+      C.Init((size_t) npoints, e0, e1);
+    }
+    std::string P;
+    make_pol(pol, angle, P);
+    s->CalculateSpectrum(v3(obs), C, P, angle, v3(hdir), v3(pdir), 0, nthreads, 0, -1, std::vector<int>(),
+                         precision, max_level, max_level_extended, quantity);
+    if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
+    for (size_t i = 0; i != C.GetNPoints(); ++i) {
+      out_energy[i] = C.GetEnergy(i);
+      out_value[i] = C.GetFlux(i);
+    }
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+static void make_rect(TSurfacePoints_Rectangle& R, const char* plane, int n1, int n2, double w1, double w2,
+                      double const* rotations, double const* translation, int normal) {
+  R.Init(plane, n1, n2, w1, w2, v3(rotations), v3(translation), normal);
+}
+
+int ref_flux_rectangle(void* h, double energy_eV, const char* plane, int n1, int n2, double w1, double w2,
+                       double const* rotations, double const* translation, int normal, int dim,
+                       const char* pol, double angle, double const* hdir, double const* pdir,
+                       int nthreads, double precision, int max_level, int max_level_extended, int quantity,
+                       double* out_xyz, double* out_value, int* out_all_converged) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    TSurfacePoints_Rectangle R;
+    make_rect(R, plane, n1, n2, w1, w2, rotations, translation, normal);
+    T3DScalarContainer C;
+    std::string P;
+    make_pol(pol, angle, P);
+    s->CalculateFlux(R, energy_eV, C, P, angle, v3(hdir), v3(pdir), 0, nthreads, 0, -1, std::vector<int>(),
+                     precision, max_level, max_level_extended, dim, quantity);
+    if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
+    for (size_t i = 0; i != C.GetNPoints(); ++i) {
+      T3DScalar const& p = C.GetPoint(i);
+      out_xyz[3 * i + 0] = p.GetX().GetX();
+      out_xyz[3 * i + 1] = p.GetX().GetY();
+      out_xyz[3 * i + 2] = p.GetX().GetZ();
+      out_value[i] = p.GetV();
+    }
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+int ref_power_density_rectangle(void* h, const char* plane, int n1, int n2, double w1, double w2,
+                                double const* rotations, double const* translation, int normal, int dim,
+                                int nthreads, double precision, int max_level, int max_level_extended,
+                                int quantity, double* out_xyz, double* out_value, int* out_all_converged) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    TSurfacePoints_Rectangle R;
+    make_rect(R, plane, n1, n2, w1, w2, rotations, translation, normal);
+    T3DScalarContainer C;
+    bool const Directional = normal == 0 ? false : true;
+    s->CalculatePowerDensity(R, C, dim, Directional, precision, max_level, max_level_extended, 0, nthreads, 0,
+                             -1, std::vector<int>(), quantity);
+    if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
+    for (size_t i = 0; i != C.GetNPoints(); ++i) {
+      T3DScalar const& p = C.GetPoint(i);
+      out_xyz[3 * i + 0] = p.GetX().GetX();
+      out_xyz[3 * i + 1] = p.GetX().GetY();
+      out_xyz[3 * i + 2] = p.GetX().GetZ();
+      out_value[i] = p.GetV();
+    }
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+}  // extern "C"
