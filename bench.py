@@ -1,0 +1,244 @@
+"""Benchmark: Lienard-Wiechert integrand evaluations per second on one MI355X.
+
+Workload (one step), a planar undulator (By 1.0 T, period 49 mm, 31 periods,
+3 GeV, 0.5 A), trajectory z in [-1, 1] m with 20001 RK4 points:
+  * trajectory: RK4 + spline on the device, level samples rebuilt every step
+  * flux map 201 x 201 on +-10 mm at 30 m, at 152.11 eV (first harmonic) and
+    at 260 eV (the literal energy in the configuration)
+  * flux sweep 51 x 51 on the same rectangle x 512 energies, 100-3000 eV
+One "eval" is one (observation point, energy, trajectory sample) integrand
+evaluation; a point converged at level L costs 2^(L+1)-1 evals.
+
+The headline mode (default "mixed") runs the single-energy maps with the
+per-point kernel (FP32 sin/cos of the exact FP64 phase) and the sweep with the
+observation x energy tiled kernel.  The plain FP64 path (one lane per output
+point, everything FP64, the reference-faithful baseline) is timed on the same
+workload for the ratio, and the mixed result is compared with it point by point.
+
+Prints ONE JSON line (rank 0, N=1).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic FP64 flops per eval (divisions and square roots count as one flop, sin/cos counted apart):
+#   per-point kernel: amplitude + phase 76, phase-step bookkeeping 3
+#   tiled kernel: phase 1, phase-step bookkeeping 2, complex MAC 12 per eval, plus 66 per (obs, sample)
+FLOPS_POINT = 79.0
+FLOPS_TILE_EVAL = 15.0
+FLOPS_TILE_SAMPLE = 66.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def scenario(s):
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.049], nperiods=31)
+    s.set_ctstartstop(0, 2)
+    s.set_npoints_trajectory(20001)
+
+
+def rect(n, w=0.02):
+    from oscars_amd import geometry as G
+    obs, nv, cont = G.rectangle("XY", n, n, w, w, [0, 0, 0], [0, 0, 30], 0, 2)
+    return np.array(obs)
+
+
+class Workload:
+    def __init__(self, sr_obj, n_map=201, n_sweep=51, n_e=512):
+        self.s = sr_obj
+        self.obs_map = rect(n_map)
+        self.obs_sweep = rect(n_sweep)
+        self.e_sweep = np.array([100 + (3000 - 100) / (n_e - 1) * i for i in range(n_e)])
+        self.map_energies = [152.11, 260.0]
+
+    def step(self, mode):
+        """One pass; returns dict of per-phase evals / kernel ms / outputs."""
+        s = self.s
+        dev = s._device()
+        # RK4 + spline on the device; level samples are rebuilt inside the step
+        dev.trajectories(s._fields, [s._om_particle(s._particle)])
+        pol = [0.0] * 6
+        out = {"evals": 0.0, "kernel_ms": {}, "values": {}}
+        map_mode = "plain" if mode == "plain" else "point_mixed"
+        for e in self.map_energies:
+            v, lev, cv, _, st = dev.flux(0, self.obs_map, [e], pol, dev.conv(mode=map_mode))
+            out["evals"] += st["evals"]
+            out["kernel_ms"]["map_%g" % e] = st["kernel_ms"]
+            out["values"]["map_%g" % e] = v
+        sweep_mode = {"plain": "plain", "tiled": "tiled", "mixed": "mixed"}[mode]
+        v, lev, cv, _, st = dev.flux(0, self.obs_sweep, self.e_sweep, pol, dev.conv(mode=sweep_mode))
+        out["evals"] += st["evals"]
+        out["sweep_evals"] = st["evals"]
+        out["sweep_levels"] = lev
+        out["kernel_ms"]["sweep"] = st["kernel_ms"]
+        out["values"]["sweep"] = v
+        return out
+
+
+def timed(work, mode, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        work.step(mode)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = None
+    evals = 0.0
+    kms = {}
+    for _ in range(steps):
+        res = work.step(mode)
+        evals += res["evals"]
+        for k, v in res["kernel_ms"].items():
+            kms[k] = kms.get(k, 0.0) + v
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0), evals, {k: v / steps for k, v in kms.items()}, res
+
+
+def rel_errors(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.abs(b)
+    pointwise = np.abs(a - b) / np.where(den > 0, den, 1.0)
+    floored = np.abs(a - b) / np.maximum(den, 1e-3 * np.max(den))
+    l2 = np.linalg.norm(a - b) / np.linalg.norm(b)
+    return dict(max_pointwise=float(np.max(pointwise)), max_floored_1e3=float(np.max(floored)), l2=float(l2))
+
+
+def cpu_baseline(work, sweep_levels, n_e_sample=8):
+    """Reference C++ core (oracle/_ref/libref.so, built from the reference
+    sources) on this host, nthreads = host cores, on a bounded sample: the
+    201x201 map at 152.11 eV and the 51x51 sweep at n_e_sample energies.
+    Falls back to the C restatement (kind 'port') when the reference build is absent."""
+    from oracle import pyoracle as P
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    kind = "reference" if P.ref_available() else "port"
+    sim = P.RefSim(nthreads=ncores) if kind == "reference" else P.Oracle(nthreads=ncores)
+    scenario(sim)
+    if kind == "port":
+        sim.calculate_trajectory()
+    idx = np.linspace(0, len(work.e_sweep) - 1, n_e_sample).round().astype(int)
+    levels = sweep_levels.reshape(len(work.obs_sweep), len(work.e_sweep))[:, idx]
+    t0 = time.perf_counter()
+    if kind == "reference":
+        _, _, _ = sim.calculate_flux_rectangle(152.11, [201, 201], [0.02, 0.02], translation=[0, 0, 30])
+    else:
+        sim.calculate_flux_rectangle(152.11, [201, 201], [0.02, 0.02], translation=[0, 0, 30])
+    t_map = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k, e in enumerate(work.e_sweep[idx]):
+        sim.calculate_flux_rectangle(float(e), [51, 51], [0.02, 0.02], translation=[0, 0, 30])
+    t_sweep = time.perf_counter() - t0
+    return dict(t_map=t_map, t_sweep=t_sweep, sweep_evals=float(np.sum(2.0 ** (levels + 1) - 1)), kind=kind,
+                cores=ncores, n_e=n_e_sample)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "tiled", "plain"])
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    from oscars_amd.sr import sr
+    s = sr()
+    scenario(s)
+    work = Workload(s)
+    print("# device: %s" % s._device().info(), file=sys.stderr, flush=True)
+
+    dt, evals, kms, res = timed(work, args.mode, args.steps, args.warmup)
+    ms_per_step = 1e3 * dt / args.steps
+    value = evals / dt
+    print("# %s: %.3f ms/step, %.4g evals/step, %.4g evals/s, kernel ms %s" %
+          (args.mode, ms_per_step, evals / args.steps, value, kms), file=sys.stderr, flush=True)
+
+    extra = {}
+    sweep_ms = kms["sweep"]
+    sweep_evals = res["sweep_evals"]
+    n_obs_sweep = len(work.obs_sweep)
+    # samples visited per observation point in the sweep: the tile kernel computes the amplitude once per
+    # (obs, sample, 64-energy tile); count it per (obs, energy-tile, sample) actually visited.
+    lev = res["sweep_levels"].reshape(n_obs_sweep, len(work.e_sweep))
+    if args.mode == "plain":
+        flops = FLOPS_POINT * sweep_evals
+    else:
+        tiles = [lev[:, i:i + 64].max(axis=1) for i in range(0, lev.shape[1], 64)]
+        samples = float(sum(np.sum(2.0 ** (t + 1) - 1) for t in tiles))
+        flops = FLOPS_TILE_EVAL * sweep_evals + FLOPS_TILE_SAMPLE * samples
+    achieved = flops / (sweep_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "kernel": "lw_tile_kernel<%s>" % ("true" if args.mode == "mixed" else "false")
+                if args.mode != "plain" else "lw_point_kernel<false>",
+                "kernel_ms": round(sweep_ms, 3), "evals_per_launch": sweep_evals,
+                "sincos_per_s": sweep_evals / (sweep_ms * 1e-3)}
+
+    if not args.no_plain and args.mode != "plain":
+        pdt, pevals, pkms, pres = timed(work, "plain", args.plain_steps, 0)
+        pvalue = pevals / pdt
+        extra["plain_fp64"] = {"value": pvalue, "ms_per_step": 1e3 * pdt / args.plain_steps, "kernel_ms": pkms}
+        extra["ratio_vs_plain"] = value / pvalue
+        errs = {}
+        for k in res["values"]:
+            errs[k] = rel_errors(res["values"][k], pres["values"][k])
+        extra["rel_err_vs_fp64"] = errs
+        print("# plain: %.4g evals/s ratio %.2f errs %s" % (pvalue, value / pvalue, errs), file=sys.stderr,
+              flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cb = cpu_baseline(work, res["sweep_levels"])
+            map_evals = None
+            # evals of the 152.11 eV map come from the GPU levels of the same points (identical levels)
+            st_map = work.s._device().flux(0, work.obs_map, [152.11], [0.0] * 6,
+                                           work.s._device().conv(mode="plain"))[4]
+            map_evals = st_map["evals"]
+            cpu_value = (map_evals + cb["sweep_evals"]) / (cb["t_map"] + cb["t_sweep"])
+            cpu = {"value": cpu_value, "unit": "evals/s", "cores": cb["cores"], "kind": cb["kind"],
+                   "sample": "201x201 map at 152.11 eV + 51x51 sweep at %d of the 512 energies, %.1f s" %
+                             (cb["n_e"], cb["t_map"] + cb["t_sweep"])}
+            print("# cpu baseline: %s" % cpu, file=sys.stderr, flush=True)
+        except Exception as ex:  # the baseline is reported, never required
+            cpu = {"value": None, "error": str(ex)}
+
+    line = {
+        "metric": "LW integrand evals/s (obs x omega x traj), forward, single electron",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64" if args.mode != "mixed" else "fp64 phase/amplitude, fp32 sincos",
+        "data": "synthetic",
+        "config": {"workload": "planar undulator 1.0 T/49 mm/31 periods, 3 GeV 0.5 A, 20001-point RK4; "
+                               "flux 201x201 (+-10 mm @30 m) at 152.11 and 260 eV + 51x51 x 512 energies "
+                               "100-3000 eV", "mode": args.mode, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line, default=float))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+/* oscars_mi.h - C ABI of the MI355X (gfx950) synchrotron-radiation hot path.
+ *
+ * Drop-in replacement for the reference's GPU boundary
+ * (include/OSCARSSR_Cuda.h:28-66 of dhidas/OSCARS), redesigned as a true C ABI:
+ * plain pointers and sizes, no C++ types, status codes instead of exceptions.
+ * The caller (the C++/Python host, see oscars_amd/) samples particles and
+ * describes fields; this library integrates trajectories on the device and
+ * evaluates the Lienard-Wiechert sums.  Everything is synchronous: results
+ * are in the caller's buffers when a call returns.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   om_device_count   <- OSCARSSR_Cuda_GetDeviceCount        include/OSCARSSR_Cuda.h:28
+ *   om_device_info    <- OSCARSSR_Cuda_GetDeviceProperties   include/OSCARSSR_Cuda.h:30
+ *   om_trajectories   <- the host-side OSR.CalculateTrajectory() call inside the
+ *                        CUDA drivers (src/OSCARSSR_Cuda.cu:602-607, 1085-1088) and
+ *                        OSCARSSR::CalculateTrajectoryRK4 (src/OSCARSSR.cc:910-1009)
+ *   om_flux           <- OSCARSSR_Cuda_CalculateFluxGPU         include/OSCARSSR_Cuda.h:32-44
+ *   om_spectrum       <- OSCARSSR_Cuda_CalculateSpectrumGPU     include/OSCARSSR_Cuda.h:46-57
+ *   om_power_density  <- OSCARSSR_Cuda_CalculatePowerDensityGPU include/OSCARSSR_Cuda.h:59-66
+ * All functions return 0 on success; om_last_error() gives the message.
+ * A context is not thread-safe: use one per host thread.
+ */
+#ifndef OSCARS_MI_H
+#define OSCARS_MI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct om_context om_context;
+
+enum { OM_FIELD_UNDULATOR = 0, OM_FIELD_EPU = 1, OM_FIELD_GRID = 2 };
+/* Kernel families: PLAIN one lane per output point, all FP64 (bit-identical
+ * phase; the reference-faithful baseline); TILED one wave per observation
+ * point x 64 energies sharing the per-sample amplitude, FP64; MIXED the tiled
+ * kernel with FP32 sin/cos of the exactly reduced FP64 phase; POINT_MIXED the
+ * per-point kernel with FP32 sin/cos (single-energy maps). */
+enum { OM_MODE_PLAIN = 0, OM_MODE_TILED = 1, OM_MODE_MIXED = 2, OM_MODE_POINT_MIXED = 3 };
+enum { OM_Q_VALUE = 0, OM_Q_PRECISION = 1, OM_Q_LEVEL = 2 };
+
+/* One magnetic field model (src/TField3D_IdealUndulator.cc, TField3D_IdealEPU.cc,
+ * TField3D_Grid.cc).  Undulator/EPU: field (fieldB for the EPU's second
+ * component), period vector, nperiods, centre (translation), phase, taper.
+ * Grid: the OSCARS text-file header values (start, step, n per axis; exactly
+ * one axis with n > 1), the 3*n samples and the rotations/translation. */
+typedef struct {
+  int type;
+  int nperiods;
+  double field[3];
+  double fieldB[3];
+  double period[3];
+  double translation[3];
+  double phase;
+  double taper;
+  double grid_start[3];
+  double grid_step[3];
+  long grid_n[3];
+  const double* grid_data;
+  double rotations[3];
+} om_field;
+
+/* One particle: initial position [m], beta, T0 [m] (c*t, as the reference),
+ * charge [C], mass [kg], Q/M/gamma, current [A] and its time window. */
+typedef struct {
+  double x0[3];
+  double beta0[3];
+  double t0;
+  double q;
+  double m;
+  double gamma;
+  double qomg;
+  double current;
+  double ctstart;
+  double ctstop;
+  long npoints;
+} om_particle;
+
+/* Convergence and output controls (reference kwargs precision, max_level,
+ * max_level_extended, quantity) plus the kernel family. */
+typedef struct {
+  double precision;
+  int max_level;
+  int max_level_extended;
+  int quantity;
+  int mode;
+  double weight;
+  int accumulate; /* 1: out_value += weight*value (Kahan-compensated across calls) */
+} om_conv;
+
+typedef struct {
+  double evals;        /* sum over points of 2^(L+1)-1 */
+  double kernel_ms;    /* LW kernels only, HIP events on the library stream */
+  double total_ms;     /* whole call, host wall clock */
+  int max_level;       /* highest level any point reached */
+  long n_unconverged;
+} om_stats;
+
+int om_device_count(void);
+int om_device_info(int device, char* buf, size_t len);
+
+int om_context_create(int device, om_context** out);
+void om_context_destroy(om_context* ctx);
+const char* om_last_error(const om_context* ctx);
+
+/* Integrate np trajectories on the device (RK4 + clamped cubic spline); they
+ * stay resident in the context, indexed 0..np-1.  Replaces any previous set. */
+int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const om_particle* particles, int np);
+/* Knots of trajectory i: *n in = capacity, out = count; any pointer may be NULL. */
+int om_trajectory_get(om_context* ctx, int i, long* n, double* t, double* p9, double* ypp9);
+/* Level samples of trajectory i (2^level rows of t,x,y,z,bx,by,bz,ax,ay,az). */
+int om_level_get(om_context* ctx, int i, int level, int extended, double* rows);
+/* Status of trajectory i: 0 ok, 1 |beta| >= 1 met during integration. */
+int om_trajectory_status(om_context* ctx, int i);
+
+/* Flux on n_obs observation points x n_e photon energies (point index
+ * obs*n_e + e); n_e = 1 is the reference's flux map at one energy.
+ * pol6: polarization vector (Re, Im) x 3, all zeros for 'all'.
+ * out_e: complex E (6 doubles per point) after C0*dt and polarization; may be NULL. */
+int om_flux(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
+            const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
+            double* out_e, om_stats* stats);
+/* Spectrum at one observation point (om_flux with n_obs = 1 and the
+ * spectrum's |current| constant). */
+int om_spectrum(om_context* ctx, int particle, const double* obs, const double* energies_eV, long n_e,
+                const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
+                double* out_e, om_stats* stats);
+/* Power density [W/mm^2] on n_obs points with one surface normal. */
+int om_power_density(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* normal,
+                     int directional, const om_conv* conv, double* out_value, int* out_level,
+                     unsigned char* out_conv, om_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

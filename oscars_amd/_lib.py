@@ -1,0 +1,201 @@
+"""ctypes bindings of the C ABI in include/oscars_mi.h (liboscars_mi.so).
+
+The library is built in-tree by ``oscars_amd/csrc/Makefile`` (called from
+``__graft_entry__.build()``).  There is no CPU fallback: if the library is
+missing, or no HIP device is visible, every compute call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboscars_mi.so")
+
+OM_FIELD_UNDULATOR, OM_FIELD_EPU, OM_FIELD_GRID = 0, 1, 2
+OM_MODE_PLAIN, OM_MODE_TILED, OM_MODE_MIXED, OM_MODE_POINT_MIXED = 0, 1, 2, 3
+MODES = {"plain": OM_MODE_PLAIN, "tiled": OM_MODE_TILED, "mixed": OM_MODE_MIXED, "point_mixed": OM_MODE_POINT_MIXED}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_ubyte)
+_lp = C.POINTER(C.c_long)
+
+
+class om_field(C.Structure):
+    _fields_ = [("type", C.c_int), ("nperiods", C.c_int),
+                ("field", C.c_double * 3), ("fieldB", C.c_double * 3), ("period", C.c_double * 3),
+                ("translation", C.c_double * 3), ("phase", C.c_double), ("taper", C.c_double),
+                ("grid_start", C.c_double * 3), ("grid_step", C.c_double * 3), ("grid_n", C.c_long * 3),
+                ("grid_data", _dp), ("rotations", C.c_double * 3)]
+
+
+class om_particle(C.Structure):
+    _fields_ = [("x0", C.c_double * 3), ("beta0", C.c_double * 3), ("t0", C.c_double), ("q", C.c_double),
+                ("m", C.c_double), ("gamma", C.c_double), ("qomg", C.c_double), ("current", C.c_double),
+                ("ctstart", C.c_double), ("ctstop", C.c_double), ("npoints", C.c_long)]
+
+
+class om_conv(C.Structure):
+    _fields_ = [("precision", C.c_double), ("max_level", C.c_int), ("max_level_extended", C.c_int),
+                ("quantity", C.c_int), ("mode", C.c_int), ("weight", C.c_double), ("accumulate", C.c_int)]
+
+
+class om_stats(C.Structure):
+    _fields_ = [("evals", C.c_double), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
+                ("max_level", C.c_int), ("n_unconverged", C.c_long)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
+           "om_trajectories", "om_trajectory_get", "om_level_get", "om_trajectory_status", "om_flux",
+           "om_spectrum", "om_power_density"]
+
+_lib = None
+
+
+def lib():
+    """Load liboscars_mi.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("oscars_amd: %s is missing - build it with __graft_entry__.build() "
+                           "(there is no CPU fallback)" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    L.om_device_count.restype = C.c_int
+    L.om_device_info.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
+    L.om_context_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    L.om_context_destroy.argtypes = [C.c_void_p]
+    L.om_last_error.argtypes = [C.c_void_p]
+    L.om_last_error.restype = C.c_char_p
+    L.om_trajectories.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.POINTER(om_particle), C.c_int]
+    L.om_trajectory_get.argtypes = [C.c_void_p, C.c_int, _lp, _dp, _dp, _dp]
+    L.om_level_get.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
+    L.om_trajectory_status.argtypes = [C.c_void_p, C.c_int]
+    L.om_flux.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, _dp, C.c_long, _dp, C.POINTER(om_conv), _dp, _ip,
+                          _up, _dp, C.POINTER(om_stats)]
+    L.om_spectrum.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_long, _dp, C.POINTER(om_conv), _dp, _ip, _up,
+                              _dp, C.POINTER(om_stats)]
+    L.om_power_density.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, _dp, C.c_int, C.POINTER(om_conv), _dp,
+                                   _ip, _up, C.POINTER(om_stats)]
+    _lib = L
+    return L
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class Device:
+    """One HIP device context (device buffers, stream, trajectory set)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        n = L.om_device_count()
+        if n < 1:
+            raise ValueError("You are requesting the GPU, but none were found")
+        h = C.c_void_p()
+        rc = L.om_context_create(int(device), C.byref(h))
+        if rc:
+            raise ValueError("could not create a HIP context on device %d (rc=%d)" % (device, rc))
+        self.L = L
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.L.om_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc:
+            raise ValueError(self.L.om_last_error(self.h).decode())
+
+    def info(self):
+        buf = C.create_string_buffer(512)
+        self.L.om_device_info(self.device, buf, 512)
+        return buf.value.decode()
+
+    def trajectories(self, fields, particles):
+        fa = (om_field * max(1, len(fields)))(*fields)
+        pa = (om_particle * len(particles))(*particles)
+        self._chk(self.L.om_trajectories(self.h, fa, len(fields), pa, len(particles)))
+        for i in range(len(particles)):
+            if self.L.om_trajectory_status(self.h, i) != 0:
+                raise ValueError("trajectory %d: |beta| >= 1 met during RK4 integration" % i)
+
+    def trajectory(self, i=0):
+        n = C.c_long(0)
+        self._chk(self.L.om_trajectory_get(self.h, i, C.byref(n), None, None, None))
+        t = np.zeros(n.value)
+        p = np.zeros((n.value, 9))
+        ypp = np.zeros((n.value, 9))
+        self._chk(self.L.om_trajectory_get(self.h, i, C.byref(n), _ptr(t), _ptr(p), _ptr(ypp)))
+        return t, p, ypp
+
+    def level(self, i, L, extended=False):
+        out = np.zeros((2 ** L, 10))
+        self._chk(self.L.om_level_get(self.h, i, int(L), int(extended), _ptr(out)))
+        return out
+
+    @staticmethod
+    def conv(precision=0.01, max_level=-2, max_level_extended=0, quantity=0, mode="plain", weight=1.0,
+             accumulate=False):
+        c = om_conv()
+        c.precision = precision
+        c.max_level = max_level
+        c.max_level_extended = max_level_extended
+        c.quantity = quantity
+        c.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+        c.weight = weight
+        c.accumulate = int(accumulate)
+        return c
+
+    def flux(self, particle, obs_xyz, energies_eV, pol6, conv, out=None, want_e=False):
+        obs = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(-1, 3)
+        e = np.ascontiguousarray(np.atleast_1d(energies_eV), dtype=np.float64)
+        n = len(obs) * len(e)
+        out = np.zeros(n) if out is None else out
+        lev = np.zeros(n, dtype=np.int32)
+        cv = np.zeros(n, dtype=np.uint8)
+        ef = np.zeros((n, 6)) if want_e else None
+        pol = np.ascontiguousarray(pol6, dtype=np.float64)
+        st = om_stats()
+        self._chk(self.L.om_flux(self.h, int(particle), _ptr(obs), len(obs), _ptr(e), len(e), _ptr(pol),
+                                 C.byref(conv), _ptr(out), _ptr(lev, _ip), _ptr(cv, _up), _ptr(ef), C.byref(st)))
+        return out, lev, cv, (ef[:, 0::2] + 1j * ef[:, 1::2]) if want_e else None, st.as_dict()
+
+    def spectrum(self, particle, obs, energies_eV, pol6, conv, out=None, want_e=False):
+        o = np.ascontiguousarray(obs, dtype=np.float64)
+        e = np.ascontiguousarray(energies_eV, dtype=np.float64)
+        n = len(e)
+        out = np.zeros(n) if out is None else out
+        lev = np.zeros(n, dtype=np.int32)
+        cv = np.zeros(n, dtype=np.uint8)
+        ef = np.zeros((n, 6)) if want_e else None
+        pol = np.ascontiguousarray(pol6, dtype=np.float64)
+        st = om_stats()
+        self._chk(self.L.om_spectrum(self.h, int(particle), _ptr(o), _ptr(e), n, _ptr(pol), C.byref(conv),
+                                     _ptr(out), _ptr(lev, _ip), _ptr(cv, _up), _ptr(ef), C.byref(st)))
+        return out, lev, cv, (ef[:, 0::2] + 1j * ef[:, 1::2]) if want_e else None, st.as_dict()
+
+    def power_density(self, particle, obs_xyz, normal, directional, conv, out=None):
+        obs = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(-1, 3)
+        n = len(obs)
+        out = np.zeros(n) if out is None else out
+        lev = np.zeros(n, dtype=np.int32)
+        cv = np.zeros(n, dtype=np.uint8)
+        nv = np.ascontiguousarray(normal, dtype=np.float64)
+        st = om_stats()
+        self._chk(self.L.om_power_density(self.h, int(particle), _ptr(obs), n, _ptr(nv), int(directional),
+                                          C.byref(conv), _ptr(out), _ptr(lev, _ip), _ptr(cv, _up), C.byref(st)))
+        return out, lev, cv, st.as_dict()

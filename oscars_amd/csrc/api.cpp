@@ -1,0 +1,761 @@
+// Host side of the C ABI (include/oscars_mi.h): device context, trajectory
+// set, level-sample cache and the level-by-level driver of the LW kernels.
+//
+// Reference behaviour restated (paths relative to the reference root):
+//   level stop rules     src/OSCARSSR.cc:1866-1868, 4108-4110, 2836-2838
+//   RK4 point counts     src/OSCARSSR.cc:914-919
+//   spectrum/flux/power constants src/OSCARSSR.cc:1882-1885, 4123-4126, 2957
+//   undulator geometry   src/TField3D_IdealUndulator.cc:101-106, 123-125
+//   EPU phase shifts     src/TField3D_IdealEPU.cc:126-150
+//   grid header handling src/TField3D_Grid.cc:413-437
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/oscars_mi.h"
+#include "om_types.h"
+
+namespace om {
+struct Levels {
+  const double* rows[kMaxLevel + 1];
+  double dt_incl[kMaxLevel + 1];
+  double beta_diff[kMaxLevel + 1];
+};
+struct State {
+  double* sum;
+  double* comp;
+  double* lastmag;
+  double* prec;
+  int* level;
+  int* conv;
+};
+struct LwArgs {
+  Levels lv;
+  State st;
+  int La, Lb;
+  long npts;
+  const double* obs;
+  const double* omega;
+  long n_omega;
+  double pol[6];
+  int use_pol;
+  double precision;
+};
+struct PdArgs {
+  Levels lv;
+  State st;
+  int La, Lb;
+  long npts;
+  const double* obs;
+  double normal[3];
+  int directional;
+  double precision;
+  double two_over_gamma;
+};
+struct FinArgs {
+  State st;
+  long npts;
+  int kind;
+  int quantity;
+  double C0, C2;
+  double pd_const;
+  int directional;
+  double dt_incl[kMaxLevel + 1];
+  double pol[6];
+  int use_pol;
+  double weight;
+  int accumulate;
+  double* out;
+  double* out_comp;
+  double* out_e;
+  int* out_level;
+  unsigned char* out_conv;
+};
+struct LevelJob {
+  const double* t;
+  const double* p;
+  const double* ypp;
+  long knot_offset;
+  long n;
+  double tstart, tstop;
+  double* out;
+  long npts;
+  double t_first, spacing;
+};
+}  // namespace om
+
+extern "C" {
+hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
+                         int np, double* t, double* p, hipStream_t s);
+hipError_t om_launch_spline(const om::TrajDesc* desc, int np, const double* t, const double* p, double* ypp,
+                            double* u, hipStream_t s);
+hipError_t om_launch_level(const om::LevelJob* J, hipStream_t s);
+hipError_t om_launch_beta_diff(const double* rows, long npts, double* out, hipStream_t s);
+hipError_t om_launch_lw_point(const om::LwArgs* A, int mixed, int split, hipStream_t s);
+hipError_t om_launch_lw_tile(const om::LwArgs* A, long n_obs, int mixed, int split, hipStream_t s);
+hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s);
+hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s);
+hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s);
+hipError_t om_launch_count_active(const om::State* st, long npts, int* out, hipStream_t s);
+}
+
+namespace {
+
+// include/TOSCARS.h:20-76
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kSqrt2Pi = 2.5066282746310002416123552393401;
+constexpr double kC = 2.99792458e8;
+constexpr double kH = 6.62606876e-34;
+constexpr double kHbar = 1.054571596e-34;
+constexpr double kQe = 1.602176462e-19;
+constexpr double kEps0 = 8.854187817E-12;
+constexpr double kMu0 = 1.2566370614E-6;
+inline double two_pi() { return 2.0 * kPi; }
+inline double four_pi() { return 4.0 * kPi; }
+inline double ev_to_omega(double e) { return e * kQe / kHbar; }
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    if (p) check(hipFree(p), "hipFree");
+    p = nullptr;
+    n = 0;
+    if (count == 0) return;
+    check(hipMalloc((void**) &p, count * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+  void release() {
+    if (p) (void) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct LevelCache {
+  double* rows[om::kMaxLevel + 1] = {};
+  int extended[om::kMaxLevel + 1] = {};
+  double beta_diff[om::kMaxLevel + 1] = {};
+  bool have_bd[om::kMaxLevel + 1] = {};
+};
+
+}  // namespace
+
+struct om_context {
+  int device = 0;
+  int split = 0;  // waves per workgroup for the LW kernels (0: chosen from the grid size)
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  // trajectories
+  int np = 0;
+  std::vector<om::TrajDesc> desc;
+  DevBuf<om::TrajDesc> d_desc;
+  DevBuf<double> d_t, d_p, d_ypp, d_u;
+  std::vector<double> tstart, tstop;
+  std::vector<om_particle> parts;
+  std::vector<LevelCache> levels;
+  DevBuf<om::FieldDev> d_fields;
+  std::vector<DevBuf<double>> grid_bufs;
+  // work buffers
+  DevBuf<double> d_obs, d_omega, d_sum, d_comp, d_lastmag, d_prec, d_out, d_outcomp, d_oute;
+  DevBuf<int> d_level, d_conv, d_count, d_outlevel;
+  DevBuf<unsigned char> d_outconv;
+  DevBuf<double> d_bd;
+
+  void free_levels() {
+    for (auto& lc : levels)
+      for (int L = 0; L <= om::kMaxLevel; ++L)
+        if (lc.rows[L]) {
+          (void) hipFree(lc.rows[L]);
+          lc.rows[L] = nullptr;
+        }
+    levels.clear();
+  }
+
+  ~om_context() {
+    free_levels();
+    for (auto& g : grid_bufs) g.release();
+    d_desc.release(); d_t.release(); d_p.release(); d_ypp.release(); d_u.release(); d_fields.release();
+    d_obs.release(); d_omega.release(); d_sum.release(); d_comp.release(); d_lastmag.release();
+    d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
+    d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
+    if (ev0) (void) hipEventDestroy(ev0);
+    if (ev1) (void) hipEventDestroy(ev1);
+    if (stream) (void) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+template <class F>
+int guarded(om_context* ctx, F&& f) {
+  try {
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    f();
+    return 0;
+  } catch (std::exception const& e) {
+    ctx->err = e.what();
+    return 1;
+  }
+}
+
+// src/TField3D_IdealUndulator.cc:101-106, 123-125; src/TField3D_IdealEPU.cc:126-150
+om::FieldDev make_field(om_context* ctx, const om_field& f) {
+  om::FieldDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.type = f.type;
+  d.nperiods = f.nperiods;
+  for (int k = 0; k < 3; ++k) {
+    d.field[k] = f.field[k];
+    d.fieldB[k] = f.fieldB[k];
+    d.period[k] = f.period[k];
+    d.center[k] = f.translation[k];
+  }
+  d.phase = f.phase;
+  d.taper = f.taper;
+  double pm = std::sqrt(f.period[0] * f.period[0] + f.period[1] * f.period[1] + f.period[2] * f.period[2]);
+  d.period_length = pm;
+  for (int k = 0; k < 3; ++k) d.unit[k] = f.period[k] / pm;
+  d.und_length = pm * (double) (f.nperiods + 2);
+  if (f.type == OM_FIELD_UNDULATOR) {
+    d.phase_shift_a = f.phase * pm / two_pi();
+  } else if (f.type == OM_FIELD_EPU) {
+    d.phase_shift_a = f.phase / 2. * pm / two_pi();
+    d.phase_shift_b = -f.phase / 2. * pm / two_pi();
+  } else if (f.type == OM_FIELD_GRID) {
+    int nx = (int) f.grid_n[0], ny = (int) f.grid_n[1], nz = (int) f.grid_n[2];
+    if (nx < 1 || ny < 1 || nz < 1) throw std::out_of_range("invalid number of points in at least one dimension");
+    int naxes = (nx > 1) + (ny > 1) + (nz > 1);
+    if (naxes != 1) throw std::invalid_argument("only one-dimensional field grids are supported on the GPU path");
+    int nn[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+      // src/TField3D_Grid.cc:413-437 (no scaling)
+      double middle = f.grid_start[a] + f.grid_step[a] * (nn[a] - 1) / 2.;
+      double start = middle - f.grid_step[a] * (nn[a] - 1) / 2.;
+      d.n[a] = nn[a];
+      d.start[a] = start;
+      d.step[a] = f.grid_step[a];
+      d.stop[a] = start + (double) (size_t) (nn[a] - 1) * f.grid_step[a];
+      if (nn[a] > 1) d.axis = a;
+    }
+    long n = (long) nx * ny * nz;
+    if (n < 2) throw std::invalid_argument("field grid needs at least two samples");
+    std::vector<double> data(3 * n);
+    for (long i = 0; i < n; ++i) {
+      double v[3] = {f.grid_data[3 * i], f.grid_data[3 * i + 1], f.grid_data[3 * i + 2]};
+      // samples are rotated by the same angles as the point (src/TField3D_Grid.cc:536-538)
+      double s = std::sin(f.rotations[0]), c = std::cos(f.rotations[0]), yy = v[1];
+      v[1] = c * yy - s * v[2];
+      v[2] = s * yy + c * v[2];
+      s = std::sin(f.rotations[1]); c = std::cos(f.rotations[1]);
+      double zz = v[2];
+      v[2] = c * zz - s * v[0];
+      v[0] = s * zz + c * v[0];
+      s = std::sin(f.rotations[2]); c = std::cos(f.rotations[2]);
+      double xx = v[0];
+      v[0] = c * xx - s * v[1];
+      v[1] = s * xx + c * v[1];
+      data[3 * i] = v[0]; data[3 * i + 1] = v[1]; data[3 * i + 2] = v[2];
+    }
+    ctx->grid_bufs.emplace_back();
+    DevBuf<double>& g = ctx->grid_bufs.back();
+    g.ensure(data.size());
+    check(hipMemcpy(g.p, data.data(), data.size() * sizeof(double), hipMemcpyHostToDevice), "grid upload");
+    d.data = g.p;
+    for (int k = 0; k < 3; ++k) {
+      d.rot[k] = f.rotations[k];
+      d.rot_sin[k] = std::sin(f.rotations[k]);
+      d.rot_cos[k] = std::cos(f.rotations[k]);
+      d.trans[k] = f.translation[k];
+    }
+  } else {
+    throw std::invalid_argument("unknown field type");
+  }
+  return d;
+}
+
+// level stop rules, src/OSCARSSR.cc:1866-1868
+void level_stops(const om_conv& c, int& stop_mem, int& stop_ext) {
+  stop_mem = (c.max_level >= -1 && c.max_level <= om::kMaxLevel) ? c.max_level : om::kMaxLevel;
+  stop_ext = c.max_level_extended > stop_mem ? c.max_level_extended : stop_mem;
+}
+
+double dt_inclusive(double ts, double te, int L) { return (te - ts) / (std::pow(2., L + 1) - 1); }
+
+// Make sure levels [0, Lb] of trajectory ip exist on the device with the right
+// (memory / extended) spacing.
+void ensure_levels(om_context* ctx, int ip, int Lb, int stop_mem) {
+  LevelCache& lc = ctx->levels[ip];
+  const om::TrajDesc& D = ctx->desc[ip];
+  double ts = ctx->tstart[ip], te = ctx->tstop[ip];
+  for (int L = 0; L <= Lb; ++L) {
+    int ext = L > stop_mem ? 1 : 0;
+    if (lc.rows[L] && lc.extended[L] == ext) continue;
+    long np = 1L << L;
+    if (!lc.rows[L]) check(hipMalloc((void**) &lc.rows[L], (size_t) np * 10 * sizeof(double)), "hipMalloc level");
+    lc.extended[L] = ext;
+    lc.have_bd[L] = false;
+    om::LevelJob J;
+    J.t = ctx->d_t.p;
+    J.p = ctx->d_p.p;
+    J.ypp = ctx->d_ypp.p;
+    J.knot_offset = D.knot_offset;
+    J.n = D.n;
+    J.tstart = ts;
+    J.tstop = te;
+    J.out = lc.rows[L];
+    J.npts = np;
+    // src/TParticleTrajectoryInterpolated.cc:138-147, 264-270 and
+    // src/TParticleTrajectoryInterpolatedPoints.cc:33-37
+    J.spacing = ext ? (te - ts) / (std::pow(2., L) - 1) : (te - ts) / std::pow(2., L);
+    J.t_first = ts + (te - ts) / std::pow(2., L + 1);
+    check(om_launch_level(&J, ctx->stream), "level kernel");
+  }
+}
+
+void ensure_beta_diff(om_context* ctx, int ip, int Lb) {
+  LevelCache& lc = ctx->levels[ip];
+  ctx->d_bd.ensure(om::kMaxLevel + 1);
+  bool any = false;
+  for (int L = 0; L <= Lb; ++L) {
+    if (lc.have_bd[L]) continue;
+    check(om_launch_beta_diff(lc.rows[L], 1L << L, ctx->d_bd.p + L, ctx->stream), "beta diff");
+    any = true;
+  }
+  if (!any) return;
+  double h[om::kMaxLevel + 1];
+  check(hipMemcpyAsync(h, ctx->d_bd.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "bd copy");
+  check(hipStreamSynchronize(ctx->stream), "sync");
+  for (int L = 0; L <= Lb; ++L) {
+    if (!lc.have_bd[L]) {
+      lc.beta_diff[L] = h[L];
+      lc.have_bd[L] = true;
+    }
+  }
+}
+
+void fill_levels(om_context* ctx, int ip, int Lb, om::Levels& lv) {
+  std::memset(&lv, 0, sizeof(lv));
+  LevelCache& lc = ctx->levels[ip];
+  for (int L = 0; L <= om::kMaxLevel; ++L) {
+    lv.rows[L] = L <= Lb ? lc.rows[L] : nullptr;
+    lv.dt_incl[L] = dt_inclusive(ctx->tstart[ip], ctx->tstop[ip], L);
+    lv.beta_diff[L] = lc.beta_diff[L];
+  }
+}
+
+void state_for(om_context* ctx, long npts, bool mixed, om::State& st) {
+  ctx->d_sum.ensure(6 * npts);
+  ctx->d_comp.ensure(6 * npts);
+  ctx->d_lastmag.ensure(npts);
+  ctx->d_prec.ensure(npts);
+  ctx->d_level.ensure(npts);
+  ctx->d_conv.ensure(npts);
+  st.sum = ctx->d_sum.p;
+  st.comp = mixed ? ctx->d_comp.p : nullptr;
+  st.lastmag = ctx->d_lastmag.p;
+  st.prec = ctx->d_prec.p;
+  st.level = ctx->d_level.p;
+  st.conv = ctx->d_conv.p;
+  check(om_launch_state_reset(&st, npts, ctx->stream), "state reset");
+}
+
+long count_active(om_context* ctx, const om::State& st, long npts) {
+  ctx->d_count.ensure(1);
+  int zero = 0;
+  check(hipMemcpyAsync(ctx->d_count.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream), "count init");
+  check(om_launch_count_active(&st, npts, ctx->d_count.p, ctx->stream), "count active");
+  int h = 0;
+  check(hipMemcpyAsync(&h, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "count copy");
+  check(hipStreamSynchronize(ctx->stream), "sync");
+  return h;
+}
+
+// First launch covers levels up to this one; later launches add one level
+// at a time for the points still running.
+constexpr int kFirstLevels = 13;
+
+void check_particle(om_context* ctx, int ip) {
+  if (ip < 0 || ip >= ctx->np) throw std::out_of_range("particle index out of range (no trajectory computed?)");
+  if (ctx->desc[ip].status != 0)
+    throw std::out_of_range("trajectory integration met |beta| >= 1; step halving is not supported on the device");
+}
+
+template <class T>
+void upload(om_context* ctx, DevBuf<T>& b, const T* h, size_t n) {
+  b.ensure(n);
+  check(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_value, int* out_level,
+                    unsigned char* out_conv, double* out_e, om_stats* stats, const om_conv& conv) {
+  ctx->d_out.ensure(npts);
+  ctx->d_outcomp.ensure(npts);
+  ctx->d_outlevel.ensure(npts);
+  ctx->d_outconv.ensure(npts);
+  if (out_e) ctx->d_oute.ensure(6 * npts);
+  F.out = ctx->d_out.p;
+  F.out_comp = ctx->d_outcomp.p;
+  F.out_e = out_e ? ctx->d_oute.p : nullptr;
+  F.out_level = ctx->d_outlevel.p;
+  F.out_conv = ctx->d_outconv.p;
+  F.weight = conv.weight;
+  F.accumulate = conv.accumulate;
+  if (conv.accumulate) {
+    // The running sum lives in the caller's buffer; the compensation term is
+    // kept on the device between calls of the same size.
+    check(hipMemcpyAsync(ctx->d_out.p, out_value, npts * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+          "acc upload");
+  }
+  check(om_launch_finalize(&F, ctx->stream), "finalize");
+  check(hipMemcpyAsync(out_value, ctx->d_out.p, npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "out");
+  std::vector<int> lev(npts);
+  std::vector<unsigned char> cv(npts);
+  check(hipMemcpyAsync(lev.data(), ctx->d_outlevel.p, npts * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+        "lev");
+  check(hipMemcpyAsync(cv.data(), ctx->d_outconv.p, npts, hipMemcpyDeviceToHost, ctx->stream), "conv");
+  if (out_e)
+    check(hipMemcpyAsync(out_e, ctx->d_oute.p, 6 * npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+          "efield");
+  check(hipStreamSynchronize(ctx->stream), "sync");
+  if (out_level) std::memcpy(out_level, lev.data(), npts * sizeof(int));
+  if (out_conv) std::memcpy(out_conv, cv.data(), npts);
+  if (stats) {
+    double ev = 0;
+    int mx = -1;
+    long nu = 0;
+    for (long i = 0; i < npts; ++i) {
+      if (lev[i] >= 0) ev += std::pow(2., lev[i] + 1) - 1;
+      mx = lev[i] > mx ? lev[i] : mx;
+      nu += cv[i] ? 0 : 1;
+    }
+    stats->evals = ev;
+    stats->max_level = mx;
+    stats->n_unconverged = nu;
+  }
+}
+
+int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
+            const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
+            double* out_e, om_stats* stats, bool spectrum) {
+  return guarded(ctx, [&] {
+    Timer wall;
+    check_particle(ctx, ip);
+    if (!conv) throw std::invalid_argument("conv is NULL");
+    if (n_obs < 1 || n_e < 1) throw std::length_error("no points specified");
+    if (conv->mode < OM_MODE_PLAIN || conv->mode > OM_MODE_POINT_MIXED) throw std::invalid_argument("unknown mode");
+    long npts = n_obs * n_e;
+    int stop_mem, stop_ext;
+    level_stops(*conv, stop_mem, stop_ext);
+    if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
+    std::vector<double> om(n_e);
+    for (long i = 0; i < n_e; ++i) om[i] = ev_to_omega(energies_eV[i]);
+    upload(ctx, ctx->d_obs, obs_xyz, 3 * n_obs);
+    upload(ctx, ctx->d_omega, om.data(), n_e);
+    bool mixed = conv->mode == OM_MODE_MIXED;
+    om::LwArgs A;
+    std::memset(&A, 0, sizeof(A));
+    state_for(ctx, npts, mixed, A.st);
+    A.npts = npts;
+    A.obs = ctx->d_obs.p;
+    A.omega = ctx->d_omega.p;
+    A.n_omega = n_e;
+    double pm2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      A.pol[2 * k] = pol6 ? pol6[2 * k] : 0;
+      A.pol[2 * k + 1] = pol6 ? pol6[2 * k + 1] : 0;
+      double r = A.pol[2 * k] * A.pol[2 * k] - A.pol[2 * k + 1] * (-A.pol[2 * k + 1]);
+      pm2 = k == 0 ? r : pm2 + r;
+    }
+    A.use_pol = pm2 > 0.001;
+    A.precision = conv->precision;
+    float kms = 0;
+    int La = 0;
+    int Lb = stop_ext < kFirstLevels ? stop_ext : kFirstLevels;
+    while (true) {
+      ensure_levels(ctx, ip, Lb, stop_mem);
+      fill_levels(ctx, ip, Lb, A.lv);
+      A.La = La;
+      A.Lb = Lb;
+      check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+      if (conv->mode == OM_MODE_PLAIN || conv->mode == OM_MODE_POINT_MIXED) {
+        check(om_launch_lw_point(&A, conv->mode == OM_MODE_POINT_MIXED ? 1 : 0, ctx->split, ctx->stream),
+              "lw_point");
+      } else {
+        check(om_launch_lw_tile(&A, n_obs, mixed ? 1 : 0, ctx->split, ctx->stream), "lw_tile");
+      }
+      check(hipEventRecord(ctx->ev1, ctx->stream), "event");
+      check(hipEventSynchronize(ctx->ev1), "sync");
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
+      kms += ms;
+      if (Lb >= stop_ext) break;
+      if (count_active(ctx, A.st, npts) == 0) break;
+      La = Lb + 1;
+      Lb = La;
+    }
+    const om_particle& P = ctx->parts[ip];
+    om::FinArgs F;
+    std::memset(&F, 0, sizeof(F));
+    F.st = A.st;
+    F.npts = npts;
+    F.kind = 0;
+    F.quantity = conv->quantity;
+    // src/OSCARSSR.cc:1882-1885 (spectrum: |I|) and 4123-4126 (flux: I)
+    F.C0 = P.q / (four_pi() * kC * kEps0 * kSqrt2Pi);
+    F.C2 = spectrum ? four_pi() * std::fabs(P.current) / (kH * std::fabs(P.q) * kMu0 * kC) * 1e-6 * 0.001
+                    : four_pi() * P.current / (kH * std::fabs(P.q) * kMu0 * kC) * 1e-6 * 0.001;
+    for (int L = 0; L <= om::kMaxLevel; ++L) F.dt_incl[L] = A.lv.dt_incl[L];
+    std::memcpy(F.pol, A.pol, sizeof(F.pol));
+    F.use_pol = A.use_pol;
+    finish_outputs(ctx, F, npts, out_value, out_level, out_conv, out_e, stats, *conv);
+    if (stats) {
+      stats->kernel_ms = kms;
+      stats->total_ms = wall.ms();
+    }
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int om_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int om_device_info(int device, char* buf, size_t len) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return 1;
+  std::snprintf(buf, len, "%s (%s) CUs=%d clock=%d kHz mem=%.1f GiB L2=%d KiB", p.name, p.gcnArchName,
+                p.multiProcessorCount, p.clockRate, (double) p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0),
+                p.l2CacheSize / 1024);
+  return 0;
+}
+
+int om_context_create(int device, om_context** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  int n = om_device_count();
+  if (n < 1) return 2;
+  if (device < 0 || device >= n) return 3;
+  om_context* c = new om_context();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return 4;
+  }
+  *out = c;
+  return 0;
+}
+
+void om_context_destroy(om_context* ctx) {
+  if (!ctx) return;
+  (void) hipSetDevice(ctx->device);
+  delete ctx;
+}
+
+const char* om_last_error(const om_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const om_particle* particles, int np) {
+  return guarded(ctx, [&] {
+    if (nfields < 0 || nfields > om::kMaxFields) throw std::out_of_range("too many fields");
+    if (np < 1) throw std::length_error("no particles");
+    ctx->free_levels();
+    for (auto& g : ctx->grid_bufs) g.release();
+    ctx->grid_bufs.clear();
+    ctx->grid_bufs.reserve(nfields);
+    std::vector<om::FieldDev> F;
+    for (int i = 0; i < nfields; ++i) F.push_back(make_field(ctx, fields[i]));
+    ctx->d_fields.ensure(nfields > 0 ? nfields : 1);
+    if (nfields > 0) upload(ctx, ctx->d_fields, F.data(), F.size());
+    ctx->np = 0;
+    ctx->desc.assign(np, om::TrajDesc());
+    ctx->parts.assign(particles, particles + np);
+    ctx->tstart.assign(np, 0);
+    ctx->tstop.assign(np, 0);
+    long total = 0;
+    for (int i = 0; i < np; ++i) {
+      const om_particle& P = particles[i];
+      om::TrajDesc& D = ctx->desc[i];
+      std::memset(&D, 0, sizeof(D));
+      // src/OSCARSSR.cc:871-884 argument checks, 914-919 point counts
+      if (P.ctstart > P.t0 || P.ctstop < P.t0)
+        throw std::out_of_range("start time is greater than T0 or stop time is less than T0");
+      if (P.ctstart >= P.ctstop)
+        throw std::out_of_range("start time is greater than stop time.  check that set_ctstartstop is set");
+      if (P.npoints < 2) throw std::length_error("Not enough points in this trajectory");
+      double DeltaT = ((P.ctstop - P.ctstart) / kC / (double) (size_t) (P.npoints - 1));
+      D.nf = (long) (size_t) (1 + (P.ctstop - P.t0) / kC / DeltaT);
+      D.nb = (long) (size_t) ((P.t0 - P.ctstart) / kC / DeltaT);
+      D.n = D.nf + D.nb;
+      if (D.n < 3) throw std::length_error("TSpline1D3 does not see enough points for interpolation.  Please use >= 3 points");
+      D.dt = DeltaT;
+      D.t0_over_c = P.t0 / kC;
+      D.qomg = P.qomg;
+      for (int k = 0; k < 3; ++k) {
+        D.x0[k] = P.x0[k];
+        D.b0[k] = P.beta0[k];
+      }
+      D.knot_offset = total;
+      total += D.n;
+      // trajectory time window: first and last knot times
+      ctx->tstart[i] = D.nb > 0 ? D.t0_over_c + (-DeltaT) * (double) D.nb : D.t0_over_c;
+      ctx->tstop[i] = D.t0_over_c + DeltaT * (double) (D.nf - 1);
+    }
+    ctx->d_t.ensure(total);
+    ctx->d_p.ensure(9 * total);
+    ctx->d_ypp.ensure(9 * total);
+    ctx->d_u.ensure(9 * total);
+    upload(ctx, ctx->d_desc, ctx->desc.data(), np);
+    om::FieldDev none;
+    std::memset(&none, 0, sizeof(none));
+    check(om_launch_rk4(ctx->d_fields.p, nfields > 0 ? &F[0] : &none, nfields, ctx->d_desc.p, np, ctx->d_t.p,
+                        ctx->d_p.p, ctx->stream), "rk4");
+    check(om_launch_spline(ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->d_ypp.p, ctx->d_u.p, ctx->stream),
+          "spline");
+    check(hipMemcpyAsync(ctx->desc.data(), ctx->d_desc.p, np * sizeof(om::TrajDesc), hipMemcpyDeviceToHost,
+                         ctx->stream), "desc copy");
+    check(hipStreamSynchronize(ctx->stream), "sync");
+    ctx->levels.assign(np, LevelCache());
+    ctx->np = np;
+  });
+}
+
+int om_trajectory_status(om_context* ctx, int i) {
+  if (!ctx || i < 0 || i >= ctx->np) return -1;
+  return ctx->desc[i].status;
+}
+
+int om_trajectory_get(om_context* ctx, int i, long* n, double* t, double* p9, double* ypp9) {
+  return guarded(ctx, [&] {
+    if (i < 0 || i >= ctx->np) throw std::out_of_range("trajectory index out of range");
+    const om::TrajDesc& D = ctx->desc[i];
+    if (!n) throw std::invalid_argument("n is NULL");
+    long cap = *n;
+    *n = D.n;
+    if (cap < D.n) return;
+    if (t)
+      check(hipMemcpy(t, ctx->d_t.p + D.knot_offset, D.n * sizeof(double), hipMemcpyDeviceToHost), "t");
+    if (p9)
+      check(hipMemcpy(p9, ctx->d_p.p + 9 * D.knot_offset, 9 * D.n * sizeof(double), hipMemcpyDeviceToHost), "p");
+    if (ypp9)
+      check(hipMemcpy(ypp9, ctx->d_ypp.p + 9 * D.knot_offset, 9 * D.n * sizeof(double), hipMemcpyDeviceToHost),
+            "ypp");
+  });
+}
+
+int om_level_get(om_context* ctx, int i, int level, int extended, double* rows) {
+  return guarded(ctx, [&] {
+    check_particle(ctx, i);
+    if (level < 0 || level > om::kMaxLevel) throw std::out_of_range("level out of range");
+    ensure_levels(ctx, i, level, extended ? level - 1 : om::kMaxLevel);
+    check(hipStreamSynchronize(ctx->stream), "sync");
+    check(hipMemcpy(rows, ctx->levels[i].rows[level], (1L << level) * 10 * sizeof(double), hipMemcpyDeviceToHost),
+          "level copy");
+  });
+}
+
+int om_flux(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
+            const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
+            double* out_e, om_stats* stats) {
+  return lw_sums(ctx, particle, obs_xyz, n_obs, energies_eV, n_e, pol6, conv, out_value, out_level, out_conv, out_e,
+                 stats, false);
+}
+
+int om_spectrum(om_context* ctx, int particle, const double* obs, const double* energies_eV, long n_e,
+                const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
+                double* out_e, om_stats* stats) {
+  return lw_sums(ctx, particle, obs, 1, energies_eV, n_e, pol6, conv, out_value, out_level, out_conv, out_e, stats,
+                 true);
+}
+
+int om_power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normal,
+                     int directional, const om_conv* conv, double* out_value, int* out_level,
+                     unsigned char* out_conv, om_stats* stats) {
+  return guarded(ctx, [&] {
+    Timer wall;
+    check_particle(ctx, ip);
+    if (!conv) throw std::invalid_argument("conv is NULL");
+    if (n_obs < 1) throw std::length_error("no points specified");
+    int stop_mem, stop_ext;
+    level_stops(*conv, stop_mem, stop_ext);
+    if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
+    upload(ctx, ctx->d_obs, obs_xyz, 3 * n_obs);
+    om::PdArgs A;
+    std::memset(&A, 0, sizeof(A));
+    state_for(ctx, n_obs, false, A.st);
+    A.npts = n_obs;
+    A.obs = ctx->d_obs.p;
+    for (int k = 0; k < 3; ++k) A.normal[k] = normal[k];
+    A.directional = directional;
+    A.precision = conv->precision;
+    A.two_over_gamma = 2. / (ctx->parts[ip].gamma);
+    float kms = 0;
+    int La = 0;
+    int Lb = stop_ext < kFirstLevels ? stop_ext : kFirstLevels;
+    while (true) {
+      ensure_levels(ctx, ip, Lb, stop_mem);
+      ensure_beta_diff(ctx, ip, Lb);
+      fill_levels(ctx, ip, Lb, A.lv);
+      A.La = La;
+      A.Lb = Lb;
+      check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+      check(om_launch_pd_point(&A, ctx->stream), "pd_point");
+      check(hipEventRecord(ctx->ev1, ctx->stream), "event");
+      check(hipEventSynchronize(ctx->ev1), "sync");
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
+      kms += ms;
+      if (Lb >= stop_ext) break;
+      if (count_active(ctx, A.st, n_obs) == 0) break;
+      La = Lb + 1;
+      Lb = La;
+    }
+    const om_particle& P = ctx->parts[ip];
+    om::FinArgs F;
+    std::memset(&F, 0, sizeof(F));
+    F.st = A.st;
+    F.npts = n_obs;
+    F.kind = 1;
+    F.quantity = conv->quantity;
+    // src/OSCARSSR.cc:2957
+    F.pd_const = std::fabs(P.q * P.current) / (16 * (kPi * kPi) * kEps0 * kC);
+    F.directional = directional;
+    for (int L = 0; L <= om::kMaxLevel; ++L) F.dt_incl[L] = A.lv.dt_incl[L];
+    finish_outputs(ctx, F, n_obs, out_value, out_level, out_conv, nullptr, stats, *conv);
+    if (stats) {
+      stats->kernel_ms = kms;
+      stats->total_ms = wall.ms();
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,611 @@
+// Lienard-Wiechert frequency-domain sums for gfx950.
+//
+// Reference behaviour restated (paths relative to the reference root):
+//   spectrum integrand     src/OSCARSSR.cc:1939-2038
+//   flux-map integrand     src/OSCARSSR.cc:4167-4266
+//   power-density integrand src/OSCARSSR.cc:2843-2966
+//
+// Output point = (observation point, photon energy) for spectrum / flux, or an
+// observation point for power density.  Each point sums the trajectory level
+// samples 0..L and stops at the first L > 8 meeting the reference's
+// convergence test; the per-point state (complex E sum, last |E|^2, level,
+// converged flag) lives in HBM so a launch can process a range of levels and
+// the host can continue with more levels for the points that need them.
+//
+// Kernels:
+//   lw_point_kernel   one lane = one output point, everything per sample in FP64
+//                     (the plain FP64 path; the trajectory row is wave-uniform and
+//                     is read through the scalar cache)
+//   lw_tile_kernel    one wave = one observation point x 64 photon energies; the
+//                     per-(obs, sample) retarded time tau = t + D/c and real
+//                     amplitude vector V are computed once per sample by one lane
+//                     and broadcast through LDS to the 64 energy lanes, which each
+//                     evaluate phi = -omega*tau, sin/cos and the complex MAC.
+//                     FP64 variant: bit-identical phase, FP64 sin/cos polynomial;
+//                     mixed variant: the same FP64 phase and exact FP64
+//                     reduction, then an FP32 sin/cos polynomial.  Amplitudes
+//                     and the complex accumulation stay FP64 in both: with
+//                     packed FP32 disabled, v_fma_f32 and v_fma_f64 issue at
+//                     the same rate on gfx950, so FP32 there would cost
+//                     accuracy and save nothing.
+//   pd_point_kernel   power density, one lane = one observation point
+//   lw_finalize_kernel  C0*dt scaling, polarization projection, flux / precision /
+//                     level output, optional weighted Kahan accumulation over particles
+#include "om_device.h"
+#include "om_types.h"
+
+namespace om {
+
+struct Levels {
+  const double* rows[kMaxLevel + 1];  // 2^L rows of 10 doubles per level (device)
+  double dt_incl[kMaxLevel + 1];      // (tstop - tstart) / (2^(L+1) - 1)
+  double beta_diff[kMaxLevel + 1];    // per level max |beta_i - beta_{i-1}| (power density)
+};
+
+struct State {
+  double* sum;      // 6 doubles per point (Re/Im of Ex, Ey, Ez), power density: sum[0]
+  double* comp;     // mixed path: Kahan compensation, 6 per point (may be null)
+  double* lastmag;  // |E|^2 (or power sum) after the previous level
+  double* prec;     // last relative change
+  int* level;       // last level summed (-1 before any)
+  int* conv;        // 1 once converged
+};
+
+struct LwArgs {
+  Levels lv;
+  State st;
+  int La, Lb;
+  long npts;
+  const double* obs;    // 3 per observation point
+  const double* omega;  // angular frequency per energy
+  long n_omega;         // point p -> (obs p / n_omega, energy p % n_omega)
+  double pol[6];
+  int use_pol;
+  double precision;
+};
+
+__device__ __forceinline__ void cmul(double ar, double ai, double br, double bi, double& rr, double& ri) {
+  double ac = ar * br, bd = ai * bi, ad = ar * bi, bc = ai * br;
+  rr = ac - bd;
+  ri = ad + bc;
+}
+
+// E <- (E . P) P with the bilinear dot (src/TVector3DC.cc:123-127)
+__device__ __forceinline__ void pol_project(double E[6], const double P[6]) {
+  double sr = 0, si = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double r, i;
+    cmul(E[2 * k], E[2 * k + 1], P[2 * k], P[2 * k + 1], r, i);
+    if (k == 0) { sr = r; si = i; } else { sr += r; si += i; }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) cmul(P[2 * k], P[2 * k + 1], sr, si, E[2 * k], E[2 * k + 1]);
+}
+
+__device__ __forceinline__ double cmag2(const double E[6]) {
+  double s = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double r = E[2 * k] * E[2 * k] - E[2 * k + 1] * (-E[2 * k + 1]);
+    s = k == 0 ? r : s + r;
+  }
+  return s;
+}
+
+// Per-level convergence test, src/OSCARSSR.cc:1997-2012.
+__device__ __forceinline__ bool level_converged(const LwArgs& A, int L, const double sum[6], double maxd,
+                                                double& lastmag, double& prec) {
+  double E[6];
+  double dt = A.lv.dt_incl[L];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) E[k] = sum[k] * dt;
+  if (A.use_pol) pol_project(E, A.pol);
+  double mag = cmag2(E);
+  prec = fabs(mag - lastmag) / lastmag;
+  bool c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
+  lastmag = mag;
+  return c;
+}
+
+// Real amplitude vector of one sample, src/OSCARSSR.cc:1975-1990:
+//   V = (1-|B|^2)(N-B)/(D^2 (1-N.B)^2) + N x ((N-B) x A)/(D (1-N.B)^2)
+// and tau = t + D/c, both in the reference's operation order.
+struct Amp {
+  double tau;
+  V3 V;
+};
+
+__device__ __forceinline__ Amp amplitude(V3 obs, const double* __restrict__ s) {
+  V3 X = v3(s[1], s[2], s[3]);
+  V3 B = v3(s[4], s[5], s[6]);
+  V3 Ac = v3(s[7], s[8], s[9]);
+  V3 R = sub(obs, X);
+  double D2 = mag2(R);
+  double D = sqrt(D2);
+  V3 N = v3(R.x / D, R.y / D, R.z / D);
+  Amp a;
+  a.tau = s[0] + D / kC;
+  double s1 = 1 - mag2(B);
+  V3 NmB = sub(N, B);
+  double om1 = 1 - dot(N, B);
+  double om2 = om1 * om1;
+  double den1 = D * D * om2;
+  double den2 = D * om2;
+  V3 c2 = cross(N, cross(NmB, Ac));
+  a.V = v3(NmB.x * s1 / den1 + c2.x / den2, NmB.y * s1 / den1 + c2.y / den2, NmB.z * s1 / den1 + c2.z / den2);
+  return a;
+}
+
+// Wave-scope barrier for LDS data exchanged only inside one wave (the waves of
+// a workgroup walk slices of different lengths, so a workgroup barrier cannot
+// be used inside the chunk loop).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------- level-slice combination
+// A workgroup of S waves serves 64 output points; wave w sums the contiguous
+// slice [ns*w/S, ns*(w+1)/S) of each level.  Each wave leaves its partial
+// complex sum, its largest phase step inside the slice and the phases of its
+// first and last sample in LDS; every wave then forms the same totals, in wave
+// order, so all waves agree on which points have converged.
+template <int S>
+struct SliceLds {
+  double ps[S][6][64];
+  double maxd[S][64];
+  double first[S][64];
+  double last[S][64];
+};
+
+template <int S>
+__device__ __forceinline__ void slice_publish(SliceLds<S>& sh, int w, int lane, const double ps[6], double maxd,
+                                              double first, double last) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sh.ps[w][k][lane] = ps[k];
+  sh.maxd[w][lane] = maxd;
+  sh.first[w][lane] = first;
+  sh.last[w][lane] = last;
+}
+
+template <int S>
+__device__ __forceinline__ void slice_combine(SliceLds<S>& sh, int lane, long ns, double tot[6], double& maxd) {
+  maxd = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) tot[k] = 0;
+  bool have_prev = false;
+  double prev_last = 0;
+  for (int w = 0; w < S; ++w) {
+    long i0 = ns * w / S, i1 = ns * (w + 1) / S;
+    if (i1 <= i0) continue;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tot[k] += sh.ps[w][k][lane];
+    maxd = fmax(maxd, sh.maxd[w][lane]);
+    if (have_prev) maxd = fmax(maxd, fabs(sh.first[w][lane] - prev_last));
+    prev_last = sh.last[w][lane];
+    have_prev = true;
+  }
+}
+
+__device__ __forceinline__ void sincos_mode(bool mixed, double phi, double& sn, double& cs) {
+  if (mixed) {
+    float sf, cf;
+    sincos_reduced32(reduce_pio2(phi), &sf, &cf);
+    sn = (double) sf;
+    cs = (double) cf;
+  } else {
+    sincos_huge64(phi, &sn, &cs);
+  }
+}
+
+// ---------------------------------------------------------------- one lane per point
+// MIXED = false is the plain FP64 path; MIXED = true evaluates sin/cos with the
+// FP32 polynomial on the exactly reduced FP64 phase.  Every lane reads the same
+// trajectory row (scalar loads) and computes its own amplitude and phase.
+template <bool MIXED, int S>
+__global__ void __launch_bounds__(64 * S) lw_point_kernel(LwArgs A) {
+  __shared__ SliceLds<S> sh;
+  int lane = threadIdx.x & 63;
+  int w = threadIdx.x >> 6;
+  long p = (long) blockIdx.x * 64 + lane;
+  bool valid = p < A.npts;
+  long pc = valid ? p : 0;
+  long io = pc / A.n_omega;
+  long iw = pc - io * A.n_omega;
+  V3 obs = v3(A.obs[3 * io], A.obs[3 * io + 1], A.obs[3 * io + 2]);
+  double negw = -A.omega[iw];
+  int lev = A.st.level[pc];
+  bool active = valid && !A.st.conv[pc] && lev == A.La - 1;
+  double sum[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sum[k] = A.st.sum[6 * pc + k];
+  double lastmag = A.st.lastmag[pc];
+  double prec = A.st.prec[pc];
+  for (int L = A.La; L <= A.Lb; ++L) {
+    if (!__any(active)) break;
+    const double* __restrict__ rows = A.lv.rows[L];
+    long ns = 1L << L;
+    long i0 = ns * w / S, i1 = ns * (w + 1) / S;
+    double ps[6] = {0, 0, 0, 0, 0, 0};
+    double maxd = 0, first = 0, lastphase = 0;
+    if (i1 > i0) {
+      Amp a0 = amplitude(obs, rows + 10 * i0);
+      first = negw * a0.tau;
+      lastphase = first;
+    }
+    for (long i = i0; i < i1; ++i) {
+      Amp a = amplitude(obs, rows + 10 * i);
+      double phi = negw * a.tau;
+      maxd = fmax(maxd, fabs(phi - lastphase));
+      lastphase = phi;
+      double sn, cs;
+      sincos_mode(MIXED, phi, sn, cs);
+      ps[0] = __builtin_fma(a.V.x, cs, ps[0]); ps[1] = __builtin_fma(a.V.x, sn, ps[1]);
+      ps[2] = __builtin_fma(a.V.y, cs, ps[2]); ps[3] = __builtin_fma(a.V.y, sn, ps[3]);
+      ps[4] = __builtin_fma(a.V.z, cs, ps[4]); ps[5] = __builtin_fma(a.V.z, sn, ps[5]);
+    }
+    double tot[6];
+    if (S > 1) {
+      slice_publish<S>(sh, w, lane, ps, maxd, first, lastphase);
+      __syncthreads();
+      slice_combine<S>(sh, lane, ns, tot, maxd);
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) tot[k] = ps[k];
+    }
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sum[k] += tot[k];
+      lev = L;
+      if (level_converged(A, L, sum, maxd, lastmag, prec)) {
+        A.st.conv[pc] = 1;
+        active = false;
+      }
+    }
+  }
+  if (w == 0 && valid && lev >= A.La) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) A.st.sum[6 * pc + k] = sum[k];
+    A.st.lastmag[pc] = lastmag;
+    A.st.prec[pc] = prec;
+    A.st.level[pc] = lev;
+  }
+}
+
+// ---------------------------------------------------------------- tiled obs x 64 energies
+// One workgroup of S waves per (observation point, 64-energy tile).  Within a
+// wave, lane j computes tau and V of sample base+j of the wave's slice and
+// leaves them in LDS; then all 64 lanes (energies) walk those 64 samples.
+template <bool MIXED, int S>
+__global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_w) {
+  __shared__ SliceLds<S> sh;
+  __shared__ double s_tau[S][64];
+  __shared__ double s_v[S][3][64];
+  int lane = threadIdx.x & 63;
+  int w = threadIdx.x >> 6;
+  long tile = blockIdx.x;
+  long io = tile / n_tiles_w;
+  long iw = (tile - io * n_tiles_w) * 64 + lane;
+  bool valid = iw < A.n_omega;
+  long iwc = valid ? iw : A.n_omega - 1;
+  long p = io * A.n_omega + iwc;
+  V3 obs = v3(A.obs[3 * io], A.obs[3 * io + 1], A.obs[3 * io + 2]);
+  double negw = -A.omega[iwc];
+  int lev = A.st.level[p];
+  bool active = valid && !A.st.conv[p] && lev == A.La - 1;
+  double sum[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sum[k] = A.st.sum[6 * p + k];
+  double lastmag = A.st.lastmag[p];
+  double prec = A.st.prec[p];
+  double* tau_w = s_tau[w];
+  double* vx_w = s_v[w][0];
+  double* vy_w = s_v[w][1];
+  double* vz_w = s_v[w][2];
+  for (int L = A.La; L <= A.Lb; ++L) {
+    if (!__any(active)) break;
+    const double* __restrict__ rows = A.lv.rows[L];
+    long ns = 1L << L;
+    long i0 = ns * w / S, i1 = ns * (w + 1) / S;
+    double ps[6] = {0, 0, 0, 0, 0, 0};
+    double maxd = 0, first = 0, lastphase = 0;
+    for (long base = i0; base < i1; base += 64) {
+      long si = base + lane;
+      if (si < i1) {
+        Amp a = amplitude(obs, rows + 10 * si);
+        tau_w[lane] = a.tau;
+        vx_w[lane] = a.V.x;
+        vy_w[lane] = a.V.y;
+        vz_w[lane] = a.V.z;
+      }
+      wave_sync();  // the wave's own LDS writes are visible to all its lanes
+      int cnt = (int) ((i1 - base) < 64 ? (i1 - base) : 64);
+      if (base == i0) {
+        first = negw * tau_w[0];
+        lastphase = first;
+      }
+      int k = 0;
+      for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          double phi = negw * tau_w[k + u];
+          maxd = fmax(maxd, fabs(phi - lastphase));
+          lastphase = phi;
+          double sn, cs;
+          sincos_mode(MIXED, phi, sn, cs);
+          double vx = vx_w[k + u], vy = vy_w[k + u], vz = vz_w[k + u];
+          ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+          ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+          ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+        }
+      }
+      for (; k < cnt; ++k) {
+        double phi = negw * tau_w[k];
+        maxd = fmax(maxd, fabs(phi - lastphase));
+        lastphase = phi;
+        double sn, cs;
+        sincos_mode(MIXED, phi, sn, cs);
+        double vx = vx_w[k], vy = vy_w[k], vz = vz_w[k];
+        ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+        ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+        ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+      }
+      wave_sync();  // all lanes done reading before the next chunk is written
+    }
+    double tot[6];
+    if (S > 1) {
+      slice_publish<S>(sh, w, lane, ps, maxd, first, lastphase);
+      __syncthreads();
+      slice_combine<S>(sh, lane, ns, tot, maxd);
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) tot[k] = ps[k];
+    }
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sum[k] += tot[k];
+      lev = L;
+      if (level_converged(A, L, sum, maxd, lastmag, prec)) {
+        A.st.conv[p] = 1;
+        active = false;
+      }
+    }
+  }
+  if (w == 0 && valid && lev >= A.La) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) A.st.sum[6 * p + k] = sum[k];
+    A.st.lastmag[p] = lastmag;
+    A.st.prec[p] = prec;
+    A.st.level[p] = lev;
+  }
+}
+
+// ---------------------------------------------------------------- power density
+struct PdArgs {
+  Levels lv;
+  State st;
+  int La, Lb;
+  long npts;
+  const double* obs;
+  double normal[3];
+  int directional;
+  double precision;
+  double two_over_gamma;
+};
+
+__global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
+  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = p < A.npts;
+  long pc = valid ? p : 0;
+  V3 obs = v3(A.obs[3 * pc], A.obs[3 * pc + 1], A.obs[3 * pc + 2]);
+  V3 nrm = v3(A.normal[0], A.normal[1], A.normal[2]);
+  int lev = A.st.level[pc];
+  bool active = valid && !A.st.conv[pc] && lev == A.La - 1;
+  double sum = A.st.sum[6 * pc];
+  double lastsum = A.st.lastmag[pc];
+  double prec = A.st.prec[pc];
+  for (int L = A.La; L <= A.Lb; ++L) {
+    if (!__any(active)) break;
+    const double* __restrict__ rows = A.lv.rows[L];
+    long ns = 1L << L;
+    for (long i = 0; i < ns; ++i) {
+      const double* s = rows + 10 * i;
+      V3 X = v3(s[1], s[2], s[3]);
+      V3 B = v3(s[4], s[5], s[6]);
+      V3 Ac = v3(s[7], s[8], s[9]);
+      V3 R = sub(obs, X);
+      double r2 = mag2(R);
+      double m = sqrt(r2);
+      V3 N1 = v3(R.x / m, R.y / m, R.z / m);
+      double ndn = dot(N1, nrm);
+      V3 Num = cross(N1, cross(sub(N1, B), Ac));
+      double x = 1 - dot(B, N1);
+      double x2 = x * x;
+      double den = x2 * x2 * x;
+      // (Num.N2)^2 + (Num.N3)^2 = |Num|^2 because Num is perpendicular to N1
+      double term = mag2(Num) / den / r2 * ndn;
+      bool skip = A.directional && ndn <= 0;
+      if (active && !skip) sum += term;
+    }
+    if (active) {
+      lev = L;
+      double thisSum = sum * A.lv.dt_incl[L];
+      prec = fabs(thisSum - lastsum) / lastsum;
+      double bd = A.lv.beta_diff[L];
+      bool c = (L > 8 && prec < A.precision && bd < A.two_over_gamma) ||
+               (L > 8 && bd < A.two_over_gamma && thisSum == lastsum);
+      bool c2 = !c && L > 8 && thisSum == lastsum;
+      if (c2) prec = 0;
+      lastsum = thisSum;
+      if (c || c2) {
+        A.st.conv[pc] = 1;
+        active = false;
+      }
+    }
+  }
+  if (valid && lev >= A.La) {
+    A.st.sum[6 * pc] = sum;
+    A.st.lastmag[pc] = lastsum;
+    A.st.prec[pc] = prec;
+    A.st.level[pc] = lev;
+  }
+}
+
+// ---------------------------------------------------------------- finalize
+struct FinArgs {
+  State st;
+  long npts;
+  int kind;       // 0 spectrum/flux, 1 power density
+  int quantity;   // 0 value, 1 precision, 2 level
+  double C0, C2;  // spectrum/flux constants (src/OSCARSSR.cc:1882-1885)
+  double pd_const;
+  int directional;
+  double dt_incl[kMaxLevel + 1];
+  double pol[6];
+  int use_pol;
+  double weight;
+  int accumulate;      // add weight*value into out with Kahan compensation (out_comp)
+  double* out;
+  double* out_comp;
+  double* out_e;       // 6 per point (may be null)
+  int* out_level;
+  unsigned char* out_conv;
+};
+
+__global__ void __launch_bounds__(256) lw_finalize_kernel(FinArgs F) {
+  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= F.npts) return;
+  int L = F.st.level[p];
+  int conv = F.st.conv[p];
+  double v;
+  if (F.kind == 0) {
+    double E[6];
+    double f = F.C0 * F.dt_incl[L];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) E[k] = F.st.sum[6 * p + k] * f;
+    if (F.use_pol) pol_project(E, F.pol);
+    if (F.out_e) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) F.out_e[6 * p + k] = E[k];
+    }
+    v = F.C2 * cmag2(E);
+  } else {
+    double s = F.st.sum[6 * p] * (F.pd_const * F.dt_incl[L]);
+    s /= 1e6;
+    if (!F.directional && s < 0) s *= -1;
+    v = s;
+  }
+  if (F.quantity == 1) v = F.st.prec[p];
+  else if (F.quantity == 2) v = conv ? (double) L : -1.0;
+  v *= F.weight;
+  if (F.accumulate) {
+    double sum = F.out[p];
+    double y = v - F.out_comp[p];
+    double t = sum + y;
+    F.out_comp[p] = (t - sum) - y;
+    F.out[p] = t;
+  } else {
+    F.out[p] = v;
+  }
+  if (F.out_level) F.out_level[p] = L;
+  if (F.out_conv) F.out_conv[p] = (unsigned char) conv;
+}
+
+__global__ void __launch_bounds__(256) state_reset_kernel(State st, long npts) {
+  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    st.sum[6 * p + k] = 0;
+    if (st.comp) st.comp[6 * p + k] = 0;
+  }
+  st.lastmag[p] = -1;
+  st.prec[p] = -1;
+  st.level[p] = -1;
+  st.conv[p] = 0;
+}
+
+__global__ void __launch_bounds__(256) count_active_kernel(State st, long npts, int* out) {
+  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  int a = (p < npts && !st.conv[p]) ? 1 : 0;
+  // wave-level reduce then one atomic per wave (vector memory atomic)
+  unsigned long long m = __ballot(a);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, __popcll(m));
+}
+
+}  // namespace om
+
+// ---------------------------------------------------------------- launchers (host)
+// Waves per workgroup: enough to put ~4096 waves on the chip when there are
+// few output points, one wave per 64 points otherwise.
+static int split_for(long groups) {
+  long target = 4096;
+  int s = 1;
+  while (s < 8 && groups * s < target) s *= 2;
+  return s;
+}
+
+template <bool M>
+static void launch_point(const om::LwArgs* A, int split, hipStream_t s) {
+  long blocks = (A->npts + 63) / 64;
+  dim3 g((unsigned) blocks);
+  switch (split) {
+    case 1: hipLaunchKernelGGL((om::lw_point_kernel<M, 1>), g, dim3(64), 0, s, *A); break;
+    case 2: hipLaunchKernelGGL((om::lw_point_kernel<M, 2>), g, dim3(128), 0, s, *A); break;
+    case 4: hipLaunchKernelGGL((om::lw_point_kernel<M, 4>), g, dim3(256), 0, s, *A); break;
+    default: hipLaunchKernelGGL((om::lw_point_kernel<M, 8>), g, dim3(512), 0, s, *A); break;
+  }
+}
+
+template <bool M>
+static void launch_tile(const om::LwArgs* A, long n_obs, int split, hipStream_t s) {
+  long ntw = (A->n_omega + 63) / 64;
+  dim3 g((unsigned) (n_obs * ntw));
+  switch (split) {
+    case 1: hipLaunchKernelGGL((om::lw_tile_kernel<M, 1>), g, dim3(64), 0, s, *A, ntw); break;
+    case 2: hipLaunchKernelGGL((om::lw_tile_kernel<M, 2>), g, dim3(128), 0, s, *A, ntw); break;
+    case 4: hipLaunchKernelGGL((om::lw_tile_kernel<M, 4>), g, dim3(256), 0, s, *A, ntw); break;
+    default: hipLaunchKernelGGL((om::lw_tile_kernel<M, 8>), g, dim3(512), 0, s, *A, ntw); break;
+  }
+}
+
+// split <= 0 picks the number of waves per workgroup from the grid size.
+extern "C" hipError_t om_launch_lw_point(const om::LwArgs* A, int mixed, int split, hipStream_t s) {
+  if (split <= 0) split = split_for((A->npts + 63) / 64);
+  if (mixed) launch_point<true>(A, split, s); else launch_point<false>(A, split, s);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_lw_tile(const om::LwArgs* A, long n_obs, int mixed, int split, hipStream_t s) {
+  if (split <= 0) split = split_for(n_obs * ((A->n_omega + 63) / 64));
+  if (mixed) launch_tile<true>(A, n_obs, split, s); else launch_tile<false>(A, n_obs, split, s);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s) {
+  long blocks = (A->npts + 255) / 256;
+  hipLaunchKernelGGL(om::pd_point_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *A);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s) {
+  long blocks = (F->npts + 255) / 256;
+  hipLaunchKernelGGL(om::lw_finalize_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *F);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s) {
+  long blocks = (npts + 255) / 256;
+  hipLaunchKernelGGL(om::state_reset_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *st, npts);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_count_active(const om::State* st, long npts, int* out, hipStream_t s) {
+  long blocks = (npts + 255) / 256;
+  hipLaunchKernelGGL(om::count_active_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *st, npts, out);
+  return hipGetLastError();
+}

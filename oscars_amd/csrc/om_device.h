@@ -1,0 +1,105 @@
+// Device-side helpers shared by the gfx950 kernels.
+//
+// All FP64 expressions that feed the retarded-time phase keep the reference's
+// association order (src/TVector3D.cc, src/OSCARSSR.cc:1975-1993) and the
+// library is compiled with -ffp-contract=off, so that the phase argument is
+// bit-identical to the reference CPU path.  FMAs are written explicitly only
+// where they cannot change that argument (sin/cos polynomials, exact
+// Cody-Waite reduction, accumulations).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sincos_coeffs.h"
+
+namespace om {
+
+// include/TOSCARS.h:20-76 (values kept verbatim; they are physical facts)
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kC = 2.99792458e8;
+
+// ---------------------------------------------------------------- vectors
+struct V3 {
+  double x, y, z;
+};
+
+__device__ __forceinline__ V3 v3(double x, double y, double z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ double mag2(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+__device__ __forceinline__ double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// src/TVector3D.cc Cross: (fY*Vz - Vy*fZ, fZ*Vx - Vz*fX, fX*Vy - Vx*fY)
+__device__ __forceinline__ V3 cross(V3 a, V3 v) {
+  return V3{a.y * v.z - v.y * a.z, a.z * v.x - v.z * a.x, a.x * v.y - v.x * a.y};
+}
+
+// ---------------------------------------------------------------- sin/cos of huge phases
+// The phase phi = -omega (t + D/c) reaches ~5e11 rad.  Reduction by pi/2 with
+// a two-part constant: k = nearest integer to phi*2/pi (obtained with the
+// 1.5*2^52 rounding constant, whose low mantissa bits then hold k mod 2^32),
+// r = phi - k*P1 is exact (phi and k*P1 are both multiples of 2^-52 and
+// |r| < 1), then r -= k*P2 rounds once.  |r| <= pi/4 + 2e-5.
+constexpr double kTwoOverPi = 0.63661977236758134308;
+constexpr double kPio2Hi = 1.5707963267948966192;      // nearest double to pi/2
+constexpr double kPio2Lo = 6.123233995736766036e-17;   // pi/2 - kPio2Hi (rounded)
+constexpr double kRoundMagic = 6755399441055744.0;     // 1.5 * 2^52
+
+struct Reduced {
+  double r;
+  int q;
+};
+
+__device__ __forceinline__ Reduced reduce_pio2(double phi) {
+  double kk = __builtin_fma(phi, kTwoOverPi, kRoundMagic);
+  int q = (int) (uint32_t) __double_as_longlong(kk);
+  double k = kk - kRoundMagic;
+  double r = __builtin_fma(-k, kPio2Hi, phi);
+  r = __builtin_fma(-k, kPio2Lo, r);
+  return Reduced{r, q};
+}
+
+// FP64 polynomial sin/cos on the reduced argument, quadrant fix-up by q & 3.
+__device__ __forceinline__ void sincos_reduced64(Reduced a, double* s, double* c) {
+  double r = a.r;
+  double u = r * r;
+  double ps = __builtin_fma(u, OM_S64_5, OM_S64_4);
+  ps = __builtin_fma(u, ps, OM_S64_3);
+  ps = __builtin_fma(u, ps, OM_S64_2);
+  ps = __builtin_fma(u, ps, OM_S64_1);
+  ps = __builtin_fma(u, ps, OM_S64_0);
+  double sr = __builtin_fma(r * u, ps, r);
+  double pc = __builtin_fma(u, OM_C64_5, OM_C64_4);
+  pc = __builtin_fma(u, pc, OM_C64_3);
+  pc = __builtin_fma(u, pc, OM_C64_2);
+  pc = __builtin_fma(u, pc, OM_C64_1);
+  pc = __builtin_fma(u, pc, OM_C64_0);
+  double cr = __builtin_fma(u * u, pc, __builtin_fma(-0.5, u, 1.0));
+  int q = a.q;
+  double ss = (q & 1) ? cr : sr;
+  double cc = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+__device__ __forceinline__ void sincos_huge64(double phi, double* s, double* c) {
+  sincos_reduced64(reduce_pio2(phi), s, c);
+}
+
+// FP32 polynomial sin/cos on the FP64-reduced argument (mixed-precision path).
+__device__ __forceinline__ void sincos_reduced32(Reduced a, float* s, float* c) {
+  float r = (float) a.r;
+  float u = r * r;
+  float ps = __builtin_fmaf(u, OM_S32_2, OM_S32_1);
+  ps = __builtin_fmaf(u, ps, OM_S32_0);
+  float sr = __builtin_fmaf(r * u, ps, r);
+  float pc = __builtin_fmaf(u, OM_C32_2, OM_C32_1);
+  pc = __builtin_fmaf(u, pc, OM_C32_0);
+  float cr = __builtin_fmaf(u * u, pc, __builtin_fmaf(-0.5f, u, 1.0f));
+  int q = a.q;
+  float ss = (q & 1) ? cr : sr;
+  float cc = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+}  // namespace om

@@ -1,0 +1,478 @@
+"""Drop-in ``oscars.sr``-style simulation object running on one MI355X.
+
+Mirrors the reference Python API for the radiation hot path
+(src/OSCARSSR_Python.cc: ``set_particle_beam`` :3561, ``add_bfield_undulator``
+:1309, ``add_bfield_epu`` :1479, ``add_bfield_file`` :526, ``set_ctstartstop``
+:345, ``set_npoints_trajectory`` :409, ``calculate_trajectory`` :5181,
+``calculate_spectrum`` :5653, ``calculate_flux_rectangle`` :7977,
+``calculate_power_density_rectangle`` :6471): same keyword names, defaults,
+return shapes and ValueError messages.  Every calculation runs on the GPU
+through liboscars_mi.so; there is no CPU path, so ``gpu=0`` still uses the GPU.
+
+Extensions beyond the reference (keyword ``mode`` and the method
+``calculate_flux_rectangle_energies``) select the kernel family and evaluate
+many energies in one call.
+"""
+import math
+import sys
+
+import numpy as np
+
+from . import _lib
+from . import geometry as G
+
+_NOANGLE = 112188979912321
+
+
+def _v3(x, what):
+    try:
+        v = [float(a) for a in x]
+    except TypeError:
+        raise ValueError("Incorrect format in '%s'" % what)
+    if len(v) != 3:
+        raise ValueError("Incorrect format in '%s'" % what)
+    return v
+
+
+def _warn(msg):
+    sys.stderr.write(msg)
+
+
+class sr:
+    """Simulation object: fields, one particle beam, trajectory window, and the
+    spectrum / flux / power-density calculations."""
+
+    def __init__(self, nthreads=2, gpu=0, device=0):
+        self._fields = []        # list of dicts kept for om_field construction
+        self._grids = []         # keep numpy arrays alive
+        self._beam = None
+        self._particle = None
+        self._ctstart = 0.0
+        self._ctstop = 0.0
+        self._npoints = 0
+        self._npoints_per_meter = 10000
+        self._nthreads = nthreads
+        self._gpu_global = gpu
+        self._device_index = device
+        self._dev = None
+        self._traj_valid = False
+        self.last_stats = None
+
+    # ------------------------------------------------------------ device
+    def _device(self):
+        if self._dev is None:
+            self._dev = _lib.Device(self._device_index)
+        return self._dev
+
+    def print_gpu(self):
+        print(self._device().info())
+
+    # ------------------------------------------------------------ fields
+    def add_bfield_undulator(self, bfield, period, nperiods, phase=0.0, rotations=None, translation=None, taper=0.0,
+                             frequency=0.0, frequency_phase=0.0, time_offset=0.0, name=""):
+        if frequency != 0:
+            raise ValueError("time-dependent fields are not supported by this implementation")
+        rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+        tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        f = _lib.om_field()
+        f.type = _lib.OM_FIELD_UNDULATOR
+        f.field[:] = G.rot_xyz(_v3(bfield, "bfield"), rot)
+        f.period[:] = G.rot_xyz(_v3(period, "period"), rot)
+        f.nperiods = int(nperiods)
+        f.translation[:] = tr
+        f.phase = float(phase)
+        f.taper = float(taper)
+        self._fields.append(f)
+        self._traj_valid = False
+
+    def add_bfield_epu(self, bfieldA, bfieldB, period, nperiods, phase=0.0, rotations=None, translation=None,
+                       taper=0.0, frequency=0.0, frequency_phase=0.0, time_offset=0.0, name=""):
+        if frequency != 0:
+            raise ValueError("time-dependent fields are not supported by this implementation")
+        rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+        tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        f = _lib.om_field()
+        f.type = _lib.OM_FIELD_EPU
+        f.field[:] = G.rot_xyz(_v3(bfieldA, "bfieldA"), rot)
+        f.fieldB[:] = G.rot_xyz(_v3(bfieldB, "bfieldB"), rot)
+        f.period[:] = G.rot_xyz(_v3(period, "period"), rot)
+        f.nperiods = int(nperiods)
+        f.translation[:] = tr
+        f.phase = float(phase)
+        f.taper = float(taper)
+        self._fields.append(f)
+        self._traj_valid = False
+
+    def add_bfield_grid(self, start, step, npoints, samples, rotations=None, translation=None):
+        """Regular 1-D field map from header values and a (n, 3) sample array
+        (what add_bfield_file reads from an OSCARS-format file)."""
+        s = np.ascontiguousarray(samples, dtype=np.float64).reshape(-1, 3)
+        f = _lib.om_field()
+        f.type = _lib.OM_FIELD_GRID
+        f.grid_start[:] = [float(v) for v in start]
+        f.grid_step[:] = [float(v) for v in step]
+        f.grid_n[:] = [int(v) for v in npoints]
+        if int(np.prod(list(npoints))) != len(s):
+            raise ValueError("error reading file.  Check format")
+        f.grid_data = s.ctypes.data_as(_lib._dp)
+        f.rotations[:] = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+        f.translation[:] = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        self._grids.append(s)
+        self._fields.append(f)
+        self._traj_valid = False
+
+    def add_bfield_file(self, ifile, iformat="OSCARS", rotations=None, translation=None, scale=None, name=""):
+        if iformat.upper() != "OSCARS":
+            raise ValueError("incorrect format given")
+        if scale:
+            raise ValueError("'scale' is not supported by this implementation")
+        start, step, npts, data = read_oscars_field_file(ifile)
+        self.add_bfield_grid(start, step, npts, data, rotations, translation)
+
+    def clear_bfields(self):
+        self._fields = []
+        self._grids = []
+        self._traj_valid = False
+
+    # ------------------------------------------------------------ beam / trajectory window
+    def set_particle_beam(self, type="electron", name="", energy_GeV=-1, d0=None, x0=None, beam="",
+                          sigma_energy_GeV=0, t0=0, current=0, weight=1, rotations=None, translation=None,
+                          horizontal_direction=None, beta=None, alpha=None, gamma=None, emittance=None, eta=None,
+                          lattice_reference=None, mass=0, charge=0, ctstartstop=None, distribution=""):
+        if beam:
+            raise ValueError("predefined beams are not supported by this implementation")
+        if energy_GeV == -1:
+            energy_GeV = 0
+        if energy_GeV < 0:
+            raise ValueError("energy must be > 0")
+        pos = _v3(x0, "x0") if x0 is not None else [0.0, 0.0, 0.0]
+        direction = _v3(d0, "d0") if d0 is not None else [0.0, 0.0, 1.0]
+        rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+        tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        pos = G.rot_xyz(pos, rot)
+        direction = G.rot_xyz(direction, rot)
+        pos = [pos[k] + tr[k] for k in range(3)]
+        p = G.ideal_particle(type, energy_GeV, pos, direction, t0, current, charge, mass)
+        self._beam = dict(type=type, energy_GeV=energy_GeV, x0=pos, d0=direction, t0=float(t0),
+                          current=float(current), sigma_energy_GeV=float(sigma_energy_GeV), beta=beta, alpha=alpha,
+                          emittance=emittance, horizontal_direction=horizontal_direction, q=p.q, m=p.m)
+        self._particle = p
+        if ctstartstop is not None:
+            if len(ctstartstop) != 2:
+                raise ValueError("Incorrect format in 'ctstartstop'")
+            self.set_ctstartstop(ctstartstop[0], ctstartstop[1])
+        else:
+            self.set_ctstartstop(-3, 3)
+        self._traj_valid = False
+
+    def set_new_particle(self, beam="", particle=""):
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        if particle.lower() in ("", "ideal"):
+            b = self._beam
+            self._particle = G.ideal_particle(b["type"], b["energy_GeV"], b["x0"], b["d0"], b["t0"], b["current"],
+                                              0, b["m"])
+        else:
+            raise ValueError("random particles: use nparticles= or set_particle()")
+        self._traj_valid = False
+
+    def set_particle(self, x0, beta0, t0=0.0):
+        """Explicit initial conditions for the current particle (extension)."""
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        b = self._beam
+        self._particle = G.Particle(b["q"], b["m"], x0, beta0, t0, b["current"])
+        self._traj_valid = False
+
+    def set_ctstartstop(self, start, stop):
+        # src/OSCARSSR.cc:637-647
+        self._npoints = int(self._npoints_per_meter * (stop - start))
+        self._ctstart = float(start)
+        self._ctstop = float(stop)
+        self._traj_valid = False
+
+    def set_npoints_trajectory(self, npoints):
+        if npoints < 1:
+            raise ValueError("npoints must be >= 1")
+        self._npoints = int(npoints)
+        self._traj_valid = False
+
+    def set_npoints_per_meter_trajectory(self, npoints):
+        self._npoints_per_meter = int(npoints)
+        if self._ctstart != self._ctstop:
+            self.set_ctstartstop(self._ctstart, self._ctstop)
+
+    def get_ctstart(self):
+        return self._ctstart
+
+    def get_ctstop(self):
+        return self._ctstop
+
+    def get_npoints_trajectory(self):
+        return self._npoints
+
+    def _om_particle(self, p, ctstart=None, ctstop=None, npoints=None):
+        o = _lib.om_particle()
+        o.x0[:] = p.x0
+        o.beta0[:] = p.b0
+        o.t0 = p.t0
+        o.q = p.q
+        o.m = p.m
+        o.gamma = p.gamma
+        o.qomg = p.qomg
+        o.current = p.current
+        o.ctstart = self._ctstart if ctstart is None else ctstart
+        o.ctstop = self._ctstop if ctstop is None else ctstop
+        o.npoints = self._npoints if npoints is None else npoints
+        return o
+
+    def _ensure_trajectory(self):
+        if self._particle is None:
+            raise ValueError("No particle beam defined")
+        if self._traj_valid:
+            return
+        if self._ctstart > self._particle.t0 or self._ctstop < self._particle.t0:
+            raise ValueError("start time is greater than T0 or stop time is less than T0")
+        if self._ctstart >= self._ctstop:
+            raise ValueError("start time is greater than stop time.  check that set_ctstartstop is set")
+        self._device().trajectories(self._fields, [self._om_particle(self._particle)])
+        self._traj_valid = True
+
+    def calculate_trajectory(self):
+        """[[t, [x, y, z], [bx, by, bz], [ax, ay, az]], ...] like the reference."""
+        self._ensure_trajectory()
+        t, p, _ = self._device().trajectory(0)
+        return [[float(t[i]), p[i, 0:3].tolist(), p[i, 3:6].tolist(), p[i, 6:9].tolist()] for i in range(len(t))]
+
+    def get_trajectory_arrays(self):
+        """(t, knots[n, 9], spline second derivatives[n, 9]) as numpy arrays (extension)."""
+        self._ensure_trajectory()
+        return self._device().trajectory(0)
+
+    # ------------------------------------------------------------ argument helpers
+    @staticmethod
+    def _quantity(q, pd=False):
+        Q = q.upper()
+        if Q == ("POWER DENSITY" if pd else "FLUX"):
+            return 0
+        if Q == "PRECISION":
+            return 1
+        if Q == "LEVEL":
+            return 2
+        if pd:
+            raise ValueError("'quantity' must be: 'power density', 'precision', 'level', or blank")
+        raise ValueError("'quantity' must be: 'flux', 'precision', 'level', or blank")
+
+    @staticmethod
+    def _check_common(nparticles, nthreads, gpu):
+        if nparticles < 0:
+            raise ValueError("'nparticles' must be >= 1 (sort of)")
+        if nthreads < 0:
+            raise ValueError("'nthreads' must be > 0")
+        if gpu not in (0, 1, -1):
+            raise ValueError("'gpu' must be 0 or 1")
+        if nthreads > 0 and gpu == 1:
+            raise ValueError("gpu is 1 and nthreads > 0.  Both are not currently allowed.")
+
+    @staticmethod
+    def _polarization(polarization, angle, hdir, pdir):
+        if polarization == "all" and angle != _NOANGLE:
+            raise ValueError("cannot specify 'all' and 'angle'.  This is to save you")
+        pol = "all"
+        if polarization:
+            pol = polarization
+        elif angle != _NOANGLE:
+            pol = "linear"
+        return G.polarization_vector(pol, angle if angle != _NOANGLE else 0.0, hdir, pdir)
+
+    def _particles_for(self, nparticles):
+        """Particles to sum: the current one (nparticles=0) or nparticles samples."""
+        if nparticles == 0:
+            return None
+        from .beam import sample_particles
+        return sample_particles(self._beam, nparticles)
+
+    def _run_particles(self, nparticles, fn):
+        """Call fn(device, particle_index, weight, accumulate) for every particle."""
+        plist = self._particles_for(nparticles)
+        if plist is None:
+            self._ensure_trajectory()
+            return fn(self._device(), 0, 1.0, False)
+        dev = self._device()
+        dev.trajectories(self._fields, [self._om_particle(p) for p in plist])
+        self._traj_valid = False
+        w = 1.0 / float(len(plist))
+        out = None
+        for i in range(len(plist)):
+            out = fn(dev, i, w, i > 0, out)
+        return out
+
+    # ------------------------------------------------------------ spectrum
+    def calculate_spectrum(self, obs=None, npoints=0, energy_range_eV=None, energy_points_eV=None, points_eV=None,
+                           polarization="", angle=_NOANGLE, horizontal_direction=None, propogation_direction=None,
+                           precision=0.01, max_level=-2, max_level_extended=0, nparticles=0, nthreads=0, gpu=-1,
+                           ngpu=None, quantity="flux", ofile="", bofile="", mode="tiled"):
+        self._check_common(nparticles, nthreads, 0 if gpu == 0 else gpu)
+        if points_eV is not None and energy_points_eV is None:
+            energy_points_eV = points_eV
+        if energy_points_eV is not None:
+            if len(energy_points_eV) == 0:
+                raise ValueError("empty list of energy points")
+            e = [float(v) for v in energy_points_eV]
+        else:
+            e0 = e1 = 0.0
+            if energy_range_eV is not None:
+                if len(energy_range_eV) != 2:
+                    raise ValueError("'energy_range_eV' must be a list of length 2")
+                e0, e1 = float(energy_range_eV[0]), float(energy_range_eV[1])
+            if npoints < 1:
+                npoints = abs(int(e1 - e0) + 1) if math.fabs(e1 - e0) + 1 > 100 else 100
+            e = G.energies(npoints, e0, e1)
+        if obs is None:
+            raise ValueError("Incorrect format in 'obs'")
+        o = _v3(obs, "obs")
+        hdir = _v3(horizontal_direction, "horizontal_direction") if horizontal_direction else [1.0, 0.0, 0.0]
+        pdir = _v3(propogation_direction, "propogation_direction") if propogation_direction else [0.0, 0.0, 1.0]
+        q = self._quantity(quantity)
+        pol = self._polarization(polarization, angle, hdir, pdir)
+        e_arr = np.array(e)
+
+        def run(dev, ip, w, acc, out=None):
+            conv = dev.conv(precision, max_level, max_level_extended, q, mode, w, acc)
+            v, lev, cv, _, st = dev.spectrum(ip, o, e_arr, pol, conv, out=out)
+            self.last_stats = st
+            self._last_levels = lev
+            self._last_conv = cv
+            return v
+
+        v = self._run_particles(nparticles, run)
+        if not self._last_conv.all():
+            _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
+        return [[e[i], float(v[i])] for i in range(len(e))]
+
+    # ------------------------------------------------------------ flux
+    def _rect(self, npoints, plane, width, rotations, translation, normal, dim, x0x1x2):
+        if dim not in (2, 3):
+            raise ValueError("'dim' must be 2 or 3")
+        if npoints is None or len(npoints) != 2:
+            raise ValueError("'npoints' must be [int, int]")
+        n1, n2 = int(npoints[0]), int(npoints[1])
+        if n1 <= 0 or n2 <= 0:
+            raise ValueError("an entry in 'npoints' is <= 0")
+        if abs(normal) > 1:
+            raise ValueError("'normal' must be -1, 0, or 1")
+        if x0x1x2 is not None:
+            raise ValueError("'x0x1x2' surfaces are not supported by this implementation")
+        rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+        tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        if width is None or len(width) != 2 or not (width[0] > 0 and width[1] > 0):
+            raise ValueError("must specify 'width' (positive only)")
+        return G.rectangle(plane, n1, n2, float(width[0]), float(width[1]), rot, tr, normal, dim)
+
+    def calculate_flux_rectangle(self, energy_eV, npoints=None, plane="XY", normal=0, dim=2, width=None,
+                                 rotations=None, translation=None, x0x1x2=None, polarization="", angle=_NOANGLE,
+                                 horizontal_direction=None, propogation_direction=None, nparticles=0, nthreads=0,
+                                 gpu=-1, ngpu=None, precision=0.01, max_level=-2, max_level_extended=0,
+                                 quantity="flux", ofile="", bofile="", mode="plain"):
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        self._check_common(nparticles, nthreads, 0 if gpu == 0 else gpu)
+        obs, nv, cont = self._rect(npoints, plane, width, rotations, translation, normal, dim, x0x1x2)
+        hdir = _v3(horizontal_direction, "translation") if horizontal_direction else [1.0, 0.0, 0.0]
+        pdir = _v3(propogation_direction, "translation") if propogation_direction else [0.0, 0.0, 1.0]
+        q = self._quantity(quantity)
+        pol = self._polarization(polarization, angle, hdir, pdir)
+        obs_a = np.array(obs)
+
+        def run(dev, ip, w, acc, out=None):
+            conv = dev.conv(precision, max_level, max_level_extended, q, mode, w, acc)
+            v, lev, cv, _, st = dev.flux(ip, obs_a, [float(energy_eV)], pol, conv, out=out)
+            self.last_stats = st
+            self._last_levels = lev
+            self._last_conv = cv
+            return v
+
+        v = self._run_particles(nparticles, run)
+        if not self._last_conv.all():
+            _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
+        return [[list(cont[i]), float(v[i])] for i in range(len(cont))]
+
+    def calculate_flux_rectangle_energies(self, energies_eV, npoints=None, plane="XY", normal=0, dim=2, width=None,
+                                          rotations=None, translation=None, polarization="", angle=_NOANGLE,
+                                          horizontal_direction=None, propogation_direction=None, nparticles=0,
+                                          precision=0.01, max_level=-2, max_level_extended=0, quantity="flux",
+                                          mode="tiled"):
+        """Flux on a rectangle at many energies in one call (extension).
+        Returns (coords[n_obs, 3], flux[n_obs, n_energies])."""
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        obs, nv, cont = self._rect(npoints, plane, width, rotations, translation, normal, dim, None)
+        hdir = _v3(horizontal_direction, "translation") if horizontal_direction else [1.0, 0.0, 0.0]
+        pdir = _v3(propogation_direction, "translation") if propogation_direction else [0.0, 0.0, 1.0]
+        q = self._quantity(quantity)
+        pol = self._polarization(polarization, angle, hdir, pdir)
+        obs_a = np.array(obs)
+        e = np.ascontiguousarray(energies_eV, dtype=np.float64)
+
+        def run(dev, ip, w, acc, out=None):
+            conv = dev.conv(precision, max_level, max_level_extended, q, mode, w, acc)
+            v, lev, cv, _, st = dev.flux(ip, obs_a, e, pol, conv, out=out)
+            self.last_stats = st
+            self._last_levels = lev
+            self._last_conv = cv
+            return v
+
+        v = self._run_particles(nparticles, run)
+        return np.array(cont), v.reshape(len(obs), len(e))
+
+    # ------------------------------------------------------------ power density
+    def calculate_power_density_rectangle(self, npoints=None, plane="XY", normal=0, width=None, rotations=None,
+                                          translation=None, x0x1x2=None, nparticles=0, gpu=-1, ngpu=None,
+                                          nthreads=0, precision=0.01, max_level=-2, max_level_extended=0, dim=2,
+                                          quantity="power density", ofile="", bofile=""):
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        if npoints is None:
+            npoints = [51, 51]
+        if width is None:
+            width = [0.010, 0.010]
+        self._check_common(nparticles, nthreads, 0 if gpu == 0 else gpu)
+        obs, nv, cont = self._rect(npoints, plane, width, rotations, translation, normal, dim, x0x1x2)
+        q = self._quantity(quantity, pd=True)
+        directional = normal != 0
+        obs_a = np.array(obs)
+
+        def run(dev, ip, w, acc, out=None):
+            conv = dev.conv(precision, max_level, max_level_extended, q, "plain", w, acc)
+            v, lev, cv, st = dev.power_density(ip, obs_a, nv, directional, conv, out=out)
+            self.last_stats = st
+            self._last_levels = lev
+            self._last_conv = cv
+            return v
+
+        v = self._run_particles(nparticles, run)
+        if not self._last_conv.all():
+            _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
+        return [[list(cont[i]), float(v[i])] for i in range(len(cont))]
+
+
+def read_oscars_field_file(path):
+    """Header values and samples of an OSCARS-format text field map
+    (src/TField3D_Grid.cc:353-548: a comment line, nine header lines whose
+    first token is the value, then one 'Bx By Bz' line per grid point)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    if len(lines) < 10:
+        raise ValueError("error reading file.  Check format")
+    hv = [float(l.split()[0]) for l in lines[1:10]]
+    start = [hv[0], hv[3], hv[6]]
+    step = [hv[1], hv[4], hv[7]]
+    npts = [int(hv[2]), int(hv[5]), int(hv[8])]
+    if min(npts) < 1:
+        raise ValueError("invalid number of points in at least one dimension")
+    n = npts[0] * npts[1] * npts[2]
+    rows = lines[10:10 + n]
+    if len(rows) < n:
+        raise ValueError("error reading file.  Check format")
+    data = np.array([[float(v) for v in l.split()[:3]] for l in rows])
+    return start, step, npts, data

@@ -1,0 +1,32 @@
+"""Shared scenario set-up for the tests: the same calls drive the product
+(oscars_amd.sr), the oracle and, where built, the reference core."""
+import math
+
+UNDULATOR = dict(bfield=[0, 1, 0], period=[0, 0, 0.049], nperiods=31)
+EPU = dict(bfieldA=[0, 0.6, 0], bfieldB=[0.6, 0, 0], period=[0, 0, 0.057], nperiods=40, phase=math.pi / 2)
+
+
+def planar(s, npoints=20001):
+    """Planar undulator scenario (1.0 T, 49 mm, 31 periods, 3 GeV, 0.5 A)."""
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_undulator(**UNDULATOR)
+    s.set_ctstartstop(0, 2)
+    s.set_npoints_trajectory(npoints)
+
+
+def epu(s, npoints=40001):
+    """Elliptically polarising undulator scenario (0.6 T x 0.6 T, 57 mm, 40 periods)."""
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_epu(**EPU)
+    s.set_ctstartstop(0, 3)
+    s.set_npoints_trajectory(npoints)
+
+
+def rel_err(a, b, floor_frac=0.0):
+    """max |a-b| / max(|b|, floor_frac * max|b|) elementwise."""
+    import numpy as np
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    den = np.maximum(np.abs(b), floor_frac * np.max(np.abs(b)))
+    den = np.where(den == 0, 1.0, den)
+    return float(np.max(np.abs(a - b) / den))

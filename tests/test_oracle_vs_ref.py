@@ -1,0 +1,138 @@
+"""The C oracle against the reference's compiled C++ core (oracle/_ref), where
+it was built from the reference sources.  Every comparison is bit-exact,
+including the thread-partition-dependent reporting of levels (carry_quirk)."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.common import planar, epu
+
+pytestmark = pytest.mark.skipif(not __import__("oracle.pyoracle", fromlist=["x"]).ref_available(),
+                                reason="reference core not built (oracle/_ref/libref.so)")
+
+
+def pair(setup, nthreads=3, **kw):
+    from oracle.pyoracle import Oracle, RefSim
+    o = Oracle(nthreads=nthreads)
+    r = RefSim(nthreads=nthreads)
+    setup(o, **kw)
+    setup(r, **kw)
+    return o, r
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+
+
+def test_particle_and_fields():
+    o, r = pair(planar)
+    assert o.particle() == {k: (list(map(float, v)) if isinstance(v, list) else float(v))
+                            for k, v in r.particle().items()}
+    for z in np.linspace(-0.9, 0.9, 37):
+        for x in (0.0, 1e-3):
+            assert same(o.get_bfield([x, 0, z]), r.get_bfield([x, 0, z]))
+
+
+@pytest.mark.parametrize("npts", [1001, 2000])
+def test_trajectory_backward_and_forward(npts):
+    """T0 inside the window: points both before and after T0."""
+    def setup(s):
+        s.set_particle_beam(energy_GeV=2.5, x0=[0.001, 0, -0.2], d0=[0.0005, 0, 1], current=0.3)
+        s.add_bfield_undulator(bfield=[0, 0.8, 0.05], period=[0, 0, 0.05], nperiods=10, phase=0.3, taper=0.01)
+        s.set_ctstartstop(-0.5, 0.6)
+        s.set_npoints_trajectory(npts)
+    o, r = pair(setup)
+    T = o.calculate_trajectory()
+    t, p, ypp = r.calculate_trajectory()
+    assert same(T.t, t) and same(T.p, p) and same(T.ypp, ypp)
+
+
+def test_epu_trajectory_and_spectrum():
+    o, r = pair(epu, npoints=8001)
+    T = o.calculate_trajectory()
+    t, p, ypp = r.calculate_trajectory()
+    assert same(T.t, t) and same(T.p, p)
+    for pol in ("", "cl", "cr", "lv"):
+        e, v, lev, conv, ef = o.calculate_spectrum([0.0005, 0.0002, 20], npoints=23, energy_range_eV=[200, 900],
+                                                   polarization=pol)
+        re, rv, rconv = r.calculate_spectrum([0.0005, 0.0002, 20], npoints=23, energy_range_eV=[200, 900],
+                                             polarization=pol)
+        assert same(v, rv), pol
+
+
+@pytest.mark.parametrize("plane,rot", [("XY", (0, 0, 0)), ("YX", (0, 0, 0.2)), ("XZ", (math.pi / 2, 0, 0)),
+                                       ("ZY", (0, 0.1, 0))])
+def test_flux_planes_and_rotations(plane, rot):
+    o, r = pair(planar, npoints=8001)
+    kw = dict(plane=plane, rotations=rot, translation=[0.001, 0, 30])
+    cont, v, lev, conv, ef, obs = o.calculate_flux_rectangle(300.0, [5, 4], [0.01, 0.008], **kw)
+    xyz, rv, rconv = r.calculate_flux_rectangle(300.0, [5, 4], [0.01, 0.008], **kw)
+    assert same(cont, xyz) and same(v, rv)
+
+
+@pytest.mark.parametrize("angle", [0.0, 0.7])
+def test_linear_polarization_angle(angle):
+    o, r = pair(planar, npoints=8001)
+    e, v, _, _, _ = o.calculate_spectrum([0, 0.0003, 30], npoints=11, energy_range_eV=[140, 170], angle=angle)
+    re, rv, _ = r.calculate_spectrum([0, 0.0003, 30], npoints=11, energy_range_eV=[140, 170], angle=angle)
+    assert same(v, rv)
+
+
+@pytest.mark.parametrize("nthreads", [1, 3])
+@pytest.mark.parametrize("quantity", ["flux", "level", "precision"])
+def test_nonconverged_carry(nthreads, quantity):
+    """max_level below the convergence threshold: the reported level/precision
+    follows the reference's per-thread carry, which depends on nthreads."""
+    o, r = pair(planar, nthreads=nthreads, npoints=8001)
+    kw = dict(max_level=8, quantity=quantity)
+    e, v, lev, conv, _ = o.calculate_spectrum([0, 0, 30], npoints=17, energy_range_eV=[100, 500], **kw)
+    re, rv, rconv = r.calculate_spectrum([0, 0, 30], npoints=17, energy_range_eV=[100, 500], **kw)
+    assert same(v, rv)
+    assert bool(conv.all()) == rconv
+    cont, v, lev, conv, _, _ = o.calculate_flux_rectangle(200.0, [4, 3], [0.01, 0.01], translation=[0, 0, 30],
+                                                          max_level=9, **{"quantity": quantity})
+    xyz, rv, rconv = r.calculate_flux_rectangle(200.0, [4, 3], [0.01, 0.01], translation=[0, 0, 30], max_level=9,
+                                                quantity=quantity)
+    assert same(v, rv)
+
+
+def test_extended_levels():
+    """Levels above max_level are 'extended' (spacing T/(2^L - 1))."""
+    o, r = pair(planar, npoints=8001)
+    kw = dict(max_level=9, max_level_extended=12)
+    e, v, lev, conv, _ = o.calculate_spectrum([0, 0, 30], npoints=9, energy_range_eV=[1000, 2000], **kw)
+    re, rv, _ = r.calculate_spectrum([0, 0, 30], npoints=9, energy_range_eV=[1000, 2000], **kw)
+    assert same(v, rv)
+
+
+@pytest.mark.parametrize("normal", [0, -1, 1])
+def test_power_density_directional(normal):
+    o, r = pair(epu, npoints=8001)
+    kw = dict(translation=[0.01, 0, 20], rotations=(0.2, 0.3, 0), normal=normal)
+    cont, v, lev, conv, obs = o.calculate_power_density_rectangle([6, 5], [0.08, 0.08], **kw)
+    xyz, rv, rconv = r.calculate_power_density_rectangle([6, 5], [0.08, 0.08], **kw)
+    assert same(cont, xyz) and same(v, rv)
+
+
+def test_grid_file(tmp_path):
+    path = str(tmp_path / "g.txt")
+    z = np.linspace(-1, 1, 801)
+    by = np.where(np.abs(z) < 0.8, np.sin(2 * np.pi * z / 0.05), 0.0)
+    with open(path, "w") as f:
+        f.write("# test map\n0\n0\n1\n0\n0\n1\n-1 # z0\n0.0025\n801\n")
+        for b in by:
+            f.write("0 %.17g 0\n" % b)
+
+    def setup(s):
+        s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+        s.add_bfield_file(path)
+        s.set_ctstartstop(0, 2)
+        s.set_npoints_trajectory(3001)
+    o, r = pair(setup)
+    T = o.calculate_trajectory()
+    t, p, ypp = r.calculate_trajectory()
+    assert same(T.p, p)
+    e, v, _, _, _ = o.calculate_spectrum([0, 0, 30], npoints=9, energy_range_eV=[300, 900])
+    re, rv, _ = r.calculate_spectrum([0, 0, 30], npoints=9, energy_range_eV=[300, 900])
+    assert same(v, rv)
