@@ -93,8 +93,9 @@ struct LevelJob {
 extern "C" {
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s);
+hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
 hipError_t om_launch_spline(const om::TrajDesc* desc, int np, const double* t, const double* p, double* ypp,
-                            double* u, hipStream_t s);
+                            double* u, double* sig, double* sm1, long total, hipStream_t s);
 hipError_t om_launch_level(const om::LevelJob* J, hipStream_t s);
 hipError_t om_launch_beta_diff(const double* rows, long npts, double* out, hipStream_t s);
 hipError_t om_launch_lw_point(const om::LwArgs* A, int mixed, int split, hipStream_t s);
@@ -149,7 +150,8 @@ struct DevBuf {
 };
 
 struct LevelCache {
-  double* rows[om::kMaxLevel + 1] = {};
+  double* rows[om::kMaxLevel + 1] = {};  // device buffers, kept across trajectory sets
+  bool built[om::kMaxLevel + 1] = {};    // rows hold samples of the current trajectory
   int extended[om::kMaxLevel + 1] = {};
   double beta_diff[om::kMaxLevel + 1] = {};
   bool have_bd[om::kMaxLevel + 1] = {};
@@ -167,7 +169,7 @@ struct om_context {
   int np = 0;
   std::vector<om::TrajDesc> desc;
   DevBuf<om::TrajDesc> d_desc;
-  DevBuf<double> d_t, d_p, d_ypp, d_u;
+  DevBuf<double> d_t, d_p, d_ypp, d_u, d_sig, d_sm1;
   std::vector<double> tstart, tstop;
   std::vector<om_particle> parts;
   std::vector<LevelCache> levels;
@@ -193,6 +195,7 @@ struct om_context {
     free_levels();
     for (auto& g : grid_bufs) g.release();
     d_desc.release(); d_t.release(); d_p.release(); d_ypp.release(); d_u.release(); d_fields.release();
+    d_sig.release(); d_sm1.release();
     d_obs.release(); d_omega.release(); d_sum.release(); d_comp.release(); d_lastmag.release();
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
@@ -232,6 +235,7 @@ om::FieldDev make_field(om_context* ctx, const om_field& f) {
   d.taper = f.taper;
   double pm = std::sqrt(f.period[0] * f.period[0] + f.period[1] * f.period[1] + f.period[2] * f.period[2]);
   d.period_length = pm;
+  d.inv_period_length = 1.0 / pm;
   for (int k = 0; k < 3; ++k) d.unit[k] = f.period[k] / pm;
   d.und_length = pm * (double) (f.nperiods + 2);
   if (f.type == OM_FIELD_UNDULATOR) {
@@ -252,6 +256,7 @@ om::FieldDev make_field(om_context* ctx, const om_field& f) {
       d.n[a] = nn[a];
       d.start[a] = start;
       d.step[a] = f.grid_step[a];
+      d.inv_step[a] = 1.0 / f.grid_step[a];
       d.stop[a] = start + (double) (size_t) (nn[a] - 1) * f.grid_step[a];
       if (nn[a] > 1) d.axis = a;
     }
@@ -307,9 +312,10 @@ void ensure_levels(om_context* ctx, int ip, int Lb, int stop_mem) {
   double ts = ctx->tstart[ip], te = ctx->tstop[ip];
   for (int L = 0; L <= Lb; ++L) {
     int ext = L > stop_mem ? 1 : 0;
-    if (lc.rows[L] && lc.extended[L] == ext) continue;
+    if (lc.rows[L] && lc.built[L] && lc.extended[L] == ext) continue;
     long np = 1L << L;
     if (!lc.rows[L]) check(hipMalloc((void**) &lc.rows[L], (size_t) np * 10 * sizeof(double)), "hipMalloc level");
+    lc.built[L] = true;
     lc.extended[L] = ext;
     lc.have_bd[L] = false;
     om::LevelJob J;
@@ -449,7 +455,7 @@ void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_valu
     int mx = -1;
     long nu = 0;
     for (long i = 0; i < npts; ++i) {
-      if (lev[i] >= 0) ev += std::pow(2., lev[i] + 1) - 1;
+      if (lev[i] >= 0) ev += std::ldexp(1.0, lev[i] + 1) - 1;
       mx = lev[i] > mx ? lev[i] : mx;
       nu += cv[i] ? 0 : 1;
     }
@@ -588,7 +594,12 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
   return guarded(ctx, [&] {
     if (nfields < 0 || nfields > om::kMaxFields) throw std::out_of_range("too many fields");
     if (np < 1) throw std::length_error("no particles");
-    ctx->free_levels();
+    if ((int) ctx->levels.size() != np) ctx->free_levels();
+    for (auto& lc : ctx->levels)
+      for (int L = 0; L <= om::kMaxLevel; ++L) {
+        lc.built[L] = false;
+        lc.have_bd[L] = false;
+      }
     for (auto& g : ctx->grid_bufs) g.release();
     ctx->grid_bufs.clear();
     ctx->grid_bufs.reserve(nfields);
@@ -634,17 +645,20 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->d_p.ensure(9 * total);
     ctx->d_ypp.ensure(9 * total);
     ctx->d_u.ensure(9 * total);
+    ctx->d_sig.ensure(total);
+    ctx->d_sm1.ensure(total);
     upload(ctx, ctx->d_desc, ctx->desc.data(), np);
     om::FieldDev none;
     std::memset(&none, 0, sizeof(none));
     check(om_launch_rk4(ctx->d_fields.p, nfields > 0 ? &F[0] : &none, nfields, ctx->d_desc.p, np, ctx->d_t.p,
                         ctx->d_p.p, ctx->stream), "rk4");
-    check(om_launch_spline(ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->d_ypp.p, ctx->d_u.p, ctx->stream),
-          "spline");
+    check(om_launch_knot_finish(ctx->d_p.p, total, ctx->stream), "knot finish");
+    check(om_launch_spline(ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->d_ypp.p, ctx->d_u.p, ctx->d_sig.p,
+                           ctx->d_sm1.p, total, ctx->stream), "spline");
     check(hipMemcpyAsync(ctx->desc.data(), ctx->d_desc.p, np * sizeof(om::TrajDesc), hipMemcpyDeviceToHost,
                          ctx->stream), "desc copy");
     check(hipStreamSynchronize(ctx->stream), "sync");
-    ctx->levels.assign(np, LevelCache());
+    if ((int) ctx->levels.size() != np) ctx->levels.assign(np, LevelCache());
     ctx->np = np;
   });
 }

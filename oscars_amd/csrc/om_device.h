@@ -33,6 +33,17 @@ __device__ __forceinline__ V3 cross(V3 a, V3 v) {
   return V3{a.y * v.z - v.y * a.z, a.z * v.x - v.z * a.x, a.x * v.y - v.x * a.y};
 }
 
+// ---------------------------------------------------------------- exact quotient
+// q / b for a divisor whose correctly rounded reciprocal inv = RN(1/b) is
+// known: y = RN(q*inv) is within one ulp of q/b, the remainder q - y*b is exact
+// in one FMA, and one correction gives RN(q/b) (Markstein's theorem), i.e. the
+// same double as the IEEE division, in three instructions.
+__device__ __forceinline__ double div_by(double q, double b, double inv) {
+  double y = q * inv;
+  double e = __builtin_fma(-y, b, q);
+  return __builtin_fma(e, inv, y);
+}
+
 // ---------------------------------------------------------------- sin/cos of huge phases
 // The phase phi = -omega (t + D/c) reaches ~5e11 rad.  Reduction by pi/2 with
 // a two-part constant: k = nearest integer to phi*2/pi (obtained with the
@@ -83,6 +94,14 @@ __device__ __forceinline__ void sincos_reduced64(Reduced a, double* s, double* c
 
 __device__ __forceinline__ void sincos_huge64(double phi, double* s, double* c) {
   sincos_reduced64(reduce_pio2(phi), s, c);
+}
+
+// sin(x) for the field models (|x| up to ~1e5 rad): the same exact reduction
+// and polynomials, without a large-argument path; within about one ulp.
+__device__ __forceinline__ double sin64(double x) {
+  double s, c;
+  sincos_reduced64(reduce_pio2(x), &s, &c);
+  return s;
 }
 
 // FP32 polynomial sin/cos on the FP64-reduced argument (mixed-precision path).

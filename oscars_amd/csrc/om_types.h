@@ -19,10 +19,11 @@ struct FieldDev {
   double field[3], fieldB[3], period[3], center[3];
   double phase, taper;
   double period_length, unit[3], und_length;
+  double inv_period_length;  // RN(1/period_length), for the exact Markstein quotient
   double phase_shift_a, phase_shift_b;
   // 1-D grid (src/TField3D_Grid.cc:153-278, open interval, linear interpolation)
   long n[3];
-  double start[3], step[3], stop[3];
+  double start[3], step[3], stop[3], inv_step[3];
   double rot[3], trans[3];
   double rot_sin[3], rot_cos[3];
   const double* data;  // device pointer, 3 per sample

@@ -35,7 +35,7 @@ __device__ __forceinline__ void undulator_part(const FieldDev& f, double D, doub
     term = true;
     scale = (D < -L / 2. + PhaseShift + PL / 2. || D > L / 2. + PhaseShift - PL / 2.) ? 0.25 : 0.75;
   }
-  double s = sin(2.0 * kPi * (D - PhaseShift) / PL);
+  double s = sin64(div_by(2.0 * kPi * (D - PhaseShift), PL, f.inv_period_length));
   double b0 = term ? a0 * scale : a0;
   double b1 = term ? a1 * scale : a1;
   double b2 = term ? a2 * scale : a2;
@@ -79,15 +79,16 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
   double coord = a == 0 ? X[0] : (a == 1 ? X[1] : X[2]);
   double start = a == 0 ? f.start[0] : (a == 1 ? f.start[1] : f.start[2]);
   double step = a == 0 ? f.step[0] : (a == 1 ? f.step[1] : f.step[2]);
+  double istep = a == 0 ? f.inv_step[0] : (a == 1 ? f.inv_step[1] : f.inv_step[2]);
   long na = a == 0 ? f.n[0] : (a == 1 ? f.n[1] : f.n[2]);
-  long n0 = (long) (size_t) ((coord - start) / step);
+  long n0 = (long) (size_t) div_by(coord - start, step, istep);
   if (n0 > na - 2) n0 = na - 2;  // keeps the n0+1 read inside the map
   double dd = (coord - start) - (double) n0 * step;
   const double* lo = f.data + 3 * n0;
   double l0 = lo[0], l1 = lo[1], l2 = lo[2], h0 = lo[3], h1 = lo[4], h2 = lo[5];
-  out[0] = (l0 + (h0 - l0) * dd / step) * 1.0;
-  out[1] = (l1 + (h1 - l1) * dd / step) * 1.0;
-  out[2] = (l2 + (h2 - l2) * dd / step) * 1.0;
+  out[0] = (l0 + div_by((h0 - l0) * dd, step, istep)) * 1.0;
+  out[1] = (l1 + div_by((h1 - l1) * dd, step, istep)) * 1.0;
+  out[2] = (l2 + div_by((h2 - l2) * dd, step, istep)) * 1.0;
 }
 
 __device__ __forceinline__ void get_b(const FieldDev* __restrict__ F, int nf, double x, double y, double z,
@@ -163,13 +164,24 @@ __device__ __forceinline__ bool rk4(RkCtx& c, const FieldDev& f0, double* y, con
   return true;
 }
 
+// The knot is stored with velocity and acceleration in SI units; knot_finish_kernel
+// divides them by c afterwards (in parallel), which keeps six divisions off the
+// sequential RK4 chain and gives the same doubles.
 __device__ __forceinline__ void store_knot(double* __restrict__ t, double* __restrict__ p, long k, double tt,
                                            const double* x, const double* d) {
   t[k] = tt;
   double* o = p + 9 * k;
   o[0] = x[0]; o[1] = x[2]; o[2] = x[4];
-  o[3] = x[1] / kC; o[4] = x[3] / kC; o[5] = x[5] / kC;
-  o[6] = d[1] / kC; o[7] = d[3] / kC; o[8] = d[5] / kC;
+  o[3] = x[1]; o[4] = x[3]; o[5] = x[5];
+  o[6] = d[1]; o[7] = d[3]; o[8] = d[5];
+}
+
+__global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p, long total) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total * 6) return;
+  long k = i / 6;
+  int c = 3 + (int) (i - k * 6);
+  p[9 * k + c] = p[9 * k + c] / kC;
 }
 
 // One thread per particle.  Forward points go to knot nb+i, backward points to
@@ -215,47 +227,73 @@ __global__ void __launch_bounds__(64) rk4_kernel(const FieldDev f0, const FieldD
 }
 
 // ---------------------------------------------------------------- spline
-// include/TOMATH.h:141-203.  One lane per (particle, channel); the 9 lanes of
-// a particle write 9 consecutive doubles per knot.  u is scratch of the same
-// shape as ypp.
+// include/TOMATH.h:141-203.  spline_pre_kernel computes, in parallel, the
+// parts of the tridiagonal sweep that do not depend on the previous knot:
+// sig_i, sig_i - 1 and w_i = (secant difference) * 6 / (t[i+1] - t[i-1]) per
+// channel.  spline_kernel then runs the sequential recurrences
+//   p = sig*ypp[i-1] + 2,  ypp[i] = (sig-1)/p,  u[i] = (w_i - sig*u[i-1])/p
+// and the back substitution, one lane per (particle, channel).
+__global__ void __launch_bounds__(256) spline_pre_kernel(const TrajDesc* __restrict__ desc, int np,
+                                                         const double* __restrict__ t, const double* __restrict__ p,
+                                                         double* __restrict__ sig, double* __restrict__ sm1,
+                                                         double* __restrict__ u, long total) {
+  long g = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  // locate the particle of knot g (few particles: linear scan over descriptors)
+  int ip = 0;
+  while (ip + 1 < np && desc[ip + 1].knot_offset <= g) ++ip;
+  long o = desc[ip].knot_offset;
+  long N = desc[ip].n;
+  long i = g - o;
+  if (i < 1 || i >= N - 1) return;
+  const double* X = t + o;
+  double s = (X[i] - X[i - 1]) / (X[i + 1] - X[i - 1]);
+  sig[g] = s;
+  sm1[g] = s - 1.;
+  const double* Y = p + 9 * o;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    double ui = (Y[9 * (i + 1) + c] - Y[9 * i + c]) / (X[i + 1] - X[i]) -
+                (Y[9 * i + c] - Y[9 * (i - 1) + c]) / (X[i] - X[i - 1]);
+    u[9 * g + c] = ui * 6. / (X[i + 1] - X[i - 1]);
+  }
+}
+
 __global__ void __launch_bounds__(64) spline_kernel(const TrajDesc* __restrict__ desc, int np,
                                                     const double* __restrict__ t, const double* __restrict__ p,
+                                                    const double* __restrict__ sig, const double* __restrict__ sm1,
                                                     double* __restrict__ ypp, double* __restrict__ u) {
   int lane = threadIdx.x;
   int ip = blockIdx.x;
   if (ip >= np || lane >= 9) return;
   long N = desc[ip].n;
   long o = desc[ip].knot_offset;
+  if (N < 3) return;
   const double* X = t + o;
   const double* Y = p + 9 * o + lane;
+  const double* SG = sig + o;
+  const double* SM = sm1 + o;
   double* P = ypp + 9 * o + lane;
   double* U = u + 9 * o + lane;
-  if (N < 3) return;
   P[0] = -0.5;
-  double x0 = X[0], x1 = X[1], y0 = Y[0], y1 = Y[9];
-  double dydx0 = (y1 - y0) / (x1 - x0);
-  U[0] = ((y1 - y0) / (x1 - x0) - dydx0) * (3. / (x1 - x0));
+  double dydx0 = (Y[9] - Y[0]) / (X[1] - X[0]);
+  U[0] = ((Y[9] - Y[0]) / (X[1] - X[0]) - dydx0) * (3. / (X[1] - X[0]));
   double pprev = -0.5, uprev = U[0];
-  // forward sweep; the next knot is loaded one iteration ahead so the loads
-  // overlap the division chain
-  double x2 = X[2], y2 = Y[18];
+  double s_n = SG[1], m_n = SM[1], w_n = U[9];
   for (long i = 1; i < N - 1; ++i) {
-    double x3 = 0, y3 = 0;
-    if (i + 2 < N) {
-      x3 = X[i + 2];
-      y3 = Y[9 * (i + 2)];
+    double s = s_n, m = m_n, w = w_n;
+    if (i + 1 < N - 1) {  // next knot's inputs, loaded ahead of the dependent chain
+      s_n = SG[i + 1];
+      m_n = SM[i + 1];
+      w_n = U[9 * (i + 1)];
     }
-    double sig = (x1 - x0) / (x2 - x0);
-    double pp = sig * pprev + 2.0;
-    double yi = (sig - 1.) / pp;
-    double ui = (y2 - y1) / (x2 - x1) - (y1 - y0) / (x1 - x0);
-    ui = (ui * 6. / (x2 - x0) - sig * uprev) / pp;
+    double pp = s * pprev + 2.0;
+    double yi = m / pp;
+    double ui = (w - s * uprev) / pp;
     P[9 * i] = yi;
     U[9 * i] = ui;
     pprev = yi;
     uprev = ui;
-    x0 = x1; x1 = x2; x2 = x3;
-    y0 = y1; y1 = y2; y2 = y3;
   }
   double qn = 0.5;
   double xa = X[N - 2], xb = X[N - 1], ya = Y[9 * (N - 2)], yb = Y[9 * (N - 1)];
@@ -376,9 +414,20 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
   return hipGetLastError();
 }
 
+extern "C" hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s) {
+  long blocks = (total * 6 + 255) / 256;
+  hipLaunchKernelGGL(om::knot_finish_kernel, dim3((unsigned) blocks), dim3(256), 0, s, p, total);
+  return hipGetLastError();
+}
+
+// sig and sm1 are scratch arrays of one double per knot
 extern "C" hipError_t om_launch_spline(const om::TrajDesc* desc, int np, const double* t, const double* p,
-                                       double* ypp, double* u, hipStream_t s) {
-  hipLaunchKernelGGL(om::spline_kernel, dim3(np), dim3(64), 0, s, desc, np, t, p, ypp, u);
+                                       double* ypp, double* u, double* sig, double* sm1, long total,
+                                       hipStream_t s) {
+  long blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(om::spline_pre_kernel, dim3((unsigned) blocks), dim3(256), 0, s, desc, np, t, p, sig, sm1, u,
+                     total);
+  hipLaunchKernelGGL(om::spline_kernel, dim3(np), dim3(64), 0, s, desc, np, t, p, sig, sm1, ypp, u);
   return hipGetLastError();
 }
 
