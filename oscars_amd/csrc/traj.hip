@@ -102,6 +102,47 @@ __device__ __forceinline__ void get_b(const FieldDev* __restrict__ F, int nf, do
   B[0] = s0; B[1] = s1; B[2] = s2;
 }
 
+// ---------------------------------------------------------------- compact undulator / EPU
+// The few numbers an ideal undulator or EPU needs, small enough to stay in
+// registers for the whole integration.  The region tests become selects:
+// outside the magnet the amplitude is multiplied by zero, which gives the same
+// sum as the reference's early return.
+struct UndParams {
+  double c0, c1, c2, u0, u1, u2;  // centre, unit vector along the period
+  double taper, half_len, PL, inv_PL, half_PL;
+  double psA, psB;
+  double a0, a1, a2, b0, b1, b2;  // field A, field B (zero for a planar undulator)
+  int epu;
+};
+
+__device__ __forceinline__ double und_scale(const UndParams& f, double D, double PS) {
+  double L2 = f.half_len;
+  bool out = D > L2 + PS || D < -L2 + PS;
+  bool term = D < -L2 + PS + f.PL || D > L2 + PS - f.PL;
+  bool quarter = D < -L2 + PS + f.half_PL || D > L2 + PS - f.half_PL;
+  double sc = term ? (quarter ? 0.25 : 0.75) : 1.0;
+  return out ? 0.0 : sc;
+}
+
+__device__ __forceinline__ void und_eval(const UndParams& f, double x, double y, double z, double B[3]) {
+  double D = (x - f.c0) * f.u0 + (y - f.c1) * f.u1 + (z - f.c2) * f.u2;
+  double tc = 1 + D * f.taper;
+  double scA = und_scale(f, D, f.psA);
+  double sA = sin64(div_by(2.0 * kPi * (D - f.psA), f.PL, f.inv_PL));
+  // terminated poles: (field*scale)*sin*taper; full poles: field*sin*taper (scale 1 is exact)
+  double v0 = f.a0 * scA * sA * tc, v1 = f.a1 * scA * sA * tc, v2 = f.a2 * scA * sA * tc;
+  if (f.epu) {
+    double scB = und_scale(f, D, f.psB);
+    double sB = sin64(div_by(2.0 * kPi * (D - f.psB), f.PL, f.inv_PL));
+    v0 = v0 + f.b0 * scB * sB * tc;
+    v1 = v1 + f.b1 * scB * sB * tc;
+    v2 = v2 + f.b2 * scB * sB * tc;
+  }
+  B[0] = 0.0 + v0;
+  B[1] = 0.0 + v1;
+  B[2] = 0.0 + v2;
+}
+
 // ---------------------------------------------------------------- RK4
 struct RkCtx {
   const FieldDev* F;
@@ -110,9 +151,9 @@ struct RkCtx {
   int err;
 };
 
-// ONE: exactly one field, passed by value in f0 so it lives in registers.
+// ONE: exactly one undulator/EPU field, passed by value in f0 so it lives in registers.
 template <bool ONE>
-__device__ __forceinline__ void derivs(RkCtx& c, const FieldDev& f0, const double* x, double* d) {
+__device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const double* x, double* d) {
   // |beta| >= 1 test of src/OSCARSSR.cc:1260-1264; only its sign is used, so
   // the product with 1/c^2 stands in for the division without changing it
   // except within one rounding of |beta| = 1.
@@ -123,11 +164,7 @@ __device__ __forceinline__ void derivs(RkCtx& c, const FieldDev& f0, const doubl
   }
   double B[3];
   if (ONE) {
-    double v[3];
-    field_eval(f0, x[0], x[2], x[4], v);
-    B[0] = 0.0 + v[0];
-    B[1] = 0.0 + v[1];
-    B[2] = 0.0 + v[2];
+    und_eval(f0, x[0], x[2], x[4], B);
   } else {
     get_b(c.F, c.nf, x[0], x[2], x[4], B);
   }
@@ -143,7 +180,7 @@ __device__ __forceinline__ void derivs(RkCtx& c, const FieldDev& f0, const doubl
 // halves the step recursively; that branch never triggers for physical beams,
 // so the kernel reports it as an error instead of recursing).
 template <bool ONE>
-__device__ __forceinline__ bool rk4(RkCtx& c, const FieldDev& f0, double* y, const double* dydx, double h) {
+__device__ __forceinline__ bool rk4(RkCtx& c, const UndParams& f0, double* y, const double* dydx, double h) {
   double dym[6], dyt[6], yt[6];
   double hh = h * 0.5;
   double h6 = h / 6.0;
@@ -187,10 +224,10 @@ __global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p
 // One thread per particle.  Forward points go to knot nb+i, backward points to
 // knot nb-1-i, which is the order the reference reaches after its two array
 // reversals.
-// ONE: the single field arrives by value in the kernel arguments (read
-// through the scalar cache into SGPRs); otherwise fields are read from F.
+// ONE: a single undulator/EPU arrives by value in the kernel arguments (read
+// through the scalar cache into registers); otherwise fields are read from F.
 template <bool ONE>
-__global__ void __launch_bounds__(64) rk4_kernel(const FieldDev f0, const FieldDev* __restrict__ F, int nfields,
+__global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const FieldDev* __restrict__ F, int nfields,
                                                  TrajDesc* __restrict__ desc, int np, double* __restrict__ t,
                                                  double* __restrict__ p) {
   int ip = blockIdx.x * blockDim.x + threadIdx.x;
@@ -403,13 +440,25 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
                                     om::TrajDesc* desc, int np, double* t, double* p, hipStream_t s) {
   int threads = 64;
   int blocks = (np + threads - 1) / threads;
-  if (nfields == 1) {
-    hipLaunchKernelGGL(om::rk4_kernel<true>, dim3(blocks), dim3(threads), 0, s, *F0_host, F, nfields, desc, np, t,
-                       p);
+  om::UndParams u;
+  memset(&u, 0, sizeof(u));
+  const om::FieldDev& f = *F0_host;
+  if (nfields == 1 && (f.type == om::kUndulator || f.type == om::kEPU)) {
+    u.c0 = f.center[0]; u.c1 = f.center[1]; u.c2 = f.center[2];
+    u.u0 = f.unit[0]; u.u1 = f.unit[1]; u.u2 = f.unit[2];
+    u.taper = f.taper;
+    u.half_len = f.und_length / 2.;
+    u.PL = f.period_length;
+    u.inv_PL = f.inv_period_length;
+    u.half_PL = f.period_length / 2.;
+    u.psA = f.phase_shift_a;
+    u.psB = f.phase_shift_b;
+    u.a0 = f.field[0]; u.a1 = f.field[1]; u.a2 = f.field[2];
+    u.b0 = f.fieldB[0]; u.b1 = f.fieldB[1]; u.b2 = f.fieldB[2];
+    u.epu = f.type == om::kEPU;
+    hipLaunchKernelGGL(om::rk4_kernel<true>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc, np, t, p);
   } else {
-    om::FieldDev none;
-    memset(&none, 0, sizeof(none));
-    hipLaunchKernelGGL(om::rk4_kernel<false>, dim3(blocks), dim3(threads), 0, s, none, F, nfields, desc, np, t, p);
+    hipLaunchKernelGGL(om::rk4_kernel<false>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc, np, t, p);
   }
   return hipGetLastError();
 }

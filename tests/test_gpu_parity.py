@@ -36,11 +36,20 @@ def test_trajectory_knots(oracle_planar, gpu_planar):
     T = oracle_planar.traj
     assert len(t) == len(T.t) == 20001
     assert np.array_equal(t, T.t)
-    # RK4 in the device uses the device sin(); report how many knots are bit-identical
+    # The device RK4 evaluates the field's sine with its own polynomial, which
+    # can differ from the C library's in the last bit; differences are measured
+    # per channel against that channel's largest magnitude.
     same = np.mean(np.all(p == T.p, axis=1))
     print("bit-identical knots: %.4f" % same)
-    assert rel_err(p, T.p, 1e-12) < 1e-12
-    assert rel_err(ypp, T.ypp, 1e-9) < 1e-9
+    scale = np.max(np.abs(T.p), axis=0)
+    scale[scale == 0] = 1
+    err = np.max(np.abs(p - T.p) / scale)
+    yscale = np.max(np.abs(T.ypp), axis=0)
+    yscale[yscale == 0] = 1
+    yerr = np.max(np.abs(ypp - T.ypp) / yscale)
+    print("knots: max channel-relative difference %.3e, spline %.3e" % (err, yerr))
+    assert err < 1e-12
+    assert yerr < 1e-10
 
 
 @gpu
@@ -50,7 +59,9 @@ def test_level_samples(oracle_planar, gpu_planar):
         a = dev.level(0, L)
         b = oracle_planar.traj.level(L)
         assert np.array_equal(a[:, 0], b[:, 0]), L
-        assert rel_err(a[:, 1:], b[:, 1:], 1e-12) < 1e-12, L
+        scale = np.max(np.abs(b[:, 1:]), axis=0)
+        scale[scale == 0] = 1
+        assert np.max(np.abs(a[:, 1:] - b[:, 1:]) / scale) < 1e-12, L
 
 
 @gpu
