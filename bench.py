@@ -57,6 +57,7 @@ class Workload:
         self.obs_sweep = rect(n_sweep)
         self.e_sweep = np.array([100 + (3000 - 100) / (n_e - 1) * i for i in range(n_e)])
         self.map_energies = [152.11, 260.0]
+        self.only_sweep = False
 
     def step(self, mode):
         """One pass; returns dict of per-phase evals / kernel ms / outputs."""
@@ -67,7 +68,7 @@ class Workload:
         pol = [0.0] * 6
         out = {"evals": 0.0, "kernel_ms": {}, "values": {}}
         map_mode = "plain" if mode == "plain" else "point_mixed"
-        for e in self.map_energies:
+        for e in ([] if self.only_sweep else self.map_energies):
             v, lev, cv, _, st = dev.flux(0, self.obs_map, [e], pol, dev.conv(mode=map_mode))
             out["evals"] += st["evals"]
             out["kernel_ms"]["map_%g" % e] = st["kernel_ms"]
@@ -148,7 +149,21 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--only-sweep", action="store_true",
+                    help="time only the 51x51 x 512-energy sweep (for counter runs of the dominant kernel)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device-to-device copy and exit (reference for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
                           "n_gpus": args.gpus}))
@@ -159,6 +174,7 @@ def main():
     s = sr()
     scenario(s)
     work = Workload(s)
+    work.only_sweep = args.only_sweep
     print("# device: %s" % s._device().info(), file=sys.stderr, flush=True)
 
     dt, evals, kms, res = timed(work, args.mode, args.steps, args.warmup)
