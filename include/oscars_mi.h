@@ -131,6 +131,37 @@ int om_power_density(om_context* ctx, int particle, const double* obs_xyz, long 
                      int directional, const om_conv* conv, double* out_value, int* out_level,
                      unsigned char* out_conv, om_stats* stats);
 
+/* ---- batched multi-particle sums (one persistent launch) ----
+ * Replaces the reference's multi-particle loop (src/OSCARSSR.cc:4016-4063,
+ * 1778-1818 and the CUDA host pipeline src/OSCARSSR_Cuda.cu:974-1147), which
+ * recomputes each particle's trajectory and sums the particles one after the
+ * other.  Particle i of the resident trajectory set (om_trajectories) adds
+ * weight[i] * flux to the output grid of request req_of[i]; requests may have
+ * different observation grids and energy lists, particles different
+ * trajectory lengths.  Levels 0..max_level of every particle are materialised;
+ * points still unconverged there are counted in stats->n_unconverged. */
+typedef struct {
+  long n_obs;
+  const double* obs_xyz;     /* 3 per point */
+  long n_e;
+  const double* energies_eV;
+} om_request;
+
+typedef struct {
+  int waves_per_cu; /* persistent grid = CUs x this; 0: the tuned value (8 before any tuning) */
+  int max_level;    /* highest level materialised and summed; 0: 14 */
+} om_batch_opts;
+
+int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int* req_of, const double* weight,
+                   const double* pol6, const om_conv* conv, const om_batch_opts* opts);
+/* One step: [RK4 + spline when recompute_trajectories] + level samples + all
+ * sums, replayed from a captured HIP graph; out[r] receives request r's grid
+ * (n_obs x n_e, energy fastest). */
+int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out, om_stats* stats);
+/* Times the persistent kernel on the first sample_items work units for each
+ * candidate waves-per-CU value and keeps the fastest (returned in *best). */
+int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_items, int* best, double* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,14 @@ class om_conv(C.Structure):
                 ("quantity", C.c_int), ("mode", C.c_int), ("weight", C.c_double), ("accumulate", C.c_int)]
 
 
+class om_request(C.Structure):
+    _fields_ = [("n_obs", C.c_long), ("obs_xyz", _dp), ("n_e", C.c_long), ("energies_eV", _dp)]
+
+
+class om_batch_opts(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_int), ("max_level", C.c_int)]
+
+
 class om_stats(C.Structure):
     _fields_ = [("evals", C.c_double), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
                 ("max_level", C.c_int), ("n_unconverged", C.c_long)]
@@ -51,7 +59,7 @@ class om_stats(C.Structure):
 
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
            "om_trajectories", "om_trajectory_get", "om_level_get", "om_trajectory_status", "om_flux",
-           "om_spectrum", "om_power_density"]
+           "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune"]
 
 _lib = None
 
@@ -81,6 +89,10 @@ def lib():
                               _dp, C.POINTER(om_stats)]
     L.om_power_density.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, _dp, C.c_int, C.POINTER(om_conv), _dp,
                                    _ip, _up, C.POINTER(om_stats)]
+    L.om_batch_setup.argtypes = [C.c_void_p, C.POINTER(om_request), C.c_int, _ip, _dp, _dp, C.POINTER(om_conv),
+                                 C.POINTER(om_batch_opts)]
+    L.om_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(om_stats)]
+    L.om_batch_tune.argtypes = [C.c_void_p, _ip, C.c_int, C.c_long, _ip, _dp]
     _lib = L
     return L
 
@@ -199,3 +211,40 @@ class Device:
         self._chk(self.L.om_power_density(self.h, int(particle), _ptr(obs), n, _ptr(nv), int(directional),
                                           C.byref(conv), _ptr(out), _ptr(lev, _ip), _ptr(cv, _up), C.byref(st)))
         return out, lev, cv, st.as_dict()
+
+    # ------------------------------------------------------------ batched multi-particle sums
+    def batch_setup(self, requests, req_of, weight, pol6, conv, waves_per_cu=0, max_level=0):
+        """requests: list of (obs_xyz[n,3], energies[m]); particle i adds weight[i]*flux to requests[req_of[i]]."""
+        self._batch_keep = []
+        ra = (om_request * len(requests))()
+        shapes = []
+        for r, (obs, e) in enumerate(requests):
+            o = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+            en = np.ascontiguousarray(e, dtype=np.float64)
+            self._batch_keep += [o, en]
+            ra[r].n_obs = len(o)
+            ra[r].obs_xyz = _ptr(o)
+            ra[r].n_e = len(en)
+            ra[r].energies_eV = _ptr(en)
+            shapes.append((len(o), len(en)))
+        ro = np.ascontiguousarray(req_of, dtype=np.int32)
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        pol = np.ascontiguousarray(pol6, dtype=np.float64)
+        opts = om_batch_opts(int(waves_per_cu), int(max_level))
+        self._chk(self.L.om_batch_setup(self.h, ra, len(requests), _ptr(ro, _ip), _ptr(w), _ptr(pol),
+                                        C.byref(conv), C.byref(opts)))
+        self._batch_shapes = shapes
+
+    def batch_run(self, recompute_trajectories=False):
+        outs = [np.zeros(n * m) for n, m in self._batch_shapes]
+        arr = (_dp * len(outs))(*[_ptr(o) for o in outs])
+        st = om_stats()
+        self._chk(self.L.om_batch_run(self.h, int(bool(recompute_trajectories)), arr, C.byref(st)))
+        return [o.reshape(n, m) for o, (n, m) in zip(outs, self._batch_shapes)], st.as_dict()
+
+    def batch_tune(self, candidates=(2, 4, 8, 16, 32), sample_items=20000):
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(c))
+        best = C.c_int(0)
+        self._chk(self.L.om_batch_tune(self.h, _ptr(c, _ip), len(c), int(sample_items), C.byref(best), _ptr(ms)))
+        return best.value, dict(zip([int(x) for x in c], ms.tolist()))

@@ -88,9 +88,49 @@ struct LevelJob {
   long npts;
   double t_first, spacing;
 };
+struct LevelJobBatch {
+  const double* t;
+  const double* p;
+  const double* ypp;
+  const TrajDesc* desc;
+  const double* tspan;
+  double* rows;
+  long stride;
+  int np;
+  int Lcap;
+  int stop_mem;
+};
+struct ReqDev {
+  const double* obs;
+  const double* omega;
+  double* out;
+  long n_obs, n_e, n_tiles_e;
+};
+struct BatchArgs {
+  const double* rows;
+  long stride;
+  int Lb;
+  const double* tspan;
+  const int* req_of;
+  const double* weight;
+  const long* item_prefix;
+  int np;
+  const ReqDev* req;
+  double C0, C2;
+  double pol[6];
+  int use_pol;
+  double precision;
+  unsigned long long* counter;
+  long total_items;
+  unsigned long long* n_unconverged;
+  unsigned long long* evals;
+};
 }  // namespace om
 
 extern "C" {
+hipError_t om_launch_level_batch(const om::LevelJobBatch* J, hipStream_t s);
+hipError_t om_launch_batch_clear(double* out, long n, unsigned long long* counters, int nc, hipStream_t s);
+hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s);
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
@@ -162,6 +202,7 @@ struct LevelCache {
 struct om_context {
   int device = 0;
   int split = 0;  // waves per workgroup for the LW kernels (0: chosen from the grid size)
+  int batch_wpc = 0;  // persistent batch kernel: waves per CU chosen by om_batch_tune (0: default)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
@@ -180,6 +221,36 @@ struct om_context {
   DevBuf<int> d_level, d_conv, d_count, d_outlevel;
   DevBuf<unsigned char> d_outconv;
   DevBuf<double> d_bd;
+  std::vector<om::FieldDev> fields_host;
+  int nfields = 0;
+  long total_knots = 0;
+  // batched multi-particle state
+  struct Batch {
+    bool ready = false;
+    int np = 0, n_req = 0, Lb = 0, stop_mem = 0, mixed = 0, groups = 0;
+    long stride = 0, total_items = 0, total_out = 0;
+    std::vector<long> out_off, out_n;
+    DevBuf<double> rows, tspan, obs, omega, out, weight;
+    DevBuf<int> req_of;
+    DevBuf<long> prefix;
+    DevBuf<om::ReqDev> req;
+    DevBuf<unsigned long long> counters;
+    om::BatchArgs args;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // [without, with] trajectory recomputation
+    void drop_graphs() {
+      for (auto& e : exec)
+        if (e) {
+          (void) hipGraphExecDestroy(e);
+          e = nullptr;
+        }
+    }
+    void release() {
+      drop_graphs();
+      rows.release(); tspan.release(); obs.release(); omega.release(); out.release(); weight.release();
+      req_of.release(); prefix.release(); req.release(); counters.release();
+      ready = false;
+    }
+  } batch;
 
   void free_levels() {
     for (auto& lc : levels)
@@ -192,6 +263,7 @@ struct om_context {
   }
 
   ~om_context() {
+    batch.release();
     free_levels();
     for (auto& g : grid_bufs) g.release();
     d_desc.release(); d_t.release(); d_p.release(); d_ypp.release(); d_u.release(); d_fields.release();
@@ -465,6 +537,18 @@ void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_valu
   }
 }
 
+// RK4 + knot finish + spline for the resident descriptors (also used inside
+// graph capture by the batched path).
+void launch_trajectories(om_context* ctx, int np) {
+  om::FieldDev none;
+  std::memset(&none, 0, sizeof(none));
+  check(om_launch_rk4(ctx->d_fields.p, ctx->nfields > 0 ? &ctx->fields_host[0] : &none, ctx->nfields,
+                      ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->stream), "rk4");
+  check(om_launch_knot_finish(ctx->d_p.p, ctx->total_knots, ctx->stream), "knot finish");
+  check(om_launch_spline(ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->d_ypp.p, ctx->d_u.p, ctx->d_sig.p,
+                         ctx->d_sm1.p, ctx->total_knots, ctx->stream), "spline");
+}
+
 int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
             const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
             double* out_e, om_stats* stats, bool spectrum) {
@@ -605,6 +689,10 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->grid_bufs.reserve(nfields);
     std::vector<om::FieldDev> F;
     for (int i = 0; i < nfields; ++i) F.push_back(make_field(ctx, fields[i]));
+    ctx->batch.drop_graphs();
+    ctx->batch.ready = false;
+    ctx->fields_host = F;
+    ctx->nfields = nfields;
     ctx->d_fields.ensure(nfields > 0 ? nfields : 1);
     if (nfields > 0) upload(ctx, ctx->d_fields, F.data(), F.size());
     ctx->np = 0;
@@ -641,6 +729,7 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
       ctx->tstart[i] = D.nb > 0 ? D.t0_over_c + (-DeltaT) * (double) D.nb : D.t0_over_c;
       ctx->tstop[i] = D.t0_over_c + DeltaT * (double) (D.nf - 1);
     }
+    ctx->total_knots = total;
     ctx->d_t.ensure(total);
     ctx->d_p.ensure(9 * total);
     ctx->d_ypp.ensure(9 * total);
@@ -648,13 +737,7 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->d_sig.ensure(total);
     ctx->d_sm1.ensure(total);
     upload(ctx, ctx->d_desc, ctx->desc.data(), np);
-    om::FieldDev none;
-    std::memset(&none, 0, sizeof(none));
-    check(om_launch_rk4(ctx->d_fields.p, nfields > 0 ? &F[0] : &none, nfields, ctx->d_desc.p, np, ctx->d_t.p,
-                        ctx->d_p.p, ctx->stream), "rk4");
-    check(om_launch_knot_finish(ctx->d_p.p, total, ctx->stream), "knot finish");
-    check(om_launch_spline(ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->d_ypp.p, ctx->d_u.p, ctx->d_sig.p,
-                           ctx->d_sm1.p, total, ctx->stream), "spline");
+    launch_trajectories(ctx, np);
     check(hipMemcpyAsync(ctx->desc.data(), ctx->d_desc.p, np * sizeof(om::TrajDesc), hipMemcpyDeviceToHost,
                          ctx->stream), "desc copy");
     check(hipStreamSynchronize(ctx->stream), "sync");
@@ -769,6 +852,210 @@ int om_power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs,
       stats->kernel_ms = kms;
       stats->total_ms = wall.ms();
     }
+  });
+}
+
+int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int* req_of, const double* weight,
+                   const double* pol6, const om_conv* conv, const om_batch_opts* opts) {
+  return guarded(ctx, [&] {
+    auto& B = ctx->batch;
+    B.release();
+    int np = ctx->np;
+    if (np < 1) throw std::out_of_range("no trajectories: call om_trajectories first");
+    if (n_req < 1 || !req || !req_of || !weight || !conv) throw std::invalid_argument("missing batch inputs");
+    if (conv->mode != OM_MODE_TILED && conv->mode != OM_MODE_MIXED)
+      throw std::invalid_argument("batched sums use the tiled kernels: mode must be TILED or MIXED");
+    for (int i = 0; i < np; ++i) {
+      if (ctx->desc[i].status != 0) throw std::out_of_range("trajectory integration met |beta| >= 1");
+      if (req_of[i] < 0 || req_of[i] >= n_req) throw std::out_of_range("req_of out of range");
+      if (ctx->parts[i].q != ctx->parts[0].q || ctx->parts[i].current != ctx->parts[0].current)
+        throw std::invalid_argument("all particles of a batch must share charge and current");
+    }
+    int stop_mem, stop_ext;
+    level_stops(*conv, stop_mem, stop_ext);
+    int cap = (opts && opts->max_level > 0) ? opts->max_level : 14;
+    if (cap > 20) throw std::out_of_range("batch max_level above 20 is not supported");
+    B.Lb = stop_ext < cap ? stop_ext : cap;
+    if (B.Lb < 0) throw std::out_of_range("max_level leaves no level to sum");
+    B.stop_mem = stop_mem;
+    B.stride = (2L << B.Lb) - 1;
+    B.np = np;
+    B.n_req = n_req;
+    // requests: observation points, angular frequencies, output regions
+    std::vector<double> obs, om;
+    std::vector<om::ReqDev> rd(n_req);
+    std::vector<long> obs_off(n_req), om_off(n_req);
+    B.out_off.assign(n_req, 0);
+    B.out_n.assign(n_req, 0);
+    long total_out = 0;
+    for (int r = 0; r < n_req; ++r) {
+      if (req[r].n_obs < 1 || req[r].n_e < 1) throw std::length_error("no points specified");
+      obs_off[r] = (long) obs.size();
+      om_off[r] = (long) om.size();
+      for (long k = 0; k < 3 * req[r].n_obs; ++k) obs.push_back(req[r].obs_xyz[k]);
+      for (long k = 0; k < req[r].n_e; ++k) om.push_back(ev_to_omega(req[r].energies_eV[k]));
+      B.out_off[r] = total_out;
+      B.out_n[r] = req[r].n_obs * req[r].n_e;
+      total_out += B.out_n[r];
+    }
+    B.total_out = total_out;
+    upload(ctx, B.obs, obs.data(), obs.size());
+    upload(ctx, B.omega, om.data(), om.size());
+    B.out.ensure(total_out);
+    for (int r = 0; r < n_req; ++r) {
+      rd[r].obs = B.obs.p + obs_off[r];
+      rd[r].omega = B.omega.p + om_off[r];
+      rd[r].out = B.out.p + B.out_off[r];
+      rd[r].n_obs = req[r].n_obs;
+      rd[r].n_e = req[r].n_e;
+      rd[r].n_tiles_e = (req[r].n_e + 63) / 64;
+    }
+    upload(ctx, B.req, rd.data(), n_req);
+    std::vector<long> prefix(np + 1, 0);
+    for (int i = 0; i < np; ++i) {
+      const om::ReqDev& R = rd[req_of[i]];
+      prefix[i + 1] = prefix[i] + R.n_obs * R.n_tiles_e;
+    }
+    B.total_items = prefix[np];
+    upload(ctx, B.prefix, prefix.data(), prefix.size());
+    upload(ctx, B.req_of, req_of, np);
+    upload(ctx, B.weight, weight, np);
+    std::vector<double> ts(2 * np);
+    for (int i = 0; i < np; ++i) {
+      ts[2 * i] = ctx->tstart[i];
+      ts[2 * i + 1] = ctx->tstop[i];
+    }
+    upload(ctx, B.tspan, ts.data(), ts.size());
+    B.rows.ensure((size_t) np * B.stride * 10);
+    B.counters.ensure(3);
+    const om_particle& P = ctx->parts[0];
+    om::BatchArgs& A = B.args;
+    std::memset(&A, 0, sizeof(A));
+    A.rows = B.rows.p;
+    A.stride = B.stride;
+    A.Lb = B.Lb;
+    A.tspan = B.tspan.p;
+    A.req_of = B.req_of.p;
+    A.weight = B.weight.p;
+    A.item_prefix = B.prefix.p;
+    A.np = np;
+    A.req = B.req.p;
+    // src/OSCARSSR.cc:4123-4126
+    A.C0 = P.q / (four_pi() * kC * kEps0 * kSqrt2Pi);
+    A.C2 = four_pi() * P.current / (kH * std::fabs(P.q) * kMu0 * kC) * 1e-6 * 0.001;
+    double pm2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      A.pol[2 * k] = pol6 ? pol6[2 * k] : 0;
+      A.pol[2 * k + 1] = pol6 ? pol6[2 * k + 1] : 0;
+      double r = A.pol[2 * k] * A.pol[2 * k] - A.pol[2 * k + 1] * (-A.pol[2 * k + 1]);
+      pm2 = k == 0 ? r : pm2 + r;
+    }
+    A.use_pol = pm2 > 0.001;
+    A.precision = conv->precision;
+    A.counter = B.counters.p;
+    A.n_unconverged = B.counters.p + 1;
+    A.evals = B.counters.p + 2;
+    A.total_items = B.total_items;
+    B.mixed = conv->mode == OM_MODE_MIXED ? 1 : 0;
+    int cus = 0;
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "attribute");
+    int wpc = (opts && opts->waves_per_cu > 0) ? opts->waves_per_cu : (ctx->batch_wpc > 0 ? ctx->batch_wpc : 8);
+    B.groups = cus * wpc;
+    B.ready = true;
+  });
+}
+
+namespace {
+void batch_levels(om_context* ctx) {
+  auto& B = ctx->batch;
+  om::LevelJobBatch J;
+  J.t = ctx->d_t.p;
+  J.p = ctx->d_p.p;
+  J.ypp = ctx->d_ypp.p;
+  J.desc = ctx->d_desc.p;
+  J.tspan = B.tspan.p;
+  J.rows = B.rows.p;
+  J.stride = B.stride;
+  J.np = B.np;
+  J.Lcap = B.Lb;
+  J.stop_mem = B.stop_mem;
+  check(om_launch_level_batch(&J, ctx->stream), "level batch");
+}
+}  // namespace
+
+int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out, om_stats* stats) {
+  return guarded(ctx, [&] {
+    Timer wall;
+    auto& B = ctx->batch;
+    if (!B.ready) throw std::logic_error("om_batch_setup has not been called for the current trajectories");
+    int g = recompute_trajectories ? 1 : 0;
+    if (!B.exec[g]) {
+      hipGraph_t graph;
+      check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      if (g) launch_trajectories(ctx, B.np);
+      batch_levels(ctx);
+      check(om_launch_batch_clear(B.out.p, B.total_out, B.counters.p, 3, ctx->stream), "clear");
+      check(om_launch_lw_batch(&B.args, B.mixed, B.groups, ctx->stream), "lw batch");
+      check(hipStreamEndCapture(ctx->stream, &graph), "end capture");
+      hipError_t e = hipGraphInstantiate(&B.exec[g], graph, nullptr, nullptr, 0);
+      (void) hipGraphDestroy(graph);
+      check(e, "graph instantiate");
+    }
+    check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    check(hipGraphLaunch(B.exec[g], ctx->stream), "graph launch");
+    check(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    check(hipEventSynchronize(ctx->ev1), "sync");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
+    unsigned long long cnt[3];
+    check(hipMemcpy(cnt, B.counters.p, sizeof(cnt), hipMemcpyDeviceToHost), "counters");
+    if (out) {
+      for (int r = 0; r < B.n_req; ++r)
+        if (out[r])
+          check(hipMemcpy(out[r], B.out.p + B.out_off[r], B.out_n[r] * sizeof(double), hipMemcpyDeviceToHost),
+                "out");
+    }
+    if (stats) {
+      stats->evals = (double) cnt[2];
+      stats->kernel_ms = ms;
+      stats->total_ms = wall.ms();
+      stats->max_level = B.Lb;
+      stats->n_unconverged = (long) cnt[1];
+    }
+  });
+}
+
+int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_items, int* best, double* ms_out) {
+  return guarded(ctx, [&] {
+    auto& B = ctx->batch;
+    if (!B.ready) throw std::logic_error("om_batch_setup has not been called");
+    if (n < 1) throw std::invalid_argument("no candidates");
+    int cus = 0;
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "attribute");
+    batch_levels(ctx);
+    om::BatchArgs A = B.args;
+    A.total_items = sample_items < B.total_items ? sample_items : B.total_items;
+    double best_ms = 1e300;
+    int best_c = candidates[0];
+    for (int i = 0; i < n; ++i) {
+      if (candidates[i] < 1 || candidates[i] > 64) throw std::out_of_range("waves per CU out of range");
+      check(om_launch_batch_clear(B.out.p, B.total_out, B.counters.p, 3, ctx->stream), "clear");
+      check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+      check(om_launch_lw_batch(&A, B.mixed, cus * candidates[i], ctx->stream), "lw batch");
+      check(hipEventRecord(ctx->ev1, ctx->stream), "event");
+      check(hipEventSynchronize(ctx->ev1), "sync");
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
+      if (ms_out) ms_out[i] = ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_c = candidates[i];
+      }
+    }
+    ctx->batch_wpc = best_c;
+    B.groups = cus * best_c;
+    B.drop_graphs();
+    if (best) *best = best_c;
   });
 }
 

@@ -1,0 +1,264 @@
+// Batched multi-particle Lienard-Wiechert sums in one persistent launch.
+//
+// Every particle of the resident trajectory set contributes weight * flux to
+// one request (an observation grid x an energy list); requests may differ in
+// grid size and energy count, and particles in trajectory length.  The work
+// unit is (particle, observation point, 64-energy tile); a persistent grid of
+// single-wave workgroups pulls units from a device counter (a vector atomic on
+// global memory), runs the reference's level loop for its 64 energy lanes and
+// adds the weighted flux into the request's output with FP64 atomics
+// (an incoherent sum over particles; the summation order, and so the last
+// bits, depend on scheduling).
+//
+// Reference behaviour restated (paths relative to the reference root):
+//   multi-particle loop with weight 1/N   src/OSCARSSR.cc:4016-4063, 1778-1818
+//   level loop and convergence           src/OSCARSSR.cc:4179-4245
+//   level sample times                   src/TParticleTrajectoryInterpolated.cc:124-161, 264-270
+//
+// Level samples of all particles live in one block: particle p, level L starts
+// at row p*stride + 2^L - 1 (stride = 2^(Lcap+1) - 1 rows of 10 doubles).
+#include "lw_common.h"
+
+namespace om {
+
+struct LevelJobBatch {
+  const double* t;
+  const double* p;
+  const double* ypp;
+  const TrajDesc* desc;
+  const double* tspan;  // per particle: tstart, tstop (2 doubles)
+  double* rows;
+  long stride;          // rows per particle
+  int np;
+  int Lcap;
+  int stop_mem;         // levels above it use the extended spacing
+};
+
+// TOMATH.h GetValue with the interval search of level_kernel (traj.hip).
+__device__ __forceinline__ void spline_value_b(const double* __restrict__ X, const double* __restrict__ Y,
+                                               const double* __restrict__ YPP, long n, double x, double* out) {
+  double h0 = (X[n - 1] - X[0]) / (double) (n - 1);
+  long k = (long) ((x - X[0]) / h0);
+  if (k < 0) k = 0;
+  if (k > n - 2) k = n - 2;
+  while (k < n - 2 && X[k + 1] <= x) ++k;
+  while (k > 0 && X[k] > x) --k;
+  long klo = k, khi = k + 1;
+  double h = X[khi] - X[klo];
+  double a = (X[khi] - x) / h;
+  double b = (x - X[klo]) / h;
+  double ca = a * a * a - a;
+  double cb = b * b * b - b;
+  const double* Y0 = Y + 9 * klo;
+  const double* Y1 = Y + 9 * khi;
+  const double* P0 = YPP + 9 * klo;
+  const double* P1 = YPP + 9 * khi;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) out[c] = Y0[c] * a + Y1[c] * b + (P0[c] * ca + P1[c] * cb) * (h * h) / 6.;
+}
+
+__global__ void __launch_bounds__(256) level_batch_kernel(LevelJobBatch J) {
+  long g = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long) J.np * J.stride) return;
+  int ip = (int) (g / J.stride);
+  long s = g - (long) ip * J.stride;
+  int L = 63 - __clzll((unsigned long long) (s + 1));  // s in [2^L - 1, 2^(L+1) - 1)
+  long i = s - ((1L << L) - 1);
+  double ts = J.tspan[2 * ip], te = J.tspan[2 * ip + 1];
+  // src/TParticleTrajectoryInterpolated.cc:138-147, 264-270 and
+  // src/TParticleTrajectoryInterpolatedPoints.cc:33-37 (same operations as the host)
+  double p2L = (double) (1L << L);
+  double spacing = L > J.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
+  double tt = ts + (te - ts) / (2.0 * p2L) + spacing * (double) i;
+  TrajDesc D = J.desc[ip];
+  double v[9];
+  spline_value_b(J.t + D.knot_offset, J.p + 9 * D.knot_offset, J.ypp + 9 * D.knot_offset, D.n, tt, v);
+  double* o = J.rows + 10 * g;
+  o[0] = tt;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) o[1 + c] = v[c];
+}
+
+struct ReqDev {
+  const double* obs;    // 3 per point
+  const double* omega;  // per energy
+  double* out;          // n_obs * n_e
+  long n_obs, n_e, n_tiles_e;
+};
+
+struct BatchArgs {
+  const double* rows;
+  long stride;
+  int Lb;               // highest level summed
+  const double* tspan;
+  const int* req_of;
+  const double* weight;
+  const long* item_prefix;  // np + 1
+  int np;
+  const ReqDev* req;
+  double C0, C2;
+  double pol[6];
+  int use_pol;
+  double precision;
+  unsigned long long* counter;
+  long total_items;
+  unsigned long long* n_unconverged;
+  unsigned long long* evals;  // sum over lanes of 2^(L+1)-1 (as integer)
+};
+
+template <bool MIXED>
+__global__ void __launch_bounds__(64) lw_batch_kernel(BatchArgs A) {
+  __shared__ double s_tau[64];
+  __shared__ double s_v[3][64];
+  int lane = threadIdx.x;
+  while (true) {
+    unsigned long long it = 0;
+    if (lane == 0) it = atomicAdd(A.counter, 1ULL);
+    it = __shfl(it, 0);
+    if ((long) it >= A.total_items) break;
+    long item = (long) it;
+    // particle of this work unit: last p with item_prefix[p] <= item
+    int lo = 0, hi = A.np;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (A.item_prefix[mid] <= item) lo = mid; else hi = mid;
+    }
+    int ip = lo;
+    long local = item - A.item_prefix[ip];
+    const ReqDev R = A.req[A.req_of[ip]];
+    long io = local / R.n_tiles_e;
+    long et = local - io * R.n_tiles_e;
+    long ie = et * 64 + lane;
+    bool valid = ie < R.n_e;
+    long iec = valid ? ie : R.n_e - 1;
+    V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
+    double negw = -R.omega[iec];
+    double T = A.tspan[2 * ip + 1] - A.tspan[2 * ip];
+    const double* prow = A.rows + 10 * (long) ip * A.stride;
+    double sum[6] = {0, 0, 0, 0, 0, 0};
+    double lastmag = -1, prec = -1;
+    bool active = valid;
+    bool conv = false;
+    int lev = 0;
+    for (int L = 0; L <= A.Lb; ++L) {
+      if (!__any(active)) break;
+      const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
+      long ns = 1L << L;
+      double ps[6] = {0, 0, 0, 0, 0, 0};
+      double maxd = 0, lastphase = 0;
+      for (long base = 0; base < ns; base += 64) {
+        long si = base + lane;
+        if (si < ns) {
+          Amp a = amplitude(obs, rows + 10 * si);
+          s_tau[lane] = a.tau;
+          s_v[0][lane] = a.V.x;
+          s_v[1][lane] = a.V.y;
+          s_v[2][lane] = a.V.z;
+        }
+        wave_sync();
+        int cnt = (int) ((ns - base) < 64 ? (ns - base) : 64);
+        if (base == 0) lastphase = negw * s_tau[0];
+        int k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            double phi = negw * s_tau[k + u];
+            maxd = fmax(maxd, fabs(phi - lastphase));
+            lastphase = phi;
+            double sn, cs;
+            sincos_mode(MIXED, phi, sn, cs);
+            double vx = s_v[0][k + u], vy = s_v[1][k + u], vz = s_v[2][k + u];
+            ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+            ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+            ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+          }
+        }
+        for (; k < cnt; ++k) {
+          double phi = negw * s_tau[k];
+          maxd = fmax(maxd, fabs(phi - lastphase));
+          lastphase = phi;
+          double sn, cs;
+          sincos_mode(MIXED, phi, sn, cs);
+          double vx = s_v[0][k], vy = s_v[1][k], vz = s_v[2][k];
+          ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+          ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+          ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+        }
+        wave_sync();
+      }
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sum[k] += ps[k];
+        lev = L;
+        // convergence test, src/OSCARSSR.cc:4227-4239
+        double E[6];
+        double dt = T / (2.0 * (double) (1L << L) - 1);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) E[k] = sum[k] * dt;
+        if (A.use_pol) pol_project(E, A.pol);
+        double mag = cmag2(E);
+        prec = fabs(mag - lastmag) / lastmag;
+        bool c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
+        lastmag = mag;
+        if (c) {
+          conv = true;
+          active = false;
+        }
+      }
+    }
+    if (valid) {
+      double dt = T / (2.0 * (double) (1L << lev) - 1);
+      double f = A.C0 * dt;
+      double E[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) E[k] = sum[k] * f;
+      if (A.use_pol) pol_project(E, A.pol);
+      double flux = A.C2 * cmag2(E) * A.weight[ip];
+      atomicAdd(&R.out[io * R.n_e + ie], flux);
+    }
+    // bookkeeping: lanes that stopped without converging, and evaluations done
+    unsigned long long nc = __ballot(valid && !conv);
+    unsigned long long ev = valid ? (unsigned long long) ((2L << lev) - 1) : 0ULL;
+    for (int off = 32; off > 0; off >>= 1) ev += __shfl_down(ev, off);
+    if (lane == 0) {
+      if (nc) atomicAdd(A.n_unconverged, (unsigned long long) __popcll(nc));
+      atomicAdd(A.evals, ev);
+    }
+  }
+}
+
+// Clears n doubles (outputs) and the counters without a memset node, so the
+// sequence can be captured in a HIP graph and replayed.
+__global__ void __launch_bounds__(256) batch_clear_kernel(double* __restrict__ out, long n,
+                                                          unsigned long long* __restrict__ counters, int nc) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = 0.0;
+  if (i < nc) counters[i] = 0ULL;
+}
+
+}  // namespace om
+
+extern "C" hipError_t om_launch_level_batch(const om::LevelJobBatch* J, hipStream_t s) {
+  long total = (long) J->np * J->stride;
+  long blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(om::level_batch_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *J);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_batch_clear(double* out, long n, unsigned long long* counters, int nc,
+                                            hipStream_t s) {
+  long m = n > nc ? n : nc;
+  long blocks = (m + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(om::batch_clear_kernel, dim3((unsigned) blocks), dim3(256), 0, s, out, n, counters, nc);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s) {
+  if (mixed) {
+    hipLaunchKernelGGL(om::lw_batch_kernel<true>, dim3(n_groups), dim3(64), 0, s, *A);
+  } else {
+    hipLaunchKernelGGL(om::lw_batch_kernel<false>, dim3(n_groups), dim3(64), 0, s, *A);
+  }
+  return hipGetLastError();
+}

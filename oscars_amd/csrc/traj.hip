@@ -276,9 +276,13 @@ __global__ void __launch_bounds__(256) spline_pre_kernel(const TrajDesc* __restr
                                                          double* __restrict__ u, long total) {
   long g = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
-  // locate the particle of knot g (few particles: linear scan over descriptors)
-  int ip = 0;
-  while (ip + 1 < np && desc[ip + 1].knot_offset <= g) ++ip;
+  // particle of knot g: last descriptor with knot_offset <= g
+  int lo = 0, hi = np;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (desc[mid].knot_offset <= g) lo = mid; else hi = mid;
+  }
+  int ip = lo;
   long o = desc[ip].knot_offset;
   long N = desc[ip].n;
   long i = g - o;

@@ -285,12 +285,35 @@ class sr:
             pol = "linear"
         return G.polarization_vector(pol, angle if angle != _NOANGLE else 0.0, hdir, pdir)
 
+    def set_seed(self, seed):
+        """Seed of the beam sampler used by nparticles= (numpy PCG64)."""
+        self._seed = int(seed)
+        self._draws = 0
+
     def _particles_for(self, nparticles):
         """Particles to sum: the current one (nparticles=0) or nparticles samples."""
         if nparticles == 0:
             return None
         from .beam import sample_particles
-        return sample_particles(self._beam, nparticles)
+        self._draws = getattr(self, "_draws", 0) + 1
+        rng = np.random.Generator(np.random.PCG64([getattr(self, "_seed", 0), self._draws]))
+        return sample_particles(self._beam, nparticles, rng=rng)
+
+    def _batched(self, nparticles, obs, energies, pol, precision, max_level, max_level_extended, mode):
+        """nparticles particles in one persistent launch (tiled / mixed kernels)."""
+        from . import multiparticle as M
+        plist = self._particles_for(nparticles)
+        dev = self._device()
+        windows = [(self._ctstart, self._ctstop, self._npoints)] * len(plist)
+        w = [1.0 / len(plist)] * len(plist)
+        bf = M.BatchedFlux(dev, self._fields, plist, windows, [(np.asarray(obs), np.asarray(energies))],
+                           [0] * len(plist), w, mode=mode, polarization=pol, precision=precision,
+                           max_level=max_level, max_level_extended=max_level_extended)
+        self._traj_valid = False
+        outs, st = bf.run(recompute_trajectories=False)
+        self.last_stats = st
+        self._last_conv = np.array([st["n_unconverged"] == 0])
+        return outs[0].reshape(-1)
 
     def _run_particles(self, nparticles, fn):
         """Call fn(device, particle_index, weight, accumulate) for every particle."""
@@ -345,7 +368,10 @@ class sr:
             self._last_conv = cv
             return v
 
-        v = self._run_particles(nparticles, run)
+        if nparticles > 0 and mode in ("tiled", "mixed") and q == 0:
+            v = self._batched(nparticles, [o], e_arr, pol, precision, max_level, max_level_extended, mode)
+        else:
+            v = self._run_particles(nparticles, run)
         if not self._last_conv.all():
             _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
         return [[e[i], float(v[i])] for i in range(len(e))]
@@ -422,7 +448,10 @@ class sr:
             self._last_conv = cv
             return v
 
-        v = self._run_particles(nparticles, run)
+        if nparticles > 0 and mode in ("tiled", "mixed") and q == 0:
+            v = self._batched(nparticles, obs_a, e, pol, precision, max_level, max_level_extended, mode)
+        else:
+            v = self._run_particles(nparticles, run)
         return np.array(cont), v.reshape(len(obs), len(e))
 
     # ------------------------------------------------------------ power density
