@@ -140,6 +140,86 @@ def cpu_baseline(work, sweep_levels, n_e_sample=8):
                 cores=ncores, n_e=n_e_sample)
 
 
+def batched_bench(args):
+    """Config: 2048 electrons from a seeded Gaussian beam through the planar
+    undulator, uneven trajectories (8001-20001 points from random start z),
+    16 requests of n x n points (n in 1..41, +-2 mm at 30 m) x 128-1024 energies
+    (100-3000 eV), summed incoherently in one persistent launch per step (HIP
+    graph: RK4 + spline + level samples + sums).  The FP64 tiled batch is the
+    error reference; the plain FP64 path is timed on a particle subset."""
+    import torch
+    from oscars_amd import beam as B
+    from oscars_amd import multiparticle as M
+    from oscars_amd.sr import sr
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    beam = M.config_beam()
+    parts = B.sample_particles(beam, args.particles, rng=rng)
+    windows = M.uneven_windows(parts, rng)
+    reqs = M.random_requests(16, rng)
+    req_of, weights = M.assign_round_robin(args.particles, len(reqs))
+    s = sr()
+    s.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.049], nperiods=31)
+    dev = s._device()
+    print("# device: %s" % dev.info(), file=sys.stderr, flush=True)
+    res = {}
+    for mode in ([args.mode] if args.no_plain else [args.mode, "tiled"]):
+        if mode in res:
+            continue
+        bf = M.BatchedFlux(dev, s._fields, parts, windows, reqs, req_of, weights, mode=mode)
+        best, tt = bf.tune(candidates=(2, 4, 8, 16), sample_items=20000)
+        print("# %s tuner: best %d waves/CU %s" % (mode, best, tt), file=sys.stderr, flush=True)
+        for _ in range(args.warmup):
+            _, st = bf.run(recompute_trajectories=True)
+            print("# %s warmup step %.1f ms" % (mode, st["kernel_ms"]), file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        evals = 0.0
+        kms = 0.0
+        for _ in range(args.steps):
+            outs, st = bf.run(recompute_trajectories=True)
+            evals += st["evals"]
+            kms += st["kernel_ms"]
+            print("# %s step %.1f ms" % (mode, st["kernel_ms"]), file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[mode] = dict(value=evals / dt, ms=1e3 * dt / args.steps, evals=evals / args.steps, outs=outs,
+                         unconverged=st["n_unconverged"], graph_ms=kms / args.steps, wpc=best)
+        print("# %s: %.1f ms/step %.4g evals/s unconverged %d" % (mode, res[mode]["ms"], res[mode]["value"],
+                                                                  st["n_unconverged"]), file=sys.stderr, flush=True)
+    head = res[args.mode]
+    extra = {}
+    if "tiled" in res and args.mode != "tiled":
+        extra["rel_err_vs_fp64_per_request"] = [rel_errors(a, b) for a, b in zip(head["outs"], res["tiled"]["outs"])]
+    if not args.no_plain:
+        # plain FP64 GPU path, particle by particle, on a subset of particles
+        nsub = min(args.plain_particles, args.particles)
+        dev.trajectories(s._fields, M.om_particles(parts[:nsub], windows[:nsub]))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pe = 0.0
+        for i in range(nsub):
+            obs, e = reqs[req_of[i]]
+            v, lev, cv, _, st = dev.flux(i, obs, e, [0.0] * 6, dev.conv(mode="plain"))
+            pe += st["evals"]
+        torch.cuda.synchronize()
+        pdt = time.perf_counter() - t0
+        extra["plain_fp64"] = {"value": pe / pdt, "particles": nsub}
+        extra["ratio_vs_plain"] = head["value"] / (pe / pdt)
+    line = {
+        "metric": "LW integrand evals/s (obs x omega x traj), forward, batched multi-particle",
+        "value": head["value"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64" if args.mode != "mixed" else "fp64 phase/amplitude, fp32 sincos", "data": "synthetic",
+        "config": {"workload": "%d electrons (seed %d, Gaussian beam 0.9 nm/8 pm, beta 1.5/0.8 m, sigE/E 9e-4), "
+                               "uneven 8001-20001-point trajectories, 16 requests n x n (n 1..41, +-2 mm @30 m) x "
+                               "128-1024 energies 100-3000 eV, incoherent sum, HIP graph" % (args.particles, args.seed),
+                   "mode": args.mode, "parallelism": "single GPU", "waves_per_cu": head["wpc"]},
+        "evals_per_step": head["evals"], "unconverged": head["unconverged"],
+    }
+    line.update(extra)
+    print(json.dumps(line, default=float))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +233,11 @@ def main():
                     help="time only the 51x51 x 512-energy sweep (for counter runs of the dominant kernel)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device-to-device copy and exit (reference for FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--workload", default="maps", choices=["maps", "batched"],
+                    help="maps: single-electron flux maps + energy sweep (default); batched: multi-particle sums")
+    ap.add_argument("--particles", type=int, default=2048)
+    ap.add_argument("--plain-particles", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -170,6 +255,8 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "batched":
+        return batched_bench(args)
     from oscars_amd.sr import sr
     s = sr()
     scenario(s)

@@ -162,6 +162,27 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
  * candidate waves-per-CU value and keeps the fastest (returned in *best). */
 int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_items, int* best, double* ms_out);
 
+/* ---- reverse pass (adjoint) ------------------------------------------------
+ * The reference has no derivative path; these entry points differentiate its
+ * flux calculation (OSCARSSR::CalculateFlux, src/OSCARSSR.cc:3905-4068 with the
+ * point loop :4075-4275) at a frozen level: every output point sums levels
+ * 0..level (precision 0, so the level test never stops early).
+ *
+ * om_adjoint_forward: RK4 + spline for one particle (ctstart must equal T0),
+ * keeping the state every ceil(sqrt(N)) steps, then flux (n_obs x n_e, energy
+ * fastest) with polarization 'all'.  Grid fields must carry no rotation.
+ * grad_field: index into fields of the grid whose samples are differentiated
+ * (-1: none). */
+int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int grad_field,
+                       const om_particle* particle, const double* obs_xyz, long n_obs, const double* energies_eV,
+                       long n_e, int level, double* out_flux, om_stats* stats);
+/* Given dLoss/dflux (n_obs x n_e) for the last om_adjoint_forward: dLoss/d(grid
+ * samples) (3 per sample, x y z field components; NULL to skip), dLoss/dx0 [m]
+ * and dLoss/dbeta0 (at fixed gamma).  stats->evals counts the (point, energy,
+ * sample) terms differentiated. */
+int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad_field, double* grad_x0,
+                        double* grad_beta0, om_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

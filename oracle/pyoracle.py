@@ -378,6 +378,26 @@ class Oracle:
 _ref = None
 
 
+def fdref_flux(samples, z0, dz, x0, beta0, current, ctstart, ctstop, npoints, obs, energies_eV, level,
+               long_double=True):
+    """Frozen-level flux (n_obs x n_e) through a 1-D z grid map from fdref.c, in
+    long double (finite-difference reference) or double (checked against the oracle)."""
+    L = oracle_lib()
+    fn = L.fdref_flux_ld if long_double else L.fdref_flux_d
+    fn.argtypes = [_dp, C.c_long, C.c_double, C.c_double, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_long,
+                   _dp, C.c_long, _dp, C.c_long, C.c_int, _dp]
+    s = np.ascontiguousarray(samples, dtype=np.float64).reshape(-1, 3)
+    obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+    e = np.ascontiguousarray(energies_eV, dtype=np.float64).reshape(-1)
+    out = np.zeros(len(obs) * len(e))
+    rc = fn(s.ctypes.data_as(_dp), len(s), float(z0), float(dz), _v3(x0), _v3(beta0), float(current),
+            float(ctstart), float(ctstop), int(npoints), obs.ctypes.data_as(_dp), len(obs), e.ctypes.data_as(_dp),
+            len(e), int(level), out.ctypes.data_as(_dp))
+    if rc:
+        raise ValueError("fdref_flux: bad arguments")
+    return out.reshape(len(obs), len(e))
+
+
 def ref_available():
     return os.path.exists(LIB_REF)
 

@@ -59,7 +59,8 @@ class om_stats(C.Structure):
 
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
            "om_trajectories", "om_trajectory_get", "om_level_get", "om_trajectory_status", "om_flux",
-           "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune"]
+           "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
+           "om_adjoint_forward", "om_adjoint_backward"]
 
 _lib = None
 
@@ -93,6 +94,9 @@ def lib():
                                  C.POINTER(om_batch_opts)]
     L.om_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(om_stats)]
     L.om_batch_tune.argtypes = [C.c_void_p, _ip, C.c_int, C.c_long, _ip, _dp]
+    L.om_adjoint_forward.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.c_int, C.POINTER(om_particle),
+                                     _dp, C.c_long, _dp, C.c_long, C.c_int, _dp, C.POINTER(om_stats)]
+    L.om_adjoint_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(om_stats)]
     _lib = L
     return L
 
@@ -241,6 +245,31 @@ class Device:
         st = om_stats()
         self._chk(self.L.om_batch_run(self.h, int(bool(recompute_trajectories)), arr, C.byref(st)))
         return [o.reshape(n, m) for o, (n, m) in zip(outs, self._batch_shapes)], st.as_dict()
+
+    def adjoint_forward(self, fields, grad_field, particle, obs_xyz, energies_eV, level):
+        """Frozen-level flux (n_obs x n_e) of one particle; keeps what adjoint_backward needs."""
+        fa = (om_field * max(1, len(fields)))(*fields)
+        obs = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(-1, 3)
+        e = np.ascontiguousarray(energies_eV, dtype=np.float64).reshape(-1)
+        out = np.zeros(len(obs) * len(e))
+        st = om_stats()
+        self._chk(self.L.om_adjoint_forward(self.h, fa, len(fields), int(grad_field), C.byref(particle), _ptr(obs),
+                                            len(obs), _ptr(e), len(e), int(level), _ptr(out), C.byref(st)))
+        self._adj_shape = (len(obs), len(e))
+        return out.reshape(len(obs), len(e)), st.as_dict()
+
+    def adjoint_backward(self, dloss_dflux, n_grid):
+        g = np.ascontiguousarray(dloss_dflux, dtype=np.float64).reshape(-1)
+        if g.size != self._adj_shape[0] * self._adj_shape[1]:
+            raise ValueError("dloss_dflux has %d values, the forward produced %d" %
+                             (g.size, self._adj_shape[0] * self._adj_shape[1]))
+        gf = np.zeros((max(n_grid, 1), 3))
+        gx = np.zeros(3)
+        gb = np.zeros(3)
+        st = om_stats()
+        self._chk(self.L.om_adjoint_backward(self.h, _ptr(g), _ptr(gf) if n_grid > 0 else None, _ptr(gx), _ptr(gb),
+                                             C.byref(st)))
+        return gf[:n_grid], gx, gb, st.as_dict()
 
     def batch_tune(self, candidates=(2, 4, 8, 16, 32), sample_items=20000):
         c = np.ascontiguousarray(candidates, dtype=np.int32)

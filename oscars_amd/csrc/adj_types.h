@@ -1,0 +1,34 @@
+// Kernel argument blocks of the reverse pass, shared by adjoint.hip and the host library.
+#pragma once
+
+#include "om_types.h"
+
+namespace om {
+
+struct AdjLwArgs {
+  const double* rows;   // ns samples x 10 (t, x, beta, a/c), levels 0..L concatenated
+  long ns;
+  const double* obs;    // n_obs x 3
+  long n_obs;
+  const double* omega;  // n_e angular frequencies
+  long n_e;
+  const double* g;      // n_obs x n_e x 6: dLoss/d(Re S_k), dLoss/d(Im S_k)
+  long obs_per_chunk;
+  double* partial;      // n_chunks x ns x 9
+};
+
+struct AdjRkArgs {
+  const FieldDev* F;
+  int nfields;
+  int gfield;
+  double* gbar;        // 3 per sample of the differentiated grid
+  double qomg, dt;
+  long nf, K, nseg;
+  const double* ckpt;  // nseg x 6
+  const double* kb;    // nf x 9: dLoss/d(knot x, beta, a/c)
+  double* segmap;      // nseg x 42: M (6x6 row-major), b (6)
+  double* lam;         // (nseg + 1) x 6: lambda at segment starts
+  double* ybuf;        // nf x 6 recomputed states
+};
+
+}  // namespace om

@@ -1,0 +1,607 @@
+// Reverse pass (adjoint) of frozen-level flux values with respect to the
+// samples of one grid field and the particle's initial position and velocity.
+//
+// Forward chain, restated from the reference (paths relative to its root):
+//   flux  F = C2 |C0 dt_L S|^2,  S = sum_j A_j exp(i phi_j)      src/OSCARSSR.cc:4179-4245 (fixed level L)
+//   level samples s_j = spline(t_j)                              include/TOMATH.h:92-126
+//   spline second derivatives from the knots                     include/TOMATH.h:141-203
+//   knots = RK4 states, acceleration stored one step late        src/OSCARSSR.cc:910-1009, 1321-1386
+//   field = linear interpolation of grid samples                 src/TField3D_Grid.cc:153-278
+// Each kernel below applies the transpose of the derivative of one stage.
+//
+// RK4 reverse with checkpoints: the forward keeps the state every K = ceil(sqrt(N))
+// steps.  The adjoint state obeys the affine recurrence
+//   lambda_i = J_i^T lambda_{i+1} + c_i
+// (J_i the Jacobian of step i, c_i the direct knot terms).  Segments of K steps
+// are independent given their checkpoint, so
+//   A: each segment recomputes its states and composes its own affine map
+//      lambda_{i0} = M lambda_{i1} + b (tangents through the four RK4 stages),
+//   B: one thread chains the sqrt(N) segment maps from the end,
+//   C: each segment recomputes its states again and sweeps back from its
+//      lambda_{i1}, scattering the field-sample gradients (FP64 atomics).
+// The sequential depth is about 3 K steps instead of 3 N.
+#include "adj_types.h"
+#include "lw_common.h"
+#include "rk4_common.h"
+
+namespace om {
+
+// ---------------------------------------------------------------- checkpoints
+// Raw knots (before the division of velocity and acceleration by c) hold the
+// RK4 state of every forward step; keep every K-th one in state order.
+__global__ void __launch_bounds__(256) adj_ckpt_kernel(const double* __restrict__ p_raw, long K, long nseg,
+                                                       double* __restrict__ ckpt) {
+  long s = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const double* o = p_raw + 9 * (s * K);
+  double* c = ckpt + 6 * s;
+  c[0] = o[0]; c[1] = o[3];
+  c[2] = o[1]; c[3] = o[4];
+  c[4] = o[2]; c[5] = o[5];
+}
+
+// ---------------------------------------------------------------- LW sum
+__device__ __forceinline__ V3 vscale(V3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 vadd(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+
+// One thread per trajectory sample, a chunk of observation points per
+// workgroup row.  With S_k += A_k (cos phi + i sin phi), phi = -w tau, the
+// energy loop of one point accumulates
+//   P_k = sum_w (gr_k cos phi + gi_k sin phi)              = dLoss/dA_k
+//   Q   = sum_w (-w) sum_k A_k (gi_k cos phi - gr_k sin phi) = dLoss/dtau
+// and the amplitude A(x, beta, a) and tau = t + D/c are then differentiated by hand.
+__global__ void __launch_bounds__(256) adj_lw_kernel(AdjLwArgs A) {
+  long j = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long o0 = (long) blockIdx.y * A.obs_per_chunk;
+  long o1 = o0 + A.obs_per_chunk < A.n_obs ? o0 + A.obs_per_chunk : A.n_obs;
+  bool valid = j < A.ns;
+  const double* s = A.rows + 10 * (valid ? j : A.ns - 1);
+  double t = s[0];
+  V3 X = v3(s[1], s[2], s[3]);
+  V3 Bt = v3(s[4], s[5], s[6]);
+  V3 Ac = v3(s[7], s[8], s[9]);
+  double b2 = mag2(Bt);
+  double gx[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (long o = o0; o < o1; ++o) {
+    V3 ob = v3(A.obs[3 * o], A.obs[3 * o + 1], A.obs[3 * o + 2]);
+    V3 R = sub(ob, X);
+    double D = sqrt(mag2(R));
+    V3 N = v3(R.x / D, R.y / D, R.z / D);
+    V3 W = sub(N, Bt);
+    double k = 1 - dot(N, Bt);
+    double d1 = D * D * k * k, d2 = D * k * k;
+    V3 Y = cross(W, Ac);
+    V3 NY = cross(N, Y);
+    V3 Am = vadd(vscale(W, (1 - b2) / d1), vscale(NY, 1.0 / d2));
+    double tau = t + D / kC;
+    V3 P = v3(0, 0, 0);
+    double Q = 0;
+    const double* go = A.g + 6 * A.n_e * o;
+    for (long e = 0; e < A.n_e; ++e) {
+      double w = A.omega[e];
+      double phi = (-w) * tau;
+      double sn, cs;
+      sincos_huge64(phi, &sn, &cs);
+      const double* ge = go + 6 * e;
+      P.x += ge[0] * cs + ge[1] * sn;
+      P.y += ge[2] * cs + ge[3] * sn;
+      P.z += ge[4] * cs + ge[5] * sn;
+      double dphi = Am.x * (ge[1] * cs - ge[0] * sn) + Am.y * (ge[3] * cs - ge[2] * sn) +
+                    Am.z * (ge[5] * cs - ge[4] * sn);
+      Q += (-w) * dphi;
+    }
+    // s = P . A(x, beta, a), dLoss/dtau = Q
+    double pw = dot(P, W);
+    double s1 = (1 - b2) * pw / d1;
+    V3 PxN = cross(P, N);
+    double s2 = dot(P, NY) / d2;
+    V3 Wb = vadd(vscale(P, (1 - b2) / d1), vscale(cross(Ac, PxN), 1.0 / d2));
+    V3 ab = vscale(cross(PxN, W), 1.0 / d2);
+    V3 Nb = vscale(cross(Y, P), 1.0 / d2);
+    double kb = -2 * (s1 + s2) / k;
+    double b2b = -pw / d1;
+    double Db = -2 * s1 / D - s2 / D + Q / kC;
+    Nb = vadd(vadd(Nb, Wb), vscale(Bt, -kb));
+    V3 bb = vadd(vadd(vscale(Wb, -1.0), vscale(N, -kb)), vscale(Bt, 2 * b2b));
+    double nbn = dot(Nb, N);
+    V3 Rb = vadd(vscale(Nb, 1.0 / D), vscale(N, Db - nbn / D));
+    gx[0] -= Rb.x; gx[1] -= Rb.y; gx[2] -= Rb.z;
+    gx[3] += bb.x; gx[4] += bb.y; gx[5] += bb.z;
+    gx[6] += ab.x; gx[7] += ab.y; gx[8] += ab.z;
+  }
+  if (!valid) return;
+  double* out = A.partial + 9 * ((long) blockIdx.y * A.ns + j);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) out[c] = gx[c];
+}
+
+// Sum of the per-chunk partials in a fixed order (deterministic).
+__global__ void __launch_bounds__(256) adj_chunk_sum_kernel(const double* __restrict__ partial, long ns, int nchunks,
+                                                            double* __restrict__ out) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 9 * ns) return;
+  double s = 0;
+  for (int c = 0; c < nchunks; ++c) s += partial[(long) c * 9 * ns + i];
+  out[i] = s;
+}
+
+// ---------------------------------------------------------------- spline evaluation
+__global__ void __launch_bounds__(256) adj_klo_kernel(const double* __restrict__ rows, long ns,
+                                                      const double* __restrict__ X, long n, int* __restrict__ klo) {
+  long j = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  klo[j] = (int) spline_interval(X, n, rows[10 * j]);
+}
+
+// First index in klo[0..cnt) with klo[i] >= k (klo is non-decreasing within a level).
+__device__ __forceinline__ long lower_bound_int(const int* __restrict__ a, long cnt, int k) {
+  long lo = 0, hi = cnt;
+  while (lo < hi) {
+    long mid = (lo + hi) >> 1;
+    if (a[mid] < k) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Transpose of s_j = a Y_lo + b Y_hi + (ca Y''_lo + cb Y''_hi) h^2/6, gathered
+// per knot (no atomics): knot k collects the samples whose interval starts at
+// k (coefficients a, ca) and at k-1 (coefficients b, cb), level by level.
+__global__ void __launch_bounds__(256) adj_knot_gather_kernel(const double* __restrict__ rows,
+                                                              const double* __restrict__ sb, const int* __restrict__ klo,
+                                                              int L, const double* __restrict__ X, long n,
+                                                              double* __restrict__ yb, double* __restrict__ yppb) {
+  long k = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double ay[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double ap[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int lev = 0; lev <= L; ++lev) {
+    long off = (1L << lev) - 1, cnt = 1L << lev;
+    const int* kl = klo + off;
+    for (int side = 0; side < 2; ++side) {
+      int want = (int) k - side;  // side 0: sample interval starts at k; side 1: at k-1
+      if (want < 0 || want > n - 2) continue;
+      long i = lower_bound_int(kl, cnt, want);
+      for (; i < cnt && kl[i] == want; ++i) {
+        long js = off + i;
+        double x = rows[10 * js];
+        long lo = want, hi = want + 1;
+        double h = X[hi] - X[lo];
+        double a = (X[hi] - x) / h;
+        double b = (x - X[lo]) / h;
+        double cy = side == 0 ? a : b;
+        double cc = side == 0 ? (a * a * a - a) : (b * b * b - b);
+        double cp = cc * (h * h) / 6.;
+        const double* g = sb + 9 * js;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+          ay[c] += cy * g[c];
+          ap[c] += cp * g[c];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    yb[9 * k + c] = ay[c];
+    yppb[9 * k + c] = ap[c];
+  }
+}
+
+// Transpose of the spline build (include/TOMATH.h:141-203 with the end slopes
+// equal to the end secants, so u[0] = un = 0 identically):
+//   p_i = sig_i P_{i-1} + 2, P_i = (sig_i - 1)/p_i, U_i = (w_i - sig_i U_{i-1})/p_i,
+//   Y''_{N-1} = -U_{N-2}/(P_{N-2} + 2), Y''_k = P_k Y''_{k+1} + U_k,
+//   w_i = ((Y_{i+1}-Y_i)/h_i - (Y_i-Y_{i-1})/h_{i-1}) 6/(t_{i+1}-t_{i-1}).
+// One lane per channel; lane 0 first stores P and p (they depend on t only).
+__global__ void __launch_bounds__(64) adj_spline_kernel(const double* __restrict__ X, long n,
+                                                        const double* __restrict__ yppb, double* __restrict__ yb,
+                                                        double* __restrict__ Pw, double* __restrict__ ppw,
+                                                        double* __restrict__ ub) {
+  int c = threadIdx.x;
+  if (c == 0) {
+    double Pp = -0.5;
+    Pw[0] = Pp;
+    ppw[0] = 1;
+    for (long i = 1; i < n - 1; ++i) {
+      double sig = (X[i] - X[i - 1]) / (X[i + 1] - X[i - 1]);
+      double pp = sig * Pp + 2.0;
+      Pp = (sig - 1.0) / pp;
+      Pw[i] = Pp;
+      ppw[i] = pp;
+    }
+  }
+  __syncthreads();
+  if (c >= 9) return;
+  // back substitution, transposed: total adjoint of Y''_k in increasing k
+  double acc = yppb[c];
+  ub[c] = acc;
+  for (long k = 1; k < n; ++k) {
+    acc = yppb[9 * k + c] + Pw[k - 1] * acc;
+    if (k < n - 1) ub[9 * k + c] = acc;
+  }
+  // Y''_{N-1} = (0 - 0.5 U_{N-2}) / (0.5 P_{N-2} + 1)
+  double alpha = -0.5 / (0.5 * Pw[n - 2] + 1.);
+  double carry = alpha * acc;
+  for (long i = n - 2; i >= 1; --i) {
+    double u = ub[9 * i + c] + carry;
+    double sig = (X[i] - X[i - 1]) / (X[i + 1] - X[i - 1]);
+    double wb = u / ppw[i];
+    carry = -sig * wb;
+    double cc = wb * 6. / (X[i + 1] - X[i - 1]);
+    double h1 = X[i + 1] - X[i], h0 = X[i] - X[i - 1];
+    yb[9 * (i + 1) + c] += cc / h1;
+    yb[9 * i + c] += -cc / h1 - cc / h0;
+    yb[9 * (i - 1) + c] += cc / h0;
+  }
+}
+
+// ---------------------------------------------------------------- field derivatives
+// Field, its Jacobian dB_i/dx_j, and for the differentiated grid the lower
+// sample index and the interpolation weight of the upper sample.
+struct FieldD {
+  double B[3];
+  double J[3][3];
+  long n0;  // -1: outside the differentiated grid
+  double w;
+};
+
+__device__ __forceinline__ void und_part_grad(const FieldDev& f, double D, double PS, const double* a, double tc,
+                                              double B[3], double dBdD[3]) {
+  double L = f.und_length, PL = f.period_length;
+  if (D > L / 2. + PS || D < -L / 2. + PS) return;
+  double scale = 1;
+  if (D < -L / 2. + PS + PL || D > L / 2. + PS - PL)
+    scale = (D < -L / 2. + PS + PL / 2. || D > L / 2. + PS - PL / 2.) ? 0.25 : 0.75;
+  double sn, cs;
+  sincos_huge64(div_by(2.0 * kPi * (D - PS), PL, f.inv_period_length), &sn, &cs);
+  double k = 2.0 * kPi / PL;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    B[i] += a[i] * scale * sn * tc;
+    dBdD[i] += a[i] * scale * (cs * k * tc + sn * f.taper);
+  }
+}
+
+__device__ __forceinline__ void field_grad(const FieldDev* __restrict__ F, int nf, int gfield, const double* y,
+                                           FieldD& r) {
+  double x = y[0], yy = y[2], z = y[4];
+  for (int i = 0; i < 3; ++i) {
+    r.B[i] = 0;
+    for (int j = 0; j < 3; ++j) r.J[i][j] = 0;
+  }
+  r.n0 = -1;
+  r.w = 0;
+  for (int fi = 0; fi < nf; ++fi) {
+    const FieldDev& f = F[fi];
+    if (f.type == kUndulator || f.type == kEPU) {
+      double D = (x - f.center[0]) * f.unit[0] + (yy - f.center[1]) * f.unit[1] + (z - f.center[2]) * f.unit[2];
+      double tc = 1 + D * f.taper;
+      double B[3] = {0, 0, 0}, dB[3] = {0, 0, 0};
+      und_part_grad(f, D, f.phase_shift_a, f.field, tc, B, dB);
+      if (f.type == kEPU) und_part_grad(f, D, f.phase_shift_b, f.fieldB, tc, B, dB);
+      for (int i = 0; i < 3; ++i) {
+        r.B[i] += B[i];
+        for (int j = 0; j < 3; ++j) r.J[i][j] += dB[i] * f.unit[j];
+      }
+      continue;
+    }
+    // grid without rotation (the host rejects rotated grids for the reverse pass)
+    double X[3] = {x - f.trans[0], yy - f.trans[1], z - f.trans[2]};
+    if (f.n[0] > 1 && (X[0] <= f.start[0] || X[0] >= f.stop[0])) continue;
+    if (f.n[1] > 1 && (X[1] <= f.start[1] || X[1] >= f.stop[1])) continue;
+    if (f.n[2] > 1 && (X[2] <= f.start[2] || X[2] >= f.stop[2])) continue;
+    int a = f.axis;
+    double coord = X[a], start = f.start[a], step = f.step[a], istep = f.inv_step[a];
+    long na = f.n[a];
+    long n0 = (long) (size_t) div_by(coord - start, step, istep);
+    if (n0 > na - 2) n0 = na - 2;
+    double dd = (coord - start) - (double) n0 * step;
+    const double* lo = f.data + 3 * n0;
+    for (int i = 0; i < 3; ++i) {
+      r.B[i] += lo[i] + div_by((lo[3 + i] - lo[i]) * dd, step, istep);
+      r.J[i][a] += (lo[3 + i] - lo[i]) / step;
+    }
+    if (fi == gfield) {
+      r.n0 = n0;
+      r.w = dd / step;
+    }
+  }
+}
+
+// f(y) = (v, qomg v x B(x)) in state order (x, vx, y, vy, z, vz)
+__device__ __forceinline__ void fval(const FieldD& fd, double qomg, const double* y, double* d) {
+  d[0] = y[1];
+  d[1] = qomg * (-y[5] * fd.B[1] + y[3] * fd.B[2]);
+  d[2] = y[3];
+  d[3] = qomg * (y[5] * fd.B[0] - y[1] * fd.B[2]);
+  d[4] = y[5];
+  d[5] = qomg * (y[1] * fd.B[1] - y[3] * fd.B[0]);
+}
+
+// tangent: out = Jf(y) . t
+__device__ __forceinline__ void fjac_vec(const FieldD& fd, double qomg, const double* y, const double* t, double* out) {
+  double dx[3] = {t[0], t[2], t[4]};
+  double dv[3] = {t[1], t[3], t[5]};
+  double v[3] = {y[1], y[3], y[5]};
+  double dB[3];
+  for (int i = 0; i < 3; ++i) dB[i] = fd.J[i][0] * dx[0] + fd.J[i][1] * dx[1] + fd.J[i][2] * dx[2];
+  // d(v x B) = dv x B + v x dB
+  out[0] = dv[0];
+  out[2] = dv[1];
+  out[4] = dv[2];
+  out[1] = qomg * ((dv[1] * fd.B[2] - dv[2] * fd.B[1]) + (v[1] * dB[2] - v[2] * dB[1]));
+  out[3] = qomg * ((dv[2] * fd.B[0] - dv[0] * fd.B[2]) + (v[2] * dB[0] - v[0] * dB[2]));
+  out[5] = qomg * ((dv[0] * fd.B[1] - dv[1] * fd.B[0]) + (v[0] * dB[1] - v[1] * dB[0]));
+}
+
+// adjoint: ybar += Jf(y)^T w, and the grid-sample gradient of w . f(y) into gbar
+__device__ __forceinline__ void fjac_tvec(const FieldD& fd, double qomg, const double* y, const double* w,
+                                          double* ybar, double* gbar) {
+  double wa[3] = {w[1], w[3], w[5]};
+  double v[3] = {y[1], y[3], y[5]};
+  // v-bar = w_pos + qomg (B x w_a);  B-bar = qomg (w_a x v)
+  ybar[1] += w[0] + qomg * (fd.B[1] * wa[2] - fd.B[2] * wa[1]);
+  ybar[3] += w[2] + qomg * (fd.B[2] * wa[0] - fd.B[0] * wa[2]);
+  ybar[5] += w[4] + qomg * (fd.B[0] * wa[1] - fd.B[1] * wa[0]);
+  double Bb[3] = {qomg * (wa[1] * v[2] - wa[2] * v[1]), qomg * (wa[2] * v[0] - wa[0] * v[2]),
+                  qomg * (wa[0] * v[1] - wa[1] * v[0])};
+  for (int j = 0; j < 3; ++j) ybar[2 * j] += Bb[0] * fd.J[0][j] + Bb[1] * fd.J[1][j] + Bb[2] * fd.J[2][j];
+  if (gbar && fd.n0 >= 0) {
+    double* g0 = gbar + 3 * fd.n0;
+    for (int i = 0; i < 3; ++i) {
+      atomicAdd(g0 + i, (1 - fd.w) * Bb[i]);
+      atomicAdd(g0 + 3 + i, fd.w * Bb[i]);
+    }
+  }
+}
+
+
+
+// c_i: direct knot terms and the late-stored acceleration terms of state i.
+// Acceleration knot a_{i+1} = f_a(y_i)/c, and a_0 = f_a(y_0)/c.
+__device__ __forceinline__ void direct_terms(const AdjRkArgs& A, long i, const FieldD& fd, const double* y,
+                                             double* c, double* gbar) {
+  const double* k = A.kb + 9 * i;
+  for (int r = 0; r < 6; ++r) c[r] = 0;
+  c[0] = k[0]; c[2] = k[1]; c[4] = k[2];
+  c[1] = k[3] / kC; c[3] = k[4] / kC; c[5] = k[5] / kC;
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  bool any = false;
+  if (i + 1 < A.nf) {
+    const double* k1 = A.kb + 9 * (i + 1);
+    w[1] += k1[6] / kC; w[3] += k1[7] / kC; w[5] += k1[8] / kC;
+    any = true;
+  }
+  if (i == 0) {
+    w[1] += k[6] / kC; w[3] += k[7] / kC; w[5] += k[8] / kC;
+    any = true;
+  }
+  if (any) fjac_tvec(fd, A.qomg, y, w, c, gbar);
+}
+
+// Phase A: per segment, lambda_{i0} = M lambda_{i1} + b.
+__global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
+  long s = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.nseg) return;
+  long i0 = s * A.K, i1 = i0 + A.K < A.nf ? i0 + A.K : A.nf;
+  double y[6];
+  for (int r = 0; r < 6; ++r) y[r] = A.ckpt[6 * s + r];
+  double M[6][6], b[6];
+  for (int r = 0; r < 6; ++r) {
+    b[r] = 0;
+    for (int q = 0; q < 6; ++q) M[r][q] = r == q ? 1.0 : 0.0;
+  }
+  RkCtx rc{A.F, A.nfields, A.qomg, 0};
+  UndParams none{};
+  double h = A.dt, hh = h * 0.5, h6 = h / 6.0;
+  for (long i = i0; i < i1; ++i) {
+    FieldD f0;
+    field_grad(A.F, A.nfields, A.gfield, y, f0);
+    double c[6];
+    direct_terms(A, i, f0, y, c, nullptr);
+    for (int r = 0; r < 6; ++r) {
+      double acc = b[r];
+      for (int q = 0; q < 6; ++q) acc += M[r][q] * c[q];
+      b[r] = acc;
+    }
+    if (i == A.nf - 1) break;
+    // stage states and their field derivatives
+    double k1[6], y1[6], k2[6], y2[6], k3[6], y3[6];
+    fval(f0, A.qomg, y, k1);
+    for (int r = 0; r < 6; ++r) y1[r] = y[r] + hh * k1[r];
+    FieldD f1;
+    field_grad(A.F, A.nfields, A.gfield, y1, f1);
+    fval(f1, A.qomg, y1, k2);
+    for (int r = 0; r < 6; ++r) y2[r] = y[r] + hh * k2[r];
+    FieldD f2;
+    field_grad(A.F, A.nfields, A.gfield, y2, f2);
+    fval(f2, A.qomg, y2, k3);
+    for (int r = 0; r < 6; ++r) y3[r] = y[r] + h * k3[r];
+    FieldD f3;
+    field_grad(A.F, A.nfields, A.gfield, y3, f3);
+    // J column by column: J e_q
+    double J[6][6];
+    for (int q = 0; q < 6; ++q) {
+      double e[6] = {0, 0, 0, 0, 0, 0};
+      e[q] = 1.0;
+      double d1[6], d2[6], d3[6], d4[6], t[6];
+      fjac_vec(f0, A.qomg, y, e, d1);
+      for (int r = 0; r < 6; ++r) t[r] = e[r] + hh * d1[r];
+      fjac_vec(f1, A.qomg, y1, t, d2);
+      for (int r = 0; r < 6; ++r) t[r] = e[r] + hh * d2[r];
+      fjac_vec(f2, A.qomg, y2, t, d3);
+      for (int r = 0; r < 6; ++r) t[r] = e[r] + h * d3[r];
+      fjac_vec(f3, A.qomg, y3, t, d4);
+      for (int r = 0; r < 6; ++r) J[r][q] = e[r] + h6 * (d1[r] + d4[r] + 2.0 * (d2[r] + d3[r]));
+    }
+    // M <- M J^T
+    double Mn[6][6];
+    for (int r = 0; r < 6; ++r)
+      for (int q = 0; q < 6; ++q) {
+        double acc = 0;
+        for (int m = 0; m < 6; ++m) acc += M[r][m] * J[q][m];
+        Mn[r][q] = acc;
+      }
+    for (int r = 0; r < 6; ++r)
+      for (int q = 0; q < 6; ++q) M[r][q] = Mn[r][q];
+    // next state, as the forward computes it
+    double d[6];
+    derivs<false>(rc, none, y, d);
+    rk4<false>(rc, none, y, d, A.dt);
+  }
+  double* o = A.segmap + 42 * s;
+  for (int r = 0; r < 6; ++r)
+    for (int q = 0; q < 6; ++q) o[6 * r + q] = M[r][q];
+  for (int r = 0; r < 6; ++r) o[36 + r] = b[r];
+}
+
+// Phase B: chain the segment maps from the end (lambda after the last knot is 0).
+__global__ void __launch_bounds__(64) adj_chain_kernel(AdjRkArgs A) {
+  if (threadIdx.x != 0) return;
+  double l[6] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < 6; ++r) A.lam[6 * A.nseg + r] = 0;
+  for (long s = A.nseg - 1; s >= 0; --s) {
+    const double* o = A.segmap + 42 * s;
+    double n[6];
+    for (int r = 0; r < 6; ++r) {
+      double acc = o[36 + r];
+      for (int q = 0; q < 6; ++q) acc += o[6 * r + q] * l[q];
+      n[r] = acc;
+    }
+    for (int r = 0; r < 6; ++r) {
+      l[r] = n[r];
+      A.lam[6 * s + r] = n[r];
+    }
+  }
+}
+
+// Phase C: per segment, recompute the states and sweep back from lambda_{i1},
+// scattering the grid-sample gradients.
+__global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
+  long s = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.nseg) return;
+  long i0 = s * A.K, i1 = i0 + A.K < A.nf ? i0 + A.K : A.nf;
+  RkCtx rc{A.F, A.nfields, A.qomg, 0};
+  UndParams none{};
+  double y[6];
+  for (int r = 0; r < 6; ++r) y[r] = A.ckpt[6 * s + r];
+  for (long i = i0; i < i1; ++i) {
+    for (int r = 0; r < 6; ++r) A.ybuf[6 * i + r] = y[r];
+    if (i + 1 < i1) {
+      double d[6];
+      derivs<false>(rc, none, y, d);
+      rk4<false>(rc, none, y, d, A.dt);
+    }
+  }
+  double lam[6];
+  for (int r = 0; r < 6; ++r) lam[r] = A.lam[6 * (s + 1) + r];
+  double h = A.dt, hh = h * 0.5, h6 = h / 6.0;
+  for (long i = i1 - 1; i >= i0; --i) {
+    for (int r = 0; r < 6; ++r) y[r] = A.ybuf[6 * i + r];
+    FieldD f0;
+    field_grad(A.F, A.nfields, A.gfield, y, f0);
+    double yb[6] = {0, 0, 0, 0, 0, 0};
+    if (i < A.nf - 1) {
+      double k1[6], y1[6], k2[6], y2[6], k3[6], y3[6];
+      fval(f0, A.qomg, y, k1);
+      for (int r = 0; r < 6; ++r) y1[r] = y[r] + hh * k1[r];
+      FieldD f1;
+      field_grad(A.F, A.nfields, A.gfield, y1, f1);
+      fval(f1, A.qomg, y1, k2);
+      for (int r = 0; r < 6; ++r) y2[r] = y[r] + hh * k2[r];
+      FieldD f2;
+      field_grad(A.F, A.nfields, A.gfield, y2, f2);
+      fval(f2, A.qomg, y2, k3);
+      for (int r = 0; r < 6; ++r) y3[r] = y[r] + h * k3[r];
+      FieldD f3;
+      field_grad(A.F, A.nfields, A.gfield, y3, f3);
+      // yo = y + h6 (k1 + k4 + 2 (k2 + k3))
+      double k1b[6], k2b[6], k3b[6], k4b[6];
+      for (int r = 0; r < 6; ++r) {
+        yb[r] = lam[r];
+        k1b[r] = h6 * lam[r];
+        k4b[r] = h6 * lam[r];
+        k2b[r] = 2.0 * h6 * lam[r];
+        k3b[r] = 2.0 * h6 * lam[r];
+      }
+      double sb[6];
+      for (int r = 0; r < 6; ++r) sb[r] = 0;
+      fjac_tvec(f3, A.qomg, y3, k4b, sb, A.gbar);  // k4 = f(y3), y3 = y + h k3
+      for (int r = 0; r < 6; ++r) {
+        yb[r] += sb[r];
+        k3b[r] += h * sb[r];
+        sb[r] = 0;
+      }
+      fjac_tvec(f2, A.qomg, y2, k3b, sb, A.gbar);  // k3 = f(y2), y2 = y + hh k2
+      for (int r = 0; r < 6; ++r) {
+        yb[r] += sb[r];
+        k2b[r] += hh * sb[r];
+        sb[r] = 0;
+      }
+      fjac_tvec(f1, A.qomg, y1, k2b, sb, A.gbar);  // k2 = f(y1), y1 = y + hh k1
+      for (int r = 0; r < 6; ++r) {
+        yb[r] += sb[r];
+        k1b[r] += hh * sb[r];
+      }
+      fjac_tvec(f0, A.qomg, y, k1b, yb, A.gbar);  // k1 = f(y)
+    }
+    double c[6];
+    direct_terms(A, i, f0, y, c, A.gbar);
+    for (int r = 0; r < 6; ++r) lam[r] = yb[r] + c[r];
+  }
+  if (s == 0) {
+    for (int r = 0; r < 6; ++r) A.lam[r] = lam[r];
+  }
+}
+
+__global__ void __launch_bounds__(256) adj_fill_kernel(double* __restrict__ p, long n) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+}  // namespace om
+
+// ---------------------------------------------------------------- launchers (host)
+extern "C" hipError_t om_launch_adj_ckpt(const double* p_raw, long K, long nseg, double* ckpt, hipStream_t s) {
+  long blocks = (nseg + 255) / 256;
+  hipLaunchKernelGGL(om::adj_ckpt_kernel, dim3((unsigned) blocks), dim3(256), 0, s, p_raw, K, nseg, ckpt);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_adj_lw(const om::AdjLwArgs* A, int nchunks, hipStream_t s) {
+  long bx = (A->ns + 255) / 256;
+  hipLaunchKernelGGL(om::adj_lw_kernel, dim3((unsigned) bx, (unsigned) nchunks), dim3(256), 0, s, *A);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_adj_chunk_sum(const double* partial, long ns, int nchunks, double* out,
+                                              hipStream_t s) {
+  long blocks = (9 * ns + 255) / 256;
+  hipLaunchKernelGGL(om::adj_chunk_sum_kernel, dim3((unsigned) blocks), dim3(256), 0, s, partial, ns, nchunks, out);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_adj_spline(const double* rows, long ns, const double* sb, int L, const double* X,
+                                           long n, int* klo, double* yb, double* yppb, double* Pw, double* ppw,
+                                           double* ub, hipStream_t s) {
+  hipLaunchKernelGGL(om::adj_klo_kernel, dim3((unsigned) ((ns + 255) / 256)), dim3(256), 0, s, rows, ns, X, n, klo);
+  hipLaunchKernelGGL(om::adj_knot_gather_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, rows, sb, klo,
+                     L, X, n, yb, yppb);
+  hipLaunchKernelGGL(om::adj_spline_kernel, dim3(1), dim3(64), 0, s, X, n, yppb, yb, Pw, ppw, ub);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_adj_rk4(const om::AdjRkArgs* A, hipStream_t s) {
+  long blocks = (A->nseg + 63) / 64;
+  hipLaunchKernelGGL(om::adj_seg_map_kernel, dim3((unsigned) blocks), dim3(64), 0, s, *A);
+  hipLaunchKernelGGL(om::adj_chain_kernel, dim3(1), dim3(64), 0, s, *A);
+  hipLaunchKernelGGL(om::adj_seg_reverse_kernel, dim3((unsigned) blocks), dim3(64), 0, s, *A);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_adj_fill(double* p, long n, hipStream_t s) {
+  long blocks = (n + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(om::adj_fill_kernel, dim3((unsigned) blocks), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}

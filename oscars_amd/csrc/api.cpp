@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/oscars_mi.h"
+#include "adj_types.h"
 #include "om_types.h"
 
 namespace om {
@@ -144,6 +145,14 @@ hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s);
 hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s);
 hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s);
 hipError_t om_launch_count_active(const om::State* st, long npts, int* out, hipStream_t s);
+hipError_t om_launch_adj_ckpt(const double* p_raw, long K, long nseg, double* ckpt, hipStream_t s);
+hipError_t om_launch_adj_lw(const om::AdjLwArgs* A, int nchunks, hipStream_t s);
+hipError_t om_launch_adj_chunk_sum(const double* partial, long ns, int nchunks, double* out, hipStream_t s);
+hipError_t om_launch_adj_spline(const double* rows, long ns, const double* sb, int L, const double* X, long n,
+                                int* klo, double* yb, double* yppb, double* Pw, double* ppw, double* ub,
+                                hipStream_t s);
+hipError_t om_launch_adj_rk4(const om::AdjRkArgs* A, hipStream_t s);
+hipError_t om_launch_adj_fill(double* p, long n, hipStream_t s);
 }
 
 namespace {
@@ -252,6 +261,25 @@ struct om_context {
     }
   } batch;
 
+  // reverse pass: checkpoints and the forward results it needs
+  struct Adj {
+    bool armed = false;   // the next trajectory launch keeps checkpoints
+    bool ready = false;   // a forward pass is stored for om_adjoint_backward
+    long K = 0, nseg = 0, nf = 0, ngrid = 0;
+    int gfield = -1, L = 0;
+    long n_obs = 0, n_e = 0;
+    double C0 = 0, C2 = 0, dt_L = 0;
+    std::vector<double> E;  // 6 per output: C0 dt_L S (no polarization projection)
+    DevBuf<double> ckpt, rows, obs, omega, g, partial, sb, yb, yppb, Pw, ppw, ub, segmap, lam, ybuf, gbar;
+    DevBuf<int> klo;
+    void release() {
+      ckpt.release(); rows.release(); obs.release(); omega.release(); g.release(); partial.release();
+      sb.release(); yb.release(); yppb.release(); Pw.release(); ppw.release(); ub.release();
+      segmap.release(); lam.release(); ybuf.release(); gbar.release(); klo.release();
+      ready = false;
+    }
+  } adj;
+
   void free_levels() {
     for (auto& lc : levels)
       for (int L = 0; L <= om::kMaxLevel; ++L)
@@ -264,6 +292,7 @@ struct om_context {
 
   ~om_context() {
     batch.release();
+    adj.release();
     free_levels();
     for (auto& g : grid_bufs) g.release();
     d_desc.release(); d_t.release(); d_p.release(); d_ypp.release(); d_u.release(); d_fields.release();
@@ -544,6 +573,16 @@ void launch_trajectories(om_context* ctx, int np) {
   std::memset(&none, 0, sizeof(none));
   check(om_launch_rk4(ctx->d_fields.p, ctx->nfields > 0 ? &ctx->fields_host[0] : &none, ctx->nfields,
                       ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->stream), "rk4");
+  if (ctx->adj.armed) {
+    // states every K steps of particle 0, taken from the raw knots before the division by c
+    auto& A = ctx->adj;
+    A.nf = ctx->desc[0].nf;
+    A.K = (long) std::ceil(std::sqrt((double) A.nf));
+    A.nseg = (A.nf + A.K - 1) / A.K;
+    A.ckpt.ensure(6 * A.nseg);
+    check(om_launch_adj_ckpt(ctx->d_p.p + 9 * ctx->desc[0].knot_offset, A.K, A.nseg, A.ckpt.p, ctx->stream),
+          "checkpoints");
+  }
   check(om_launch_knot_finish(ctx->d_p.p, ctx->total_knots, ctx->stream), "knot finish");
   check(om_launch_spline(ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->d_ypp.p, ctx->d_u.p, ctx->d_sig.p,
                          ctx->d_sm1.p, ctx->total_knots, ctx->stream), "spline");
@@ -691,6 +730,7 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     for (int i = 0; i < nfields; ++i) F.push_back(make_field(ctx, fields[i]));
     ctx->batch.drop_graphs();
     ctx->batch.ready = false;
+    ctx->adj.ready = false;
     ctx->fields_host = F;
     ctx->nfields = nfields;
     ctx->d_fields.ensure(nfields > 0 ? nfields : 1);
@@ -1056,6 +1096,183 @@ int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_ite
     B.groups = cus * best_c;
     B.drop_graphs();
     if (best) *best = best_c;
+  });
+}
+
+
+int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int grad_field,
+                       const om_particle* particle, const double* obs_xyz, long n_obs, const double* energies_eV,
+                       long n_e, int level, double* out_flux, om_stats* stats) {
+  int rc = guarded(ctx, [&] {
+    ctx->adj.ready = false;
+    if (!particle || !obs_xyz || !energies_eV || !out_flux) throw std::invalid_argument("missing adjoint inputs");
+    if (n_obs < 1 || n_e < 1) throw std::length_error("no points specified");
+    if (level < 0 || level > 20) throw std::out_of_range("adjoint level must be in 0..20");
+    if (nfields < 0 || (nfields > 0 && !fields)) throw std::invalid_argument("fields");
+    if (grad_field < -1 || grad_field >= nfields) throw std::out_of_range("grad_field out of range");
+    for (int i = 0; i < nfields; ++i)
+      if (fields[i].type == OM_FIELD_GRID &&
+          (fields[i].rotations[0] != 0 || fields[i].rotations[1] != 0 || fields[i].rotations[2] != 0))
+        throw std::invalid_argument("the reverse pass supports grid fields without rotations only");
+    if (grad_field >= 0 && fields[grad_field].type != OM_FIELD_GRID)
+      throw std::invalid_argument("grad_field must name a grid field");
+    ctx->adj.armed = true;
+  });
+  if (rc) return rc;
+  rc = om_trajectories(ctx, fields, nfields, particle, 1);
+  ctx->adj.armed = false;
+  if (rc) return rc;
+  std::vector<double> E(6 * n_obs * n_e);
+  rc = guarded(ctx, [&] {
+    if (ctx->desc[0].status != 0) throw std::out_of_range("trajectory integration met |beta| >= 1");
+    if (ctx->desc[0].nb != 0)
+      throw std::invalid_argument("the reverse pass needs ctstart == T0 (no backward integration)");
+  });
+  if (rc) return rc;
+  om_conv c;
+  std::memset(&c, 0, sizeof(c));
+  c.precision = 0;  // no level test: every point sums levels 0..level
+  c.max_level = level;
+  c.max_level_extended = 0;
+  c.quantity = 0;
+  c.mode = OM_MODE_TILED;
+  c.weight = 1;
+  c.accumulate = 0;
+  rc = lw_sums(ctx, 0, obs_xyz, n_obs, energies_eV, n_e, nullptr, &c, out_flux, nullptr, nullptr, E.data(), stats,
+               false);
+  if (rc) return rc;
+  return guarded(ctx, [&] {
+    auto& A = ctx->adj;
+    const om_particle& P = ctx->parts[0];
+    A.gfield = grad_field;
+    A.ngrid = 0;
+    if (grad_field >= 0) {
+      const om::FieldDev& f = ctx->fields_host[grad_field];
+      A.ngrid = f.n[0] * f.n[1] * f.n[2];
+    }
+    A.L = level;
+    A.n_obs = n_obs;
+    A.n_e = n_e;
+    A.E.swap(E);
+    A.C0 = P.q / (four_pi() * kC * kEps0 * kSqrt2Pi);
+    A.C2 = four_pi() * P.current / (kH * std::fabs(P.q) * kMu0 * kC) * 1e-6 * 0.001;
+    A.dt_L = dt_inclusive(ctx->tstart[0], ctx->tstop[0], level);
+    std::vector<double> om(n_e);
+    for (long i = 0; i < n_e; ++i) om[i] = ev_to_omega(energies_eV[i]);
+    upload(ctx, A.obs, obs_xyz, 3 * n_obs);
+    upload(ctx, A.omega, om.data(), n_e);
+    A.ready = true;
+  });
+}
+
+int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad_field, double* grad_x0,
+                        double* grad_beta0, om_stats* stats) {
+  return guarded(ctx, [&] {
+    Timer wall;
+    auto& A = ctx->adj;
+    if (!A.ready) throw std::logic_error("om_adjoint_forward has not been called for the current trajectory");
+    if (!dloss_dflux) throw std::invalid_argument("dloss_dflux is NULL");
+    const long npts = A.n_obs * A.n_e;
+    const om::TrajDesc& D = ctx->desc[0];
+    // dLoss/dS from dLoss/dF: F = C2 |E|^2, E = C0 dt_L S
+    std::vector<double> g(6 * npts);
+    double f = 2.0 * A.C2 * A.C0 * A.dt_L;
+    for (long p = 0; p < npts; ++p)
+      for (int k = 0; k < 6; ++k) g[6 * p + k] = dloss_dflux[p] * f * A.E[6 * p + k];
+    upload(ctx, A.g, g.data(), g.size());
+    // level samples 0..L of particle 0, concatenated
+    int stop_mem = A.L;
+    ensure_levels(ctx, 0, A.L, stop_mem);
+    long ns = (2L << A.L) - 1;
+    A.rows.ensure(10 * ns);
+    for (int L = 0; L <= A.L; ++L)
+      check(hipMemcpyAsync(A.rows.p + 10 * ((1L << L) - 1), ctx->levels[0].rows[L], (1L << L) * 10 * sizeof(double),
+                           hipMemcpyDeviceToDevice, ctx->stream), "rows");
+    hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
+    check(hipEventRecord(e0, ctx->stream), "event");
+    // 1. LW sum: dLoss/d(sample x, beta, a/c)
+    int cus = 0;
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "attribute");
+    long target = (long) cus * 8 * 64;  // about 8 waves per CU
+    long nchunks = (target + ns - 1) / ns;
+    if (nchunks > A.n_obs) nchunks = A.n_obs;
+    if (nchunks < 1) nchunks = 1;
+    long per = (A.n_obs + nchunks - 1) / nchunks;
+    nchunks = (A.n_obs + per - 1) / per;
+    A.partial.ensure(9 * ns * nchunks);
+    A.sb.ensure(9 * ns);
+    om::AdjLwArgs L;
+    L.rows = A.rows.p;
+    L.ns = ns;
+    L.obs = A.obs.p;
+    L.n_obs = A.n_obs;
+    L.omega = A.omega.p;
+    L.n_e = A.n_e;
+    L.g = A.g.p;
+    L.obs_per_chunk = per;
+    L.partial = A.partial.p;
+    check(om_launch_adj_lw(&L, (int) nchunks, ctx->stream), "adjoint LW");
+    check(om_launch_adj_chunk_sum(A.partial.p, ns, (int) nchunks, A.sb.p, ctx->stream), "adjoint sum");
+    // 2. spline evaluation and spline build: dLoss/d(knots)
+    long n = D.n;
+    A.klo.ensure(ns);
+    A.yb.ensure(9 * n);
+    A.yppb.ensure(9 * n);
+    A.Pw.ensure(n);
+    A.ppw.ensure(n);
+    A.ub.ensure(9 * n);
+    const double* X = ctx->d_t.p + D.knot_offset;
+    check(om_launch_adj_spline(A.rows.p, ns, A.sb.p, A.L, X, n, A.klo.p, A.yb.p, A.yppb.p, A.Pw.p, A.ppw.p, A.ub.p,
+                               ctx->stream), "adjoint spline");
+    // 3. RK4 with checkpoints: dLoss/d(initial state), dLoss/d(grid samples)
+    A.segmap.ensure(42 * A.nseg);
+    A.lam.ensure(6 * (A.nseg + 1));
+    A.ybuf.ensure(6 * A.nf);
+    A.gbar.ensure(A.ngrid > 0 ? 3 * A.ngrid : 3);
+    check(om_launch_adj_fill(A.gbar.p, A.ngrid > 0 ? 3 * A.ngrid : 3, ctx->stream), "clear");
+    om::AdjRkArgs R;
+    R.F = ctx->d_fields.p;
+    R.nfields = ctx->nfields;
+    R.gfield = A.gfield;
+    R.gbar = A.ngrid > 0 ? A.gbar.p : nullptr;
+    R.qomg = D.qomg;
+    R.dt = D.dt;
+    R.nf = A.nf;
+    R.K = A.K;
+    R.nseg = A.nseg;
+    R.ckpt = A.ckpt.p;
+    R.kb = A.yb.p;
+    R.segmap = A.segmap.p;
+    R.lam = A.lam.p;
+    R.ybuf = A.ybuf.p;
+    check(om_launch_adj_rk4(&R, ctx->stream), "adjoint rk4");
+    check(hipEventRecord(e1, ctx->stream), "event");
+    double lam0[6];
+    check(hipMemcpyAsync(lam0, A.lam.p, sizeof(lam0), hipMemcpyDeviceToHost, ctx->stream), "lambda");
+    if (grad_field && A.ngrid > 0)
+      check(hipMemcpyAsync(grad_field, A.gbar.p, 3 * A.ngrid * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+            "grad field");
+    check(hipStreamSynchronize(ctx->stream), "sync");
+    if (grad_x0) {
+      grad_x0[0] = lam0[0];
+      grad_x0[1] = lam0[2];
+      grad_x0[2] = lam0[4];
+    }
+    // state velocity v = beta c
+    if (grad_beta0) {
+      grad_beta0[0] = lam0[1] * kC;
+      grad_beta0[1] = lam0[3] * kC;
+      grad_beta0[2] = lam0[5] * kC;
+    }
+    if (stats) {
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+      stats->evals = (double) npts * (double) ns;
+      stats->kernel_ms = ms;
+      stats->total_ms = wall.ms();
+      stats->max_level = A.L;
+      stats->n_unconverged = 0;
+    }
   });
 }
 

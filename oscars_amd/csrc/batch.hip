@@ -18,6 +18,7 @@
 // Level samples of all particles live in one block: particle p, level L starts
 // at row p*stride + 2^L - 1 (stride = 2^(Lcap+1) - 1 rows of 10 doubles).
 #include "lw_common.h"
+#include "rk4_common.h"
 
 namespace om {
 
@@ -34,29 +35,6 @@ struct LevelJobBatch {
   int stop_mem;         // levels above it use the extended spacing
 };
 
-// TOMATH.h GetValue with the interval search of level_kernel (traj.hip).
-__device__ __forceinline__ void spline_value_b(const double* __restrict__ X, const double* __restrict__ Y,
-                                               const double* __restrict__ YPP, long n, double x, double* out) {
-  double h0 = (X[n - 1] - X[0]) / (double) (n - 1);
-  long k = (long) ((x - X[0]) / h0);
-  if (k < 0) k = 0;
-  if (k > n - 2) k = n - 2;
-  while (k < n - 2 && X[k + 1] <= x) ++k;
-  while (k > 0 && X[k] > x) --k;
-  long klo = k, khi = k + 1;
-  double h = X[khi] - X[klo];
-  double a = (X[khi] - x) / h;
-  double b = (x - X[klo]) / h;
-  double ca = a * a * a - a;
-  double cb = b * b * b - b;
-  const double* Y0 = Y + 9 * klo;
-  const double* Y1 = Y + 9 * khi;
-  const double* P0 = YPP + 9 * klo;
-  const double* P1 = YPP + 9 * khi;
-#pragma unroll
-  for (int c = 0; c < 9; ++c) out[c] = Y0[c] * a + Y1[c] * b + (P0[c] * ca + P1[c] * cb) * (h * h) / 6.;
-}
-
 __global__ void __launch_bounds__(256) level_batch_kernel(LevelJobBatch J) {
   long g = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (long) J.np * J.stride) return;
@@ -72,7 +50,7 @@ __global__ void __launch_bounds__(256) level_batch_kernel(LevelJobBatch J) {
   double tt = ts + (te - ts) / (2.0 * p2L) + spacing * (double) i;
   TrajDesc D = J.desc[ip];
   double v[9];
-  spline_value_b(J.t + D.knot_offset, J.p + 9 * D.knot_offset, J.ypp + 9 * D.knot_offset, D.n, tt, v);
+  spline_value(J.t + D.knot_offset, J.p + 9 * D.knot_offset, J.ypp + 9 * D.knot_offset, D.n, tt, v);
   double* o = J.rows + 10 * g;
   o[0] = tt;
 #pragma unroll

@@ -19,187 +19,9 @@
 
 #include "om_device.h"
 #include "om_types.h"
+#include "rk4_common.h"
 
 namespace om {
-
-// ---------------------------------------------------------------- fields
-// Written without pointers into the field description or run-time array
-// indices, so a single field held by value stays in registers.
-__device__ __forceinline__ void undulator_part(const FieldDev& f, double D, double PhaseShift, double a0, double a1,
-                                               double a2, double taper_corr, double out[3], bool add) {
-  double L = f.und_length, PL = f.period_length;
-  if (D > L / 2. + PhaseShift || D < -L / 2. + PhaseShift) return;
-  double scale = 1;
-  bool term = false;
-  if (D < -L / 2. + PhaseShift + PL || D > L / 2. + PhaseShift - PL) {
-    term = true;
-    scale = (D < -L / 2. + PhaseShift + PL / 2. || D > L / 2. + PhaseShift - PL / 2.) ? 0.25 : 0.75;
-  }
-  double s = sin64(div_by(2.0 * kPi * (D - PhaseShift), PL, f.inv_period_length));
-  double b0 = term ? a0 * scale : a0;
-  double b1 = term ? a1 * scale : a1;
-  double b2 = term ? a2 * scale : a2;
-  double v0 = b0 * s * taper_corr, v1 = b1 * s * taper_corr, v2 = b2 * s * taper_corr;
-  out[0] = add ? out[0] + v0 : v0;
-  out[1] = add ? out[1] + v1 : v1;
-  out[2] = add ? out[2] + v2 : v2;
-}
-
-__device__ __forceinline__ void rot_xyz(double v[3], const FieldDev& f) {
-  // src/TVector3D.cc RotateSelfX, Y, Z with the angle's sin/cos precomputed on the host
-  double s = f.rot_sin[0], c = f.rot_cos[0], yy = v[1];
-  v[1] = c * yy - s * v[2];
-  v[2] = s * yy + c * v[2];
-  s = f.rot_sin[1]; c = f.rot_cos[1];
-  double zz = v[2];
-  v[2] = c * zz - s * v[0];
-  v[0] = s * zz + c * v[0];
-  s = f.rot_sin[2]; c = f.rot_cos[2];
-  double xx = v[0];
-  v[0] = c * xx - s * v[1];
-  v[1] = s * xx + c * v[1];
-}
-
-__device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y, double z, double out[3]) {
-  out[0] = out[1] = out[2] = 0;
-  if (f.type == kUndulator || f.type == kEPU) {
-    double D = (x - f.center[0]) * f.unit[0] + (y - f.center[1]) * f.unit[1] + (z - f.center[2]) * f.unit[2];
-    double tc = 1 + D * f.taper;
-    undulator_part(f, D, f.phase_shift_a, f.field[0], f.field[1], f.field[2], tc, out, false);
-    if (f.type == kEPU) undulator_part(f, D, f.phase_shift_b, f.fieldB[0], f.fieldB[1], f.fieldB[2], tc, out, true);
-    return;
-  }
-  double X[3] = {x, y, z};
-  rot_xyz(X, f);
-  X[0] -= f.trans[0]; X[1] -= f.trans[1]; X[2] -= f.trans[2];
-  if (f.n[0] > 1 && (X[0] <= f.start[0] || X[0] >= f.stop[0])) return;
-  if (f.n[1] > 1 && (X[1] <= f.start[1] || X[1] >= f.stop[1])) return;
-  if (f.n[2] > 1 && (X[2] <= f.start[2] || X[2] >= f.stop[2])) return;
-  int a = f.axis;
-  double coord = a == 0 ? X[0] : (a == 1 ? X[1] : X[2]);
-  double start = a == 0 ? f.start[0] : (a == 1 ? f.start[1] : f.start[2]);
-  double step = a == 0 ? f.step[0] : (a == 1 ? f.step[1] : f.step[2]);
-  double istep = a == 0 ? f.inv_step[0] : (a == 1 ? f.inv_step[1] : f.inv_step[2]);
-  long na = a == 0 ? f.n[0] : (a == 1 ? f.n[1] : f.n[2]);
-  long n0 = (long) (size_t) div_by(coord - start, step, istep);
-  if (n0 > na - 2) n0 = na - 2;  // keeps the n0+1 read inside the map
-  double dd = (coord - start) - (double) n0 * step;
-  const double* lo = f.data + 3 * n0;
-  double l0 = lo[0], l1 = lo[1], l2 = lo[2], h0 = lo[3], h1 = lo[4], h2 = lo[5];
-  out[0] = (l0 + div_by((h0 - l0) * dd, step, istep)) * 1.0;
-  out[1] = (l1 + div_by((h1 - l1) * dd, step, istep)) * 1.0;
-  out[2] = (l2 + div_by((h2 - l2) * dd, step, istep)) * 1.0;
-}
-
-__device__ __forceinline__ void get_b(const FieldDev* __restrict__ F, int nf, double x, double y, double z,
-                                      double B[3]) {
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (int i = 0; i < nf; ++i) {
-    double v[3];
-    field_eval(F[i], x, y, z, v);
-    s0 += v[0]; s1 += v[1]; s2 += v[2];
-  }
-  B[0] = s0; B[1] = s1; B[2] = s2;
-}
-
-// ---------------------------------------------------------------- compact undulator / EPU
-// The few numbers an ideal undulator or EPU needs, small enough to stay in
-// registers for the whole integration.  The region tests become selects:
-// outside the magnet the amplitude is multiplied by zero, which gives the same
-// sum as the reference's early return.
-struct UndParams {
-  double c0, c1, c2, u0, u1, u2;  // centre, unit vector along the period
-  double taper, half_len, PL, inv_PL, half_PL;
-  double psA, psB;
-  double a0, a1, a2, b0, b1, b2;  // field A, field B (zero for a planar undulator)
-  int epu;
-};
-
-__device__ __forceinline__ double und_scale(const UndParams& f, double D, double PS) {
-  double L2 = f.half_len;
-  bool out = D > L2 + PS || D < -L2 + PS;
-  bool term = D < -L2 + PS + f.PL || D > L2 + PS - f.PL;
-  bool quarter = D < -L2 + PS + f.half_PL || D > L2 + PS - f.half_PL;
-  double sc = term ? (quarter ? 0.25 : 0.75) : 1.0;
-  return out ? 0.0 : sc;
-}
-
-__device__ __forceinline__ void und_eval(const UndParams& f, double x, double y, double z, double B[3]) {
-  double D = (x - f.c0) * f.u0 + (y - f.c1) * f.u1 + (z - f.c2) * f.u2;
-  double tc = 1 + D * f.taper;
-  double scA = und_scale(f, D, f.psA);
-  double sA = sin64(div_by(2.0 * kPi * (D - f.psA), f.PL, f.inv_PL));
-  // terminated poles: (field*scale)*sin*taper; full poles: field*sin*taper (scale 1 is exact)
-  double v0 = f.a0 * scA * sA * tc, v1 = f.a1 * scA * sA * tc, v2 = f.a2 * scA * sA * tc;
-  if (f.epu) {
-    double scB = und_scale(f, D, f.psB);
-    double sB = sin64(div_by(2.0 * kPi * (D - f.psB), f.PL, f.inv_PL));
-    v0 = v0 + f.b0 * scB * sB * tc;
-    v1 = v1 + f.b1 * scB * sB * tc;
-    v2 = v2 + f.b2 * scB * sB * tc;
-  }
-  B[0] = 0.0 + v0;
-  B[1] = 0.0 + v1;
-  B[2] = 0.0 + v2;
-}
-
-// ---------------------------------------------------------------- RK4
-struct RkCtx {
-  const FieldDev* F;
-  int nf;
-  double qomg;
-  int err;
-};
-
-// ONE: exactly one undulator/EPU field, passed by value in f0 so it lives in registers.
-template <bool ONE>
-__device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const double* x, double* d) {
-  // |beta| >= 1 test of src/OSCARSSR.cc:1260-1264; only its sign is used, so
-  // the product with 1/c^2 stands in for the division without changing it
-  // except within one rounding of |beta| = 1.
-  double OneMinus = 1. - (x[1] * x[1] + x[3] * x[3] + x[5] * x[5]) * (1.0 / (kC * kC));
-  if (OneMinus <= 0) {
-    c.err = 1;
-    return;
-  }
-  double B[3];
-  if (ONE) {
-    und_eval(f0, x[0], x[2], x[4], B);
-  } else {
-    get_b(c.F, c.nf, x[0], x[2], x[4], B);
-  }
-  d[0] = x[1];
-  d[1] = c.qomg * (-x[5] * B[1] + x[3] * B[2]);
-  d[2] = x[3];
-  d[3] = c.qomg * (x[5] * B[0] - x[1] * B[2]);
-  d[4] = x[5];
-  d[5] = c.qomg * (x[1] * B[1] - x[3] * B[0]);
-}
-
-// Classic RK4 step; returns false when |beta| >= 1 was met (the reference then
-// halves the step recursively; that branch never triggers for physical beams,
-// so the kernel reports it as an error instead of recursing).
-template <bool ONE>
-__device__ __forceinline__ bool rk4(RkCtx& c, const UndParams& f0, double* y, const double* dydx, double h) {
-  double dym[6], dyt[6], yt[6];
-  double hh = h * 0.5;
-  double h6 = h / 6.0;
-  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
-  derivs<ONE>(c, f0, yt, dyt);
-  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
-  derivs<ONE>(c, f0, yt, dym);
-  for (int i = 0; i < 6; ++i) {
-    yt[i] = y[i] + h * dym[i];
-    dym[i] += dyt[i];
-  }
-  derivs<ONE>(c, f0, yt, dyt);
-  double yo[6];
-  for (int i = 0; i < 6; ++i) yo[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
-  bool bad = 1. - (yo[1] * yo[1] + yo[3] * yo[3] + yo[5] * yo[5]) * (1.0 / (kC * kC)) <= 0;
-  if (c.err || bad) return false;
-  for (int i = 0; i < 6; ++i) y[i] = yo[i];
-  return true;
-}
 
 // The knot is stored with velocity and acceleration in SI units; knot_finish_kernel
 // divides them by c afterwards (in parallel), which keeps six divisions off the
@@ -377,31 +199,6 @@ struct LevelJob {
   long npts;            // 2^L
   double t_first, spacing;
 };
-
-// include/TOMATH.h:92-126 GetValue.  The interval is found from a guess on the
-// uniform knot grid and then moved to the reference bisection's answer
-// (largest klo <= n-2 with t[klo] <= x), which is exact for increasing knots.
-__device__ __forceinline__ void spline_value(const double* __restrict__ X, const double* __restrict__ Y,
-                                             const double* __restrict__ YPP, long n, double x, double* out) {
-  double h0 = (X[n - 1] - X[0]) / (double) (n - 1);
-  long k = (long) ((x - X[0]) / h0);
-  if (k < 0) k = 0;
-  if (k > n - 2) k = n - 2;
-  while (k < n - 2 && X[k + 1] <= x) ++k;
-  while (k > 0 && X[k] > x) --k;
-  long klo = k, khi = k + 1;
-  double h = X[khi] - X[klo];
-  double a = (X[khi] - x) / h;
-  double b = (x - X[klo]) / h;
-  double ca = a * a * a - a;
-  double cb = b * b * b - b;
-  const double* Y0 = Y + 9 * klo;
-  const double* Y1 = Y + 9 * khi;
-  const double* P0 = YPP + 9 * klo;
-  const double* P1 = YPP + 9 * khi;
-#pragma unroll
-  for (int c = 0; c < 9; ++c) out[c] = Y0[c] * a + Y1[c] * b + (P0[c] * ca + P1[c] * cb) * (h * h) / 6.;
-}
 
 __global__ void __launch_bounds__(256) level_kernel(LevelJob J) {
   long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
