@@ -446,8 +446,8 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
       for (int q = 0; q < 6; ++q) M[r][q] = Mn[r][q];
     // next state, as the forward computes it
     double d[6];
-    derivs<false>(rc, none, y, d);
-    rk4<false>(rc, none, y, d, A.dt);
+    derivs<kFieldsGeneric>(rc, none, y, d);
+    rk4<kFieldsGeneric>(rc, none, y, d, A.dt);
   }
   double* o = A.segmap + 42 * s;
   for (int r = 0; r < 6; ++r)
@@ -489,8 +489,8 @@ __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
     for (int r = 0; r < 6; ++r) A.ybuf[6 * i + r] = y[r];
     if (i + 1 < i1) {
       double d[6];
-      derivs<false>(rc, none, y, d);
-      rk4<false>(rc, none, y, d, A.dt);
+      derivs<kFieldsGeneric>(rc, none, y, d);
+      rk4<kFieldsGeneric>(rc, none, y, d, A.dt);
     }
   }
   double lam[6];

@@ -142,9 +142,46 @@ struct RkCtx {
   int err;
 };
 
-// ONE: exactly one undulator/EPU field, passed by value in f0 so it lives in registers.
-template <bool ONE>
+// Field-model variants of the equations of motion:
+//   M = kFieldsGeneric  fields read from F (any number, any type)
+//   M = kOneUndulator   exactly one undulator/EPU, passed by value in f0 (registers)
+//   M = kAxisUndX/Y/Z   one planar undulator with its period along z, no taper and
+//                       a field along x, y or z only.  The general formula then
+//                       reduces exactly: D = (z - c_z) (the other terms of the dot
+//                       product are zeros), the taper factor is 1, and the two zero
+//                       field components contribute zeros to v x B.  Only the sign
+//                       of an exactly-zero result can differ, which never changes a
+//                       nonzero value downstream.  The |beta| >= 1 test is left to
+//                       the end of the step (rk4), where it is checked on the result.
+constexpr int kFieldsGeneric = 0, kOneUndulator = 1, kAxisUndX = 2, kAxisUndY = 3, kAxisUndZ = 4;
+
+template <int M>
 __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const double* x, double* d) {
+  if (M >= kAxisUndX) {
+    constexpr int K = M - kAxisUndX;
+    double D = x[4] - f0.c2;
+    double sc = und_scale(f0, D, f0.psA);
+    double s = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
+    double a = K == 0 ? f0.a0 : (K == 1 ? f0.a1 : f0.a2);
+    double v = a * sc * s;
+    d[0] = x[1];
+    d[2] = x[3];
+    d[4] = x[5];
+    if (K == 0) {
+      d[1] = 0.0;
+      d[3] = c.qomg * (x[5] * v);
+      d[5] = c.qomg * (-(x[3] * v));
+    } else if (K == 1) {
+      d[1] = c.qomg * (-(x[5] * v));
+      d[3] = 0.0;
+      d[5] = c.qomg * (x[1] * v);
+    } else {
+      d[1] = c.qomg * (x[3] * v);
+      d[3] = c.qomg * (-(x[1] * v));
+      d[5] = 0.0;
+    }
+    return;
+  }
   // |beta| >= 1 test of src/OSCARSSR.cc:1260-1264; only its sign is used, so
   // the product with 1/c^2 stands in for the division without changing it
   // except within one rounding of |beta| = 1.
@@ -154,7 +191,7 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
     return;
   }
   double B[3];
-  if (ONE) {
+  if (M == kOneUndulator) {
     und_eval(f0, x[0], x[2], x[4], B);
   } else {
     get_b(c.F, c.nf, x[0], x[2], x[4], B);
@@ -170,20 +207,20 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
 // Classic RK4 step; returns false when |beta| >= 1 was met (the reference then
 // halves the step recursively; that branch never triggers for physical beams,
 // so the kernel reports it as an error instead of recursing).
-template <bool ONE>
+template <int M>
 __device__ __forceinline__ bool rk4(RkCtx& c, const UndParams& f0, double* y, const double* dydx, double h) {
   double dym[6], dyt[6], yt[6];
   double hh = h * 0.5;
   double h6 = h / 6.0;
   for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
-  derivs<ONE>(c, f0, yt, dyt);
+  derivs<M>(c, f0, yt, dyt);
   for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
-  derivs<ONE>(c, f0, yt, dym);
+  derivs<M>(c, f0, yt, dym);
   for (int i = 0; i < 6; ++i) {
     yt[i] = y[i] + h * dym[i];
     dym[i] += dyt[i];
   }
-  derivs<ONE>(c, f0, yt, dyt);
+  derivs<M>(c, f0, yt, dyt);
   double yo[6];
   for (int i = 0; i < 6; ++i) yo[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
   bool bad = 1. - (yo[1] * yo[1] + yo[3] * yo[3] + yo[5] * yo[5]) * (1.0 / (kC * kC)) <= 0;

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p
 // reversals.
 // ONE: a single undulator/EPU arrives by value in the kernel arguments (read
 // through the scalar cache into registers); otherwise fields are read from F.
-template <bool ONE>
+template <int M>
 __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const FieldDev* __restrict__ F, int nfields,
                                                  TrajDesc* __restrict__ desc, int np, double* __restrict__ t,
                                                  double* __restrict__ p) {
@@ -63,13 +63,13 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const Field
   x[2] = D.x0[1]; x[3] = D.b0[1] * kC;
   x[4] = D.x0[2]; x[5] = D.b0[2] * kC;
   int status = 0;
-  derivs<ONE>(c, f0, x, dxdt);
+  derivs<M>(c, f0, x, dxdt);
   for (long i = 0; i < D.nf; ++i) {
     double tt = D.t0_over_c + D.dt * (double) i;
-    if (i == 0) derivs<ONE>(c, f0, x, dxdt);
+    if (i == 0) derivs<M>(c, f0, x, dxdt);
     store_knot(tk, pk, D.nb + i, tt, x, dxdt);
-    derivs<ONE>(c, f0, x, dxdt);
-    if (!rk4<ONE>(c, f0, x, dxdt, D.dt)) { status = 1; break; }
+    derivs<M>(c, f0, x, dxdt);
+    if (!rk4<M>(c, f0, x, dxdt, D.dt)) { status = 1; break; }
   }
   x[0] = D.x0[0]; x[1] = D.b0[0] * kC;
   x[2] = D.x0[1]; x[3] = D.b0[1] * kC;
@@ -77,8 +77,8 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const Field
   double dtr = -D.dt;
   for (long i = 0; i < D.nb && status == 0; ++i) {
     double t2 = D.t0_over_c + dtr * (double) (i + 1);
-    derivs<ONE>(c, f0, x, dxdt);
-    if (!rk4<ONE>(c, f0, x, dxdt, dtr)) { status = 1; break; }
+    derivs<M>(c, f0, x, dxdt);
+    if (!rk4<M>(c, f0, x, dxdt, dtr)) { status = 1; break; }
     store_knot(tk, pk, D.nb - 1 - i, t2, x, dxdt);
   }
   if (c.err) status = 1;
@@ -257,9 +257,26 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     u.a0 = f.field[0]; u.a1 = f.field[1]; u.a2 = f.field[2];
     u.b0 = f.fieldB[0]; u.b1 = f.fieldB[1]; u.b2 = f.fieldB[2];
     u.epu = f.type == om::kEPU;
-    hipLaunchKernelGGL(om::rk4_kernel<true>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc, np, t, p);
+    // planar undulator along z, untapered, field along one axis: the reduced equations
+    int nz = (f.field[0] != 0) + (f.field[1] != 0) + (f.field[2] != 0);
+    bool axis_z = f.unit[0] == 0 && f.unit[1] == 0 && f.unit[2] == 1;
+    if (f.type == om::kUndulator && axis_z && f.taper == 0 && nz == 1) {
+      if (f.field[0] != 0)
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndX>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc, np,
+                           t, p);
+      else if (f.field[1] != 0)
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndY>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc, np,
+                           t, p);
+      else
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc, np,
+                           t, p);
+    } else {
+      hipLaunchKernelGGL(om::rk4_kernel<om::kOneUndulator>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc,
+                         np, t, p);
+    }
   } else {
-    hipLaunchKernelGGL(om::rk4_kernel<false>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc, np, t, p);
+    hipLaunchKernelGGL(om::rk4_kernel<om::kFieldsGeneric>, dim3(blocks), dim3(threads), 0, s, u, F, nfields, desc, np,
+                       t, p);
   }
   return hipGetLastError();
 }
