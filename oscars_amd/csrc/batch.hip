@@ -84,17 +84,27 @@ struct BatchArgs {
   unsigned long long* evals;  // sum over lanes of 2^(L+1)-1 (as integer)
 };
 
+// Lane 0's value as a wave-uniform scalar (two 32-bit readfirstlane moves).
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  unsigned lo = (unsigned) __builtin_amdgcn_readfirstlane((int) (unsigned) v);
+  unsigned hi = (unsigned) __builtin_amdgcn_readfirstlane((int) (unsigned) (v >> 32));
+  return ((unsigned long long) hi << 32) | lo;
+}
+
 template <bool MIXED>
 __global__ void __launch_bounds__(64) lw_batch_kernel(BatchArgs A) {
   __shared__ double s_tau[64];
   __shared__ double s_v[3][64];
   int lane = threadIdx.x;
   while (true) {
-    unsigned long long it = 0;
-    if (lane == 0) it = atomicAdd(A.counter, 1ULL);
-    it = __shfl(it, 0);
-    if ((long) it >= A.total_items) break;
-    long item = (long) it;
+    // Next work unit.  Every lane executes the atomic (lane 0 adds 1, the others
+    // 0) and the index is taken from lane 0 with readfirstlane, so it is a
+    // scalar and the loop exit is a uniform branch.  No lane-conditional block
+    // surrounds a cross-lane operation, which keeps the compiler from splitting
+    // the wave between two copies of the loop.
+    unsigned long long mine = atomicAdd(A.counter, lane == 0 ? 1ULL : 0ULL);
+    long item = (long) uniform_u64(mine);
+    if (item >= A.total_items) break;
     // particle of this work unit: last p with item_prefix[p] <= item
     int lo = 0, hi = A.np;
     while (hi - lo > 1) {
@@ -195,13 +205,12 @@ __global__ void __launch_bounds__(64) lw_batch_kernel(BatchArgs A) {
       atomicAdd(&R.out[io * R.n_e + ie], flux);
     }
     // bookkeeping: lanes that stopped without converging, and evaluations done
+    // (all lanes take part; only lane 0 adds a nonzero amount)
     unsigned long long nc = __ballot(valid && !conv);
     unsigned long long ev = valid ? (unsigned long long) ((2L << lev) - 1) : 0ULL;
     for (int off = 32; off > 0; off >>= 1) ev += __shfl_down(ev, off);
-    if (lane == 0) {
-      if (nc) atomicAdd(A.n_unconverged, (unsigned long long) __popcll(nc));
-      atomicAdd(A.evals, ev);
-    }
+    atomicAdd(A.n_unconverged, lane == 0 ? (unsigned long long) __popcll(nc) : 0ULL);
+    atomicAdd(A.evals, lane == 0 ? ev : 0ULL);
   }
 }
 
