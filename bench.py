@@ -220,6 +220,88 @@ def batched_bench(args):
     print(json.dumps(line, default=float))
 
 
+def adjoint_bench(dev, steps=2):
+    """Config: gradient of sum (F - F_ideal)^2 over a 51 x 51 grid (+-5 mm at 30 m) at
+    E1 - 1.5, E1, 3 E1 - 4.5, 3 E1 (E1 = 152.11 eV) with respect to the 4001 samples of
+    a By(z) map over +-1 m (planar undulator 1.0 T / 49 mm / 31 periods with 0.5 %
+    per-pole amplitude errors and 20 um pole jitter, seed 5) and the beam angles,
+    through the LW sum and the 20001-point RK4 trajectory.  The level is frozen at
+    the highest level the error-free map converges at (precision 0.01)."""
+    from oscars_amd import _lib
+    from oscars_amd import adjoint as A
+    from oscars_amd import geometry as G
+    n, z0, dz = 4001, -1.0, 5e-4
+    err = A.undulator_pole_map(n, z0, dz, sigma_amp=0.005, sigma_z=20e-6, seed=5)
+    ideal = A.undulator_pole_map(n, z0, dz)
+    obs = rect(51, 0.01)
+    e1 = 152.11
+    energies = np.array([e1 - 1.5, e1, 3 * e1 - 4.5, 3 * e1])
+    f, keep = A.grid_field([0, 0, z0], [0, 0, dz], [1, 1, n], ideal)
+    p0 = G.ideal_particle("electron", 3.0, [0, 0, -1], [0, 0, 1], 0.0, 0.5)
+    from oscars_amd import multiparticle as M
+    dev.trajectories([f], M.om_particles([p0], [(0.0, 2.0, 20001)]))
+    _, lev, _, _, _ = dev.flux(0, obs, energies, [0.0] * 6, dev.conv(mode="tiled"))
+    L = int(lev.max())
+    prob = A.FieldMapProblem(dev, [0, 0, z0], [0, 0, dz], [1, 1, n], 3.0, [0, 0, -1], 0.0, 2.0, 20001, obs, energies, L)
+    f_ideal = prob.forward(ideal)
+    tf = tb = 0.0
+    for k in range(steps + 1):
+        t0 = time.perf_counter()
+        F = prob.forward(err)
+        t1 = time.perf_counter()
+        gf, ga, gx = prob.backward(2.0 * (F - f_ideal))
+        t2 = time.perf_counter()
+        if k > 0:  # first pass is warm-up
+            tf += t1 - t0
+            tb += t2 - t1
+    tf /= steps
+    tb /= steps
+    evals = float(len(obs) * len(energies) * ((2 << L) - 1))
+    return {"workload": "51x51 at 30 m x 4 energies, 4001-sample By map, 20001-point RK4, level %d" % L,
+            "evals_per_pass": evals, "forward_ms": 1e3 * tf, "backward_ms": 1e3 * tb,
+            "backward_kernel_ms": prob.last_stats["backward"]["kernel_ms"],
+            "forward_evals_per_s": evals / tf, "adjoint_evals_per_s": evals / tb,
+            "fwd_plus_adjoint_evals_per_s": 2 * evals / (tf + tb),
+            "grad_norm_field": float(np.linalg.norm(gf)), "grad_angles": [float(v) for v in ga],
+            "fd_check": "tests/test_gpu_adjoint.py (long-double central differences, 1e-4)"}
+
+
+def batched_share(args, particles):
+    """The multi-particle workload at `particles` electrons (the per-GPU share of the
+    2048-electron job on an 8-GPU node at 256), mixed vs FP64 tiled."""
+    import torch
+    from oscars_amd import beam as B
+    from oscars_amd import multiparticle as M
+    from oscars_amd.sr import sr
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    parts = B.sample_particles(M.config_beam(), 2048, rng=rng)
+    windows = M.uneven_windows(parts, rng)
+    reqs = M.random_requests(16, rng)
+    parts, windows = parts[:particles], windows[:particles]
+    req_of, weights = M.assign_round_robin(particles, len(reqs))
+    s = sr()
+    s.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.049], nperiods=31)
+    dev = s._device()
+    out = {}
+    res = {}
+    for mode in ("mixed", "tiled"):
+        bf = M.BatchedFlux(dev, s._fields, parts, windows, reqs, req_of, weights, mode=mode)
+        best, tt = bf.tune(candidates=(2, 4, 8, 16), sample_items=20000)
+        bf.run(recompute_trajectories=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs, st = bf.run(recompute_trajectories=True)
+        dt = time.perf_counter() - t0
+        res[mode] = outs
+        out[mode] = {"evals_per_s": st["evals"] / dt, "ms_per_step": 1e3 * dt, "evals": st["evals"],
+                     "unconverged": st["n_unconverged"], "waves_per_cu": best}
+        print("# batched %d %s: %.1f ms, %.4g evals/s" % (particles, mode, 1e3 * dt, st["evals"] / dt),
+              file=sys.stderr, flush=True)
+    out["rel_err_mixed_vs_fp64"] = max(rel_errors(a, b)["max_floored_1e3"] for a, b in zip(res["mixed"], res["tiled"]))
+    out["particles"] = particles
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +320,9 @@ def main():
     ap.add_argument("--particles", type=int, default=2048)
     ap.add_argument("--plain-particles", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--batch-share", type=int, default=256,
+                    help="particles of the multi-particle share reported beside the headline (0: skip)")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -339,6 +424,18 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if not args.only_sweep:
+        if not args.no_adjoint:
+            try:
+                line["adjoint"] = adjoint_bench(s._device())
+                print("# adjoint: %s" % line["adjoint"], file=sys.stderr, flush=True)
+            except Exception as ex:
+                line["adjoint"] = {"error": str(ex)}
+        if args.batch_share > 0:
+            try:
+                line["batched"] = batched_share(args, args.batch_share)
+            except Exception as ex:
+                line["batched"] = {"error": str(ex)}
     line.update(extra)
     print(json.dumps(line, default=float))
 

@@ -138,8 +138,10 @@ int om_power_density(om_context* ctx, int particle, const double* obs_xyz, long 
  * other.  Particle i of the resident trajectory set (om_trajectories) adds
  * weight[i] * flux to the output grid of request req_of[i]; requests may have
  * different observation grids and energy lists, particles different
- * trajectory lengths.  Levels 0..max_level of every particle are materialised;
- * points still unconverged there are counted in stats->n_unconverged. */
+ * trajectory lengths.  Levels 0..opts->max_level of every particle are stored
+ * as sample rows; deeper levels are interpolated in the kernel, so every point
+ * follows the conv's level rule; points unconverged at its last level are
+ * counted in stats->n_unconverged. */
 typedef struct {
   long n_obs;
   const double* obs_xyz;     /* 3 per point */
@@ -149,7 +151,8 @@ typedef struct {
 
 typedef struct {
   int waves_per_cu; /* persistent grid = CUs x this; 0: the tuned value (8 before any tuning) */
-  int max_level;    /* highest level materialised and summed; 0: 14 */
+  int max_level;    /* highest level held as stored sample rows (0: 14); deeper levels, up to
+                       the conv's max_level rule, are interpolated inside the sum kernel */
 } om_batch_opts;
 
 int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int* req_of, const double* weight,

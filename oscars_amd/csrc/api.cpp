@@ -111,6 +111,12 @@ struct BatchArgs {
   const double* rows;
   long stride;
   int Lb;
+  int Lmax;
+  int stop_mem;
+  const double* t;
+  const double* p;
+  const double* ypp;
+  const TrajDesc* desc;
   const double* tspan;
   const int* req_of;
   const double* weight;
@@ -236,7 +242,7 @@ struct om_context {
   // batched multi-particle state
   struct Batch {
     bool ready = false;
-    int np = 0, n_req = 0, Lb = 0, stop_mem = 0, mixed = 0, groups = 0;
+    int np = 0, n_req = 0, Lb = 0, Lmax = 0, stop_mem = 0, mixed = 0, groups = 0;
     long stride = 0, total_items = 0, total_out = 0;
     std::vector<long> out_off, out_n;
     DevBuf<double> rows, tspan, obs, omega, out, weight;
@@ -914,9 +920,10 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     int stop_mem, stop_ext;
     level_stops(*conv, stop_mem, stop_ext);
     int cap = (opts && opts->max_level > 0) ? opts->max_level : 14;
-    if (cap > 20) throw std::out_of_range("batch max_level above 20 is not supported");
+    if (cap > 20) throw std::out_of_range("batch rows above level 20 are not supported");
+    if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
     B.Lb = stop_ext < cap ? stop_ext : cap;
-    if (B.Lb < 0) throw std::out_of_range("max_level leaves no level to sum");
+    B.Lmax = stop_ext;
     B.stop_mem = stop_mem;
     B.stride = (2L << B.Lb) - 1;
     B.np = np;
@@ -974,6 +981,12 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     A.rows = B.rows.p;
     A.stride = B.stride;
     A.Lb = B.Lb;
+    A.Lmax = B.Lmax;
+    A.stop_mem = B.stop_mem;
+    A.t = ctx->d_t.p;
+    A.p = ctx->d_p.p;
+    A.ypp = ctx->d_ypp.p;
+    A.desc = ctx->d_desc.p;
     A.tspan = B.tspan.p;
     A.req_of = B.req_of.p;
     A.weight = B.weight.p;
@@ -1059,7 +1072,7 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
       stats->evals = (double) cnt[2];
       stats->kernel_ms = ms;
       stats->total_ms = wall.ms();
-      stats->max_level = B.Lb;
+      stats->max_level = B.Lmax;
       stats->n_unconverged = (long) cnt[1];
     }
   });

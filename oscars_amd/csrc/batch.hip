@@ -17,6 +17,9 @@
 //
 // Level samples of all particles live in one block: particle p, level L starts
 // at row p*stride + 2^L - 1 (stride = 2^(Lcap+1) - 1 rows of 10 doubles).
+// Deeper levels (rare: slowly converging points) are interpolated from the
+// spline inside the sum kernel with the same expressions, so a batch follows
+// the same level rule as the single-particle path up to max_level.
 #include "lw_common.h"
 #include "rk4_common.h"
 
@@ -67,7 +70,13 @@ struct ReqDev {
 struct BatchArgs {
   const double* rows;
   long stride;
-  int Lb;               // highest level summed
+  int Lb;               // highest level held in rows (materialised by level_batch_kernel)
+  int Lmax;             // highest level summed; levels above Lb are interpolated in the kernel
+  int stop_mem;         // levels above it use the extended spacing
+  const double* t;      // knots, spline second derivatives and descriptors (levels above Lb)
+  const double* p;
+  const double* ypp;
+  const TrajDesc* desc;
   const double* tspan;
   const int* req_of;
   const double* weight;
@@ -123,21 +132,36 @@ __global__ void __launch_bounds__(64) lw_batch_kernel(BatchArgs A) {
     double negw = -R.omega[iec];
     double T = A.tspan[2 * ip + 1] - A.tspan[2 * ip];
     const double* prow = A.rows + 10 * (long) ip * A.stride;
+    const double ts = A.tspan[2 * ip], te = A.tspan[2 * ip + 1];
+    const TrajDesc D = A.desc[ip];
     double sum[6] = {0, 0, 0, 0, 0, 0};
     double lastmag = -1, prec = -1;
     bool active = valid;
     bool conv = false;
     int lev = 0;
-    for (int L = 0; L <= A.Lb; ++L) {
+    for (int L = 0; L <= A.Lmax; ++L) {
       if (!__any(active)) break;
       const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
       long ns = 1L << L;
+      // sample times of level L (the expressions of level_batch_kernel)
+      double p2L = (double) (1L << L);
+      double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
+      double tfirst = ts + (te - ts) / (2.0 * p2L);
+      bool stored = L <= A.Lb;
       double ps[6] = {0, 0, 0, 0, 0, 0};
       double maxd = 0, lastphase = 0;
       for (long base = 0; base < ns; base += 64) {
         long si = base + lane;
         if (si < ns) {
-          Amp a = amplitude(obs, rows + 10 * si);
+          double srow[10];
+          const double* sp = rows + 10 * si;
+          if (!stored) {
+            srow[0] = tfirst + spacing * (double) si;
+            spline_value(A.t + D.knot_offset, A.p + 9 * D.knot_offset, A.ypp + 9 * D.knot_offset, D.n, srow[0],
+                         srow + 1);
+            sp = srow;
+          }
+          Amp a = amplitude(obs, sp);
           s_tau[lane] = a.tau;
           s_v[0][lane] = a.V.x;
           s_v[1][lane] = a.V.y;

@@ -106,13 +106,17 @@ def test_adjoint_matches_long_double_finite_differences(gradients):
     def loss(s, ang):
         return float(np.sum((fdref_flux(s, ang) - f_ideal) ** 2))
 
-    picks = [(2000, 1), (1500, 1), (2612, 1), (420, 1), (3560, 1), (2200, 0), (1777, 2)]
-    for i, c in picks:
+    # Bz couples to the motion only through the small vx: its derivatives are
+    # ~1e-4 of the others, so its difference quotient uses a 30x larger step to
+    # stay above the same absolute noise (truncation remains negligible).
+    picks = [(2000, 1, 3e-4), (1500, 1, 3e-4), (2612, 1, 3e-4), (420, 1, 3e-4), (3560, 1, 3e-4), (2200, 0, 3e-4),
+             (1777, 2, 1e-2)]
+    for i, c, step in picks:
         def fun(d, i=i, c=c):
             s = err.copy()
             s[i, c] += d
             return loss(s, (0.0, 0.0))
-        fd = fd4(fun, 3e-4)
+        fd = fd4(fun, step)
         rel = abs(gf[i, c] - fd) / abs(fd)
         print("sample %d comp %d: adjoint %.10e fd %.10e rel %.2e" % (i, c, gf[i, c], fd, rel))
         assert rel < 1e-4
