@@ -142,7 +142,8 @@ hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int
                          int np, double* t, double* p, hipStream_t s);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
 hipError_t om_launch_spline(const om::TrajDesc* desc, int np, const double* t, const double* p, double* ypp,
-                            double* u, double* sig, double* sm1, long total, hipStream_t s);
+                            double* u, double* sig, double* sm1, double* pv, double* ppv, double* u2, long total,
+                            hipStream_t s);
 hipError_t om_launch_level(const om::LevelJob* J, hipStream_t s);
 hipError_t om_launch_beta_diff(const double* rows, long npts, double* out, hipStream_t s);
 hipError_t om_launch_lw_point(const om::LwArgs* A, int mixed, int split, hipStream_t s);
@@ -155,8 +156,8 @@ hipError_t om_launch_adj_ckpt(const double* p_raw, long K, long nseg, double* ck
 hipError_t om_launch_adj_lw(const om::AdjLwArgs* A, int nchunks, hipStream_t s);
 hipError_t om_launch_adj_chunk_sum(const double* partial, long ns, int nchunks, double* out, hipStream_t s);
 hipError_t om_launch_adj_spline(const double* rows, long ns, const double* sb, int L, const double* X, long n,
-                                int* klo, double* yb, double* yppb, double* Pw, double* ppw, double* ub,
-                                hipStream_t s);
+                                int* klo, double* yb, double* yppb, double* Pw, double* ppw, double* sgw, double* tb,
+                                double* cc, hipStream_t s);
 hipError_t om_launch_adj_rk4(const om::AdjRkArgs* A, hipStream_t s);
 hipError_t om_launch_adj_fill(double* p, long n, hipStream_t s);
 }
@@ -225,7 +226,7 @@ struct om_context {
   int np = 0;
   std::vector<om::TrajDesc> desc;
   DevBuf<om::TrajDesc> d_desc;
-  DevBuf<double> d_t, d_p, d_ypp, d_u, d_sig, d_sm1;
+  DevBuf<double> d_t, d_p, d_ypp, d_u, d_sig, d_sm1, d_pv, d_ppv, d_u2;
   std::vector<double> tstart, tstop;
   std::vector<om_particle> parts;
   std::vector<LevelCache> levels;
@@ -276,11 +277,12 @@ struct om_context {
     long n_obs = 0, n_e = 0;
     double C0 = 0, C2 = 0, dt_L = 0;
     std::vector<double> E;  // 6 per output: C0 dt_L S (no polarization projection)
-    DevBuf<double> ckpt, rows, obs, omega, g, partial, sb, yb, yppb, Pw, ppw, ub, segmap, lam, ybuf, gbar;
+    DevBuf<double> ckpt, rows, obs, omega, g, partial, sb, yb, yppb, Pw, ppw, sgw, ub, cc, segmap, lam, ybuf, gbar;
     DevBuf<int> klo;
     void release() {
       ckpt.release(); rows.release(); obs.release(); omega.release(); g.release(); partial.release();
-      sb.release(); yb.release(); yppb.release(); Pw.release(); ppw.release(); ub.release();
+      sb.release(); yb.release(); yppb.release(); Pw.release(); ppw.release(); sgw.release(); ub.release();
+      cc.release();
       segmap.release(); lam.release(); ybuf.release(); gbar.release(); klo.release();
       ready = false;
     }
@@ -302,7 +304,7 @@ struct om_context {
     free_levels();
     for (auto& g : grid_bufs) g.release();
     d_desc.release(); d_t.release(); d_p.release(); d_ypp.release(); d_u.release(); d_fields.release();
-    d_sig.release(); d_sm1.release();
+    d_sig.release(); d_sm1.release(); d_pv.release(); d_ppv.release(); d_u2.release();
     d_obs.release(); d_omega.release(); d_sum.release(); d_comp.release(); d_lastmag.release();
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
@@ -591,7 +593,8 @@ void launch_trajectories(om_context* ctx, int np) {
   }
   check(om_launch_knot_finish(ctx->d_p.p, ctx->total_knots, ctx->stream), "knot finish");
   check(om_launch_spline(ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->d_ypp.p, ctx->d_u.p, ctx->d_sig.p,
-                         ctx->d_sm1.p, ctx->total_knots, ctx->stream), "spline");
+                         ctx->d_sm1.p, ctx->d_pv.p, ctx->d_ppv.p, ctx->d_u2.p, ctx->total_knots, ctx->stream),
+        "spline");
 }
 
 int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
@@ -782,6 +785,9 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->d_u.ensure(9 * total);
     ctx->d_sig.ensure(total);
     ctx->d_sm1.ensure(total);
+    ctx->d_pv.ensure(total);
+    ctx->d_ppv.ensure(total);
+    ctx->d_u2.ensure(9 * total);
     upload(ctx, ctx->d_desc, ctx->desc.data(), np);
     launch_trajectories(ctx, np);
     check(hipMemcpyAsync(ctx->desc.data(), ctx->d_desc.p, np * sizeof(om::TrajDesc), hipMemcpyDeviceToHost,
@@ -1233,10 +1239,12 @@ int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad
     A.yppb.ensure(9 * n);
     A.Pw.ensure(n);
     A.ppw.ensure(n);
+    A.sgw.ensure(n);
     A.ub.ensure(9 * n);
+    A.cc.ensure(9 * n);
     const double* X = ctx->d_t.p + D.knot_offset;
-    check(om_launch_adj_spline(A.rows.p, ns, A.sb.p, A.L, X, n, A.klo.p, A.yb.p, A.yppb.p, A.Pw.p, A.ppw.p, A.ub.p,
-                               ctx->stream), "adjoint spline");
+    check(om_launch_adj_spline(A.rows.p, ns, A.sb.p, A.L, X, n, A.klo.p, A.yb.p, A.yppb.p, A.Pw.p, A.ppw.p, A.sgw.p,
+                               A.ub.p, A.cc.p, ctx->stream), "adjoint spline");
     // 3. RK4 with checkpoints: dLoss/d(initial state), dLoss/d(grid samples)
     A.segmap.ensure(42 * A.nseg);
     A.lam.ensure(6 * (A.nseg + 1));

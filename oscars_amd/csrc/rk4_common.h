@@ -135,11 +135,20 @@ __device__ __forceinline__ void und_eval(const UndParams& f, double x, double y,
 }
 
 // ---------------------------------------------------------------- RK4
+// A single 1-D grid along z without rotation, with its samples staged in LDS
+// (kGridZ variant of the equations of motion).
+struct GridZ {
+  const double* data;  // 3 per sample (LDS when staged)
+  long n;
+  double start, stop, step, inv_step, trans;
+};
+
 struct RkCtx {
   const FieldDev* F;
   int nf;
   double qomg;
   int err;
+  GridZ g;
 };
 
 // Field-model variants of the equations of motion:
@@ -153,7 +162,24 @@ struct RkCtx {
 //                       of an exactly-zero result can differ, which never changes a
 //                       nonzero value downstream.  The |beta| >= 1 test is left to
 //                       the end of the step (rk4), where it is checked on the result.
-constexpr int kFieldsGeneric = 0, kOneUndulator = 1, kAxisUndX = 2, kAxisUndY = 3, kAxisUndZ = 4;
+//   M = kGridZ          one 1-D grid along z without rotation, samples in LDS: the
+//                       grid branch of field_eval with the identity rotation and the
+//                       x/y range tests (n = 1 there) removed, which changes no value.
+constexpr int kFieldsGeneric = 0, kOneUndulator = 1, kAxisUndX = 2, kAxisUndY = 3, kAxisUndZ = 4, kGridZ = 5;
+
+__device__ __forceinline__ void grid_z_eval(const GridZ& g, double z, double B[3]) {
+  double coord = z - g.trans;
+  if (coord <= g.start || coord >= g.stop) {
+    B[0] = B[1] = B[2] = 0;
+    return;
+  }
+  long n0 = (long) (size_t) div_by(coord - g.start, g.step, g.inv_step);
+  if (n0 > g.n - 2) n0 = g.n - 2;
+  double dd = (coord - g.start) - (double) n0 * g.step;
+  const double* lo = g.data + 3 * n0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) B[k] = lo[k] + div_by((lo[3 + k] - lo[k]) * dd, g.step, g.inv_step);
+}
 
 template <int M>
 __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const double* x, double* d) {
@@ -191,7 +217,9 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
     return;
   }
   double B[3];
-  if (M == kOneUndulator) {
+  if (M == kGridZ) {
+    grid_z_eval(c.g, x[4], B);
+  } else if (M == kOneUndulator) {
     und_eval(f0, x[0], x[2], x[4], B);
   } else {
     get_b(c.F, c.nf, x[0], x[2], x[4], B);
