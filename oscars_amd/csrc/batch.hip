@@ -93,6 +93,16 @@ struct BatchArgs {
   unsigned long long* evals;  // sum over lanes of 2^(L+1)-1 (as integer)
 };
 
+// Amplitude of a sample interpolated from the spline at time tt (levels above
+// the stored rows; rare).
+__device__ __forceinline__ Amp interpolated_amplitude(const BatchArgs& A, const TrajDesc& D, V3 obs,
+                                                                double tt) {
+  double srow[10];
+  srow[0] = tt;
+  spline_value(A.t + D.knot_offset, A.p + 9 * D.knot_offset, A.ypp + 9 * D.knot_offset, D.n, tt, srow + 1);
+  return amplitude(obs, srow);
+}
+
 // Lane 0's value as a wave-uniform scalar (two 32-bit readfirstlane moves).
 __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
   unsigned lo = (unsigned) __builtin_amdgcn_readfirstlane((int) (unsigned) v);
@@ -100,8 +110,10 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return ((unsigned long long) hi << 32) | lo;
 }
 
+// Four waves per SIMD: the register budget is set for the hot stored-row path;
+// the rare interpolation path may spill.
 template <bool MIXED>
-__global__ void __launch_bounds__(64) lw_batch_kernel(BatchArgs A) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw_batch_kernel(BatchArgs A) {
   __shared__ double s_tau[64];
   __shared__ double s_v[3][64];
   int lane = threadIdx.x;
@@ -153,15 +165,12 @@ __global__ void __launch_bounds__(64) lw_batch_kernel(BatchArgs A) {
       for (long base = 0; base < ns; base += 64) {
         long si = base + lane;
         if (si < ns) {
-          double srow[10];
-          const double* sp = rows + 10 * si;
-          if (!stored) {
-            srow[0] = tfirst + spacing * (double) si;
-            spline_value(A.t + D.knot_offset, A.p + 9 * D.knot_offset, A.ypp + 9 * D.knot_offset, D.n, srow[0],
-                         srow + 1);
-            sp = srow;
+          Amp a;
+          if (stored) {
+            a = amplitude(obs, rows + 10 * si);
+          } else {
+            a = interpolated_amplitude(A, D, obs, tfirst + spacing * (double) si);
           }
-          Amp a = amplitude(obs, sp);
           s_tau[lane] = a.tau;
           s_v[0][lane] = a.V.x;
           s_v[1][lane] = a.V.y;
