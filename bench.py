@@ -369,8 +369,16 @@ def main():
         samples = float(sum(np.sum(2.0 ** (t + 1) - 1) for t in tiles))
         flops = FLOPS_TILE_EVAL * sweep_evals + FLOPS_TILE_SAMPLE * samples
     achieved = flops / (sweep_ms * 1e-3) / 1e12
+    # HBM bytes per launch of the sweep kernel, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    # on this workload and scaled by the counters' reading of a plain device copy (profiles/)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "lw_tile_sweep_traffic.json")
+    if os.path.exists(tfile) and args.mode == "mixed":
+        with open(tfile) as f:
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": "lw_tile_kernel<%s>" % ("true" if args.mode == "mixed" else "false")
                 if args.mode != "plain" else "lw_point_kernel<false>",
                 "kernel_ms": round(sweep_ms, 3), "evals_per_launch": sweep_evals,

@@ -140,3 +140,32 @@ def test_power_density_epu(normal):
     e = rel_err(v, ov, 1e-6)
     print("power density normal=%d: max rel err %.3e" % (normal, e))
     assert e < 1e-9
+
+
+@gpu
+def test_grid_field_trajectory():
+    """1-D z grid map (4001 samples, staged in LDS by the RK4 kernel) against the oracle's knots."""
+    from oracle.pyoracle import Oracle
+    from oscars_amd import adjoint as A
+    from oscars_amd.sr import sr
+    samples = A.undulator_pole_map(4001, -1.0, 5e-4, sigma_amp=0.005, sigma_z=20e-6, seed=7)
+    o = Oracle(nthreads=8)
+    o.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    o.add_bfield_grid([0, 0, -1.0], [0, 0, 5e-4], [1, 1, 4001], samples)
+    o.set_ctstartstop(0, 2)
+    o.set_npoints_trajectory(4001)
+    ref = o._traj()
+    s = sr()
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_grid([0, 0, -1.0], [0, 0, 5e-4], [1, 1, 4001], samples)
+    s.set_ctstartstop(0, 2)
+    s.set_npoints_trajectory(4001)
+    t, p, ypp = s.get_trajectory_arrays()
+    print("grid trajectory: zero knots %d of %d; x at end %.6e (oracle %.6e)" %
+          (int(np.sum(np.all(p == 0, axis=1))), len(p), p[-1, 0], ref.p[-1, 0]))
+    scale = np.max(np.abs(ref.p), axis=0)
+    scale[scale == 0] = 1
+    err = np.max(np.abs(p - ref.p) / scale)
+    print("grid trajectory: max channel-relative difference %.3e" % err)
+    assert np.array_equal(t, ref.t)
+    assert err < 1e-12
