@@ -183,7 +183,7 @@ __device__ __forceinline__ void grid_z_eval(const GridZ& g, double z, double B[3
 
 template <int M>
 __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const double* x, double* d) {
-  if (M >= kAxisUndX) {
+  if (M >= kAxisUndX && M <= kAxisUndZ) {
     constexpr int K = M - kAxisUndX;
     double D = x[4] - f0.c2;
     double sc = und_scale(f0, D, f0.psA);
