@@ -47,6 +47,7 @@ struct LwArgs {
   double pol[6];
   int use_pol;
   double precision;
+  const int* list;
 };
 struct PdArgs {
   Levels lv;
@@ -148,6 +149,9 @@ hipError_t om_launch_level(const om::LevelJob* J, hipStream_t s);
 hipError_t om_launch_beta_diff(const double* rows, long npts, double* out, hipStream_t s);
 hipError_t om_launch_lw_point(const om::LwArgs* A, int mixed, int split, hipStream_t s);
 hipError_t om_launch_lw_tile(const om::LwArgs* A, long n_obs, int mixed, int split, hipStream_t s);
+hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, int mixed, int split, hipStream_t s);
+hipError_t om_launch_tile_list(const om::State* st, long n_obs, long n_omega, int La, int* list, int* ntiles,
+                               hipStream_t s);
 hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s);
 hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s);
 hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s);
@@ -234,7 +238,7 @@ struct om_context {
   std::vector<DevBuf<double>> grid_bufs;
   // work buffers
   DevBuf<double> d_obs, d_omega, d_sum, d_comp, d_lastmag, d_prec, d_out, d_outcomp, d_oute;
-  DevBuf<int> d_level, d_conv, d_count, d_outlevel;
+  DevBuf<int> d_level, d_conv, d_count, d_outlevel, d_list;
   DevBuf<unsigned char> d_outconv;
   DevBuf<double> d_bd;
   std::vector<om::FieldDev> fields_host;
@@ -308,6 +312,7 @@ struct om_context {
     d_obs.release(); d_omega.release(); d_sum.release(); d_comp.release(); d_lastmag.release();
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
+    d_list.release();
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) (void) hipStreamDestroy(stream);
@@ -506,6 +511,7 @@ long count_active(om_context* ctx, const om::State& st, long npts) {
 // First launch covers levels up to this one; later launches add one level
 // at a time for the points still running.
 constexpr int kFirstLevels = 13;
+constexpr int kListFrom = 9;  // first level that can stop a point (src/OSCARSSR.cc:2006: L > 8)
 
 void check_particle(om_context* ctx, int ip) {
   if (ip < 0 || ip >= ctx->np) throw std::out_of_range("particle index out of range (no trajectory computed?)");
@@ -633,16 +639,23 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     A.precision = conv->precision;
     float kms = 0;
     int La = 0;
-    int Lb = stop_ext < kFirstLevels ? stop_ext : kFirstLevels;
+    bool tiled = !(conv->mode == OM_MODE_PLAIN || conv->mode == OM_MODE_POINT_MIXED);
+    // Tiled kernels: levels 0..8 (no point can stop before level 9) in one dense
+    // launch, then one launch per level over compacted tiles of active points.
+    int Lb = tiled ? (stop_ext < kListFrom - 1 ? stop_ext : kListFrom - 1)
+                   : (stop_ext < kFirstLevels ? stop_ext : kFirstLevels);
+    long ntiles = 0;
     while (true) {
       ensure_levels(ctx, ip, Lb, stop_mem);
       fill_levels(ctx, ip, Lb, A.lv);
       A.La = La;
       A.Lb = Lb;
       check(hipEventRecord(ctx->ev0, ctx->stream), "event");
-      if (conv->mode == OM_MODE_PLAIN || conv->mode == OM_MODE_POINT_MIXED) {
+      if (!tiled) {
         check(om_launch_lw_point(&A, conv->mode == OM_MODE_POINT_MIXED ? 1 : 0, ctx->split, ctx->stream),
               "lw_point");
+      } else if (A.list) {
+        check(om_launch_lw_tile_list(&A, ntiles, mixed ? 1 : 0, ctx->split, ctx->stream), "lw_tile");
       } else {
         check(om_launch_lw_tile(&A, n_obs, mixed ? 1 : 0, ctx->split, ctx->stream), "lw_tile");
       }
@@ -652,9 +665,21 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
       kms += ms;
       if (Lb >= stop_ext) break;
-      if (count_active(ctx, A.st, npts) == 0) break;
       La = Lb + 1;
       Lb = La;
+      if (tiled) {
+        ctx->d_list.ensure((size_t) n_obs * ((n_e + 63) / 64) * 64);
+        ctx->d_count.ensure(1);
+        check(om_launch_tile_list(&A.st, n_obs, n_e, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "tile list");
+        int nt = 0;
+        check(hipMemcpyAsync(&nt, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "ntiles");
+        check(hipStreamSynchronize(ctx->stream), "sync");
+        if (nt == 0) break;
+        ntiles = nt;
+        A.list = ctx->d_list.p;
+      } else if (count_active(ctx, A.st, npts) == 0) {
+        break;
+      }
     }
     const om_particle& P = ctx->parts[ip];
     om::FinArgs F;

@@ -61,6 +61,7 @@ struct LwArgs {
   double pol[6];
   int use_pol;
   double precision;
+  const int* list;      // tile kernel: 64 point indices per tile (-1 pads), or null for dense tiles
 };
 
 // Per-level convergence test, src/OSCARSSR.cc:1997-2012.
@@ -208,9 +209,19 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
   int lane = threadIdx.x & 63;
   int w = threadIdx.x >> 6;
   long tile = blockIdx.x;
-  long io = tile / n_tiles_w;
-  long iw = (tile - io * n_tiles_w) * 64 + lane;
-  bool valid = iw < A.n_omega;
+  long io, iw;
+  bool valid;
+  if (A.list) {
+    // compacted tile: active points of one observation point (the first entry is never a pad)
+    int pt = A.list[64 * tile + lane];
+    io = A.list[64 * tile] / A.n_omega;
+    valid = pt >= 0;
+    iw = valid ? pt - io * A.n_omega : 0;
+  } else {
+    io = tile / n_tiles_w;
+    iw = (tile - io * n_tiles_w) * 64 + lane;
+    valid = iw < A.n_omega;
+  }
   long iwc = valid ? iw : A.n_omega - 1;
   long p = io * A.n_omega + iwc;
   V3 obs = v3(A.obs[3 * io], A.obs[3 * io + 1], A.obs[3 * io + 2]);
@@ -459,6 +470,55 @@ __global__ void __launch_bounds__(256) count_active_kernel(State st, long npts, 
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, __popcll(m));
 }
 
+// Tiles of the next level, one workgroup per observation point: its points
+// still active (unconverged, summed up to level La - 1) in increasing energy
+// order, padded with -1 to a multiple of 64 and appended at a position taken
+// from a tile counter.  Converged lanes then no longer ride along in partly
+// finished tiles (the order of the tiles does not affect any result).
+__global__ void __launch_bounds__(256) tile_list_kernel(State st, long n_omega, int La, int* __restrict__ list,
+                                                        int* __restrict__ ntiles) {
+  __shared__ int wcount[4];
+  __shared__ int base_s, total_s;
+  long io = blockIdx.x;
+  int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long p0 = io * n_omega;
+  int cnt = 0;
+  for (long e = tid; e < n_omega; e += 256) cnt += (!st.conv[p0 + e] && st.level[p0 + e] == La - 1) ? 1 : 0;
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
+  if (lane == 0) wcount[wv] = cnt;
+  __syncthreads();
+  if (tid == 0) {
+    int total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    total_s = total;
+    base_s = total > 0 ? atomicAdd(ntiles, (total + 63) / 64) : 0;
+  }
+  __syncthreads();
+  int total = total_s;
+  if (total == 0) return;
+  int* out = list + 64L * base_s;
+  int running = 0;
+  for (long c = 0; c < n_omega; c += 256) {
+    long e = c + tid;
+    bool a = e < n_omega && !st.conv[p0 + e] && st.level[p0 + e] == La - 1;
+    unsigned long long m = __ballot(a);
+    int before = __popcll(m & ((1ULL << lane) - 1ULL));
+    __syncthreads();  // wcount of the previous chunk has been read
+    if (lane == 0) wcount[wv] = __popcll(m);
+    __syncthreads();
+    int woff = 0;
+    for (int k = 0; k < wv; ++k) woff += wcount[k];
+    if (a) out[running + woff + before] = (int) (p0 + e);
+    running += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+  }
+  int padded = (total + 63) / 64 * 64;
+  for (int i = total + tid; i < padded; i += 256) out[i] = -1;
+}
+
+__global__ void __launch_bounds__(64) int_fill_kernel(int* __restrict__ p, int n, int v) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 }  // namespace om
 
 // ---------------------------------------------------------------- launchers (host)
@@ -505,6 +565,38 @@ extern "C" hipError_t om_launch_lw_point(const om::LwArgs* A, int mixed, int spl
 extern "C" hipError_t om_launch_lw_tile(const om::LwArgs* A, long n_obs, int mixed, int split, hipStream_t s) {
   if (split <= 0) split = split_for(n_obs * ((A->n_omega + 63) / 64));
   if (mixed) launch_tile<true>(A, n_obs, split, s); else launch_tile<false>(A, n_obs, split, s);
+  return hipGetLastError();
+}
+
+// Compacted tiles: A->list holds ntiles x 64 point indices (launch_tile's grid
+// is n_obs x ceil(n_omega/64) tiles, so pass ntiles as n_obs with n_omega <= 64
+// tiles per "observation": the kernel reads io from the list instead).
+extern "C" hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, int mixed, int split, hipStream_t s) {
+  if (split <= 0) split = split_for(ntiles);
+  dim3 g((unsigned) ntiles);
+  if (mixed) {
+    switch (split) {
+      case 1: hipLaunchKernelGGL((om::lw_tile_kernel<true, 1>), g, dim3(64), 0, s, *A, 1L); break;
+      case 2: hipLaunchKernelGGL((om::lw_tile_kernel<true, 2>), g, dim3(128), 0, s, *A, 1L); break;
+      case 4: hipLaunchKernelGGL((om::lw_tile_kernel<true, 4>), g, dim3(256), 0, s, *A, 1L); break;
+      default: hipLaunchKernelGGL((om::lw_tile_kernel<true, 8>), g, dim3(512), 0, s, *A, 1L); break;
+    }
+  } else {
+    switch (split) {
+      case 1: hipLaunchKernelGGL((om::lw_tile_kernel<false, 1>), g, dim3(64), 0, s, *A, 1L); break;
+      case 2: hipLaunchKernelGGL((om::lw_tile_kernel<false, 2>), g, dim3(128), 0, s, *A, 1L); break;
+      case 4: hipLaunchKernelGGL((om::lw_tile_kernel<false, 4>), g, dim3(256), 0, s, *A, 1L); break;
+      default: hipLaunchKernelGGL((om::lw_tile_kernel<false, 8>), g, dim3(512), 0, s, *A, 1L); break;
+    }
+  }
+  return hipGetLastError();
+}
+
+// Builds the compacted tile list for level La (counter cleared first by a kernel).
+extern "C" hipError_t om_launch_tile_list(const om::State* st, long n_obs, long n_omega, int La, int* list,
+                                          int* ntiles, hipStream_t s) {
+  hipLaunchKernelGGL(om::int_fill_kernel, dim3(1), dim3(64), 0, s, ntiles, 1, 0);
+  hipLaunchKernelGGL(om::tile_list_kernel, dim3((unsigned) n_obs), dim3(256), 0, s, *st, n_omega, La, list, ntiles);
   return hipGetLastError();
 }
 
