@@ -48,6 +48,7 @@ struct LwArgs {
   int use_pol;
   double precision;
   const int* list;
+  long n_list;
 };
 struct PdArgs {
   Levels lv;
@@ -152,6 +153,7 @@ hipError_t om_launch_lw_tile(const om::LwArgs* A, long n_obs, int mixed, int spl
 hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, int mixed, int split, hipStream_t s);
 hipError_t om_launch_tile_list(const om::State* st, long n_obs, long n_omega, int La, int* list, int* ntiles,
                                hipStream_t s);
+hipError_t om_launch_point_list(const om::State* st, long npts, int La, int* list, int* count, hipStream_t s);
 hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s);
 hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s);
 hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s);
@@ -640,10 +642,10 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     float kms = 0;
     int La = 0;
     bool tiled = !(conv->mode == OM_MODE_PLAIN || conv->mode == OM_MODE_POINT_MIXED);
-    // Tiled kernels: levels 0..8 (no point can stop before level 9) in one dense
-    // launch, then one launch per level over compacted tiles of active points.
-    int Lb = tiled ? (stop_ext < kListFrom - 1 ? stop_ext : kListFrom - 1)
-                   : (stop_ext < kFirstLevels ? stop_ext : kFirstLevels);
+    // Levels 0..8 (no point can stop before level 9) in one dense launch, then
+    // one launch per level over the points still active, compacted into full
+    // waves (tiles of one observation point for the tiled kernels).
+    int Lb = stop_ext < kListFrom - 1 ? stop_ext : kListFrom - 1;
     long ntiles = 0;
     while (true) {
       ensure_levels(ctx, ip, Lb, stop_mem);
@@ -667,19 +669,19 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       if (Lb >= stop_ext) break;
       La = Lb + 1;
       Lb = La;
-      if (tiled) {
-        ctx->d_list.ensure((size_t) n_obs * ((n_e + 63) / 64) * 64);
-        ctx->d_count.ensure(1);
+      ctx->d_list.ensure((size_t) n_obs * ((n_e + 63) / 64) * 64);
+      ctx->d_count.ensure(1);
+      if (tiled)
         check(om_launch_tile_list(&A.st, n_obs, n_e, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "tile list");
-        int nt = 0;
-        check(hipMemcpyAsync(&nt, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "ntiles");
-        check(hipStreamSynchronize(ctx->stream), "sync");
-        if (nt == 0) break;
-        ntiles = nt;
-        A.list = ctx->d_list.p;
-      } else if (count_active(ctx, A.st, npts) == 0) {
-        break;
-      }
+      else
+        check(om_launch_point_list(&A.st, npts, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "point list");
+      int nt = 0;
+      check(hipMemcpyAsync(&nt, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "list count");
+      check(hipStreamSynchronize(ctx->stream), "sync");
+      if (nt == 0) break;
+      ntiles = nt;
+      A.list = ctx->d_list.p;
+      A.n_list = nt;
     }
     const om_particle& P = ctx->parts[ip];
     om::FinArgs F;

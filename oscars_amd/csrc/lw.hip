@@ -61,7 +61,9 @@ struct LwArgs {
   double pol[6];
   int use_pol;
   double precision;
-  const int* list;      // tile kernel: 64 point indices per tile (-1 pads), or null for dense tiles
+  const int* list;      // null: dense grid.  Tile kernel: 64 point indices per tile (-1 pads);
+                        // point kernel: n_list point indices, one per lane.
+  long n_list;
 };
 
 // Per-level convergence test, src/OSCARSSR.cc:1997-2012.
@@ -131,8 +133,16 @@ __global__ void __launch_bounds__(64 * S) lw_point_kernel(LwArgs A) {
   __shared__ SliceLds<S> sh;
   int lane = threadIdx.x & 63;
   int w = threadIdx.x >> 6;
-  long p = (long) blockIdx.x * 64 + lane;
-  bool valid = p < A.npts;
+  long g = (long) blockIdx.x * 64 + lane;
+  bool valid;
+  long p;
+  if (A.list) {  // compacted: the points still active at this level
+    valid = g < A.n_list;
+    p = valid ? (long) A.list[g] : 0;
+  } else {
+    valid = g < A.npts;
+    p = g;
+  }
   long pc = valid ? p : 0;
   long io = pc / A.n_omega;
   long iw = pc - io * A.n_omega;
@@ -514,6 +524,20 @@ __global__ void __launch_bounds__(256) tile_list_kernel(State st, long n_omega, 
   for (int i = total + tid; i < padded; i += 256) out[i] = -1;
 }
 
+// Points still active at level La (unconverged, summed up to La - 1), compacted
+// for the point kernel.  Every lane takes part in the counter atomic (lane 0
+// adds its wave's count) and the wave's base comes from readfirstlane.
+__global__ void __launch_bounds__(256) point_list_kernel(State st, long npts, int La, int* __restrict__ list,
+                                                         int* __restrict__ count) {
+  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  bool a = p < npts && !st.conv[p] && st.level[p] == La - 1;
+  unsigned long long m = __ballot(a);
+  int mine = atomicAdd(count, lane == 0 ? __popcll(m) : 0);
+  int base = __builtin_amdgcn_readfirstlane(mine);
+  if (a) list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int) p;
+}
+
 __global__ void __launch_bounds__(64) int_fill_kernel(int* __restrict__ p, int n, int v) {
   int i = threadIdx.x;
   if (i < n) p[i] = v;
@@ -533,7 +557,7 @@ static int split_for(long groups) {
 
 template <bool M>
 static void launch_point(const om::LwArgs* A, int split, hipStream_t s) {
-  long blocks = (A->npts + 63) / 64;
+  long blocks = ((A->list ? A->n_list : A->npts) + 63) / 64;
   dim3 g((unsigned) blocks);
   switch (split) {
     case 1: hipLaunchKernelGGL((om::lw_point_kernel<M, 1>), g, dim3(64), 0, s, *A); break;
@@ -557,7 +581,7 @@ static void launch_tile(const om::LwArgs* A, long n_obs, int split, hipStream_t 
 
 // split <= 0 picks the number of waves per workgroup from the grid size.
 extern "C" hipError_t om_launch_lw_point(const om::LwArgs* A, int mixed, int split, hipStream_t s) {
-  if (split <= 0) split = split_for((A->npts + 63) / 64);
+  if (split <= 0) split = split_for(((A->list ? A->n_list : A->npts) + 63) / 64);
   if (mixed) launch_point<true>(A, split, s); else launch_point<false>(A, split, s);
   return hipGetLastError();
 }
@@ -589,6 +613,15 @@ extern "C" hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, i
       default: hipLaunchKernelGGL((om::lw_tile_kernel<false, 8>), g, dim3(512), 0, s, *A, 1L); break;
     }
   }
+  return hipGetLastError();
+}
+
+// Builds the compacted point list for level La (counter cleared first by a kernel).
+extern "C" hipError_t om_launch_point_list(const om::State* st, long npts, int La, int* list, int* count,
+                                           hipStream_t s) {
+  hipLaunchKernelGGL(om::int_fill_kernel, dim3(1), dim3(64), 0, s, count, 1, 0);
+  hipLaunchKernelGGL(om::point_list_kernel, dim3((unsigned) ((npts + 255) / 256)), dim3(256), 0, s, *st, npts, La,
+                     list, count);
   return hipGetLastError();
 }
 
