@@ -80,7 +80,7 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ void sincos_mode(bool mixed, double phi, double& sn, double& cs) {
   if (mixed) {
     float sf, cf;
-    sincos_half32(reduce_pi(phi), &sf, &cf);
+    sincos_reduced32(reduce_pio2(phi), &sf, &cf);
     sn = (double) sf;
     cs = (double) cf;
   } else {

@@ -121,39 +121,4 @@ __device__ __forceinline__ void sincos_reduced32(Reduced a, float* s, float* c) 
   *c = ((q + 1) & 2) ? -cc : cc;
 }
 
-// Reduction by pi for the FP32 path: k = nearest integer to phi/pi,
-// r = phi - k*kPiHi is exact (both are multiples of 2^-51 and |r| < 2), then
-// r -= k*kPiLo rounds once; |r| <= pi/2 + 2e-5 and sin, cos of phi are
-// (-1)^k sin r, (-1)^k cos r: one sign flip instead of a quadrant swap.
-constexpr double kOneOverPi = 0.31830988618379067154;
-constexpr double kPiHi = 3.141592653589793116;      // nearest double to pi
-constexpr double kPiLo = 1.2246467991473532e-16;    // pi - kPiHi (rounded)
-
-__device__ __forceinline__ Reduced reduce_pi(double phi) {
-  double kk = __builtin_fma(phi, kOneOverPi, kRoundMagic);
-  int q = (int) (uint32_t) __double_as_longlong(kk);
-  double k = kk - kRoundMagic;
-  double r = __builtin_fma(-k, kPiHi, phi);
-  r = __builtin_fma(-k, kPiLo, r);
-  return Reduced{r, q};
-}
-
-// FP32 polynomial sin/cos on the FP64 half-period reduction (degree 9 / 10 on
-// |r| <= pi/2; about one FP32 ulp); the sign of both is (-1)^q.
-__device__ __forceinline__ void sincos_half32(Reduced a, float* s, float* c) {
-  float r = (float) a.r;
-  float u = r * r;
-  float ps = __builtin_fmaf(u, OM_S32H_3, OM_S32H_2);
-  ps = __builtin_fmaf(u, ps, OM_S32H_1);
-  ps = __builtin_fmaf(u, ps, OM_S32H_0);
-  float sr = __builtin_fmaf(r * u, ps, r);
-  float pc = __builtin_fmaf(u, OM_C32H_3, OM_C32H_2);
-  pc = __builtin_fmaf(u, pc, OM_C32H_1);
-  pc = __builtin_fmaf(u, pc, OM_C32H_0);
-  float cr = __builtin_fmaf(u * u, pc, __builtin_fmaf(-0.5f, u, 1.0f));
-  uint32_t sgn = ((uint32_t) a.q) << 31;
-  *s = __uint_as_float(__float_as_uint(sr) ^ sgn);
-  *c = __uint_as_float(__float_as_uint(cr) ^ sgn);
-}
-
 }  // namespace om
