@@ -372,6 +372,7 @@ def main():
     # HBM bytes per launch of the sweep kernel, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
     # on this workload and scaled by the counters' reading of a plain device copy (profiles/)
     traffic = None
+    tj = {}
     tfile = os.path.join(ROOT, "profiles", "lw_tile_sweep_traffic.json")
     if os.path.exists(tfile) and args.mode == "mixed":
         with open(tfile) as f:
@@ -382,7 +383,11 @@ def main():
                 "kernel": "lw_tile_kernel<%s>" % ("true" if args.mode == "mixed" else "false")
                 if args.mode != "plain" else "lw_point_kernel<false>",
                 "kernel_ms": round(sweep_ms, 3), "evals_per_launch": sweep_evals,
-                "sincos_per_s": sweep_evals / (sweep_ms * 1e-3)}
+                "sincos_per_s": sweep_evals / (sweep_ms * 1e-3),
+                "scope": "all launches of the sweep's tiled kernel (dense levels 0..8, then one per level)",
+                "traffic_source": tj.get("source"),
+                "valu_issue_frac": tj.get("valu_issue_frac"),
+                "valu_wave_instr_per_eval": tj.get("valu_wave_instr_per_eval")}
 
     if not args.no_plain and args.mode != "plain":
         pdt, pevals, pkms, pres = timed(work, "plain", args.plain_steps, 0)
@@ -414,7 +419,9 @@ def main():
             cpu = {"value": None, "error": str(ex)}
 
     line = {
-        "metric": "LW integrand evals/s (obs x omega x traj), forward, single electron",
+        "metric": "LW integrand evals/s (obs\u00d7\u03c9\u00d7traj) fwd+adjoint vs plain GPU at 1e-5 rel err",
+        "value_is": "forward evals/s of the mixed path on the flux-map workload; adjoint and batched legs in "
+                    "'adjoint' and 'batched', the plain FP64 GPU path in 'plain_fp64', errors in 'rel_err_vs_fp64'",
         "value": value,
         "unit": "evals/s",
         "n_gpus": 1,
