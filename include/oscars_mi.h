@@ -95,6 +95,8 @@ typedef struct {
   double total_ms;     /* whole call, host wall clock */
   int max_level;       /* highest level any point reached */
   long n_unconverged;
+  long n_rechecked;    /* mixed modes: points whose stop decision lay within the FP32 error
+                          bound and were recomputed in FP64 */
 } om_stats;
 
 int om_device_count(void);

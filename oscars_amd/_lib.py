@@ -51,7 +51,7 @@ class om_batch_opts(C.Structure):
 
 class om_stats(C.Structure):
     _fields_ = [("evals", C.c_double), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
-                ("max_level", C.c_int), ("n_unconverged", C.c_long)]
+                ("max_level", C.c_int), ("n_unconverged", C.c_long), ("n_rechecked", C.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -30,11 +30,12 @@ struct Levels {
 };
 struct State {
   double* sum;
-  double* comp;
+  double* err;
   double* lastmag;
   double* prec;
   int* level;
   int* conv;
+  int* border;
 };
 struct LwArgs {
   Levels lv;
@@ -154,6 +155,7 @@ hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, int mixed, i
 hipError_t om_launch_tile_list(const om::State* st, long n_obs, long n_omega, int La, int* list, int* ntiles,
                                hipStream_t s);
 hipError_t om_launch_point_list(const om::State* st, long npts, int La, int* list, int* count, hipStream_t s);
+hipError_t om_launch_border_list(const om::State* st, long npts, int* list, int* count, hipStream_t s);
 hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s);
 hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s);
 hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s);
@@ -239,8 +241,8 @@ struct om_context {
   DevBuf<om::FieldDev> d_fields;
   std::vector<DevBuf<double>> grid_bufs;
   // work buffers
-  DevBuf<double> d_obs, d_omega, d_sum, d_comp, d_lastmag, d_prec, d_out, d_outcomp, d_oute;
-  DevBuf<int> d_level, d_conv, d_count, d_outlevel, d_list;
+  DevBuf<double> d_obs, d_omega, d_sum, d_err, d_lastmag, d_prec, d_out, d_outcomp, d_oute;
+  DevBuf<int> d_level, d_conv, d_count, d_outlevel, d_list, d_border;
   DevBuf<unsigned char> d_outconv;
   DevBuf<double> d_bd;
   std::vector<om::FieldDev> fields_host;
@@ -311,10 +313,10 @@ struct om_context {
     for (auto& g : grid_bufs) g.release();
     d_desc.release(); d_t.release(); d_p.release(); d_ypp.release(); d_u.release(); d_fields.release();
     d_sig.release(); d_sm1.release(); d_pv.release(); d_ppv.release(); d_u2.release();
-    d_obs.release(); d_omega.release(); d_sum.release(); d_comp.release(); d_lastmag.release();
+    d_obs.release(); d_omega.release(); d_sum.release(); d_err.release(); d_lastmag.release();
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
-    d_list.release();
+    d_list.release(); d_border.release();
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) (void) hipStreamDestroy(stream);
@@ -485,13 +487,17 @@ void fill_levels(om_context* ctx, int ip, int Lb, om::Levels& lv) {
 
 void state_for(om_context* ctx, long npts, bool mixed, om::State& st) {
   ctx->d_sum.ensure(6 * npts);
-  ctx->d_comp.ensure(6 * npts);
   ctx->d_lastmag.ensure(npts);
   ctx->d_prec.ensure(npts);
   ctx->d_level.ensure(npts);
   ctx->d_conv.ensure(npts);
+  if (mixed) {
+    ctx->d_err.ensure(2 * npts);
+    ctx->d_border.ensure(npts);
+  }
   st.sum = ctx->d_sum.p;
-  st.comp = mixed ? ctx->d_comp.p : nullptr;
+  st.err = mixed ? ctx->d_err.p : nullptr;
+  st.border = mixed ? ctx->d_border.p : nullptr;
   st.lastmag = ctx->d_lastmag.p;
   st.prec = ctx->d_prec.p;
   st.level = ctx->d_level.p;
@@ -623,9 +629,10 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     upload(ctx, ctx->d_obs, obs_xyz, 3 * n_obs);
     upload(ctx, ctx->d_omega, om.data(), n_e);
     bool mixed = conv->mode == OM_MODE_MIXED;
+    bool mixed_any = mixed || conv->mode == OM_MODE_POINT_MIXED;
     om::LwArgs A;
     std::memset(&A, 0, sizeof(A));
-    state_for(ctx, npts, mixed, A.st);
+    state_for(ctx, npts, mixed_any, A.st);
     A.npts = npts;
     A.obs = ctx->d_obs.p;
     A.omega = ctx->d_omega.p;
@@ -647,6 +654,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     // waves (tiles of one observation point for the tiled kernels).
     int Lb = stop_ext < kListFrom - 1 ? stop_ext : kListFrom - 1;
     long ntiles = 0;
+    long n_recheck = 0;
     while (true) {
       ensure_levels(ctx, ip, Lb, stop_mem);
       fill_levels(ctx, ip, Lb, A.lv);
@@ -683,6 +691,50 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       A.list = ctx->d_list.p;
       A.n_list = nt;
     }
+    // Mixed path: points whose stop decision lay within the FP32 error bound are
+    // recomputed from level 0 with the FP64 per-point kernel (their state was
+    // reset when listed), so every point stops where the FP64 path stops.
+    if (mixed_any) {
+      ctx->d_list.ensure((size_t) n_obs * ((n_e + 63) / 64) * 64);
+      ctx->d_count.ensure(1);
+      check(om_launch_border_list(&A.st, npts, ctx->d_list.p, ctx->d_count.p, ctx->stream), "border list");
+      int nb = 0;
+      check(hipMemcpyAsync(&nb, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "border count");
+      check(hipStreamSynchronize(ctx->stream), "sync");
+      n_recheck = nb;
+      if (nb > 0) {
+        om::State st64 = A.st;
+        st64.err = nullptr;
+        st64.border = nullptr;
+        A.st = st64;
+        A.list = ctx->d_list.p;
+        A.n_list = nb;
+        La = 0;
+        Lb = stop_ext < kListFrom - 1 ? stop_ext : kListFrom - 1;
+        while (true) {
+          ensure_levels(ctx, ip, Lb, stop_mem);
+          fill_levels(ctx, ip, Lb, A.lv);
+          A.La = La;
+          A.Lb = Lb;
+          check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+          check(om_launch_lw_point(&A, 0, ctx->split, ctx->stream), "lw_point recheck");
+          check(hipEventRecord(ctx->ev1, ctx->stream), "event");
+          check(hipEventSynchronize(ctx->ev1), "sync");
+          float ms = 0;
+          check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
+          kms += ms;
+          if (Lb >= stop_ext) break;
+          La = Lb + 1;
+          Lb = La;
+          check(om_launch_point_list(&A.st, npts, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "point list");
+          int nt = 0;
+          check(hipMemcpyAsync(&nt, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "count");
+          check(hipStreamSynchronize(ctx->stream), "sync");
+          if (nt == 0) break;
+          A.n_list = nt;
+        }
+      }
+    }
     const om_particle& P = ctx->parts[ip];
     om::FinArgs F;
     std::memset(&F, 0, sizeof(F));
@@ -701,6 +753,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     if (stats) {
       stats->kernel_ms = kms;
       stats->total_ms = wall.ms();
+      stats->n_rechecked = n_recheck;
     }
   });
 }

@@ -43,11 +43,13 @@ struct Levels {
 
 struct State {
   double* sum;      // 6 doubles per point (Re/Im of Ex, Ey, Ez), power density: sum[0]
-  double* comp;     // mixed path: Kahan compensation, 6 per point (may be null)
+  double* err;      // mixed path (else null): 2 per point, sum over samples of |V|_1 and the
+                    // error bound of lastmag
   double* lastmag;  // |E|^2 (or power sum) after the previous level
   double* prec;     // last relative change
   int* level;       // last level summed (-1 before any)
   int* conv;        // 1 once converged
+  int* border;      // mixed path (else null): 1 if a level decision lay within the error bound
 };
 
 struct LwArgs {
@@ -81,6 +83,26 @@ __device__ __forceinline__ bool level_converged(const LwArgs& A, int L, const do
   return c;
 }
 
+// Mixed path: bound of the error the FP32 sin/cos puts into this level's
+// decision.  Each FP32 cos/sin is within kSinCos32Err of the exact value, so
+// every real component of the sum is off by at most kSinCos32Err * an, with
+// an = sum over the samples so far of |V|_1; E = sum*dt, the polarization
+// projection does not increase the norm, and |E|^2 moves by at most
+// dm = 2 |E| d + d^2 with d = sqrt(6) kSinCos32Err an dt.  The relative change
+// prec then moves by at most (dm + dm_previous)/lastmag (1 + prec); a decision
+// whose margin |prec - precision| is within twice that is "borderline" (the
+// FP64 path might decide the other way) and the point is recomputed in FP64.
+constexpr double kSinCos32Err = 2.5e-7;
+
+__device__ __forceinline__ bool level_borderline(const LwArgs& A, int L, double mag_prev, double mag, double prec,
+                                                 double maxd, double an, double& dm_last) {
+  double d = 2.449489742783178 * kSinCos32Err * an * A.lv.dt_incl[L];
+  double dm = 2.0 * sqrt(mag) * d + d * d;
+  double b = 2.0 * (dm + dm_last) / mag_prev * (1.0 + prec);
+  dm_last = dm;
+  return L > 8 && maxd < kPi && fabs(prec - A.precision) <= b;
+}
+
 // ---------------------------------------------------------------- level-slice combination
 // A workgroup of S waves serves 64 output points; wave w sums the contiguous
 // slice [ns*w/S, ns*(w+1)/S) of each level.  Each wave leaves its partial
@@ -93,6 +115,7 @@ struct SliceLds {
   double maxd[S][64];
   double first[S][64];
   double last[S][64];
+  double an[S][64];
 };
 
 template <int S>
@@ -155,13 +178,19 @@ __global__ void __launch_bounds__(64 * S) lw_point_kernel(LwArgs A) {
   for (int k = 0; k < 6; ++k) sum[k] = A.st.sum[6 * pc + k];
   double lastmag = A.st.lastmag[pc];
   double prec = A.st.prec[pc];
+  double an_run = 0, dm_last = 0;
+  bool border = false;
+  if (MIXED) {
+    an_run = A.st.err[2 * pc];
+    dm_last = A.st.err[2 * pc + 1];
+  }
   for (int L = A.La; L <= A.Lb; ++L) {
     if (!__any(active)) break;
     const double* __restrict__ rows = A.lv.rows[L];
     long ns = 1L << L;
     long i0 = ns * w / S, i1 = ns * (w + 1) / S;
     double ps[6] = {0, 0, 0, 0, 0, 0};
-    double maxd = 0, first = 0, lastphase = 0;
+    double maxd = 0, first = 0, lastphase = 0, an = 0;
     if (i1 > i0) {
       Amp a0 = amplitude(obs, rows + 10 * i0);
       first = negw * a0.tau;
@@ -177,12 +206,18 @@ __global__ void __launch_bounds__(64 * S) lw_point_kernel(LwArgs A) {
       ps[0] = __builtin_fma(a.V.x, cs, ps[0]); ps[1] = __builtin_fma(a.V.x, sn, ps[1]);
       ps[2] = __builtin_fma(a.V.y, cs, ps[2]); ps[3] = __builtin_fma(a.V.y, sn, ps[3]);
       ps[4] = __builtin_fma(a.V.z, cs, ps[4]); ps[5] = __builtin_fma(a.V.z, sn, ps[5]);
+      if (MIXED) an += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);
     }
     double tot[6];
     if (S > 1) {
       slice_publish<S>(sh, w, lane, ps, maxd, first, lastphase);
+      if (MIXED) sh.an[w][lane] = an;
       __syncthreads();
       slice_combine<S>(sh, lane, ns, tot, maxd);
+      if (MIXED) {
+        an = 0;
+        for (int v = 0; v < S; ++v) an += sh.an[v][lane];
+      }
       __syncthreads();
     } else {
 #pragma unroll
@@ -192,7 +227,13 @@ __global__ void __launch_bounds__(64 * S) lw_point_kernel(LwArgs A) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) sum[k] += tot[k];
       lev = L;
-      if (level_converged(A, L, sum, maxd, lastmag, prec)) {
+      double mag_prev = lastmag;
+      bool c = level_converged(A, L, sum, maxd, lastmag, prec);
+      if (MIXED) {
+        an_run += an;
+        border = border || level_borderline(A, L, mag_prev, lastmag, prec, maxd, an_run, dm_last);
+      }
+      if (c) {
         A.st.conv[pc] = 1;
         active = false;
       }
@@ -204,6 +245,11 @@ __global__ void __launch_bounds__(64 * S) lw_point_kernel(LwArgs A) {
     A.st.lastmag[pc] = lastmag;
     A.st.prec[pc] = prec;
     A.st.level[pc] = lev;
+    if (MIXED) {
+      A.st.err[2 * pc] = an_run;
+      A.st.err[2 * pc + 1] = dm_last;
+      if (border) A.st.border[pc] = 1;
+    }
   }
 }
 
@@ -243,6 +289,12 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
   for (int k = 0; k < 6; ++k) sum[k] = A.st.sum[6 * p + k];
   double lastmag = A.st.lastmag[p];
   double prec = A.st.prec[p];
+  double an_run = 0, dm_last = 0;
+  bool border = false;
+  if (MIXED) {
+    an_run = A.st.err[2 * p];
+    dm_last = A.st.err[2 * p + 1];
+  }
   double* tau_w = s_tau[w];
   double* vx_w = s_v[w][0];
   double* vy_w = s_v[w][1];
@@ -253,7 +305,7 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
     long ns = 1L << L;
     long i0 = ns * w / S, i1 = ns * (w + 1) / S;
     double ps[6] = {0, 0, 0, 0, 0, 0};
-    double maxd = 0, first = 0, lastphase = 0;
+    double maxd = 0, first = 0, lastphase = 0, an = 0;
     for (long base = i0; base < i1; base += 64) {
       long si = base + lane;
       if (si < i1) {
@@ -262,6 +314,7 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
         vx_w[lane] = a.V.x;
         vy_w[lane] = a.V.y;
         vz_w[lane] = a.V.z;
+        if (MIXED) an += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);  // this lane's samples
       }
       wave_sync();  // the wave's own LDS writes are visible to all its lanes
       int cnt = (int) ((i1 - base) < 64 ? (i1 - base) : 64);
@@ -297,11 +350,19 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
       }
       wave_sync();  // all lanes done reading before the next chunk is written
     }
+    if (MIXED) {  // the samples' |V|_1 summed over the wave (the same for every energy lane)
+      for (int off = 32; off > 0; off >>= 1) an += __shfl_xor(an, off);
+    }
     double tot[6];
     if (S > 1) {
       slice_publish<S>(sh, w, lane, ps, maxd, first, lastphase);
+      if (MIXED) sh.an[w][lane] = an;
       __syncthreads();
       slice_combine<S>(sh, lane, ns, tot, maxd);
+      if (MIXED) {
+        an = 0;
+        for (int v = 0; v < S; ++v) an += sh.an[v][lane];
+      }
       __syncthreads();
     } else {
 #pragma unroll
@@ -311,7 +372,13 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
 #pragma unroll
       for (int k = 0; k < 6; ++k) sum[k] += tot[k];
       lev = L;
-      if (level_converged(A, L, sum, maxd, lastmag, prec)) {
+      double mag_prev = lastmag;
+      bool c = level_converged(A, L, sum, maxd, lastmag, prec);
+      if (MIXED) {
+        an_run += an;
+        border = border || level_borderline(A, L, mag_prev, lastmag, prec, maxd, an_run, dm_last);
+      }
+      if (c) {
         A.st.conv[p] = 1;
         active = false;
       }
@@ -323,6 +390,11 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
     A.st.lastmag[p] = lastmag;
     A.st.prec[p] = prec;
     A.st.level[p] = lev;
+    if (MIXED) {
+      A.st.err[2 * p] = an_run;
+      A.st.err[2 * p + 1] = dm_last;
+      if (border) A.st.border[p] = 1;
+    }
   }
 }
 
@@ -462,10 +534,32 @@ __global__ void __launch_bounds__(256) state_reset_kernel(State st, long npts) {
   long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts) return;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    st.sum[6 * p + k] = 0;
-    if (st.comp) st.comp[6 * p + k] = 0;
+  for (int k = 0; k < 6; ++k) st.sum[6 * p + k] = 0;
+  if (st.err) {
+    st.err[2 * p] = 0;
+    st.err[2 * p + 1] = 0;
   }
+  if (st.border) st.border[p] = 0;
+  st.lastmag[p] = -1;
+  st.prec[p] = -1;
+  st.level[p] = -1;
+  st.conv[p] = 0;
+}
+
+// Points flagged by the mixed path, listed (any order) and reset to the start
+// of the level loop for the FP64 recomputation.
+__global__ void __launch_bounds__(256) border_list_kernel(State st, long npts, int* __restrict__ list,
+                                                          int* __restrict__ count) {
+  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  bool a = p < npts && st.border[p];
+  unsigned long long m = __ballot(a);
+  int mine = atomicAdd(count, lane == 0 ? __popcll(m) : 0);
+  int base = __builtin_amdgcn_readfirstlane(mine);
+  if (!a) return;
+  list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int) p;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) st.sum[6 * p + k] = 0;
   st.lastmag[p] = -1;
   st.prec[p] = -1;
   st.level[p] = -1;
@@ -613,6 +707,14 @@ extern "C" hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, i
       default: hipLaunchKernelGGL((om::lw_tile_kernel<false, 8>), g, dim3(512), 0, s, *A, 1L); break;
     }
   }
+  return hipGetLastError();
+}
+
+// Lists (and resets) the points the mixed path flagged as borderline.
+extern "C" hipError_t om_launch_border_list(const om::State* st, long npts, int* list, int* count, hipStream_t s) {
+  hipLaunchKernelGGL(om::int_fill_kernel, dim3(1), dim3(64), 0, s, count, 1, 0);
+  hipLaunchKernelGGL(om::border_list_kernel, dim3((unsigned) ((npts + 255) / 256)), dim3(256), 0, s, *st, npts, list,
+                     count);
   return hipGetLastError();
 }
 

@@ -98,6 +98,26 @@ def test_spectrum(oracle_planar, gpu_planar, mode, tol):
 
 
 @gpu
+@pytest.mark.parametrize("mode,ref", [("mixed", "tiled"), ("point_mixed", "plain")])
+def test_mixed_levels_match_fp64(gpu_planar, mode, ref):
+    """Every stop decision of the mixed paths matches the FP64 path: decisions
+    within the FP32 error bound are recomputed in FP64 (counted in
+    n_rechecked)."""
+    kw = dict(npoints=[41, 41], width=[0.02, 0.02], translation=[0, 0, 30])
+    e = np.linspace(140.0, 460.0, 64) if mode == "mixed" else np.array([152.11])
+    _, v64 = gpu_planar.calculate_flux_rectangle_energies(e, mode=ref, **kw)
+    lev64 = gpu_planar._last_levels.copy()
+    _, v32 = gpu_planar.calculate_flux_rectangle_energies(e, mode=mode, **kw)
+    lev32 = gpu_planar._last_levels
+    st = gpu_planar.last_stats
+    err = rel_err(v32.ravel(), v64.ravel(), 1e-6)
+    print("%s vs %s: %d points, level mismatches %d, rechecked %d, max rel err %.3e"
+          % (mode, ref, lev32.size, int(np.sum(lev32 != lev64)), st["n_rechecked"], err))
+    assert np.array_equal(lev32, lev64)
+    assert err < 1e-5
+
+
+@gpu
 @pytest.mark.parametrize("pol", ["lh", "lv", "cl", "cr"])
 def test_spectrum_polarization(oracle_planar, gpu_planar, pol):
     res = gpu_planar.calculate_spectrum(obs=[0.001, 0.0005, 30], npoints=40, energy_range_eV=[120, 800],
