@@ -66,13 +66,15 @@ class Workload:
         # RK4 + spline on the device; level samples are rebuilt inside the step
         dev.trajectories(s._fields, [s._om_particle(s._particle)])
         pol = [0.0] * 6
-        out = {"evals": 0.0, "kernel_ms": {}, "values": {}}
+        out = {"evals": 0.0, "kernel_ms": {}, "values": {}, "levels": {}, "rechecked": {}}
         map_mode = "plain" if mode == "plain" else "point_mixed"
         for e in ([] if self.only_sweep else self.map_energies):
             v, lev, cv, _, st = dev.flux(0, self.obs_map, [e], pol, dev.conv(mode=map_mode))
             out["evals"] += st["evals"]
             out["kernel_ms"]["map_%g" % e] = st["kernel_ms"]
             out["values"]["map_%g" % e] = v
+            out["levels"]["map_%g" % e] = lev
+            out["rechecked"]["map_%g" % e] = st["n_rechecked"]
         sweep_mode = {"plain": "plain", "tiled": "tiled", "mixed": "mixed"}[mode]
         v, lev, cv, _, st = dev.flux(0, self.obs_sweep, self.e_sweep, pol, dev.conv(mode=sweep_mode))
         out["evals"] += st["evals"]
@@ -80,6 +82,8 @@ class Workload:
         out["sweep_levels"] = lev
         out["kernel_ms"]["sweep"] = st["kernel_ms"]
         out["values"]["sweep"] = v
+        out["levels"]["sweep"] = lev
+        out["rechecked"]["sweep"] = st["n_rechecked"]
         return out
 
 
@@ -294,7 +298,8 @@ def batched_share(args, particles):
         dt = time.perf_counter() - t0
         res[mode] = outs
         out[mode] = {"evals_per_s": st["evals"] / dt, "ms_per_step": 1e3 * dt, "evals": st["evals"],
-                     "unconverged": st["n_unconverged"], "waves_per_cu": best}
+                     "unconverged": st["n_unconverged"], "rechecked_fp64": st["n_rechecked"],
+                     "waves_per_cu": best}
         print("# batched %d %s: %.1f ms, %.4g evals/s" % (particles, mode, 1e3 * dt, st["evals"] / dt),
               file=sys.stderr, flush=True)
     out["rel_err_mixed_vs_fp64"] = max(rel_errors(a, b)["max_floored_1e3"] for a, b in zip(res["mixed"], res["tiled"]))
@@ -335,7 +340,7 @@ def main():
         print(json.dumps({"copy_bytes": a.numel() * 8}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+        print(json.dumps({"error": "bench.py runs on one GPU only; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
@@ -397,6 +402,9 @@ def main():
         errs = {}
         for k in res["values"]:
             errs[k] = rel_errors(res["values"][k], pres["values"][k])
+            errs[k]["level_mismatches"] = int(np.sum(res["levels"][k] != pres["levels"][k]))
+            errs[k]["points"] = int(res["levels"][k].size)
+            errs[k]["rechecked_fp64"] = int(res["rechecked"][k])
         extra["rel_err_vs_fp64"] = errs
         print("# plain: %.4g evals/s ratio %.2f errs %s" % (pvalue, value / pvalue, errs), file=sys.stderr,
               flush=True)

@@ -143,7 +143,10 @@ int om_power_density(om_context* ctx, int particle, const double* obs_xyz, long 
  * trajectory lengths.  Levels 0..opts->max_level of every particle are stored
  * as sample rows; deeper levels are interpolated in the kernel, so every point
  * follows the conv's level rule; points unconverged at its last level are
- * counted in stats->n_unconverged. */
+ * counted in stats->n_unconverged.  MIXED mode: points whose stop decision lay
+ * within the FP32 sin/cos error bound are recomputed in FP64 in the same run
+ * (stats->n_rechecked counts the work units, of up to 64 energies, that held
+ * such points), so stopping levels match the FP64 kernel. */
 typedef struct {
   long n_obs;
   const double* obs_xyz;     /* 3 per point */
@@ -163,8 +166,9 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
  * sums, replayed from a captured HIP graph; out[r] receives request r's grid
  * (n_obs x n_e, energy fastest). */
 int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out, om_stats* stats);
-/* Times the persistent kernel on the first sample_items work units for each
- * candidate waves-per-CU value and keeps the fastest (returned in *best). */
+/* Times the persistent kernel on the first max(sample_items, 16 x CUs x the
+ * largest candidate) work units for each candidate waves-per-CU value and
+ * keeps the fastest (returned in *best). */
 int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_items, int* best, double* ms_out);
 
 /* ---- reverse pass (adjoint) ------------------------------------------------

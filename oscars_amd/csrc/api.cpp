@@ -134,6 +134,8 @@ struct BatchArgs {
   long total_items;
   unsigned long long* n_unconverged;
   unsigned long long* evals;
+  unsigned long long* fix_list;
+  unsigned long long* n_fix;
 };
 }  // namespace om
 
@@ -156,6 +158,7 @@ hipError_t om_launch_tile_list(const om::State* st, long n_obs, long n_omega, in
                                hipStream_t s);
 hipError_t om_launch_point_list(const om::State* st, long npts, int La, int* list, int* count, hipStream_t s);
 hipError_t om_launch_border_list(const om::State* st, long npts, int* list, int* count, hipStream_t s);
+hipError_t om_launch_lw_wave(const om::LwArgs* A, hipStream_t s);
 hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s);
 hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s);
 hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s);
@@ -258,7 +261,7 @@ struct om_context {
     DevBuf<int> req_of;
     DevBuf<long> prefix;
     DevBuf<om::ReqDev> req;
-    DevBuf<unsigned long long> counters;
+    DevBuf<unsigned long long> counters, fix;
     om::BatchArgs args;
     hipGraphExec_t exec[2] = {nullptr, nullptr};  // [without, with] trajectory recomputation
     void drop_graphs() {
@@ -271,7 +274,7 @@ struct om_context {
     void release() {
       drop_graphs();
       rows.release(); tspan.release(); obs.release(); omega.release(); out.release(); weight.release();
-      req_of.release(); prefix.release(); req.release(); counters.release();
+      req_of.release(); prefix.release(); req.release(); counters.release(); fix.release();
       ready = false;
     }
   } batch;
@@ -655,6 +658,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     int Lb = stop_ext < kListFrom - 1 ? stop_ext : kListFrom - 1;
     long ntiles = 0;
     long n_recheck = 0;
+    int Ldone = 0;  // deepest level summed so far (its samples exist)
     while (true) {
       ensure_levels(ctx, ip, Lb, stop_mem);
       fill_levels(ctx, ip, Lb, A.lv);
@@ -674,6 +678,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       float ms = 0;
       check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
       kms += ms;
+      Ldone = Lb;
       if (Lb >= stop_ext) break;
       La = Lb + 1;
       Lb = La;
@@ -692,8 +697,9 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       A.n_list = nt;
     }
     // Mixed path: points whose stop decision lay within the FP32 error bound are
-    // recomputed from level 0 with the FP64 per-point kernel (their state was
-    // reset when listed), so every point stops where the FP64 path stops.
+    // recomputed from level 0 in FP64, one wave per point (their state was
+    // reset when listed), so every point stops where the FP64 path stops: one
+    // launch over the levels the main pass reached, then one per deeper level.
     if (mixed_any) {
       ctx->d_list.ensure((size_t) n_obs * ((n_e + 63) / 64) * 64);
       ctx->d_count.ensure(1);
@@ -710,14 +716,14 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
         A.list = ctx->d_list.p;
         A.n_list = nb;
         La = 0;
-        Lb = stop_ext < kListFrom - 1 ? stop_ext : kListFrom - 1;
+        Lb = Ldone;
         while (true) {
           ensure_levels(ctx, ip, Lb, stop_mem);
           fill_levels(ctx, ip, Lb, A.lv);
           A.La = La;
           A.Lb = Lb;
           check(hipEventRecord(ctx->ev0, ctx->stream), "event");
-          check(om_launch_lw_point(&A, 0, ctx->split, ctx->stream), "lw_point recheck");
+          check(om_launch_lw_wave(&A, ctx->stream), "lw_wave recheck");
           check(hipEventRecord(ctx->ev1, ctx->stream), "event");
           check(hipEventSynchronize(ctx->ev1), "sync");
           float ms = 0;
@@ -987,6 +993,9 @@ int om_power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs,
   });
 }
 
+// batch counters: work-unit counter, unconverged lanes, evaluations, borderline lanes listed
+constexpr int kBatchCounters = 4;
+
 int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int* req_of, const double* weight,
                    const double* pol6, const om_conv* conv, const om_batch_opts* opts) {
   return guarded(ctx, [&] {
@@ -1060,7 +1069,7 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     }
     upload(ctx, B.tspan, ts.data(), ts.size());
     B.rows.ensure((size_t) np * B.stride * 10);
-    B.counters.ensure(3);
+    B.counters.ensure(kBatchCounters);
     const om_particle& P = ctx->parts[0];
     om::BatchArgs& A = B.args;
     std::memset(&A, 0, sizeof(A));
@@ -1094,8 +1103,13 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     A.counter = B.counters.p;
     A.n_unconverged = B.counters.p + 1;
     A.evals = B.counters.p + 2;
+    A.n_fix = B.counters.p + 3;
     A.total_items = B.total_items;
     B.mixed = conv->mode == OM_MODE_MIXED ? 1 : 0;
+    if (B.mixed) {
+      B.fix.ensure(2 * B.total_items);  // (unit, lane mask): at most one entry per unit
+      A.fix_list = B.fix.p;
+    }
     int cus = 0;
     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "attribute");
     int wpc = (opts && opts->waves_per_cu > 0) ? opts->waves_per_cu : (ctx->batch_wpc > 0 ? ctx->batch_wpc : 8);
@@ -1133,7 +1147,7 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
       check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
       if (g) launch_trajectories(ctx, B.np);
       batch_levels(ctx);
-      check(om_launch_batch_clear(B.out.p, B.total_out, B.counters.p, 3, ctx->stream), "clear");
+      check(om_launch_batch_clear(B.out.p, B.total_out, B.counters.p, kBatchCounters, ctx->stream), "clear");
       check(om_launch_lw_batch(&B.args, B.mixed, B.groups, ctx->stream), "lw batch");
       check(hipStreamEndCapture(ctx->stream, &graph), "end capture");
       hipError_t e = hipGraphInstantiate(&B.exec[g], graph, nullptr, nullptr, 0);
@@ -1146,7 +1160,7 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
     check(hipEventSynchronize(ctx->ev1), "sync");
     float ms = 0;
     check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
-    unsigned long long cnt[3];
+    unsigned long long cnt[kBatchCounters];
     check(hipMemcpy(cnt, B.counters.p, sizeof(cnt), hipMemcpyDeviceToHost), "counters");
     if (out) {
       for (int r = 0; r < B.n_req; ++r)
@@ -1160,6 +1174,7 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
       stats->total_ms = wall.ms();
       stats->max_level = B.Lmax;
       stats->n_unconverged = (long) cnt[1];
+      stats->n_rechecked = (long) cnt[3];
     }
   });
 }
@@ -1173,12 +1188,17 @@ int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_ite
     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "attribute");
     batch_levels(ctx);
     om::BatchArgs A = B.args;
-    A.total_items = sample_items < B.total_items ? sample_items : B.total_items;
+    // every wave of the largest grid takes about 16 units, so the timing is not
+    // dominated by the tail of a few deep units
+    int cmax = 1;
+    for (int i = 0; i < n; ++i) cmax = candidates[i] > cmax ? candidates[i] : cmax;
+    long sample = sample_items > 16L * cus * cmax ? sample_items : 16L * cus * cmax;
+    A.total_items = sample < B.total_items ? sample : B.total_items;
     double best_ms = 1e300;
     int best_c = candidates[0];
     for (int i = 0; i < n; ++i) {
       if (candidates[i] < 1 || candidates[i] > 64) throw std::out_of_range("waves per CU out of range");
-      check(om_launch_batch_clear(B.out.p, B.total_out, B.counters.p, 3, ctx->stream), "clear");
+      check(om_launch_batch_clear(B.out.p, B.total_out, B.counters.p, kBatchCounters, ctx->stream), "clear");
       check(hipEventRecord(ctx->ev0, ctx->stream), "event");
       check(om_launch_lw_batch(&A, B.mixed, cus * candidates[i], ctx->stream), "lw batch");
       check(hipEventRecord(ctx->ev1, ctx->stream), "event");

@@ -91,6 +91,11 @@ struct BatchArgs {
   long total_items;
   unsigned long long* n_unconverged;
   unsigned long long* evals;  // sum over lanes of 2^(L+1)-1 (as integer)
+  // mixed path: units with lanes whose stop decision lay within the FP32 error
+  // bound, as (item, lane mask) pairs (room for every unit), recomputed in FP64
+  // by lw_batch_fix_kernel
+  unsigned long long* fix_list;
+  unsigned long long* n_fix;  // units listed
 };
 
 // Amplitude of a sample interpolated from the spline at time tt (levels above
@@ -110,6 +115,160 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return ((unsigned long long) hi << 32) | lo;
 }
 
+// One work unit (particle, observation point, 64-energy tile): the reference's
+// level loop for the lanes of lane_mask whose energy exists, weighted flux
+// added into the request's output.  Lane j computes the amplitude of sample
+// base+j and leaves it in LDS; then every lane walks those 64 samples at its
+// own energy.  MIXED: lanes whose stop decision lay within the FP32 error
+// bound add nothing; the unit is listed with their mask for lw_batch_fix_kernel.
+template <bool MIXED>
+__device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, unsigned long long lane_mask, int lane,
+                                           double* s_tau, double (*s_v)[64]) {
+  // particle of this work unit: last p with item_prefix[p] <= item
+  int lo = 0, hi = A.np;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (A.item_prefix[mid] <= item) lo = mid; else hi = mid;
+  }
+  int ip = lo;
+  long local = item - A.item_prefix[ip];
+  const ReqDev R = A.req[A.req_of[ip]];
+  long io = local / R.n_tiles_e;
+  long et = local - io * R.n_tiles_e;
+  long ie = et * 64 + lane;
+  bool valid = ie < R.n_e && ((lane_mask >> lane) & 1ULL);
+  long iec = ie < R.n_e ? ie : R.n_e - 1;
+  V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
+  double negw = -R.omega[iec];
+  double T = A.tspan[2 * ip + 1] - A.tspan[2 * ip];
+  const double* prow = A.rows + 10 * (long) ip * A.stride;
+  const double ts = A.tspan[2 * ip], te = A.tspan[2 * ip + 1];
+  const TrajDesc D = A.desc[ip];
+  double sum[6] = {0, 0, 0, 0, 0, 0};
+  double lastmag = -1, prec = -1;
+  bool active = valid;
+  bool conv = false;
+  int lev = 0;
+  double an_run = 0, dm_last = 0;
+  bool border = false;
+  for (int L = 0; L <= A.Lmax; ++L) {
+    if (!__any(active)) break;
+    const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
+    long ns = 1L << L;
+    // sample times of level L (the expressions of level_batch_kernel)
+    double p2L = (double) (1L << L);
+    double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
+    double tfirst = ts + (te - ts) / (2.0 * p2L);
+    bool stored = L <= A.Lb;
+    double ps[6] = {0, 0, 0, 0, 0, 0};
+    double maxd = 0, lastphase = 0;
+    double anl = 0;  // mixed path: sum of |V|_1 over the samples this lane computed
+    for (long base = 0; base < ns; base += 64) {
+      long si = base + lane;
+      if (si < ns) {
+        Amp a;
+        if (stored) {
+          a = amplitude(obs, rows + 10 * si);
+        } else {
+          a = interpolated_amplitude(A, D, obs, tfirst + spacing * (double) si);
+        }
+        s_tau[lane] = a.tau;
+        s_v[0][lane] = a.V.x;
+        s_v[1][lane] = a.V.y;
+        s_v[2][lane] = a.V.z;
+        if (MIXED) anl += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);
+      }
+      wave_sync();
+      int cnt = (int) ((ns - base) < 64 ? (ns - base) : 64);
+      if (base == 0) lastphase = negw * s_tau[0];
+      int k = 0;
+      for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          double phi = negw * s_tau[k + u];
+          maxd = fmax(maxd, fabs(phi - lastphase));
+          lastphase = phi;
+          double sn, cs;
+          sincos_mode(MIXED, phi, sn, cs);
+          double vx = s_v[0][k + u], vy = s_v[1][k + u], vz = s_v[2][k + u];
+          ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+          ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+          ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+        }
+      }
+      for (; k < cnt; ++k) {
+        double phi = negw * s_tau[k];
+        maxd = fmax(maxd, fabs(phi - lastphase));
+        lastphase = phi;
+        double sn, cs;
+        sincos_mode(MIXED, phi, sn, cs);
+        double vx = s_v[0][k], vy = s_v[1][k], vz = s_v[2][k];
+        ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+        ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+        ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+      }
+      wave_sync();
+    }
+    if (MIXED) {
+      // every lane walked the same samples: the level's |V|_1 sum is wave-uniform
+      for (int off = 32; off > 0; off >>= 1) anl += __shfl_xor(anl, off);
+    }
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sum[k] += ps[k];
+      lev = L;
+      // convergence test, src/OSCARSSR.cc:4227-4239
+      double E[6];
+      double dt = T / (2.0 * (double) (1L << L) - 1);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) E[k] = sum[k] * dt;
+      if (A.use_pol) pol_project(E, A.pol);
+      double mag = cmag2(E);
+      double mag_prev = lastmag;
+      prec = fabs(mag - lastmag) / lastmag;
+      bool c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
+      lastmag = mag;
+      if (MIXED) {
+        an_run += anl;
+        border = border || level_borderline(L, dt, A.precision, mag_prev, mag, prec, maxd, an_run, dm_last);
+      }
+      if (c) {
+        conv = true;
+        active = false;
+      }
+    }
+  }
+  // mixed path: the unit is listed once, with the mask of its borderline lanes
+  // (all lanes take part in the atomic; lanes 0 and 1 store the entry)
+  bool fixed = false;
+  if (MIXED) {
+    bool want = valid && border;
+    unsigned long long m = __ballot(want);
+    unsigned long long slot = atomicAdd(A.n_fix, (lane == 0 && m != 0ULL) ? 1ULL : 0ULL);
+    unsigned long long j = uniform_u64(slot);
+    if (m != 0ULL && lane < 2) A.fix_list[2 * j + lane] = lane == 0 ? (unsigned long long) item : m;
+    fixed = want;
+  }
+  if (valid && !fixed) {
+    double dt = T / (2.0 * (double) (1L << lev) - 1);
+    double f = A.C0 * dt;
+    double E[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) E[k] = sum[k] * f;
+    if (A.use_pol) pol_project(E, A.pol);
+    double flux = A.C2 * cmag2(E) * A.weight[ip];
+    atomicAdd(&R.out[io * R.n_e + ie], flux);
+  }
+  // bookkeeping: lanes that stopped without converging, and evaluations done
+  // (all lanes take part; only lane 0 adds a nonzero amount)
+  bool counted = valid && !fixed;
+  unsigned long long nc = __ballot(counted && !conv);
+  unsigned long long ev = counted ? (unsigned long long) ((2L << lev) - 1) : 0ULL;
+  for (int off = 32; off > 0; off >>= 1) ev += __shfl_down(ev, off);
+  atomicAdd(A.n_unconverged, lane == 0 ? (unsigned long long) __popcll(nc) : 0ULL);
+  atomicAdd(A.evals, lane == 0 ? ev : 0ULL);
+}
+
 // Four waves per SIMD: the register budget is set for the hot stored-row path;
 // the rare interpolation path may spill.
 template <bool MIXED>
@@ -126,7 +285,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw
     unsigned long long mine = atomicAdd(A.counter, lane == 0 ? 1ULL : 0ULL);
     long item = (long) uniform_u64(mine);
     if (item >= A.total_items) break;
-    // particle of this work unit: last p with item_prefix[p] <= item
+    batch_unit<MIXED>(A, item, ~0ULL, lane, s_tau, s_v);
+  }
+}
+
+// FP64 recomputation of the lanes the mixed batch kernel listed: for each
+// listed unit, each lane of its mask (one energy) is summed by a whole wave,
+// the 64 lanes splitting each level's samples (a few points per unit: a wave
+// per point costs about 1/64 of re-walking the unit).  Levels, convergence
+// test and weighting are the FP64 kernel's.  A grid-stride loop over the
+// listed count (read from device memory, so the launch can sit in a captured
+// graph); each lane adds the flux of its own energy at the end of the unit.
+__global__ void __launch_bounds__(64) lw_batch_fix_kernel(BatchArgs A) {
+  int lane = threadIdx.x;
+  long n = (long) uniform_u64(*A.n_fix);
+  for (long e = blockIdx.x; e < n; e += gridDim.x) {
+    long item = (long) uniform_u64(A.fix_list[2 * e]);
+    unsigned long long mask = uniform_u64(A.fix_list[2 * e + 1]);
     int lo = 0, hi = A.np;
     while (hi - lo > 1) {
       int mid = (lo + hi) >> 1;
@@ -137,79 +312,37 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw
     const ReqDev R = A.req[A.req_of[ip]];
     long io = local / R.n_tiles_e;
     long et = local - io * R.n_tiles_e;
-    long ie = et * 64 + lane;
-    bool valid = ie < R.n_e;
-    long iec = valid ? ie : R.n_e - 1;
     V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
-    double negw = -R.omega[iec];
-    double T = A.tspan[2 * ip + 1] - A.tspan[2 * ip];
-    const double* prow = A.rows + 10 * (long) ip * A.stride;
     const double ts = A.tspan[2 * ip], te = A.tspan[2 * ip + 1];
+    const double T = te - ts;
+    const double* prow = A.rows + 10 * (long) ip * A.stride;
     const TrajDesc D = A.desc[ip];
-    double sum[6] = {0, 0, 0, 0, 0, 0};
-    double lastmag = -1, prec = -1;
-    bool active = valid;
-    bool conv = false;
-    int lev = 0;
-    for (int L = 0; L <= A.Lmax; ++L) {
-      if (!__any(active)) break;
-      const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
-      long ns = 1L << L;
-      // sample times of level L (the expressions of level_batch_kernel)
-      double p2L = (double) (1L << L);
-      double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
-      double tfirst = ts + (te - ts) / (2.0 * p2L);
-      bool stored = L <= A.Lb;
-      double ps[6] = {0, 0, 0, 0, 0, 0};
-      double maxd = 0, lastphase = 0;
-      for (long base = 0; base < ns; base += 64) {
-        long si = base + lane;
-        if (si < ns) {
-          Amp a;
-          if (stored) {
-            a = amplitude(obs, rows + 10 * si);
-          } else {
-            a = interpolated_amplitude(A, D, obs, tfirst + spacing * (double) si);
-          }
-          s_tau[lane] = a.tau;
-          s_v[0][lane] = a.V.x;
-          s_v[1][lane] = a.V.y;
-          s_v[2][lane] = a.V.z;
+    double my_flux = 0;
+    unsigned long long n_conv = 0, ev = 0;
+    for (unsigned long long m = mask; m != 0ULL; m &= m - 1ULL) {
+      int b = __builtin_ctzll(m);
+      long ie = et * 64 + b;
+      double negw = -R.omega[ie];
+      double sum[6] = {0, 0, 0, 0, 0, 0};
+      double lastmag = -1, prec = -1;
+      int lev = 0;
+      bool c = false;
+      for (int L = 0; L <= A.Lmax && !c; ++L) {
+        long ns = 1L << L;
+        double p2L = (double) (1L << L);
+        double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
+        double tfirst = ts + (te - ts) / (2.0 * p2L);
+        const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
+        double tot[6], maxd;
+        if (L <= A.Lb) {
+          wave_level_sum(ns, negw, lane, [&](long i) { return amplitude(obs, rows + 10 * i); }, tot, maxd);
+        } else {
+          wave_level_sum(ns, negw, lane,
+                         [&](long i) { return interpolated_amplitude(A, D, obs, tfirst + spacing * (double) i); },
+                         tot, maxd);
         }
-        wave_sync();
-        int cnt = (int) ((ns - base) < 64 ? (ns - base) : 64);
-        if (base == 0) lastphase = negw * s_tau[0];
-        int k = 0;
-        for (; k + 4 <= cnt; k += 4) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            double phi = negw * s_tau[k + u];
-            maxd = fmax(maxd, fabs(phi - lastphase));
-            lastphase = phi;
-            double sn, cs;
-            sincos_mode(MIXED, phi, sn, cs);
-            double vx = s_v[0][k + u], vy = s_v[1][k + u], vz = s_v[2][k + u];
-            ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
-            ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
-            ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
-          }
-        }
-        for (; k < cnt; ++k) {
-          double phi = negw * s_tau[k];
-          maxd = fmax(maxd, fabs(phi - lastphase));
-          lastphase = phi;
-          double sn, cs;
-          sincos_mode(MIXED, phi, sn, cs);
-          double vx = s_v[0][k], vy = s_v[1][k], vz = s_v[2][k];
-          ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
-          ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
-          ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
-        }
-        wave_sync();
-      }
-      if (active) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) sum[k] += ps[k];
+        for (int k = 0; k < 6; ++k) sum[k] += tot[k];
         lev = L;
         // convergence test, src/OSCARSSR.cc:4227-4239
         double E[6];
@@ -219,15 +352,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw
         if (A.use_pol) pol_project(E, A.pol);
         double mag = cmag2(E);
         prec = fabs(mag - lastmag) / lastmag;
-        bool c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
+        c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
         lastmag = mag;
-        if (c) {
-          conv = true;
-          active = false;
-        }
       }
-    }
-    if (valid) {
       double dt = T / (2.0 * (double) (1L << lev) - 1);
       double f = A.C0 * dt;
       double E[6];
@@ -235,14 +362,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw
       for (int k = 0; k < 6; ++k) E[k] = sum[k] * f;
       if (A.use_pol) pol_project(E, A.pol);
       double flux = A.C2 * cmag2(E) * A.weight[ip];
-      atomicAdd(&R.out[io * R.n_e + ie], flux);
+      my_flux = lane == b ? flux : my_flux;
+      n_conv += c ? 1ULL : 0ULL;
+      ev += (unsigned long long) ((2L << lev) - 1);
     }
-    // bookkeeping: lanes that stopped without converging, and evaluations done
-    // (all lanes take part; only lane 0 adds a nonzero amount)
-    unsigned long long nc = __ballot(valid && !conv);
-    unsigned long long ev = valid ? (unsigned long long) ((2L << lev) - 1) : 0ULL;
-    for (int off = 32; off > 0; off >>= 1) ev += __shfl_down(ev, off);
-    atomicAdd(A.n_unconverged, lane == 0 ? (unsigned long long) __popcll(nc) : 0ULL);
+    if ((mask >> lane) & 1ULL) atomicAdd(&R.out[io * R.n_e + et * 64 + lane], my_flux);
+    unsigned long long nu = (unsigned long long) __popcll(mask) - n_conv;
+    atomicAdd(A.n_unconverged, lane == 0 ? nu : 0ULL);
     atomicAdd(A.evals, lane == 0 ? ev : 0ULL);
   }
 }
@@ -277,6 +403,7 @@ extern "C" hipError_t om_launch_batch_clear(double* out, long n, unsigned long l
 extern "C" hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s) {
   if (mixed) {
     hipLaunchKernelGGL(om::lw_batch_kernel<true>, dim3(n_groups), dim3(64), 0, s, *A);
+    hipLaunchKernelGGL(om::lw_batch_fix_kernel, dim3(n_groups), dim3(64), 0, s, *A);
   } else {
     hipLaunchKernelGGL(om::lw_batch_kernel<false>, dim3(n_groups), dim3(64), 0, s, *A);
   }

@@ -83,26 +83,6 @@ __device__ __forceinline__ bool level_converged(const LwArgs& A, int L, const do
   return c;
 }
 
-// Mixed path: bound of the error the FP32 sin/cos puts into this level's
-// decision.  Each FP32 cos/sin is within kSinCos32Err of the exact value, so
-// every real component of the sum is off by at most kSinCos32Err * an, with
-// an = sum over the samples so far of |V|_1; E = sum*dt, the polarization
-// projection does not increase the norm, and |E|^2 moves by at most
-// dm = 2 |E| d + d^2 with d = sqrt(6) kSinCos32Err an dt.  The relative change
-// prec then moves by at most (dm + dm_previous)/lastmag (1 + prec); a decision
-// whose margin |prec - precision| is within twice that is "borderline" (the
-// FP64 path might decide the other way) and the point is recomputed in FP64.
-constexpr double kSinCos32Err = 2.5e-7;
-
-__device__ __forceinline__ bool level_borderline(const LwArgs& A, int L, double mag_prev, double mag, double prec,
-                                                 double maxd, double an, double& dm_last) {
-  double d = 2.449489742783178 * kSinCos32Err * an * A.lv.dt_incl[L];
-  double dm = 2.0 * sqrt(mag) * d + d * d;
-  double b = 2.0 * (dm + dm_last) / mag_prev * (1.0 + prec);
-  dm_last = dm;
-  return L > 8 && maxd < kPi && fabs(prec - A.precision) <= b;
-}
-
 // ---------------------------------------------------------------- level-slice combination
 // A workgroup of S waves serves 64 output points; wave w sums the contiguous
 // slice [ns*w/S, ns*(w+1)/S) of each level.  Each wave leaves its partial
@@ -231,7 +211,8 @@ __global__ void __launch_bounds__(64 * S) lw_point_kernel(LwArgs A) {
       bool c = level_converged(A, L, sum, maxd, lastmag, prec);
       if (MIXED) {
         an_run += an;
-        border = border || level_borderline(A, L, mag_prev, lastmag, prec, maxd, an_run, dm_last);
+        border = border ||
+                 level_borderline(L, A.lv.dt_incl[L], A.precision, mag_prev, lastmag, prec, maxd, an_run, dm_last);
       }
       if (c) {
         A.st.conv[pc] = 1;
@@ -376,7 +357,8 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
       bool c = level_converged(A, L, sum, maxd, lastmag, prec);
       if (MIXED) {
         an_run += an;
-        border = border || level_borderline(A, L, mag_prev, lastmag, prec, maxd, an_run, dm_last);
+        border = border ||
+                 level_borderline(L, A.lv.dt_incl[L], A.precision, mag_prev, lastmag, prec, maxd, an_run, dm_last);
       }
       if (c) {
         A.st.conv[p] = 1;
@@ -528,6 +510,45 @@ __global__ void __launch_bounds__(256) lw_finalize_kernel(FinArgs F) {
   }
   if (F.out_level) F.out_level[p] = L;
   if (F.out_conv) F.out_conv[p] = (unsigned char) conv;
+}
+
+// FP64 recomputation of listed points (the mixed path's borderline points):
+// one wave per point, the lanes splitting each level's samples, levels La..Lb
+// from the point's state.  Four waves (points) per workgroup.
+__global__ void __launch_bounds__(256) lw_wave_kernel(LwArgs A) {
+  int lane = threadIdx.x & 63;
+  long g = (long) blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= A.n_list) return;  // whole waves leave; no workgroup barrier below
+  long p = A.list[g];
+  if (A.st.conv[p] || A.st.level[p] != A.La - 1) return;
+  long io = p / A.n_omega;
+  long iw = p - io * A.n_omega;
+  V3 obs = v3(A.obs[3 * io], A.obs[3 * io + 1], A.obs[3 * io + 2]);
+  double negw = -A.omega[iw];
+  double sum[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sum[k] = A.st.sum[6 * p + k];
+  double lastmag = A.st.lastmag[p];
+  double prec = A.st.prec[p];
+  int lev = A.La - 1;
+  bool c = false;
+  for (int L = A.La; L <= A.Lb && !c; ++L) {
+    const double* __restrict__ rows = A.lv.rows[L];
+    double tot[6], maxd;
+    wave_level_sum(1L << L, negw, lane, [&](long i) { return amplitude(obs, rows + 10 * i); }, tot, maxd);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sum[k] += tot[k];
+    lev = L;
+    c = level_converged(A, L, sum, maxd, lastmag, prec);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) A.st.sum[6 * p + k] = sum[k];
+    A.st.lastmag[p] = lastmag;
+    A.st.prec[p] = prec;
+    A.st.level[p] = lev;
+    if (c) A.st.conv[p] = 1;
+  }
 }
 
 __global__ void __launch_bounds__(256) state_reset_kernel(State st, long npts) {
@@ -707,6 +728,12 @@ extern "C" hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, i
       default: hipLaunchKernelGGL((om::lw_tile_kernel<false, 8>), g, dim3(512), 0, s, *A, 1L); break;
     }
   }
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_lw_wave(const om::LwArgs* A, hipStream_t s) {
+  long blocks = (A->n_list + 3) / 4;
+  if (blocks > 0) hipLaunchKernelGGL(om::lw_wave_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *A);
   return hipGetLastError();
 }
 

@@ -88,4 +88,69 @@ __device__ __forceinline__ void sincos_mode(bool mixed, double phi, double& sn, 
   }
 }
 
+// One level of one point's sum in FP64 with the 64 lanes of a wave splitting
+// the samples (lane j takes samples j, j+64, ...; amp_at(i) gives sample i's
+// amplitude).  The largest phase step between consecutive samples takes the
+// previous sample's phase from the neighbouring lane.  The partial sums are
+// combined by a butterfly, so every lane ends with the same totals (pairs add
+// the same two values).  Used to recompute few points: a whole wave per point
+// keeps their latency low.
+template <class AmpAt>
+__device__ __forceinline__ void wave_level_sum(long ns, double negw, int lane, AmpAt amp_at, double tot[6],
+                                               double& maxd) {
+  double ps[6] = {0, 0, 0, 0, 0, 0};
+  double md = 0, prev_last = 0;
+  for (long base = 0; base < ns; base += 64) {
+    long i = base + lane;
+    bool ok = i < ns;
+    double phi = 0;
+    if (ok) {
+      Amp a = amp_at(i);
+      phi = negw * a.tau;
+      double sn, cs;
+      sincos_huge64(phi, &sn, &cs);
+      ps[0] = __builtin_fma(a.V.x, cs, ps[0]); ps[1] = __builtin_fma(a.V.x, sn, ps[1]);
+      ps[2] = __builtin_fma(a.V.y, cs, ps[2]); ps[3] = __builtin_fma(a.V.y, sn, ps[3]);
+      ps[4] = __builtin_fma(a.V.z, cs, ps[4]); ps[5] = __builtin_fma(a.V.z, sn, ps[5]);
+    }
+    double prev = __shfl_up(phi, 1);
+    if (lane == 0) prev = base == 0 ? phi : prev_last;
+    if (ok) md = fmax(md, fabs(phi - prev));
+    prev_last = __shfl(phi, 63);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ps[k] += __shfl_xor(ps[k], off);
+    md = fmax(md, __shfl_xor(md, off));
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) tot[k] = ps[k];
+  maxd = md;
+}
+
+// Mixed path: bound of the error the FP32 sin/cos puts into a level's stop
+// decision.  The FP32 cos/sin of the exactly reduced phase is within
+// kSinCos32Err of the exact value (largest error over 1e8 reduced arguments in
+// [-pi/4, pi/4]: 9.35e-8, for cos near 0; the bound adds margin).  Each real
+// component of the sum, Re/Im of sum_i V_k,i e^{i phi}, is then off by at most
+// kSinCos32Err * sum_i |V_k,i|, so the complex vector E = sum * dt is off by
+// d <= sqrt(2) kSinCos32Err an dt, an = sum over the samples so far of |V|_1
+// (the polarization projection does not increase the norm), and |E|^2 by
+// dm <= 2 |E| d + 3 d^2 (|E| from the mixed sum, within d of the exact one).
+// prec = |mag - lastmag| / lastmag then moves by at most
+// (dm + dm_last (1 + prec)) / (lastmag - dm_last).  A decision whose margin
+// |prec - precision| is within 1.25 times that bound is "borderline" (the FP64
+// path might decide the other way); such points are recomputed in FP64.
+constexpr double kSinCos32Err = 1.2e-7;
+
+__device__ __forceinline__ bool level_borderline(int L, double dt, double precision, double mag_prev, double mag,
+                                                 double prec, double maxd, double an, double& dm_last) {
+  double d = 1.4142135623730951 * kSinCos32Err * an * dt;
+  double dm = 2.0 * sqrt(mag) * d + 3.0 * d * d;
+  double den = mag_prev - dm_last;
+  double b = 1.25 * (dm + dm_last * (1.0 + prec)) / den;
+  dm_last = dm;
+  return L > 8 && maxd < kPi && (den <= 0.0 || fabs(prec - precision) <= b);
+}
+
 }  // namespace om

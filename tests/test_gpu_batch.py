@@ -118,3 +118,29 @@ def test_sr_nparticles_batched_matches_particle_loop():
     e = rel_err(a, c, 1e-6)
     print("batched vs loop: %.3e" % e)
     assert e < 1e-9
+
+
+@gpu
+def test_batched_mixed_levels_match_fp64():
+    """Mixed batch against the FP64 batch on a denser case: identical levels
+    (equal evaluation counts: borderline lanes are recomputed in FP64 inside
+    the same graph) and per-point agreement to 1e-5."""
+    from oscars_amd import multiparticle as M
+    from oscars_amd.sr import sr
+    parts, windows, _, _, _ = scenario(n_particles=24, seed=7)
+    grid = np.array([[x, y, 30.0] for x in np.linspace(-0.002, 0.002, 5) for y in np.linspace(-0.001, 0.001, 5)])
+    reqs = [(grid, np.linspace(140.0, 600.0, 256)), (grid[::3], np.linspace(440.0, 470.0, 100))]
+    req_of, weights = M.assign_round_robin(len(parts), len(reqs))
+    s = sr()
+    s.add_bfield_undulator(**UNDULATOR)
+    dev = s._device()
+    res = {}
+    for mode in ("tiled", "mixed"):
+        bf = M.BatchedFlux(dev, s._fields, parts, windows, reqs, req_of, weights, mode=mode)
+        res[mode] = bf.run(recompute_trajectories=False)
+    (o64, s64), (o32, s32) = res["tiled"], res["mixed"]
+    errs = [rel_err(a, b, 1e-6) for a, b in zip(o32, o64)]
+    print("mixed batch: evals %g (fp64 %g), units rechecked %d, max rel err %.3e"
+          % (s32["evals"], s64["evals"], s32["n_rechecked"], max(errs)))
+    assert s32["evals"] == s64["evals"]
+    assert max(errs) < 1e-5
