@@ -143,7 +143,8 @@ int om_power_density(om_context* ctx, int particle, const double* obs_xyz, long 
  * trajectory lengths.  Levels 0..opts->max_level of every particle are stored
  * as sample rows; deeper levels are interpolated in the kernel, so every point
  * follows the conv's level rule; points unconverged at its last level are
- * counted in stats->n_unconverged.  MIXED mode: points whose stop decision lay
+ * counted in stats->n_unconverged.  Points still active at level 18 are
+ * finished one wave per point (their samples split over the wave's lanes).  MIXED mode: points whose stop decision lay
  * within the FP32 sin/cos error bound are recomputed in FP64 in the same run
  * (stats->n_rechecked counts the work units, of up to 64 energies, that held
  * such points), so stopping levels match the FP64 kernel. */
@@ -156,8 +157,8 @@ typedef struct {
 
 typedef struct {
   int waves_per_cu; /* persistent grid = CUs x this; 0: the tuned value (8 before any tuning) */
-  int max_level;    /* highest level held as stored sample rows (0: 14); deeper levels, up to
-                       the conv's max_level rule, are interpolated inside the sum kernel */
+  int max_level;    /* highest level held as stored sample rows (0: up to 17 within 16 GiB);
+                       deeper levels, up to the conv's max_level rule, are interpolated */
 } om_batch_opts;
 
 int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int* req_of, const double* weight,

@@ -20,124 +20,9 @@
 
 #include "../../include/oscars_mi.h"
 #include "adj_types.h"
+#include "launch_types.h"
 #include "om_types.h"
 
-namespace om {
-struct Levels {
-  const double* rows[kMaxLevel + 1];
-  double dt_incl[kMaxLevel + 1];
-  double beta_diff[kMaxLevel + 1];
-};
-struct State {
-  double* sum;
-  double* err;
-  double* lastmag;
-  double* prec;
-  int* level;
-  int* conv;
-  int* border;
-};
-struct LwArgs {
-  Levels lv;
-  State st;
-  int La, Lb;
-  long npts;
-  const double* obs;
-  const double* omega;
-  long n_omega;
-  double pol[6];
-  int use_pol;
-  double precision;
-  const int* list;
-  long n_list;
-};
-struct PdArgs {
-  Levels lv;
-  State st;
-  int La, Lb;
-  long npts;
-  const double* obs;
-  double normal[3];
-  int directional;
-  double precision;
-  double two_over_gamma;
-};
-struct FinArgs {
-  State st;
-  long npts;
-  int kind;
-  int quantity;
-  double C0, C2;
-  double pd_const;
-  int directional;
-  double dt_incl[kMaxLevel + 1];
-  double pol[6];
-  int use_pol;
-  double weight;
-  int accumulate;
-  double* out;
-  double* out_comp;
-  double* out_e;
-  int* out_level;
-  unsigned char* out_conv;
-};
-struct LevelJob {
-  const double* t;
-  const double* p;
-  const double* ypp;
-  long knot_offset;
-  long n;
-  double tstart, tstop;
-  double* out;
-  long npts;
-  double t_first, spacing;
-};
-struct LevelJobBatch {
-  const double* t;
-  const double* p;
-  const double* ypp;
-  const TrajDesc* desc;
-  const double* tspan;
-  double* rows;
-  long stride;
-  int np;
-  int Lcap;
-  int stop_mem;
-};
-struct ReqDev {
-  const double* obs;
-  const double* omega;
-  double* out;
-  long n_obs, n_e, n_tiles_e;
-};
-struct BatchArgs {
-  const double* rows;
-  long stride;
-  int Lb;
-  int Lmax;
-  int stop_mem;
-  const double* t;
-  const double* p;
-  const double* ypp;
-  const TrajDesc* desc;
-  const double* tspan;
-  const int* req_of;
-  const double* weight;
-  const long* item_prefix;
-  int np;
-  const ReqDev* req;
-  double C0, C2;
-  double pol[6];
-  int use_pol;
-  double precision;
-  unsigned long long* counter;
-  long total_items;
-  unsigned long long* n_unconverged;
-  unsigned long long* evals;
-  unsigned long long* fix_list;
-  unsigned long long* n_fix;
-};
-}  // namespace om
 
 extern "C" {
 hipError_t om_launch_level_batch(const om::LevelJobBatch* J, hipStream_t s);
@@ -262,6 +147,11 @@ struct om_context {
     DevBuf<long> prefix;
     DevBuf<om::ReqDev> req;
     DevBuf<unsigned long long> counters, fix;
+    DevBuf<double> ws;  // per persistent wave: the state of one unit's energies
+    DevBuf<int> wsi;
+    DevBuf<unsigned long long> deep_code;  // points handed to the one-wave-per-point kernel
+    DevBuf<double> deep_state;
+    DevBuf<int> deep_level, deep_flags;
     om::BatchArgs args;
     hipGraphExec_t exec[2] = {nullptr, nullptr};  // [without, with] trajectory recomputation
     void drop_graphs() {
@@ -275,6 +165,8 @@ struct om_context {
       drop_graphs();
       rows.release(); tspan.release(); obs.release(); omega.release(); out.release(); weight.release();
       req_of.release(); prefix.release(); req.release(); counters.release(); fix.release();
+      ws.release(); wsi.release();
+      deep_code.release(); deep_state.release(); deep_level.release(); deep_flags.release();
       ready = false;
     }
   } batch;
@@ -993,8 +885,25 @@ int om_power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs,
   });
 }
 
-// batch counters: work-unit counter, unconverged lanes, evaluations, borderline lanes listed
-constexpr int kBatchCounters = 4;
+// batch counters: work-unit counter, unconverged points, evaluations, borderline points listed,
+// points handed to the one-wave-per-point kernel
+constexpr int kBatchCounters = 5;
+constexpr long kBatchFixCap = 1L << 27;
+constexpr long kBatchDeepCap = 1L << 22;
+constexpr int kBatchDeepLevel = 18;          // levels from here on: one wave per point
+constexpr int kBatchRowsCap = 17;            // highest level stored as rows by default ...
+constexpr double kBatchRowsBytes = 16.0 * (1L << 30);  // ... within this much memory
+
+namespace {
+// Per-wave scratch of the persistent batch kernel for a grid of `groups` waves.
+void batch_scratch(om_context* ctx, long groups, om::BatchArgs& A) {
+  auto& B = ctx->batch;
+  B.ws.ensure((size_t) groups * om::kBatchWsD * om::kBatchEB);
+  B.wsi.ensure((size_t) groups * om::kBatchWsI * om::kBatchEB);
+  A.ws = B.ws.p;
+  A.wsi = B.wsi.p;
+}
+}  // namespace
 
 int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int* req_of, const double* weight,
                    const double* pol6, const om_conv* conv, const om_batch_opts* opts) {
@@ -1014,9 +923,11 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     }
     int stop_mem, stop_ext;
     level_stops(*conv, stop_mem, stop_ext);
-    int cap = (opts && opts->max_level > 0) ? opts->max_level : 14;
+    int cap = (opts && opts->max_level > 0) ? opts->max_level : kBatchRowsCap;
     if (cap > 20) throw std::out_of_range("batch rows above level 20 are not supported");
     if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
+    if (!(opts && opts->max_level > 0))
+      while (cap > 10 && (double) np * (double) ((2L << cap) - 1) * 80.0 > kBatchRowsBytes) --cap;
     B.Lb = stop_ext < cap ? stop_ext : cap;
     B.Lmax = stop_ext;
     B.stop_mem = stop_mem;
@@ -1050,13 +961,15 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
       rd[r].out = B.out.p + B.out_off[r];
       rd[r].n_obs = req[r].n_obs;
       rd[r].n_e = req[r].n_e;
-      rd[r].n_tiles_e = (req[r].n_e + 63) / 64;
+      rd[r].n_blocks_e = (req[r].n_e + om::kBatchEB - 1) / om::kBatchEB;
     }
     upload(ctx, B.req, rd.data(), n_req);
     std::vector<long> prefix(np + 1, 0);
+    long total_points = 0;
     for (int i = 0; i < np; ++i) {
       const om::ReqDev& R = rd[req_of[i]];
-      prefix[i + 1] = prefix[i] + R.n_obs * R.n_tiles_e;
+      prefix[i + 1] = prefix[i] + R.n_obs * R.n_blocks_e;
+      total_points += R.n_obs * R.n_e;
     }
     B.total_items = prefix[np];
     upload(ctx, B.prefix, prefix.data(), prefix.size());
@@ -1104,16 +1017,31 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     A.n_unconverged = B.counters.p + 1;
     A.evals = B.counters.p + 2;
     A.n_fix = B.counters.p + 3;
+    A.n_deep = B.counters.p + 4;
+    A.Ldeep = kBatchDeepLevel;
+    A.deep_cap = total_points < kBatchDeepCap ? total_points : kBatchDeepCap;
+    B.deep_code.ensure(A.deep_cap);
+    B.deep_state.ensure((size_t) om::kBatchWsD * A.deep_cap);
+    B.deep_level.ensure(A.deep_cap);
+    B.deep_flags.ensure(A.deep_cap);
+    A.deep_code = B.deep_code.p;
+    A.deep_state = B.deep_state.p;
+    A.deep_level = B.deep_level.p;
+    A.deep_flags = B.deep_flags.p;
     A.total_items = B.total_items;
     B.mixed = conv->mode == OM_MODE_MIXED ? 1 : 0;
     if (B.mixed) {
-      B.fix.ensure(2 * B.total_items);  // (unit, lane mask): at most one entry per unit
+      // room for every point up to kBatchFixCap; points flagged beyond it keep the mixed result
+      A.fix_cap = total_points < kBatchFixCap ? total_points : kBatchFixCap;
+      B.fix.ensure(A.fix_cap);
       A.fix_list = B.fix.p;
     }
     int cus = 0;
     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "attribute");
     int wpc = (opts && opts->waves_per_cu > 0) ? opts->waves_per_cu : (ctx->batch_wpc > 0 ? ctx->batch_wpc : 8);
+    if (wpc > 64) throw std::out_of_range("waves per CU out of range");
     B.groups = cus * wpc;
+    batch_scratch(ctx, B.groups, A);
     B.ready = true;
   });
 }
@@ -1194,6 +1122,10 @@ int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_ite
     for (int i = 0; i < n; ++i) cmax = candidates[i] > cmax ? candidates[i] : cmax;
     long sample = sample_items > 16L * cus * cmax ? sample_items : 16L * cus * cmax;
     A.total_items = sample < B.total_items ? sample : B.total_items;
+    if (cmax > 64) throw std::out_of_range("waves per CU out of range");
+    batch_scratch(ctx, (long) cus * cmax, A);
+    B.args.ws = A.ws;
+    B.args.wsi = A.wsi;
     double best_ms = 1e300;
     int best_c = candidates[0];
     for (int i = 0; i < n; ++i) {

@@ -3,12 +3,13 @@
 // Every particle of the resident trajectory set contributes weight * flux to
 // one request (an observation grid x an energy list); requests may differ in
 // grid size and energy count, and particles in trajectory length.  The work
-// unit is (particle, observation point, 64-energy tile); a persistent grid of
-// single-wave workgroups pulls units from a device counter (a vector atomic on
-// global memory), runs the reference's level loop for its 64 energy lanes and
-// adds the weighted flux into the request's output with FP64 atomics
-// (an incoherent sum over particles; the summation order, and so the last
-// bits, depend on scheduling).
+// unit is (particle, observation point, block of up to kBatchEB energies); a
+// persistent grid of single-wave workgroups pulls units from a device counter
+// (a vector atomic on global memory), runs the reference's level loop for the
+// block's energies, re-packing the still-active ones into full waves at every
+// level, and adds the weighted flux into the request's output with FP64
+// atomics (an incoherent sum over particles; the summation order, and so the
+// last bits, depend on scheduling).
 //
 // Reference behaviour restated (paths relative to the reference root):
 //   multi-particle loop with weight 1/N   src/OSCARSSR.cc:4016-4063, 1778-1818
@@ -25,18 +26,6 @@
 
 namespace om {
 
-struct LevelJobBatch {
-  const double* t;
-  const double* p;
-  const double* ypp;
-  const TrajDesc* desc;
-  const double* tspan;  // per particle: tstart, tstop (2 doubles)
-  double* rows;
-  long stride;          // rows per particle
-  int np;
-  int Lcap;
-  int stop_mem;         // levels above it use the extended spacing
-};
 
 __global__ void __launch_bounds__(256) level_batch_kernel(LevelJobBatch J) {
   long g = (long) blockIdx.x * blockDim.x + threadIdx.x;
@@ -60,43 +49,7 @@ __global__ void __launch_bounds__(256) level_batch_kernel(LevelJobBatch J) {
   for (int c = 0; c < 9; ++c) o[1 + c] = v[c];
 }
 
-struct ReqDev {
-  const double* obs;    // 3 per point
-  const double* omega;  // per energy
-  double* out;          // n_obs * n_e
-  long n_obs, n_e, n_tiles_e;
-};
 
-struct BatchArgs {
-  const double* rows;
-  long stride;
-  int Lb;               // highest level held in rows (materialised by level_batch_kernel)
-  int Lmax;             // highest level summed; levels above Lb are interpolated in the kernel
-  int stop_mem;         // levels above it use the extended spacing
-  const double* t;      // knots, spline second derivatives and descriptors (levels above Lb)
-  const double* p;
-  const double* ypp;
-  const TrajDesc* desc;
-  const double* tspan;
-  const int* req_of;
-  const double* weight;
-  const long* item_prefix;  // np + 1
-  int np;
-  const ReqDev* req;
-  double C0, C2;
-  double pol[6];
-  int use_pol;
-  double precision;
-  unsigned long long* counter;
-  long total_items;
-  unsigned long long* n_unconverged;
-  unsigned long long* evals;  // sum over lanes of 2^(L+1)-1 (as integer)
-  // mixed path: units with lanes whose stop decision lay within the FP32 error
-  // bound, as (item, lane mask) pairs (room for every unit), recomputed in FP64
-  // by lw_batch_fix_kernel
-  unsigned long long* fix_list;
-  unsigned long long* n_fix;  // units listed
-};
 
 // Amplitude of a sample interpolated from the spline at time tt (levels above
 // the stored rows; rare).
@@ -115,158 +68,216 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return ((unsigned long long) hi << 32) | lo;
 }
 
-// One work unit (particle, observation point, 64-energy tile): the reference's
-// level loop for the lanes of lane_mask whose energy exists, weighted flux
-// added into the request's output.  Lane j computes the amplitude of sample
-// base+j and leaves it in LDS; then every lane walks those 64 samples at its
-// own energy.  MIXED: lanes whose stop decision lay within the FP32 error
-// bound add nothing; the unit is listed with their mask for lw_batch_fix_kernel.
+// One work unit (particle ip, observation point io, energies e0 .. e0+n_eb-1):
+// the reference's level loop for each energy, weighted flux added into the
+// request's output.  Level by level, the energies still active are packed into
+// groups of 64 lanes (so lanes do not idle once their energy has converged);
+// for each group, lane j computes the amplitude of sample base+j and leaves it
+// in LDS, then every lane walks those 64 samples at its own energy.  The
+// per-energy state lives in the wave's scratch (ws, wsi) between levels.
+// MIXED: energies whose stop decision lay within the FP32 error bound add
+// nothing here; they are listed for lw_batch_fix_kernel.
 template <bool MIXED>
-__device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, unsigned long long lane_mask, int lane,
-                                           double* s_tau, double (*s_v)[64]) {
+__device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, int lane, double* __restrict__ ws,
+                                           int* __restrict__ wsi, double* s_tau, double (*s_v)[64], int* s_list) {
   // particle of this work unit: last p with item_prefix[p] <= item
   int lo = 0, hi = A.np;
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
     if (A.item_prefix[mid] <= item) lo = mid; else hi = mid;
   }
-  int ip = lo;
-  long local = item - A.item_prefix[ip];
+  const int ip = lo;
+  const long local = item - A.item_prefix[ip];
   const ReqDev R = A.req[A.req_of[ip]];
-  long io = local / R.n_tiles_e;
-  long et = local - io * R.n_tiles_e;
-  long ie = et * 64 + lane;
-  bool valid = ie < R.n_e && ((lane_mask >> lane) & 1ULL);
-  long iec = ie < R.n_e ? ie : R.n_e - 1;
-  V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
-  double negw = -R.omega[iec];
-  double T = A.tspan[2 * ip + 1] - A.tspan[2 * ip];
-  const double* prow = A.rows + 10 * (long) ip * A.stride;
+  const long io = local / R.n_blocks_e;
+  const long e0 = (local - io * R.n_blocks_e) * kBatchEB;
+  const int n_eb = (int) (R.n_e - e0 < kBatchEB ? R.n_e - e0 : kBatchEB);
+  const V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
   const double ts = A.tspan[2 * ip], te = A.tspan[2 * ip + 1];
+  const double T = te - ts;
+  const double* prow = A.rows + 10 * (long) ip * A.stride;
   const TrajDesc D = A.desc[ip];
-  double sum[6] = {0, 0, 0, 0, 0, 0};
-  double lastmag = -1, prec = -1;
-  bool active = valid;
-  bool conv = false;
-  int lev = 0;
-  double an_run = 0, dm_last = 0;
-  bool border = false;
-  for (int L = 0; L <= A.Lmax; ++L) {
-    if (!__any(active)) break;
-    const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
-    long ns = 1L << L;
-    // sample times of level L (the expressions of level_batch_kernel)
-    double p2L = (double) (1L << L);
-    double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
-    double tfirst = ts + (te - ts) / (2.0 * p2L);
-    bool stored = L <= A.Lb;
-    double ps[6] = {0, 0, 0, 0, 0, 0};
-    double maxd = 0, lastphase = 0;
-    double anl = 0;  // mixed path: sum of |V|_1 over the samples this lane computed
-    for (long base = 0; base < ns; base += 64) {
-      long si = base + lane;
-      if (si < ns) {
-        Amp a;
-        if (stored) {
-          a = amplitude(obs, rows + 10 * si);
-        } else {
-          a = interpolated_amplitude(A, D, obs, tfirst + spacing * (double) si);
-        }
-        s_tau[lane] = a.tau;
-        s_v[0][lane] = a.V.x;
-        s_v[1][lane] = a.V.y;
-        s_v[2][lane] = a.V.z;
-        if (MIXED) anl += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);
-      }
-      wave_sync();
-      int cnt = (int) ((ns - base) < 64 ? (ns - base) : 64);
-      if (base == 0) lastphase = negw * s_tau[0];
-      int k = 0;
-      for (; k + 4 <= cnt; k += 4) {
+  double* w_sum = ws;                    // 6 x kBatchEB
+  double* w_lastmag = ws + 6 * kBatchEB;
+  double* w_prec = ws + 7 * kBatchEB;
+  double* w_an = ws + 8 * kBatchEB;      // mixed: sum of |V|_1 over the samples so far
+  double* w_dm = ws + 9 * kBatchEB;      // mixed: error bound of lastmag
+  int* w_level = wsi;
+  int* w_flags = wsi + kBatchEB;         // bit 0 converged, bit 1 borderline (mixed), bit 2 handed over
+  for (int e = lane; e < n_eb; e += 64) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          double phi = negw * s_tau[k + u];
+    for (int k = 0; k < 6; ++k) w_sum[k * kBatchEB + e] = 0;
+    w_lastmag[e] = -1;
+    w_prec[e] = -1;
+    w_an[e] = 0;
+    w_dm[e] = 0;
+    w_level[e] = -1;
+    w_flags[e] = 0;
+  }
+  __syncthreads();  // the state written above is read by other lanes
+  for (int L = 0; L <= A.Lmax; ++L) {
+    if (L == A.Ldeep) {
+      // hand the energies still active to the one-wave-per-point kernel
+      for (int t = 0; t < n_eb; t += 64) {
+        int e = t + lane;
+        int ec = e < n_eb ? e : 0;
+        bool a = e < n_eb && (w_flags[ec] & 1) == 0;
+        unsigned long long m = __ballot(a);
+        unsigned long long first = atomicAdd(A.n_deep, lane == 0 ? (unsigned long long) __popcll(m) : 0ULL);
+        unsigned long long j = uniform_u64(first) + (unsigned long long) __popcll(m & ((1ULL << lane) - 1ULL));
+        if (a && j < (unsigned long long) A.deep_cap) {
+          A.deep_code[j] = (unsigned long long) item * kBatchEB + (unsigned long long) e;
+#pragma unroll
+          for (int k = 0; k < kBatchWsD; ++k) A.deep_state[k * A.deep_cap + j] = ws[k * kBatchEB + e];
+          A.deep_level[j] = w_level[e];
+          A.deep_flags[j] = w_flags[e];
+          w_flags[e] |= 4;
+        }
+      }
+      __syncthreads();
+    }
+    // pack the energies still active (not converged, not handed over)
+    int n_act = 0;
+    for (int t = 0; t < n_eb; t += 64) {
+      int e = t + lane;
+      bool a = e < n_eb && (w_flags[e < n_eb ? e : 0] & 5) == 0;
+      unsigned long long m = __ballot(a);
+      if (a) s_list[n_act + __popcll(m & ((1ULL << lane) - 1ULL))] = e;
+      n_act += __popcll(m);
+    }
+    __syncthreads();
+    if (n_act == 0) break;
+    const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
+    const long ns = 1L << L;
+    // sample times of level L (the expressions of level_batch_kernel)
+    const double p2L = (double) (1L << L);
+    const double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
+    const double tfirst = ts + (te - ts) / (2.0 * p2L);
+    const bool stored = L <= A.Lb;
+    const double dt = T / (2.0 * (double) (1L << L) - 1);
+    for (int g = 0; g < n_act; g += 64) {
+      const bool valid = g + lane < n_act;
+      const int e = s_list[valid ? g + lane : g];  // spare lanes shadow the group's first energy
+      const double negw = -R.omega[e0 + e];
+      double ps[6] = {0, 0, 0, 0, 0, 0};
+      double maxd = 0, lastphase = 0;
+      double anl = 0;  // mixed path: sum of |V|_1 over the samples this lane computed
+      for (long base = 0; base < ns; base += 64) {
+        long si = base + lane;
+        if (si < ns) {
+          Amp a;
+          if (stored) {
+            a = amplitude(obs, rows + 10 * si);
+          } else {
+            a = interpolated_amplitude(A, D, obs, tfirst + spacing * (double) si);
+          }
+          s_tau[lane] = a.tau;
+          s_v[0][lane] = a.V.x;
+          s_v[1][lane] = a.V.y;
+          s_v[2][lane] = a.V.z;
+          if (MIXED) anl += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);
+        }
+        wave_sync();
+        int cnt = (int) ((ns - base) < 64 ? (ns - base) : 64);
+        if (base == 0) lastphase = negw * s_tau[0];
+        int k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            double phi = negw * s_tau[k + u];
+            maxd = fmax(maxd, fabs(phi - lastphase));
+            lastphase = phi;
+            double sn, cs;
+            sincos_mode(MIXED, phi, sn, cs);
+            double vx = s_v[0][k + u], vy = s_v[1][k + u], vz = s_v[2][k + u];
+            ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+            ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+            ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+          }
+        }
+        for (; k < cnt; ++k) {
+          double phi = negw * s_tau[k];
           maxd = fmax(maxd, fabs(phi - lastphase));
           lastphase = phi;
           double sn, cs;
           sincos_mode(MIXED, phi, sn, cs);
-          double vx = s_v[0][k + u], vy = s_v[1][k + u], vz = s_v[2][k + u];
+          double vx = s_v[0][k], vy = s_v[1][k], vz = s_v[2][k];
           ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
           ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
           ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
         }
+        wave_sync();
       }
-      for (; k < cnt; ++k) {
-        double phi = negw * s_tau[k];
-        maxd = fmax(maxd, fabs(phi - lastphase));
-        lastphase = phi;
-        double sn, cs;
-        sincos_mode(MIXED, phi, sn, cs);
-        double vx = s_v[0][k], vy = s_v[1][k], vz = s_v[2][k];
-        ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
-        ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
-        ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
-      }
-      wave_sync();
-    }
-    if (MIXED) {
-      // every lane walked the same samples: the level's |V|_1 sum is wave-uniform
-      for (int off = 32; off > 0; off >>= 1) anl += __shfl_xor(anl, off);
-    }
-    if (active) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k) sum[k] += ps[k];
-      lev = L;
-      // convergence test, src/OSCARSSR.cc:4227-4239
-      double E[6];
-      double dt = T / (2.0 * (double) (1L << L) - 1);
-#pragma unroll
-      for (int k = 0; k < 6; ++k) E[k] = sum[k] * dt;
-      if (A.use_pol) pol_project(E, A.pol);
-      double mag = cmag2(E);
-      double mag_prev = lastmag;
-      prec = fabs(mag - lastmag) / lastmag;
-      bool c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
-      lastmag = mag;
       if (MIXED) {
-        an_run += anl;
-        border = border || level_borderline(L, dt, A.precision, mag_prev, mag, prec, maxd, an_run, dm_last);
+        // every lane walked the same samples: the level's |V|_1 sum is wave-uniform
+        for (int off = 32; off > 0; off >>= 1) anl += __shfl_xor(anl, off);
       }
-      if (c) {
-        conv = true;
-        active = false;
+      if (valid) {
+        double sum[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sum[k] = w_sum[k * kBatchEB + e] + ps[k];
+        // convergence test, src/OSCARSSR.cc:4227-4239
+        double E[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) E[k] = sum[k] * dt;
+        if (A.use_pol) pol_project(E, A.pol);
+        const double mag = cmag2(E);
+        const double mag_prev = w_lastmag[e];
+        const double prec = fabs(mag - mag_prev) / mag_prev;
+        const bool c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == mag_prev);
+        int flags = w_flags[e] | (c ? 1 : 0);
+        if (MIXED) {
+          double an_run = w_an[e] + anl;
+          double dm_last = w_dm[e];
+          if (level_borderline(L, dt, A.precision, mag_prev, mag, prec, maxd, an_run, dm_last)) flags |= 2;
+          w_an[e] = an_run;
+          w_dm[e] = dm_last;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w_sum[k * kBatchEB + e] = sum[k];
+        w_lastmag[e] = mag;
+        w_prec[e] = prec;
+        w_level[e] = L;
+        w_flags[e] = flags;
       }
     }
+    __syncthreads();  // this level's state is read by other lanes at the next one
   }
-  // mixed path: the unit is listed once, with the mask of its borderline lanes
-  // (all lanes take part in the atomic; lanes 0 and 1 store the entry)
-  bool fixed = false;
-  if (MIXED) {
-    bool want = valid && border;
-    unsigned long long m = __ballot(want);
-    unsigned long long slot = atomicAdd(A.n_fix, (lane == 0 && m != 0ULL) ? 1ULL : 0ULL);
-    unsigned long long j = uniform_u64(slot);
-    if (m != 0ULL && lane < 2) A.fix_list[2 * j + lane] = lane == 0 ? (unsigned long long) item : m;
-    fixed = want;
-  }
-  if (valid && !fixed) {
-    double dt = T / (2.0 * (double) (1L << lev) - 1);
-    double f = A.C0 * dt;
-    double E[6];
+  // outputs, fix-up list and bookkeeping for the block's energies (all lanes
+  // take part in the atomics; only lane 0 adds nonzero counts)
+  for (int t = 0; t < n_eb; t += 64) {
+    const int e = t + lane;
+    const int ec = e < n_eb ? e : 0;
+    const int lev = w_level[ec];
+    const int flags = w_flags[ec];
+    const bool v = e < n_eb && (flags & 4) == 0;  // handed-over energies finish in the deep kernel
+    bool fixed = false;
+    if (MIXED) {
+      bool want = v && (flags & 2);
+      unsigned long long m = __ballot(want);
+      unsigned long long first = atomicAdd(A.n_fix, lane == 0 ? (unsigned long long) __popcll(m) : 0ULL);
+      unsigned long long j = uniform_u64(first) + (unsigned long long) __popcll(m & ((1ULL << lane) - 1ULL));
+      if (want && j < (unsigned long long) A.fix_cap) {
+        A.fix_list[j] = (unsigned long long) item * kBatchEB + (unsigned long long) e;
+        fixed = true;
+      }
+    }
+    if (v && !fixed) {
+      double f = A.C0 * (T / (2.0 * (double) (1L << lev) - 1));
+      double E[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) E[k] = sum[k] * f;
-    if (A.use_pol) pol_project(E, A.pol);
-    double flux = A.C2 * cmag2(E) * A.weight[ip];
-    atomicAdd(&R.out[io * R.n_e + ie], flux);
+      for (int k = 0; k < 6; ++k) E[k] = w_sum[k * kBatchEB + e] * f;
+      if (A.use_pol) pol_project(E, A.pol);
+      atomicAdd(&R.out[io * R.n_e + e0 + e], A.C2 * cmag2(E) * A.weight[ip]);
+    }
+    const bool counted = v && !fixed;
+    unsigned long long nc = __ballot(counted && !(flags & 1));
+    unsigned long long ev = counted ? (unsigned long long) ((2L << lev) - 1) : 0ULL;
+    for (int off = 32; off > 0; off >>= 1) ev += __shfl_down(ev, off);
+    atomicAdd(A.n_unconverged, lane == 0 ? (unsigned long long) __popcll(nc) : 0ULL);
+    atomicAdd(A.evals, lane == 0 ? ev : 0ULL);
   }
-  // bookkeeping: lanes that stopped without converging, and evaluations done
-  // (all lanes take part; only lane 0 adds a nonzero amount)
-  bool counted = valid && !fixed;
-  unsigned long long nc = __ballot(counted && !conv);
-  unsigned long long ev = counted ? (unsigned long long) ((2L << lev) - 1) : 0ULL;
-  for (int off = 32; off > 0; off >>= 1) ev += __shfl_down(ev, off);
-  atomicAdd(A.n_unconverged, lane == 0 ? (unsigned long long) __popcll(nc) : 0ULL);
-  atomicAdd(A.evals, lane == 0 ? ev : 0ULL);
+  __syncthreads();  // the next unit overwrites the scratch
 }
 
 // Four waves per SIMD: the register budget is set for the hot stored-row path;
@@ -275,7 +286,10 @@ template <bool MIXED>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw_batch_kernel(BatchArgs A) {
   __shared__ double s_tau[64];
   __shared__ double s_v[3][64];
+  __shared__ int s_list[kBatchEB];
   int lane = threadIdx.x;
+  double* ws = A.ws + (long) blockIdx.x * kBatchWsD * kBatchEB;
+  int* wsi = A.wsi + (long) blockIdx.x * kBatchWsI * kBatchEB;
   while (true) {
     // Next work unit.  Every lane executes the atomic (lane 0 adds 1, the others
     // 0) and the index is taken from lane 0 with readfirstlane, so it is a
@@ -285,91 +299,150 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw
     unsigned long long mine = atomicAdd(A.counter, lane == 0 ? 1ULL : 0ULL);
     long item = (long) uniform_u64(mine);
     if (item >= A.total_items) break;
-    batch_unit<MIXED>(A, item, ~0ULL, lane, s_tau, s_v);
+    batch_unit<MIXED>(A, item, lane, ws, wsi, s_tau, s_v, s_list);
   }
 }
 
-// FP64 recomputation of the lanes the mixed batch kernel listed: for each
-// listed unit, each lane of its mask (one energy) is summed by a whole wave,
-// the 64 lanes splitting each level's samples (a few points per unit: a wave
-// per point costs about 1/64 of re-walking the unit).  Levels, convergence
-// test and weighting are the FP64 kernel's.  A grid-stride loop over the
-// listed count (read from device memory, so the launch can sit in a captured
-// graph); each lane adds the flux of its own energy at the end of the unit.
-__global__ void __launch_bounds__(64) lw_batch_fix_kernel(BatchArgs A) {
-  int lane = threadIdx.x;
-  long n = (long) uniform_u64(*A.n_fix);
-  for (long e = blockIdx.x; e < n; e += gridDim.x) {
-    long item = (long) uniform_u64(A.fix_list[2 * e]);
-    unsigned long long mask = uniform_u64(A.fix_list[2 * e + 1]);
-    int lo = 0, hi = A.np;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (A.item_prefix[mid] <= item) lo = mid; else hi = mid;
+// Waves per point in the per-point kernels: each level's samples are split
+// into kPointWaves contiguous slices, one per wave, combined in LDS.
+constexpr int kPointWaves = 8;
+
+// One point (code = item * kBatchEB + energy-in-block) summed by a workgroup
+// of kPointWaves waves from level L0 with the given state; then its flux,
+// counts and (MIXED, borderline) its fix-up entry.  Levels, convergence test
+// and weighting are the batch kernel's.  Every thread returns with the same
+// decisions; thread 0 does the atomics.
+template <bool MIXED>
+__device__ __forceinline__ void batch_point(const BatchArgs& A, unsigned long long code, int L0, double sum[6],
+                                            double lastmag, double an_run, double dm_last, int flags, int lev,
+                                            double (*sh)[kPointWaves], unsigned long long* sh_slot) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  long item = (long) (code / kBatchEB);
+  long e_in = (long) (code - (unsigned long long) item * kBatchEB);
+  int lo = 0, hi = A.np;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (A.item_prefix[mid] <= item) lo = mid; else hi = mid;
+  }
+  const int ip = lo;
+  const long local = item - A.item_prefix[ip];
+  const ReqDev R = A.req[A.req_of[ip]];
+  const long io = local / R.n_blocks_e;
+  const long ie = (local - io * R.n_blocks_e) * kBatchEB + e_in;
+  const V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
+  const double ts = A.tspan[2 * ip], te = A.tspan[2 * ip + 1];
+  const double T = te - ts;
+  const double* prow = A.rows + 10 * (long) ip * A.stride;
+  const TrajDesc D = A.desc[ip];
+  const double negw = -R.omega[ie];
+  bool c = false;
+  for (int L = L0; L <= A.Lmax && !c; ++L) {
+    const long ns = 1L << L;
+    const double p2L = (double) (1L << L);
+    const double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
+    const double tfirst = ts + (te - ts) / (2.0 * p2L);
+    const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
+    const long i0 = ns * w / kPointWaves, i1 = ns * (w + 1) / kPointWaves;
+    double t6[6], md, al;
+    if (L <= A.Lb) {
+      slice_level_sum_rows<MIXED>(rows, i0, i1, obs, negw, lane, t6, md, al);
+    } else {
+      slice_level_sum<MIXED>(
+          i0, i1, negw, lane, [&](long i) { return interpolated_amplitude(A, D, obs, tfirst + spacing * (double) i); },
+          t6, md, al);
     }
-    int ip = lo;
-    long local = item - A.item_prefix[ip];
-    const ReqDev R = A.req[A.req_of[ip]];
-    long io = local / R.n_tiles_e;
-    long et = local - io * R.n_tiles_e;
-    V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
-    const double ts = A.tspan[2 * ip], te = A.tspan[2 * ip + 1];
-    const double T = te - ts;
-    const double* prow = A.rows + 10 * (long) ip * A.stride;
-    const TrajDesc D = A.desc[ip];
-    double my_flux = 0;
-    unsigned long long n_conv = 0, ev = 0;
-    for (unsigned long long m = mask; m != 0ULL; m &= m - 1ULL) {
-      int b = __builtin_ctzll(m);
-      long ie = et * 64 + b;
-      double negw = -R.omega[ie];
-      double sum[6] = {0, 0, 0, 0, 0, 0};
-      double lastmag = -1, prec = -1;
-      int lev = 0;
-      bool c = false;
-      for (int L = 0; L <= A.Lmax && !c; ++L) {
-        long ns = 1L << L;
-        double p2L = (double) (1L << L);
-        double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
-        double tfirst = ts + (te - ts) / (2.0 * p2L);
-        const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
-        double tot[6], maxd;
-        if (L <= A.Lb) {
-          wave_level_sum(ns, negw, lane, [&](long i) { return amplitude(obs, rows + 10 * i); }, tot, maxd);
-        } else {
-          wave_level_sum(ns, negw, lane,
-                         [&](long i) { return interpolated_amplitude(A, D, obs, tfirst + spacing * (double) i); },
-                         tot, maxd);
-        }
+    if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) sum[k] += tot[k];
-        lev = L;
-        // convergence test, src/OSCARSSR.cc:4227-4239
-        double E[6];
-        double dt = T / (2.0 * (double) (1L << L) - 1);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) E[k] = sum[k] * dt;
-        if (A.use_pol) pol_project(E, A.pol);
-        double mag = cmag2(E);
-        prec = fabs(mag - lastmag) / lastmag;
-        c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
-        lastmag = mag;
-      }
-      double dt = T / (2.0 * (double) (1L << lev) - 1);
-      double f = A.C0 * dt;
-      double E[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) E[k] = sum[k] * f;
-      if (A.use_pol) pol_project(E, A.pol);
-      double flux = A.C2 * cmag2(E) * A.weight[ip];
-      my_flux = lane == b ? flux : my_flux;
-      n_conv += c ? 1ULL : 0ULL;
-      ev += (unsigned long long) ((2L << lev) - 1);
+      for (int k = 0; k < 6; ++k) sh[k][w] = t6[k];
+      sh[6][w] = md;
+      sh[7][w] = al;
     }
-    if ((mask >> lane) & 1ULL) atomicAdd(&R.out[io * R.n_e + et * 64 + lane], my_flux);
-    unsigned long long nu = (unsigned long long) __popcll(mask) - n_conv;
-    atomicAdd(A.n_unconverged, lane == 0 ? nu : 0ULL);
-    atomicAdd(A.evals, lane == 0 ? ev : 0ULL);
+    __syncthreads();
+    double tot[6] = {0, 0, 0, 0, 0, 0};
+    double maxd = 0, anl = 0;
+    for (int v = 0; v < kPointWaves; ++v) {  // in slice order: every thread forms the same totals
+#pragma unroll
+      for (int k = 0; k < 6; ++k) tot[k] += sh[k][v];
+      maxd = fmax(maxd, sh[6][v]);
+      anl += sh[7][v];
+    }
+    __syncthreads();  // sh is rewritten at the next level
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sum[k] += tot[k];
+    lev = L;
+    // convergence test, src/OSCARSSR.cc:4227-4239
+    double E[6];
+    double dt = T / (2.0 * (double) (1L << L) - 1);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) E[k] = sum[k] * dt;
+    if (A.use_pol) pol_project(E, A.pol);
+    double mag = cmag2(E);
+    double prec = fabs(mag - lastmag) / lastmag;
+    c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
+    if (MIXED) {
+      an_run += anl;
+      if (level_borderline(L, dt, A.precision, lastmag, mag, prec, maxd, an_run, dm_last)) flags |= 2;
+    }
+    lastmag = mag;
+  }
+  c = c || (flags & 1);
+  bool fixed = false;
+  if (MIXED && (flags & 2)) {  // the same for every thread
+    if (threadIdx.x == 0) *sh_slot = atomicAdd(A.n_fix, 1ULL);
+    __syncthreads();
+    unsigned long long j = *sh_slot;
+    __syncthreads();
+    if (j < (unsigned long long) A.fix_cap) {
+      if (threadIdx.x == 0) A.fix_list[j] = code;
+      fixed = true;
+    }
+  }
+  if (threadIdx.x == 0 && !fixed) {
+    double f = A.C0 * (T / (2.0 * (double) (1L << lev) - 1));
+    double E[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) E[k] = sum[k] * f;
+    if (A.use_pol) pol_project(E, A.pol);
+    atomicAdd(&R.out[io * R.n_e + ie], A.C2 * cmag2(E) * A.weight[ip]);
+    if (!c) atomicAdd(A.n_unconverged, 1ULL);
+    atomicAdd(A.evals, (unsigned long long) ((2L << lev) - 1));
+  }
+}
+
+// Points the batch kernel handed over at level Ldeep (slowly converging, with
+// many samples per level): one workgroup per point from its stored state.  A
+// grid-stride loop over the handed-over count (read from device memory, so the
+// launch can sit in a captured graph).
+template <bool MIXED>
+__global__ void __launch_bounds__(64 * kPointWaves) lw_batch_deep_kernel(BatchArgs A) {
+  __shared__ double sh[8][kPointWaves];
+  __shared__ unsigned long long sh_slot;
+  long n = (long) *A.n_deep;
+  if (n > A.deep_cap) n = A.deep_cap;
+  for (long k = blockIdx.x; k < n; k += gridDim.x) {
+    unsigned long long code = A.deep_code[k];
+    double st[kBatchWsD];
+#pragma unroll
+    for (int q = 0; q < kBatchWsD; ++q) st[q] = A.deep_state[q * A.deep_cap + k];
+    int lev = A.deep_level[k];
+    int flags = A.deep_flags[k];
+    batch_point<MIXED>(A, code, lev + 1, st, st[6], st[8], st[9], flags, lev, sh, &sh_slot);
+  }
+}
+
+// FP64 recomputation of the points the mixed kernels listed: one workgroup per
+// point from level 0 (a few points per unit: far cheaper than re-walking the
+// unit), so their levels and values are the FP64 path's.
+__global__ void __launch_bounds__(64 * kPointWaves) lw_batch_fix_kernel(BatchArgs A) {
+  __shared__ double sh[8][kPointWaves];
+  __shared__ unsigned long long sh_slot;
+  long n = (long) *A.n_fix;
+  if (n > A.fix_cap) n = A.fix_cap;
+  for (long k = blockIdx.x; k < n; k += gridDim.x) {
+    unsigned long long code = A.fix_list[k];
+    double sum[6] = {0, 0, 0, 0, 0, 0};
+    batch_point<false>(A, code, 0, sum, -1.0, 0.0, 0.0, 0, 0, sh, &sh_slot);
   }
 }
 
@@ -400,12 +473,18 @@ extern "C" hipError_t om_launch_batch_clear(double* out, long n, unsigned long l
   return hipGetLastError();
 }
 
+// The batch: the persistent kernel, the per-point kernel for the points it
+// handed over, and (mixed) the FP64 recomputation of borderline points.
 extern "C" hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s) {
+  int pg = n_groups / om::kPointWaves > 0 ? n_groups / om::kPointWaves : 1;  // same wave count
+  dim3 pb(64 * om::kPointWaves);
   if (mixed) {
     hipLaunchKernelGGL(om::lw_batch_kernel<true>, dim3(n_groups), dim3(64), 0, s, *A);
-    hipLaunchKernelGGL(om::lw_batch_fix_kernel, dim3(n_groups), dim3(64), 0, s, *A);
+    hipLaunchKernelGGL(om::lw_batch_deep_kernel<true>, dim3(pg), pb, 0, s, *A);
+    hipLaunchKernelGGL(om::lw_batch_fix_kernel, dim3(pg), pb, 0, s, *A);
   } else {
     hipLaunchKernelGGL(om::lw_batch_kernel<false>, dim3(n_groups), dim3(64), 0, s, *A);
+    hipLaunchKernelGGL(om::lw_batch_deep_kernel<false>, dim3(pg), pb, 0, s, *A);
   }
   return hipGetLastError();
 }

@@ -35,38 +35,8 @@
 
 namespace om {
 
-struct Levels {
-  const double* rows[kMaxLevel + 1];  // 2^L rows of 10 doubles per level (device)
-  double dt_incl[kMaxLevel + 1];      // (tstop - tstart) / (2^(L+1) - 1)
-  double beta_diff[kMaxLevel + 1];    // per level max |beta_i - beta_{i-1}| (power density)
-};
 
-struct State {
-  double* sum;      // 6 doubles per point (Re/Im of Ex, Ey, Ez), power density: sum[0]
-  double* err;      // mixed path (else null): 2 per point, sum over samples of |V|_1 and the
-                    // error bound of lastmag
-  double* lastmag;  // |E|^2 (or power sum) after the previous level
-  double* prec;     // last relative change
-  int* level;       // last level summed (-1 before any)
-  int* conv;        // 1 once converged
-  int* border;      // mixed path (else null): 1 if a level decision lay within the error bound
-};
 
-struct LwArgs {
-  Levels lv;
-  State st;
-  int La, Lb;
-  long npts;
-  const double* obs;    // 3 per observation point
-  const double* omega;  // angular frequency per energy
-  long n_omega;         // point p -> (obs p / n_omega, energy p % n_omega)
-  double pol[6];
-  int use_pol;
-  double precision;
-  const int* list;      // null: dense grid.  Tile kernel: 64 point indices per tile (-1 pads);
-                        // point kernel: n_list point indices, one per lane.
-  long n_list;
-};
 
 // Per-level convergence test, src/OSCARSSR.cc:1997-2012.
 __device__ __forceinline__ bool level_converged(const LwArgs& A, int L, const double sum[6], double maxd,
@@ -381,17 +351,6 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
 }
 
 // ---------------------------------------------------------------- power density
-struct PdArgs {
-  Levels lv;
-  State st;
-  int La, Lb;
-  long npts;
-  const double* obs;
-  double normal[3];
-  int directional;
-  double precision;
-  double two_over_gamma;
-};
 
 __global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
   long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
@@ -452,25 +411,6 @@ __global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
 }
 
 // ---------------------------------------------------------------- finalize
-struct FinArgs {
-  State st;
-  long npts;
-  int kind;       // 0 spectrum/flux, 1 power density
-  int quantity;   // 0 value, 1 precision, 2 level
-  double C0, C2;  // spectrum/flux constants (src/OSCARSSR.cc:1882-1885)
-  double pd_const;
-  int directional;
-  double dt_incl[kMaxLevel + 1];
-  double pol[6];
-  int use_pol;
-  double weight;
-  int accumulate;      // add weight*value into out with Kahan compensation (out_comp)
-  double* out;
-  double* out_comp;
-  double* out_e;       // 6 per point (may be null)
-  int* out_level;
-  unsigned char* out_conv;
-};
 
 __global__ void __launch_bounds__(256) lw_finalize_kernel(FinArgs F) {
   long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
@@ -534,8 +474,8 @@ __global__ void __launch_bounds__(256) lw_wave_kernel(LwArgs A) {
   bool c = false;
   for (int L = A.La; L <= A.Lb && !c; ++L) {
     const double* __restrict__ rows = A.lv.rows[L];
-    double tot[6], maxd;
-    wave_level_sum(1L << L, negw, lane, [&](long i) { return amplitude(obs, rows + 10 * i); }, tot, maxd);
+    double tot[6], maxd, an;
+    slice_level_sum_rows<false>(rows, 0, 1L << L, obs, negw, lane, tot, maxd, an);
 #pragma unroll
     for (int k = 0; k < 6; ++k) sum[k] += tot[k];
     lev = L;

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "om_device.h"
+#include "launch_types.h"
 #include "om_types.h"
 
 namespace om {
@@ -88,33 +89,37 @@ __device__ __forceinline__ void sincos_mode(bool mixed, double phi, double& sn, 
   }
 }
 
-// One level of one point's sum in FP64 with the 64 lanes of a wave splitting
-// the samples (lane j takes samples j, j+64, ...; amp_at(i) gives sample i's
-// amplitude).  The largest phase step between consecutive samples takes the
-// previous sample's phase from the neighbouring lane.  The partial sums are
-// combined by a butterfly, so every lane ends with the same totals (pairs add
-// the same two values).  Used to recompute few points: a whole wave per point
-// keeps their latency low.
-template <class AmpAt>
-__device__ __forceinline__ void wave_level_sum(long ns, double negw, int lane, AmpAt amp_at, double tot[6],
-                                               double& maxd) {
+// Samples [i0, i1) of one level of one point's sum, the 64 lanes of a wave
+// splitting them (lane j takes samples i0+j, i0+j+64, ...; amp_at(i) gives
+// sample i's amplitude).  MIXED: FP32 sin/cos of the exactly reduced phase, and
+// `an` returns the slice's sum of |V|_1 (for the error bound); else FP64.  The
+// largest phase step between consecutive samples takes the previous sample's
+// phase from the neighbouring lane (for sample i0 > 0, from amp_at(i0 - 1)).
+// The partial sums are combined by a butterfly, so every lane ends with the
+// same totals (pairs add the same two values).  Used for few points with many
+// samples: whole waves per point keep their latency low.
+template <bool MIXED, class AmpAt>
+__device__ __forceinline__ void slice_level_sum(long i0, long i1, double negw, int lane, AmpAt amp_at,
+                                                double tot[6], double& maxd, double& an) {
   double ps[6] = {0, 0, 0, 0, 0, 0};
-  double md = 0, prev_last = 0;
-  for (long base = 0; base < ns; base += 64) {
+  double md = 0, al = 0;
+  double prev_last = i0 > 0 && i1 > i0 ? negw * amp_at(i0 - 1).tau : 0.0;
+  for (long base = i0; base < i1; base += 64) {
     long i = base + lane;
-    bool ok = i < ns;
+    bool ok = i < i1;
     double phi = 0;
     if (ok) {
       Amp a = amp_at(i);
       phi = negw * a.tau;
       double sn, cs;
-      sincos_huge64(phi, &sn, &cs);
+      sincos_mode(MIXED, phi, sn, cs);
       ps[0] = __builtin_fma(a.V.x, cs, ps[0]); ps[1] = __builtin_fma(a.V.x, sn, ps[1]);
       ps[2] = __builtin_fma(a.V.y, cs, ps[2]); ps[3] = __builtin_fma(a.V.y, sn, ps[3]);
       ps[4] = __builtin_fma(a.V.z, cs, ps[4]); ps[5] = __builtin_fma(a.V.z, sn, ps[5]);
+      if (MIXED) al += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);
     }
     double prev = __shfl_up(phi, 1);
-    if (lane == 0) prev = base == 0 ? phi : prev_last;
+    if (lane == 0) prev = (base == 0) ? phi : prev_last;
     if (ok) md = fmax(md, fabs(phi - prev));
     prev_last = __shfl(phi, 63);
   }
@@ -122,10 +127,68 @@ __device__ __forceinline__ void wave_level_sum(long ns, double negw, int lane, A
 #pragma unroll
     for (int k = 0; k < 6; ++k) ps[k] += __shfl_xor(ps[k], off);
     md = fmax(md, __shfl_xor(md, off));
+    if (MIXED) al += __shfl_xor(al, off);
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) tot[k] = ps[k];
   maxd = md;
+  an = al;
+}
+
+// slice_level_sum over stored sample rows (10 doubles per sample), with the
+// next chunk's rows loaded before the current chunk is computed, so the row
+// latency overlaps the arithmetic.  Same operations and order as
+// slice_level_sum with amp_at(i) = amplitude(obs, rows + 10 i).
+template <bool MIXED>
+__device__ __forceinline__ void slice_level_sum_rows(const double* __restrict__ rows, long i0, long i1, V3 obs,
+                                                     double negw, int lane, double tot[6], double& maxd,
+                                                     double& an) {
+  double ps[6] = {0, 0, 0, 0, 0, 0};
+  double md = 0, al = 0;
+  double prev_last = i0 > 0 && i1 > i0 ? negw * amplitude(obs, rows + 10 * (i0 - 1)).tau : 0.0;
+  double cur[10], nxt[10];
+  const long valid0 = i0 + lane < i1 ? i0 + lane : i0;  // a row inside the slice (or row i0)
+  {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) nxt[q] = rows[10 * valid0 + q];
+  }
+  for (long base = i0; base < i1; base += 64) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) cur[q] = nxt[q];
+    {
+      long in = base + 64 + lane;
+      in = in < i1 ? in : valid0;  // past the end: reload a valid row (unused)
+#pragma unroll
+      for (int q = 0; q < 10; ++q) nxt[q] = rows[10 * in + q];
+    }
+    long i = base + lane;
+    bool ok = i < i1;
+    double phi = 0;
+    if (ok) {
+      Amp a = amplitude(obs, cur);
+      phi = negw * a.tau;
+      double sn, cs;
+      sincos_mode(MIXED, phi, sn, cs);
+      ps[0] = __builtin_fma(a.V.x, cs, ps[0]); ps[1] = __builtin_fma(a.V.x, sn, ps[1]);
+      ps[2] = __builtin_fma(a.V.y, cs, ps[2]); ps[3] = __builtin_fma(a.V.y, sn, ps[3]);
+      ps[4] = __builtin_fma(a.V.z, cs, ps[4]); ps[5] = __builtin_fma(a.V.z, sn, ps[5]);
+      if (MIXED) al += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);
+    }
+    double prev = __shfl_up(phi, 1);
+    if (lane == 0) prev = (base == 0) ? phi : prev_last;
+    if (ok) md = fmax(md, fabs(phi - prev));
+    prev_last = __shfl(phi, 63);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ps[k] += __shfl_xor(ps[k], off);
+    md = fmax(md, __shfl_xor(md, off));
+    if (MIXED) al += __shfl_xor(al, off);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) tot[k] = ps[k];
+  maxd = md;
+  an = al;
 }
 
 // Mixed path: bound of the error the FP32 sin/cos puts into a level's stop

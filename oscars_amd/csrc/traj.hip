@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include "om_device.h"
+#include "launch_types.h"
 #include "om_types.h"
 #include "rk4_common.h"
 
@@ -233,17 +234,6 @@ __global__ void __launch_bounds__(256) spline_b_kernel(const TrajDesc* __restric
 }
 
 // ---------------------------------------------------------------- level samples
-struct LevelJob {
-  const double* t;     // packed knot times
-  const double* p;
-  const double* ypp;
-  long knot_offset;
-  long n;
-  double tstart, tstop;
-  double* out;          // destination rows (10 doubles) for this level
-  long npts;            // 2^L
-  double t_first, spacing;
-};
 
 __global__ void __launch_bounds__(256) level_kernel(LevelJob J) {
   long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
