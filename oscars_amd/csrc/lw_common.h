@@ -42,7 +42,10 @@ __device__ __forceinline__ double cmag2(const double E[6]) {
 
 // Real amplitude vector of one sample, src/OSCARSSR.cc:1975-1990:
 //   V = (1-|B|^2)(N-B)/(D^2 (1-N.B)^2) + N x ((N-B) x A)/(D (1-N.B)^2)
-// and tau = t + D/c, both in the reference's operation order.
+// and tau = t + D/c.  tau and N = R/D follow the reference's operations
+// exactly (the phase, and 1 - N.B, which cancels to ~1/(2 gamma^2), are
+// sensitive to the last bit); the two final scalings of V multiply by
+// reciprocals instead of dividing each component (V moves by ~1e-16 relative).
 struct Amp {
   double tau;
   V3 V;
@@ -65,7 +68,9 @@ __device__ __forceinline__ Amp amplitude(V3 obs, const double* __restrict__ s) {
   double den1 = D * D * om2;
   double den2 = D * om2;
   V3 c2 = cross(N, cross(NmB, Ac));
-  a.V = v3(NmB.x * s1 / den1 + c2.x / den2, NmB.y * s1 / den1 + c2.y / den2, NmB.z * s1 / den1 + c2.z / den2);
+  double q1 = s1 / den1;
+  double q2 = 1.0 / den2;
+  a.V = v3(NmB.x * q1 + c2.x * q2, NmB.y * q1 + c2.y * q2, NmB.z * q1 + c2.z * q2);
   return a;
 }
 
