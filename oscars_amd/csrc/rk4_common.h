@@ -187,9 +187,11 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
     constexpr int K = M - kAxisUndX;
     double D = x[4] - f0.c2;
     double sc = und_scale(f0, D, f0.psA);
-    double s = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
     double a = K == 0 ? f0.a0 : (K == 1 ? f0.a1 : f0.a2);
-    double v = a * sc * s;
+    // outside the magnet the field is zero (a * 0 * s up to the sign of zero):
+    // the sine is skipped there
+    double v = 0.0;
+    if (sc != 0.0) v = a * sc * sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
     d[0] = x[1];
     d[2] = x[3];
     d[4] = x[5];
