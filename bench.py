@@ -51,16 +51,31 @@ def rect(n, w=0.02):
 
 
 class Workload:
+    """The step's inputs and outputs live in HBM (torch tensors) before the timed
+    region: the flux calls take device pointers (om_flux_dev), so a step moves
+    no grid or result over PCIe."""
+
     def __init__(self, sr_obj, n_map=201, n_sweep=51, n_e=512):
+        import torch
         self.s = sr_obj
         self.obs_map = rect(n_map)
         self.obs_sweep = rect(n_sweep)
         self.e_sweep = np.array([100 + (3000 - 100) / (n_e - 1) * i for i in range(n_e)])
         self.map_energies = [152.11, 260.0]
         self.only_sweep = False
+        cuda = dict(device="cuda")
+        self.t_obs_map = torch.tensor(self.obs_map, dtype=torch.float64, **cuda)
+        self.t_obs_sweep = torch.tensor(self.obs_sweep, dtype=torch.float64, **cuda)
+        self.t_e_sweep = torch.tensor(self.e_sweep, dtype=torch.float64, **cuda)
+        self.t_e_map = {e: torch.tensor([e], dtype=torch.float64, **cuda) for e in self.map_energies}
+        self.outs = {}
+        for k, n in [("map_%g" % e, len(self.obs_map)) for e in self.map_energies] + \
+                    [("sweep", len(self.obs_sweep) * n_e)]:
+            self.outs[k] = (torch.zeros(n, dtype=torch.float64, **cuda), torch.zeros(n, dtype=torch.int32, **cuda),
+                            torch.zeros(n, dtype=torch.uint8, **cuda))
 
     def step(self, mode):
-        """One pass; returns dict of per-phase evals / kernel ms / outputs."""
+        """One pass; returns dict of per-phase evals / kernel ms / output tensors (device)."""
         s = self.s
         dev = s._device()
         # RK4 + spline on the device; level samples are rebuilt inside the step
@@ -69,22 +84,51 @@ class Workload:
         out = {"evals": 0.0, "kernel_ms": {}, "values": {}, "levels": {}, "rechecked": {}}
         map_mode = "plain" if mode == "plain" else "point_mixed"
         for e in ([] if self.only_sweep else self.map_energies):
-            v, lev, cv, _, st = dev.flux(0, self.obs_map, [e], pol, dev.conv(mode=map_mode))
+            k = "map_%g" % e
+            v, lev, cv = self.outs[k]
+            st = dev.flux_dev(0, self.t_obs_map, self.t_e_map[e], pol, dev.conv(mode=map_mode), v, lev, cv)
             out["evals"] += st["evals"]
-            out["kernel_ms"]["map_%g" % e] = st["kernel_ms"]
-            out["values"]["map_%g" % e] = v
-            out["levels"]["map_%g" % e] = lev
-            out["rechecked"]["map_%g" % e] = st["n_rechecked"]
+            out["kernel_ms"][k] = st["kernel_ms"]
+            out["values"][k] = v
+            out["levels"][k] = lev
+            out["rechecked"][k] = st["n_rechecked"]
         sweep_mode = {"plain": "plain", "tiled": "tiled", "mixed": "mixed"}[mode]
-        v, lev, cv, _, st = dev.flux(0, self.obs_sweep, self.e_sweep, pol, dev.conv(mode=sweep_mode))
+        v, lev, cv = self.outs["sweep"]
+        st = dev.flux_dev(0, self.t_obs_sweep, self.t_e_sweep, pol, dev.conv(mode=sweep_mode), v, lev, cv)
         out["evals"] += st["evals"]
         out["sweep_evals"] = st["evals"]
-        out["sweep_levels"] = lev
         out["kernel_ms"]["sweep"] = st["kernel_ms"]
         out["values"]["sweep"] = v
         out["levels"]["sweep"] = lev
         out["rechecked"]["sweep"] = st["n_rechecked"]
         return out
+
+
+def host_api_step_ms(work, mode, steps=2):
+    """ms per step of the same workload through the host-buffer entry points
+    (om_flux: grids uploaded and results copied back over PCIe every call)."""
+    import torch
+    s = work.s
+    dev = s._device()
+    pol = [0.0] * 6
+    map_mode = "plain" if mode == "plain" else "point_mixed"
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.trajectories(s._fields, [s._om_particle(s._particle)])
+        for e in work.map_energies:
+            dev.flux(0, work.obs_map, [e], pol, dev.conv(mode=map_mode))
+        dev.flux(0, work.obs_sweep, work.e_sweep, pol, dev.conv(mode=mode))
+    torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def to_host(res):
+    """Copies a step's device outputs to numpy (outside the timed region)."""
+    for key in ("values", "levels"):
+        res[key] = {k: t.cpu().numpy() for k, t in res[key].items()}
+    res["sweep_levels"] = res["levels"]["sweep"]
+    return res
 
 
 def timed(work, mode, steps, warmup):
@@ -103,7 +147,7 @@ def timed(work, mode, steps, warmup):
             kms[k] = kms.get(k, 0.0) + v
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    return (t1 - t0), evals, {k: v / steps for k, v in kms.items()}, res
+    return (t1 - t0), evals, {k: v / steps for k, v in kms.items()}, to_host(res)
 
 
 def rel_errors(a, b):
@@ -408,6 +452,12 @@ def main():
         extra["rel_err_vs_fp64"] = errs
         print("# plain: %.4g evals/s ratio %.2f errs %s" % (pvalue, value / pvalue, errs), file=sys.stderr,
               flush=True)
+
+    if not args.only_sweep:
+        hms = host_api_step_ms(work, args.mode)
+        extra["host_buffer_api"] = {"ms_per_step": hms, "evals_per_s": evals / args.steps / (hms * 1e-3),
+                                    "note": "same step through om_flux (host grids and results, PCIe included)"}
+        print("# host-buffer API: %.3f ms/step" % hms, file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

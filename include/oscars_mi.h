@@ -133,6 +133,23 @@ int om_power_density(om_context* ctx, int particle, const double* obs_xyz, long 
                      int directional, const om_conv* conv, double* out_value, int* out_level,
                      unsigned char* out_conv, om_stats* stats);
 
+/* Device-resident variants: the same calculations with obs_xyz, energies_eV
+ * and every output (out_value, out_level, out_conv, out_e) device pointers on
+ * the context's device (e.g. torch tensors' data pointers); pol6, normal, conv
+ * and stats stay host pointers.  No host transfer of inputs or outputs; the
+ * call still returns once the outputs are written.  No counterpart in the
+ * reference (its GPU entry points take host vectors); for callers that keep
+ * their grids and results in HBM. */
+int om_flux_dev(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* energies_eV,
+                long n_e, const double* pol6, const om_conv* conv, double* out_value, int* out_level,
+                unsigned char* out_conv, double* out_e, om_stats* stats);
+int om_spectrum_dev(om_context* ctx, int particle, const double* obs, const double* energies_eV, long n_e,
+                    const double* pol6, const om_conv* conv, double* out_value, int* out_level,
+                    unsigned char* out_conv, double* out_e, om_stats* stats);
+int om_power_density_dev(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* normal,
+                         int directional, const om_conv* conv, double* out_value, int* out_level,
+                         unsigned char* out_conv, om_stats* stats);
+
 /* ---- batched multi-particle sums (one persistent launch) ----
  * Replaces the reference's multi-particle loop (src/OSCARSSR.cc:4016-4063,
  * 1778-1818 and the CUDA host pipeline src/OSCARSSR_Cuda.cu:974-1147), which

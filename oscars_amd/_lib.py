@@ -60,7 +60,7 @@ class om_stats(C.Structure):
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
            "om_trajectories", "om_trajectory_get", "om_level_get", "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
-           "om_adjoint_forward", "om_adjoint_backward"]
+           "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
 
 _lib = None
 
@@ -73,6 +73,15 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("oscars_amd: %s is missing - build it with __graft_entry__.build() "
                            "(there is no CPU fallback)" % LIB_PATH)
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7) as
+    # the one this library links; the first loaded serves the whole process, and
+    # a torch built against another ROCm release finds no GPU when it is not its
+    # own.  Loading torch first keeps one runtime, torch's, for both (device
+    # tensors then pass straight to the *_dev entry points).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.om_device_count.restype = C.c_int
     L.om_device_info.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
@@ -90,6 +99,13 @@ def lib():
                               _dp, C.POINTER(om_stats)]
     L.om_power_density.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, _dp, C.c_int, C.POINTER(om_conv), _dp,
                                    _ip, _up, C.POINTER(om_stats)]
+    _vp = C.c_void_p  # device addresses
+    L.om_flux_dev.argtypes = [C.c_void_p, C.c_int, _vp, C.c_long, _vp, C.c_long, _dp, C.POINTER(om_conv), _vp, _vp,
+                              _vp, _vp, C.POINTER(om_stats)]
+    L.om_spectrum_dev.argtypes = [C.c_void_p, C.c_int, _vp, _vp, C.c_long, _dp, C.POINTER(om_conv), _vp, _vp, _vp,
+                                  _vp, C.POINTER(om_stats)]
+    L.om_power_density_dev.argtypes = [C.c_void_p, C.c_int, _vp, C.c_long, _dp, C.c_int, C.POINTER(om_conv), _vp,
+                                       _vp, _vp, C.POINTER(om_stats)]
     L.om_batch_setup.argtypes = [C.c_void_p, C.POINTER(om_request), C.c_int, _ip, _dp, _dp, C.POINTER(om_conv),
                                  C.POINTER(om_batch_opts)]
     L.om_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(om_stats)]
@@ -215,6 +231,59 @@ class Device:
         self._chk(self.L.om_power_density(self.h, int(particle), _ptr(obs), n, _ptr(nv), int(directional),
                                           C.byref(conv), _ptr(out), _ptr(lev, _ip), _ptr(cv, _up), C.byref(st)))
         return out, lev, cv, st.as_dict()
+
+    # ------------------------------------------------------------ device-resident variants
+    @staticmethod
+    def _dev_ptr(t, dtype, n, what):
+        """Address of a contiguous GPU torch tensor of `n` elements of `dtype` (None: NULL)."""
+        if t is None:
+            return None
+        if not (t.is_cuda and t.dtype == dtype and t.is_contiguous() and t.numel() == n):
+            raise ValueError("%s: need a contiguous %s GPU tensor of %d elements" % (what, dtype, n))
+        return t.data_ptr()
+
+    def flux_dev(self, particle, obs_t, e_t, pol6, conv, out_t, lev_t=None, conv_t=None, efield_t=None):
+        """om_flux on device tensors: obs_t (n_obs, 3) and e_t (n_e) float64, out_t (n_obs*n_e) float64,
+        lev_t int32, conv_t uint8, efield_t (n_obs*n_e, 6) float64 (optional).  Returns the stats."""
+        import torch
+        n_obs, n_e = obs_t.shape[0], e_t.numel()
+        n = n_obs * n_e
+        pol = np.ascontiguousarray(pol6, dtype=np.float64)
+        st = om_stats()
+        self._chk(self.L.om_flux_dev(self.h, int(particle), self._dev_ptr(obs_t, torch.float64, 3 * n_obs, "obs"),
+                                     n_obs, self._dev_ptr(e_t, torch.float64, n_e, "energies"), n_e, _ptr(pol),
+                                     C.byref(conv), self._dev_ptr(out_t, torch.float64, n, "out"),
+                                     self._dev_ptr(lev_t, torch.int32, n, "levels"),
+                                     self._dev_ptr(conv_t, torch.uint8, n, "converged"),
+                                     self._dev_ptr(efield_t, torch.float64, 6 * n, "efield"), C.byref(st)))
+        return st.as_dict()
+
+    def spectrum_dev(self, particle, obs_t, e_t, pol6, conv, out_t, lev_t=None, conv_t=None, efield_t=None):
+        """om_spectrum on device tensors (obs_t: 3 float64); see flux_dev."""
+        import torch
+        n = e_t.numel()
+        pol = np.ascontiguousarray(pol6, dtype=np.float64)
+        st = om_stats()
+        self._chk(self.L.om_spectrum_dev(self.h, int(particle), self._dev_ptr(obs_t, torch.float64, 3, "obs"),
+                                         self._dev_ptr(e_t, torch.float64, n, "energies"), n, _ptr(pol),
+                                         C.byref(conv), self._dev_ptr(out_t, torch.float64, n, "out"),
+                                         self._dev_ptr(lev_t, torch.int32, n, "levels"),
+                                         self._dev_ptr(conv_t, torch.uint8, n, "converged"),
+                                         self._dev_ptr(efield_t, torch.float64, 6 * n, "efield"), C.byref(st)))
+        return st.as_dict()
+
+    def power_density_dev(self, particle, obs_t, normal, directional, conv, out_t, lev_t=None, conv_t=None):
+        """om_power_density on device tensors; see flux_dev."""
+        import torch
+        n = obs_t.shape[0]
+        nv = np.ascontiguousarray(normal, dtype=np.float64)
+        st = om_stats()
+        self._chk(self.L.om_power_density_dev(self.h, int(particle), self._dev_ptr(obs_t, torch.float64, 3 * n, "obs"),
+                                              n, _ptr(nv), int(directional), C.byref(conv),
+                                              self._dev_ptr(out_t, torch.float64, n, "out"),
+                                              self._dev_ptr(lev_t, torch.int32, n, "levels"),
+                                              self._dev_ptr(conv_t, torch.uint8, n, "converged"), C.byref(st)))
+        return st.as_dict()
 
     # ------------------------------------------------------------ batched multi-particle sums
     def batch_setup(self, requests, req_of, weight, pol6, conv, waves_per_cu=0, max_level=0):

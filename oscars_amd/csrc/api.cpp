@@ -44,6 +44,9 @@ hipError_t om_launch_tile_list(const om::State* st, long n_obs, long n_omega, in
 hipError_t om_launch_point_list(const om::State* st, long npts, int La, int* list, int* count, hipStream_t s);
 hipError_t om_launch_border_list(const om::State* st, long npts, int* list, int* count, hipStream_t s);
 hipError_t om_launch_lw_wave(const om::LwArgs* A, hipStream_t s);
+hipError_t om_launch_omega_from_ev(const double* e, long n, double* om, double qe, double hbar, hipStream_t s);
+hipError_t om_launch_level_stats(const int* level, const unsigned char* conv, long n, unsigned long long* out,
+                                 hipStream_t s);
 hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s);
 hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s);
 hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s);
@@ -132,6 +135,7 @@ struct om_context {
   DevBuf<double> d_obs, d_omega, d_sum, d_err, d_lastmag, d_prec, d_out, d_outcomp, d_oute;
   DevBuf<int> d_level, d_conv, d_count, d_outlevel, d_list, d_border;
   DevBuf<unsigned char> d_outconv;
+  DevBuf<unsigned long long> d_stats;
   DevBuf<double> d_bd;
   std::vector<om::FieldDev> fields_host;
   int nfields = 0;
@@ -435,51 +439,61 @@ struct Timer {
   }
 };
 
+// Finalize kernel into the caller's buffers: host pointers (copied back) or,
+// with dev_io, device pointers written in place.  Statistics are reduced on
+// the device.
 void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_value, int* out_level,
-                    unsigned char* out_conv, double* out_e, om_stats* stats, const om_conv& conv) {
-  ctx->d_out.ensure(npts);
+                    unsigned char* out_conv, double* out_e, om_stats* stats, const om_conv& conv, bool dev_io) {
   ctx->d_outcomp.ensure(npts);
-  ctx->d_outlevel.ensure(npts);
-  ctx->d_outconv.ensure(npts);
-  if (out_e) ctx->d_oute.ensure(6 * npts);
-  F.out = ctx->d_out.p;
   F.out_comp = ctx->d_outcomp.p;
-  F.out_e = out_e ? ctx->d_oute.p : nullptr;
-  F.out_level = ctx->d_outlevel.p;
-  F.out_conv = ctx->d_outconv.p;
   F.weight = conv.weight;
   F.accumulate = conv.accumulate;
-  if (conv.accumulate) {
-    // The running sum lives in the caller's buffer; the compensation term is
-    // kept on the device between calls of the same size.
-    check(hipMemcpyAsync(ctx->d_out.p, out_value, npts * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
-          "acc upload");
+  if (dev_io) {
+    ctx->d_outlevel.ensure(npts);
+    ctx->d_outconv.ensure(npts);
+    F.out = out_value;  // a running sum (accumulate) is already there
+    F.out_e = out_e;
+    F.out_level = out_level ? out_level : ctx->d_outlevel.p;
+    F.out_conv = out_conv ? out_conv : ctx->d_outconv.p;
+  } else {
+    ctx->d_out.ensure(npts);
+    ctx->d_outlevel.ensure(npts);
+    ctx->d_outconv.ensure(npts);
+    if (out_e) ctx->d_oute.ensure(6 * npts);
+    F.out = ctx->d_out.p;
+    F.out_e = out_e ? ctx->d_oute.p : nullptr;
+    F.out_level = ctx->d_outlevel.p;
+    F.out_conv = ctx->d_outconv.p;
+    if (conv.accumulate) {
+      // The running sum lives in the caller's buffer; the compensation term is
+      // kept on the device between calls of the same size.
+      check(hipMemcpyAsync(ctx->d_out.p, out_value, npts * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+            "acc upload");
+    }
   }
   check(om_launch_finalize(&F, ctx->stream), "finalize");
-  check(hipMemcpyAsync(out_value, ctx->d_out.p, npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "out");
-  std::vector<int> lev(npts);
-  std::vector<unsigned char> cv(npts);
-  check(hipMemcpyAsync(lev.data(), ctx->d_outlevel.p, npts * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
-        "lev");
-  check(hipMemcpyAsync(cv.data(), ctx->d_outconv.p, npts, hipMemcpyDeviceToHost, ctx->stream), "conv");
-  if (out_e)
-    check(hipMemcpyAsync(out_e, ctx->d_oute.p, 6 * npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
-          "efield");
-  check(hipStreamSynchronize(ctx->stream), "sync");
-  if (out_level) std::memcpy(out_level, lev.data(), npts * sizeof(int));
-  if (out_conv) std::memcpy(out_conv, cv.data(), npts);
+  unsigned long long st3[3] = {0, 0, 0};
   if (stats) {
-    double ev = 0;
-    int mx = -1;
-    long nu = 0;
-    for (long i = 0; i < npts; ++i) {
-      if (lev[i] >= 0) ev += std::ldexp(1.0, lev[i] + 1) - 1;
-      mx = lev[i] > mx ? lev[i] : mx;
-      nu += cv[i] ? 0 : 1;
-    }
-    stats->evals = ev;
-    stats->max_level = mx;
-    stats->n_unconverged = nu;
+    ctx->d_stats.ensure(3);
+    check(om_launch_level_stats(F.out_level, F.out_conv, npts, ctx->d_stats.p, ctx->stream), "level stats");
+    check(hipMemcpyAsync(st3, ctx->d_stats.p, sizeof(st3), hipMemcpyDeviceToHost, ctx->stream), "stats");
+  }
+  if (!dev_io) {
+    check(hipMemcpyAsync(out_value, ctx->d_out.p, npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+          "out");
+    if (out_level)
+      check(hipMemcpyAsync(out_level, ctx->d_outlevel.p, npts * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+            "lev");
+    if (out_conv) check(hipMemcpyAsync(out_conv, ctx->d_outconv.p, npts, hipMemcpyDeviceToHost, ctx->stream), "conv");
+    if (out_e)
+      check(hipMemcpyAsync(out_e, ctx->d_oute.p, 6 * npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+            "efield");
+  }
+  check(hipStreamSynchronize(ctx->stream), "sync");
+  if (stats) {
+    stats->evals = (double) st3[0];
+    stats->max_level = (int) st3[1] - 1;
+    stats->n_unconverged = (long) st3[2];
   }
 }
 
@@ -508,7 +522,7 @@ void launch_trajectories(om_context* ctx, int np) {
 
 int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
             const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
-            double* out_e, om_stats* stats, bool spectrum) {
+            double* out_e, om_stats* stats, bool spectrum, bool dev_io) {
   return guarded(ctx, [&] {
     Timer wall;
     check_particle(ctx, ip);
@@ -519,10 +533,19 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     int stop_mem, stop_ext;
     level_stops(*conv, stop_mem, stop_ext);
     if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
-    std::vector<double> om(n_e);
-    for (long i = 0; i < n_e; ++i) om[i] = ev_to_omega(energies_eV[i]);
-    upload(ctx, ctx->d_obs, obs_xyz, 3 * n_obs);
-    upload(ctx, ctx->d_omega, om.data(), n_e);
+    if (!obs_xyz || !energies_eV || !out_value) throw std::invalid_argument("missing inputs or outputs");
+    if (dev_io) {
+      ctx->d_obs.ensure(3 * n_obs);
+      ctx->d_omega.ensure(n_e);
+      check(hipMemcpyAsync(ctx->d_obs.p, obs_xyz, 3 * n_obs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
+            "obs");
+      check(om_launch_omega_from_ev(energies_eV, n_e, ctx->d_omega.p, kQe, kHbar, ctx->stream), "omega");
+    } else {
+      std::vector<double> om(n_e);
+      for (long i = 0; i < n_e; ++i) om[i] = ev_to_omega(energies_eV[i]);
+      upload(ctx, ctx->d_obs, obs_xyz, 3 * n_obs);
+      upload(ctx, ctx->d_omega, om.data(), n_e);
+    }
     bool mixed = conv->mode == OM_MODE_MIXED;
     bool mixed_any = mixed || conv->mode == OM_MODE_POINT_MIXED;
     om::LwArgs A;
@@ -647,7 +670,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     for (int L = 0; L <= om::kMaxLevel; ++L) F.dt_incl[L] = A.lv.dt_incl[L];
     std::memcpy(F.pol, A.pol, sizeof(F.pol));
     F.use_pol = A.use_pol;
-    finish_outputs(ctx, F, npts, out_value, out_level, out_conv, out_e, stats, *conv);
+    finish_outputs(ctx, F, npts, out_value, out_level, out_conv, out_e, stats, *conv, dev_io);
     if (stats) {
       stats->kernel_ms = kms;
       stats->total_ms = wall.ms();
@@ -814,28 +837,50 @@ int om_flux(om_context* ctx, int particle, const double* obs_xyz, long n_obs, co
             const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
             double* out_e, om_stats* stats) {
   return lw_sums(ctx, particle, obs_xyz, n_obs, energies_eV, n_e, pol6, conv, out_value, out_level, out_conv, out_e,
-                 stats, false);
+                 stats, false, false);
 }
 
 int om_spectrum(om_context* ctx, int particle, const double* obs, const double* energies_eV, long n_e,
                 const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
                 double* out_e, om_stats* stats) {
   return lw_sums(ctx, particle, obs, 1, energies_eV, n_e, pol6, conv, out_value, out_level, out_conv, out_e, stats,
-                 true);
+                 true, false);
 }
 
-int om_power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normal,
-                     int directional, const om_conv* conv, double* out_value, int* out_level,
-                     unsigned char* out_conv, om_stats* stats) {
+int om_flux_dev(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
+                const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
+                double* out_e, om_stats* stats) {
+  return lw_sums(ctx, particle, obs_xyz, n_obs, energies_eV, n_e, pol6, conv, out_value, out_level, out_conv, out_e,
+                 stats, false, true);
+}
+
+int om_spectrum_dev(om_context* ctx, int particle, const double* obs, const double* energies_eV, long n_e,
+                    const double* pol6, const om_conv* conv, double* out_value, int* out_level,
+                    unsigned char* out_conv, double* out_e, om_stats* stats) {
+  return lw_sums(ctx, particle, obs, 1, energies_eV, n_e, pol6, conv, out_value, out_level, out_conv, out_e, stats,
+                 true, true);
+}
+
+namespace {
+int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normal, int directional,
+                  const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv, om_stats* stats,
+                  bool dev_io) {
   return guarded(ctx, [&] {
     Timer wall;
     check_particle(ctx, ip);
     if (!conv) throw std::invalid_argument("conv is NULL");
     if (n_obs < 1) throw std::length_error("no points specified");
+    if (!obs_xyz || !normal || !out_value) throw std::invalid_argument("missing inputs or outputs");
     int stop_mem, stop_ext;
     level_stops(*conv, stop_mem, stop_ext);
     if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
-    upload(ctx, ctx->d_obs, obs_xyz, 3 * n_obs);
+    if (dev_io) {
+      ctx->d_obs.ensure(3 * n_obs);
+      check(hipMemcpyAsync(ctx->d_obs.p, obs_xyz, 3 * n_obs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
+            "obs");
+    } else {
+      upload(ctx, ctx->d_obs, obs_xyz, 3 * n_obs);
+    }
     om::PdArgs A;
     std::memset(&A, 0, sizeof(A));
     state_for(ctx, n_obs, false, A.st);
@@ -877,12 +922,27 @@ int om_power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs,
     F.pd_const = std::fabs(P.q * P.current) / (16 * (kPi * kPi) * kEps0 * kC);
     F.directional = directional;
     for (int L = 0; L <= om::kMaxLevel; ++L) F.dt_incl[L] = A.lv.dt_incl[L];
-    finish_outputs(ctx, F, n_obs, out_value, out_level, out_conv, nullptr, stats, *conv);
+    finish_outputs(ctx, F, n_obs, out_value, out_level, out_conv, nullptr, stats, *conv, dev_io);
     if (stats) {
       stats->kernel_ms = kms;
       stats->total_ms = wall.ms();
     }
   });
+}
+}  // namespace
+
+int om_power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normal,
+                     int directional, const om_conv* conv, double* out_value, int* out_level,
+                     unsigned char* out_conv, om_stats* stats) {
+  return power_density(ctx, ip, obs_xyz, n_obs, normal, directional, conv, out_value, out_level, out_conv, stats,
+                       false);
+}
+
+int om_power_density_dev(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normal,
+                         int directional, const om_conv* conv, double* out_value, int* out_level,
+                         unsigned char* out_conv, om_stats* stats) {
+  return power_density(ctx, ip, obs_xyz, n_obs, normal, directional, conv, out_value, out_level, out_conv, stats,
+                       true);
 }
 
 // batch counters: work-unit counter, unconverged points, evaluations, borderline points listed,
@@ -1190,7 +1250,7 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
   c.weight = 1;
   c.accumulate = 0;
   rc = lw_sums(ctx, 0, obs_xyz, n_obs, energies_eV, n_e, nullptr, &c, out_flux, nullptr, nullptr, E.data(), stats,
-               false);
+               false, false);
   if (rc) return rc;
   return guarded(ctx, [&] {
     auto& A = ctx->adj;

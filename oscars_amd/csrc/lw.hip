@@ -593,6 +593,41 @@ __global__ void __launch_bounds__(256) point_list_kernel(State st, long npts, in
   if (a) list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int) p;
 }
 
+// Photon energy [eV] to angular frequency, e * qe / hbar (the host's operations).
+__global__ void __launch_bounds__(256) omega_from_ev_kernel(const double* __restrict__ e, long n,
+                                                            double* __restrict__ om, double qe, double hbar) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) om[i] = e[i] * qe / hbar;
+}
+
+// Sum over points of 2^(L+1) - 1 (integer), highest level, unconverged count,
+// into out[0..2] (cleared by the launcher).  Every lane takes part in the
+// atomics (lane 0 of each wave adds its wave's totals).
+__global__ void __launch_bounds__(256) level_stats_kernel(const int* __restrict__ level,
+                                                          const unsigned char* __restrict__ conv, long n,
+                                                          unsigned long long* __restrict__ out) {
+  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  int L = p < n ? level[p] : -1;
+  unsigned long long ev = L >= 0 ? (unsigned long long) ((2L << L) - 1) : 0ULL;
+  unsigned long long nu = (p < n && !conv[p]) ? 1ULL : 0ULL;
+  int mx = L;
+  for (int off = 32; off > 0; off >>= 1) {
+    ev += __shfl_down(ev, off);
+    nu += __shfl_down(nu, off);
+    int o = __shfl_down(mx, off);
+    mx = o > mx ? o : mx;
+  }
+  atomicAdd(&out[0], lane == 0 ? ev : 0ULL);
+  atomicAdd(&out[2], lane == 0 ? nu : 0ULL);
+  atomicMax(&out[1], lane == 0 ? (unsigned long long) (mx + 1) : 0ULL);  // level + 1 (0: none)
+}
+
+__global__ void __launch_bounds__(64) u64_fill_kernel(unsigned long long* __restrict__ p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0ULL;
+}
+
 __global__ void __launch_bounds__(64) int_fill_kernel(int* __restrict__ p, int n, int v) {
   int i = threadIdx.x;
   if (i < n) p[i] = v;
@@ -674,6 +709,22 @@ extern "C" hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, i
 extern "C" hipError_t om_launch_lw_wave(const om::LwArgs* A, hipStream_t s) {
   long blocks = (A->n_list + 3) / 4;
   if (blocks > 0) hipLaunchKernelGGL(om::lw_wave_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *A);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_omega_from_ev(const double* e, long n, double* om, double qe, double hbar,
+                                               hipStream_t s) {
+  hipLaunchKernelGGL(om::omega_from_ev_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, e, n, om, qe,
+                     hbar);
+  return hipGetLastError();
+}
+
+// out[0] = sum of 2^(L+1)-1, out[1] = highest level + 1, out[2] = unconverged points.
+extern "C" hipError_t om_launch_level_stats(const int* level, const unsigned char* conv, long n,
+                                            unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(om::u64_fill_kernel, dim3(1), dim3(64), 0, s, out, 3);
+  hipLaunchKernelGGL(om::level_stats_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, level, conv, n,
+                     out);
   return hipGetLastError();
 }
 

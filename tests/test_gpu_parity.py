@@ -189,3 +189,45 @@ def test_grid_field_trajectory():
     print("grid trajectory: max channel-relative difference %.3e" % err)
     assert np.array_equal(t, ref.t)
     assert err < 1e-12
+
+
+@gpu
+def test_device_resident_entry_points_match_host(gpu_planar):
+    """om_flux_dev / om_spectrum_dev / om_power_density_dev on torch device
+    tensors give the host entry points' values and levels bit for bit."""
+    import torch
+    from oscars_amd import geometry as G
+    dev = gpu_planar._device()
+    obs, _, _ = G.rectangle("XY", 9, 7, 0.01, 0.01, [0, 0, 0], [0, 0, 30], 0, 2)
+    obs = np.array(obs)
+    e = np.array([152.11, 300.0, 455.0])
+    pol = [0.0] * 6
+    for mode in ("plain", "mixed", "point_mixed"):
+        conv = dev.conv(mode=mode)
+        v, lev, cv, ef, st = dev.flux(0, obs, e, pol, conv, want_e=True)
+        obs_t = torch.tensor(obs, dtype=torch.float64, device="cuda")
+        e_t = torch.tensor(e, dtype=torch.float64, device="cuda")
+        n = len(obs) * len(e)
+        out_t = torch.zeros(n, dtype=torch.float64, device="cuda")
+        lev_t = torch.zeros(n, dtype=torch.int32, device="cuda")
+        cv_t = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        ef_t = torch.zeros(n, 6, dtype=torch.float64, device="cuda")
+        st_d = dev.flux_dev(0, obs_t, e_t, pol, conv, out_t, lev_t, cv_t, ef_t)
+        assert np.array_equal(out_t.cpu().numpy(), v), mode
+        assert np.array_equal(lev_t.cpu().numpy(), lev), mode
+        assert np.array_equal(cv_t.cpu().numpy(), cv), mode
+        efd = ef_t.cpu().numpy()
+        assert np.array_equal(efd[:, 0::2] + 1j * efd[:, 1::2], ef), mode
+        assert st_d["evals"] == st["evals"] and st_d["max_level"] == st["max_level"]
+    conv = dev.conv()
+    v, lev, cv, _, _ = dev.spectrum(0, [0, 0, 30], e, pol, conv)
+    out_t = torch.zeros(len(e), dtype=torch.float64, device="cuda")
+    dev.spectrum_dev(0, torch.tensor([0.0, 0.0, 30.0], dtype=torch.float64, device="cuda"),
+                     torch.tensor(e, dtype=torch.float64, device="cuda"), pol, conv, out_t)
+    assert np.array_equal(out_t.cpu().numpy(), v)
+    v, lev, cv, _ = dev.power_density(0, obs, [0, 0, 1], 0, conv)
+    out_t = torch.zeros(len(obs), dtype=torch.float64, device="cuda")
+    lev_t = torch.zeros(len(obs), dtype=torch.int32, device="cuda")
+    dev.power_density_dev(0, torch.tensor(obs, dtype=torch.float64, device="cuda"), [0, 0, 1], 0, conv, out_t, lev_t)
+    assert np.array_equal(out_t.cpu().numpy(), v)
+    assert np.array_equal(lev_t.cpu().numpy(), lev)
