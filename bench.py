@@ -9,9 +9,11 @@ Workload (one step), a planar undulator (By 1.0 T, period 49 mm, 31 periods,
 One "eval" is one (observation point, energy, trajectory sample) integrand
 evaluation; a point converged at level L costs 2^(L+1)-1 evals.
 
-The headline mode (default "mixed") runs the single-energy maps with the
-per-point kernel (FP32 sin/cos of the exact FP64 phase) and the sweep with the
-observation x energy tiled kernel.  The plain FP64 path (one lane per output
+The headline mode (default "mixed") runs the single-energy maps with the FP64
+per-point kernel (with one energy there is nothing to share across lanes, the
+amplitude dominates, and the FP32 sin/cos variant is no faster once its
+borderline points are rechecked) and the sweep with the observation x energy
+tiled kernel with FP32 sin/cos of the exact FP64 phase.  The plain FP64 path (one lane per output
 point, everything FP64, the reference-faithful baseline) is timed on the same
 workload for the ratio, and the mixed result is compared with it point by point.
 
@@ -82,7 +84,7 @@ class Workload:
         dev.trajectories(s._fields, [s._om_particle(s._particle)])
         pol = [0.0] * 6
         out = {"evals": 0.0, "kernel_ms": {}, "values": {}, "levels": {}, "rechecked": {}}
-        map_mode = "plain" if mode == "plain" else "point_mixed"
+        map_mode = "plain"  # see the module docstring
         for e in ([] if self.only_sweep else self.map_energies):
             k = "map_%g" % e
             v, lev, cv = self.outs[k]
@@ -111,7 +113,7 @@ def host_api_step_ms(work, mode, steps=2):
     s = work.s
     dev = s._device()
     pol = [0.0] * 6
-    map_mode = "plain" if mode == "plain" else "point_mixed"
+    map_mode = "plain"
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):

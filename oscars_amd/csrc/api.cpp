@@ -43,7 +43,7 @@ hipError_t om_launch_tile_list(const om::State* st, long n_obs, long n_omega, in
                                hipStream_t s);
 hipError_t om_launch_point_list(const om::State* st, long npts, int La, int* list, int* count, hipStream_t s);
 hipError_t om_launch_border_list(const om::State* st, long npts, int* list, int* count, hipStream_t s);
-hipError_t om_launch_lw_wave(const om::LwArgs* A, hipStream_t s);
+hipError_t om_launch_lw_wave(const om::LwArgs* A, int mixed, hipStream_t s);
 hipError_t om_launch_omega_from_ev(const double* e, long n, double* om, double qe, double hbar, hipStream_t s);
 hipError_t om_launch_level_stats(const int* level, const unsigned char* conv, long n, unsigned long long* out,
                                  hipStream_t s);
@@ -418,6 +418,9 @@ long count_active(om_context* ctx, const om::State& st, long npts) {
 // First launch covers levels up to this one; later launches add one level
 // at a time for the points still running.
 constexpr int kFirstLevels = 13;
+// Per-point modes: levels with at most this many active points run one wave per
+// point (the per-point kernel would fill only a fraction of the GPU).
+constexpr long kWaveListMax = 16384;
 constexpr int kListFrom = 9;  // first level that can stop a point (src/OSCARSSR.cc:2006: L > 8)
 
 void check_particle(om_context* ctx, int ip) {
@@ -580,7 +583,10 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       A.La = La;
       A.Lb = Lb;
       check(hipEventRecord(ctx->ev0, ctx->stream), "event");
-      if (!tiled) {
+      if (!tiled && A.list && A.n_list <= kWaveListMax) {
+        // few points left: one wave per point
+        check(om_launch_lw_wave(&A, conv->mode == OM_MODE_POINT_MIXED ? 1 : 0, ctx->stream), "lw_wave");
+      } else if (!tiled) {
         check(om_launch_lw_point(&A, conv->mode == OM_MODE_POINT_MIXED ? 1 : 0, ctx->split, ctx->stream),
               "lw_point");
       } else if (A.list) {
@@ -638,7 +644,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
           A.La = La;
           A.Lb = Lb;
           check(hipEventRecord(ctx->ev0, ctx->stream), "event");
-          check(om_launch_lw_wave(&A, ctx->stream), "lw_wave recheck");
+          check(om_launch_lw_wave(&A, 0, ctx->stream), "lw_wave recheck");
           check(hipEventRecord(ctx->ev1, ctx->stream), "event");
           check(hipEventSynchronize(ctx->ev1), "sync");
           float ms = 0;

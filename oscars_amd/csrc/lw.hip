@@ -452,9 +452,13 @@ __global__ void __launch_bounds__(256) lw_finalize_kernel(FinArgs F) {
   if (F.out_conv) F.out_conv[p] = (unsigned char) conv;
 }
 
-// FP64 recomputation of listed points (the mixed path's borderline points):
-// one wave per point, the lanes splitting each level's samples, levels La..Lb
-// from the point's state.  Four waves (points) per workgroup.
+// Listed points, one wave per point, the lanes splitting each level's
+// samples, levels La..Lb from the point's state.  Used where few points are
+// active (the last levels of a map, and the FP64 recomputation of the mixed
+// path's borderline points): a wave per point keeps their latency low.
+// MIXED: FP32 sin/cos with the error-bound bookkeeping of the per-point kernel.
+// Four waves (points) per workgroup.
+template <bool MIXED>
 __global__ void __launch_bounds__(256) lw_wave_kernel(LwArgs A) {
   int lane = threadIdx.x & 63;
   long g = (long) blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -470,16 +474,28 @@ __global__ void __launch_bounds__(256) lw_wave_kernel(LwArgs A) {
   for (int k = 0; k < 6; ++k) sum[k] = A.st.sum[6 * p + k];
   double lastmag = A.st.lastmag[p];
   double prec = A.st.prec[p];
+  double an_run = 0, dm_last = 0;
+  bool border = false;
+  if (MIXED) {
+    an_run = A.st.err[2 * p];
+    dm_last = A.st.err[2 * p + 1];
+  }
   int lev = A.La - 1;
   bool c = false;
   for (int L = A.La; L <= A.Lb && !c; ++L) {
     const double* __restrict__ rows = A.lv.rows[L];
     double tot[6], maxd, an;
-    slice_level_sum_rows<false>(rows, 0, 1L << L, obs, negw, lane, tot, maxd, an);
+    slice_level_sum_rows<MIXED>(rows, 0, 1L << L, obs, negw, lane, tot, maxd, an);
 #pragma unroll
     for (int k = 0; k < 6; ++k) sum[k] += tot[k];
     lev = L;
+    double mag_prev = lastmag;
     c = level_converged(A, L, sum, maxd, lastmag, prec);
+    if (MIXED) {
+      an_run += an;
+      border = border ||
+               level_borderline(L, A.lv.dt_incl[L], A.precision, mag_prev, lastmag, prec, maxd, an_run, dm_last);
+    }
   }
   if (lane == 0) {
 #pragma unroll
@@ -488,6 +504,11 @@ __global__ void __launch_bounds__(256) lw_wave_kernel(LwArgs A) {
     A.st.prec[p] = prec;
     A.st.level[p] = lev;
     if (c) A.st.conv[p] = 1;
+    if (MIXED) {
+      A.st.err[2 * p] = an_run;
+      A.st.err[2 * p + 1] = dm_last;
+      if (border) A.st.border[p] = 1;
+    }
   }
 }
 
@@ -706,9 +727,14 @@ extern "C" hipError_t om_launch_lw_tile_list(const om::LwArgs* A, long ntiles, i
   return hipGetLastError();
 }
 
-extern "C" hipError_t om_launch_lw_wave(const om::LwArgs* A, hipStream_t s) {
+extern "C" hipError_t om_launch_lw_wave(const om::LwArgs* A, int mixed, hipStream_t s) {
   long blocks = (A->n_list + 3) / 4;
-  if (blocks > 0) hipLaunchKernelGGL(om::lw_wave_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *A);
+  if (blocks > 0) {
+    if (mixed)
+      hipLaunchKernelGGL(om::lw_wave_kernel<true>, dim3((unsigned) blocks), dim3(256), 0, s, *A);
+    else
+      hipLaunchKernelGGL(om::lw_wave_kernel<false>, dim3((unsigned) blocks), dim3(256), 0, s, *A);
+  }
   return hipGetLastError();
 }
 
