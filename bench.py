@@ -76,12 +76,20 @@ class Workload:
             self.outs[k] = (torch.zeros(n, dtype=torch.float64, **cuda), torch.zeros(n, dtype=torch.int32, **cuda),
                             torch.zeros(n, dtype=torch.uint8, **cuda))
 
+    def traj(self):
+        """RK4 + spline on the device (returns once queued on this context's stream)."""
+        s = self.s
+        s._device().trajectories(s._fields, [s._om_particle(s._particle)], check_status=False)
+
     def step(self, mode):
         """One pass; returns dict of per-phase evals / kernel ms / output tensors (device)."""
+        self.traj()
+        return self.sums(mode)
+
+    def sums(self, mode):
+        """The step's flux sums on the resident trajectory (level samples are rebuilt here)."""
         s = self.s
         dev = s._device()
-        # RK4 + spline on the device; level samples are rebuilt inside the step
-        dev.trajectories(s._fields, [s._om_particle(s._particle)])
         pol = [0.0] * 6
         out = {"evals": 0.0, "kernel_ms": {}, "values": {}, "levels": {}, "rechecked": {}}
         map_mode = "plain"  # see the module docstring
@@ -133,20 +141,39 @@ def to_host(res):
     return res
 
 
-def timed(work, mode, steps, warmup):
+def run_steps(works, mode, n):
+    """n steps.  With two workloads (two contexts, each on its own HIP stream) the
+    steps are software-pipelined: the trajectory of step k+1 is queued on the
+    other context before step k's sums, so the serial RK4 (one wave) runs beside
+    the sums instead of before them.  Every step still integrates its own
+    trajectory and computes all its sums."""
+    res, evals, kms = None, 0.0, {}
+    if n <= 0:
+        return res, evals, kms
+    if len(works) == 2:
+        works[0].traj()
+        for k in range(n):
+            if k + 1 < n:
+                works[(k + 1) % 2].traj()
+            res = works[k % 2].sums(mode)
+            evals += res["evals"]
+            for key, v in res["kernel_ms"].items():
+                kms[key] = kms.get(key, 0.0) + v
+    else:
+        for _ in range(n):
+            res = works[0].step(mode)
+            evals += res["evals"]
+            for key, v in res["kernel_ms"].items():
+                kms[key] = kms.get(key, 0.0) + v
+    return res, evals, kms
+
+
+def timed(works, mode, steps, warmup):
     import torch
-    for _ in range(warmup):
-        work.step(mode)
+    run_steps(works, mode, warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = None
-    evals = 0.0
-    kms = {}
-    for _ in range(steps):
-        res = work.step(mode)
-        evals += res["evals"]
-        for k, v in res["kernel_ms"].items():
-            kms[k] = kms.get(k, 0.0) + v
+    res, evals, kms = run_steps(works, mode, steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return (t1 - t0), evals, {k: v / steps for k, v in kms.items()}, to_host(res)
@@ -356,10 +383,12 @@ def batched_share(args, particles):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="mixed", choices=["mixed", "tiled", "plain"])
-    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=4)
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="run the steps one after the other on one context (no trajectory/sums overlap)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--only-sweep", action="store_true",
@@ -398,9 +427,15 @@ def main():
     scenario(s)
     work = Workload(s)
     work.only_sweep = args.only_sweep
+    works = [work]
+    if not args.no_pipeline:
+        s2 = sr()
+        scenario(s2)
+        works.append(Workload(s2))
+        works[1].only_sweep = args.only_sweep
     print("# device: %s" % s._device().info(), file=sys.stderr, flush=True)
 
-    dt, evals, kms, res = timed(work, args.mode, args.steps, args.warmup)
+    dt, evals, kms, res = timed(works, args.mode, args.steps, args.warmup)
     ms_per_step = 1e3 * dt / args.steps
     value = evals / dt
     print("# %s: %.3f ms/step, %.4g evals/step, %.4g evals/s, kernel ms %s" %
@@ -440,8 +475,15 @@ def main():
                 "valu_issue_frac": tj.get("valu_issue_frac"),
                 "valu_wave_instr_per_eval": tj.get("valu_wave_instr_per_eval")}
 
+    if len(works) == 2:
+        # the same steps one after the other on one context (trajectory, then sums)
+        sdt, sevals, _, _ = timed([work], args.mode, 2, 0)
+        extra["sequential"] = {"ms_per_step": 1e3 * sdt / 2, "evals_per_s": sevals / sdt,
+                               "note": "steps not pipelined: each step's RK4 runs before its sums"}
+        print("# sequential: %.3f ms/step" % (1e3 * sdt / 2), file=sys.stderr, flush=True)
+
     if not args.no_plain and args.mode != "plain":
-        pdt, pevals, pkms, pres = timed(work, "plain", args.plain_steps, 0)
+        pdt, pevals, pkms, pres = timed(works, "plain", args.plain_steps, 1 if len(works) == 2 else 0)
         pvalue = pevals / pdt
         extra["plain_fp64"] = {"value": pvalue, "ms_per_step": 1e3 * pdt / args.plain_steps, "kernel_ms": pkms}
         extra["ratio_vs_plain"] = value / pvalue
@@ -495,7 +537,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "planar undulator 1.0 T/49 mm/31 periods, 3 GeV 0.5 A, 20001-point RK4; "
                                "flux 201x201 (+-10 mm @30 m) at 152.11 and 260 eV + 51x51 x 512 energies "
-                               "100-3000 eV", "mode": args.mode, "parallelism": "single GPU"},
+                               "100-3000 eV", "mode": args.mode, "parallelism": "single GPU",
+                   "steps_pipelined": len(works) == 2},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

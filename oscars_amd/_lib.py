@@ -157,10 +157,15 @@ class Device:
         self.L.om_device_info(self.device, buf, 512)
         return buf.value.decode()
 
-    def trajectories(self, fields, particles):
+    def trajectories(self, fields, particles, check_status=True):
+        """RK4 + spline for `particles`.  check_status=False returns as soon as the
+        work is queued; a |beta| >= 1 trajectory is then reported by the next
+        call that uses it (every sum checks the status first)."""
         fa = (om_field * max(1, len(fields)))(*fields)
         pa = (om_particle * len(particles))(*particles)
         self._chk(self.L.om_trajectories(self.h, fa, len(fields), pa, len(particles)))
+        if not check_status:
+            return
         for i in range(len(particles)):
             if self.L.om_trajectory_status(self.h, i) != 0:
                 raise ValueError("trajectory %d: |beta| >= 1 met during RK4 integration" % i)

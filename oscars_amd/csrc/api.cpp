@@ -104,6 +104,42 @@ struct DevBuf {
   }
 };
 
+// Pinned host staging for small uploads that must not block the host (a copy
+// from pageable memory can): the buffer is rewritten only after the copy
+// queued from it has run.
+struct PinnedStage {
+  void* p = nullptr;
+  size_t n = 0;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+  void* get(size_t bytes) {
+    if (pending) check(hipEventSynchronize(done), "staging wait");
+    pending = false;
+    if (!done) check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "event");
+    if (bytes > n) {
+      if (p) check(hipHostFree(p), "hipHostFree");
+      p = nullptr;
+      n = 0;
+      check(hipHostMalloc(&p, bytes, 0), "hipHostMalloc");
+      n = bytes;
+    }
+    return p;
+  }
+  void queued(hipStream_t s) {
+    check(hipEventRecord(done, s), "event");
+    pending = true;
+  }
+  void release() {
+    if (pending && done) (void) hipEventSynchronize(done);
+    if (p) (void) hipHostFree(p);
+    if (done) (void) hipEventDestroy(done);
+    p = nullptr;
+    n = 0;
+    done = nullptr;
+    pending = false;
+  }
+};
+
 struct LevelCache {
   double* rows[om::kMaxLevel + 1] = {};  // device buffers, kept across trajectory sets
   bool built[om::kMaxLevel + 1] = {};    // rows hold samples of the current trajectory
@@ -125,6 +161,7 @@ struct om_context {
   int np = 0;
   std::vector<om::TrajDesc> desc;
   DevBuf<om::TrajDesc> d_desc;
+  PinnedStage stage_desc, stage_fields, stage_rb;
   DevBuf<double> d_t, d_p, d_ypp, d_u, d_sig, d_sm1, d_pv, d_ppv, d_u2;
   std::vector<double> tstart, tstop;
   std::vector<om_particle> parts;
@@ -140,6 +177,7 @@ struct om_context {
   std::vector<om::FieldDev> fields_host;
   int nfields = 0;
   long total_knots = 0;
+  bool desc_pending = false;  // trajectories launched, status not read back yet
   // batched multi-particle state
   struct Batch {
     bool ready = false;
@@ -211,6 +249,7 @@ struct om_context {
     free_levels();
     for (auto& g : grid_bufs) g.release();
     d_desc.release(); d_t.release(); d_p.release(); d_ypp.release(); d_u.release(); d_fields.release();
+    stage_desc.release(); stage_fields.release(); stage_rb.release();
     d_sig.release(); d_sm1.release(); d_pv.release(); d_ppv.release(); d_u2.release();
     d_obs.release(); d_omega.release(); d_sum.release(); d_err.release(); d_lastmag.release();
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
@@ -223,6 +262,17 @@ struct om_context {
 };
 
 namespace {
+
+// Device-to-host read on the context's stream through pinned memory, then a wait
+// for this stream only (a copy into pageable memory can also wait for work
+// queued on other streams).
+void read_back(om_context* ctx, void* dst, const void* dsrc, size_t bytes) {
+  void* ph = ctx->stage_rb.get(bytes);
+  check(hipMemcpyAsync(ph, dsrc, bytes, hipMemcpyDeviceToHost, ctx->stream), "read back");
+  check(hipStreamSynchronize(ctx->stream), "sync");
+  std::memcpy(dst, ph, bytes);
+}
+
 
 template <class F>
 int guarded(om_context* ctx, F&& f) {
@@ -364,8 +414,7 @@ void ensure_beta_diff(om_context* ctx, int ip, int Lb) {
   }
   if (!any) return;
   double h[om::kMaxLevel + 1];
-  check(hipMemcpyAsync(h, ctx->d_bd.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "bd copy");
-  check(hipStreamSynchronize(ctx->stream), "sync");
+  read_back(ctx, h, ctx->d_bd.p, sizeof(h));
   for (int L = 0; L <= Lb; ++L) {
     if (!lc.have_bd[L]) {
       lc.beta_diff[L] = h[L];
@@ -410,8 +459,7 @@ long count_active(om_context* ctx, const om::State& st, long npts) {
   check(hipMemcpyAsync(ctx->d_count.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream), "count init");
   check(om_launch_count_active(&st, npts, ctx->d_count.p, ctx->stream), "count active");
   int h = 0;
-  check(hipMemcpyAsync(&h, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "count copy");
-  check(hipStreamSynchronize(ctx->stream), "sync");
+  read_back(ctx, &h, ctx->d_count.p, sizeof(int));
   return h;
 }
 
@@ -423,8 +471,16 @@ constexpr int kFirstLevels = 13;
 constexpr long kWaveListMax = 16384;
 constexpr int kListFrom = 9;  // first level that can stop a point (src/OSCARSSR.cc:2006: L > 8)
 
+// Waits for the last om_trajectories and reads back the per-trajectory status.
+void sync_desc(om_context* ctx) {
+  if (!ctx->desc_pending) return;
+  read_back(ctx, ctx->desc.data(), ctx->d_desc.p, ctx->desc.size() * sizeof(om::TrajDesc));
+  ctx->desc_pending = false;
+}
+
 void check_particle(om_context* ctx, int ip) {
   if (ip < 0 || ip >= ctx->np) throw std::out_of_range("particle index out of range (no trajectory computed?)");
+  sync_desc(ctx);
   if (ctx->desc[ip].status != 0)
     throw std::out_of_range("trajectory integration met |beta| >= 1; step halving is not supported on the device");
 }
@@ -433,6 +489,16 @@ template <class T>
 void upload(om_context* ctx, DevBuf<T>& b, const T* h, size_t n) {
   b.ensure(n);
   check(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+}
+
+// upload through pinned staging: returns without waiting for the device
+template <class T>
+void upload_staged(om_context* ctx, PinnedStage& st, DevBuf<T>& b, const T* h, size_t n) {
+  b.ensure(n);
+  void* ph = st.get(n * sizeof(T));
+  std::memcpy(ph, h, n * sizeof(T));
+  check(hipMemcpyAsync(b.p, ph, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  st.queued(ctx->stream);
 }
 
 struct Timer {
@@ -479,7 +545,6 @@ void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_valu
   if (stats) {
     ctx->d_stats.ensure(3);
     check(om_launch_level_stats(F.out_level, F.out_conv, npts, ctx->d_stats.p, ctx->stream), "level stats");
-    check(hipMemcpyAsync(st3, ctx->d_stats.p, sizeof(st3), hipMemcpyDeviceToHost, ctx->stream), "stats");
   }
   if (!dev_io) {
     check(hipMemcpyAsync(out_value, ctx->d_out.p, npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
@@ -492,7 +557,8 @@ void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_valu
       check(hipMemcpyAsync(out_e, ctx->d_oute.p, 6 * npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
             "efield");
   }
-  check(hipStreamSynchronize(ctx->stream), "sync");
+  if (stats) read_back(ctx, st3, ctx->d_stats.p, sizeof(st3));  // also waits for the copies above
+  else check(hipStreamSynchronize(ctx->stream), "sync");
   if (stats) {
     stats->evals = (double) st3[0];
     stats->max_level = (int) st3[1] - 1;
@@ -610,8 +676,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       else
         check(om_launch_point_list(&A.st, npts, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "point list");
       int nt = 0;
-      check(hipMemcpyAsync(&nt, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "list count");
-      check(hipStreamSynchronize(ctx->stream), "sync");
+      read_back(ctx, &nt, ctx->d_count.p, sizeof(int));
       if (nt == 0) break;
       ntiles = nt;
       A.list = ctx->d_list.p;
@@ -626,8 +691,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       ctx->d_count.ensure(1);
       check(om_launch_border_list(&A.st, npts, ctx->d_list.p, ctx->d_count.p, ctx->stream), "border list");
       int nb = 0;
-      check(hipMemcpyAsync(&nb, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "border count");
-      check(hipStreamSynchronize(ctx->stream), "sync");
+      read_back(ctx, &nb, ctx->d_count.p, sizeof(int));
       n_recheck = nb;
       if (nb > 0) {
         om::State st64 = A.st;
@@ -655,8 +719,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
           Lb = La;
           check(om_launch_point_list(&A.st, npts, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "point list");
           int nt = 0;
-          check(hipMemcpyAsync(&nt, ctx->d_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "count");
-          check(hipStreamSynchronize(ctx->stream), "sync");
+          read_back(ctx, &nt, ctx->d_count.p, sizeof(int));
           if (nt == 0) break;
           A.n_list = nt;
         }
@@ -750,7 +813,7 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->fields_host = F;
     ctx->nfields = nfields;
     ctx->d_fields.ensure(nfields > 0 ? nfields : 1);
-    if (nfields > 0) upload(ctx, ctx->d_fields, F.data(), F.size());
+    if (nfields > 0) upload_staged(ctx, ctx->stage_fields, ctx->d_fields, F.data(), F.size());
     ctx->np = 0;
     ctx->desc.assign(np, om::TrajDesc());
     ctx->parts.assign(particles, particles + np);
@@ -795,11 +858,11 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->d_pv.ensure(total);
     ctx->d_ppv.ensure(total);
     ctx->d_u2.ensure(9 * total);
-    upload(ctx, ctx->d_desc, ctx->desc.data(), np);
+    upload_staged(ctx, ctx->stage_desc, ctx->d_desc, ctx->desc.data(), np);
     launch_trajectories(ctx, np);
-    check(hipMemcpyAsync(ctx->desc.data(), ctx->d_desc.p, np * sizeof(om::TrajDesc), hipMemcpyDeviceToHost,
-                         ctx->stream), "desc copy");
-    check(hipStreamSynchronize(ctx->stream), "sync");
+    // The integration runs on; its status is read back by the first call that
+    // needs it (sync_desc), so the host can queue other work meanwhile.
+    ctx->desc_pending = true;
     if ((int) ctx->levels.size() != np) ctx->levels.assign(np, LevelCache());
     ctx->np = np;
   });
@@ -807,6 +870,7 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
 
 int om_trajectory_status(om_context* ctx, int i) {
   if (!ctx || i < 0 || i >= ctx->np) return -1;
+  if (guarded(ctx, [&] { sync_desc(ctx); })) return -1;
   return ctx->desc[i].status;
 }
 
@@ -981,6 +1045,7 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     if (n_req < 1 || !req || !req_of || !weight || !conv) throw std::invalid_argument("missing batch inputs");
     if (conv->mode != OM_MODE_TILED && conv->mode != OM_MODE_MIXED)
       throw std::invalid_argument("batched sums use the tiled kernels: mode must be TILED or MIXED");
+    sync_desc(ctx);
     for (int i = 0; i < np; ++i) {
       if (ctx->desc[i].status != 0) throw std::out_of_range("trajectory integration met |beta| >= 1");
       if (req_of[i] < 0 || req_of[i] >= n_req) throw std::out_of_range("req_of out of range");
@@ -1155,7 +1220,7 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
     float ms = 0;
     check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
     unsigned long long cnt[kBatchCounters];
-    check(hipMemcpy(cnt, B.counters.p, sizeof(cnt), hipMemcpyDeviceToHost), "counters");
+    read_back(ctx, cnt, B.counters.p, sizeof(cnt));
     if (out) {
       for (int r = 0; r < B.n_req; ++r)
         if (out[r])
@@ -1241,6 +1306,7 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
   if (rc) return rc;
   std::vector<double> E(6 * n_obs * n_e);
   rc = guarded(ctx, [&] {
+    sync_desc(ctx);
     if (ctx->desc[0].status != 0) throw std::out_of_range("trajectory integration met |beta| >= 1");
     if (ctx->desc[0].nb != 0)
       throw std::invalid_argument("the reverse pass needs ctstart == T0 (no backward integration)");

@@ -54,6 +54,10 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const GridZ
                                                  int nfields, TrajDesc* __restrict__ desc, int np,
                                                  double* __restrict__ t, double* __restrict__ p) {
   extern __shared__ double lds_grid[];
+  // Highest issue priority for these waves: the integration is one dependent
+  // chain per particle, and when it runs beside other kernels (the previous
+  // step's sums) it should not wait behind their waves on its SIMD.
+  __builtin_amdgcn_s_setprio(3);
   int ip = blockIdx.x * blockDim.x + threadIdx.x;
   RkCtx c{F, nfields, 0.0, 0, g};
   if (M == kGridZ) {
