@@ -530,16 +530,37 @@ __global__ void __launch_bounds__(256) state_reset_kernel(State st, long npts) {
 
 // Points flagged by the mixed path, listed (any order) and reset to the start
 // of the level loop for the FP64 recomputation.
+// Slots for the threads of a 256-thread workgroup with `want` set, taken from
+// *counter with one atomic per workgroup (a counter shared by every workgroup
+// of a large grid serialises its atomics): wave counts are combined in LDS, the
+// base is broadcast, and each thread's slot follows its wave's and lane's order.
+// Every thread of the workgroup must call it (it contains barriers).
+__device__ __forceinline__ int block_reserve(bool want, int* counter, int* sh /* 5 ints of LDS */) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long m = __ballot(want);
+  if (lane == 0) sh[w] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int i = 0; i < 4; ++i) {
+      int c = sh[i];
+      sh[i] = tot;
+      tot += c;
+    }
+    sh[4] = tot > 0 ? atomicAdd(counter, tot) : 0;
+  }
+  __syncthreads();
+  return sh[4] + sh[w] + __popcll(m & ((1ULL << lane) - 1ULL));
+}
+
 __global__ void __launch_bounds__(256) border_list_kernel(State st, long npts, int* __restrict__ list,
                                                           int* __restrict__ count) {
+  __shared__ int sh[5];
   long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  int lane = threadIdx.x & 63;
   bool a = p < npts && st.border[p];
-  unsigned long long m = __ballot(a);
-  int mine = atomicAdd(count, lane == 0 ? __popcll(m) : 0);
-  int base = __builtin_amdgcn_readfirstlane(mine);
+  int slot = block_reserve(a, count, sh);
   if (!a) return;
-  list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int) p;
+  list[slot] = (int) p;
 #pragma unroll
   for (int k = 0; k < 6; ++k) st.sum[6 * p + k] = 0;
   st.lastmag[p] = -1;
@@ -605,13 +626,11 @@ __global__ void __launch_bounds__(256) tile_list_kernel(State st, long n_omega, 
 // adds its wave's count) and the wave's base comes from readfirstlane.
 __global__ void __launch_bounds__(256) point_list_kernel(State st, long npts, int La, int* __restrict__ list,
                                                          int* __restrict__ count) {
+  __shared__ int sh[5];
   long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  int lane = threadIdx.x & 63;
   bool a = p < npts && !st.conv[p] && st.level[p] == La - 1;
-  unsigned long long m = __ballot(a);
-  int mine = atomicAdd(count, lane == 0 ? __popcll(m) : 0);
-  int base = __builtin_amdgcn_readfirstlane(mine);
-  if (a) list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int) p;
+  int slot = block_reserve(a, count, sh);
+  if (a) list[slot] = (int) p;
 }
 
 // Photon energy [eV] to angular frequency, e * qe / hbar (the host's operations).
@@ -622,26 +641,49 @@ __global__ void __launch_bounds__(256) omega_from_ev_kernel(const double* __rest
 }
 
 // Sum over points of 2^(L+1) - 1 (integer), highest level, unconverged count,
-// into out[0..2] (cleared by the launcher).  Every lane takes part in the
-// atomics (lane 0 of each wave adds its wave's totals).
+// into out[0..2] (cleared by the launcher): each thread takes kStatsPer points,
+// waves reduce with shuffles, and thread 0 of each workgroup does the atomics.
+constexpr int kStatsPer = 8;
+
 __global__ void __launch_bounds__(256) level_stats_kernel(const int* __restrict__ level,
                                                           const unsigned char* __restrict__ conv, long n,
                                                           unsigned long long* __restrict__ out) {
-  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  int lane = threadIdx.x & 63;
-  int L = p < n ? level[p] : -1;
-  unsigned long long ev = L >= 0 ? (unsigned long long) ((2L << L) - 1) : 0ULL;
-  unsigned long long nu = (p < n && !conv[p]) ? 1ULL : 0ULL;
-  int mx = L;
+  __shared__ unsigned long long sh[3][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long ev = 0, nu = 0;
+  int mx = -1;
+  for (int k = 0; k < kStatsPer; ++k) {
+    long p = ((long) blockIdx.x * kStatsPer + k) * blockDim.x + threadIdx.x;
+    if (p < n) {
+      int L = level[p];
+      ev += L >= 0 ? (unsigned long long) ((2L << L) - 1) : 0ULL;
+      nu += conv[p] ? 0ULL : 1ULL;
+      mx = L > mx ? L : mx;
+    }
+  }
   for (int off = 32; off > 0; off >>= 1) {
     ev += __shfl_down(ev, off);
     nu += __shfl_down(nu, off);
     int o = __shfl_down(mx, off);
     mx = o > mx ? o : mx;
   }
-  atomicAdd(&out[0], lane == 0 ? ev : 0ULL);
-  atomicAdd(&out[2], lane == 0 ? nu : 0ULL);
-  atomicMax(&out[1], lane == 0 ? (unsigned long long) (mx + 1) : 0ULL);  // level + 1 (0: none)
+  if (lane == 0) {
+    sh[0][w] = ev;
+    sh[1][w] = nu;
+    sh[2][w] = (unsigned long long) (mx + 1);  // level + 1 (0: none)
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long e = 0, u = 0, m = 0;
+    for (int i = 0; i < 4; ++i) {
+      e += sh[0][i];
+      u += sh[1][i];
+      m = sh[2][i] > m ? sh[2][i] : m;
+    }
+    atomicAdd(&out[0], e);
+    atomicAdd(&out[2], u);
+    atomicMax(&out[1], m);
+  }
 }
 
 __global__ void __launch_bounds__(64) u64_fill_kernel(unsigned long long* __restrict__ p, int n) {
@@ -749,8 +791,9 @@ extern "C" hipError_t om_launch_omega_from_ev(const double* e, long n, double* o
 extern "C" hipError_t om_launch_level_stats(const int* level, const unsigned char* conv, long n,
                                             unsigned long long* out, hipStream_t s) {
   hipLaunchKernelGGL(om::u64_fill_kernel, dim3(1), dim3(64), 0, s, out, 3);
-  hipLaunchKernelGGL(om::level_stats_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, level, conv, n,
-                     out);
+  long per_block = 256L * om::kStatsPer;
+  hipLaunchKernelGGL(om::level_stats_kernel, dim3((unsigned) ((n + per_block - 1) / per_block)), dim3(256), 0, s,
+                     level, conv, n, out);
   return hipGetLastError();
 }
 
