@@ -140,6 +140,43 @@ struct PinnedStage {
   }
 };
 
+// HIP event pairs around kernel launches, read once after the stream has been
+// synchronised (no wait per launch).  Events are created on first use and kept.
+struct LaunchTimer {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  void begin(hipStream_t s) {
+    if (ev.size() < 2 * used + 2) {
+      hipEvent_t a = nullptr, b = nullptr;
+      check(hipEventCreate(&a), "event");
+      check(hipEventCreate(&b), "event");
+      ev.push_back(a);
+      ev.push_back(b);
+    }
+    check(hipEventRecord(ev[2 * used], s), "event");
+  }
+  void end(hipStream_t s) {
+    check(hipEventRecord(ev[2 * used + 1], s), "event");
+    ++used;
+  }
+  // Sum of the recorded intervals [ms]; the stream must have been synchronised.
+  double take() {
+    double t = 0;
+    for (size_t i = 0; i < used; ++i) {
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "elapsed");
+      t += ms;
+    }
+    used = 0;
+    return t;
+  }
+  void release() {
+    for (auto e : ev) (void) hipEventDestroy(e);
+    ev.clear();
+    used = 0;
+  }
+};
+
 struct LevelCache {
   double* rows[om::kMaxLevel + 1] = {};  // device buffers, kept across trajectory sets
   bool built[om::kMaxLevel + 1] = {};    // rows hold samples of the current trajectory
@@ -162,6 +199,7 @@ struct om_context {
   std::vector<om::TrajDesc> desc;
   DevBuf<om::TrajDesc> d_desc;
   PinnedStage stage_desc, stage_fields, stage_rb;
+  LaunchTimer timer;
   DevBuf<double> d_t, d_p, d_ypp, d_u, d_sig, d_sm1, d_pv, d_ppv, d_u2;
   std::vector<double> tstart, tstop;
   std::vector<om_particle> parts;
@@ -250,6 +288,7 @@ struct om_context {
     for (auto& g : grid_bufs) g.release();
     d_desc.release(); d_t.release(); d_p.release(); d_ypp.release(); d_u.release(); d_fields.release();
     stage_desc.release(); stage_fields.release(); stage_rb.release();
+    timer.release();
     d_sig.release(); d_sm1.release(); d_pv.release(); d_ppv.release(); d_u2.release();
     d_obs.release(); d_omega.release(); d_sum.release(); d_err.release(); d_lastmag.release();
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
@@ -455,8 +494,6 @@ void state_for(om_context* ctx, long npts, bool mixed, om::State& st) {
 
 long count_active(om_context* ctx, const om::State& st, long npts) {
   ctx->d_count.ensure(1);
-  int zero = 0;
-  check(hipMemcpyAsync(ctx->d_count.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream), "count init");
   check(om_launch_count_active(&st, npts, ctx->d_count.p, ctx->stream), "count active");
   int h = 0;
   read_back(ctx, &h, ctx->d_count.p, sizeof(int));
@@ -594,6 +631,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
             double* out_e, om_stats* stats, bool spectrum, bool dev_io) {
   return guarded(ctx, [&] {
     Timer wall;
+    ctx->timer.used = 0;
     check_particle(ctx, ip);
     if (!conv) throw std::invalid_argument("conv is NULL");
     if (n_obs < 1 || n_e < 1) throw std::length_error("no points specified");
@@ -648,7 +686,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       fill_levels(ctx, ip, Lb, A.lv);
       A.La = La;
       A.Lb = Lb;
-      check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+      ctx->timer.begin(ctx->stream);
       if (!tiled && A.list && A.n_list <= kWaveListMax) {
         // few points left: one wave per point
         check(om_launch_lw_wave(&A, conv->mode == OM_MODE_POINT_MIXED ? 1 : 0, ctx->stream), "lw_wave");
@@ -660,11 +698,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       } else {
         check(om_launch_lw_tile(&A, n_obs, mixed ? 1 : 0, ctx->split, ctx->stream), "lw_tile");
       }
-      check(hipEventRecord(ctx->ev1, ctx->stream), "event");
-      check(hipEventSynchronize(ctx->ev1), "sync");
-      float ms = 0;
-      check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
-      kms += ms;
+      ctx->timer.end(ctx->stream);
       Ldone = Lb;
       if (Lb >= stop_ext) break;
       La = Lb + 1;
@@ -707,13 +741,9 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
           fill_levels(ctx, ip, Lb, A.lv);
           A.La = La;
           A.Lb = Lb;
-          check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+          ctx->timer.begin(ctx->stream);
           check(om_launch_lw_wave(&A, 0, ctx->stream), "lw_wave recheck");
-          check(hipEventRecord(ctx->ev1, ctx->stream), "event");
-          check(hipEventSynchronize(ctx->ev1), "sync");
-          float ms = 0;
-          check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
-          kms += ms;
+          ctx->timer.end(ctx->stream);
           if (Lb >= stop_ext) break;
           La = Lb + 1;
           Lb = La;
@@ -740,6 +770,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     std::memcpy(F.pol, A.pol, sizeof(F.pol));
     F.use_pol = A.use_pol;
     finish_outputs(ctx, F, npts, out_value, out_level, out_conv, out_e, stats, *conv, dev_io);
+    kms = ctx->timer.take();  // the stream is synchronised
     if (stats) {
       stats->kernel_ms = kms;
       stats->total_ms = wall.ms();
@@ -937,6 +968,7 @@ int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, co
                   bool dev_io) {
   return guarded(ctx, [&] {
     Timer wall;
+    ctx->timer.used = 0;
     check_particle(ctx, ip);
     if (!conv) throw std::invalid_argument("conv is NULL");
     if (n_obs < 1) throw std::length_error("no points specified");
@@ -969,13 +1001,9 @@ int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, co
       fill_levels(ctx, ip, Lb, A.lv);
       A.La = La;
       A.Lb = Lb;
-      check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+      ctx->timer.begin(ctx->stream);
       check(om_launch_pd_point(&A, ctx->stream), "pd_point");
-      check(hipEventRecord(ctx->ev1, ctx->stream), "event");
-      check(hipEventSynchronize(ctx->ev1), "sync");
-      float ms = 0;
-      check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
-      kms += ms;
+      ctx->timer.end(ctx->stream);
       if (Lb >= stop_ext) break;
       if (count_active(ctx, A.st, n_obs) == 0) break;
       La = Lb + 1;
@@ -993,6 +1021,7 @@ int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, co
     F.directional = directional;
     for (int L = 0; L <= om::kMaxLevel; ++L) F.dt_incl[L] = A.lv.dt_incl[L];
     finish_outputs(ctx, F, n_obs, out_value, out_level, out_conv, nullptr, stats, *conv, dev_io);
+    kms = ctx->timer.take();  // the stream is synchronised
     if (stats) {
       stats->kernel_ms = kms;
       stats->total_ms = wall.ms();

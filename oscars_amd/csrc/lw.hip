@@ -840,7 +840,9 @@ extern "C" hipError_t om_launch_state_reset(const om::State* st, long npts, hipS
   return hipGetLastError();
 }
 
+// Active (unconverged) points into *out (cleared first by a kernel).
 extern "C" hipError_t om_launch_count_active(const om::State* st, long npts, int* out, hipStream_t s) {
+  hipLaunchKernelGGL(om::int_fill_kernel, dim3(1), dim3(64), 0, s, out, 1, 0);
   long blocks = (npts + 255) / 256;
   hipLaunchKernelGGL(om::count_active_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *st, npts, out);
   return hipGetLastError();
