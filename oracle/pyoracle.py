@@ -433,8 +433,57 @@ def ref_lib():
         L.ref_power_density_rectangle.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_double,
                                                   C.c_double, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double,
                                                   C.c_int, C.c_int, C.c_int, _dp, _dp, _ip]
+        L.ref_write_spectrum.argtypes = [C.c_char_p, C.c_char_p, C.c_long, _dp, _dp]
+        L.ref_write_points.argtypes = [C.c_char_p, C.c_char_p, C.c_long, _dp, _dp, C.c_int]
+        L.ref_average_spectra.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_char_p, C.c_char_p,
+                                          C.POINTER(C.c_long)]
+        L.ref_average_points.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_char_p, C.c_char_p,
+                                         C.POINTER(C.c_long), _dp, _dp, C.c_long]
         _ref = L
     return _ref
+
+
+def _ref_chk(L, rc):
+    if rc:
+        raise ValueError(L.ref_last_error().decode())
+
+
+def ref_write_spectrum(text_path, bin_path, energies, values):
+    """The reference's TSpectrumContainer writers on the given values."""
+    L = ref_lib()
+    e = np.ascontiguousarray(energies, dtype=np.float64)
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    _ref_chk(L, L.ref_write_spectrum((text_path or "").encode(), (bin_path or "").encode(), len(e),
+                                     e.ctypes.data_as(_dp), v.ctypes.data_as(_dp)))
+
+
+def ref_write_points(text_path, bin_path, xyz, values, dim):
+    """The reference's T3DScalarContainer writers on the given points and values."""
+    L = ref_lib()
+    p = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    _ref_chk(L, L.ref_write_points((text_path or "").encode(), (bin_path or "").encode(), len(p),
+                                   p.ctypes.data_as(_dp), v.ctypes.data_as(_dp), int(dim)))
+
+
+def ref_average_spectra(files, binary, out_text="", out_bin=""):
+    L = ref_lib()
+    arr = (C.c_char_p * len(files))(*[f.encode() for f in files])
+    n = C.c_long(0)
+    _ref_chk(L, L.ref_average_spectra(int(binary), len(files), arr, out_text.encode(), out_bin.encode(),
+                                      C.byref(n)))
+    return n.value
+
+
+def ref_average_points(files, binary, dim, out_text="", out_bin="", cap=1 << 20):
+    L = ref_lib()
+    arr = (C.c_char_p * len(files))(*[f.encode() for f in files])
+    n = C.c_long(0)
+    xyz = np.zeros((cap, 3))
+    v = np.zeros(cap)
+    _ref_chk(L, L.ref_average_points(int(binary), len(files), arr, int(dim), out_text.encode(), out_bin.encode(),
+                                     C.byref(n), xyz.ctypes.data_as(_dp), v.ctypes.data_as(_dp), cap))
+    return xyz[:n.value], v[:n.value]
 
 
 _NOANGLE = 112188979912321.0

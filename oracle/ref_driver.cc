@@ -279,4 +279,64 @@ int ref_power_density_rectangle(void* h, const char* plane, int n1, int n2, doub
   return 0;
 }
 
+// ---- the reference's own output containers: writers and file averaging
+// (src/TSpectrumContainer.cc, src/T3DScalarContainer.cc), driven from given
+// values so their files can be compared byte for byte.
+
+int ref_write_spectrum(const char* text_path, const char* bin_path, long n, double const* e, double const* v) {
+  try {
+    TSpectrumContainer S;
+    for (long i = 0; i != n; ++i) S.AddPoint(e[i], v[i]);
+    if (text_path && text_path[0]) S.WriteToFileText(text_path);
+    if (bin_path && bin_path[0]) S.WriteToFileBinary(bin_path);
+  } catch (std::exception const& ex) { return fail(ex); }
+  return 0;
+}
+
+int ref_write_points(const char* text_path, const char* bin_path, long n, double const* xyz, double const* v,
+                     int dim) {
+  try {
+    T3DScalarContainer C;
+    for (long i = 0; i != n; ++i) C.AddPoint(v3(xyz + 3 * i), v[i]);
+    if (text_path && text_path[0]) C.WriteToFileText(text_path, dim);
+    if (bin_path && bin_path[0]) C.WriteToFileBinary(bin_path, dim);
+  } catch (std::exception const& ex) { return fail(ex); }
+  return 0;
+}
+
+// binary = 0: text files, 1: binary files.  The averaged container is written to
+// out_text / out_bin (either may be empty) and its point count returned in *n.
+int ref_average_spectra(int binary, int nfiles, const char* const* files, const char* out_text, const char* out_bin,
+                        long* n) {
+  try {
+    std::vector<std::string> F(files, files + nfiles);
+    TSpectrumContainer S;
+    if (binary) S.AverageFromFilesBinary(F); else S.AverageFromFilesText(F);
+    if (out_text && out_text[0]) S.WriteToFileText(out_text);
+    if (out_bin && out_bin[0]) S.WriteToFileBinary(out_bin);
+    *n = (long) S.GetNPoints();
+  } catch (std::exception const& ex) { return fail(ex); }
+  return 0;
+}
+
+int ref_average_points(int binary, int nfiles, const char* const* files, int dim, const char* out_text,
+                       const char* out_bin, long* n, double* out_xyz, double* out_value, long cap) {
+  try {
+    std::vector<std::string> F(files, files + nfiles);
+    T3DScalarContainer C;
+    if (binary) C.AverageFromFilesBinary(F, dim); else C.AverageFromFilesText(F, dim);
+    if (out_text && out_text[0]) C.WriteToFileText(out_text, dim);
+    if (out_bin && out_bin[0]) C.WriteToFileBinary(out_bin, dim);
+    *n = (long) C.GetNPoints();
+    for (size_t i = 0; i != C.GetNPoints() && (long) i < cap; ++i) {
+      T3DScalar const& p = C.GetPoint(i);
+      out_xyz[3 * i + 0] = p.GetX().GetX();
+      out_xyz[3 * i + 1] = p.GetX().GetY();
+      out_xyz[3 * i + 2] = p.GetX().GetZ();
+      out_value[i] = p.GetV();
+    }
+  } catch (std::exception const& ex) { return fail(ex); }
+  return 0;
+}
+
 }  // extern "C"

@@ -1,0 +1,271 @@
+"""Output files and averaging of the reference's result containers.
+
+Restated from (paths relative to the reference root):
+  spectrum text / binary writers     src/TSpectrumContainer.cc:282-350
+  spectrum averaging (text, binary,  src/TSpectrumContainer.cc:372-610 and the
+  in-memory spectra)                 Python wrapper src/OSCARSSR_Python.cc:8460-8598
+  point-value (flux, power density)  src/T3DScalarContainer.cc:396-470
+  text / binary writers
+  point-value averaging              src/T3DScalarContainer.cc:104-380 and
+                                     src/OSCARSSR_Python.cc:8694-8790
+
+Formats: text is C++ std::scientific at the default precision (6 digits after
+the point), one point per line, fields separated by one space; binary is the
+values cast to float32, native byte order, with no header.  Averages keep the
+reference's order of operations, including its compensated running sum, so
+averaged values equal the reference's to the last bit.
+"""
+import math
+
+import numpy as np
+
+
+def _fmt(x):
+    # as a C++ stream prints: a NaN keeps its sign ("-nan" for the default NaN of
+    # an invalid operation on x86)
+    if math.isnan(x):
+        return "-nan" if math.copysign(1.0, x) < 0 else "nan"
+    return "%.6e" % x
+
+
+# ---------------------------------------------------------------- spectra
+def write_spectrum_text(path, energies, values, header=""):
+    """TSpectrumContainer::WriteToFileText: optional header line, then 'E V' per line."""
+    try:
+        f = open(path, "w")
+    except OSError:
+        raise ValueError("cannot open file for writing")
+    with f:
+        if header != "":
+            f.write(header + "\n")
+        for e, v in zip(energies, values):
+            f.write(_fmt(e) + " " + _fmt(v) + "\n")
+
+
+def write_spectrum_binary(path, energies, values):
+    """TSpectrumContainer::WriteToFileBinary: (float32 E, float32 V) pairs."""
+    a = np.empty((len(energies), 2), dtype=np.float32)
+    with np.errstate(over="ignore"):  # beyond the float32 range: inf, as the C++ cast gives
+        a[:, 0] = np.asarray(energies, dtype=np.float64)
+        a[:, 1] = np.asarray(values, dtype=np.float64)
+    try:
+        f = open(path, "wb")
+    except OSError:
+        raise ValueError("cannot open file for binary write")
+    with f:
+        f.write(a.tobytes())
+
+
+class _Compensated:
+    """Values with the reference containers' compensated running sum
+    (AddToFlux / AddToPoint: y = v - c; t = s + y; c = (t - s) - y; s = t)."""
+
+    def __init__(self):
+        self.x = []
+        self.v = []
+        self.c = []
+
+    def add_point(self, x, v):
+        self.x.append(x)
+        self.v.append(v)
+        self.c.append(0.0)
+
+    def add_to(self, i, v):
+        s = self.v[i]
+        y = v - self.c[i]
+        t = s + y
+        self.c[i] = (t - s) - y
+        self.v[i] = t
+
+
+def _text_records(path, width):
+    """Whitespace-separated numbers of a text file, `width` per record; reading
+    stops at the first incomplete or unparsable record (as istream >> does)."""
+    try:
+        toks = open(path).read().split()
+    except OSError:
+        return None
+    out = []
+    for i in range(0, len(toks) - width + 1, width):
+        try:
+            out.append([float(t) for t in toks[i:i + width]])
+        except ValueError:
+            break
+    return out
+
+
+def _binary_records(path, width):
+    data = open(path, "rb").read()
+    n = len(data) // (4 * width)
+    a = np.frombuffer(data[:n * 4 * width], dtype=np.float32).reshape(n, width)
+    return [[float(x) for x in row] for row in a]
+
+
+def _average(records, weights, err_len):
+    """Point-by-point weighted sum over inputs: points come from input 0 and
+    the loop ends when input 0 runs out; a later input that runs out first is
+    an error (the reference's lock-step read)."""
+    acc = _Compensated()
+    n0 = len(records[0])
+    for ip in range(n0):
+        for i, rec in enumerate(records):
+            if ip >= len(rec):
+                raise ValueError(err_len(i))
+            row = rec[ip]
+            if i == 0:
+                acc.add_point(row[:-1], row[-1] * weights[i])
+            else:
+                acc.add_to(ip, row[-1] * weights[i])
+    return acc
+
+
+def average_spectra(ifiles=None, bifiles=None, spectra=None, ofile="", bofile="", weights=None):
+    """sr.average_spectra: average spectra from text files, binary files or
+    in-memory [[E, V], ...] lists; returns [[E, V], ...] and optionally writes
+    the result (src/OSCARSSR_Python.cc:8460-8598)."""
+    ifiles = list(ifiles or [])
+    bifiles = list(bifiles or [])
+    spectra = list(spectra or [])
+    weights = list(weights or [])
+    nt, nb, ns, nw = len(ifiles), len(bifiles), len(spectra), len(weights)
+    if nt != 0 and nb != 0 and ns != 0:
+        raise ValueError("either text or binary or spectra may be added, but not a combination.")
+    if nt + nb + ns < 1:
+        raise ValueError("No files/spectra given.  You need at least one as input in a list.")
+    if nw != 0 and nt + nb + ns != nw:
+        raise ValueError("The number of weights does not match the number of inputs (nor is it zero).")
+    if nt > 0 and nb > 0:
+        raise ValueError("Currently adding mixed types of binary and text files is not supported.")
+    if nt > 0 or nb > 0:
+        files = ifiles if nt > 0 else bifiles
+        # the wrapper passes no weights to the file readers: each file has 1/N
+        w = [1.0 / float(len(files))] * len(files)
+        recs = []
+        for p in files:
+            r = _text_records(p, 2) if nt > 0 else (_binary_records(p, 2) if _exists(p) else None)
+            if r is None:
+                raise ValueError("Cannot open one or more files of input")
+            recs.append(r)
+        acc = _average(recs, w, lambda i: "files are not the same length")
+    else:
+        n0 = len(spectra[0])
+        for s in spectra:
+            if len(s) != n0:
+                raise ValueError("Incorrect size in one of the spectra")
+        w = weights if nw != 0 else [1.0 / float(ns)] * ns
+        acc = _Compensated()
+        for k, s in enumerate(spectra):
+            for ip in range(n0):
+                x = float(s[ip][0])
+                y = float(s[ip][1]) * w[k]
+                if k == 0:
+                    acc.add_point([x], y)
+                else:
+                    acc.add_to(ip, y)
+    e = [x[0] for x in acc.x]
+    if ofile != "":
+        write_spectrum_text(ofile, e, acc.v)
+    if bofile != "":
+        write_spectrum_binary(bofile, e, acc.v)
+    return [[e[i], acc.v[i]] for i in range(len(e))]
+
+
+def _exists(p):
+    try:
+        open(p, "rb").close()
+        return True
+    except OSError:
+        return False
+
+
+# ---------------------------------------------------------------- point values (flux, power density)
+def _xyz(points):
+    out = []
+    for p in points:
+        p = list(p)
+        out.append(p + [0.0] * (3 - len(p)))
+    return out
+
+
+def write_points_text(path, points, values, dim=2):
+    """T3DScalarContainer::WriteToFileText: 'X Y V' (dim 2) or 'X Y Z V' (dim 3) per line."""
+    if dim not in (2, 3):
+        raise ValueError("incorrect dimensions")
+    try:
+        f = open(path, "w")
+    except OSError:
+        raise ValueError("cannot open output file")
+    with f:
+        for p, v in zip(_xyz(points), values):
+            if dim == 2:
+                f.write(_fmt(p[0]) + " " + _fmt(p[1]) + " " + _fmt(v) + "\n")
+            else:
+                f.write(_fmt(p[0]) + " " + _fmt(p[1]) + " " + _fmt(p[2]) + " " + _fmt(v) + "\n")
+
+
+def write_points_binary(path, points, values, dim=2):
+    """T3DScalarContainer::WriteToFileBinary: float32 (X, Y, V) or (X, Y, Z, V) records."""
+    if dim not in (2, 3):
+        raise ValueError("incorrect dimensions")
+    p = np.asarray(_xyz(points), dtype=np.float64).reshape(-1, 3)
+    a = np.empty((len(p), dim + 1), dtype=np.float32)
+    with np.errstate(over="ignore"):  # beyond the float32 range: inf, as the C++ cast gives
+        a[:, 0] = p[:, 0]
+        a[:, 1] = p[:, 1]
+        if dim == 3:
+            a[:, 2] = p[:, 2]
+        a[:, dim] = np.asarray(values, dtype=np.float64)
+    try:
+        f = open(path, "wb")
+    except OSError:
+        raise ValueError("cannot open out file named: " + path)
+    with f:
+        f.write(a.tobytes())
+
+
+def average_points(ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
+    """sr.average_flux / sr.average_power_density: average point-value files
+    (same points, same order); returns [[[x, y, z], v], ...]
+    (src/OSCARSSR_Python.cc:8694-8790, src/T3DScalarContainer.cc:104-380)."""
+    ifiles = list(ifiles or [])
+    bifiles = list(bifiles or [])
+    if len(ifiles) != 0 and len(bifiles) != 0:
+        raise ValueError("either text or binary files may be added, but not both.")
+    if len(ifiles) + len(bifiles) < 1:
+        raise ValueError("No files given.  You need at least one file as input in a list.")
+    files = ifiles or bifiles
+    n = float(len(files))
+    if ifiles:
+        if dim not in (2, 3):
+            raise ValueError("'dim' must be 2 or 3")
+        # text inputs are not checked for being open: an unreadable file reads as empty
+        recs = [(_text_records(p, dim + 1) or []) for p in files]
+        err = lambda i: "Inconsistent file format found in file " + files[i]  # noqa: E731
+    else:
+        for p in files:
+            if not _exists(p):
+                raise ValueError("Cannot open input file: " + p)
+        if dim not in (2, 3):
+            raise ValueError("T3DScalarContainer::AverageFromFilesBinary I do not know how to calculate in this "
+                             "dimension")
+        recs = [_binary_records(p, dim + 1) for p in files]
+        err = lambda i: ("T3DScalarContainer::AverageFromFilesBinary index here must be 0%s  Please report this."  # noqa
+                         % ("." if dim == 2 else ".."))
+    acc = _Compensated()
+    for ip in range(len(recs[0])):
+        for i, rec in enumerate(recs):
+            if ip >= len(rec):
+                raise ValueError(err(i))
+            row = rec[ip]
+            v = row[-1] / n
+            if i == 0:
+                xyz = row[:-1] + ([0.0] if dim == 2 else [])
+                acc.add_point(xyz, v)
+            else:
+                acc.add_to(ip, v)
+    if ofile != "":
+        write_points_text(ofile, acc.x, acc.v, dim)
+    if bofile != "":
+        write_points_binary(bofile, acc.x, acc.v, dim)
+    return [[list(acc.x[i]), acc.v[i]] for i in range(len(acc.x))]
+

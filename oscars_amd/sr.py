@@ -20,6 +20,7 @@ import numpy as np
 
 from . import _lib
 from . import geometry as G
+from . import io as IO
 
 _NOANGLE = 112188979912321
 
@@ -374,6 +375,11 @@ class sr:
             v = self._run_particles(nparticles, run)
         if not self._last_conv.all():
             _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
+        # src/OSCARSSR_Python.cc:5918-5924
+        if ofile != "":
+            IO.write_spectrum_text(ofile, e, v)
+        if bofile != "":
+            IO.write_spectrum_binary(bofile, e, v)
         return [[e[i], float(v[i])] for i in range(len(e))]
 
     # ------------------------------------------------------------ flux
@@ -421,6 +427,11 @@ class sr:
         v = self._run_particles(nparticles, run)
         if not self._last_conv.all():
             _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
+        # src/OSCARSSR_Python.cc:8310-8318: the container's points (local x1, x2 for dim 2)
+        if ofile != "":
+            IO.write_points_text(ofile, cont, v, dim)
+        if bofile != "":
+            IO.write_points_binary(bofile, cont, v, dim)
         return [[list(cont[i]), float(v[i])] for i in range(len(cont))]
 
     def calculate_flux_rectangle_energies(self, energies_eV, npoints=None, plane="XY", normal=0, dim=2, width=None,
@@ -482,7 +493,29 @@ class sr:
         v = self._run_particles(nparticles, run)
         if not self._last_conv.all():
             _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
+        # src/OSCARSSR_Python.cc:6755-6763
+        if ofile != "":
+            IO.write_points_text(ofile, cont, v, dim)
+        if bofile != "":
+            IO.write_points_binary(bofile, cont, v, dim)
         return [[list(cont[i]), float(v[i])] for i in range(len(cont))]
+
+    # ------------------------------------------------------------ averaging of saved results
+    @staticmethod
+    def average_spectra(ifiles=None, bifiles=None, spectra=None, ofile="", bofile="", weights=None):
+        """Average spectra from text files, binary files or [[E, V], ...] lists
+        (src/OSCARSSR_Python.cc:8460-8598)."""
+        return IO.average_spectra(ifiles, bifiles, spectra, ofile, bofile, weights)
+
+    @staticmethod
+    def average_flux(ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
+        """Average flux files of the same points (src/OSCARSSR_Python.cc:8694-8790)."""
+        return IO.average_points(ifiles, bifiles, ofile, bofile, dim)
+
+    @staticmethod
+    def average_power_density(ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
+        """Average power-density files of the same points (src/OSCARSSR_Python.cc:8694-8790)."""
+        return IO.average_points(ifiles, bifiles, ofile, bofile, dim)
 
 
 def read_oscars_field_file(path):

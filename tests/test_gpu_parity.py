@@ -231,3 +231,22 @@ def test_device_resident_entry_points_match_host(gpu_planar):
     dev.power_density_dev(0, torch.tensor(obs, dtype=torch.float64, device="cuda"), [0, 0, 1], 0, conv, out_t, lev_t)
     assert np.array_equal(out_t.cpu().numpy(), v)
     assert np.array_equal(lev_t.cpu().numpy(), lev)
+
+
+@gpu
+def test_output_files(gpu_planar, tmp_path):
+    """ofile / bofile of the drop-in methods write the returned values in the
+    reference's formats (the writers themselves are pinned in tests/test_io.py)."""
+    from oscars_amd import io as IO
+    res = gpu_planar.calculate_spectrum(obs=[0, 0, 30], npoints=20, energy_range_eV=[100, 500],
+                                        ofile=str(tmp_path / "s.txt"), bofile=str(tmp_path / "s.dat"))
+    IO.write_spectrum_text(str(tmp_path / "r.txt"), [r[0] for r in res], [r[1] for r in res])
+    assert (tmp_path / "s.txt").read_bytes() == (tmp_path / "r.txt").read_bytes()
+    assert len((tmp_path / "s.dat").read_bytes()) == 20 * 8
+    res = gpu_planar.calculate_flux_rectangle(energy_eV=152.11, npoints=[5, 4], width=[0.01, 0.01],
+                                              translation=[0, 0, 30], ofile=str(tmp_path / "f.txt"),
+                                              bofile=str(tmp_path / "f.dat"))
+    lines = (tmp_path / "f.txt").read_text().splitlines()
+    assert len(lines) == 20 and len(lines[0].split()) == 3
+    avg = gpu_planar.average_flux(ifiles=[str(tmp_path / "f.txt")] * 2)
+    assert np.allclose([a[1] for a in avg], [float(l.split()[2]) for l in lines], rtol=0, atol=0)
