@@ -433,6 +433,15 @@ def ref_lib():
         L.ref_power_density_rectangle.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_double,
                                                   C.c_double, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double,
                                                   C.c_int, C.c_int, C.c_int, _dp, _dp, _ip]
+        L.ref_set_beam_full.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_double, C.c_double, _dp, _dp,
+                                        C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_char_p]
+        L.ref_set_seed.argtypes = [C.c_void_p, C.c_int]
+        L.ref_set_nparticles.argtypes = [C.c_int]
+        L.ref_random_normal.argtypes = [C.c_void_p]
+        L.ref_random_normal.restype = C.c_double
+        L.ref_random_uniform.argtypes = [C.c_void_p]
+        L.ref_random_uniform.restype = C.c_double
+        L.ref_new_particle.argtypes = [C.c_void_p]
         L.ref_write_spectrum.argtypes = [C.c_char_p, C.c_char_p, C.c_long, _dp, _dp]
         L.ref_write_points.argtypes = [C.c_char_p, C.c_char_p, C.c_long, _dp, _dp, C.c_int]
         L.ref_average_spectra.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_char_p, C.c_char_p,
@@ -526,6 +535,35 @@ class RefSim:
     def set_particle_beam(self, energy_GeV, x0=(0, 0, 0), d0=(0, 0, 1), current=0.0, type="electron"):
         self._chk(self.L.ref_set_beam(self.h, float(energy_GeV), _v3(x0), _v3(d0), float(current)))
 
+    def set_particle_beam_full(self, energy_GeV, x0=(0, 0, 0), d0=(0, 0, 1), t0=0.0, current=0.0,
+                               horizontal_direction=None, emittance=None, sigma_energy_GeV=0.0, beta=None,
+                               alpha=None, gamma=None, lattice_reference=None, eta=None, distribution=None):
+        """set_particle_beam with the beam-spread arguments; None = not given."""
+        def opt(v, n):
+            if v is None:
+                return None
+            a = np.ascontiguousarray(v, dtype=np.float64).reshape(n)
+            self._keep.append(a)
+            return a.ctypes.data_as(_dp)
+        self._keep = []
+        self._chk(self.L.ref_set_beam_full(self.h, float(energy_GeV), _v3(x0), _v3(d0), float(t0), float(current),
+                                           opt(horizontal_direction, 3), opt(emittance, 2), float(sigma_energy_GeV),
+                                           opt(beta, 2), opt(alpha, 2), opt(gamma, 2), opt(lattice_reference, 3),
+                                           opt(eta, 2), None if distribution is None else distribution.encode()))
+
+    def set_seed(self, seed):
+        self.L.ref_set_seed(self.h, int(seed))
+
+    def random_normal(self):
+        return self.L.ref_random_normal(self.h)
+
+    def random_uniform(self):
+        return self.L.ref_random_uniform(self.h)
+
+    def new_particle(self):
+        """Draw the next random particle of the beam and make it current."""
+        self._chk(self.L.ref_new_particle(self.h))
+
     def set_particle(self, x0, beta0, t0=0.0):
         self._chk(self.L.ref_set_particle(self.h, _v3(x0), _v3(beta0), float(t0)))
 
@@ -557,8 +595,10 @@ class RefSim:
 
     def calculate_spectrum(self, obs, npoints=0, energy_range_eV=None, energy_points_eV=None, polarization="",
                            angle=None, horizontal_direction=(1, 0, 0), propogation_direction=(0, 0, 1),
-                           precision=0.01, max_level=-2, max_level_extended=0, quantity="flux", nthreads=0):
+                           precision=0.01, max_level=-2, max_level_extended=0, quantity="flux", nthreads=0,
+                           nparticles=0):
         q = {"FLUX": 0, "PRECISION": 1, "LEVEL": 2}[quantity.upper()]
+        self.L.ref_set_nparticles(int(nparticles))
         if energy_points_eV is not None:
             e_in = np.ascontiguousarray(energy_points_eV, dtype=np.float64)
             n = len(e_in)
@@ -581,8 +621,9 @@ class RefSim:
     def calculate_flux_rectangle(self, energy_eV, npoints, width, plane="XY", translation=(0, 0, 0),
                                  rotations=(0, 0, 0), normal=0, dim=2, polarization="", angle=None,
                                  horizontal_direction=(1, 0, 0), propogation_direction=(0, 0, 1), precision=0.01,
-                                 max_level=-2, max_level_extended=0, quantity="flux", nthreads=0):
+                                 max_level=-2, max_level_extended=0, quantity="flux", nthreads=0, nparticles=0):
         q = {"FLUX": 0, "PRECISION": 1, "LEVEL": 2}[quantity.upper()]
+        self.L.ref_set_nparticles(int(nparticles))
         n = npoints[0] * npoints[1]
         xyz = np.zeros((n, 3))
         v = np.zeros(n)
@@ -599,8 +640,10 @@ class RefSim:
 
     def calculate_power_density_rectangle(self, npoints, width, plane="XY", translation=(0, 0, 0),
                                           rotations=(0, 0, 0), normal=0, dim=2, precision=0.01, max_level=-2,
-                                          max_level_extended=0, quantity="power density", nthreads=0):
+                                          max_level_extended=0, quantity="power density", nthreads=0,
+                                          nparticles=0):
         q = {"POWER DENSITY": 0, "PRECISION": 1, "LEVEL": 2}[quantity.upper()]
+        self.L.ref_set_nparticles(int(nparticles))
         n = npoints[0] * npoints[1]
         xyz = np.zeros((n, 3))
         v = np.zeros(n)

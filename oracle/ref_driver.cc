@@ -30,6 +30,7 @@
 
 namespace {
 thread_local std::string g_err;
+int g_nparticles = 0;  // NParticles of the next calculate_* calls (0: the current particle)
 
 int fail(std::exception const& e) {
   g_err = e.what();
@@ -102,6 +103,68 @@ int ref_set_particle(void* h, double const* x0, double const* beta0, double t0) 
     TParticleA& P = current_particle(s);
     P.SetInitialParticleConditions(v3(x0), v3(beta0), t0);
     P.ResetTrajectoryData();
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+// sr.set_particle_beam with the beam-spread arguments, in the binding's order
+// (src/OSCARSSR_Python.cc:3873-4040).  A null pointer is an argument not given.
+int ref_set_beam_full(void* h, double energy_GeV, double const* x0, double const* d0, double t0, double current,
+                      double const* horizontal, double const* emittance, double sigma_energy_GeV,
+                      double const* beta, double const* alpha, double const* gamma,
+                      double const* lattice_reference, double const* eta, char const* distribution) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    s->ClearParticleBeams();
+    TVector3D Direction = v3(d0);
+    TVector3D Horizontal = horizontal ? v3(horizontal) : -Direction.Orthogonal();
+    Horizontal = Horizontal.UnitVector();
+    TParticleBeam* b = &(s->AddParticleBeam("electron", "", v3(x0), Direction, energy_GeV, t0, current, 1));
+    b->SetHorizontalDirection(Horizontal);
+    if (emittance) {
+      b->SetEmittance(TVector2D(emittance[0], emittance[1]));
+      b->SetBeamDistribution(TParticleBeam::kBeamDistribution_Gaussian);
+    } else {
+      b->SetBeamDistribution(TParticleBeam::kBeamDistribution_Filament);
+    }
+    if (distribution) b->SetBeamDistribution(std::string(distribution));
+    if (sigma_energy_GeV < 0) throw std::invalid_argument("'sigma_energy_GeV' cannot be less than zero");
+    if (sigma_energy_GeV > 0) b->SetSigmaEnergyGeV(sigma_energy_GeV);
+    TVector2D Beta = beta ? TVector2D(beta[0], beta[1]) : TVector2D(0, 0);
+    TVector2D Alpha = alpha ? TVector2D(alpha[0], alpha[1]) : TVector2D(0, 0);
+    TVector2D Gamma = gamma ? TVector2D(gamma[0], gamma[1]) : TVector2D(0, 0);
+    int const has = (beta ? 4 : 0) | (alpha ? 2 : 0) | (gamma ? 1 : 0);
+    bool const has_ref = lattice_reference != 0;
+    TVector3D L = has_ref ? v3(lattice_reference) : TVector3D(0, 0, 0);
+    b->SetTwissLatticeReference(L);
+    switch (has) {
+      case 7: b->SetTwissParameters(Beta, Alpha, Gamma, L, has_ref); break;
+      case 6: b->SetTwissBetaAlpha(Beta, Alpha, L, has_ref); break;
+      case 5: b->SetTwissBetaGamma(Beta, Gamma, L, has_ref); break;
+      case 3: b->SetTwissAlphaGamma(Alpha, Gamma, L, has_ref); break;
+      case 4: b->SetTwissBetaAlpha(Beta, TVector2D(0, 0), L, has_ref); break;
+      default: break;
+    }
+    b->SetEta(eta ? TVector2D(eta[0], eta[1]) : TVector2D(0, 0));
+    s->SetCTStartStop(-3, 3);
+    s->SetNewParticle("", "ideal");
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+// nparticles= of the calculate_* calls that follow.
+void ref_set_nparticles(int n) { g_nparticles = n; }
+
+// sr.set_seed (include/OSCARSPY.h:85-87) and the draws of the global stream.
+void ref_set_seed(void* h, int seed) { static_cast<OSCARSSR*>(h)->SetSeed(seed); }
+double ref_random_normal(void* h) { return static_cast<OSCARSSR*>(h)->GetRandomNormal(); }
+double ref_random_uniform(void* h) { return static_cast<OSCARSSR*>(h)->GetRandomUniform(); }
+
+// One random particle of the beam as the multi-particle loops take it
+// (src/OSCARSSR.cc:497-505, called per particle at src/OSCARSSR.cc:1786).
+int ref_new_particle(void* h) {
+  try {
+    static_cast<OSCARSSR*>(h)->SetNewParticle();
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -213,7 +276,7 @@ int ref_spectrum(void* h, double const* obs, int npoints, double e0, double e1, 
     }
     std::string P;
     make_pol(pol, angle, P);
-    s->CalculateSpectrum(v3(obs), C, P, angle, v3(hdir), v3(pdir), 0, nthreads, 0, -1, std::vector<int>(),
+    s->CalculateSpectrum(v3(obs), C, P, angle, v3(hdir), v3(pdir), g_nparticles, nthreads, 0, -1, std::vector<int>(),
                          precision, max_level, max_level_extended, quantity);
     if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
     for (size_t i = 0; i != C.GetNPoints(); ++i) {
@@ -241,7 +304,7 @@ int ref_flux_rectangle(void* h, double energy_eV, const char* plane, int n1, int
     T3DScalarContainer C;
     std::string P;
     make_pol(pol, angle, P);
-    s->CalculateFlux(R, energy_eV, C, P, angle, v3(hdir), v3(pdir), 0, nthreads, 0, -1, std::vector<int>(),
+    s->CalculateFlux(R, energy_eV, C, P, angle, v3(hdir), v3(pdir), g_nparticles, nthreads, 0, -1, std::vector<int>(),
                      precision, max_level, max_level_extended, dim, quantity);
     if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
     for (size_t i = 0; i != C.GetNPoints(); ++i) {
@@ -265,7 +328,7 @@ int ref_power_density_rectangle(void* h, const char* plane, int n1, int n2, doub
     make_rect(R, plane, n1, n2, w1, w2, rotations, translation, normal);
     T3DScalarContainer C;
     bool const Directional = normal == 0 ? false : true;
-    s->CalculatePowerDensity(R, C, dim, Directional, precision, max_level, max_level_extended, 0, nthreads, 0,
+    s->CalculatePowerDensity(R, C, dim, Directional, precision, max_level, max_level_extended, g_nparticles, nthreads, 0,
                              -1, std::vector<int>(), quantity);
     if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
     for (size_t i = 0; i != C.GetNPoints(); ++i) {

@@ -14,6 +14,8 @@ its integrand.  Reference files restated (paths relative to the reference root):
   src/OSCARSSR.cc:1887-1919                  polarization vectors
   src/TVector3D.cc:208-295                   rotations
 """
+import ctypes
+import ctypes.util
 import math
 
 PI = 3.14159265358979323846
@@ -61,14 +63,38 @@ def orthogonal(v):
     return [-v[2], 0.0, v[0]] if yy < zz else [v[1], -v[0], 0.0]
 
 
+def _libm_sincos():
+    try:
+        f = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6").sincos
+    except (OSError, AttributeError):
+        return None
+    f.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    f.restype = None
+    return f
+
+
+_SINCOS = _libm_sincos()
+
+
+def sincos(x):
+    """(sin x, cos x) as compiled C++ gets them where one function takes both
+    of one argument: the compiler merges the two calls into the C library's
+    sincos, whose last bit can differ from separate sin and cos calls."""
+    if _SINCOS is None:
+        return math.sin(x), math.cos(x)
+    s, c = ctypes.c_double(), ctypes.c_double()
+    _SINCOS(float(x), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
 def rot_xyz(v, r):
     """TVector3D::RotateSelfXYZ: about X, then Y, then Z."""
     x, y, z = float(v[0]), float(v[1]), float(v[2])
-    s, c = math.sin(r[0]), math.cos(r[0])
+    s, c = sincos(r[0])
     y, z = c * y - s * z, s * y + c * z
-    s, c = math.sin(r[1]), math.cos(r[1])
+    s, c = sincos(r[1])
     z, x = c * z - s * x, s * z + c * x
-    s, c = math.sin(r[2]), math.cos(r[2])
+    s, c = sincos(r[2])
     x, y = c * x - s * y, s * x + c * y
     return [x, y, z]
 
@@ -76,7 +102,7 @@ def rot_xyz(v, r):
 def rot_axis(o, A, vin):
     """TVector3D::RotateSelf(angle, axis)."""
     V = unit(vin)
-    ca, sa = math.cos(A), math.sin(A)
+    sa, ca = sincos(A)
     M = [[ca + V[0] * V[0] * (1 - ca), V[0] * V[1] * (1 - ca) - V[2] * sa, V[0] * V[2] * (1 - ca) + V[1] * sa],
          [V[1] * V[0] * (1 - ca) + V[2] * sa, ca + V[1] * V[1] * (1 - ca), V[1] * V[2] * (1 - ca) - V[0] * sa],
          [V[2] * V[0] * (1 - ca) - V[1] * sa, V[2] * V[1] * (1 - ca) + V[0] * sa, ca + V[2] * V[2] * (1 - ca)]]

@@ -19,6 +19,7 @@ import sys
 import numpy as np
 
 from . import _lib
+from . import beam as B
 from . import geometry as G
 from . import io as IO
 
@@ -31,6 +32,16 @@ def _v3(x, what):
     except TypeError:
         raise ValueError("Incorrect format in '%s'" % what)
     if len(v) != 3:
+        raise ValueError("Incorrect format in '%s'" % what)
+    return v
+
+
+def _v2(x, what):
+    try:
+        v = [float(a) for a in x]
+    except TypeError:
+        raise ValueError("Incorrect format in '%s'" % what)
+    if len(v) != 2:
         raise ValueError("Incorrect format in '%s'" % what)
     return v
 
@@ -150,13 +161,27 @@ class sr:
         direction = _v3(d0, "d0") if d0 is not None else [0.0, 0.0, 1.0]
         rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
         tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        # src/OSCARSSR_Python.cc:3874-3885: the horizontal direction comes
+        # from the direction as given, before the rotations
+        hdir = _v3(horizontal_direction, "horizontal_direction") if horizontal_direction is not None else \
+            [-v for v in G.orthogonal(direction)]
+        hdir = G.unit(hdir)
         pos = G.rot_xyz(pos, rot)
         direction = G.rot_xyz(direction, rot)
         pos = [pos[k] + tr[k] for k in range(3)]
         p = G.ideal_particle(type, energy_GeV, pos, direction, t0, current, charge, mass)
+        if sigma_energy_GeV < 0:
+            raise ValueError("'sigma_energy_GeV' cannot be less than zero")
+        two = lambda v, name: None if v is None else _v2(v, name)  # noqa: E731
+        beam_p = dict(emittance=two(emittance, "emittance"), beta=two(beta, "beta"), alpha=two(alpha, "alpha"),
+                      gamma=two(gamma, "gamma"), eta=two(eta, "gamma"),
+                      lattice_reference=None if lattice_reference is None else _v3(lattice_reference,
+                                                                                   "lattice_reference"))
+        B.twiss_from(beam_p["beta"], beam_p["alpha"], beam_p["gamma"])  # the reference's range checks
         self._beam = dict(type=type, energy_GeV=energy_GeV, x0=pos, d0=direction, t0=float(t0),
-                          current=float(current), sigma_energy_GeV=float(sigma_energy_GeV), beta=beta, alpha=alpha,
-                          emittance=emittance, horizontal_direction=horizontal_direction, q=p.q, m=p.m)
+                          current=float(current), sigma_energy_GeV=float(sigma_energy_GeV),
+                          horizontal_unit=hdir, distribution=B.distribution_of(emittance, distribution),
+                          q=p.q, m=p.m, **beam_p)
         self._particle = p
         if ctstartstop is not None:
             if len(ctstartstop) != 2:
@@ -169,12 +194,13 @@ class sr:
     def set_new_particle(self, beam="", particle=""):
         if self._beam is None:
             raise ValueError("No particle beam defined")
-        if particle.lower() in ("", "ideal"):
+        if particle.lower() == "ideal":
             b = self._beam
             self._particle = G.ideal_particle(b["type"], b["energy_GeV"], b["x0"], b["d0"], b["t0"], b["current"],
-                                              0, b["m"])
+                                              b["q"], b["m"])
         else:
-            raise ValueError("random particles: use nparticles= or set_particle()")
+            # src/TParticleBeam.cc:667-689: anything but "ideal" draws a random particle
+            self._particle = B.sample_particles(self._beam, 1)[0]
         self._traj_valid = False
 
     def set_particle(self, x0, beta0, t0=0.0):
@@ -287,18 +313,17 @@ class sr:
         return G.polarization_vector(pol, angle if angle != _NOANGLE else 0.0, hdir, pdir)
 
     def set_seed(self, seed):
-        """Seed of the beam sampler used by nparticles= (numpy PCG64)."""
-        self._seed = int(seed)
-        self._draws = 0
+        """Restart the global random stream (include/OSCARSPY.h:85-87): the
+        same seed gives the reference's particles, draw for draw."""
+        B.set_seed(seed)
 
     def _particles_for(self, nparticles):
-        """Particles to sum: the current one (nparticles=0) or nparticles samples."""
+        """Particles to sum: the current one (nparticles=0) or nparticles drawn
+        from the global stream in the reference's order (one new particle per
+        iteration of src/OSCARSSR.cc:1782-1790)."""
         if nparticles == 0:
             return None
-        from .beam import sample_particles
-        self._draws = getattr(self, "_draws", 0) + 1
-        rng = np.random.Generator(np.random.PCG64([getattr(self, "_seed", 0), self._draws]))
-        return sample_particles(self._beam, nparticles, rng=rng)
+        return B.sample_particles(self._beam, nparticles)
 
     def _batched(self, nparticles, obs, energies, pol, precision, max_level, max_level_extended, mode):
         """nparticles particles in one persistent launch (tiled / mixed kernels)."""

@@ -30,3 +30,22 @@ def rel_err(a, b, floor_frac=0.0):
     den = np.maximum(np.abs(b), floor_frac * np.max(np.abs(b)))
     den = np.where(den == 0, 1.0, den)
     return float(np.max(np.abs(a - b) / den))
+
+
+# Beams with spread for the sampling tests: set_particle_beam keyword sets
+# (x0, d0 and the beam-spread arguments; unset arguments are left out).
+BEAM_CASES = [
+    dict(energy_GeV=3.0, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5, sigma_energy_GeV=0.0027,
+         emittance=[0.9e-9, 8e-12], beta=[1.5, 0.8]),
+    dict(energy_GeV=3.0, x0=[0.001, -0.002, -1.5], d0=[0.01, 0.002, 1], current=0.5, sigma_energy_GeV=0.0027,
+         emittance=[0.9e-9, 8e-12], beta=[1.5, 0.8], alpha=[0.3, -0.2], lattice_reference=[0, 0, 0.7],
+         eta=[0.01, 0.002], horizontal_direction=[1, 0.1, 0]),
+    dict(energy_GeV=3.0, x0=[0, 0, -1.5], d0=[0, 0, 1], sigma_energy_GeV=0.001, emittance=[0.5e-9, 8e-12],
+         beta=[2.0, 1.0], gamma=[0.7, 1.2]),
+    dict(energy_GeV=3.0, x0=[0, 0, -1.5], d0=[0, 0, 1], sigma_energy_GeV=0.001, emittance=[0.5e-9, 8e-12],
+         alpha=[0.1, 0.2], gamma=[0.7, 1.2]),
+    dict(energy_GeV=3.0, x0=[0, 0, -1.5], d0=[0, 0, 1], sigma_energy_GeV=0.001, emittance=[0.5e-9, 8e-12],
+         beta=[2.0, 1.0], alpha=[0.5, -0.5], gamma=[0.625, 1.25], distribution="KV"),
+    dict(energy_GeV=3.0, x0=[0, 0, -1.5], d0=[0, 0, 1], sigma_energy_GeV=0.001),
+    dict(energy_GeV=3.0, x0=[0, 0, -1.5], d0=[0, 0, 1], sigma_energy_GeV=0.001, distribution="gaussian"),
+]
