@@ -434,7 +434,7 @@ def ref_lib():
                                                   C.c_double, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double,
                                                   C.c_int, C.c_int, C.c_int, _dp, _dp, _ip]
         L.ref_set_beam_full.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_double, C.c_double, _dp, _dp,
-                                        C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_char_p]
+                                        C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_char_p, C.c_char_p]
         L.ref_set_seed.argtypes = [C.c_void_p, C.c_int]
         L.ref_set_nparticles.argtypes = [C.c_int]
         L.ref_random_normal.argtypes = [C.c_void_p]
@@ -537,8 +537,10 @@ class RefSim:
 
     def set_particle_beam_full(self, energy_GeV, x0=(0, 0, 0), d0=(0, 0, 1), t0=0.0, current=0.0,
                                horizontal_direction=None, emittance=None, sigma_energy_GeV=0.0, beta=None,
-                               alpha=None, gamma=None, lattice_reference=None, eta=None, distribution=None):
-        """set_particle_beam with the beam-spread arguments; None = not given."""
+                               alpha=None, gamma=None, lattice_reference=None, eta=None, distribution=None,
+                               beam=None):
+        """set_particle_beam with the beam-spread arguments; None = not given
+        (with a predefined ``beam``, d0=None and energy_GeV < 0 too)."""
         def opt(v, n):
             if v is None:
                 return None
@@ -546,10 +548,11 @@ class RefSim:
             self._keep.append(a)
             return a.ctypes.data_as(_dp)
         self._keep = []
-        self._chk(self.L.ref_set_beam_full(self.h, float(energy_GeV), _v3(x0), _v3(d0), float(t0), float(current),
+        self._chk(self.L.ref_set_beam_full(self.h, float(energy_GeV), _v3(x0), opt(d0, 3), float(t0), float(current),
                                            opt(horizontal_direction, 3), opt(emittance, 2), float(sigma_energy_GeV),
                                            opt(beta, 2), opt(alpha, 2), opt(gamma, 2), opt(lattice_reference, 3),
-                                           opt(eta, 2), None if distribution is None else distribution.encode()))
+                                           opt(eta, 2), None if distribution is None else distribution.encode(),
+                                           None if beam is None else beam.encode()))
 
     def set_seed(self, seed):
         self.L.ref_set_seed(self.h, int(seed))

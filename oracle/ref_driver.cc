@@ -112,19 +112,29 @@ int ref_set_particle(void* h, double const* x0, double const* beta0, double t0) 
 int ref_set_beam_full(void* h, double energy_GeV, double const* x0, double const* d0, double t0, double current,
                       double const* horizontal, double const* emittance, double sigma_energy_GeV,
                       double const* beta, double const* alpha, double const* gamma,
-                      double const* lattice_reference, double const* eta, char const* distribution) {
+                      double const* lattice_reference, double const* eta, char const* distribution,
+                      char const* predefined) {
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);
     s->ClearParticleBeams();
-    TVector3D Direction = v3(d0);
+    TVector3D Direction = d0 ? v3(d0) : TVector3D(0, 0, 1);
     TVector3D Horizontal = horizontal ? v3(horizontal) : -Direction.Orthogonal();
     Horizontal = Horizontal.UnitVector();
-    TParticleBeam* b = &(s->AddParticleBeam("electron", "", v3(x0), Direction, energy_GeV, t0, current, 1));
+    TParticleBeam* b = 0x0;
+    if (predefined) {
+      // a predefined beam (null d0: direction not given; energy < 0: not given)
+      b = &(s->AddParticleBeam(std::string(predefined), "", 1));
+      if (energy_GeV >= 0) b->SetE0(energy_GeV);
+      if (d0) b->SetU0(Direction);
+      b->SetX0(v3(x0));
+    } else {
+      b = &(s->AddParticleBeam("electron", "", v3(x0), Direction, energy_GeV, t0, current, 1));
+    }
     b->SetHorizontalDirection(Horizontal);
     if (emittance) {
       b->SetEmittance(TVector2D(emittance[0], emittance[1]));
       b->SetBeamDistribution(TParticleBeam::kBeamDistribution_Gaussian);
-    } else {
+    } else if (!predefined) {
       b->SetBeamDistribution(TParticleBeam::kBeamDistribution_Filament);
     }
     if (distribution) b->SetBeamDistribution(std::string(distribution));

@@ -170,6 +170,29 @@ def beam_directions(u0, horizontal):
     return h, [-vd[0], -vd[1], -vd[2]]
 
 
+# src/TParticleBeam.cc:135-200: name aliases -> electron beam at 3 GeV, 0.5 A
+_PREDEFINED = {
+    "NSLSII": dict(beta=[1.5, 0.8], sigma_energy_GeV=0.0, distribution="filament"),
+    "NSLSII-LONGSTRAIGHT": dict(beta=[18.0, 3.1], sigma_energy_GeV=3. * 0.00089, distribution="gaussian"),
+    "NSLSII-SHORTSTRAIGHT": dict(beta=[1.5, 0.8], sigma_energy_GeV=3. * 0.00089, distribution="gaussian"),
+}
+
+
+def predefined_beam(name):
+    """Parameters of a predefined beam (NSLSII, NSLSII-LONGSTRAIGHT,
+    NSLSII-SHORTSTRAIGHT; "NSLS2" and "NSLS-II" are aliases of "NSLSII")."""
+    u = name.upper()
+    for alias in ("NSLS-II", "NSLS2"):
+        if u == alias or u.startswith(alias + "-"):
+            u = "NSLSII" + u[len(alias):]
+            break
+    if u not in _PREDEFINED:
+        raise ValueError("Error in predefined beam name / definition")
+    d = dict(_PREDEFINED[u])
+    d.update(energy_GeV=3.0, current=0.5, emittance=[0.55e-9, 0.008e-9])
+    return d
+
+
 def distribution_of(emittance, distribution=""):
     d = (distribution or "").lower()
     if d == "":

@@ -151,24 +151,35 @@ class sr:
                           sigma_energy_GeV=0, t0=0, current=0, weight=1, rotations=None, translation=None,
                           horizontal_direction=None, beta=None, alpha=None, gamma=None, emittance=None, eta=None,
                           lattice_reference=None, mass=0, charge=0, ctstartstop=None, distribution=""):
+        # src/OSCARSSR_Python.cc:3795-4040
+        pre = None
         if beam:
-            raise ValueError("predefined beams are not supported by this implementation")
+            pre = B.predefined_beam(beam)
+            type, current, t0 = "electron", pre["current"], 0.0
+            if energy_GeV >= 0:
+                pre["energy_GeV"] = energy_GeV
         if energy_GeV == -1:
             energy_GeV = 0
-        if energy_GeV < 0:
-            raise ValueError("energy must be > 0")
         pos = _v3(x0, "x0") if x0 is not None else [0.0, 0.0, 0.0]
+        if energy_GeV < 0:
+            raise ValueError("We do not currently support negative energy beams.  Please change 'energy_GeV' to >= 0")
         direction = _v3(d0, "d0") if d0 is not None else [0.0, 0.0, 1.0]
         rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
         tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
-        # src/OSCARSSR_Python.cc:3874-3885: the horizontal direction comes
-        # from the direction as given, before the rotations
+        # the horizontal direction comes from the direction as given, before
+        # the rotations
         hdir = _v3(horizontal_direction, "horizontal_direction") if horizontal_direction is not None else \
             [-v for v in G.orthogonal(direction)]
         hdir = G.unit(hdir)
         pos = G.rot_xyz(pos, rot)
         direction = G.rot_xyz(direction, rot)
         pos = [pos[k] + tr[k] for k in range(3)]
+        if pre is not None:
+            # a predefined beam keeps its own direction unless d0 is given
+            direction = direction if d0 is not None else [0.0, 0.0, 1.0]
+            energy_GeV = pre["energy_GeV"]
+        elif type == "custom" and (mass == 0 or charge == 0):
+            raise ValueError("'mass' or 'charge' is zero")
         p = G.ideal_particle(type, energy_GeV, pos, direction, t0, current, charge, mass)
         if sigma_energy_GeV < 0:
             raise ValueError("'sigma_energy_GeV' cannot be less than zero")
@@ -177,11 +188,21 @@ class sr:
                       gamma=two(gamma, "gamma"), eta=two(eta, "gamma"),
                       lattice_reference=None if lattice_reference is None else _v3(lattice_reference,
                                                                                    "lattice_reference"))
-        B.twiss_from(beam_p["beta"], beam_p["alpha"], beam_p["gamma"])  # the reference's range checks
+        user_twiss = B.twiss_from(beam_p["beta"], beam_p["alpha"], beam_p["gamma"])  # the reference's range checks
+        dist = B.distribution_of(emittance, "")
+        if pre is not None:
+            if beam_p["emittance"] is None:
+                beam_p["emittance"] = pre["emittance"]
+                dist = pre["distribution"]
+            if not sigma_energy_GeV > 0:
+                sigma_energy_GeV = pre["sigma_energy_GeV"]
+            if user_twiss is None:
+                beam_p.update(beta=pre["beta"], alpha=[0.0, 0.0], gamma=None)
+        if distribution:
+            dist = B.distribution_of(emittance, distribution)
         self._beam = dict(type=type, energy_GeV=energy_GeV, x0=pos, d0=direction, t0=float(t0),
                           current=float(current), sigma_energy_GeV=float(sigma_energy_GeV),
-                          horizontal_unit=hdir, distribution=B.distribution_of(emittance, distribution),
-                          q=p.q, m=p.m, **beam_p)
+                          horizontal_unit=hdir, distribution=dist, q=p.q, m=p.m, **beam_p)
         self._particle = p
         if ctstartstop is not None:
             if len(ctstartstop) != 2:

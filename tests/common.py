@@ -49,3 +49,14 @@ BEAM_CASES = [
     dict(energy_GeV=3.0, x0=[0, 0, -1.5], d0=[0, 0, 1], sigma_energy_GeV=0.001),
     dict(energy_GeV=3.0, x0=[0, 0, -1.5], d0=[0, 0, 1], sigma_energy_GeV=0.001, distribution="gaussian"),
 ]
+
+# Predefined beams with and without overrides (set_particle_beam keywords).
+PREDEFINED_CASES = [
+    dict(beam="NSLSII-LONGSTRAIGHT", x0=[0, 0, -1]),
+    dict(beam="NSLS2-SHORTSTRAIGHT", x0=[0, 0, -1], energy_GeV=2.5, sigma_energy_GeV=0.004,
+         lattice_reference=[0, 0, 0.5]),
+    dict(beam="nsls-ii", x0=[0, 0, -1]),
+    dict(beam="NSLSII", x0=[0, 0, -1], distribution="gaussian"),
+    dict(beam="NSLSII-LONGSTRAIGHT", x0=[0.0001, 0, -1], d0=[0.001, 0, 1], emittance=[1e-9, 1e-11], beta=[2, 1],
+         alpha=[0.2, 0.1], eta=[0.02, 0]),
+]

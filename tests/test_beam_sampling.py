@@ -20,7 +20,7 @@ import pytest
 from oscars_amd import beam as B
 from oscars_amd import geometry as G
 from oscars_amd.sr import sr
-from tests.common import BEAM_CASES
+from tests.common import BEAM_CASES, PREDEFINED_CASES
 
 GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "beam_sampling.npz"))
 
@@ -81,6 +81,28 @@ def test_uniform_and_normal_forms():
 def test_particles_match_reference_fixture(case):
     got = _sr_particles(BEAM_CASES[case], 42, GOLD["particles"].shape[1])
     assert np.array_equal(got, GOLD["particles"][case])
+
+
+def _sr_predefined(kw, n):
+    s = sr()
+    s.set_particle_beam(**kw)
+    rows = [s._particle.x0 + s._particle.b0]
+    s.set_seed(42)
+    rows += [p.x0 + p.b0 for p in s._particles_for(n)]
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("case", range(len(PREDEFINED_CASES)))
+def test_predefined_beams_match_reference_fixture(case):
+    """Predefined beams (src/TParticleBeam.cc:135-200) with the binding's
+    overrides: the ideal particle they start with, then 100 draws."""
+    g = GOLD["predefined"][case]
+    assert np.array_equal(_sr_predefined(PREDEFINED_CASES[case], g.shape[0] - 1), g)
+
+
+def test_predefined_beam_errors():
+    with pytest.raises(ValueError, match="Error in predefined beam name"):
+        sr().set_particle_beam(beam="NSLS-III")
 
 
 def test_sincos_is_the_library_pair():
@@ -167,6 +189,22 @@ def test_particles_live_against_reference_build(case):
             p = s.particle()
             ref.append(p["x0"] + p["b0"])
         assert np.array_equal(_sr_particles(kw, seed, 1000), np.array(ref)), (case, seed)
+
+
+@pytest.mark.parametrize("case", range(len(PREDEFINED_CASES)))
+def test_predefined_live_against_reference_build(case):
+    P = _ref()
+    from tests.golden.make_beam_golden import ref_args
+    s = P.RefSim()
+    kw = PREDEFINED_CASES[case]
+    s.set_particle_beam_full(**ref_args(kw))
+    rows = [s.particle()["x0"] + s.particle()["b0"]]
+    s.set_seed(42)
+    for _ in range(1000):
+        s.new_particle()
+        p = s.particle()
+        rows.append(p["x0"] + p["b0"])
+    assert np.array_equal(_sr_predefined(kw, 1000), np.array(rows))
 
 
 def test_nparticles_spectrum_matches_reference_build():
