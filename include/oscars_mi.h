@@ -109,6 +109,17 @@ const char* om_last_error(const om_context* ctx);
 /* Integrate np trajectories on the device (RK4 + clamped cubic spline); they
  * stay resident in the context, indexed 0..np-1.  Replaces any previous set. */
 int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const om_particle* particles, int np);
+/* A trajectory given by the caller instead of integrated: set_trajectory and
+ * its file readers (src/OSCARSSR_Python.cc:5374-5566, src/OSCARSSR.cc:1140-1171).
+ * n >= 3 knots at strictly increasing times t [s]; p9 per knot = x [m], beta,
+ * a/c [1/s].  construct bit 0 builds beta from the positions' spline slope / c,
+ * bit 1 builds a/c from beta's spline slope (src/TParticleTrajectoryPoints.cc:820-862);
+ * those columns of p9 are then ignored.  The knots are splined as the reference
+ * sets up its interpolated trajectory (src/TParticleA.cc:358-369).  Replaces any
+ * previous set with this one trajectory (index 0); particle supplies Q/M/gamma
+ * and the current.  Batched sums need integrated trajectories. */
+int om_trajectory_set(om_context* ctx, const om_particle* particle, long n, const double* t, const double* p9,
+                      int construct);
 /* Knots of trajectory i: *n in = capacity, out = count; any pointer may be NULL. */
 int om_trajectory_get(om_context* ctx, int i, long* n, double* t, double* p9, double* ypp9);
 /* Level samples of trajectory i (2^level rows of t,x,y,z,bx,by,bz,ax,ay,az). */

@@ -437,6 +437,9 @@ def ref_lib():
                                         C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_char_p, C.c_char_p]
         L.ref_set_seed.argtypes = [C.c_void_p, C.c_int]
         L.ref_set_nparticles.argtypes = [C.c_int]
+        L.ref_write_trajectory.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p]
+        L.ref_read_trajectory.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]
+        L.ref_set_trajectory_points.argtypes = [C.c_void_p, C.c_long, _dp, C.c_int, C.c_int]
         L.ref_random_normal.argtypes = [C.c_void_p]
         L.ref_random_normal.restype = C.c_double
         L.ref_random_uniform.argtypes = [C.c_void_p]
@@ -590,6 +593,27 @@ class RefSim:
         ypp = np.zeros((n, 9))
         self.L.ref_spline_ypp(self.h, ypp.ctypes.data_as(_dp))
         return a[:, 0].copy(), a[:, 1:].copy(), ypp
+
+    def trajectory(self):
+        """Knots of the current trajectory: (t[n], p9[n, 9], spline ypp[n, 9])."""
+        n = self.L.ref_trajectory_npoints(self.h)
+        a = np.zeros((n, 10))
+        self.L.ref_trajectory(self.h, a.ctypes.data_as(_dp))
+        ypp = np.zeros((n, 9))
+        self.L.ref_spline_ypp(self.h, ypp.ctypes.data_as(_dp))
+        return a[:, 0].copy(), a[:, 1:].copy(), ypp
+
+    def write_trajectory(self, text_path=None, bin_path=None, fmt=""):
+        self._chk(self.L.ref_write_trajectory(self.h, text_path.encode() if text_path else None,
+                                              bin_path.encode() if bin_path else None, fmt.encode()))
+
+    def read_trajectory(self, path, fmt="", binary=False):
+        self._chk(self.L.ref_read_trajectory(self.h, path.encode(), fmt.encode(), int(binary)))
+
+    def set_trajectory_points(self, t, p9, has_b=True, has_bp=True):
+        rows = np.ascontiguousarray(np.column_stack([t, p9]), dtype=np.float64)
+        self._chk(self.L.ref_set_trajectory_points(self.h, len(rows), rows.ctypes.data_as(_dp), int(has_b),
+                                                   int(has_bp)))
 
     def level(self, L):
         out = np.zeros((2 ** L, 10))

@@ -230,6 +230,46 @@ int ref_trajectory(void* h, double* out) {
   return 0;
 }
 
+// calculate_trajectory's ofile/bofile writers on the current trajectory
+// (src/OSCARSSR_Python.cc:5217-5229); a null path writes nothing.
+int ref_write_trajectory(void* h, const char* text_path, const char* bin_path, const char* format) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    if (text_path) s->WriteTrajectory(text_path, format);
+    if (bin_path) s->WriteTrajectoryBinary(bin_path, format);
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+// set_trajectory(ifile=...) / set_trajectory(bifile=...) for the current particle
+// (src/OSCARSSR.cc:1140-1171).
+int ref_read_trajectory(void* h, const char* path, const char* format, int binary) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    if (binary) s->CurrentParticleReadTrajectoryBinary(path, format);
+    else s->CurrentParticleReadTrajectory(path, format);
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+// set_trajectory(trajectory=[...]): n points of [t, x(3), b(3), a/c(3)]; has_b /
+// has_bp say which of b and a/c the points carry (src/OSCARSSR_Python.cc:5460-5506).
+int ref_set_trajectory_points(void* h, long n, double const* rows, int has_b, int has_bp) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    TParticleTrajectoryPoints& T = s->GetNewTrajectory();
+    for (long i = 0; i < n; ++i) {
+      double const* r = rows + 10 * i;
+      T.AddPoint(v3(r + 1), has_b ? v3(r + 4) : TVector3D(0, 0, 0), has_b && has_bp ? v3(r + 7) : TVector3D(0, 0, 0),
+                 r[0]);
+    }
+    if (!has_b) T.ConstructBetaAtPoints();
+    if (!has_bp) T.ConstructAoverCAtPoints();
+    s->SetupTrajectoryInterpolated();
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
 // Spline second derivatives: n rows of 9 values
 int ref_spline_ypp(void* h, double* out) {
   OSCARSSR* s = static_cast<OSCARSSR*>(h);

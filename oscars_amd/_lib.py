@@ -58,7 +58,7 @@ class om_stats(C.Structure):
 
 
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
-           "om_trajectories", "om_trajectory_get", "om_level_get", "om_trajectory_status", "om_flux",
+           "om_trajectories", "om_trajectory_set", "om_trajectory_get", "om_level_get", "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
 
@@ -91,6 +91,7 @@ def lib():
     L.om_last_error.restype = C.c_char_p
     L.om_trajectories.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.POINTER(om_particle), C.c_int]
     L.om_trajectory_get.argtypes = [C.c_void_p, C.c_int, _lp, _dp, _dp, _dp]
+    L.om_trajectory_set.argtypes = [C.c_void_p, C.POINTER(om_particle), C.c_long, _dp, _dp, C.c_int]
     L.om_level_get.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
     L.om_trajectory_status.argtypes = [C.c_void_p, C.c_int]
     L.om_flux.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, _dp, C.c_long, _dp, C.POINTER(om_conv), _dp, _ip,
@@ -115,6 +116,11 @@ def lib():
     L.om_adjoint_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(om_stats)]
     _lib = L
     return L
+
+
+def device_count():
+    """GPUs visible to the library (0 without one)."""
+    return int(lib().om_device_count())
 
 
 def _ptr(a, t=_dp):
@@ -169,6 +175,15 @@ class Device:
         for i in range(len(particles)):
             if self.L.om_trajectory_status(self.h, i) != 0:
                 raise ValueError("trajectory %d: |beta| >= 1 met during RK4 integration" % i)
+
+    def trajectory_set(self, particle, t, p9, construct=0):
+        """Make (t[n], p9[n, 9]) the one resident trajectory; construct bit 0 /
+        bit 1 rebuild beta / a/c from the spline slopes on the device."""
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        p9 = np.ascontiguousarray(p9, dtype=np.float64).reshape(-1, 9)
+        if len(p9) != len(t):
+            raise ValueError("t and p9 differ in length")
+        self._chk(self.L.om_trajectory_set(self.h, C.byref(particle), len(t), _ptr(t), _ptr(p9), int(construct)))
 
     def trajectory(self, i=0):
         n = C.c_long(0)

@@ -31,6 +31,8 @@ hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, h
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
+hipError_t om_launch_spline_user(const double* t, double* p, double* ypp, double* u, long n, int construct,
+                                 hipStream_t s);
 hipError_t om_launch_spline(const om::TrajDesc* desc, int np, const double* t, const double* p, double* ypp,
                             double* u, double* sig, double* sm1, double* pv, double* ppv, double* u2, long total,
                             hipStream_t s);
@@ -216,6 +218,7 @@ struct om_context {
   int nfields = 0;
   long total_knots = 0;
   bool desc_pending = false;  // trajectories launched, status not read back yet
+  bool user_traj = false;     // slot 0 holds knots given by the caller (nothing to recompute)
   // batched multi-particle state
   struct Batch {
     bool ready = false;
@@ -846,6 +849,7 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->d_fields.ensure(nfields > 0 ? nfields : 1);
     if (nfields > 0) upload_staged(ctx, ctx->stage_fields, ctx->d_fields, F.data(), F.size());
     ctx->np = 0;
+    ctx->user_traj = false;
     ctx->desc.assign(np, om::TrajDesc());
     ctx->parts.assign(particles, particles + np);
     ctx->tstart.assign(np, 0);
@@ -896,6 +900,58 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->desc_pending = true;
     if ((int) ctx->levels.size() != np) ctx->levels.assign(np, LevelCache());
     ctx->np = np;
+  });
+}
+
+int om_trajectory_set(om_context* ctx, const om_particle* particle, long n, const double* t, const double* p9,
+                      int construct) {
+  return guarded(ctx, [&] {
+    if (!particle || !t || !p9) throw std::invalid_argument("missing trajectory inputs");
+    if (construct < 0 || construct > 3) throw std::invalid_argument("construct must be 0..3");
+    // src/TParticleA.cc:358-369 and include/TOMATH.h:141-160
+    if (n < 2) throw std::length_error("Not enough points in this trajectory");
+    if (n < 3) throw std::length_error("TSpline1D3 does not see enough points for interpolation.  Please use >= 3 points");
+    for (long i = 1; i < n; ++i)
+      if (!(t[i] > t[i - 1])) throw std::invalid_argument("trajectory times must increase strictly");
+    sync_desc(ctx);
+    if (ctx->levels.size() != 1) ctx->free_levels();
+    for (auto& lc : ctx->levels)
+      for (int L = 0; L <= om::kMaxLevel; ++L) {
+        lc.built[L] = false;
+        lc.have_bd[L] = false;
+      }
+    ctx->batch.drop_graphs();
+    ctx->batch.ready = false;
+    ctx->adj.ready = false;
+    ctx->np = 0;
+    ctx->user_traj = true;
+    om::TrajDesc D;
+    std::memset(&D, 0, sizeof(D));
+    D.n = D.nf = n;
+    D.t0_over_c = t[0];
+    D.qomg = particle->qomg;
+    for (int k = 0; k < 3; ++k) {
+      D.x0[k] = p9[k];
+      D.b0[k] = p9[3 + k];
+    }
+    ctx->desc.assign(1, D);
+    ctx->parts.assign(particle, particle + 1);
+    ctx->tstart.assign(1, t[0]);
+    ctx->tstop.assign(1, t[n - 1]);
+    ctx->total_knots = n;
+    ctx->d_t.ensure(n);
+    ctx->d_p.ensure(9 * n);
+    ctx->d_ypp.ensure(9 * n);
+    ctx->d_u.ensure(9 * n);
+    upload(ctx, ctx->d_desc, ctx->desc.data(), 1);
+    upload(ctx, ctx->d_t, t, n);
+    upload(ctx, ctx->d_p, p9, 9 * n);
+    check(om_launch_spline_user(ctx->d_t.p, ctx->d_p.p, ctx->d_ypp.p, ctx->d_u.p, n, construct, ctx->stream),
+          "user spline");
+    // the host buffers are the caller's: wait before returning
+    check(hipStreamSynchronize(ctx->stream), "sync");
+    if (ctx->levels.size() != 1) ctx->levels.assign(1, LevelCache());
+    ctx->np = 1;
   });
 }
 
@@ -1074,6 +1130,7 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     if (n_req < 1 || !req || !req_of || !weight || !conv) throw std::invalid_argument("missing batch inputs");
     if (conv->mode != OM_MODE_TILED && conv->mode != OM_MODE_MIXED)
       throw std::invalid_argument("batched sums use the tiled kernels: mode must be TILED or MIXED");
+    if (ctx->user_traj) throw std::invalid_argument("batched sums need integrated trajectories (om_trajectories)");
     sync_desc(ctx);
     for (int i = 0; i < np; ++i) {
       if (ctx->desc[i].status != 0) throw std::out_of_range("trajectory integration met |beta| >= 1");

@@ -237,6 +237,53 @@ __global__ void __launch_bounds__(256) spline_b_kernel(const TrajDesc* __restric
   ypp[9 * g + c] = v;
 }
 
+// Trajectories given by the user (any increasing knot times): the sequential
+// sweep of include/TOMATH.h:141-203 itself, one lane per channel c0..c0+nc-1
+// of the nine knot channels; u is scratch of the knot layout.
+__global__ void __launch_bounds__(64) spline_seq_kernel(const double* __restrict__ X, const double* __restrict__ p,
+                                                        double* __restrict__ ypp, double* __restrict__ u, long n,
+                                                        int c0, int nc) {
+  if ((int) threadIdx.x >= nc) return;
+  const int c = c0 + (int) threadIdx.x;
+  const double* Y = p + c;
+  double* P = ypp + c;
+  double* U = u + c;
+  const double dydx0 = (Y[9] - Y[0]) / (X[1] - X[0]);
+  P[0] = -0.5;
+  U[0] = (3. / (X[1] - X[0])) * ((Y[9] - Y[0]) / (X[1] - X[0]) - dydx0);
+  for (long i = 1; i < n - 1; ++i) {
+    const double sig = (X[i] - X[i - 1]) / (X[i + 1] - X[i - 1]);
+    const double pp = sig * P[9 * (i - 1)] + 2.0;
+    P[9 * i] = (sig - 1.) / pp;
+    const double ui = (Y[9 * (i + 1)] - Y[9 * i]) / (X[i + 1] - X[i]) - (Y[9 * i] - Y[9 * (i - 1)]) / (X[i] - X[i - 1]);
+    U[9 * i] = (6. * ui / (X[i + 1] - X[i - 1]) - sig * U[9 * (i - 1)]) / pp;
+  }
+  const double qn = 0.5;
+  const double hN = X[n - 1] - X[n - 2];
+  const double dydxN = (Y[9 * (n - 1)] - Y[9 * (n - 2)]) / hN;
+  const double un = (3. / hN) * (dydxN - (Y[9 * (n - 1)] - Y[9 * (n - 2)]) / hN);
+  P[9 * (n - 1)] = (un - qn * U[9 * (n - 2)]) / (qn * P[9 * (n - 2)] + 1.);
+  for (long k = n - 2; k >= 0; --k) P[9 * k] = P[9 * k] * P[9 * (k + 1)] + U[9 * k];
+}
+
+// Spline slope at the knots (include/TOMATH.h:131-139 GetDerivative), divided
+// by div, from channels c_src..c_src+2 into c_dst..c_dst+2: the reference's
+// construction of beta from positions (div = c) and of a/c from beta (div = 1)
+// (src/TParticleTrajectoryPoints.cc:820-862).
+__global__ void __launch_bounds__(256) knot_slope_kernel(const double* __restrict__ X, double* __restrict__ p,
+                                                         const double* __restrict__ ypp, long n, int c_src,
+                                                         int c_dst, double div) {
+  long g = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= 3 * n) return;
+  long i = g / 3;
+  int k = (int) (g - 3 * i);
+  long j = i >= n - 1 ? n - 2 : i;
+  const double h = X[j + 1] - X[j];
+  const double y0 = p[9 * j + c_src + k], y1 = p[9 * (j + 1) + c_src + k];
+  const double d = (y1 - y0) / h - ypp[9 * j + c_src + k] * h / 3 - ypp[9 * (j + 1) + c_src + k] * h / 6;
+  p[9 * i + c_dst + k] = d / div;
+}
+
 // ---------------------------------------------------------------- level samples
 
 __global__ void __launch_bounds__(256) level_kernel(LevelJob J) {
@@ -356,6 +403,23 @@ extern "C" hipError_t om_launch_spline(const om::TrajDesc* desc, int np, const d
   hipLaunchKernelGGL(om::spline_p_kernel, dim3(b1), dim3(256), 0, s, desc, np, sig, sm1, pv, ppv, total);
   hipLaunchKernelGGL(om::spline_u_kernel, dim3(b9), dim3(256), 0, s, desc, np, t, p, sig, ppv, u, u2, total);
   hipLaunchKernelGGL(om::spline_b_kernel, dim3(b9), dim3(256), 0, s, desc, np, t, p, pv, u2, ypp, total);
+  return hipGetLastError();
+}
+
+// Spline of a user trajectory (one particle's knots), with the optional
+// construction of beta (construct & 1) and a/c (construct & 2) first.
+extern "C" hipError_t om_launch_spline_user(const double* t, double* p, double* ypp, double* u, long n,
+                                            int construct, hipStream_t s) {
+  unsigned b3 = (unsigned) ((3 * n + 255) / 256);
+  if (construct & 1) {
+    hipLaunchKernelGGL(om::spline_seq_kernel, dim3(1), dim3(64), 0, s, t, p, ypp, u, n, 0, 3);
+    hipLaunchKernelGGL(om::knot_slope_kernel, dim3(b3), dim3(256), 0, s, t, p, ypp, n, 0, 3, om::kC);
+  }
+  if (construct & 2) {
+    hipLaunchKernelGGL(om::spline_seq_kernel, dim3(1), dim3(64), 0, s, t, p, ypp, u, n, 3, 3);
+    hipLaunchKernelGGL(om::knot_slope_kernel, dim3(b3), dim3(256), 0, s, t, p, ypp, n, 3, 6, 1.0);
+  }
+  hipLaunchKernelGGL(om::spline_seq_kernel, dim3(1), dim3(64), 0, s, t, p, ypp, u, n, 0, 9);
   return hipGetLastError();
 }
 

@@ -269,3 +269,188 @@ def average_points(ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
         write_points_binary(bofile, acc.x, acc.v, dim)
     return [[list(acc.x[i]), acc.v[i]] for i in range(len(acc.x))]
 
+
+
+# ---------------------------------------------------------------- trajectories
+# src/TParticleTrajectoryPoints.cc:259-818.  Columns: T [s], X Y Z [m], BX BY BZ
+# (beta), BPX BPY BPZ (a/c [1/s]).
+TRAJ_DEFAULT_FORMAT = "T X Y Z BX BY BZ BPX BPY BPZ"
+_TRAJ_COLUMN = {"T": 0, "X": 1, "Y": 2, "Z": 3, "BX": 4, "BY": 5, "BZ": 6, "BPX": 7, "BPY": 8, "BPZ": 9}
+
+
+def _fmt35(x):
+    # std::scientific at precision 35
+    if math.isnan(x):
+        return "-nan" if math.copysign(1.0, x) < 0 else "nan"
+    return "%.35e" % x
+
+
+def _traj_rows(t, p9):
+    t = np.asarray(t, dtype=np.float64).reshape(-1)
+    p9 = np.asarray(p9, dtype=np.float64).reshape(-1, 9)
+    return np.column_stack([t, p9])
+
+
+def _format_words(fmt):
+    words = fmt.upper().split()
+    if len(words) < 1:
+        raise ValueError("Format must contain at least one element")
+    return words
+
+
+def write_trajectory_text(path, t, p9, fmt=""):
+    """TParticleTrajectoryPoints::WriteToFile: '# <format>' then one line per
+    knot, values in std::scientific with 35 digits after the point."""
+    try:
+        f = open(path, "w")
+    except OSError:
+        raise ValueError("cannot open output file")
+    rows = _traj_rows(t, p9)
+    with f:
+        if fmt.upper() == "":
+            f.write("# " + TRAJ_DEFAULT_FORMAT + "\n")
+            for r in rows:
+                f.write(" ".join(_fmt35(v) for v in r) + "\n")
+            return
+        words = _format_words(fmt)
+        f.write("# " + fmt + "\n")
+        for r in rows:
+            vals = []
+            for w in words:
+                if w not in _TRAJ_COLUMN:
+                    raise ValueError("format specifier not recognized")
+                vals.append(_fmt35(r[_TRAJ_COLUMN[w]]))
+            f.write(" ".join(vals) + "\n")
+
+
+def write_trajectory_binary(path, t, p9, fmt=""):
+    """TParticleTrajectoryPoints::WriteToFileBinary: int32 length and the
+    upper-case format string, then float64 values knot by knot."""
+    try:
+        f = open(path, "wb")
+    except OSError:
+        raise ValueError("cannot open output file")
+    rows = _traj_rows(t, p9)
+    with f:
+        form = fmt.upper() or TRAJ_DEFAULT_FORMAT
+        if fmt.upper() != "":
+            words = _format_words(fmt)
+        else:
+            words = TRAJ_DEFAULT_FORMAT.split()
+        f.write(np.array([len(form)], dtype=np.int32).tobytes())
+        f.write(form.encode())
+        if len(rows) == 0:
+            return
+        for w in words:
+            if w not in _TRAJ_COLUMN:
+                raise ValueError("format specifier not recognized")
+        cols = [_TRAJ_COLUMN[w] for w in words]
+        f.write(np.ascontiguousarray(rows[:, cols]).tobytes())
+
+
+_CNUM = None
+
+
+def _c_double(tok):
+    """A whole token read as istream >> double reads it (no 'inf'/'nan',
+    no underscores), else None."""
+    global _CNUM
+    if _CNUM is None:
+        import re
+        _CNUM = re.compile(r"^[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?$")
+    return float(tok) if _CNUM.match(tok) else None
+
+
+def _traj_has(words):
+    has = lambda *k: any(w in words for w in k)  # noqa: E731
+    if not (has("T") and has("X", "Y", "Z")):
+        raise ValueError("trajectory must contain time and position in 1d at a minimum")
+    return has("BX", "BY", "BZ"), has("BPX", "BPY", "BPZ")
+
+
+def _store(words, vals, row):
+    for w, v in zip(words, vals):
+        if w in _TRAJ_COLUMN:
+            row[_TRAJ_COLUMN[w]] = v
+
+
+def read_trajectory_text(path, fmt=""):
+    """TParticleTrajectoryPoints::ReadFromFileFormat.  A first line that
+    starts with '#' and names T and one of X/Y/Z gives the format (unless fmt
+    is given); any other '#' line is a comment; columns named '*' are skipped.
+    Returns (t[n], p9[n, 9], has_beta, has_beta_prime)."""
+    try:
+        lines = open(path).read().split("\n")
+    except OSError:
+        raise ValueError("cannot open input file specified")
+    if lines and lines[-1] == "":
+        lines.pop()
+    found = ""
+    head = lines[0].upper().split() if lines else []
+    if head and head[0][0] != "#":
+        start = 0
+    else:
+        start = 1
+        if head:
+            head = [head[0][1:]] + head[1:] if len(head[0]) != 1 else head[1:]
+            if "T" in head and any(k in head for k in ("X", "Y", "Z")):
+                found = "".join(" " + h for h in head)
+    form = fmt.upper()
+    if fmt == "" and found != "":
+        form = found
+    form = form.upper() or TRAJ_DEFAULT_FORMAT
+    words = form.split()
+    has_b, has_bp = _traj_has(words)
+    rows = []
+    for line in lines[start:]:
+        toks = line.split()
+        vals = []
+        it = 0
+        for w in words:
+            if w == "*":
+                it += 1
+                vals.append(0.0)
+                continue
+            v = _c_double(toks[it]) if it < len(toks) else None
+            if v is None:
+                raise ValueError("When parsing the input trajectory file the format specified does not match the "
+                                 "data in length")
+            it += 1
+            vals.append(v)
+        row = [0.0] * 10
+        _store(words, vals, row)
+        if math.sqrt(row[4] * row[4] + row[5] * row[5] + row[6] * row[6]) >= 1.0:
+            raise ValueError("Magnitude of beta >= 1 in input file.  Sorry, this is beyond einstein.  Line:\n" + line)
+        rows.append(row)
+    a = np.array(rows, dtype=np.float64).reshape(-1, 10)
+    return a[:, 0].copy(), a[:, 1:].copy(), has_b, has_bp
+
+
+def read_trajectory_binary(path, fmt=""):
+    """TParticleTrajectoryPoints::ReadFromFileBinary.  Without fmt the file
+    starts with an int32 length and the format string; with fmt the file is
+    read as data from its first byte.  A last incomplete record is dropped."""
+    try:
+        data = open(path, "rb").read()
+    except OSError:
+        raise ValueError("cannot open input file: " + path)
+    off = 0
+    form = fmt
+    if fmt == "":
+        n = int(np.frombuffer(data[:4], dtype=np.int32)[0]) if len(data) >= 4 else 0
+        if n < 1:
+            raise ValueError("Trying to read binary file format from file header failed.  Incorrect format.")
+        form = data[4:4 + n].decode("latin-1")
+        off = 4 + n
+    words = _format_words(form)
+    has_b, has_bp = _traj_has(words)
+    nrec = (len(data) - off) // (8 * len(words))
+    vals = np.frombuffer(data[off:off + 8 * len(words) * nrec], dtype=np.float64).reshape(nrec, len(words))
+    a = np.zeros((nrec, 10))
+    for k, w in enumerate(words):
+        if w in _TRAJ_COLUMN:
+            a[:, _TRAJ_COLUMN[w]] = vals[:, k]
+    for i in range(nrec):
+        if math.sqrt(a[i, 4] * a[i, 4] + a[i, 5] * a[i, 5] + a[i, 6] * a[i, 6]) >= 1.0:
+            raise ValueError("Magnitude of beta >= 1 in input file.  Sorry, this is beyond einstein.  ipoint: %d" % i)
+    return a[:, 0].copy(), a[:, 1:].copy(), has_b, has_bp

@@ -5,9 +5,11 @@ Mirrors the reference Python API for the radiation hot path
 :1309, ``add_bfield_epu`` :1479, ``add_bfield_file`` :526, ``set_ctstartstop``
 :345, ``set_npoints_trajectory`` :409, ``calculate_trajectory`` :5181,
 ``calculate_spectrum`` :5653, ``calculate_flux_rectangle`` :7977,
-``calculate_power_density_rectangle`` :6471): same keyword names, defaults,
-return shapes and ValueError messages.  Every calculation runs on the GPU
-through liboscars_mi.so; there is no CPU path, so ``gpu=0`` still uses the GPU.
+``calculate_power_density_rectangle`` :6471, ``set_trajectory`` :5374,
+``set_seed``, ``set_new_particle`` :4589, ``average_spectra`` :8460): same
+keyword names, defaults, return shapes and ValueError messages.  Every
+calculation runs on one GPU through liboscars_mi.so; there is no CPU path, so
+``gpu=0`` still uses the GPU and ``ngpu`` is accepted and ignored.
 
 Extensions beyond the reference (keyword ``mode`` and the method
 ``calculate_flux_rectangle_energies``) select the kernel family and evaluate
@@ -79,6 +81,23 @@ class sr:
     def print_gpu(self):
         print(self._device().info())
 
+    def check_gpu(self):
+        """Number of GPUs the library sees (src/OSCARSSR_Python.cc:181-190)."""
+        return _lib.device_count()
+
+    def set_gpu_global(self, gpu):
+        """Kept for the reference's API: every calculation here runs on the GPU."""
+        if gpu not in (0, 1):
+            raise ValueError("'gpu' must be 0 or 1")
+        self._gpu_global = int(gpu)
+
+    def set_nthreads_global(self, nthreads):
+        """Kept for the reference's API (its CPU thread count); the GPU path has
+        no host threads to set."""
+        if nthreads < 1:
+            raise ValueError("'nthreads' must be >= 1")
+        self._nthreads = int(nthreads)
+
     # ------------------------------------------------------------ fields
     def add_bfield_undulator(self, bfield, period, nperiods, phase=0.0, rotations=None, translation=None, taper=0.0,
                              frequency=0.0, frequency_phase=0.0, time_offset=0.0, name=""):
@@ -142,9 +161,9 @@ class sr:
         self.add_bfield_grid(start, step, npts, data, rotations, translation)
 
     def clear_bfields(self):
+        # src/OSCARSSR.cc:186-196: the current trajectory stays until a field is added
         self._fields = []
         self._grids = []
-        self._traj_valid = False
 
     # ------------------------------------------------------------ beam / trajectory window
     def set_particle_beam(self, type="electron", name="", energy_GeV=-1, d0=None, x0=None, beam="",
@@ -200,7 +219,7 @@ class sr:
                 beam_p.update(beta=pre["beta"], alpha=[0.0, 0.0], gamma=None)
         if distribution:
             dist = B.distribution_of(emittance, distribution)
-        self._beam = dict(type=type, energy_GeV=energy_GeV, x0=pos, d0=direction, t0=float(t0),
+        self._beam = dict(type=type, name=name, energy_GeV=energy_GeV, x0=pos, d0=direction, t0=float(t0),
                           current=float(current), sigma_energy_GeV=float(sigma_energy_GeV),
                           horizontal_unit=hdir, distribution=dist, q=p.q, m=p.m, **beam_p)
         self._particle = p
@@ -233,17 +252,17 @@ class sr:
         self._traj_valid = False
 
     def set_ctstartstop(self, start, stop):
-        # src/OSCARSSR.cc:637-647
+        # src/OSCARSSR.cc:637-647.  As in the reference, a trajectory already
+        # computed or set stays in use: the new window applies to the next one
+        # (a new particle or field, or calculate_trajectory).
         self._npoints = int(self._npoints_per_meter * (stop - start))
         self._ctstart = float(start)
         self._ctstop = float(stop)
-        self._traj_valid = False
 
     def set_npoints_trajectory(self, npoints):
         if npoints < 1:
             raise ValueError("npoints must be >= 1")
         self._npoints = int(npoints)
-        self._traj_valid = False
 
     def set_npoints_per_meter_trajectory(self, npoints):
         self._npoints_per_meter = int(npoints)
@@ -286,11 +305,78 @@ class sr:
         self._device().trajectories(self._fields, [self._om_particle(self._particle)])
         self._traj_valid = True
 
-    def calculate_trajectory(self):
-        """[[t, [x, y, z], [bx, by, bz], [ax, ay, az]], ...] like the reference."""
+    def calculate_trajectory(self, ofile="", bofile="", oformat=""):
+        """Integrate the current particle's trajectory (always anew, replacing
+        one given by set_trajectory) and return it as get_trajectory does;
+        ofile / bofile write it as text / binary in oformat
+        (src/OSCARSSR_Python.cc:5181-5246)."""
+        self._traj_valid = False
         self._ensure_trajectory()
+        if ofile or bofile:
+            t, p, _ = self._device().trajectory(0)
+            if ofile:
+                IO.write_trajectory_text(ofile, t, p, oformat)
+            if bofile:
+                IO.write_trajectory_binary(bofile, t, p, oformat)
+        return self.get_trajectory()
+
+    def get_trajectory(self):
+        """[[t, [x, y, z], [bx, by, bz], [ax, ay, az]], ...] of the current
+        trajectory; [] when none has been computed or set."""
+        if not self._traj_valid:
+            return []
         t, p, _ = self._device().trajectory(0)
         return [[float(t[i]), p[i, 0:3].tolist(), p[i, 3:6].tolist(), p[i, 6:9].tolist()] for i in range(len(t))]
+
+    def set_trajectory(self, beam="", trajectory=None, ifile=None, bifile=None, iformat=""):
+        """Use a given trajectory instead of integrating one: a list of
+        [t, [x, y, z]] / [t, [x, y, z], [bx, by, bz]] / [t, x, b, [ax, ay, az]]
+        points (t in s, a/c in 1/s; beta and a/c missing from the first point
+        are rebuilt from spline slopes), or a text / binary file in the
+        reference's trajectory formats (src/OSCARSSR_Python.cc:5374-5566).
+        Times must increase strictly."""
+        if self._beam is None:
+            raise ValueError("Must previously have defined a particle beam to associate this trajectory with.  try: "
+                             "add_particle_beam() or set_particle_beam() first")
+        if ifile is not None and bifile is not None:
+            raise ValueError("Cannot specify both text and binary input files")
+        if trajectory is not None and (ifile is not None or bifile is not None):
+            raise ValueError("Cannot specify both an input file and a trajectory list")
+        if beam and beam != self._beam.get("name", ""):
+            raise ValueError("no beam by that name found")
+        if beam:
+            self.set_new_particle(particle="ideal")
+        if ifile is not None or bifile is not None:
+            if ifile:
+                t, p9, has_b, has_bp = IO.read_trajectory_text(ifile, iformat)
+            elif bifile:
+                t, p9, has_b, has_bp = IO.read_trajectory_binary(bifile, iformat)
+            else:
+                return
+        elif trajectory is not None:
+            if len(trajectory) < 1:
+                raise ValueError("Input trajectory is not the correct format")
+            has_b = len(trajectory[0]) >= 3
+            has_bp = len(trajectory[0]) >= 4
+            t = np.zeros(len(trajectory))
+            p9 = np.zeros((len(trajectory), 9))
+            for i, pt in enumerate(trajectory):
+                try:
+                    if len(pt) <= 1:
+                        raise ValueError
+                    t[i] = float(pt[0])
+                    p9[i, 0:3] = _v3(pt[1], "trajectory")
+                    if has_b:
+                        p9[i, 3:6] = _v3(pt[2], "trajectory")
+                    if has_bp:
+                        p9[i, 6:9] = _v3(pt[3], "trajectory")
+                except (ValueError, TypeError, IndexError):
+                    raise ValueError("Incorrect format in 'trajectory' entry number %d" % i)
+        else:
+            raise ValueError("Need to specify where to get the trajectory from, ifile, trajectory, etc..")
+        construct = (0 if has_b else 1) | (0 if has_bp else 2)
+        self._device().trajectory_set(self._om_particle(self._particle), t, p9, construct)
+        self._traj_valid = True
 
     def get_trajectory_arrays(self):
         """(t, knots[n, 9], spline second derivatives[n, 9]) as numpy arrays (extension)."""
