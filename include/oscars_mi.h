@@ -144,6 +144,23 @@ int om_power_density(om_context* ctx, int particle, const double* obs_xyz, long 
                      int directional, const om_conv* conv, double* out_value, int* out_level,
                      unsigned char* out_conv, om_stats* stats);
 
+/* Power density on a surface of points: calculate_power_density(points=...)
+ * and calculate_power_density_line (src/OSCARSSR_Python.cc:6105-6380,
+ * 7355-7560; integrand src/OSCARSSR.cc:2817-2978).  normals[3 n_obs] holds a
+ * unit normal per point, or NULL for a surface without normals (the line),
+ * where the reference takes N.normal as 1.  Host pointers. */
+int om_power_density_surface(om_context* ctx, int particle, const double* obs_xyz, long n_obs,
+                             const double* normals, int directional, const om_conv* conv, double* out_value,
+                             int* out_level, unsigned char* out_conv, om_stats* stats);
+
+/* Total radiated power of trajectory `particle` [W]: calculate_total_power
+ * (src/OSCARSSR_Python.cc:5974-6060, src/OSCARSSR.cc:3725-3843).  conv gives
+ * precision and the level limits; *out_level is -1 when no level met the
+ * precision (the reference raises "total power calculation did not
+ * converge"), with *out_power the last level's value. */
+int om_total_power(om_context* ctx, int particle, const om_conv* conv, double* out_power, double* out_precision,
+                   int* out_level, om_stats* stats);
+
 /* Device-resident variants: the same calculations with obs_xyz, energies_eV
  * and every output (out_value, out_level, out_conv, out_e) device pointers on
  * the context's device (e.g. torch tensors' data pointers); pol6, normal, conv

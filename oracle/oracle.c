@@ -710,6 +710,8 @@ typedef struct {
   double C0, C2;
   /* power density */
   const double* normal;
+  const double* normal_list; /* per-point normals (surfaces of points), or NULL */
+  int has_normal;            /* 0: surface without normals, N1.Normal taken as 1 */
   int directional;
   double pd_const;
   /* outputs */
@@ -838,7 +840,8 @@ static void pd_point(lw_job* J, const double* Obs, const double* Normal, long i,
       v_unit(o, N2);
       v_cross(N1, N2, c);
       v_unit(c, N3);
-      double N1DotNormal = v_dot(N1, Normal); /* rectangles always have a normal */
+      /* src/OSCARSSR.cc:2846, 2908: without normals N1.Normal is taken as 1 */
+      double N1DotNormal = J->has_normal ? v_dot(N1, Normal) : 1;
       if (J->directional && N1DotNormal <= 0) continue;
       double NmB[3] = {N1[0] - B[0], N1[1] - B[1], N1[2] - B[2]};
       double c1[3], Num[3];
@@ -887,7 +890,7 @@ static void* lw_thread(void* arg) {
   for (long i = J->i0; i <= J->i1; i += J->stride) {
     if (!J->o->carry_quirk) { rp = -1; rl = -1; }
     if (J->mode == 1) {
-      pd_point(J, J->obs_list + 3 * i, J->normal, i, &rp, &rl);
+      pd_point(J, J->obs_list + 3 * i, J->normal_list ? J->normal_list + 3 * i : J->normal, i, &rp, &rl);
     } else if (J->obs_list) {
       lw_point(J, J->obs_list + 3 * i, J->omega_single, i, &rp, &rl);
     } else {
@@ -1002,12 +1005,89 @@ int orc_power_density(const orc_traj* T, const orc_particle* P, const double* ob
   J.mode = 1;
   J.obs_list = obs_xyz;
   J.normal = normal;
+  J.normal_list = NULL;
+  J.has_normal = 1;
   J.directional = directional;
   J.pd_const = fabs(P->q * P->current) / (16 * k_pi2() * K_EPS0 * K_C);
   J.out_value = out_value; J.out_level = out_level; J.out_conv = out_conv;
   run_partitioned(&J, n_obs, 0);
   lc_free(&lc);
   return 0;
+}
+
+/* Power density on a surface of points (src/OSCARSSR_Python.cc:6105-6380 and
+ * 7355-7560; src/OSCARSSR.cc:2817-2978): normals[3 n] are unit normals per
+ * point, or NULL for a surface without normals (a line of points). */
+int orc_power_density_surface(const orc_traj* T, const orc_particle* P, const double* obs_xyz,
+                              const double* normals, long n_obs, int directional, const orc_opts* o,
+                              double* out_value, int* out_level, unsigned char* out_conv) {
+  lw_job J;
+  level_cache lc;
+  int sm, se;
+  level_stops(o, &sm, &se);
+  lc_init(&lc, T, sm);
+  job_common(&J, T, P, &lc, o, NULL);
+  J.mode = 1;
+  J.obs_list = obs_xyz;
+  J.normal = NULL;
+  J.normal_list = normals;
+  J.has_normal = normals != NULL;
+  J.directional = directional;
+  J.pd_const = fabs(P->q * P->current) / (16 * k_pi2() * K_EPS0 * K_C);
+  J.out_value = out_value; J.out_level = out_level; J.out_conv = out_conv;
+  run_partitioned(&J, n_obs, 0);
+  lc_free(&lc);
+  return 0;
+}
+
+/* Total radiated power of one particle (src/OSCARSSR.cc:3725-3843): one
+ * running sum over all levels of (|A/c|^2 - |B x A/c|^2) / (1 - B.B)^3.
+ * Returns 0, or 1 when no level met the precision (the reference throws). */
+int orc_total_power(const orc_traj* T, const orc_particle* P, const orc_opts* o, double* out_power,
+                    double* out_precision, int* out_level) {
+  level_cache lc;
+  int sm, se;
+  level_stops(o, &sm, &se);
+  lc_init(&lc, T, sm);
+  double Result_Precision = -1;
+  int Result_Level = -1;
+  double LastSum = -1;
+  double Sum = 0;
+  for (int L = 0; L <= se; ++L) {
+    double Last_Beta[3] = {0, 0, 0};
+    double BetaDiffMax = -1;
+    const double* S = lc_get(&lc, L);
+    long NT = (long) pow(2, L);
+    for (long iT = 0; iT != NT; ++iT) {
+      const double* B = S + 10 * iT + 4;
+      const double* A = S + 10 * iT + 7;
+      double Gamma6 = 1. / pow(1 - v_dot(B, B), 3);
+      double dB[3] = {B[0] - Last_Beta[0], B[1] - Last_Beta[1], B[2] - Last_Beta[2]};
+      double BetaDiff = sqrt(v_mag2(dB));
+      if (iT > 0 && BetaDiff > BetaDiffMax) BetaDiffMax = BetaDiff;
+      Last_Beta[0] = B[0]; Last_Beta[1] = B[1]; Last_Beta[2] = B[2];
+      double c[3];
+      v_cross(B, A, c);
+      Sum += (v_mag2(A) - v_mag2(c)) * Gamma6;
+    }
+    double ThisSum = Sum * orc_dt_inclusive(T, L);
+    Result_Precision = fabs(ThisSum - LastSum) / LastSum;
+    if (L > 8 && Result_Precision < o->precision && BetaDiffMax < 2. / (P->gamma)) {
+      Result_Level = L;
+      LastSum = ThisSum;
+      break;
+    } else if (L > 8 && ThisSum == LastSum) {
+      Result_Level = L;
+      Result_Precision = 0;
+      break;
+    }
+    LastSum = ThisSum;
+  }
+  lc_free(&lc);
+  *out_power = LastSum * fabs(P->q * P->current) / (6 * K_PI * K_EPS0 * K_C);
+  if (out_precision) *out_precision = Result_Precision;
+  if (out_level) *out_level = Result_Level;
+  return Result_Level == -1 ? 1 : 0;
 }
 
 double orc_evals_from_levels(long n, const int* levels) {

@@ -121,6 +121,13 @@ int orc_power_density(const orc_traj* T, const orc_particle* P, const double* ob
                       long n_obs, int directional, const orc_opts* o, double* out_value, int* out_level,
                       unsigned char* out_conv);
 
+int orc_power_density_surface(const orc_traj* T, const orc_particle* P, const double* obs_xyz,
+                              const double* normals, long n_obs, int directional, const orc_opts* o,
+                              double* out_value, int* out_level, unsigned char* out_conv);
+
+int orc_total_power(const orc_traj* T, const orc_particle* P, const orc_opts* o, double* out_power,
+                    double* out_precision, int* out_level);
+
 /* Count of integrand evaluations sum_p (2^(L_p+1) - 1) given per-point levels. */
 double orc_evals_from_levels(long n, const int* levels);
 

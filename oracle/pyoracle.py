@@ -108,6 +108,9 @@ def oracle_lib():
                                _dp]
         L.orc_power_density.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_long, C.c_int, C.c_void_p, _dp, _ip,
                                         _up]
+        L.orc_power_density_surface.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_long, C.c_int, C.c_void_p,
+                                                _dp, _ip, _up]
+        L.orc_total_power.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _ip]
         _oracle = L
     return _oracle
 
@@ -353,6 +356,32 @@ class Oracle:
                                  conv.ctypes.data_as(_up))
         return out, lev, conv
 
+    def power_density_surface(self, obs_xyz, normals=None, directional=False, precision=0.01, max_level=-2,
+                              max_level_extended=0, quantity="power density", nthreads=0, carry_quirk=True):
+        """Power density at points with per-point unit normals[n, 3], or none."""
+        T = self._traj()
+        obs = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(-1, 3)
+        n = len(obs)
+        nrm = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64).reshape(n, 3)
+        out = np.zeros(n)
+        lev = np.zeros(n, dtype=np.int32)
+        conv = np.zeros(n, dtype=np.uint8)
+        o = self._opts(precision, max_level, max_level_extended, self._quantity(quantity, pd=True), nthreads,
+                       carry_quirk)
+        self.L.orc_power_density_surface(C.byref(T.c()), C.byref(self.P), obs.ctypes.data_as(_dp),
+                                         None if nrm is None else nrm.ctypes.data_as(_dp), n, int(directional),
+                                         C.byref(o), out.ctypes.data_as(_dp), lev.ctypes.data_as(_ip),
+                                         conv.ctypes.data_as(_up))
+        return out, lev, conv
+
+    def total_power(self, precision=0.01, max_level=-2, max_level_extended=0):
+        """(power [W], precision, level); level -1 when not converged."""
+        T = self._traj()
+        o = self._opts(precision, max_level, max_level_extended, 0, 1, True)
+        pw, pr, lv = C.c_double(0), C.c_double(0), C.c_int(0)
+        self.L.orc_total_power(C.byref(T.c()), C.byref(self.P), C.byref(o), C.byref(pw), C.byref(pr), C.byref(lv))
+        return pw.value, pr.value, lv.value
+
     # -- reference-shaped entry points --
     def calculate_spectrum(self, obs, npoints=0, energy_range_eV=None, energy_points_eV=None, **kw):
         if energy_points_eV is not None:
@@ -438,6 +467,11 @@ def ref_lib():
         L.ref_set_seed.argtypes = [C.c_void_p, C.c_int]
         L.ref_set_nparticles.argtypes = [C.c_int]
         L.ref_write_trajectory.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p]
+        L.ref_power_density_points.argtypes = [C.c_void_p, C.c_long, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double,
+                                               C.c_int, C.c_int, C.c_int, _dp, _dp, _ip]
+        L.ref_power_density_line.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                             C.c_int, C.c_int, _dp, _dp, _ip]
+        L.ref_total_power.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp]
         L.ref_read_trajectory.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]
         L.ref_set_trajectory_points.argtypes = [C.c_void_p, C.c_long, _dp, C.c_int, C.c_int]
         L.ref_random_normal.argtypes = [C.c_void_p]
@@ -602,6 +636,41 @@ class RefSim:
         ypp = np.zeros((n, 9))
         self.L.ref_spline_ypp(self.h, ypp.ctypes.data_as(_dp))
         return a[:, 0].copy(), a[:, 1:].copy(), ypp
+
+    def calculate_power_density(self, points, normal=0, rotations=(0, 0, 0), translation=(0, 0, 0), precision=0.01,
+                                max_level=-2, max_level_extended=0, quantity="power density", nthreads=0,
+                                nparticles=0):
+        q = {"POWER DENSITY": 0, "POWERDENSITY": 0, "PRECISION": 1, "LEVEL": 2}[quantity.upper()]
+        self.L.ref_set_nparticles(int(nparticles))
+        xn = np.ascontiguousarray([list(p[0]) + list(p[1]) for p in points], dtype=np.float64)
+        n = len(xn)
+        xyz = np.zeros((n, 3))
+        v = np.zeros(n)
+        conv = C.c_int(0)
+        self._chk(self.L.ref_power_density_points(self.h, n, xn.ctypes.data_as(_dp), int(normal), _v3(rotations),
+                                                  _v3(translation), nthreads or self.nthreads, float(precision),
+                                                  int(max_level), int(max_level_extended), q,
+                                                  xyz.ctypes.data_as(_dp), v.ctypes.data_as(_dp), C.byref(conv)))
+        return xyz, v, bool(conv.value)
+
+    def calculate_power_density_line(self, x1, x2, npoints, normal=0, dim=1, precision=0.01, max_level=-2,
+                                     max_level_extended=0, nthreads=0, nparticles=0):
+        self.L.ref_set_nparticles(int(nparticles))
+        xyz = np.zeros((npoints, 3))
+        v = np.zeros(npoints)
+        conv = C.c_int(0)
+        self._chk(self.L.ref_power_density_line(self.h, _v3(x1), _v3(x2), int(npoints), int(normal), int(dim),
+                                                nthreads or self.nthreads, float(precision), int(max_level),
+                                                int(max_level_extended), xyz.ctypes.data_as(_dp),
+                                                v.ctypes.data_as(_dp), C.byref(conv)))
+        return xyz, v, bool(conv.value)
+
+    def calculate_total_power(self, precision=0.01, max_level=-2, max_level_extended=0, quantity="power"):
+        q = {"POWER": 0, "PRECISION": 1, "LEVEL": 2}[quantity.upper()]
+        out = C.c_double(0)
+        self._chk(self.L.ref_total_power(self.h, float(precision), int(max_level), int(max_level_extended), q,
+                                         C.byref(out)))
+        return out.value
 
     def write_trajectory(self, text_path=None, bin_path=None, fmt=""):
         self._chk(self.L.ref_write_trajectory(self.h, text_path.encode() if text_path else None,

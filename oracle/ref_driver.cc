@@ -20,6 +20,7 @@
 #include "TField3D_IdealEPU.h"
 #include "TField3D_Grid.h"
 #include "TSurfacePoints_Rectangle.h"
+#include "TSurfacePoints_3D.h"
 #include "TSpectrumContainer.h"
 #include "T3DScalarContainer.h"
 
@@ -388,6 +389,67 @@ int ref_power_density_rectangle(void* h, const char* plane, int n1, int n2, doub
       out_xyz[3 * i + 2] = p.GetX().GetZ();
       out_value[i] = p.GetV();
     }
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+static int pd_surface(OSCARSSR* s, TSurfacePoints_3D const& S, int dim, bool directional, int nthreads,
+                      double precision, int max_level, int max_level_extended, int quantity, double* out_xyz,
+                      double* out_value, int* out_all_converged) {
+  T3DScalarContainer C;
+  s->CalculatePowerDensity(S, C, dim, directional, precision, max_level, max_level_extended, g_nparticles, nthreads, 0,
+                           -1, std::vector<int>(), quantity);
+  if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
+  for (size_t i = 0; i != C.GetNPoints(); ++i) {
+    T3DScalar const& p = C.GetPoint(i);
+    out_xyz[3 * i + 0] = p.GetX().GetX();
+    out_xyz[3 * i + 1] = p.GetX().GetY();
+    out_xyz[3 * i + 2] = p.GetX().GetZ();
+    out_value[i] = p.GetV();
+  }
+  return 0;
+}
+
+// calculate_power_density(points=[[x, n], ...], normal, rotations, translation)
+// (src/OSCARSSR_Python.cc:6105-6380): points xn[6 n] (position, normal).
+int ref_power_density_points(void* h, long n, double const* xn, int normal, double const* rotations,
+                             double const* translation, int nthreads, double precision, int max_level,
+                             int max_level_extended, int quantity, double* out_xyz, double* out_value,
+                             int* out_all_converged) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    TSurfacePoints_3D S;
+    for (long i = 0; i < n; ++i) {
+      TVector3D X = v3(xn + 6 * i), N = v3(xn + 6 * i + 3);
+      X.RotateSelfXYZ(v3(rotations));
+      N.RotateSelfXYZ(v3(rotations));
+      if (normal == -1) N *= -1;
+      X += v3(translation);
+      S.AddPoint(X, N);
+    }
+    return pd_surface(s, S, 3, normal != 0, nthreads, precision, max_level, max_level_extended, quantity, out_xyz,
+                      out_value, out_all_converged);
+  } catch (std::exception const& e) { return fail(e); }
+}
+
+// calculate_power_density_line(x1, x2, npoints, normal, dim) (src/OSCARSSR_Python.cc:7355-7560)
+int ref_power_density_line(void* h, double const* x1, double const* x2, int npoints, int normal, int dim,
+                           int nthreads, double precision, int max_level, int max_level_extended, double* out_xyz,
+                           double* out_value, int* out_all_converged) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    TVector3D const Step = (v3(x2) - v3(x1)) / (npoints - 1);
+    TSurfacePoints_3D S;
+    for (int i = 0; i != npoints; ++i) S.AddPoint(v3(x1) + i * Step);
+    return pd_surface(s, S, dim, normal != 0, nthreads, precision, max_level, max_level_extended, 0, out_xyz,
+                      out_value, out_all_converged);
+  } catch (std::exception const& e) { return fail(e); }
+}
+
+// calculate_total_power (src/OSCARSSR_Python.cc:5974-6060)
+int ref_total_power(void* h, double precision, int max_level, int max_level_extended, int quantity, double* out) {
+  try {
+    *out = static_cast<OSCARSSR*>(h)->CalculateTotalPower(precision, max_level, max_level_extended, quantity);
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }

@@ -58,7 +58,8 @@ class om_stats(C.Structure):
 
 
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
-           "om_trajectories", "om_trajectory_set", "om_trajectory_get", "om_level_get", "om_trajectory_status", "om_flux",
+           "om_trajectories", "om_trajectory_set", "om_trajectory_get", "om_power_density_surface",
+           "om_total_power", "om_level_get", "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
 
@@ -100,6 +101,9 @@ def lib():
                               _dp, C.POINTER(om_stats)]
     L.om_power_density.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, _dp, C.c_int, C.POINTER(om_conv), _dp,
                                    _ip, _up, C.POINTER(om_stats)]
+    L.om_power_density_surface.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, _dp, C.c_int, C.POINTER(om_conv),
+                                           _dp, _ip, _up, C.POINTER(om_stats)]
+    L.om_total_power.argtypes = [C.c_void_p, C.c_int, C.POINTER(om_conv), _dp, _dp, _ip, C.POINTER(om_stats)]
     _vp = C.c_void_p  # device addresses
     L.om_flux_dev.argtypes = [C.c_void_p, C.c_int, _vp, C.c_long, _vp, C.c_long, _dp, C.POINTER(om_conv), _vp, _vp,
                               _vp, _vp, C.POINTER(om_stats)]
@@ -251,6 +255,29 @@ class Device:
         self._chk(self.L.om_power_density(self.h, int(particle), _ptr(obs), n, _ptr(nv), int(directional),
                                           C.byref(conv), _ptr(out), _ptr(lev, _ip), _ptr(cv, _up), C.byref(st)))
         return out, lev, cv, st.as_dict()
+
+    def power_density_surface(self, particle, obs_xyz, normals, directional, conv, out=None):
+        """Power density at points with a unit normal each (normals[n, 3]) or
+        without normals (normals None)."""
+        obs = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(-1, 3)
+        n = len(obs)
+        out = np.zeros(n) if out is None else out
+        lev = np.zeros(n, dtype=np.int32)
+        cv = np.zeros(n, dtype=np.uint8)
+        nv = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64).reshape(n, 3)
+        st = om_stats()
+        self._chk(self.L.om_power_density_surface(self.h, int(particle), _ptr(obs), n, _ptr(nv), int(directional),
+                                                  C.byref(conv), _ptr(out), _ptr(lev, _ip), _ptr(cv, _up),
+                                                  C.byref(st)))
+        return out, lev, cv, st.as_dict()
+
+    def total_power(self, particle, conv):
+        """(power [W], precision, level, stats); level -1: not converged."""
+        pw, pr, lv = C.c_double(0), C.c_double(0), C.c_int(0)
+        st = om_stats()
+        self._chk(self.L.om_total_power(self.h, int(particle), C.byref(conv), C.byref(pw), C.byref(pr), C.byref(lv),
+                                        C.byref(st)))
+        return pw.value, pr.value, lv.value, st.as_dict()
 
     # ------------------------------------------------------------ device-resident variants
     @staticmethod

@@ -31,6 +31,8 @@ hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, h
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
+int om_tp_blocks(long npts);
+hipError_t om_launch_total_power_level(const om::TpArgs* A, hipStream_t s);
 hipError_t om_launch_spline_user(const double* t, double* p, double* ypp, double* u, long n, int construct,
                                  hipStream_t s);
 hipError_t om_launch_spline(const om::TrajDesc* desc, int np, const double* t, const double* p, double* ypp,
@@ -214,6 +216,7 @@ struct om_context {
   DevBuf<unsigned char> d_outconv;
   DevBuf<unsigned long long> d_stats;
   DevBuf<double> d_bd;
+  DevBuf<double> d_normals, d_tp, d_tp_partial;
   std::vector<om::FieldDev> fields_host;
   int nfields = 0;
   long total_knots = 0;
@@ -297,6 +300,7 @@ struct om_context {
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
     d_list.release(); d_border.release();
+    d_normals.release(); d_tp.release(); d_tp_partial.release();
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) (void) hipStreamDestroy(stream);
@@ -1019,16 +1023,18 @@ int om_spectrum_dev(om_context* ctx, int particle, const double* obs, const doub
 }
 
 namespace {
+// normal: one host normal for all points; normals: one per point (host, or
+// device with dev_io), used when normal is null; neither: a surface without normals.
 int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normal, int directional,
                   const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv, om_stats* stats,
-                  bool dev_io) {
+                  bool dev_io, const double* normals = nullptr) {
   return guarded(ctx, [&] {
     Timer wall;
     ctx->timer.used = 0;
     check_particle(ctx, ip);
     if (!conv) throw std::invalid_argument("conv is NULL");
     if (n_obs < 1) throw std::length_error("no points specified");
-    if (!obs_xyz || !normal || !out_value) throw std::invalid_argument("missing inputs or outputs");
+    if (!obs_xyz || !out_value) throw std::invalid_argument("missing inputs or outputs");
     int stop_mem, stop_ext;
     level_stops(*conv, stop_mem, stop_ext);
     if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
@@ -1044,7 +1050,16 @@ int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, co
     state_for(ctx, n_obs, false, A.st);
     A.npts = n_obs;
     A.obs = ctx->d_obs.p;
-    for (int k = 0; k < 3; ++k) A.normal[k] = normal[k];
+    A.has_normal = normal || normals ? 1 : 0;
+    if (normal) {
+      for (int k = 0; k < 3; ++k) A.normal[k] = normal[k];
+    } else if (normals) {
+      ctx->d_normals.ensure(3 * n_obs);
+      check(hipMemcpyAsync(ctx->d_normals.p, normals, 3 * n_obs * sizeof(double),
+                           dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream),
+            "normals");
+      A.normals = ctx->d_normals.p;
+    }
     A.directional = directional;
     A.precision = conv->precision;
     A.two_over_gamma = 2. / (ctx->parts[ip].gamma);
@@ -1089,8 +1104,85 @@ int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, co
 int om_power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normal,
                      int directional, const om_conv* conv, double* out_value, int* out_level,
                      unsigned char* out_conv, om_stats* stats) {
+  if (!normal) {
+    if (ctx) ctx->err = "normal is NULL";
+    return 1;
+  }
   return power_density(ctx, ip, obs_xyz, n_obs, normal, directional, conv, out_value, out_level, out_conv, stats,
                        false);
+}
+
+int om_power_density_surface(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normals,
+                             int directional, const om_conv* conv, double* out_value, int* out_level,
+                             unsigned char* out_conv, om_stats* stats) {
+  return power_density(ctx, ip, obs_xyz, n_obs, nullptr, directional, conv, out_value, out_level, out_conv, stats,
+                       false, normals);
+}
+
+int om_total_power(om_context* ctx, int ip, const om_conv* conv, double* out_power, double* out_precision,
+                   int* out_level, om_stats* stats) {
+  return guarded(ctx, [&] {
+    Timer wall;
+    ctx->timer.used = 0;
+    check_particle(ctx, ip);
+    if (!conv || !out_power) throw std::invalid_argument("missing inputs or outputs");
+    int stop_mem, stop_ext;
+    level_stops(*conv, stop_mem, stop_ext);
+    if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
+    const om_particle& P = ctx->parts[ip];
+    ctx->d_tp.ensure(om::kMaxLevel + 1);
+    // src/OSCARSSR.cc:3757-3822: one running sum over levels; levels are summed
+    // in chunks on the device and the stopping rule runs here on their sums
+    double Sum = 0, LastSum = -1, Result_Precision = -1;
+    int Result_Level = -1, La = 0;
+    double evals = 0;
+    while (La <= stop_ext && Result_Level == -1) {
+      int Lb = La == 0 ? (stop_ext < kFirstLevels ? stop_ext : kFirstLevels) : La;
+      ensure_levels(ctx, ip, Lb, stop_mem);
+      ensure_beta_diff(ctx, ip, Lb);
+      ctx->d_tp_partial.ensure(om_tp_blocks(1L << Lb));
+      for (int L = La; L <= Lb; ++L) {
+        om::TpArgs T;
+        T.rows = ctx->levels[ip].rows[L];
+        T.npts = 1L << L;
+        T.partial = ctx->d_tp_partial.p;
+        T.out = ctx->d_tp.p + L;
+        ctx->timer.begin(ctx->stream);
+        check(om_launch_total_power_level(&T, ctx->stream), "total power");
+        ctx->timer.end(ctx->stream);
+      }
+      double h[om::kMaxLevel + 1];
+      read_back(ctx, h + La, ctx->d_tp.p + La, (Lb - La + 1) * sizeof(double));
+      for (int L = La; L <= Lb; ++L) {
+        Sum += h[L];
+        evals += (double) (1L << L);
+        double ThisSum = Sum * dt_inclusive(ctx->tstart[ip], ctx->tstop[ip], L);
+        Result_Precision = std::fabs(ThisSum - LastSum) / LastSum;
+        if (L > 8 && Result_Precision < conv->precision && ctx->levels[ip].beta_diff[L] < 2. / P.gamma) {
+          Result_Level = L;
+          LastSum = ThisSum;
+          break;
+        } else if (L > 8 && ThisSum == LastSum) {
+          Result_Level = L;
+          Result_Precision = 0;
+          break;
+        }
+        LastSum = ThisSum;
+      }
+      La = Lb + 1;
+    }
+    *out_power = LastSum * std::fabs(P.q * P.current) / (6 * kPi * kEps0 * kC);
+    if (out_precision) *out_precision = Result_Precision;
+    if (out_level) *out_level = Result_Level;
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->evals = evals;
+      stats->kernel_ms = ctx->timer.take();
+      stats->total_ms = wall.ms();
+      stats->max_level = Result_Level;
+      stats->n_unconverged = Result_Level == -1 ? 1 : 0;
+    }
+  });
 }
 
 int om_power_density_dev(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* normal,

@@ -46,10 +46,20 @@ struct PdArgs {
   int La, Lb;
   long npts;
   const double* obs;
-  double normal[3];
+  double normal[3];         // one normal for every point (rectangles) ...
+  const double* normals;    // ... or one per point (surfaces of points), when not null
+  int has_normal;           // 0: a surface without normals (N1.normal taken as 1)
   int directional;
   double precision;
   double two_over_gamma;
+};
+
+// Total power: per-level sums of the integrand over the level's samples.
+struct TpArgs {
+  const double* rows;   // 2^L samples of (t, x, b, a/c)
+  long npts;
+  double* partial;      // one sum per workgroup
+  double* out;          // the level's sum
 };
 
 struct FinArgs {

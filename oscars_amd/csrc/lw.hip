@@ -357,7 +357,8 @@ __global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
   bool valid = p < A.npts;
   long pc = valid ? p : 0;
   V3 obs = v3(A.obs[3 * pc], A.obs[3 * pc + 1], A.obs[3 * pc + 2]);
-  V3 nrm = v3(A.normal[0], A.normal[1], A.normal[2]);
+  V3 nrm = A.normals ? v3(A.normals[3 * pc], A.normals[3 * pc + 1], A.normals[3 * pc + 2])
+                    : v3(A.normal[0], A.normal[1], A.normal[2]);
   int lev = A.st.level[pc];
   bool active = valid && !A.st.conv[pc] && lev == A.La - 1;
   double sum = A.st.sum[6 * pc];
@@ -376,7 +377,8 @@ __global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
       double r2 = mag2(R);
       double m = sqrt(r2);
       V3 N1 = v3(R.x / m, R.y / m, R.z / m);
-      double ndn = dot(N1, nrm);
+      // src/OSCARSSR.cc:2908: a surface without normals takes N1.normal as 1
+      double ndn = A.has_normal ? dot(N1, nrm) : 1.0;
       V3 Num = cross(N1, cross(sub(N1, B), Ac));
       double x = 1 - dot(B, N1);
       double x2 = x * x;
@@ -408,6 +410,49 @@ __global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
     A.st.prec[pc] = prec;
     A.st.level[pc] = lev;
   }
+}
+
+// ---------------------------------------------------------------- total power
+// src/OSCARSSR.cc:3785-3802: the integrand (|A/c|^2 - |B x A/c|^2) / (1 - B.B)^3
+// summed over one level's samples.  Each workgroup sums a contiguous slice
+// (strided over its lanes, then a fixed tree), and tp_final_kernel adds the
+// slices in order: the result does not depend on timing.
+constexpr int kTpThreads = 256;
+
+__global__ void __launch_bounds__(kTpThreads) tp_level_kernel(TpArgs A) {
+  __shared__ double red[kTpThreads];
+  long per = (A.npts + gridDim.x - 1) / gridDim.x;
+  long i0 = (long) blockIdx.x * per, i1 = i0 + per < A.npts ? i0 + per : A.npts;
+  double s = 0;
+  for (long i = i0 + threadIdx.x; i < i1; i += kTpThreads) {
+    const double* r = A.rows + 10 * i;
+    V3 B = v3(r[4], r[5], r[6]);
+    V3 Ac = v3(r[7], r[8], r[9]);
+    double x = 1 - dot(B, B);
+    double g6 = 1. / (x * x * x);
+    s += (mag2(Ac) - mag2(cross(B, Ac))) * g6;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = kTpThreads / 2; k > 0; k >>= 1) {
+    if ((int) threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) A.partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kTpThreads) tp_final_kernel(const double* __restrict__ partial, int n,
+                                                              double* __restrict__ out) {
+  __shared__ double red[kTpThreads];
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += kTpThreads) s += partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = kTpThreads / 2; k > 0; k >>= 1) {
+    if ((int) threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
 }
 
 // ---------------------------------------------------------------- finalize
@@ -825,6 +870,20 @@ extern "C" hipError_t om_launch_tile_list(const om::State* st, long n_obs, long 
 extern "C" hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s) {
   long blocks = (A->npts + 255) / 256;
   hipLaunchKernelGGL(om::pd_point_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *A);
+  return hipGetLastError();
+}
+
+// partial must hold om_tp_blocks(npts) doubles
+extern "C" int om_tp_blocks(long npts) {
+  long b = (npts + 4L * om::kTpThreads - 1) / (4L * om::kTpThreads);
+  return (int) (b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+extern "C" hipError_t om_launch_total_power_level(const om::TpArgs* A, hipStream_t s) {
+  int blocks = om_tp_blocks(A->npts);
+  hipLaunchKernelGGL(om::tp_level_kernel, dim3((unsigned) blocks), dim3(om::kTpThreads), 0, s, *A);
+  hipLaunchKernelGGL(om::tp_final_kernel, dim3(1), dim3(om::kTpThreads), 0, s, (const double*) A->partial, blocks,
+                     A->out);
   return hipGetLastError();
 }
 

@@ -188,15 +188,16 @@ def _xyz(points):
 
 
 def write_points_text(path, points, values, dim=2):
-    """T3DScalarContainer::WriteToFileText: 'X Y V' (dim 2) or 'X Y Z V' (dim 3) per line."""
-    if dim not in (2, 3):
-        raise ValueError("incorrect dimensions")
+    """T3DScalarContainer::WriteToFileText: 'X Y V' (dim 2) or 'X Y Z V' (dim 3)
+    per line; any other dim is an error once there is a point to write."""
     try:
         f = open(path, "w")
     except OSError:
         raise ValueError("cannot open output file")
     with f:
         for p, v in zip(_xyz(points), values):
+            if dim not in (2, 3):
+                raise ValueError("incorrect dimensions")
             if dim == 2:
                 f.write(_fmt(p[0]) + " " + _fmt(p[1]) + " " + _fmt(v) + "\n")
             else:
@@ -206,6 +207,10 @@ def write_points_text(path, points, values, dim=2):
 def write_points_binary(path, points, values, dim=2):
     """T3DScalarContainer::WriteToFileBinary: float32 (X, Y, V) or (X, Y, Z, V) records."""
     if dim not in (2, 3):
+        try:
+            open(path, "wb").close()
+        except OSError:
+            raise ValueError("cannot open out file named: " + path)
         raise ValueError("incorrect dimensions")
     p = np.asarray(_xyz(points), dtype=np.float64).reshape(-1, 3)
     a = np.empty((len(p), dim + 1), dtype=np.float32)

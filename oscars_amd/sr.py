@@ -52,6 +52,28 @@ def _warn(msg):
     sys.stderr.write(msg)
 
 
+def surface_points(points, normal=0, rotations=None, translation=None):
+    """Observation points and unit normals of calculate_power_density: each
+    [[x, y, z], [nx, ny, nz]] rotated, the normal reversed for normal = -1,
+    the point translated (src/OSCARSSR_Python.cc:6208-6240, src/TSurfacePoint.cc:121)."""
+    rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+    tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+    obs, nrm = [], []
+    for p in points:
+        if len(p) != 2:
+            raise ValueError("Incorrect format in 'points'")
+        try:
+            X, N = _v3(p[0], "points"), _v3(p[1], "points")
+        except ValueError:
+            raise ValueError("Incorrect format in 'points': Point or Normal does not have 3 elements")
+        X, N = G.rot_xyz(X, rot), G.rot_xyz(N, rot)
+        if normal == -1:
+            N = [-N[0], -N[1], -N[2]]
+        obs.append([X[k] + tr[k] for k in range(3)])
+        nrm.append(G.unit(N))
+    return obs, nrm
+
+
 class sr:
     """Simulation object: fields, one particle beam, trajectory window, and the
     spectrum / flux / power-density calculations."""
@@ -387,7 +409,7 @@ class sr:
     @staticmethod
     def _quantity(q, pd=False):
         Q = q.upper()
-        if Q == ("POWER DENSITY" if pd else "FLUX"):
+        if Q in (("POWER DENSITY", "POWERDENSITY") if pd else ("FLUX",)):
             return 0
         if Q == "PRECISION":
             return 1
@@ -631,6 +653,89 @@ class sr:
         if bofile != "":
             IO.write_points_binary(bofile, cont, v, dim)
         return [[list(cont[i]), float(v[i])] for i in range(len(cont))]
+
+    def _pd_points(self, obs, normals, directional, nparticles, precision, max_level, max_level_extended, q):
+        obs_a = np.asarray(obs, dtype=np.float64).reshape(-1, 3)
+
+        def run(dev, ip, w, acc, out=None):
+            conv = dev.conv(precision, max_level, max_level_extended, q, "plain", w, acc)
+            v, lev, cv, st = dev.power_density_surface(ip, obs_a, normals, directional, conv, out=out)
+            self.last_stats = st
+            self._last_levels = lev
+            self._last_conv = cv
+            return v
+
+        return self._run_particles(nparticles, run)
+
+    def calculate_power_density(self, points, normal=0, rotations=None, translation=None, nparticles=0, gpu=-1,
+                                ngpu=None, nthreads=0, precision=0.01, max_level=-2, max_level_extended=0,
+                                quantity="power density", ofile="", bofile=""):
+        """Power density [W/mm^2] at [[x, y, z], [nx, ny, nz]] points, rotated
+        then translated (normals rotated only); returns [[[x, y, z], pd], ...]
+        in absolute coordinates (src/OSCARSSR_Python.cc:6105-6380)."""
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        obs, nrm = surface_points(points, normal, rotations, translation)
+        self._check_common(nparticles, nthreads, 0 if gpu == 0 else gpu)
+        q = self._quantity(quantity, pd=True)
+        v = self._pd_points(obs, np.array(nrm).reshape(-1, 3), normal != 0, nparticles, precision, max_level,
+                            max_level_extended, q)
+        if not self._last_conv.all():
+            _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
+        if ofile != "":
+            IO.write_points_text(ofile, obs, v, 3)
+        if bofile != "":
+            IO.write_points_binary(bofile, obs, v, 3)
+        return [[list(obs[i]), float(v[i])] for i in range(len(obs))]
+
+    def calculate_power_density_line(self, x1, x2, npoints=0, ofile="", bofile="", normal=0, nparticles=0, gpu=-1,
+                                     nthreads=0, precision=0.01, max_level=-2, max_level_extended=0, dim=1):
+        """Power density at npoints evenly spaced from x1 to x2 (a surface
+        without normals); dim 1 returns the signed distance from the middle,
+        dim 3 the points (src/OSCARSSR_Python.cc:7355-7560)."""
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        if dim not in (1, 3):
+            raise ValueError("'dim' must be 1 or 3")
+        if npoints < 2:
+            raise ValueError("'npoints' must be >= 2")
+        a = _v3(x1, "x1") if len(x1) != 0 else [0.0, 0.0, 0.0]
+        b = _v3(x2, "x2") if len(x2) != 0 else [0.0, 0.0, 0.0]
+        step = [(b[k] - a[k]) / (npoints - 1) for k in range(3)]
+        obs = [[a[k] + i * step[k] for k in range(3)] for i in range(npoints)]
+        self._check_common(nparticles, nthreads, 0 if gpu == 0 else gpu)
+        v = self._pd_points(obs, None, normal != 0, nparticles, precision, max_level, max_level_extended, 0)
+        if dim == 3:
+            pos = obs
+        else:
+            # src/OSCARSSR.cc:2712-2722: signed distance from the middle of the line
+            c = [a[k] + (obs[-1][k] - a[k]) / 2 for k in range(3)]
+            pos = []
+            for i in range(npoints):
+                d = [obs[i][k] - c[k] for k in range(3)]
+                pos.append([(-1. if i < npoints // 2 else 1.) * math.sqrt(G.mag2(d)), 0.0, 0.0])
+        if ofile != "":
+            IO.write_points_text(ofile, pos, v, dim)
+        if bofile != "":
+            IO.write_points_binary(bofile, pos, v, dim)
+        return [[list(pos[i]), float(v[i])] for i in range(npoints)]
+
+    def calculate_total_power(self, precision=0.01, max_level=-2, max_level_extended=0, quantity="power"):
+        """Total radiated power [W] of the current particle, or the precision or
+        level reached (src/OSCARSSR_Python.cc:5974-6060)."""
+        Q = quantity.upper()
+        if Q not in ("POWER", "PRECISION", "LEVEL"):
+            raise ValueError("'quantity' must be: 'power', 'precision', 'level', or blank")
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        self._ensure_trajectory()
+        dev = self._device()
+        pw, pr, lv, st = dev.total_power(0, dev.conv(precision, max_level, max_level_extended, 0, "plain"))
+        self.last_stats = st
+        if lv == -1:
+            raise ValueError("total power calculation did not converge.  Try increasing max_level or decreasing "
+                             "precision")
+        return {"POWER": pw, "PRECISION": pr, "LEVEL": float(lv)}[Q]
 
     # ------------------------------------------------------------ averaging of saved results
     @staticmethod

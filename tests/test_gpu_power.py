@@ -1,0 +1,79 @@
+"""Total power and power density on surfaces of points on the GPU against the
+oracle (pinned to the reference in tests/test_oracle_vs_ref.py).  Bars: total
+power 1e-12 relative with the same stopping level (the device sums each
+level's samples as a tree, the reference one by one); power density 1e-9
+relative with a floor at 1e-6 of the maximum, levels identical."""
+import numpy as np
+import pytest
+
+from tests.common import epu, planar, rel_err
+
+gpu = pytest.mark.gpu
+
+
+def _pair(setup, npoints):
+    from oracle.pyoracle import Oracle
+    from oscars_amd.sr import sr
+    o = Oracle(nthreads=8)
+    setup(o, npoints=npoints)
+    o.calculate_trajectory()
+    s = sr()
+    setup(s, npoints=npoints)
+    return o, s
+
+
+@gpu
+@pytest.mark.parametrize("setup", [planar, epu])
+def test_total_power(setup):
+    o, s = _pair(setup, 4001)
+    p, prec, lev = o.total_power()
+    gp = s.calculate_total_power()
+    gl = s.calculate_total_power(quantity="level")
+    gr = s.calculate_total_power(quantity="precision")
+    print("total power %s: %.12g W (oracle %.12g), level %d, precision %.3e (oracle %.3e)"
+          % (setup.__name__, gp, p, gl, gr, prec))
+    assert gl == lev
+    assert abs(gp - p) / abs(p) < 1e-12
+    assert abs(gr - prec) <= 1e-9 * abs(prec) + 1e-15
+    with pytest.raises(ValueError, match="did not converge"):
+        s.calculate_total_power(max_level=5)
+    with pytest.raises(ValueError, match="'quantity' must be"):
+        s.calculate_total_power(quantity="flux")
+
+
+POINTS = [[[x, y, 20.0], [0.1 * x, 0.2, 1.0]] for x in np.linspace(-0.03, 0.03, 7) for y in (-0.01, 0.0, 0.004)] + \
+    [[[0.0, 0.0, 20.0], [0.0, 0.0, -1.0]]]
+
+
+@gpu
+@pytest.mark.parametrize("normal", [0, 1, -1])
+def test_power_density_points(normal):
+    from oscars_amd.sr import surface_points
+    o, s = _pair(epu, 8001)
+    kw = dict(rotations=[0.01, -0.02, 0.3], translation=[0.001, 0, 0.5])
+    res = s.calculate_power_density(POINTS, normal=normal, **kw)
+    obs, nrm = surface_points(POINTS, normal, **kw)
+    assert [r[0] for r in res] == obs
+    v = np.array([r[1] for r in res])
+    ov, olev, _ = o.power_density_surface(np.array(obs), np.array(nrm), directional=normal != 0,
+                                          carry_quirk=False)
+    e = rel_err(v, ov, 1e-6) if np.any(ov) else float(np.max(np.abs(v)))
+    print("power density on points, normal %d: %.3e" % (normal, e))
+    assert e < 1e-9
+    assert np.array_equal(s._last_levels, olev)
+
+
+@gpu
+@pytest.mark.parametrize("dim", [1, 3])
+def test_power_density_line(dim):
+    o, s = _pair(planar, 4001)
+    res = s.calculate_power_density_line(x1=[-0.01, 0.0, 30], x2=[0.012, 0.002, 30], npoints=41, dim=dim)
+    obs = [[-0.01 + i * ((0.012 - -0.01) / 40), 0.0 + i * (0.002 / 40), 30.0 + i * ((30 - 30) / 40)]
+           for i in range(41)]
+    ov, olev, _ = o.power_density_surface(np.array(obs), None, carry_quirk=False)
+    v = np.array([r[1] for r in res])
+    e = rel_err(v, ov, 1e-6)
+    print("power density on a line (dim %d): %.3e" % (dim, e))
+    assert e < 1e-9
+    if dim == 3:
+        assert [r[0] for r in res] == obs

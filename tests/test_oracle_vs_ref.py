@@ -136,3 +136,52 @@ def test_grid_file(tmp_path):
     e, v, _, _, _ = o.calculate_spectrum([0, 0, 30], npoints=9, energy_range_eV=[300, 900])
     re, rv, _ = r.calculate_spectrum([0, 0, 30], npoints=9, energy_range_eV=[300, 900])
     assert same(v, rv)
+
+
+PD_POINTS = [[[x, y, 20.0], [0.1 * x, 0.2, 1.0]] for x in (-0.03, 0.0, 0.012) for y in (-0.01, 0.004)] + \
+    [[[0.0, 0.0, 20.0], [0.0, 0.0, -1.0]]]
+
+
+@pytest.mark.parametrize("normal", [0, 1, -1])
+def test_power_density_points(normal):
+    """calculate_power_density on points with normals: the product's point set-up
+    (rotate, reverse, translate, unit normal) and the oracle's per-point-normal
+    sum against the reference, bit for bit."""
+    from oscars_amd.sr import surface_points
+    o, r = pair(epu, npoints=8001)
+    kw = dict(rotations=[0.01, -0.02, 0.3], translation=[0.001, 0, 0.5])
+    xyz, rv, _ = r.calculate_power_density(PD_POINTS, normal=normal, **kw)
+    obs, nrm = surface_points(PD_POINTS, normal, **kw)
+    assert same(obs, xyz)
+    ov, _, _ = o.power_density_surface(np.array(obs), np.array(nrm), directional=normal != 0)
+    assert same(ov, rv)
+
+
+@pytest.mark.parametrize("dim", [1, 3])
+def test_power_density_line(dim):
+    """A line of points has no normals: N1.normal is taken as 1."""
+    o, r = pair(planar, npoints=4001)
+    xyz, rv, _ = r.calculate_power_density_line([-0.01, 0.0, 30], [0.012, 0.002, 30], 12, dim=dim)
+    obs = [[-0.01 + i * ((0.012 - -0.01) / 11), 0.0 + i * (0.002 / 11), 30.0 + i * ((30 - 30) / 11)]
+           for i in range(12)]
+    ov, _, _ = o.power_density_surface(np.array(obs), None)
+    assert same(ov, rv)
+    if dim == 3:
+        assert same(xyz, obs)
+    else:
+        c = np.array(obs[0]) + (np.array(obs[-1]) - np.array(obs[0])) / 2
+        d = [(-1. if i < 6 else 1.) * math.sqrt(float(np.sum((np.array(obs[i]) - c) ** 2))) for i in range(12)]
+        assert np.allclose(xyz[:, 0], d, rtol=1e-15, atol=0) and not xyz[:, 1:].any()
+
+
+@pytest.mark.parametrize("setup", [planar, epu])
+def test_total_power(setup):
+    o, r = pair(setup, npoints=4001)
+    p, prec, lev = o.total_power()
+    assert p == r.calculate_total_power()
+    assert prec == r.calculate_total_power(quantity="precision")
+    assert float(lev) == r.calculate_total_power(quantity="level")
+    # coarse max_level: the reference raises, the oracle reports level -1
+    assert o.total_power(max_level=5)[2] == -1
+    with pytest.raises(ValueError, match="did not converge"):
+        r.calculate_total_power(max_level=5)
