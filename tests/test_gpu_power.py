@@ -1,8 +1,14 @@
 """Total power and power density on surfaces of points on the GPU against the
-oracle (pinned to the reference in tests/test_oracle_vs_ref.py).  Bars: total
-power 1e-12 relative with the same stopping level (the device sums each
-level's samples as a tree, the reference one by one); power density 1e-9
-relative with a floor at 1e-6 of the maximum, levels identical."""
+oracle (pinned to the reference in tests/test_oracle_vs_ref.py).
+
+Total power: the integrand carries 1/(1 - B.B)^3 with 1 - B.B ~ 1/gamma^2
+~ 3e-8 at 3 GeV, so a last-bit difference in beta becomes ~1e-8 of a sample.
+Against the oracle on its own trajectory the bar is therefore 1e-9 (the
+device's RK4 differs from glibc's in the last bits of sin); against the oracle
+summing the device's own trajectory it is 1e-13 (left: the tree order of the
+device's level sums and x*x*x for pow(x, 3)).  The stopping level must agree.
+Power density: 1e-9 relative with a floor at 1e-6 of the maximum, levels
+identical."""
 import numpy as np
 import pytest
 
@@ -33,8 +39,14 @@ def test_total_power(setup):
     print("total power %s: %.12g W (oracle %.12g), level %d, precision %.3e (oracle %.3e)"
           % (setup.__name__, gp, p, gl, gr, prec))
     assert gl == lev
-    assert abs(gp - p) / abs(p) < 1e-12
-    assert abs(gr - prec) <= 1e-9 * abs(prec) + 1e-15
+    assert abs(gp - p) / abs(p) < 1e-9
+    assert abs(gr - prec) <= 1e-6 * abs(prec) + 1e-15
+    from oracle.pyoracle import Trajectory
+    t, pk, ypp = s._device().trajectory(0)
+    o.traj = Trajectory(t, pk, ypp)
+    p2, _, lev2 = o.total_power()
+    print("  on the device's trajectory: oracle %.15g, rel diff %.3e" % (p2, abs(gp - p2) / abs(p2)))
+    assert lev2 == gl and abs(gp - p2) / abs(p2) < 1e-13
     with pytest.raises(ValueError, match="did not converge"):
         s.calculate_total_power(max_level=5)
     with pytest.raises(ValueError, match="'quantity' must be"):
