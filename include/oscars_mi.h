@@ -124,6 +124,13 @@ int om_trajectory_set(om_context* ctx, const om_particle* particle, long n, cons
 int om_trajectory_get(om_context* ctx, int i, long* n, double* t, double* p9, double* ypp9);
 /* Level samples of trajectory i (2^level rows of t,x,y,z,bx,by,bz,ax,ay,az). */
 int om_level_get(om_context* ctx, int i, int level, int extended, double* rows);
+/* Electric field in the time domain at obs[3] from trajectory i, one value per
+ * knot: calculate_efield_vs_time (src/OSCARSSR_Python.cc:9048-9110,
+ * src/OSCARSSR.cc:4471-4512).  out_t[k] = k dt + |obs - x_k| / c [s] (dt: the
+ * integration step; 0 for a trajectory given by om_trajectory_set), out_e3[3 k]
+ * = E [V/m].  *n in = capacity, out = knot count; nothing is written when the
+ * capacity is short. */
+int om_efield_time(om_context* ctx, int i, const double* obs, long* n, double* out_t, double* out_e3);
 /* Status of trajectory i: 0 ok, 1 |beta| >= 1 met during integration. */
 int om_trajectory_status(om_context* ctx, int i);
 

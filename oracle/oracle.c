@@ -1090,6 +1090,34 @@ int orc_total_power(const orc_traj* T, const orc_particle* P, const orc_opts* o,
   return Result_Level == -1 ? 1 : 0;
 }
 
+/* Electric field in the time domain at Obs, one value per knot
+ * (src/OSCARSSR.cc:4471-4512); dt is the trajectory's step. */
+void orc_efield_time(const orc_traj* T, const orc_particle* P, const double* Obs, double dt, double* out_t,
+                     double* out_e) {
+  double C0 = P->q / (k_fourpi() * K_EPS0);
+  for (long iT = 0; iT != T->n; ++iT) {
+    const double* X = T->p + 9 * iT;
+    const double* B = X + 3;
+    const double* A = X + 6;
+    double R[3] = {Obs[0] - X[0], Obs[1] - X[1], Obs[2] - X[2]};
+    double N[3];
+    v_unit(R, N);
+    double D = sqrt(v_mag2(R));
+    double Mult = C0 * pow(1.0 / (1.0 - v_dot(N, B)), 3);
+    double NmB[3] = {N[0] - B[0], N[1] - B[1], N[2] - B[2]};
+    double r2 = v_mag2(R), b2 = v_mag2(B);
+    double c1[3], c2[3];
+    v_cross(NmB, A, c1);
+    v_cross(N, c1, c2);
+    for (int k = 0; k < 3; ++k) {
+      double Near = ((1.0 - b2) * NmB[k]) / r2;
+      double Far = ((1.0 / K_C) * c2[k]) / D;
+      out_e[3 * iT + k] = Mult * (Near + Far);
+    }
+    out_t[iT] = (double) iT * dt + D / K_C;
+  }
+}
+
 double orc_evals_from_levels(long n, const int* levels) {
   double s = 0;
   for (long i = 0; i < n; ++i) {

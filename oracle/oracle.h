@@ -128,6 +128,9 @@ int orc_power_density_surface(const orc_traj* T, const orc_particle* P, const do
 int orc_total_power(const orc_traj* T, const orc_particle* P, const orc_opts* o, double* out_power,
                     double* out_precision, int* out_level);
 
+void orc_efield_time(const orc_traj* T, const orc_particle* P, const double* Obs, double dt, double* out_t,
+                     double* out_e);
+
 /* Count of integrand evaluations sum_p (2^(L_p+1) - 1) given per-point levels. */
 double orc_evals_from_levels(long n, const int* levels);
 

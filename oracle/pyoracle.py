@@ -111,6 +111,8 @@ def oracle_lib():
         L.orc_power_density_surface.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_long, C.c_int, C.c_void_p,
                                                 _dp, _ip, _up]
         L.orc_total_power.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _ip]
+        L.orc_efield_time.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_double, _dp, _dp]
+        L.orc_efield_time.restype = None
         _oracle = L
     return _oracle
 
@@ -374,6 +376,16 @@ class Oracle:
                                          conv.ctypes.data_as(_up))
         return out, lev, conv
 
+    def efield_time(self, obs, dt):
+        """(t[n], E[n, 3]) at obs from the current trajectory's knots, step dt."""
+        T = self._traj()
+        n = len(T.t)
+        t = np.zeros(n)
+        e = np.zeros((n, 3))
+        self.L.orc_efield_time(C.byref(T.c()), C.byref(self.P), _v3(obs), float(dt), t.ctypes.data_as(_dp),
+                               e.ctypes.data_as(_dp))
+        return t, e
+
     def total_power(self, precision=0.01, max_level=-2, max_level_extended=0):
         """(power [W], precision, level); level -1 when not converged."""
         T = self._traj()
@@ -472,6 +484,7 @@ def ref_lib():
         L.ref_power_density_line.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                              C.c_int, C.c_int, _dp, _dp, _ip]
         L.ref_total_power.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp]
+        L.ref_efield_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_long), _dp, _dp]
         L.ref_read_trajectory.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]
         L.ref_set_trajectory_points.argtypes = [C.c_void_p, C.c_long, _dp, C.c_int, C.c_int]
         L.ref_random_normal.argtypes = [C.c_void_p]
@@ -664,6 +677,15 @@ class RefSim:
                                                 int(max_level_extended), xyz.ctypes.data_as(_dp),
                                                 v.ctypes.data_as(_dp), C.byref(conv)))
         return xyz, v, bool(conv.value)
+
+    def calculate_efield_vs_time(self, obs):
+        """(t[n], E[n, 3]) as the reference computes them."""
+        n = C.c_long(0)
+        self._chk(self.L.ref_efield_time(self.h, _v3(obs), C.byref(n), None, None))
+        t = np.zeros(n.value)
+        e = np.zeros((n.value, 3))
+        self._chk(self.L.ref_efield_time(self.h, _v3(obs), C.byref(n), t.ctypes.data_as(_dp), e.ctypes.data_as(_dp)))
+        return t, e
 
     def calculate_total_power(self, precision=0.01, max_level=-2, max_level_extended=0, quantity="power"):
         q = {"POWER": 0, "PRECISION": 1, "LEVEL": 2}[quantity.upper()]

@@ -446,6 +446,27 @@ int ref_power_density_line(void* h, double const* x1, double const* x2, int npoi
   } catch (std::exception const& e) { return fail(e); }
 }
 
+// calculate_efield_vs_time (src/OSCARSSR_Python.cc:9048-9110): *n in = capacity,
+// out = count; out_t[k], out_e[3 k].
+int ref_efield_time(void* h, double const* obs, long* n, double* out_t, double* out_e) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    T3DScalarContainer XYZT;
+    s->CalculateElectricFieldTimeDomain(v3(obs), XYZT);
+    long cap = *n;
+    *n = (long) XYZT.GetNPoints();
+    if (cap < *n) return 0;
+    for (long i = 0; i < *n; ++i) {
+      T3DScalar const& p = XYZT.GetPoint(i);
+      out_t[i] = p.GetV();
+      out_e[3 * i + 0] = p.GetX().GetX();
+      out_e[3 * i + 1] = p.GetX().GetY();
+      out_e[3 * i + 2] = p.GetX().GetZ();
+    }
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
 // calculate_total_power (src/OSCARSSR_Python.cc:5974-6060)
 int ref_total_power(void* h, double precision, int max_level, int max_level_extended, int quantity, double* out) {
   try {

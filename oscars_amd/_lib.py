@@ -59,7 +59,7 @@ class om_stats(C.Structure):
 
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
            "om_trajectories", "om_trajectory_set", "om_trajectory_get", "om_power_density_surface",
-           "om_total_power", "om_level_get", "om_trajectory_status", "om_flux",
+           "om_total_power", "om_efield_time", "om_level_get", "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
 
@@ -104,6 +104,7 @@ def lib():
     L.om_power_density_surface.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, _dp, C.c_int, C.POINTER(om_conv),
                                            _dp, _ip, _up, C.POINTER(om_stats)]
     L.om_total_power.argtypes = [C.c_void_p, C.c_int, C.POINTER(om_conv), _dp, _dp, _ip, C.POINTER(om_stats)]
+    L.om_efield_time.argtypes = [C.c_void_p, C.c_int, _dp, _lp, _dp, _dp]
     _vp = C.c_void_p  # device addresses
     L.om_flux_dev.argtypes = [C.c_void_p, C.c_int, _vp, C.c_long, _vp, C.c_long, _dp, C.POINTER(om_conv), _vp, _vp,
                               _vp, _vp, C.POINTER(om_stats)]
@@ -270,6 +271,16 @@ class Device:
                                                   C.byref(conv), _ptr(out), _ptr(lev, _ip), _ptr(cv, _up),
                                                   C.byref(st)))
         return out, lev, cv, st.as_dict()
+
+    def efield_time(self, particle, obs):
+        """(t[n], E[n, 3]): the time-domain electric field at obs, one value per knot."""
+        o = np.ascontiguousarray(obs, dtype=np.float64).reshape(3)
+        n = C.c_long(0)
+        self._chk(self.L.om_efield_time(self.h, int(particle), _ptr(o), C.byref(n), None, None))
+        t = np.zeros(n.value)
+        e = np.zeros((n.value, 3))
+        self._chk(self.L.om_efield_time(self.h, int(particle), _ptr(o), C.byref(n), _ptr(t), _ptr(e)))
+        return t, e
 
     def total_power(self, particle, conv):
         """(power [W], precision, level, stats); level -1: not converged."""

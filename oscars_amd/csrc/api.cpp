@@ -31,6 +31,8 @@ hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, h
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
+hipError_t om_launch_efield_time(const double* p, long n, const double* obs, double c0, double dt, double* out_t,
+                                 double* out_e, hipStream_t s);
 int om_tp_blocks(long npts);
 hipError_t om_launch_total_power_level(const om::TpArgs* A, hipStream_t s);
 hipError_t om_launch_spline_user(const double* t, double* p, double* ypp, double* u, long n, int construct,
@@ -216,7 +218,7 @@ struct om_context {
   DevBuf<unsigned char> d_outconv;
   DevBuf<unsigned long long> d_stats;
   DevBuf<double> d_bd;
-  DevBuf<double> d_normals, d_tp, d_tp_partial;
+  DevBuf<double> d_normals, d_tp, d_tp_partial, d_et;
   std::vector<om::FieldDev> fields_host;
   int nfields = 0;
   long total_knots = 0;
@@ -300,7 +302,7 @@ struct om_context {
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
     d_list.release(); d_border.release();
-    d_normals.release(); d_tp.release(); d_tp_partial.release();
+    d_normals.release(); d_tp.release(); d_tp_partial.release(); d_et.release();
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) (void) hipStreamDestroy(stream);
@@ -956,6 +958,25 @@ int om_trajectory_set(om_context* ctx, const om_particle* particle, long n, cons
     check(hipStreamSynchronize(ctx->stream), "sync");
     if (ctx->levels.size() != 1) ctx->levels.assign(1, LevelCache());
     ctx->np = 1;
+  });
+}
+
+int om_efield_time(om_context* ctx, int i, const double* obs, long* n, double* out_t, double* out_e3) {
+  return guarded(ctx, [&] {
+    check_particle(ctx, i);
+    if (!obs || !n) throw std::invalid_argument("missing inputs");
+    const om::TrajDesc& D = ctx->desc[i];
+    long cap = *n;
+    *n = D.n;
+    if (cap < D.n || !out_t || !out_e3) return;
+    ctx->d_et.ensure(4 * D.n);
+    // src/OSCARSSR.cc:4487: C0 = Q / (4 pi eps0)
+    double c0 = ctx->parts[i].q / (four_pi() * kEps0);
+    check(om_launch_efield_time(ctx->d_p.p + 9 * D.knot_offset, D.n, obs, c0, D.dt, ctx->d_et.p, ctx->d_et.p + D.n,
+                                ctx->stream),
+          "efield time");
+    read_back(ctx, out_t, ctx->d_et.p, D.n * sizeof(double));
+    read_back(ctx, out_e3, ctx->d_et.p + D.n, 3 * D.n * sizeof(double));
   });
 }
 

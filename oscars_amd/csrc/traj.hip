@@ -284,6 +284,37 @@ __global__ void __launch_bounds__(256) knot_slope_kernel(const double* __restric
   p[9 * i + c_dst + k] = d / div;
 }
 
+// Electric field in the time domain at one observer, one knot per lane
+// (src/OSCARSSR.cc:4471-4512): E = C0 / (1 - N.B)^3 (near + far field) and
+// the arrival time i dt + |R| / c.  pow(x, 3) is x * x * x here.
+__global__ void __launch_bounds__(256) efield_time_kernel(const double* __restrict__ p, long n, double ox, double oy,
+                                                          double oz, double c0, double dt,
+                                                          double* __restrict__ out_t, double* __restrict__ out_e) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* k = p + 9 * i;
+  const double Rx = ox - k[0], Ry = oy - k[1], Rz = oz - k[2];
+  const double r2 = Rx * Rx + Ry * Ry + Rz * Rz;
+  const double D = sqrt(r2);
+  const double Nx = Rx / D, Ny = Ry / D, Nz = Rz / D;
+  const double Bx = k[3], By = k[4], Bz = k[5];
+  const double Ax = k[6], Ay = k[7], Az = k[8];
+  const double q = 1.0 / (1.0 - (Nx * Bx + Ny * By + Nz * Bz));
+  const double mult = c0 * (q * q * q);
+  const double b2 = Bx * Bx + By * By + Bz * Bz;
+  const double Mx = Nx - Bx, My = Ny - By, Mz = Nz - Bz;
+  // (N - B) x A, then N x that
+  const double cx = My * Az - Ay * Mz, cy = Mz * Ax - Az * Mx, cz = Mx * Ay - Ax * My;
+  const double fx = Ny * cz - cy * Nz, fy = Nz * cx - cz * Nx, fz = Nx * cy - cx * Ny;
+  const double ic = 1.0 / kC;
+  const double nfx = ((1.0 - b2) * Mx) / r2, nfy = ((1.0 - b2) * My) / r2, nfz = ((1.0 - b2) * Mz) / r2;
+  const double ffx = (ic * fx) / D, ffy = (ic * fy) / D, ffz = (ic * fz) / D;
+  out_e[3 * i + 0] = mult * (nfx + ffx);
+  out_e[3 * i + 1] = mult * (nfy + ffy);
+  out_e[3 * i + 2] = mult * (nfz + ffz);
+  out_t[i] = (double) i * dt + D / kC;
+}
+
 // ---------------------------------------------------------------- level samples
 
 __global__ void __launch_bounds__(256) level_kernel(LevelJob J) {
@@ -420,6 +451,14 @@ extern "C" hipError_t om_launch_spline_user(const double* t, double* p, double* 
     hipLaunchKernelGGL(om::knot_slope_kernel, dim3(b3), dim3(256), 0, s, t, p, ypp, n, 3, 6, 1.0);
   }
   hipLaunchKernelGGL(om::spline_seq_kernel, dim3(1), dim3(64), 0, s, t, p, ypp, u, n, 0, 9);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_efield_time(const double* p, long n, const double* obs, double c0, double dt,
+                                            double* out_t, double* out_e, hipStream_t s) {
+  unsigned blocks = (unsigned) ((n + 255) / 256);
+  hipLaunchKernelGGL(om::efield_time_kernel, dim3(blocks), dim3(256), 0, s, p, n, obs[0], obs[1], obs[2], c0, dt,
+                     out_t, out_e);
   return hipGetLastError();
 }
 

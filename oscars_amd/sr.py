@@ -720,6 +720,20 @@ class sr:
             IO.write_points_binary(bofile, pos, v, dim)
         return [[list(pos[i]), float(v[i])] for i in range(npoints)]
 
+    def calculate_efield_vs_time(self, obs, ofile=""):
+        """Electric field in the time domain at obs: [[t, [Ex, Ey, Ez]], ...],
+        one entry per trajectory knot at t = i dt + |R| / c (uneven in the lab
+        frame) (src/OSCARSSR_Python.cc:9048-9110).  ofile writes 'Ex Ey Ez t'
+        lines, the layout the reference's container writer gives this data."""
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        o = _v3(obs, "obs")
+        self._ensure_trajectory()
+        t, e = self._device().efield_time(0, o)
+        if ofile != "":
+            IO.write_points_text(ofile, e, t, 3)
+        return [[float(t[i]), e[i].tolist()] for i in range(len(t))]
+
     def calculate_total_power(self, precision=0.01, max_level=-2, max_level_extended=0, quantity="power"):
         """Total radiated power [W] of the current particle, or the precision or
         level reached (src/OSCARSSR_Python.cc:5974-6060)."""

@@ -89,3 +89,32 @@ def test_power_density_line(dim):
     assert e < 1e-9
     if dim == 3:
         assert [r[0] for r in res] == obs
+
+
+@gpu
+@pytest.mark.parametrize("setup", [planar, epu])
+def test_efield_vs_time(setup):
+    """Time-domain E-field, relative to the largest component: against the
+    oracle on the device's own trajectory to 1e-13 (the oracle's pow(x, 3)
+    against x*x*x, otherwise the same operations; measured 3e-16), and on the
+    oracle's trajectory to 1e-9 (measured 1e-14); times to 1e-15."""
+    from oracle.pyoracle import Trajectory
+    o, s = _pair(setup, 3001)
+    dt = ((s.get_ctstop() - s.get_ctstart()) / 2.99792458e8 / float(3001 - 1))
+    for obs in ([0, 0, 30], [0.003, -0.001, 20]):
+        res = s.calculate_efield_vs_time(obs)
+        gt = np.array([r[0] for r in res])
+        ge = np.array([r[1] for r in res])
+        ot, oe = o.efield_time(obs, dt)
+        scale = np.max(np.abs(oe))
+        e1 = float(np.max(np.abs(ge - oe)) / scale)
+        t, pk, ypp = s._device().trajectory(0)
+        o2 = o.traj
+        o.traj = Trajectory(t, pk, ypp)
+        ot2, oe2 = o.efield_time(obs, dt)
+        o.traj = o2
+        e2 = float(np.max(np.abs(ge - oe2)) / scale)
+        print("efield vs time %s %s: %.3e (oracle trajectory), %.3e (device trajectory)" % (setup.__name__, obs, e1,
+                                                                                             e2))
+        assert np.max(np.abs(gt - ot2) / ot2) < 1e-15
+        assert e2 < 1e-13 and e1 < 1e-9

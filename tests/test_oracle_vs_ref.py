@@ -185,3 +185,16 @@ def test_total_power(setup):
     assert o.total_power(max_level=5)[2] == -1
     with pytest.raises(ValueError, match="did not converge"):
         r.calculate_total_power(max_level=5)
+
+
+@pytest.mark.parametrize("setup", [planar, epu])
+def test_efield_vs_time(setup):
+    """Time-domain electric field at one observer: one value per knot, time
+    i dt + |R| / c with dt the integration step."""
+    o, r = pair(setup, npoints=3001)
+    o.calculate_trajectory()
+    dt = ((o.ctstop - o.ctstart) / 2.99792458e8 / float(3001 - 1))
+    for obs in ([0, 0, 30], [0.003, -0.001, 20]):
+        rt, re = r.calculate_efield_vs_time(obs)
+        ot, oe = o.efield_time(obs, dt)
+        assert same(ot, rt) and same(oe, re)
