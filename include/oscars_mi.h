@@ -160,6 +160,15 @@ int om_power_density_surface(om_context* ctx, int particle, const double* obs_xy
                              const double* normals, int directional, const om_conv* conv, double* out_value,
                              int* out_level, unsigned char* out_conv, om_stats* stats);
 
+/* Power density at the centres of STL triangles: calculate_power_density_stl
+ * (src/OSCARSSR_Python.cc:6993-7213, src/OSCARSSR.cc:3318-3527).  tri15[15 n]
+ * per triangle: vertices A, B, C [m], the normal as read from the file, the
+ * centre.  A trajectory sample is skipped for a triangle when the segment to
+ * its centre meets another triangle (src/TTriangle3D.cc:159-215).  Host
+ * pointers. */
+int om_power_density_stl(om_context* ctx, int particle, const double* tri15, long n, const om_conv* conv,
+                         double* out_value, int* out_level, unsigned char* out_conv, om_stats* stats);
+
 /* Total radiated power of trajectory `particle` [W]: calculate_total_power
  * (src/OSCARSSR_Python.cc:5974-6060, src/OSCARSSR.cc:3725-3843).  conv gives
  * precision and the level limits; *out_level is -1 when no level met the

@@ -1090,6 +1090,130 @@ int orc_total_power(const orc_traj* T, const orc_particle* P, const orc_opts* o,
   return Result_Level == -1 ? 1 : 0;
 }
 
+/* TTriangle3D::RayIntersectionDistance (src/TTriangle3D.cc:159-215), Moller and
+ * Trumbore: distance along the (normalised) ray to triangle ABC, or -1. */
+static double ray_triangle(const double* A, const double* B, const double* Cv, const double* O, const double* Din) {
+  double D[3], e1[3], e2[3], p[3], tv[3], q[3];
+  v_unit(Din, D);
+  for (int k = 0; k < 3; ++k) {
+    e1[k] = B[k] - A[k];
+    e2[k] = Cv[k] - A[k];
+    tv[k] = O[k] - A[k];
+  }
+  v_cross(D, e2, p);
+  double det = v_dot(e1, p);
+  if (det > -1e-12 && det < 1e-12) return -1;
+  double inv_det = 1.0 / det;
+  double u = v_dot(tv, p) * inv_det;
+  if (u < 0.0 || u > 1.0) return -1;
+  v_cross(tv, e1, q);
+  double v = v_dot(D, q) * inv_det;
+  if (v < 0.0 || u + v > 1.0) return -1;
+  double t = v_dot(e2, q) * inv_det;
+  if (t < 0) return -1;
+  return t;
+}
+
+/* Power density at the centres of STL triangles (src/OSCARSSR.cc:3318-3527):
+ * tri[15 n] = A, B, C, normal (as read), centre per triangle.  A trajectory
+ * sample whose ray to the centre meets another triangle first is skipped. */
+int orc_power_density_stl(const orc_traj* T, const orc_particle* P, const double* tri, long n, const orc_opts* o,
+                          double* out_value, int* out_level, unsigned char* out_conv) {
+  level_cache lc;
+  int sm, se;
+  level_stops(o, &sm, &se);
+  lc_init(&lc, T, sm);
+  double pd_const = fabs(P->q * P->current) / (16 * k_pi2() * K_EPS0 * K_C);
+  double Result_Precision = -1;
+  int Result_Level = -1;
+  for (long i = 0; i < n; ++i) {
+    const double* Obs = tri + 15 * i + 12;
+    const double* Normal = tri + 15 * i + 9;
+    double LastSum = -1;
+    int LastLevel = 0, converged_here = 0;
+    double Sum = 0;
+    for (int L = 0; L <= se; ++L) {
+      LastLevel = L;
+      double Last_Beta[3] = {0, 0, 0};
+      double BetaDiffMax = -1;
+      const double* S = lc_get(&lc, L);
+      long NT = (long) pow(2, L);
+      for (long iT = 0; iT != NT; ++iT) {
+        const double* X = S + 10 * iT + 1;
+        const double* B = S + 10 * iT + 4;
+        const double* A = S + 10 * iT + 7;
+        double R[3] = {Obs[0] - X[0], Obs[1] - X[1], Obs[2] - X[2]};
+        double dist = sqrt(v_mag2(R)), Ru[3];
+        v_unit(R, Ru);
+        int blocked = 0;
+        for (long j = 0; j < n && !blocked; ++j) {
+          if (j == i) continue;
+          const double* tj = tri + 15 * j;
+          double t = ray_triangle(tj, tj + 3, tj + 6, X, Ru);
+          if (t > 0 && t < dist) blocked = 1;
+        }
+        if (blocked) continue;
+        double dB[3] = {B[0] - Last_Beta[0], B[1] - Last_Beta[1], B[2] - Last_Beta[2]};
+        double BetaDiff = sqrt(v_mag2(dB));
+        if (iT > 0 && BetaDiff > BetaDiffMax) BetaDiffMax = BetaDiff;
+        Last_Beta[0] = B[0]; Last_Beta[1] = B[1]; Last_Beta[2] = B[2];
+        double N1[3], N2[3], N3[3], oo[3], c[3];
+        v_unit(R, N1);
+        {
+          double xx = N1[0] < 0.0 ? -N1[0] : N1[0];
+          double yy = N1[1] < 0.0 ? -N1[1] : N1[1];
+          double zz = N1[2] < 0.0 ? -N1[2] : N1[2];
+          if (xx < yy) {
+            if (xx < zz) { oo[0] = 0; oo[1] = N1[2]; oo[2] = -N1[1]; }
+            else { oo[0] = N1[1]; oo[1] = -N1[0]; oo[2] = 0; }
+          } else {
+            if (yy < zz) { oo[0] = -N1[2]; oo[1] = 0; oo[2] = N1[0]; }
+            else { oo[0] = N1[1]; oo[1] = -N1[0]; oo[2] = 0; }
+          }
+        }
+        v_unit(oo, N2);
+        v_cross(N1, N2, c);
+        v_unit(c, N3);
+        double N1DotNormal = fabs(v_dot(N1, Normal));
+        double NmB[3] = {N1[0] - B[0], N1[1] - B[1], N1[2] - B[2]};
+        double c1[3], Num[3];
+        v_cross(NmB, A, c1);
+        v_cross(N1, c1, Num);
+        double Denominator = pow(1 - v_dot(B, N1), 5);
+        double d2 = v_dot(Num, N2), d3 = v_dot(Num, N3);
+        double r2 = v_mag2(R);
+        Sum += d2 * d2 / Denominator / r2 * N1DotNormal;
+        Sum += d3 * d3 / Denominator / r2 * N1DotNormal;
+      }
+      double ThisSum = Sum * orc_dt_inclusive(T, L);
+      Result_Precision = fabs(ThisSum - LastSum) / LastSum;
+      if ((L > 8 && Result_Precision < o->precision && BetaDiffMax < 2. / (P->gamma)) ||
+          (L > 8 && BetaDiffMax < 2. / (P->gamma) && ThisSum == LastSum)) {
+        Result_Level = L;
+        converged_here = 1;
+        break;
+      } else if (L > 8 && ThisSum == LastSum) {
+        Result_Level = L;
+        Result_Precision = 0;
+        converged_here = 1;
+        break;
+      }
+      LastSum = ThisSum;
+    }
+    Sum *= pd_const * orc_dt_inclusive(T, LastLevel);
+    Sum /= 1e6;
+    double v;
+    if (o->quantity == 1) v = Result_Precision * o->weight;
+    else if (o->quantity == 2) v = ((double) Result_Level) * o->weight;
+    else v = Sum * o->weight;
+    out_value[i] = v;
+    if (out_level) out_level[i] = o->carry_quirk ? Result_Level : (converged_here ? LastLevel : -1);
+    if (out_conv) out_conv[i] = (unsigned char) converged_here;
+  }
+  lc_free(&lc);
+  return 0;
+}
+
 /* Electric field in the time domain at Obs, one value per knot
  * (src/OSCARSSR.cc:4471-4512); dt is the trajectory's step. */
 void orc_efield_time(const orc_traj* T, const orc_particle* P, const double* Obs, double dt, double* out_t,

@@ -128,6 +128,8 @@ int orc_power_density_surface(const orc_traj* T, const orc_particle* P, const do
 int orc_total_power(const orc_traj* T, const orc_particle* P, const orc_opts* o, double* out_power,
                     double* out_precision, int* out_level);
 
+int orc_power_density_stl(const orc_traj* T, const orc_particle* P, const double* tri, long n, const orc_opts* o,
+                          double* out_value, int* out_level, unsigned char* out_conv);
 void orc_efield_time(const orc_traj* T, const orc_particle* P, const double* Obs, double dt, double* out_t,
                      double* out_e);
 

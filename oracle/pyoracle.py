@@ -112,6 +112,7 @@ def oracle_lib():
                                                 _dp, _ip, _up]
         L.orc_total_power.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _ip]
         L.orc_efield_time.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_double, _dp, _dp]
+        L.orc_power_density_stl.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_long, C.c_void_p, _dp, _ip, _up]
         L.orc_efield_time.restype = None
         _oracle = L
     return _oracle
@@ -386,6 +387,20 @@ class Oracle:
                                e.ctypes.data_as(_dp))
         return t, e
 
+    def power_density_stl(self, tri15, precision=0.01, max_level=-2, max_level_extended=0, quantity="power density",
+                          carry_quirk=True):
+        """tri15[n, 15] = A, B, C, normal, centre per triangle."""
+        T = self._traj()
+        t = np.ascontiguousarray(tri15, dtype=np.float64).reshape(-1, 15)
+        n = len(t)
+        out = np.zeros(n)
+        lev = np.zeros(n, dtype=np.int32)
+        conv = np.zeros(n, dtype=np.uint8)
+        o = self._opts(precision, max_level, max_level_extended, self._quantity(quantity, pd=True), 1, carry_quirk)
+        self.L.orc_power_density_stl(C.byref(T.c()), C.byref(self.P), t.ctypes.data_as(_dp), n, C.byref(o),
+                                     out.ctypes.data_as(_dp), lev.ctypes.data_as(_ip), conv.ctypes.data_as(_up))
+        return out, lev, conv
+
     def total_power(self, precision=0.01, max_level=-2, max_level_extended=0):
         """(power [W], precision, level); level -1 when not converged."""
         T = self._traj()
@@ -485,6 +500,10 @@ def ref_lib():
                                              C.c_int, C.c_int, _dp, _dp, _ip]
         L.ref_total_power.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp]
         L.ref_efield_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_long), _dp, _dp]
+        L.ref_add_stl.argtypes = [C.c_void_p, C.c_char_p, C.c_double, _dp, _dp, C.c_char_p]
+        L.ref_clear_stl.argtypes = [C.c_void_p]
+        L.ref_power_density_stl.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                                            C.POINTER(C.c_long), _dp, _dp]
         L.ref_read_trajectory.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]
         L.ref_set_trajectory_points.argtypes = [C.c_void_p, C.c_long, _dp, C.c_int, C.c_int]
         L.ref_random_normal.argtypes = [C.c_void_p]
@@ -677,6 +696,29 @@ class RefSim:
                                                 int(max_level_extended), xyz.ctypes.data_as(_dp),
                                                 v.ctypes.data_as(_dp), C.byref(conv)))
         return xyz, v, bool(conv.value)
+
+    def add_stl(self, ifile, rotations=(0, 0, 0), translation=(0, 0, 0), scale=1.0, name=""):
+        self._chk(self.L.ref_add_stl(self.h, ifile.encode(), float(scale), _v3(rotations), _v3(translation),
+                                     name.encode()))
+
+    def clear_stl(self):
+        self.L.ref_clear_stl(self.h)
+
+    def calculate_power_density_stl(self, precision=0.01, max_level=-2, max_level_extended=0,
+                                    quantity="power density", nparticles=0, stlofile=None):
+        """(triangles[n, 3, 3], values[n]) as the reference computes them."""
+        q = {"POWER DENSITY": 0, "POWERDENSITY": 0, "PRECISION": 1, "LEVEL": 2}[quantity.upper()]
+        self.L.ref_set_nparticles(int(nparticles))
+        n = C.c_long(0)
+        args = (float(precision), int(max_level), int(max_level_extended), q, 1,
+                stlofile.encode() if stlofile else None)
+        self._chk(self.L.ref_power_density_stl(self.h, *args, C.byref(n), None, None))
+        tri = np.zeros((n.value, 3, 3))
+        v = np.zeros(n.value)
+        self.L.ref_set_nparticles(int(nparticles))
+        self._chk(self.L.ref_power_density_stl(self.h, *args, C.byref(n), tri.ctypes.data_as(_dp),
+                                               v.ctypes.data_as(_dp)))
+        return tri, v
 
     def calculate_efield_vs_time(self, obs):
         """(t[n], E[n, 3]) as the reference computes them."""

@@ -446,6 +446,40 @@ int ref_power_density_line(void* h, double const* x1, double const* x2, int npoi
   } catch (std::exception const& e) { return fail(e); }
 }
 
+// add_stl (src/OSCARSSR_Python.cc:6852-6927)
+int ref_add_stl(void* h, const char* path, double scale, double const* rotations, double const* translation,
+                const char* name) {
+  try {
+    static_cast<OSCARSSR*>(h)->AddSTLFile(path, scale, v3(rotations), v3(translation), name);
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+void ref_clear_stl(void* h) { static_cast<OSCARSSR*>(h)->ClearSTL(); }
+
+// calculate_power_density_stl (src/OSCARSSR_Python.cc:6993-7213): *n in = capacity,
+// out = triangle count; out_tri[9 k] = A, B, C; out_value[k].  stl_out: stlofile.
+int ref_power_density_stl(void* h, double precision, int max_level, int max_level_extended, int quantity,
+                          int nthreads, const char* stl_out, long* n, double* out_tri, double* out_value) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    s->CalculatePowerDensitySTL(TVector3D(0, 0, 0), precision, max_level, max_level_extended, g_nparticles, nthreads,
+                                0, -1, std::vector<int>(), quantity);
+    TSTLContainer const& S = s->GetSTLContainer();
+    if (stl_out) S.WriteSTLFile(stl_out);
+    long cap = *n;
+    *n = (long) S.GetNPoints();
+    if (cap < *n) return 0;
+    for (long i = 0; i < *n; ++i) {
+      TTriangle3D T = S.GetPoint(i);
+      for (int v = 0; v < 3; ++v)
+        for (int k = 0; k < 3; ++k) out_tri[9 * i + 3 * v + k] = T[v][k];
+      out_value[i] = T.GetValue();
+    }
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
 // calculate_efield_vs_time (src/OSCARSSR_Python.cc:9048-9110): *n in = capacity,
 // out = count; out_t[k], out_e[3 k].
 int ref_efield_time(void* h, double const* obs, long* n, double* out_t, double* out_e) {

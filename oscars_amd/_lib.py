@@ -59,7 +59,7 @@ class om_stats(C.Structure):
 
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
            "om_trajectories", "om_trajectory_set", "om_trajectory_get", "om_power_density_surface",
-           "om_total_power", "om_efield_time", "om_level_get", "om_trajectory_status", "om_flux",
+           "om_total_power", "om_efield_time", "om_power_density_stl", "om_level_get", "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
 
@@ -105,6 +105,8 @@ def lib():
                                            _dp, _ip, _up, C.POINTER(om_stats)]
     L.om_total_power.argtypes = [C.c_void_p, C.c_int, C.POINTER(om_conv), _dp, _dp, _ip, C.POINTER(om_stats)]
     L.om_efield_time.argtypes = [C.c_void_p, C.c_int, _dp, _lp, _dp, _dp]
+    L.om_power_density_stl.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, C.POINTER(om_conv), _dp, _ip, _up,
+                                       C.POINTER(om_stats)]
     _vp = C.c_void_p  # device addresses
     L.om_flux_dev.argtypes = [C.c_void_p, C.c_int, _vp, C.c_long, _vp, C.c_long, _dp, C.POINTER(om_conv), _vp, _vp,
                               _vp, _vp, C.POINTER(om_stats)]
@@ -270,6 +272,18 @@ class Device:
         self._chk(self.L.om_power_density_surface(self.h, int(particle), _ptr(obs), n, _ptr(nv), int(directional),
                                                   C.byref(conv), _ptr(out), _ptr(lev, _ip), _ptr(cv, _up),
                                                   C.byref(st)))
+        return out, lev, cv, st.as_dict()
+
+    def power_density_stl(self, particle, tri15, conv, out=None):
+        """Power density at STL triangle centres; tri15[n, 15] = A, B, C, normal, centre."""
+        t = np.ascontiguousarray(tri15, dtype=np.float64).reshape(-1, 15)
+        n = len(t)
+        out = np.zeros(n) if out is None else out
+        lev = np.zeros(n, dtype=np.int32)
+        cv = np.zeros(n, dtype=np.uint8)
+        st = om_stats()
+        self._chk(self.L.om_power_density_stl(self.h, int(particle), _ptr(t), n, C.byref(conv), _ptr(out),
+                                              _ptr(lev, _ip), _ptr(cv, _up), C.byref(st)))
         return out, lev, cv, st.as_dict()
 
     def efield_time(self, particle, obs):

@@ -34,6 +34,7 @@ hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
 hipError_t om_launch_efield_time(const double* p, long n, const double* obs, double c0, double dt, double* out_t,
                                  double* out_e, hipStream_t s);
 int om_tp_blocks(long npts);
+hipError_t om_launch_pd_stl(const om::StlArgs* A, hipStream_t s);
 hipError_t om_launch_total_power_level(const om::TpArgs* A, hipStream_t s);
 hipError_t om_launch_spline_user(const double* t, double* p, double* ypp, double* u, long n, int construct,
                                  hipStream_t s);
@@ -218,7 +219,7 @@ struct om_context {
   DevBuf<unsigned char> d_outconv;
   DevBuf<unsigned long long> d_stats;
   DevBuf<double> d_bd;
-  DevBuf<double> d_normals, d_tp, d_tp_partial, d_et;
+  DevBuf<double> d_normals, d_tp, d_tp_partial, d_et, d_stl;
   std::vector<om::FieldDev> fields_host;
   int nfields = 0;
   long total_knots = 0;
@@ -302,7 +303,7 @@ struct om_context {
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
     d_list.release(); d_border.release();
-    d_normals.release(); d_tp.release(); d_tp_partial.release(); d_et.release();
+    d_normals.release(); d_tp.release(); d_tp_partial.release(); d_et.release(); d_stl.release();
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) (void) hipStreamDestroy(stream);
@@ -1138,6 +1139,74 @@ int om_power_density_surface(om_context* ctx, int ip, const double* obs_xyz, lon
                              unsigned char* out_conv, om_stats* stats) {
   return power_density(ctx, ip, obs_xyz, n_obs, nullptr, directional, conv, out_value, out_level, out_conv, stats,
                        false, normals);
+}
+
+int om_power_density_stl(om_context* ctx, int ip, const double* tri15, long n, const om_conv* conv, double* out_value,
+                         int* out_level, unsigned char* out_conv, om_stats* stats) {
+  return guarded(ctx, [&] {
+    Timer wall;
+    ctx->timer.used = 0;
+    check_particle(ctx, ip);
+    if (!conv || !tri15 || !out_value) throw std::invalid_argument("missing inputs or outputs");
+    if (n < 1) throw std::length_error("no points specified");
+    int stop_mem, stop_ext;
+    level_stops(*conv, stop_mem, stop_ext);
+    if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
+    // A, B - A, C - A and the normal per triangle; the centres apart
+    std::vector<double> tri(12 * n), centre(3 * n);
+    for (long i = 0; i < n; ++i) {
+      const double* t = tri15 + 15 * i;
+      for (int k = 0; k < 3; ++k) {
+        tri[12 * i + k] = t[k];
+        tri[12 * i + 3 + k] = t[3 + k] - t[k];
+        tri[12 * i + 6 + k] = t[6 + k] - t[k];
+        tri[12 * i + 9 + k] = t[9 + k];
+        centre[3 * i + k] = t[12 + k];
+      }
+    }
+    upload(ctx, ctx->d_stl, tri.data(), tri.size());
+    upload(ctx, ctx->d_obs, centre.data(), centre.size());
+    om::StlArgs A;
+    std::memset(&A, 0, sizeof(A));
+    state_for(ctx, n, false, A.st);
+    A.ntri = n;
+    A.tri = ctx->d_stl.p;
+    A.centre = ctx->d_obs.p;
+    A.precision = conv->precision;
+    A.two_over_gamma = 2. / (ctx->parts[ip].gamma);
+    int La = 0;
+    int Lb = stop_ext < kFirstLevels ? stop_ext : kFirstLevels;
+    while (true) {
+      ensure_levels(ctx, ip, Lb, stop_mem);
+      fill_levels(ctx, ip, Lb, A.lv);
+      A.La = La;
+      A.Lb = Lb;
+      ctx->timer.begin(ctx->stream);
+      check(om_launch_pd_stl(&A, ctx->stream), "pd_stl");
+      ctx->timer.end(ctx->stream);
+      if (Lb >= stop_ext) break;
+      if (count_active(ctx, A.st, n) == 0) break;
+      La = Lb + 1;
+      Lb = La;
+    }
+    const om_particle& P = ctx->parts[ip];
+    om::FinArgs F;
+    std::memset(&F, 0, sizeof(F));
+    F.st = A.st;
+    F.npts = n;
+    F.kind = 1;
+    F.quantity = conv->quantity;
+    // src/OSCARSSR.cc:3506
+    F.pd_const = std::fabs(P.q * P.current) / (16 * (kPi * kPi) * kEps0 * kC);
+    F.directional = 1;  // the terms carry |N1 . normal|: no sign to fold
+    for (int L = 0; L <= om::kMaxLevel; ++L) F.dt_incl[L] = A.lv.dt_incl[L];
+    finish_outputs(ctx, F, n, out_value, out_level, out_conv, nullptr, stats, *conv, false);
+    double kms = ctx->timer.take();
+    if (stats) {
+      stats->kernel_ms = kms;
+      stats->total_ms = wall.ms();
+    }
+  });
 }
 
 int om_total_power(om_context* ctx, int ip, const om_conv* conv, double* out_power, double* out_precision,

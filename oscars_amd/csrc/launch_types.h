@@ -54,6 +54,18 @@ struct PdArgs {
   double two_over_gamma;
 };
 
+// Power density at the centres of STL triangles with occlusion by the others.
+struct StlArgs {
+  Levels lv;
+  State st;
+  int La, Lb;
+  long ntri;
+  const double* tri;  // 12 per triangle: A, B - A, C - A, normal as read
+  const double* centre;  // 3 per triangle
+  double precision;
+  double two_over_gamma;
+};
+
 // Total power: per-level sums of the integrand over the level's samples.
 struct TpArgs {
   const double* rows;   // 2^L samples of (t, x, b, a/c)

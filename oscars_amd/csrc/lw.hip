@@ -412,6 +412,103 @@ __global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
   }
 }
 
+// ---------------------------------------------------------------- power density on STL triangles
+// src/OSCARSSR.cc:3318-3527.  One wave per triangle centre; the wave walks the
+// level's samples in order.  For each sample the 64 lanes test the other
+// triangles, 64 at a time, for a hit between the sample and the centre
+// (src/TTriangle3D.cc:159-215, Moller and Trumbore, same operations); the
+// sample is skipped when any lane finds one.  Unblocked samples add the power
+// density integrand with |N1 . normal| and move the beta-step maximum, as in
+// the reference, so every lane holds the same sum and the lane 0 stores it.
+constexpr int kStlWaves = 4;
+
+__device__ __forceinline__ bool stl_blocks(const double* __restrict__ T, V3 O, V3 D, double dist) {
+  // T: A, edge1, edge2 of one triangle; D: the unit ray direction, normalised again
+  double m = sqrt(mag2(D));
+  V3 R = v3(D.x / m, D.y / m, D.z / m);
+  V3 A = v3(T[0], T[1], T[2]), e1 = v3(T[3], T[4], T[5]), e2 = v3(T[6], T[7], T[8]);
+  V3 p = cross(R, e2);
+  double det = dot(e1, p);
+  if (det > -1e-12 && det < 1e-12) return false;
+  double inv = 1.0 / det;
+  V3 tv = sub(O, A);
+  double u = dot(tv, p) * inv;
+  if (u < 0.0 || u > 1.0) return false;
+  V3 q = cross(tv, e1);
+  double v = dot(R, q) * inv;
+  if (v < 0.0 || u + v > 1.0) return false;
+  double t = dot(e2, q) * inv;
+  if (t < 0) return false;
+  return t > 0 && t < dist;
+}
+
+__global__ void __launch_bounds__(64 * kStlWaves) pd_stl_kernel(StlArgs A) {
+  const int lane = threadIdx.x & 63;
+  const long p = (long) blockIdx.x * kStlWaves + (threadIdx.x >> 6);
+  if (p >= A.ntri) return;
+  int lev = A.st.level[p];
+  if (A.st.conv[p] || lev != A.La - 1) return;
+  const V3 obs = v3(A.centre[3 * p], A.centre[3 * p + 1], A.centre[3 * p + 2]);
+  const V3 nrm = v3(A.tri[12 * p + 9], A.tri[12 * p + 10], A.tri[12 * p + 11]);
+  double sum = A.st.sum[6 * p];
+  double lastsum = A.st.lastmag[p];
+  double prec = A.st.prec[p];
+  bool done = false;
+  for (int L = A.La; L <= A.Lb && !done; ++L) {
+    const double* __restrict__ rows = A.lv.rows[L];
+    const long ns = 1L << L;
+    V3 lastB = v3(0, 0, 0);
+    double bdmax = -1;
+    for (long i = 0; i < ns; ++i) {
+      const double* s = rows + 10 * i;
+      V3 X = v3(s[1], s[2], s[3]);
+      V3 R = sub(obs, X);
+      double r2 = mag2(R);
+      double dist = sqrt(r2);
+      V3 N1 = v3(R.x / dist, R.y / dist, R.z / dist);
+      bool blocked = false;
+      for (long base = 0; base < A.ntri; base += 64) {
+        long j = base + lane;
+        bool hit = j < A.ntri && j != p && stl_blocks(A.tri + 12 * j, X, N1, dist);
+        if (__any(hit)) {
+          blocked = true;
+          break;
+        }
+      }
+      if (blocked) continue;
+      V3 B = v3(s[4], s[5], s[6]);
+      V3 Ac = v3(s[7], s[8], s[9]);
+      V3 dB = sub(B, lastB);
+      double bd = sqrt(mag2(dB));
+      if (i > 0 && bd > bdmax) bdmax = bd;
+      lastB = B;
+      double ndn = fabs(dot(N1, nrm));
+      V3 Num = cross(N1, cross(sub(N1, B), Ac));
+      double x = 1 - dot(B, N1);
+      double x2 = x * x;
+      double den = x2 * x2 * x;
+      // (Num.N2)^2 + (Num.N3)^2 = |Num|^2 because Num is perpendicular to N1
+      sum += mag2(Num) / den / r2 * ndn;
+    }
+    lev = L;
+    double thisSum = sum * A.lv.dt_incl[L];
+    prec = fabs(thisSum - lastsum) / lastsum;
+    bool c = (L > 8 && prec < A.precision && bdmax < A.two_over_gamma) ||
+             (L > 8 && bdmax < A.two_over_gamma && thisSum == lastsum);
+    bool c2 = !c && L > 8 && thisSum == lastsum;
+    if (c2) prec = 0;
+    if (c || c2) done = true;
+    else lastsum = thisSum;
+  }
+  if (lane == 0) {
+    A.st.sum[6 * p] = sum;
+    A.st.lastmag[p] = lastsum;
+    A.st.prec[p] = prec;
+    A.st.level[p] = lev;
+    if (done) A.st.conv[p] = 1;
+  }
+}
+
 // ---------------------------------------------------------------- total power
 // src/OSCARSSR.cc:3785-3802: the integrand (|A/c|^2 - |B x A/c|^2) / (1 - B.B)^3
 // summed over one level's samples.  Each workgroup sums a contiguous slice
@@ -870,6 +967,12 @@ extern "C" hipError_t om_launch_tile_list(const om::State* st, long n_obs, long 
 extern "C" hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s) {
   long blocks = (A->npts + 255) / 256;
   hipLaunchKernelGGL(om::pd_point_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *A);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_pd_stl(const om::StlArgs* A, hipStream_t s) {
+  long blocks = (A->ntri + om::kStlWaves - 1) / om::kStlWaves;
+  hipLaunchKernelGGL(om::pd_stl_kernel, dim3((unsigned) blocks), dim3(64 * om::kStlWaves), 0, s, *A);
   return hipGetLastError();
 }
 

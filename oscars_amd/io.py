@@ -459,3 +459,53 @@ def read_trajectory_binary(path, fmt=""):
         if math.sqrt(a[i, 4] * a[i, 4] + a[i, 5] * a[i, 5] + a[i, 6] * a[i, 6]) >= 1.0:
             raise ValueError("Magnitude of beta >= 1 in input file.  Sorry, this is beyond einstein.  ipoint: %d" % i)
     return a[:, 0].copy(), a[:, 1:].copy(), has_b, has_bp
+
+
+# ---------------------------------------------------------------- STL surfaces
+# Binary STL (80-byte header, uint32 count, then per triangle float32 normal,
+# three float32 vertices and a 2-byte attribute), as the reference reads and
+# writes it (src/TTriangle3DContainer.cc:110-228, src/TSTLContainer.cc:57-113).
+_STL_REC = np.dtype([("n", "<f4", 3), ("a", "<f4", 3), ("b", "<f4", 3), ("c", "<f4", 3), ("s", "<i2")])
+
+
+def read_stl(path, scale=1.0, translation=(0.0, 0.0, 0.0)):
+    """Triangles of a binary STL file as the reference stores them:
+    tri[n, 15] = A, B, C (vertices times scale, then translated), the normal
+    as written in the file, and the centre ((A + B + C) / 3 before the
+    translation, then translated).  The reference ignores rotations here."""
+    try:
+        data = open(path, "rb").read()
+    except OSError:
+        raise ValueError("input file cannot be opened")
+    n = int(np.frombuffer(data[80:84], dtype="<u4")[0]) if len(data) >= 84 else 0
+    rec = np.frombuffer(data[84:84 + 50 * n], dtype=_STL_REC, count=n)
+    s = float(scale)
+    a = rec["a"].astype(np.float64) * s
+    b = rec["b"].astype(np.float64) * s
+    c = rec["c"].astype(np.float64) * s
+    centre = (a + b + c) / 3.
+    t = np.asarray(translation, dtype=np.float64).reshape(1, 3)
+    return np.concatenate([a + t, b + t, c + t, rec["n"].astype(np.float64), centre + t], axis=1)
+
+
+def write_stl(path, tri, scales):
+    """TSTLContainer::WriteSTLFile: the header, the count, then each triangle
+    with its vertices multiplied by its file's scale (again) as float32 and
+    a zero attribute.  scales[k] is the scale of triangle k's file."""
+    tri = np.asarray(tri, dtype=np.float64).reshape(-1, 15)
+    try:
+        f = open(path, "wb")
+    except OSError:
+        raise ValueError("output file cannot be opened")
+    rec = np.zeros(len(tri), dtype=_STL_REC)
+    sc = np.asarray(scales, dtype=np.float64).reshape(-1, 1)
+    with np.errstate(over="ignore"):
+        rec["n"] = tri[:, 9:12]
+        rec["a"] = tri[:, 0:3] * sc
+        rec["b"] = tri[:, 3:6] * sc
+        rec["c"] = tri[:, 6:9] * sc
+    with f:
+        head = b"OSCARS OSCARS OSCARS OSCARS OSCARS   "
+        f.write(head + b"\x00" * (80 - len(head)))
+        f.write(np.array([len(tri)], dtype="<u4").tobytes())
+        f.write(rec.tobytes())

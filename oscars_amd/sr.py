@@ -93,6 +93,7 @@ class sr:
         self._dev = None
         self._traj_valid = False
         self.last_stats = None
+        self._stl = []           # (name, tri[n, 15], scale) per STL object
 
     # ------------------------------------------------------------ device
     def _device(self):
@@ -719,6 +720,57 @@ class sr:
         if bofile != "":
             IO.write_points_binary(bofile, pos, v, dim)
         return [[list(pos[i]), float(v[i])] for i in range(npoints)]
+
+    # ------------------------------------------------------------ STL surfaces
+    def add_stl(self, ifile, rotations=None, translation=None, scale=1.0, name=""):
+        """Add the triangles of a binary STL file, scaled then translated
+        (src/OSCARSSR_Python.cc:6852-6927, src/TTriangle3DContainer.cc:110-228).
+        As in the reference, ``rotations`` is checked but not applied."""
+        if rotations is not None:
+            _v3(rotations, "rotations")
+        tr = _v3(translation, "translation") if translation is not None else [0.0, 0.0, 0.0]
+        my = name if name != "" else "_stl" + str(len(self._stl))
+        if any(n == my for n, _, _ in self._stl):
+            raise ValueError("stl with this name already exists")
+        self._stl.append((my, IO.read_stl(ifile, float(scale), tr), float(scale)))
+
+    def clear_stl(self):
+        self._stl = []
+
+    def remove_stl(self, name):
+        self._stl = [s for s in self._stl if s[0] != name]
+
+    def calculate_power_density_stl(self, farfield_origin=None, ofile="", bofile="", stlofile="", nparticles=0,
+                                    gpu=-1, ngpu=None, nthreads=0, precision=0.01, max_level=-2,
+                                    max_level_extended=0, quantity="power density"):
+        """Power density [W/mm^2] at the centre of every STL triangle, with the
+        samples whose path to the centre meets another triangle left out;
+        returns [[[A, B, C], pd], ...] (src/OSCARSSR_Python.cc:6993-7213).
+        ``farfield_origin`` is checked and unused, and ``ofile`` / ``bofile``
+        write nothing, as in the reference; ``stlofile`` writes the triangles."""
+        if self._beam is None:
+            raise ValueError("No particle beam defined")
+        if farfield_origin is not None:
+            _v3(farfield_origin, "farfield_origin")
+        self._check_common(nparticles, nthreads, 0 if gpu == 0 else gpu)
+        q = self._quantity(quantity, pd=True)
+        if not self._stl:
+            return []
+        tri = np.concatenate([t for _, t, _ in self._stl])
+
+        def run(dev, ip, w, acc, out=None):
+            conv = dev.conv(precision, max_level, max_level_extended, q, "plain", w, acc)
+            v, lev, cv, st = dev.power_density_stl(ip, tri, conv, out=out)
+            self.last_stats = st
+            self._last_levels = lev
+            self._last_conv = cv
+            return v
+
+        v = self._run_particles(nparticles, run)
+        if stlofile != "":
+            IO.write_stl(stlofile, tri, np.concatenate([[s] * len(t) for _, t, s in self._stl]))
+        return [[[tri[i, 0:3].tolist(), tri[i, 3:6].tolist(), tri[i, 6:9].tolist()], float(v[i])]
+                for i in range(len(tri))]
 
     def calculate_efield_vs_time(self, obs, ofile=""):
         """Electric field in the time domain at obs: [[t, [Ex, Ey, Ez]], ...],

@@ -60,3 +60,40 @@ PREDEFINED_CASES = [
     dict(beam="NSLSII-LONGSTRAIGHT", x0=[0.0001, 0, -1], d0=[0.001, 0, 1], emittance=[1e-9, 1e-11], beta=[2, 1],
          alpha=[0.2, 0.1], eta=[0.02, 0]),
 ]
+
+
+def stl_plate(x0, x1, y0, y1, z, nx, ny):
+    """Triangles (normal, a, b, c) of a flat nx x ny rectangle at height z, two per cell."""
+    import numpy as np
+    xs, ys = np.linspace(x0, x1, nx + 1), np.linspace(y0, y1, ny + 1)
+    out = []
+    for i in range(nx):
+        for j in range(ny):
+            a, b = [xs[i], ys[j], z], [xs[i + 1], ys[j], z]
+            c, d = [xs[i + 1], ys[j + 1], z], [xs[i], ys[j + 1], z]
+            out += [([0, 0, -1], a, b, c), ([0, 0, -1], a, c, d)]
+    return out
+
+
+def write_stl_triangles(path, tris, header=b"test mesh"):
+    """A binary STL file of (normal, a, b, c) triangles."""
+    import numpy as np
+    from oscars_amd import io as IO
+    rec = np.zeros(len(tris), dtype=IO._STL_REC)
+    for k, (n, a, b, c) in enumerate(tris):
+        rec[k]["n"], rec[k]["a"], rec[k]["b"], rec[k]["c"] = n, a, b, c
+    with open(path, "wb") as f:
+        f.write(header.ljust(80, b" "))
+        f.write(np.array([len(tris)], dtype="<u4").tobytes())
+        f.write(rec.tobytes())
+
+
+def stl_scene(dirname):
+    """A screen of 36 triangles at 20 m with a 4-triangle blocker at 10 m in
+    front of part of it (coordinates in mm, scale 1e-3).  Returns the two
+    (path, scale, translation) entries."""
+    import os
+    screen, blocker = os.path.join(dirname, "screen.stl"), os.path.join(dirname, "blocker.stl")
+    write_stl_triangles(screen, stl_plate(-10, 10, -5, 5, 20000, 6, 3))
+    write_stl_triangles(blocker, stl_plate(-2, 3, -1, 2, 10000, 2, 1))
+    return [(screen, 1e-3, [0.0, 0.0, 0.0]), (blocker, 1e-3, [0.0005, 0.0, 0.0])]

@@ -118,3 +118,34 @@ def test_efield_vs_time(setup):
                                                                                              e2))
         assert np.max(np.abs(gt - ot2) / ot2) < 1e-15
         assert e2 < 1e-13 and e1 < 1e-9
+
+
+@gpu
+def test_power_density_stl(tmp_path):
+    """STL screen with a blocker: on the device's own trajectory the oracle and
+    the device take the same occlusion decisions and agree to 1e-12 with the
+    same levels; on the oracle's trajectory to 1e-9 (a ray grazing a triangle
+    edge could decide differently there; none does in this scene)."""
+    from oracle.pyoracle import Trajectory
+    from tests.common import stl_scene
+    scene = stl_scene(str(tmp_path))
+    o, s = _pair(planar, 2001)
+    for k, (path, scale, tr) in enumerate(scene):
+        s.add_stl(path, scale=scale, translation=tr, name="s%d" % k)
+    res = s.calculate_power_density_stl(max_level=13)
+    v = np.array([r[1] for r in res])
+    lev = s._last_levels.copy()
+    from oscars_amd import io as IO
+    tri = np.concatenate([IO.read_stl(p, sc, t) for p, sc, t in scene])
+    ov, olev, _ = o.power_density_stl(tri, max_level=13, carry_quirk=False)
+    e1 = rel_err(v, ov, 1e-6)
+    t, pk, ypp = s._device().trajectory(0)
+    o.traj = Trajectory(t, pk, ypp)
+    ov2, olev2, _ = o.power_density_stl(tri, max_level=13, carry_quirk=False)
+    e2 = rel_err(v, ov2, 1e-6)
+    print("power density on STL (%d triangles): %.3e (oracle trajectory), %.3e (device trajectory), levels %s"
+          % (len(v), e1, e2, sorted(set(lev.tolist()))))
+    assert np.array_equal(lev, olev2) and e2 < 1e-12
+    assert e1 < 1e-9
+    with pytest.raises(ValueError, match="already exists"):
+        s.add_stl(scene[0][0], name="s0")

@@ -198,3 +198,29 @@ def test_efield_vs_time(setup):
         rt, re = r.calculate_efield_vs_time(obs)
         ot, oe = o.efield_time(obs, dt)
         assert same(ot, rt) and same(oe, re)
+
+
+def test_power_density_stl(tmp_path):
+    """STL surfaces: the product's reader gives the reference's triangles and
+    centres, the oracle's occluded power density equals the reference's bit
+    for bit, the blocker shadows part of the screen, and the STL writer
+    matches the reference's file after the 80-byte header (whose tail the
+    reference leaves uninitialised)."""
+    from oscars_amd import io as IO
+    from tests.common import stl_scene
+    scene = stl_scene(str(tmp_path))
+    o, r = pair(planar, npoints=2001)
+    for k, (path, scale, tr) in enumerate(scene):
+        r.add_stl(path, scale=scale, translation=tr, name="s%d" % k)
+    tri_r, v_r = r.calculate_power_density_stl(max_level=13, stlofile=str(tmp_path / "ref.stl"))
+    tri = np.concatenate([IO.read_stl(p, s, t) for p, s, t in scene])
+    assert same(tri[:, :9].reshape(-1, 3, 3), tri_r)
+    o.calculate_trajectory()
+    v_o, lev, _ = o.power_density_stl(tri, max_level=13)
+    assert same(v_o, v_r)
+    v_screen, _, _ = o.power_density_stl(tri[:36], max_level=13)
+    shadowed = v_o[:36] < v_screen * (1 - 1e-6)
+    assert 0 < shadowed.sum() < 36 and same(v_o[:36][~shadowed], v_screen[~shadowed])
+    IO.write_stl(str(tmp_path / "mine.stl"), tri, [s for p, s, t in scene for _ in range(len(IO.read_stl(p)))])
+    a, b = (tmp_path / "ref.stl").read_bytes(), (tmp_path / "mine.stl").read_bytes()
+    assert a[80:] == b[80:] and a[:37] == b[:37]
