@@ -493,6 +493,7 @@ def ref_lib():
                                         C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_char_p, C.c_char_p]
         L.ref_set_seed.argtypes = [C.c_void_p, C.c_int]
         L.ref_set_nparticles.argtypes = [C.c_int]
+        L.ref_set_corners.argtypes = [_dp]
         L.ref_write_trajectory.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p]
         L.ref_power_density_points.argtypes = [C.c_void_p, C.c_long, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double,
                                                C.c_int, C.c_int, C.c_int, _dp, _dp, _ip]
@@ -735,6 +736,14 @@ class RefSim:
         self._chk(self.L.ref_total_power(self.h, float(precision), int(max_level), int(max_level_extended), q,
                                          C.byref(out)))
         return out.value
+
+    def set_corners(self, x0x1x2=None):
+        """Rectangles from three corners (x0x1x2) instead of plane and widths; None: off."""
+        if x0x1x2 is None:
+            self.L.ref_set_corners(None)
+        else:
+            a = np.ascontiguousarray(x0x1x2, dtype=np.float64).reshape(9)
+            self.L.ref_set_corners(a.ctypes.data_as(_dp))
 
     def write_trajectory(self, text_path=None, bin_path=None, fmt=""):
         self._chk(self.L.ref_write_trajectory(self.h, text_path.encode() if text_path else None,

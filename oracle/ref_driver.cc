@@ -338,8 +338,25 @@ int ref_spectrum(void* h, double const* obs, int npoints, double e0, double e1, 
   return 0;
 }
 
+// Corners x0, x1, x2 (9 values) for the next rectangles instead of plane and
+// widths, or none (src/OSCARSSR_Python.cc:6633-6660): rotated, then translated.
+static std::vector<double> g_corners;
+void ref_set_corners(double const* x9) {
+  g_corners.assign(x9 ? x9 : (double const*) 0, x9 ? x9 + 9 : (double const*) 0);
+}
+
 static void make_rect(TSurfacePoints_Rectangle& R, const char* plane, int n1, int n2, double w1, double w2,
                       double const* rotations, double const* translation, int normal) {
+  if (g_corners.size() == 9) {
+    TVector3D X[3];
+    for (int i = 0; i < 3; ++i) {
+      X[i] = v3(g_corners.data() + 3 * i);
+      X[i].RotateSelfXYZ(v3(rotations));
+      X[i] += v3(translation);
+    }
+    R.Init(n1, n2, X[0], X[1], X[2], normal);
+    return;
+  }
   R.Init(plane, n1, n2, w1, w2, v3(rotations), v3(translation), normal);
 }
 

@@ -215,6 +215,41 @@ def rectangle(plane, n1, n2, w1, w2, rotations=(0, 0, 0), translation=(0, 0, 0),
     return obs, nv, cont
 
 
+def parallelogram(n1, n2, x0, x1, x2, rotations=(0, 0, 0), translation=(0, 0, 0), normal=0, dim=2):
+    """Observation points, normal and stored coordinates of a surface given by
+    three corners (TSurfacePoints_Rectangle::Init from X0, X1, X2,
+    src/TSurfacePoints_Rectangle.cc:76-123; the binding rotates, then
+    translates the corners, src/OSCARSSR_Python.cc:6633-6655)."""
+    P = []
+    for X in (x0, x1, x2):
+        X = rot_xyz(X, rotations)
+        P.append([X[k] + translation[k] for k in range(3)])
+    X0, X1, X2 = P
+    d1 = [X1[k] - X0[k] for k in range(3)]
+    d2 = [X2[k] - X0[k] for k in range(3)]
+    w1, w2 = math.sqrt(mag2(d1)), math.sqrt(mag2(d2))
+    s1, s2 = w1 / (n1 - 1), w2 / (n2 - 1)
+    a1 = [d1[k] / (n1 - 1) for k in range(3)]
+    a2 = [d2[k] / (n2 - 1) for k in range(3)]
+    nv = unit(cross(a1, a2))
+    if normal == -1:
+        nv = [nv[0] * -1, nv[1] * -1, nv[2] * -1]
+    elif normal not in (0, 1):
+        raise ValueError("fNormal is of the incorrect form.  Must be -1, 0, 1")
+    nv = unit(nv)  # each surface point normalises its normal again (src/TSurfacePoint.cc:16)
+    m1, m2 = math.sqrt(mag2(a1)), math.sqrt(mag2(a2))
+    obs, cont = [], []
+    for i in range(n1 * n2):
+        ix1, ix2 = i // n2, i % n2
+        o = [X0[k] + a1[k] * float(ix1) + a2[k] * float(ix2) for k in range(3)]
+        obs.append(o)
+        if dim == 2:
+            cont.append([s1 * ix1 - m1 * (n1 - 1) / 2., s2 * ix2 - m2 * (n2 - 1) / 2., 0.0])
+        else:
+            cont.append(o)
+    return obs, nv, cont
+
+
 def energies(n, e0, e1):
     """TSpectrumContainer::Init(N, EFirst, ELast)."""
     if n < 1:

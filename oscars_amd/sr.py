@@ -548,10 +548,16 @@ class sr:
             raise ValueError("an entry in 'npoints' is <= 0")
         if abs(normal) > 1:
             raise ValueError("'normal' must be -1, 0, or 1")
-        if x0x1x2 is not None:
-            raise ValueError("'x0x1x2' surfaces are not supported by this implementation")
         rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
         tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        if x0x1x2 is not None:
+            # src/OSCARSSR_Python.cc:6633-6655: three corners, rotated then translated
+            if len(x0x1x2) != 3:
+                raise ValueError("'x0x1x2' must have 3 XYZ points defined correctly")
+            X = [_v3(x, "x0x1x2") for x in x0x1x2]
+            if n1 < 2 or n2 < 2:
+                raise ValueError("'npoints' must be >= 2 in each direction for 'x0x1x2'")
+            return G.parallelogram(n1, n2, X[0], X[1], X[2], rot, tr, normal, dim)
         if width is None or len(width) != 2 or not (width[0] > 0 and width[1] > 0):
             raise ValueError("must specify 'width' (positive only)")
         return G.rectangle(plane, n1, n2, float(width[0]), float(width[1]), rot, tr, normal, dim)

@@ -149,3 +149,26 @@ def test_power_density_stl(tmp_path):
     assert e1 < 1e-9
     with pytest.raises(ValueError, match="already exists"):
         s.add_stl(scene[0][0], name="s0")
+
+
+@gpu
+def test_three_corner_rectangles():
+    """x0x1x2 surfaces for the power density and the flux: points from
+    geometry.parallelogram (pinned to the reference), values against the oracle."""
+    from oscars_amd import geometry as G
+    o, s = _pair(epu, 8001)
+    corners = [[-0.03, -0.02, 20.0], [0.03, -0.01, 20.5], [-0.025, 0.02, 19.8]]
+    kw = dict(rotations=[0.01, 0.2, -0.1], translation=[0.002, 0, 0.3])
+    res = s.calculate_power_density_rectangle(npoints=[7, 5], x0x1x2=corners, normal=-1, dim=3, **kw)
+    obs, nv, cont = G.parallelogram(7, 5, *corners, kw["rotations"], kw["translation"], -1, 3)
+    assert [r[0] for r in res] == cont
+    ov, _, _ = o.power_density_points(np.array(obs), nv, directional=True, carry_quirk=False)
+    e = rel_err(np.array([r[1] for r in res]), ov, 1e-6)
+    small = [[-0.002, -0.002, 20.0], [0.002, -0.0015, 20.0], [-0.0018, 0.002, 20.0]]
+    res = s.calculate_flux_rectangle(energy_eV=700.0, npoints=[6, 5], x0x1x2=small, mode="plain")
+    obs, _, cont = G.parallelogram(6, 5, *small)
+    assert [r[0] for r in res] == cont
+    fv, _, _, _ = o.flux_points(np.array(obs), 700.0, carry_quirk=False)
+    e2 = rel_err(np.array([r[1] for r in res]), fv, 1e-6)
+    print("three-corner surfaces: power density %.3e, flux %.3e" % (e, e2))
+    assert e < 1e-9 and e2 < 1e-9

@@ -224,3 +224,23 @@ def test_power_density_stl(tmp_path):
     IO.write_stl(str(tmp_path / "mine.stl"), tri, [s for p, s, t in scene for _ in range(len(IO.read_stl(p)))])
     a, b = (tmp_path / "ref.stl").read_bytes(), (tmp_path / "mine.stl").read_bytes()
     assert a[80:] == b[80:] and a[:37] == b[:37]
+
+
+@pytest.mark.parametrize("normal", [0, -1])
+def test_three_corner_rectangle(normal):
+    """x0x1x2 surfaces: the product's points, normal and stored coordinates
+    against the reference's rectangle, and the oracle's power density on them."""
+    from oscars_amd import geometry as G
+    o, r = pair(epu, npoints=4001)
+    corners = [[-0.03, -0.02, 20.0], [0.03, -0.01, 20.5], [-0.025, 0.02, 19.8]]
+    kw = dict(rotations=[0.01, 0.2, -0.1], translation=[0.002, 0, 0.3])
+    r.set_corners(corners)
+    try:
+        for dim in (2, 3):
+            xyz, rv, _ = r.calculate_power_density_rectangle([5, 4], [1, 1], normal=normal, dim=dim, **kw)
+            obs, nv, cont = G.parallelogram(5, 4, *corners, kw["rotations"], kw["translation"], normal, dim)
+            assert same(cont, xyz)
+            ov, _, _ = o.power_density_points(np.array(obs), nv, directional=normal != 0)
+            assert same(ov, rv)
+    finally:
+        r.set_corners(None)
