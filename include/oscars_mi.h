@@ -31,7 +31,14 @@ extern "C" {
 
 typedef struct om_context om_context;
 
-enum { OM_FIELD_UNDULATOR = 0, OM_FIELD_EPU = 1, OM_FIELD_GRID = 2 };
+enum {
+  OM_FIELD_UNDULATOR = 0,
+  OM_FIELD_EPU = 1,
+  OM_FIELD_GRID = 2,
+  OM_FIELD_GAUSSIAN = 3,   /* src/TField3D_Gaussian.cc:27-112 */
+  OM_FIELD_UNIFORM = 4,    /* src/TField3D_UniformBox.cc:45-120 */
+  OM_FIELD_QUADRUPOLE = 5  /* src/TField3D_Quadrupole.cc:20-83 */
+};
 /* Kernel families: PLAIN one lane per output point, all FP64 (bit-identical
  * phase; the reference-faithful baseline); TILED one wave per observation
  * point x 64 energies sharing the per-sample amplitude, FP64; MIXED the tiled
@@ -41,10 +48,16 @@ enum { OM_MODE_PLAIN = 0, OM_MODE_TILED = 1, OM_MODE_MIXED = 2, OM_MODE_POINT_MI
 enum { OM_Q_VALUE = 0, OM_Q_PRECISION = 1, OM_Q_LEVEL = 2 };
 
 /* One magnetic field model (src/TField3D_IdealUndulator.cc, TField3D_IdealEPU.cc,
- * TField3D_Grid.cc).  Undulator/EPU: field (fieldB for the EPU's second
+ * TField3D_Grid.cc, TField3D_Gaussian.cc, TField3D_UniformBox.cc,
+ * TField3D_Quadrupole.cc).  Undulator/EPU: field (fieldB for the EPU's second
  * component), period vector, nperiods, centre (translation), phase, taper.
  * Grid: the OSCARS text-file header values (start, step, n per axis; exactly
- * one axis with n > 1), the 3*n samples and the rotations/translation. */
+ * one axis with n > 1), the 3*n samples and the rotations/translation.
+ * Gaussian: peak field, centre (translation), sigma per axis (shape; <= 0:
+ * no fall-off along that axis), rotations.  Uniform: field, centre
+ * (translation), full width per axis (shape; <= 0: unbounded), rotations.
+ * Quadrupole: gradient K [T/m], width along the magnet axis, rotations,
+ * translation.  Time-dependent fields are not supported. */
 typedef struct {
   int type;
   int nperiods;
@@ -59,6 +72,9 @@ typedef struct {
   long grid_n[3];
   const double* grid_data;
   double rotations[3];
+  double shape[3];
+  double K;
+  double width;
 } om_field;
 
 /* One particle: initial position [m], beta, T0 [m] (c*t, as the reference),

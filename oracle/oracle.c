@@ -117,6 +117,54 @@ static void undulator_geometry(orc_field* f) {
   f->und_length = sqrt(v_mag2(f->period)) * (double) (f->nperiods + 2);
 }
 
+static orc_field* new_field(orc_fields* F, int type) {
+  if (F->n >= ORC_MAX_FIELDS) return NULL;
+  orc_field* f = &F->f[F->n++];
+  memset(f, 0, sizeof(*f));
+  f->type = type;
+  return f;
+}
+
+int orc_add_gaussian(orc_fields* F, const double* bfield, const double* center, const double* sigma,
+                     const double* rotations) {
+  orc_field* f = new_field(F, ORC_FIELD_GAUSSIAN);
+  if (!f) return 1;
+  for (int k = 0; k < 3; ++k) {
+    f->field[k] = bfield[k];
+    f->center[k] = center[k];
+    f->shape[k] = sigma[k];
+    f->rot[k] = rotations[k];
+  }
+  v_rot_xyz(f->field, f->rot);
+  return 0;
+}
+
+int orc_add_uniform(orc_fields* F, const double* bfield, const double* width, const double* center,
+                    const double* rotations) {
+  orc_field* f = new_field(F, ORC_FIELD_UNIFORM);
+  if (!f) return 1;
+  for (int k = 0; k < 3; ++k) {
+    f->field[k] = bfield[k];
+    f->center[k] = center[k];
+    f->shape[k] = width[k];
+    f->rot[k] = rotations[k];
+  }
+  v_rot_xyz(f->field, f->rot);
+  return 0;
+}
+
+int orc_add_quadrupole(orc_fields* F, double K, double width, const double* rotations, const double* translation) {
+  orc_field* f = new_field(F, ORC_FIELD_QUADRUPOLE);
+  if (!f) return 1;
+  f->K = K;
+  f->width = width;
+  for (int k = 0; k < 3; ++k) {
+    f->rot[k] = rotations[k];
+    f->trans[k] = translation[k];
+  }
+  return 0;
+}
+
 int orc_add_undulator(orc_fields* F, const double* bfield, const double* period, int nperiods,
                       const double* translation, double phase, double taper) {
   if (F->n >= ORC_MAX_FIELDS) return 1;
@@ -239,9 +287,35 @@ static void field_eval(const orc_field* f, const double* X, double T, double* ou
     }
     return;
   }
-  /* grid, src/TField3D_Grid.cc:153-278 (1-D branches) */
   double x[3] = {X[0], X[1], X[2]};
   v_rot_xyz(x, f->rot);
+  if (f->type == ORC_FIELD_GAUSSIAN || f->type == ORC_FIELD_UNIFORM) {
+    /* src/TField3D_Gaussian.cc:83-112, src/TField3D_UniformBox.cc:98-120 */
+    double RX[3] = {x[0] - f->center[0], x[1] - f->center[1], x[2] - f->center[2]};
+    if (f->type == ORC_FIELD_UNIFORM) {
+      if ((!(f->shape[0] <= 0) && fabs(RX[0]) > fabs(f->shape[0] / 2.)) ||
+          (!(f->shape[1] <= 0) && fabs(RX[1]) > fabs(f->shape[1] / 2.)) ||
+          (!(f->shape[2] <= 0) && fabs(RX[2]) > fabs(f->shape[2] / 2.)))
+        return;
+      out[0] = f->field[0]; out[1] = f->field[1]; out[2] = f->field[2];
+      return;
+    }
+    double Fraction = 1;
+    for (int k = 0; k < 3; ++k)
+      if (f->shape[k] > 0) Fraction *= exp(-pow((RX[k]) / f->shape[k], 2) / 2.);
+    out[0] = Fraction * f->field[0]; out[1] = Fraction * f->field[1]; out[2] = Fraction * f->field[2];
+    return;
+  }
+  if (f->type == ORC_FIELD_QUADRUPOLE) {
+    /* src/TField3D_Quadrupole.cc:62-83 */
+    x[0] -= f->trans[0]; x[1] -= f->trans[1]; x[2] -= f->trans[2];
+    if (fabs(x[2]) > f->width / 2) return;
+    double Ret[3] = {f->K * x[1], f->K * x[0], 0};
+    v_rot_xyz(Ret, f->rot);
+    out[0] = Ret[0]; out[1] = Ret[1]; out[2] = Ret[2];
+    return;
+  }
+  /* grid, src/TField3D_Grid.cc:153-278 (1-D branches) */
   x[0] -= f->trans[0]; x[1] -= f->trans[1]; x[2] -= f->trans[2];
   if (f->nx > 1 && (x[0] <= f->xstart || x[0] >= f->xstop)) return;
   if (f->ny > 1 && (x[1] <= f->ystart || x[1] >= f->ystop)) return;

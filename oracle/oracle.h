@@ -20,7 +20,8 @@ extern "C" {
 #define ORC_MAX_FIELDS 16
 #define ORC_MAX_LEVEL 26 /* TParticleA::kMaxTrajectoryLevel, include/TParticleA.h:72 */
 
-enum { ORC_FIELD_UNDULATOR = 0, ORC_FIELD_EPU = 1, ORC_FIELD_GRID = 2 };
+enum { ORC_FIELD_UNDULATOR = 0, ORC_FIELD_EPU = 1, ORC_FIELD_GRID = 2, ORC_FIELD_GAUSSIAN = 3, ORC_FIELD_UNIFORM = 4,
+       ORC_FIELD_QUADRUPOLE = 5 };
 
 typedef struct {
   int type;
@@ -35,6 +36,9 @@ typedef struct {
   int dimx; /* 1 = X, 2 = Y, 3 = Z */
   double rot[3], trans[3];
   double* data; /* 3 * n values, owned */
+  /* Gaussian (sigma) / uniform box (width) per axis, quadrupole gradient and width */
+  double shape[3];
+  double K, width;
 } orc_field;
 
 typedef struct {
@@ -59,6 +63,12 @@ typedef struct {
 /* ---- fields ---- */
 void orc_fields_init(orc_fields* F);
 void orc_fields_free(orc_fields* F);
+/* src/TField3D_Gaussian.cc:27-60, TField3D_UniformBox.cc:45-86, TField3D_Quadrupole.cc:20-40 */
+int orc_add_gaussian(orc_fields* F, const double* bfield, const double* center, const double* sigma,
+                     const double* rotations);
+int orc_add_uniform(orc_fields* F, const double* bfield, const double* width, const double* center,
+                    const double* rotations);
+int orc_add_quadrupole(orc_fields* F, double K, double width, const double* rotations, const double* translation);
 int orc_add_undulator(orc_fields* F, const double* bfield, const double* period, int nperiods,
                       const double* translation, double phase, double taper);
 int orc_add_epu(orc_fields* F, const double* bfieldA, const double* bfieldB, const double* period, int nperiods,

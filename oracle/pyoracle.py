@@ -51,6 +51,7 @@ class orc_field(C.Structure):
         ("dimx", C.c_int),
         ("rot", C.c_double * 3), ("trans", C.c_double * 3),
         ("data", _dp),
+        ("shape", C.c_double * 3), ("K", C.c_double), ("width", C.c_double),
     ]
 
 
@@ -87,6 +88,9 @@ def oracle_lib():
         L.orc_add_undulator.argtypes = [C.c_void_p, _dp, _dp, C.c_int, _dp, C.c_double, C.c_double]
         L.orc_add_epu.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, _dp, C.c_double, C.c_double]
         L.orc_add_grid.argtypes = [C.c_void_p, _dp, _dp, _lp, _dp, _dp, _dp]
+        L.orc_add_gaussian.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_add_uniform.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_add_quadrupole.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         L.orc_get_b.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.orc_particle_electron.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_double]
         L.orc_particle_set.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
@@ -205,6 +209,18 @@ class Oracle:
     def add_bfield_epu(self, bfieldA, bfieldB, period, nperiods, phase=0.0, translation=(0, 0, 0), taper=0.0):
         self.L.orc_add_epu(C.byref(self.F), _v3(bfieldA), _v3(bfieldB), _v3(period), int(nperiods),
                            _v3(translation), float(phase), float(taper))
+        self.traj = None
+
+    def add_bfield_gaussian(self, bfield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self.L.orc_add_gaussian(C.byref(self.F), _v3(bfield), _v3(translation), _v3(sigma), _v3(rotations))
+        self.traj = None
+
+    def add_bfield_uniform(self, bfield, width=(0, 0, 0), rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self.L.orc_add_uniform(C.byref(self.F), _v3(bfield), _v3(width), _v3(translation), _v3(rotations))
+        self.traj = None
+
+    def add_bfield_quadrupole(self, K, width, rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self.L.orc_add_quadrupole(C.byref(self.F), float(K), float(width), _v3(rotations), _v3(translation))
         self.traj = None
 
     def add_bfield_grid(self, start, step, npts, samples, rotations=(0, 0, 0), translation=(0, 0, 0)):
@@ -469,6 +485,9 @@ def ref_lib():
         L.ref_add_epu.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, _dp, C.c_double, C.c_double]
         L.ref_add_grid_file.argtypes = [C.c_void_p, C.c_char_p, _dp, _dp]
         L.ref_set_beam.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_double]
+        L.ref_add_gaussian.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.ref_add_uniform.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.ref_add_quadrupole.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         L.ref_set_particle.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
         L.ref_get_particle.argtypes = [C.c_void_p, _dp]
         L.ref_set_ctstartstop.argtypes = [C.c_void_p, C.c_double, C.c_double]
@@ -593,6 +612,15 @@ class RefSim:
     def add_bfield_epu(self, bfieldA, bfieldB, period, nperiods, phase=0.0, translation=(0, 0, 0), taper=0.0):
         self._chk(self.L.ref_add_epu(self.h, _v3(bfieldA), _v3(bfieldB), _v3(period), int(nperiods),
                                      _v3(translation), float(phase), float(taper)))
+
+    def add_bfield_gaussian(self, bfield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self._chk(self.L.ref_add_gaussian(self.h, _v3(bfield), _v3(translation), _v3(sigma), _v3(rotations)))
+
+    def add_bfield_uniform(self, bfield, width=(0, 0, 0), rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self._chk(self.L.ref_add_uniform(self.h, _v3(bfield), _v3(width), _v3(translation), _v3(rotations)))
+
+    def add_bfield_quadrupole(self, K, width, rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self._chk(self.L.ref_add_quadrupole(self.h, float(K), float(width), _v3(rotations), _v3(translation)))
 
     def add_bfield_file(self, ifile, iformat="OSCARS", rotations=(0, 0, 0), translation=(0, 0, 0)):
         self._chk(self.L.ref_add_grid_file(self.h, ifile.encode(), _v3(rotations), _v3(translation)))

@@ -19,6 +19,9 @@
 #include "TField3D_IdealUndulator.h"
 #include "TField3D_IdealEPU.h"
 #include "TField3D_Grid.h"
+#include "TField3D_Gaussian.h"
+#include "TField3D_UniformBox.h"
+#include "TField3D_Quadrupole.h"
 #include "TSurfacePoints_Rectangle.h"
 #include "TSurfacePoints_3D.h"
 #include "TSpectrumContainer.h"
@@ -82,6 +85,34 @@ int ref_add_grid_file(void* h, const char* path, double const* rotations, double
 }
 
 // set_particle_beam(type='electron', energy_GeV=E, x0=..., d0=..., current=I)
+// add_bfield_gaussian / add_bfield_uniform / add_bfield_quadrupole
+// (src/OSCARSSR_Python.cc: the constructors as the binding calls them)
+int ref_add_gaussian(void* h, double const* bfield, double const* translation, double const* sigma,
+                     double const* rotations) {
+  try {
+    static_cast<OSCARSSR*>(h)->AddMagneticField(
+        (TField*) new TField3D_Gaussian(v3(bfield), v3(translation), v3(sigma), v3(rotations), 0, 0, 0, ""));
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+int ref_add_uniform(void* h, double const* bfield, double const* width, double const* translation,
+                    double const* rotations) {
+  try {
+    static_cast<OSCARSSR*>(h)->AddMagneticField(
+        (TField*) new TField3D_UniformBox(v3(bfield), v3(width), v3(translation), v3(rotations), 0, 0, 0, ""));
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
+int ref_add_quadrupole(void* h, double K, double width, double const* rotations, double const* translation) {
+  try {
+    static_cast<OSCARSSR*>(h)->AddMagneticField(
+        (TField*) new TField3D_Quadrupole(K, width, v3(rotations), v3(translation), 0, 0, 0, ""));
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
 int ref_set_beam(void* h, double energy_GeV, double const* x0, double const* d0, double current) {
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboscars_mi.so")
 
-OM_FIELD_UNDULATOR, OM_FIELD_EPU, OM_FIELD_GRID = 0, 1, 2
+OM_FIELD_UNDULATOR, OM_FIELD_EPU, OM_FIELD_GRID, OM_FIELD_GAUSSIAN, OM_FIELD_UNIFORM, OM_FIELD_QUADRUPOLE = range(6)
 OM_MODE_PLAIN, OM_MODE_TILED, OM_MODE_MIXED, OM_MODE_POINT_MIXED = 0, 1, 2, 3
 MODES = {"plain": OM_MODE_PLAIN, "tiled": OM_MODE_TILED, "mixed": OM_MODE_MIXED, "point_mixed": OM_MODE_POINT_MIXED}
 
@@ -27,7 +27,8 @@ class om_field(C.Structure):
                 ("field", C.c_double * 3), ("fieldB", C.c_double * 3), ("period", C.c_double * 3),
                 ("translation", C.c_double * 3), ("phase", C.c_double), ("taper", C.c_double),
                 ("grid_start", C.c_double * 3), ("grid_step", C.c_double * 3), ("grid_n", C.c_long * 3),
-                ("grid_data", _dp), ("rotations", C.c_double * 3)]
+                ("grid_data", _dp), ("rotations", C.c_double * 3), ("shape", C.c_double * 3), ("K", C.c_double),
+                ("width", C.c_double)]
 
 
 class om_particle(C.Structure):

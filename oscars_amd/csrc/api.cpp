@@ -406,6 +406,28 @@ om::FieldDev make_field(om_context* ctx, const om_field& f) {
       d.rot_cos[k] = std::cos(f.rotations[k]);
       d.trans[k] = f.translation[k];
     }
+  } else if (f.type == OM_FIELD_GAUSSIAN || f.type == OM_FIELD_UNIFORM || f.type == OM_FIELD_QUADRUPOLE) {
+    double v[3] = {f.field[0], f.field[1], f.field[2]};
+    for (int k = 0; k < 3; ++k) {
+      d.rot[k] = f.rotations[k];
+      d.rot_sin[k] = std::sin(f.rotations[k]);
+      d.rot_cos[k] = std::cos(f.rotations[k]);
+      d.trans[k] = f.translation[k];
+      d.shape[k] = f.shape[k];
+    }
+    // the constructors rotate the field once (src/TField3D_Gaussian.cc:44, TField3D_UniformBox.cc:66)
+    double yy = v[1];
+    v[1] = d.rot_cos[0] * yy - d.rot_sin[0] * v[2];
+    v[2] = d.rot_sin[0] * yy + d.rot_cos[0] * v[2];
+    double zz = v[2];
+    v[2] = d.rot_cos[1] * zz - d.rot_sin[1] * v[0];
+    v[0] = d.rot_sin[1] * zz + d.rot_cos[1] * v[0];
+    double xx = v[0];
+    v[0] = d.rot_cos[2] * xx - d.rot_sin[2] * v[1];
+    v[1] = d.rot_sin[2] * xx + d.rot_cos[2] * v[1];
+    for (int k = 0; k < 3; ++k) d.field[k] = v[k];
+    d.K = f.K;
+    d.width = f.width;
   } else {
     throw std::invalid_argument("unknown field type");
   }
@@ -1560,6 +1582,9 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
     if (level < 0 || level > 20) throw std::out_of_range("adjoint level must be in 0..20");
     if (nfields < 0 || (nfields > 0 && !fields)) throw std::invalid_argument("fields");
     if (grad_field < -1 || grad_field >= nfields) throw std::out_of_range("grad_field out of range");
+    for (int i = 0; i < nfields; ++i)
+      if (fields[i].type > OM_FIELD_GRID)
+        throw std::invalid_argument("the reverse pass supports undulator, EPU and grid fields only");
     for (int i = 0; i < nfields; ++i)
       if (fields[i].type == OM_FIELD_GRID &&
           (fields[i].rotations[0] != 0 || fields[i].rotations[1] != 0 || fields[i].rotations[2] != 0))

@@ -8,7 +8,7 @@ namespace om {
 constexpr int kMaxLevel = 26;  // TParticleA::kMaxTrajectoryLevel (include/TParticleA.h:72)
 constexpr int kMaxFields = 16;
 
-enum FieldType : int { kUndulator = 0, kEPU = 1, kGrid = 2 };
+enum FieldType : int { kUndulator = 0, kEPU = 1, kGrid = 2, kGaussian = 3, kUniform = 4, kQuadrupole = 5 };
 
 // One magnetic field model, evaluated on the device inside the RK4 kernel.
 struct FieldDev {
@@ -27,6 +27,11 @@ struct FieldDev {
   double rot[3], trans[3];
   double rot_sin[3], rot_cos[3];
   const double* data;  // device pointer, 3 per sample
+  // Gaussian / uniform box / quadrupole: field (rotated on the host) in field[],
+  // centre in center[], sigma or full width per axis (<= 0: axis ignored),
+  // gradient and width along the axis of a quadrupole, its translation in trans[]
+  double shape[3];
+  double K, width;
 };
 
 // One particle's trajectory request (all per particle so that a batch can have

@@ -62,6 +62,33 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
   }
   double X[3] = {x, y, z};
   rot_xyz(X, f);
+  if (f.type == kGaussian || f.type == kUniform) {
+    // src/TField3D_Gaussian.cc:90-109, src/TField3D_UniformBox.cc:105-117: the
+    // point rotated by the model's angles, relative to its centre
+    double R0 = X[0] - f.center[0], R1 = X[1] - f.center[1], R2 = X[2] - f.center[2];
+    if (f.type == kUniform) {
+      if ((f.shape[0] > 0 && fabs(R0) > fabs(f.shape[0] / 2.)) || (f.shape[1] > 0 && fabs(R1) > fabs(f.shape[1] / 2.)) ||
+          (f.shape[2] > 0 && fabs(R2) > fabs(f.shape[2] / 2.)))
+        return;
+      out[0] = f.field[0]; out[1] = f.field[1]; out[2] = f.field[2];
+      return;
+    }
+    double frac = 1;
+    if (f.shape[0] > 0) { double u = R0 / f.shape[0]; frac *= exp(-(u * u) / 2.); }
+    if (f.shape[1] > 0) { double u = R1 / f.shape[1]; frac *= exp(-(u * u) / 2.); }
+    if (f.shape[2] > 0) { double u = R2 / f.shape[2]; frac *= exp(-(u * u) / 2.); }
+    out[0] = frac * f.field[0]; out[1] = frac * f.field[1]; out[2] = frac * f.field[2];
+    return;
+  }
+  if (f.type == kQuadrupole) {
+    // src/TField3D_Quadrupole.cc:66-80: (K y, K x, 0) inside the width, rotated back
+    X[0] -= f.trans[0]; X[1] -= f.trans[1]; X[2] -= f.trans[2];
+    if (fabs(X[2]) > f.width / 2) return;
+    double V[3] = {f.K * X[1], f.K * X[0], 0};
+    rot_xyz(V, f);
+    out[0] = V[0]; out[1] = V[1]; out[2] = V[2];
+    return;
+  }
   X[0] -= f.trans[0]; X[1] -= f.trans[1]; X[2] -= f.trans[2];
   if (f.n[0] > 1 && (X[0] <= f.start[0] || X[0] >= f.stop[0])) return;
   if (f.n[1] > 1 && (X[1] <= f.start[1] || X[1] >= f.stop[1])) return;

@@ -157,6 +157,53 @@ class sr:
         self._fields.append(f)
         self._traj_valid = False
 
+    def _simple_field(self, ftype, name, frequency, rotations, translation):
+        if frequency != 0:
+            raise ValueError("time-dependent fields (frequency != 0) are not supported by this implementation")
+        if len(name) > 0 and name[0] == "_":
+            raise ValueError("'name' cannot begin with '_'.  This is reserved for internal use.  Please pick a "
+                             "different name")
+        f = _lib.om_field()
+        f.type = ftype
+        f.rotations[:] = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+        f.translation[:] = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        return f
+
+    def add_bfield_gaussian(self, bfield, sigma, rotations=None, translation=None, frequency=0, frequency_phase=0,
+                            time_offset=0, name=""):
+        """Gaussian field bfield * prod exp(-(x_k / sigma_k)^2 / 2) about the
+        translation; sigma_k <= 0 leaves axis k flat (src/OSCARSSR_Python.cc,
+        add_bfield_gaussian; src/TField3D_Gaussian.cc:27-112)."""
+        B, S = _v3(bfield, "bfield"), _v3(sigma, "sigma")
+        f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, rotations, translation)
+        f.field[:] = B
+        f.shape[:] = S
+        self._fields.append(f)
+        self._traj_valid = False
+
+    def add_bfield_uniform(self, bfield, width=None, rotations=None, translation=None, frequency=0,
+                           frequency_phase=0, time_offset=0, name=""):
+        """Uniform field inside a box of full widths `width` about the
+        translation (a width <= 0 leaves that axis unbounded)
+        (src/TField3D_UniformBox.cc:45-120)."""
+        B = _v3(bfield, "bfield")
+        W = _v3(width, "width") if width is not None and len(width) != 0 else [0.0, 0.0, 0.0]
+        f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, rotations, translation)
+        f.field[:] = B
+        f.shape[:] = W
+        self._fields.append(f)
+        self._traj_valid = False
+
+    def add_bfield_quadrupole(self, K, width, rotations=None, translation=None, frequency=0, frequency_phase=0,
+                              time_offset=0, name=""):
+        """Quadrupole (K y, K x, 0) [T] within |z| <= width / 2 of its frame
+        (src/TField3D_Quadrupole.cc:20-83)."""
+        f = self._simple_field(_lib.OM_FIELD_QUADRUPOLE, name, frequency, rotations, translation)
+        f.K = float(K)
+        f.width = float(width)
+        self._fields.append(f)
+        self._traj_valid = False
+
     def add_bfield_grid(self, start, step, npoints, samples, rotations=None, translation=None):
         """Regular 1-D field map from header values and a (n, 3) sample array
         (what add_bfield_file reads from an OSCARS-format file)."""

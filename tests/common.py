@@ -97,3 +97,15 @@ def stl_scene(dirname):
     write_stl_triangles(screen, stl_plate(-10, 10, -5, 5, 20000, 6, 3))
     write_stl_triangles(blocker, stl_plate(-2, 3, -1, 2, 10000, 2, 1))
     return [(screen, 1e-3, [0.0, 0.0, 0.0]), (blocker, 1e-3, [0.0005, 0.0, 0.0])]
+
+
+def other_fields(s, npoints=4001):
+    """Gaussian, uniform-box and quadrupole fields together, rotated and
+    translated (3 GeV electron from z = -1 m)."""
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_gaussian(bfield=[0, 0.8, 0.05], sigma=[0, 0, 0.1], rotations=[0.01, 0.02, 0.3],
+                          translation=[0.0, 0.0, -0.3])
+    s.add_bfield_uniform(bfield=[0.1, 0.3, 0], width=[0, 0, 0.4], rotations=[0, 0.05, 0], translation=[0, 0, 0.4])
+    s.add_bfield_quadrupole(K=2.0, width=0.3, rotations=[0, 0, 0.1], translation=[0.001, 0, 0.0])
+    s.set_ctstartstop(0, 2)
+    s.set_npoints_trajectory(npoints)

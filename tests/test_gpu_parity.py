@@ -250,3 +250,31 @@ def test_output_files(gpu_planar, tmp_path):
     assert len(lines) == 20 and len(lines[0].split()) == 3
     avg = gpu_planar.average_flux(ifiles=[str(tmp_path / "f.txt")] * 2)
     assert np.allclose([a[1] for a in avg], [float(l.split()[2]) for l in lines], rtol=0, atol=0)
+
+
+@gpu
+def test_gaussian_uniform_quadrupole_fields():
+    """Gaussian, uniform-box and quadrupole fields on the device: the
+    trajectory against the oracle (the device's exp and sin differ from
+    glibc's in the last bits) and the spectrum to 1e-9, levels identical."""
+    from oracle.pyoracle import Oracle
+    from oscars_amd.sr import sr
+    from tests.common import other_fields
+    o = Oracle(nthreads=8)
+    other_fields(o)
+    T = o.calculate_trajectory()
+    s = sr()
+    other_fields(s)
+    s.calculate_trajectory()
+    t, p, _ = s._device().trajectory(0)
+    scale = np.maximum(np.max(np.abs(T.p), axis=0), 1e-300)
+    et = float(np.max(np.abs(p - T.p) / scale))
+    assert np.array_equal(t, T.t)
+    e = np.linspace(10.0, 2000.0, 40)
+    ov, olev, _, _ = o.spectrum([0, 0, 30], e, carry_quirk=False)
+    res = s.calculate_spectrum(obs=[0, 0, 30], energy_points_eV=list(e), mode="plain")
+    v = np.array([r[1] for r in res])
+    es = rel_err(v, ov, 1e-6)
+    print("other fields: trajectory %.3e, spectrum %.3e" % (et, es))
+    assert et < 1e-11 and es < 1e-9
+    assert np.array_equal(s._last_levels, olev)

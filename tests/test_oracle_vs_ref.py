@@ -244,3 +244,19 @@ def test_three_corner_rectangle(normal):
             assert same(ov, rv)
     finally:
         r.set_corners(None)
+
+
+def test_gaussian_uniform_quadrupole_fields():
+    """The oracle's Gaussian, uniform-box and quadrupole models: fields at
+    points inside and outside each, the trajectory and a spectrum, bit for bit."""
+    from tests.common import other_fields
+    o, r = pair(other_fields)
+    for x in ([0.001, 0.002, -0.3], [0.0003, -0.0001, 0.41], [0.002, 0.003, 0.1], [0.002, 0.003, 0.16],
+              [0.0, 0.0, 0.9]):
+        assert same(o.get_bfield(x), r.get_bfield(x))
+    rt, rp, rypp = r.calculate_trajectory()
+    T = o.calculate_trajectory()
+    assert same(T.t, rt) and same(T.p, rp) and same(T.ypp, rypp)
+    e, v, lev, conv, _ = o.calculate_spectrum([0, 0, 30], npoints=20, energy_range_eV=[10, 2000])
+    re, rv, _ = r.calculate_spectrum([0, 0, 30], npoints=20, energy_range_eV=[10, 2000])
+    assert same(v, rv)
