@@ -51,8 +51,9 @@ enum { OM_Q_VALUE = 0, OM_Q_PRECISION = 1, OM_Q_LEVEL = 2 };
  * TField3D_Grid.cc, TField3D_Gaussian.cc, TField3D_UniformBox.cc,
  * TField3D_Quadrupole.cc).  Undulator/EPU: field (fieldB for the EPU's second
  * component), period vector, nperiods, centre (translation), phase, taper.
- * Grid: the OSCARS text-file header values (start, step, n per axis; exactly
- * one axis with n > 1), the 3*n samples and the rotations/translation.
+ * Grid: the OSCARS text-file header values (start, step, n per axis; one,
+ * two or three axes with n > 1, samples with z fastest, then y, then x), the
+ * 3*n samples and the rotations/translation.
  * Gaussian: peak field, centre (translation), sigma per axis (shape; <= 0:
  * no fall-off along that axis), rotations.  Uniform: field, centre
  * (translation), full width per axis (shape; <= 0: unbounded), rotations.
@@ -75,6 +76,10 @@ typedef struct {
   double shape[3];
   double K;
   double width;
+  /* grid: 0 = the OSCARS text header (start re-centred as the reader does),
+   * 1 = start and grid_stop taken as given (a resampled OSCARS1D table) */
+  int grid_exact;
+  double grid_stop[3];
 } om_field;
 
 /* One particle: initial position [m], beta, T0 [m] (c*t, as the reference),

@@ -199,13 +199,26 @@ int orc_add_epu(orc_fields* F, const double* bfieldA, const double* bfieldB, con
 }
 
 /* src/TField3D_Grid.cc:353-548 (OSCARS text format, no scaling) */
+/* A resampled OSCARS1D table (src/TField3D_Grid.cc:572-873): start and stop as given. */
+int orc_add_grid_exact(orc_fields* F, const double* start, const double* stop, const double* step, const long* npts,
+                       const double* samples, const double* rotations, const double* translation) {
+  double zero[3] = {0, 0, 0};
+  int rc = orc_add_grid(F, zero, step, npts, samples, rotations, translation);
+  if (rc) return rc;
+  orc_field* f = &F->f[F->n - 1];
+  f->xstart = npts[0] > 1 ? start[0] : 0; f->xstop = npts[0] > 1 ? stop[0] : 0;
+  f->ystart = npts[1] > 1 ? start[1] : 0; f->ystop = npts[1] > 1 ? stop[1] : 0;
+  f->zstart = npts[2] > 1 ? start[2] : 0; f->zstop = npts[2] > 1 ? stop[2] : 0;
+  return 0;
+}
+
 int orc_add_grid(orc_fields* F, const double* start_in, const double* step_in, const long* npts,
                  const double* samples, const double* rotations, const double* translation) {
   if (F->n >= ORC_MAX_FIELDS) return 1;
   int NX = (int) npts[0], NY = (int) npts[1], NZ = (int) npts[2];
   if (NX < 1 || NY < 1) return 2;
   int hasx = NX > 1, hasy = NY > 1, hasz = NZ > 1;
-  if (hasx + hasy + hasz != 1) return 3; /* only 1-D grids are on this path */
+  if (hasx + hasy + hasz < 1) return 3;
   orc_field* f = &F->f[F->n++];
   memset(f, 0, sizeof(*f));
   f->type = ORC_FIELD_GRID;
@@ -222,7 +235,9 @@ int orc_add_grid(orc_fields* F, const double* start_in, const double* step_in, c
   f->xstop = f->xstart + (double) (size_t) (f->nx - 1) * f->xstep;
   f->ystop = f->ystart + (double) (size_t) (f->ny - 1) * f->ystep;
   f->zstop = f->zstart + (double) (size_t) (f->nz - 1) * f->zstep;
-  f->dimx = hasx ? 1 : (hasy ? 2 : 3);
+  /* 1 x, 2 y, 3 z for one axis; 12, 13, 23, 123 for several */
+  if (hasx + hasy + hasz == 1) f->dimx = hasx ? 1 : (hasy ? 2 : 3);
+  else f->dimx = hasx && hasy && hasz ? 123 : (hasx && hasy ? 12 : (hasx ? 13 : 23));
   long n = (long) NX * NY * NZ;
   f->data = (double*) malloc(3 * n * sizeof(double));
   for (long i = 0; i < n; ++i) {
@@ -320,6 +335,43 @@ static void field_eval(const orc_field* f, const double* X, double T, double* ou
   if (f->nx > 1 && (x[0] <= f->xstart || x[0] >= f->xstop)) return;
   if (f->ny > 1 && (x[1] <= f->ystart || x[1] >= f->ystop)) return;
   if (f->nz > 1 && (x[2] <= f->zstart || x[2] >= f->zstop)) return;
+  if (f->dimx > 3) {
+    /* src/TField3D_Grid.cc:174-268 */
+    size_t nx = f->nx > 1 ? (size_t) ((x[0] - f->xstart) / f->xstep) : 0;
+    double dx = f->nx > 1 ? (x[0] - f->xstart) - nx * f->xstep : 0;
+    size_t ny = f->ny > 1 ? (size_t) ((x[1] - f->ystart) / f->ystep) : 0;
+    double dy = f->ny > 1 ? (x[1] - f->ystart) - ny * f->ystep : 0;
+    size_t nz = f->nz > 1 ? (size_t) ((x[2] - f->zstart) / f->zstep) : 0;
+    double dz = f->nz > 1 ? (x[2] - f->zstart) - nz * f->zstep : 0;
+#define GI(ix, iy, iz) (f->data + 3 * ((ix) * f->ny * f->nz + (iy) * f->nz + (iz)))
+#define LERP(r, lo, hi, d, st) for (int k = 0; k < 3; ++k) (r)[k] = (lo)[k] + (d) * ((hi)[k] - (lo)[k]) / (st)
+    double v00[3], v10[3], v01[3], v11[3], v0[3], v1[3], F3[3];
+    if (f->dimx == 123) {
+      LERP(v00, GI(nx, ny, nz), GI(nx + 1, ny, nz), dx, f->xstep);
+      LERP(v10, GI(nx, ny + 1, nz), GI(nx + 1, ny + 1, nz), dx, f->xstep);
+      LERP(v01, GI(nx, ny, nz + 1), GI(nx + 1, ny, nz + 1), dx, f->xstep);
+      LERP(v11, GI(nx, ny + 1, nz + 1), GI(nx + 1, ny + 1, nz + 1), dx, f->xstep);
+      LERP(v0, v00, v10, dy, f->ystep);
+      LERP(v1, v01, v11, dy, f->ystep);
+      LERP(F3, v0, v1, dz, f->zstep);
+    } else if (f->dimx == 12) {
+      LERP(v0, GI(nx, ny, 0), GI(nx + 1, ny, 0), dx, f->xstep);
+      LERP(v1, GI(nx, ny + 1, 0), GI(nx + 1, ny + 1, 0), dx, f->xstep);
+      LERP(F3, v0, v1, dy, f->ystep);
+    } else if (f->dimx == 13) {
+      LERP(v0, GI(nx, 0, nz), GI(nx + 1, 0, nz), dx, f->xstep);
+      LERP(v1, GI(nx, 0, nz + 1), GI(nx + 1, 0, nz + 1), dx, f->xstep);
+      LERP(F3, v0, v1, dz, f->zstep);
+    } else {
+      LERP(v0, GI(0, ny, nz), GI(0, ny + 1, nz), dy, f->ystep);
+      LERP(v1, GI(0, ny, nz + 1), GI(0, ny + 1, nz + 1), dy, f->ystep);
+      LERP(F3, v0, v1, dz, f->zstep);
+    }
+#undef GI
+#undef LERP
+    for (int k = 0; k < 3; ++k) out[k] = F3[k] * cos(k_twopi() * 0 * (T + 0) + 0);
+    return;
+  }
   double coord, start, step;
   if (f->dimx == 1) { coord = x[0]; start = f->xstart; step = f->xstep; }
   else if (f->dimx == 2) { coord = x[1]; start = f->ystart; step = f->ystep; }

@@ -74,6 +74,8 @@ int orc_add_undulator(orc_fields* F, const double* bfield, const double* period,
 int orc_add_epu(orc_fields* F, const double* bfieldA, const double* bfieldB, const double* period, int nperiods,
                 const double* translation, double phase, double taper);
 /* Grid from header values + samples as the OSCARS text reader would produce them. */
+int orc_add_grid_exact(orc_fields* F, const double* start, const double* stop, const double* step, const long* npts,
+                       const double* samples, const double* rotations, const double* translation);
 int orc_add_grid(orc_fields* F, const double* start_in, const double* step_in, const long* npts,
                  const double* samples, const double* rotations, const double* translation);
 void orc_get_b(const orc_fields* F, const double* x, double t, double* out);

@@ -89,6 +89,7 @@ def oracle_lib():
         L.orc_add_epu.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, _dp, C.c_double, C.c_double]
         L.orc_add_grid.argtypes = [C.c_void_p, _dp, _dp, _lp, _dp, _dp, _dp]
         L.orc_add_gaussian.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_add_grid_exact.argtypes = [C.c_void_p, _dp, _dp, _dp, _lp, _dp, _dp, _dp]
         L.orc_add_uniform.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_add_quadrupole.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         L.orc_get_b.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
@@ -235,6 +236,16 @@ class Oracle:
     def add_bfield_file(self, ifile, iformat="OSCARS", rotations=(0, 0, 0), translation=(0, 0, 0)):
         start, step, npts, data = read_oscars_grid(ifile)
         self.add_bfield_grid(start, step, npts, data, rotations, translation)
+
+    def add_bfield_grid_exact(self, start, stop, step, npts, samples, rotations=(0, 0, 0), translation=(0, 0, 0)):
+        """A grid whose start and stop are given as they are (a resampled table)."""
+        s = np.ascontiguousarray(samples, dtype=np.float64).reshape(-1, 3)
+        self._grids.append(s)
+        rc = self.L.orc_add_grid_exact(C.byref(self.F), _v3(start), _v3(stop), _v3(step), (C.c_long * 3)(*npts),
+                                       s.ctypes.data_as(_dp), _v3(rotations), _v3(translation))
+        if rc:
+            raise ValueError("unsupported grid (rc=%d)" % rc)
+        self.traj = None
 
     def get_bfield(self, x):
         out = np.zeros(3)
@@ -486,6 +497,7 @@ def ref_lib():
         L.ref_add_grid_file.argtypes = [C.c_void_p, C.c_char_p, _dp, _dp]
         L.ref_set_beam.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_double]
         L.ref_add_gaussian.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.ref_add_field_file.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, _dp, _dp, _dp, C.c_int]
         L.ref_add_uniform.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.ref_add_quadrupole.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         L.ref_set_particle.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
@@ -513,6 +525,7 @@ def ref_lib():
         L.ref_set_seed.argtypes = [C.c_void_p, C.c_int]
         L.ref_set_nparticles.argtypes = [C.c_int]
         L.ref_set_corners.argtypes = [_dp]
+        L.ref_set_trajectory_calculation.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         L.ref_write_trajectory.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p]
         L.ref_power_density_points.argtypes = [C.c_void_p, C.c_long, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double,
                                                C.c_int, C.c_int, C.c_int, _dp, _dp, _ip]
@@ -612,6 +625,11 @@ class RefSim:
     def add_bfield_epu(self, bfieldA, bfieldB, period, nperiods, phase=0.0, translation=(0, 0, 0), taper=0.0):
         self._chk(self.L.ref_add_epu(self.h, _v3(bfieldA), _v3(bfieldB), _v3(period), int(nperiods),
                                      _v3(translation), float(phase), float(taper)))
+
+    def add_bfield_file_format(self, ifile, iformat="OSCARS", rotations=(0, 0, 0), translation=(0, 0, 0), scale=()):
+        sc = np.ascontiguousarray(list(scale) or [0.0], dtype=np.float64)
+        self._chk(self.L.ref_add_field_file(self.h, ifile.encode(), iformat.encode(), _v3(rotations), _v3(translation),
+                                            sc.ctypes.data_as(_dp), len(scale)))
 
     def add_bfield_gaussian(self, bfield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
         self._chk(self.L.ref_add_gaussian(self.h, _v3(bfield), _v3(translation), _v3(sigma), _v3(rotations)))
@@ -764,6 +782,9 @@ class RefSim:
         self._chk(self.L.ref_total_power(self.h, float(precision), int(max_level), int(max_level_extended), q,
                                          C.byref(out)))
         return out.value
+
+    def set_trajectory_calculation(self, method="RK4", precision=-1):
+        self._chk(self.L.ref_set_trajectory_calculation(self.h, method.encode(), float(precision)))
 
     def set_corners(self, x0x1x2=None):
         """Rectangles from three corners (x0x1x2) instead of plane and widths; None: off."""

@@ -84,6 +84,17 @@ int ref_add_grid_file(void* h, const char* path, double const* rotations, double
   return 0;
 }
 
+// add_bfield_file(ifile, iformat, rotations, translation, scale) (src/OSCARSSR_Python.cc:526-700)
+int ref_add_field_file(void* h, const char* path, const char* format, double const* rotations,
+                       double const* translation, double const* scaling, int nscale) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    std::vector<double> sc(scaling, scaling + nscale);
+    s->AddMagneticField(path, format, v3(rotations), v3(translation), sc, 0, 0, 0, "");
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
 // set_particle_beam(type='electron', energy_GeV=E, x0=..., d0=..., current=I)
 // add_bfield_gaussian / add_bfield_uniform / add_bfield_quadrupole
 // (src/OSCARSSR_Python.cc: the constructors as the binding calls them)
@@ -196,6 +207,14 @@ int ref_set_beam_full(void* h, double energy_GeV, double const* x0, double const
 
 // nparticles= of the calculate_* calls that follow.
 void ref_set_nparticles(int n) { g_nparticles = n; }
+
+// set_trajectory_calculation("RKAS" / "RK4", precision) (src/OSCARSSR.cc:784-800)
+int ref_set_trajectory_calculation(void* h, const char* method, double precision) {
+  try {
+    static_cast<OSCARSSR*>(h)->SetTrajectoryCalculation(method, precision);
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
 
 // sr.set_seed (include/OSCARSPY.h:85-87) and the draws of the global stream.
 void ref_set_seed(void* h, int seed) { static_cast<OSCARSSR*>(h)->SetSeed(seed); }

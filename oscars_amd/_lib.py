@@ -28,7 +28,7 @@ class om_field(C.Structure):
                 ("translation", C.c_double * 3), ("phase", C.c_double), ("taper", C.c_double),
                 ("grid_start", C.c_double * 3), ("grid_step", C.c_double * 3), ("grid_n", C.c_long * 3),
                 ("grid_data", _dp), ("rotations", C.c_double * 3), ("shape", C.c_double * 3), ("K", C.c_double),
-                ("width", C.c_double)]
+                ("width", C.c_double), ("grid_exact", C.c_int), ("grid_stop", C.c_double * 3)]
 
 
 class om_particle(C.Structure):

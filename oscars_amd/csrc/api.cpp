@@ -363,18 +363,29 @@ om::FieldDev make_field(om_context* ctx, const om_field& f) {
     int nx = (int) f.grid_n[0], ny = (int) f.grid_n[1], nz = (int) f.grid_n[2];
     if (nx < 1 || ny < 1 || nz < 1) throw std::out_of_range("invalid number of points in at least one dimension");
     int naxes = (nx > 1) + (ny > 1) + (nz > 1);
-    if (naxes != 1) throw std::invalid_argument("only one-dimensional field grids are supported on the GPU path");
+    if (naxes < 1) throw std::out_of_range("invalid dimensions");
     int nn[3] = {nx, ny, nz};
+    d.axis = -1;
+    d.dims = 0;
     for (int a = 0; a < 3; ++a) {
-      // src/TField3D_Grid.cc:413-437 (no scaling)
-      double middle = f.grid_start[a] + f.grid_step[a] * (nn[a] - 1) / 2.;
-      double start = middle - f.grid_step[a] * (nn[a] - 1) / 2.;
+      double start, stop;
+      if (f.grid_exact) {
+        // src/TField3D_Grid.cc:716-745: a resampled table keeps its first and last points
+        start = nn[a] > 1 ? f.grid_start[a] : 0;
+        stop = nn[a] > 1 ? f.grid_stop[a] : 0;
+      } else {
+        // src/TField3D_Grid.cc:413-451: header start re-centred, stop from the step
+        double middle = f.grid_start[a] + f.grid_step[a] * (nn[a] - 1) / 2.;
+        start = middle - f.grid_step[a] * (nn[a] - 1) / 2.;
+        stop = start + (double) (nn[a] - 1) * f.grid_step[a];
+      }
       d.n[a] = nn[a];
       d.start[a] = start;
       d.step[a] = f.grid_step[a];
       d.inv_step[a] = 1.0 / f.grid_step[a];
-      d.stop[a] = start + (double) (size_t) (nn[a] - 1) * f.grid_step[a];
-      if (nn[a] > 1) d.axis = a;
+      d.stop[a] = stop;
+      if (nn[a] > 1) d.dims |= 1 << a;
+      if (nn[a] > 1 && naxes == 1) d.axis = a;
     }
     long n = (long) nx * ny * nz;
     if (n < 2) throw std::invalid_argument("field grid needs at least two samples");
@@ -1582,9 +1593,13 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
     if (level < 0 || level > 20) throw std::out_of_range("adjoint level must be in 0..20");
     if (nfields < 0 || (nfields > 0 && !fields)) throw std::invalid_argument("fields");
     if (grad_field < -1 || grad_field >= nfields) throw std::out_of_range("grad_field out of range");
-    for (int i = 0; i < nfields; ++i)
+    for (int i = 0; i < nfields; ++i) {
       if (fields[i].type > OM_FIELD_GRID)
         throw std::invalid_argument("the reverse pass supports undulator, EPU and grid fields only");
+      if (fields[i].type == OM_FIELD_GRID &&
+          (fields[i].grid_n[0] > 1) + (fields[i].grid_n[1] > 1) + (fields[i].grid_n[2] > 1) != 1)
+        throw std::invalid_argument("the reverse pass supports one-dimensional grids only");
+    }
     for (int i = 0; i < nfields; ++i)
       if (fields[i].type == OM_FIELD_GRID &&
           (fields[i].rotations[0] != 0 || fields[i].rotations[1] != 0 || fields[i].rotations[2] != 0))

@@ -14,7 +14,7 @@ enum FieldType : int { kUndulator = 0, kEPU = 1, kGrid = 2, kGaussian = 3, kUnif
 struct FieldDev {
   int type;
   int nperiods;
-  int axis;  // grid: 0 x, 1 y, 2 z
+  int axis;  // grid: 0 x, 1 y, 2 z when one-dimensional, else -1
   int pad;
   double field[3], fieldB[3], period[3], center[3];
   double phase, taper;
@@ -32,6 +32,8 @@ struct FieldDev {
   // gradient and width along the axis of a quadrupole, its translation in trans[]
   double shape[3];
   double K, width;
+  int dims;  // grid: axes with more than one sample, bit 0 x, bit 1 y, bit 2 z
+  int pad2;
 };
 
 // One particle's trajectory request (all per particle so that a batch can have

@@ -51,6 +51,54 @@ __device__ __forceinline__ void rot_xyz(double v[3], const FieldDev& f) {
   v[1] = s * xx + c * v[1];
 }
 
+// Two- and three-dimensional grids (src/TField3D_Grid.cc:174-268): the cell
+// index per axis, then linear steps along x, y and z in the reference's order,
+// each ((d * (hi - lo)) / step).  The index is kept inside the map.
+__device__ __forceinline__ long grid_cell(const FieldDev& f, int a, double c, double& d) {
+  if (f.n[a] <= 1) {
+    d = 0;
+    return 0;
+  }
+  long i = (long) (size_t) div_by(c - f.start[a], f.step[a], f.inv_step[a]);
+  if (i > f.n[a] - 2) i = f.n[a] - 2;
+  d = (c - f.start[a]) - (double) i * f.step[a];
+  return i;
+}
+
+__device__ __forceinline__ void lerp3(const double* lo, const double* hi, double d, double step, double istep,
+                                      double* r) {
+  for (int k = 0; k < 3; ++k) r[k] = lo[k] + div_by(d * (hi[k] - lo[k]), step, istep);
+}
+
+__device__ __noinline__ void grid_multi(const FieldDev& f, const double* X, double out[3]) {
+  double dx, dy, dz;
+  long nx = grid_cell(f, 0, X[0], dx), ny = grid_cell(f, 1, X[1], dy), nz = grid_cell(f, 2, X[2], dz);
+  const long NY = f.n[1], NZ = f.n[2];
+  auto at = [&](long ix, long iy, long iz) { return f.data + 3 * (ix * NY * NZ + iy * NZ + iz); };
+  double v00[3], v10[3], v01[3], v11[3], v0[3], v1[3];
+  if (f.dims == 7) {
+    lerp3(at(nx, ny, nz), at(nx + 1, ny, nz), dx, f.step[0], f.inv_step[0], v00);
+    lerp3(at(nx, ny + 1, nz), at(nx + 1, ny + 1, nz), dx, f.step[0], f.inv_step[0], v10);
+    lerp3(at(nx, ny, nz + 1), at(nx + 1, ny, nz + 1), dx, f.step[0], f.inv_step[0], v01);
+    lerp3(at(nx, ny + 1, nz + 1), at(nx + 1, ny + 1, nz + 1), dx, f.step[0], f.inv_step[0], v11);
+    lerp3(v00, v10, dy, f.step[1], f.inv_step[1], v0);
+    lerp3(v01, v11, dy, f.step[1], f.inv_step[1], v1);
+    lerp3(v0, v1, dz, f.step[2], f.inv_step[2], out);
+  } else if (f.dims == 3) {
+    lerp3(at(nx, ny, 0), at(nx + 1, ny, 0), dx, f.step[0], f.inv_step[0], v0);
+    lerp3(at(nx, ny + 1, 0), at(nx + 1, ny + 1, 0), dx, f.step[0], f.inv_step[0], v1);
+    lerp3(v0, v1, dy, f.step[1], f.inv_step[1], out);
+  } else if (f.dims == 5) {
+    lerp3(at(nx, 0, nz), at(nx + 1, 0, nz), dx, f.step[0], f.inv_step[0], v0);
+    lerp3(at(nx, 0, nz + 1), at(nx + 1, 0, nz + 1), dx, f.step[0], f.inv_step[0], v1);
+    lerp3(v0, v1, dz, f.step[2], f.inv_step[2], out);
+  } else {  // 6: y and z
+    lerp3(at(0, ny, nz), at(0, ny + 1, nz), dy, f.step[1], f.inv_step[1], v0);
+    lerp3(at(0, ny, nz + 1), at(0, ny + 1, nz + 1), dy, f.step[1], f.inv_step[1], v1);
+    lerp3(v0, v1, dz, f.step[2], f.inv_step[2], out);
+  }
+}
+
 __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y, double z, double out[3]) {
   out[0] = out[1] = out[2] = 0;
   if (f.type == kUndulator || f.type == kEPU) {
@@ -93,6 +141,10 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
   if (f.n[0] > 1 && (X[0] <= f.start[0] || X[0] >= f.stop[0])) return;
   if (f.n[1] > 1 && (X[1] <= f.start[1] || X[1] >= f.stop[1])) return;
   if (f.n[2] > 1 && (X[2] <= f.start[2] || X[2] >= f.stop[2])) return;
+  if (f.axis < 0) {
+    grid_multi(f, X, out);
+    return;
+  }
   int a = f.axis;
   double coord = a == 0 ? X[0] : (a == 1 ? X[1] : X[2]);
   double start = a == 0 ? f.start[0] : (a == 1 ? f.start[1] : f.start[2]);

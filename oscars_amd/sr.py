@@ -222,13 +222,45 @@ class sr:
         self._fields.append(f)
         self._traj_valid = False
 
+    def _add_grid_exact(self, start, stop, step, npoints, samples, rotations, translation):
+        s = np.ascontiguousarray(samples, dtype=np.float64).reshape(-1, 3)
+        f = _lib.om_field()
+        f.type = _lib.OM_FIELD_GRID
+        f.grid_exact = 1
+        f.grid_start[:] = [float(v) for v in start]
+        f.grid_stop[:] = [float(v) for v in stop]
+        f.grid_step[:] = [float(v) for v in step]
+        f.grid_n[:] = [int(v) for v in npoints]
+        f.grid_data = s.ctypes.data_as(_lib._dp)
+        f.rotations[:] = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+        f.translation[:] = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        self._grids.append(s)
+        self._fields.append(f)
+        self._traj_valid = False
+
     def add_bfield_file(self, ifile, iformat="OSCARS", rotations=None, translation=None, scale=None, name=""):
-        if iformat.upper() != "OSCARS":
+        """Field map from a file: 'OSCARS' (regular 1-, 2- or 3-D grid; scale =
+        [x, y, z step factors, Bx, By, Bz factors]) or 'OSCARS1D <columns>'
+        (an irregular 1-D table resampled at 10000 points per metre)
+        (src/TField3D_Grid.cc:353-873)."""
+        fmt = iformat.upper()
+        sc = [float(v) for v in scale] if scale else []
+        if fmt == "OSCARS":
+            start, step, npts, data = read_oscars_field_file(ifile)
+            if not sc:
+                self.add_bfield_grid(start, step, npts, data, rotations, translation)
+                return
+            g = field_grid_scaled(start, step, npts, data, sc)
+            self._add_grid_exact(g[0], g[1], g[2], npts, g[3], rotations, translation)
+        elif fmt[:8] == "OSCARS1D":
+            axis, first, last, step, n, data = read_oscars1d_field_file(ifile, iformat[8:], sc)
+            start, stop, steps, npts = [0.0] * 3, [0.0] * 3, [0.0] * 3, [1, 1, 1]
+            start[axis], stop[axis], steps[axis], npts[axis] = first, last, step, n
+            self._add_grid_exact(start, stop, steps, npts, data, rotations, translation)
+        elif fmt in ("SPECTRA", "SRW", "BINARY"):
+            raise ValueError("the %s field-file format is not supported by this implementation" % fmt)
+        else:
             raise ValueError("incorrect format given")
-        if scale:
-            raise ValueError("'scale' is not supported by this implementation")
-        start, step, npts, data = read_oscars_field_file(ifile)
-        self.add_bfield_grid(start, step, npts, data, rotations, translation)
 
     def clear_bfields(self):
         # src/OSCARSSR.cc:186-196: the current trajectory stays until a field is added
@@ -320,6 +352,21 @@ class sr:
         b = self._beam
         self._particle = G.Particle(b["q"], b["m"], x0, beta0, t0, b["current"])
         self._traj_valid = False
+
+    def set_trajectory_calculation(self, method="RK4", precision=-1):
+        """The trajectory integrator (src/OSCARSSR.cc:784-800).  RK4 is the one
+        here.  The reference's RKAS stores the start time twice and its
+        trajectory spline is then NaN (and it stops with "stepsize underflow"
+        on ideal undulators), so it is refused rather than imitated."""
+        m = method.upper()
+        if m == "RK4":
+            if precision != -1:
+                raise ValueError("RK4 Trajectory does not use the precision parameter and you are attempting to "
+                                 "set it")
+            return
+        if m == "RKAS":
+            raise ValueError("RKAS trajectories are not supported by this implementation (use RK4)")
+        raise ValueError("Method requested is invalid.  Try again")
 
     def set_ctstartstop(self, start, stop):
         # src/OSCARSSR.cc:637-647.  As in the reference, a trajectory already
@@ -872,6 +919,89 @@ class sr:
     def average_power_density(ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
         """Average power-density files of the same points (src/OSCARSSR_Python.cc:8694-8790)."""
         return IO.average_points(ifiles, bifiles, ofile, bofile, dim)
+
+
+def field_grid_scaled(start, step, npts, data, scaling):
+    """Start, stop, step and samples of an OSCARS grid read with scaling
+    factors (src/TField3D_Grid.cc:405-451, 520-530): steps scaled, the start
+    re-centred about the unscaled middle, the field components scaled."""
+    st, sp, sc = [], [], []
+    for a in range(3):
+        s_in, d_in, n = float(start[a]), float(step[a]), int(npts[a])
+        d = d_in * scaling[a] if len(scaling) > a else d_in
+        middle = s_in + d_in * (n - 1) / 2.
+        s0 = middle - d * (n - 1) / 2.
+        st.append(s0)
+        sp.append(s0 + (n - 1) * d)
+        sc.append(d)
+    f = np.array(data, dtype=np.float64).reshape(-1, 3).copy()
+    for k in range(3):
+        if len(scaling) > 3 + k and scaling[3 + k] != 1:
+            f[:, k] *= scaling[3 + k]
+    return st, sp, sc, f
+
+
+def read_oscars1d_field_file(path, columns, scaling=(), comment="#"):
+    """An irregular 1-D field table, 'OSCARS1D' format with its column names
+    (X, Y or Z and any of Bx By Bz; Ex/Fx... are the same columns), resampled
+    on 10000 points per metre by linear interpolation as
+    src/TField3D_Grid.cc:572-873 does.  Returns (axis, first, last, step, n,
+    samples[n, 3])."""
+    order, axis, has = [], None, [False, False, False]
+    for c in columns.split():
+        if len(order) > 3:
+            raise ValueError("spatial or B-field dimensions are too large(index>3)")
+        if c in ("X", "Y", "Z"):
+            axis = "XYZ".index(c)
+            order.append(0)
+        elif c in ("Bx", "Ex", "Fx", "By", "Ey", "Fy", "Bz", "Ez", "Fz"):
+            k = "xyz".index(c[1])
+            has[k] = True
+            order.append(1 + k)
+        else:
+            raise ValueError("only excepts X Y Z Bx By Bz")
+    if sum(1 for o in order if o == 0) != 1:
+        raise ValueError("spatial or B-field dimensions are too large")
+    try:
+        lines = open(path).read().split("\n")
+    except OSError:
+        raise ValueError("cannot open file for reading")
+    rows = []
+    for line in lines[1:]:
+        t = line.strip(" \t")
+        if t == "" or t[0] == comment:
+            continue
+        toks = line.replace(",", " ").split()
+        v = [0.0] * 4
+        for i, o in enumerate(order):
+            try:
+                v[o] = float(toks[i])
+            except (IndexError, ValueError):
+                raise ValueError("something is incorrect with data format or iformat string")
+        for i in range(min(len(scaling), 4)):
+            if i < len(order):
+                v[order[i]] *= scaling[i]
+        rows.append(v)
+    rows.sort(key=lambda r: r[0])
+    if len(rows) < 2:
+        raise ValueError("not enough data points")
+    first, last = rows[0][0], rows[-1][0]
+    n = int((last - first) * 10000)
+    step = (last - first) / float(n - 1)
+    out = np.zeros((n, 3))
+    min_bin = after = before = 0
+    for i in range(n):
+        z = i * step + first
+        for j in range(min_bin + 1, len(rows)):
+            if rows[j][0] > z:
+                after, before, min_bin = j, j - 1, j - 1
+                break
+        a, b = rows[after], rows[before]
+        for k in range(3):
+            if has[k]:
+                slope = (a[1 + k] - b[1 + k]) / (a[0] - b[0])
+                out[i, k] = b[1 + k] + (z - b[0]) * slope
+    return axis, first, last, step, n, out
 
 
 def read_oscars_field_file(path):

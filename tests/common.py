@@ -109,3 +109,26 @@ def other_fields(s, npoints=4001):
     s.add_bfield_quadrupole(K=2.0, width=0.3, rotations=[0, 0, 0.1], translation=[0.001, 0, 0.0])
     s.set_ctstartstop(0, 2)
     s.set_npoints_trajectory(npoints)
+
+
+def write_oscars_grid(path, start, step, n, func):
+    """An OSCARS-format text field map: header, then z fastest, y, x."""
+    with open(path, "w") as f:
+        f.write("# test map\n")
+        for a in range(3):
+            f.write("%.17g\n%.17g\n%d\n" % (start[a], step[a], n[a]))
+        for ix in range(n[0]):
+            for iy in range(n[1]):
+                for iz in range(n[2]):
+                    x = [start[0] + ix * step[0], start[1] + iy * step[1], start[2] + iz * step[2]]
+                    f.write("%.17g %.17g %.17g\n" % tuple(func(*x)))
+
+
+def undulator_like(x, y, z):
+    """A 3-D test field: a 50 mm period in z within |z| < 0.45 m, with x and y dependence."""
+    k = 2 * _np.pi / 0.05
+    env = 1.0 if abs(z) < 0.45 else 0.0
+    return [0.02 * _np.cos(k * z) * x, env * 0.9 * _np.cos(k * z) * (1 + 3 * x * x), 0.1 * y]
+
+
+import numpy as _np  # noqa: E402

@@ -278,3 +278,36 @@ def test_gaussian_uniform_quadrupole_fields():
     print("other fields: trajectory %.3e, spectrum %.3e" % (et, es))
     assert et < 1e-11 and es < 1e-9
     assert np.array_equal(s._last_levels, olev)
+
+
+@gpu
+@pytest.mark.parametrize("dims", ["xyz", "xz"])
+def test_multi_dimensional_grid_file(tmp_path, dims):
+    """2-D and 3-D OSCARS field maps on the device against the oracle (whose
+    grid is pinned to the reference's in tests/test_field_files.py)."""
+    from oracle.pyoracle import Oracle
+    from oscars_amd import sr as SR
+    from tests.common import undulator_like, write_oscars_grid
+    n = [5 if "x" in dims else 1, 4 if "y" in dims else 1, 401]
+    path = str(tmp_path / "g.txt")
+    write_oscars_grid(path, [-0.004, -0.003, -0.5], [0.002, 0.002, 0.0025], n, undulator_like)
+    kw = dict(rotations=[0.001, -0.002, 0.01], translation=[0.0001, 0, 0.002])
+    o, s = Oracle(nthreads=8), SR.sr()
+    for x in (o, s):
+        x.set_particle_beam(energy_GeV=3, x0=[0.0002, -0.0001, -1], d0=[0, 0, 1], current=0.5)
+    st, sp, npts, data = SR.read_oscars_field_file(path)
+    o.add_bfield_grid(st, sp, npts, data, **kw)
+    s.add_bfield_file(path, **kw)
+    for x in (o, s):
+        x.set_ctstartstop(0, 2)
+        x.set_npoints_trajectory(4001)
+    T = o.calculate_trajectory()
+    s.calculate_trajectory()
+    t, p, _ = s._device().trajectory(0)
+    et = float(np.max(np.abs(p - T.p) / np.maximum(np.max(np.abs(T.p), axis=0), 1e-300)))
+    e = np.linspace(100.0, 600.0, 12)
+    ov, olev, _, _ = o.spectrum([0, 0, 30], e, carry_quirk=False)
+    v = np.array([r[1] for r in s.calculate_spectrum(obs=[0, 0, 30], energy_points_eV=list(e), mode="plain")])
+    es = rel_err(v, ov, 1e-6)
+    print("%s grid: trajectory %.3e, spectrum %.3e" % (dims, et, es))
+    assert et < 1e-12 and es < 1e-9
