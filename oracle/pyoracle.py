@@ -498,6 +498,8 @@ def ref_lib():
         L.ref_set_beam.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_double]
         L.ref_add_gaussian.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.ref_add_field_file.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, _dp, _dp, _dp, C.c_int]
+        L.ref_add_field_interpolated.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_char_p), C.c_char_p,
+                                                 C.c_double, _dp, _dp, _dp, C.c_int]
         L.ref_add_uniform.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.ref_add_quadrupole.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         L.ref_set_particle.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
@@ -630,6 +632,15 @@ class RefSim:
         sc = np.ascontiguousarray(list(scale) or [0.0], dtype=np.float64)
         self._chk(self.L.ref_add_field_file(self.h, ifile.encode(), iformat.encode(), _v3(rotations), _v3(translation),
                                             sc.ctypes.data_as(_dp), len(scale)))
+
+    def add_bfield_interpolated(self, mapping, iformat, parameter, rotations=(0, 0, 0), translation=(0, 0, 0),
+                                scale=()):
+        params = np.ascontiguousarray([float(m[0]) for m in mapping], dtype=np.float64)
+        files = (C.c_char_p * len(mapping))(*[str(m[1]).encode() for m in mapping])
+        sc = np.ascontiguousarray(list(scale) or [0.0], dtype=np.float64)
+        self._chk(self.L.ref_add_field_interpolated(self.h, len(mapping), params.ctypes.data_as(_dp), files,
+                                                    iformat.encode(), float(parameter), _v3(rotations),
+                                                    _v3(translation), sc.ctypes.data_as(_dp), len(scale)))
 
     def add_bfield_gaussian(self, bfield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
         self._chk(self.L.ref_add_gaussian(self.h, _v3(bfield), _v3(translation), _v3(sigma), _v3(rotations)))

@@ -95,6 +95,20 @@ int ref_add_field_file(void* h, const char* path, const char* format, double con
   return 0;
 }
 
+// add_bfield_interpolated(mapping, iformat, parameter, rotations, translation, scale)
+int ref_add_field_interpolated(void* h, int n, double const* params, const char* const* files, const char* format,
+                               double parameter, double const* rotations, double const* translation,
+                               double const* scaling, int nscale) {
+  try {
+    std::vector<std::pair<double, std::string> > M;
+    for (int i = 0; i < n; ++i) M.push_back(std::make_pair(params[i], std::string(files[i])));
+    std::vector<double> sc(scaling, scaling + nscale);
+    static_cast<OSCARSSR*>(h)->AddMagneticFieldInterpolated(M, format, parameter, v3(rotations), v3(translation), sc,
+                                                            0, 0, 0, "", "");
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
 // set_particle_beam(type='electron', energy_GeV=E, x0=..., d0=..., current=I)
 // add_bfield_gaussian / add_bfield_uniform / add_bfield_quadrupole
 // (src/OSCARSSR_Python.cc: the constructors as the binding calls them)

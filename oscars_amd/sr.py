@@ -262,6 +262,44 @@ class sr:
         else:
             raise ValueError("incorrect format given")
 
+    def add_bfield_interpolated(self, mapping, iformat, parameter, rotations=None, translation=None, scale=None,
+                                frequency=0, frequency_phase=0, time_offset=0, name="", ofile=""):
+        """A field map interpolated between maps taken at several values of a
+        parameter (e.g. an undulator gap): per sample and component, the
+        reference's cubic spline over the parameter values, evaluated at
+        `parameter` (src/TField3D_Grid.cc:1694-1940).  OSCARS-format maps."""
+        if not iformat:
+            raise ValueError("'iformat' is blank")
+        if iformat.upper() != "OSCARS":
+            raise ValueError("interpolation of %s maps is not supported by this implementation" % iformat)
+        if frequency != 0:
+            raise ValueError("time-dependent fields (frequency != 0) are not supported by this implementation")
+        if len(name) > 0 and name[0] == "_":
+            raise ValueError("'name' cannot begin with '_'.  This is reserved for internal use.  Please pick a "
+                             "different name")
+        pairs = []
+        for m in mapping:
+            if len(m) != 2:
+                raise ValueError("Incorrect format in 'mapping'")
+            pairs.append((float(m[0]), str(m[1])))
+        pairs.sort(key=lambda a: a[0])
+        maps = [read_oscars_field_file(f) for _, f in pairs]
+        for m in maps[1:]:
+            if m[0] != maps[0][0] or m[1] != maps[0][1] or m[2] != maps[0][2]:
+                raise ValueError("Header value is incorrect format")
+        start, step, npts, _ = maps[0]
+        sc = [float(v) for v in scale] if scale else []
+        samples = []
+        for m in maps:  # field factors apply to each map as it is read
+            f = np.array(m[3], dtype=np.float64).reshape(-1, 3).copy()
+            for k in range(3):
+                if len(sc) > 3 + k and sc[3 + k] != 1:
+                    f[:, k] *= sc[3 + k]
+            samples.append(f)
+        data = spline_at([a for a, _ in pairs], np.stack(samples), float(parameter))
+        g = field_grid_scaled(start, step, npts, data, sc[:3])
+        self._add_grid_exact(g[0], g[1], g[2], npts, g[3], rotations, translation)
+
     def clear_bfields(self):
         # src/OSCARSSR.cc:186-196: the current trajectory stays until a field is added
         self._fields = []
@@ -919,6 +957,47 @@ class sr:
     def average_power_density(ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
         """Average power-density files of the same points (src/OSCARSSR_Python.cc:8694-8790)."""
         return IO.average_points(ifiles, bifiles, ofile, bofile, dim)
+
+
+def spline_at(X, Y, x):
+    """include/TOMATH.h:40-203 (TSpline1D3) through the points (X[i], Y[i]),
+    Y[i] an array of any shape, evaluated at x; the operations run element by
+    element in the reference's order."""
+    X = [float(v) for v in X]
+    Y = np.asarray(Y, dtype=np.float64)
+    N = len(X)
+    if N < 3:
+        raise ValueError("TSpline1D3 does not see enough points for interpolation.  Please use >= 3 points")
+    ypp = np.empty_like(Y)
+    u = np.empty_like(Y)
+    dydx0 = (Y[1] - Y[0]) / (X[1] - X[0])
+    ypp[0] = -0.5
+    u[0] = (3. / (X[1] - X[0])) * ((Y[1] - Y[0]) / (X[1] - X[0]) - dydx0)
+    for i in range(1, N - 1):
+        sig = (X[i] - X[i - 1]) / (X[i + 1] - X[i - 1])
+        p = sig * ypp[i - 1] + 2.0
+        ypp[i] = (sig - 1.) / p
+        ui = (Y[i + 1] - Y[i]) / (X[i + 1] - X[i]) - (Y[i] - Y[i - 1]) / (X[i] - X[i - 1])
+        u[i] = (6. * ui / (X[i + 1] - X[i - 1]) - sig * u[i - 1]) / p
+    qn = 0.5
+    dydxN = (Y[N - 1] - Y[N - 2]) / (X[N - 1] - X[N - 2])
+    un = (3. / (X[N - 1] - X[N - 2])) * (dydxN - (Y[N - 1] - Y[N - 2]) / (X[N - 1] - X[N - 2]))
+    ypp[N - 1] = (un - qn * u[N - 2]) / (qn * ypp[N - 2] + 1.)
+    for k in range(N - 2, -1, -1):
+        ypp[k] = ypp[k] * ypp[k + 1] + u[k]
+    klo, khi = 0, N - 1
+    while khi - klo > 1:
+        k = (khi + klo) >> 1
+        if X[k] > x:
+            khi = k
+        else:
+            klo = k
+    h = X[khi] - X[klo]
+    if abs(h) < 1e-50:
+        raise ValueError("TSpline1D3 sees that the stepsize h is zero")
+    a = (X[khi] - x) / h
+    b = (x - X[klo]) / h
+    return a * Y[klo] + b * Y[khi] + ((a * a * a - a) * ypp[klo] + (b * b * b - b) * ypp[khi]) * (h * h) / 6.
 
 
 def field_grid_scaled(start, step, npts, data, scaling):

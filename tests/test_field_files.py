@@ -91,3 +91,29 @@ def test_oscars1d_table(tmp_path):
     o.add_bfield_grid_exact(st, sp, steps, npts, data)
     r.add_bfield_file_format(path, fmt, scale=scale)
     _finish(o, r, POINTS)
+
+
+@pytest.mark.parametrize("parameter", [12.0, 17.3, 30.0])
+def test_interpolated_maps(tmp_path, parameter):
+    """add_bfield_interpolated: four maps taken at gaps 10, 15, 22 and 30 mm,
+    given out of order, with field scaling; the spline over the gaps per sample
+    must equal the reference's, bit for bit."""
+    gaps = [22.0, 10.0, 30.0, 15.0]
+    mapping = []
+    for g in gaps:
+        path = str(tmp_path / ("gap%g.txt" % g))
+        amp = 1.8 * np.exp(-np.pi * g / 49.0)
+        write_oscars_grid(path, [-0.004, 0, -0.5], [0.002, 0, 0.0025], [5, 1, 401],
+                          lambda x, y, z, a=amp: [0.0, a * np.cos(2 * np.pi * z / 0.049) * (1 + 2 * x) *
+                                                  (abs(z) < 0.45), 0.01 * a * x])
+        mapping.append([g, path])
+    o, r = _pair()
+    sc = [1.0, 1.0, 1.0, 1.0, 0.5, 1.0]
+    s = SR.sr()
+    s.set_particle_beam(energy_GeV=3, x0=[0.0002, -0.0001, -1], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_interpolated(mapping, "OSCARS", parameter, scale=sc)
+    f = s._fields[-1]
+    n = [int(v) for v in f.grid_n]
+    o.add_bfield_grid_exact(list(f.grid_start), list(f.grid_stop), list(f.grid_step), n, s._grids[-1])
+    r.add_bfield_interpolated(mapping, "OSCARS", parameter, scale=sc)
+    _finish(o, r, POINTS)
