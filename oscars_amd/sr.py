@@ -257,8 +257,11 @@ class sr:
             start, stop, steps, npts = [0.0] * 3, [0.0] * 3, [0.0] * 3, [1, 1, 1]
             start[axis], stop[axis], steps[axis], npts[axis] = first, last, step, n
             self._add_grid_exact(start, stop, steps, npts, data, rotations, translation)
-        elif fmt in ("SPECTRA", "SRW", "BINARY"):
-            raise ValueError("the %s field-file format is not supported by this implementation" % fmt)
+        elif fmt in ("SRW", "SPECTRA"):
+            start, stop, step, npts, data = (read_srw_field_file if fmt == "SRW" else read_spectra_field_file)(ifile)
+            self._add_grid_exact(start, stop, step, npts, data, rotations, translation)
+        elif fmt == "BINARY":
+            raise ValueError("the BINARY field-file format is not supported by this implementation")
         else:
             raise ValueError("incorrect format given")
 
@@ -1081,6 +1084,65 @@ def read_oscars1d_field_file(path, columns, scaling=(), comment="#"):
                 slope = (a[1 + k] - b[1 + k]) / (a[0] - b[0])
                 out[i, k] = b[1 + k] + (z - b[0]) * slope
     return axis, first, last, step, n, out
+
+
+def _grid_limits(start, step, n):
+    return [start[a] + (n[a] - 1) * step[a] for a in range(3)]
+
+
+def read_srw_field_file(path, comment="#"):
+    """An SRW field map (src/TField3D_Grid.cc:1368-1541): a title line, nine
+    header lines '#value' (start, step, n for x, y, z; the start is used as
+    given), then 'Bx By Bz' lines with x fastest.  Returns start, stop, step,
+    n and the samples with z fastest."""
+    try:
+        lines = open(path).read().split("\n")
+    except OSError:
+        raise ValueError("cannot open file for reading SRW format")
+    hv = []
+    for line in lines[1:10]:
+        if not line or line[0] != comment:
+            raise ValueError("something is wrong with the comment character, it was not seen")
+        hv.append(float(line[1:].split()[0]))
+    start, step = [hv[0], hv[3], hv[6]], [hv[1], hv[4], hv[7]]
+    n = [int(hv[2]), int(hv[5]), int(hv[8])]
+    if n[0] < 1 or n[1] < 1:
+        raise ValueError("invalid dimensions")
+    total = n[0] * n[1] * n[2]
+    rows = lines[10:10 + total]
+    try:
+        vals = np.array([[float(v) for v in r.split()[:3]] for r in rows], dtype=np.float64)
+    except ValueError:
+        raise ValueError("input file stream failure")
+    if vals.shape != (total, 3):
+        raise ValueError("input file stream failure")
+    data = vals.reshape(n[2], n[1], n[0], 3).transpose(2, 1, 0, 3).reshape(total, 3)
+    return start, _grid_limits(start, step, n), step, n, data
+
+
+def read_spectra_field_file(path):
+    """A SPECTRA field map (src/TField3D_Grid.cc:1543-1692): a title line, then
+    'dx dy dz nx ny nz' with steps in mm; the grid is centred on the origin;
+    then 'Bx By Bz' lines with z fastest."""
+    try:
+        lines = open(path).read().split("\n")
+    except OSError:
+        raise ValueError("cannot open file")
+    h = lines[1].split()
+    step = [float(h[0]) / 1000., float(h[1]) / 1000., float(h[2]) / 1000.]
+    n = [int(h[3]), int(h[4]), int(h[5])]
+    if n[0] < 1 or n[1] < 1:
+        raise ValueError("invalid number of points in at least one dimension")
+    start = [-(step[a] * (n[a] - 1)) / 2 for a in range(3)]
+    total = n[0] * n[1] * n[2]
+    rows = lines[2:2 + total]
+    try:
+        data = np.array([[float(v) for v in r.split()[:3]] for r in rows], dtype=np.float64)
+    except ValueError:
+        raise ValueError("file stream failure")
+    if data.shape != (total, 3):
+        raise ValueError("file stream failure")
+    return start, _grid_limits(start, step, n), step, n, data
 
 
 def read_oscars_field_file(path):

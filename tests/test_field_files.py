@@ -117,3 +117,38 @@ def test_interpolated_maps(tmp_path, parameter):
     o.add_bfield_grid_exact(list(f.grid_start), list(f.grid_stop), list(f.grid_step), n, s._grids[-1])
     r.add_bfield_interpolated(mapping, "OSCARS", parameter, scale=sc)
     _finish(o, r, POINTS)
+
+
+@pytest.mark.parametrize("fmt", ["SRW", "SPECTRA"])
+def test_srw_and_spectra_maps(tmp_path, fmt):
+    """SRW maps (x fastest, '#' header, start as given) and SPECTRA maps
+    (steps in mm, centred grid, z fastest), 3-D."""
+    n = [5, 3, 401]
+    step = [0.002, 0.0015, 0.0025]
+    path = str(tmp_path / "m.txt")
+    with open(path, "w") as f:
+        if fmt == "SRW":
+            f.write("# SRW map\n")
+            start = [-0.004, -0.0015, -0.5]
+            for a in range(3):
+                f.write("#%.17g\n#%.17g\n#%d\n" % (start[a], step[a], n[a]))
+            for iz in range(n[2]):
+                for iy in range(n[1]):
+                    for ix in range(n[0]):
+                        x = [start[0] + ix * step[0], start[1] + iy * step[1], start[2] + iz * step[2]]
+                        f.write("%.17g %.17g %.17g\n" % tuple(undulator_like(*x)))
+        else:
+            f.write("SPECTRA map\n%.17g %.17g %.17g %d %d %d\n" % (step[0] * 1e3, step[1] * 1e3, step[2] * 1e3, *n))
+            start = [-(step[a] * (n[a] - 1)) / 2 for a in range(3)]
+            for ix in range(n[0]):
+                for iy in range(n[1]):
+                    for iz in range(n[2]):
+                        x = [start[0] + ix * step[0], start[1] + iy * step[1], start[2] + iz * step[2]]
+                        f.write("%.17g %.17g %.17g\n" % tuple(undulator_like(*x)))
+    o, r = _pair()
+    reader = SR.read_srw_field_file if fmt == "SRW" else SR.read_spectra_field_file
+    st, sp, stp, npts, data = reader(path)
+    kw = dict(rotations=[0.0, 0.001, 0.0], translation=[0, 0.0001, 0.001])
+    o.add_bfield_grid_exact(st, sp, stp, npts, data, **kw)
+    r.add_bfield_file_format(path, fmt, **kw)
+    _finish(o, r, POINTS)
