@@ -141,6 +141,10 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
  * and the current.  Batched sums need integrated trajectories. */
 int om_trajectory_set(om_context* ctx, const om_particle* particle, long n, const double* t, const double* p9,
                       int construct);
+/* om_trajectories with electric fields as well (add_efield_*): the equations
+ * of motion with E and B (src/OSCARSSR.cc:1186-1198, 1210-1245, 1283-1318). */
+int om_trajectories_eb(om_context* ctx, const om_field* fields, int nfields, const om_field* efields, int nefields,
+                       const om_particle* particles, int np);
 /* Knots of trajectory i: *n in = capacity, out = count; any pointer may be NULL. */
 int om_trajectory_get(om_context* ctx, int i, long* n, double* t, double* p9, double* ypp9);
 /* Level samples of trajectory i (2^level rows of t,x,y,z,bx,by,bz,ax,ay,az). */

@@ -436,10 +436,29 @@ typedef struct {
   const orc_fields* F;
   const orc_particle* P;
   int err_gamma;
+  const orc_fields* FE; /* electric fields, or NULL */
 } rk_ctx;
 
 /* src/OSCARSSR.cc:1247-1279 */
 static void derivs_b(rk_ctx* c, double t, const double* x, double* dxdt) {
+  if (c->FE && c->FE->n > 0) {
+    /* src/OSCARSSR.cc:1283-1318 */
+    double OneMinusEB = (1. - (x[1] * x[1] + x[3] * x[3] + x[5] * x[5]) / (K_C * K_C));
+    if (OneMinusEB <= 0) c->err_gamma = 1;
+    double X[3] = {x[0], x[2], x[4]};
+    double B[3], E[3];
+    orc_get_b(c->F, X, t, B);
+    orc_get_b(c->FE, X, t, E);
+    double Q = c->P->q / c->P->m * sqrt(1. - (x[1] * x[1] + x[3] * x[3] + x[5] * x[5]) / (K_C * K_C));
+    double BetaDotE = (x[1] * E[0] + x[3] * E[1] + x[5] * E[2]) / K_C;
+    dxdt[0] = x[1];
+    dxdt[1] = Q * (E[0] - x[5] * B[1] + x[3] * B[2] - x[1] * BetaDotE / K_C);
+    dxdt[2] = x[3];
+    dxdt[3] = Q * (E[1] + x[5] * B[0] - x[1] * B[2] - x[3] * BetaDotE / K_C);
+    dxdt[4] = x[5];
+    dxdt[5] = Q * (E[2] + x[1] * B[1] - x[3] * B[0] - x[5] * BetaDotE / K_C);
+    return;
+  }
   double OneMinus = (1. - (x[1] * x[1] + x[3] * x[3] + x[5] * x[5]) / (K_C * K_C));
   if (OneMinus <= 0) {
     c->err_gamma = 1;
@@ -507,7 +526,12 @@ static void reverse_rows(double* t, double* p, long n) {
  * orc_npoints_forward_backward). */
 int orc_trajectory_rk4(const orc_fields* F, const orc_particle* P, double ctstart, double ctstop, long npoints,
                        double* out_t, double* out_p) {
-  rk_ctx c = {F, P, 0};
+  return orc_trajectory_rk4_eb(F, NULL, P, ctstart, ctstop, npoints, out_t, out_p);
+}
+
+int orc_trajectory_rk4_eb(const orc_fields* F, const orc_fields* FE, const orc_particle* P, double ctstart,
+                          double ctstop, long npoints, double* out_t, double* out_p) {
+  rk_ctx c = {F, P, 0, FE};
   double DeltaT = ((ctstop - ctstart) / K_C / (double) (size_t) (npoints - 1));
   size_t NF = (size_t) (1 + (ctstop - P->t0) / K_C / DeltaT);
   size_t NB = (size_t) ((P->t0 - ctstart) / K_C / DeltaT);

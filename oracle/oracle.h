@@ -86,6 +86,9 @@ void orc_particle_set(orc_particle* P, const double* x0, const double* b0, doubl
 
 /* ---- trajectory ---- */
 long orc_npoints_forward_backward(double ctstart, double ctstop, long npoints, double t0, long* nbwd);
+/* with electric fields FE as well (NULL: none), src/OSCARSSR.cc:1283-1318 */
+int orc_trajectory_rk4_eb(const orc_fields* F, const orc_fields* FE, const orc_particle* P, double ctstart,
+                          double ctstop, long npoints, double* out_t, double* out_p);
 int orc_trajectory_rk4(const orc_fields* F, const orc_particle* P, double ctstart, double ctstop, long npoints,
                        double* out_t, double* out_p);
 int orc_spline(long n, const double* t, const double* p, double* ypp);

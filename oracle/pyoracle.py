@@ -89,6 +89,8 @@ def oracle_lib():
         L.orc_add_epu.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, _dp, C.c_double, C.c_double]
         L.orc_add_grid.argtypes = [C.c_void_p, _dp, _dp, _lp, _dp, _dp, _dp]
         L.orc_add_gaussian.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_trajectory_rk4_eb.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_long,
+                                            _dp, _dp]
         L.orc_add_grid_exact.argtypes = [C.c_void_p, _dp, _dp, _dp, _lp, _dp, _dp, _dp]
         L.orc_add_uniform.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_add_quadrupole.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
@@ -193,6 +195,7 @@ class Oracle:
     def __init__(self, nthreads=1):
         self.L = oracle_lib()
         self.F = orc_fields()
+        self.FE = orc_fields()  # electric fields
         self.P = orc_particle()
         self.ctstart, self.ctstop = 0.0, 0.0
         self.npoints = 0
@@ -214,6 +217,19 @@ class Oracle:
 
     def add_bfield_gaussian(self, bfield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
         self.L.orc_add_gaussian(C.byref(self.F), _v3(bfield), _v3(translation), _v3(sigma), _v3(rotations))
+        self.traj = None
+
+    def add_efield_gaussian(self, efield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self.L.orc_add_gaussian(C.byref(self.FE), _v3(efield), _v3(translation), _v3(sigma), _v3(rotations))
+        self.traj = None
+
+    def add_efield_uniform(self, efield, width=(0, 0, 0), rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self.L.orc_add_uniform(C.byref(self.FE), _v3(efield), _v3(width), _v3(translation), _v3(rotations))
+        self.traj = None
+
+    def add_efield_undulator(self, efield, period, nperiods, translation=(0, 0, 0)):
+        self.L.orc_add_undulator(C.byref(self.FE), _v3(efield), _v3(period), int(nperiods), _v3(translation), 0.0,
+                                 0.0)
         self.traj = None
 
     def add_bfield_uniform(self, bfield, width=(0, 0, 0), rotations=(0, 0, 0), translation=(0, 0, 0)):
@@ -282,8 +298,9 @@ class Oracle:
         n = nf + nb.value
         t = np.zeros(n)
         p = np.zeros((n, 9))
-        self.L.orc_trajectory_rk4(C.byref(self.F), C.byref(self.P), self.ctstart, self.ctstop, self.npoints,
-                                  t.ctypes.data_as(_dp), p.ctypes.data_as(_dp))
+        self.L.orc_trajectory_rk4_eb(C.byref(self.F), C.byref(self.FE) if self.FE.n > 0 else None, C.byref(self.P),
+                                     self.ctstart, self.ctstop, self.npoints, t.ctypes.data_as(_dp),
+                                     p.ctypes.data_as(_dp))
         self.traj = Trajectory(t, p)
         return self.traj
 
@@ -498,6 +515,7 @@ def ref_lib():
         L.ref_set_beam.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_double]
         L.ref_add_gaussian.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.ref_add_field_file.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, _dp, _dp, _dp, C.c_int]
+        L.ref_add_efield.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp]
         L.ref_add_field_interpolated.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_char_p), C.c_char_p,
                                                  C.c_double, _dp, _dp, _dp, C.c_int]
         L.ref_add_uniform.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
@@ -632,6 +650,18 @@ class RefSim:
         sc = np.ascontiguousarray(list(scale) or [0.0], dtype=np.float64)
         self._chk(self.L.ref_add_field_file(self.h, ifile.encode(), iformat.encode(), _v3(rotations), _v3(translation),
                                             sc.ctypes.data_as(_dp), len(scale)))
+
+    def add_efield_gaussian(self, efield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self._chk(self.L.ref_add_efield(self.h, 3, _v3(efield), _v3(sigma), _v3((0, 0, 0)), 0, _v3(rotations),
+                                        _v3(translation)))
+
+    def add_efield_uniform(self, efield, width=(0, 0, 0), rotations=(0, 0, 0), translation=(0, 0, 0)):
+        self._chk(self.L.ref_add_efield(self.h, 4, _v3(efield), _v3(width), _v3((0, 0, 0)), 0, _v3(rotations),
+                                        _v3(translation)))
+
+    def add_efield_undulator(self, efield, period, nperiods, translation=(0, 0, 0)):
+        self._chk(self.L.ref_add_efield(self.h, 0, _v3(efield), _v3((0, 0, 0)), _v3(period), int(nperiods),
+                                        _v3((0, 0, 0)), _v3(translation)))
 
     def add_bfield_interpolated(self, mapping, iformat, parameter, rotations=(0, 0, 0), translation=(0, 0, 0),
                                 scale=()):

@@ -95,6 +95,24 @@ int ref_add_field_file(void* h, const char* path, const char* format, double con
   return 0;
 }
 
+// add_efield_gaussian / add_efield_uniform / add_efield_undulator (src/OSCARSSR_Python.cc:2636, 2793, 2963)
+int ref_add_efield(void* h, int type, double const* field, double const* shape, double const* period, int nperiods,
+                   double const* rotations, double const* translation) {
+  try {
+    OSCARSSR* s = static_cast<OSCARSSR*>(h);
+    if (type == 3)
+      s->AddElectricField((TField*) new TField3D_Gaussian(v3(field), v3(translation), v3(shape), v3(rotations), 0, 0,
+                                                         0, ""));
+    else if (type == 4)
+      s->AddElectricField((TField*) new TField3D_UniformBox(v3(field), v3(shape), v3(translation), v3(rotations), 0,
+                                                           0, 0, ""));
+    else
+      s->AddElectricField((TField*) new TField3D_IdealUndulator(v3(field), v3(period), nperiods, v3(translation), 0, 0,
+                                                               0, 0, 0, ""));
+  } catch (std::exception const& e) { return fail(e); }
+  return 0;
+}
+
 // add_bfield_interpolated(mapping, iformat, parameter, rotations, translation, scale)
 int ref_add_field_interpolated(void* h, int n, double const* params, const char* const* files, const char* format,
                                double parameter, double const* rotations, double const* translation,

@@ -59,7 +59,7 @@ class om_stats(C.Structure):
 
 
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
-           "om_trajectories", "om_trajectory_set", "om_trajectory_get", "om_power_density_surface",
+           "om_trajectories", "om_trajectories_eb", "om_trajectory_set", "om_trajectory_get", "om_power_density_surface",
            "om_total_power", "om_efield_time", "om_power_density_stl", "om_level_get", "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
@@ -92,6 +92,8 @@ def lib():
     L.om_last_error.argtypes = [C.c_void_p]
     L.om_last_error.restype = C.c_char_p
     L.om_trajectories.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.POINTER(om_particle), C.c_int]
+    L.om_trajectories_eb.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.POINTER(om_field), C.c_int,
+                                     C.POINTER(om_particle), C.c_int]
     L.om_trajectory_get.argtypes = [C.c_void_p, C.c_int, _lp, _dp, _dp, _dp]
     L.om_trajectory_set.argtypes = [C.c_void_p, C.POINTER(om_particle), C.c_long, _dp, _dp, C.c_int]
     L.om_level_get.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
@@ -171,13 +173,17 @@ class Device:
         self.L.om_device_info(self.device, buf, 512)
         return buf.value.decode()
 
-    def trajectories(self, fields, particles, check_status=True):
+    def trajectories(self, fields, particles, check_status=True, efields=()):
         """RK4 + spline for `particles`.  check_status=False returns as soon as the
         work is queued; a |beta| >= 1 trajectory is then reported by the next
         call that uses it (every sum checks the status first)."""
         fa = (om_field * max(1, len(fields)))(*fields)
         pa = (om_particle * len(particles))(*particles)
-        self._chk(self.L.om_trajectories(self.h, fa, len(fields), pa, len(particles)))
+        if efields:
+            ea = (om_field * len(efields))(*efields)
+            self._chk(self.L.om_trajectories_eb(self.h, fa, len(fields), ea, len(efields), pa, len(particles)))
+        else:
+            self._chk(self.L.om_trajectories(self.h, fa, len(fields), pa, len(particles)))
         if not check_status:
             return
         for i in range(len(particles)):

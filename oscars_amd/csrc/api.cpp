@@ -29,7 +29,7 @@ hipError_t om_launch_level_batch(const om::LevelJobBatch* J, hipStream_t s);
 hipError_t om_launch_batch_clear(double* out, long n, unsigned long long* counters, int nc, hipStream_t s);
 hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s);
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
-                         int np, double* t, double* p, hipStream_t s);
+                         int np, double* t, double* p, hipStream_t s, int nefields);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
 hipError_t om_launch_efield_time(const double* p, long n, const double* obs, double c0, double dt, double* out_t,
                                  double* out_e, hipStream_t s);
@@ -222,6 +222,7 @@ struct om_context {
   DevBuf<double> d_normals, d_tp, d_tp_partial, d_et, d_stl;
   std::vector<om::FieldDev> fields_host;
   int nfields = 0;
+  int nefields = 0;  // electric fields, after the magnetic ones in d_fields
   long total_knots = 0;
   bool desc_pending = false;  // trajectories launched, status not read back yet
   bool user_traj = false;     // slot 0 holds knots given by the caller (nothing to recompute)
@@ -652,7 +653,7 @@ void launch_trajectories(om_context* ctx, int np) {
   om::FieldDev none;
   std::memset(&none, 0, sizeof(none));
   check(om_launch_rk4(ctx->d_fields.p, ctx->nfields > 0 ? &ctx->fields_host[0] : &none, ctx->nfields,
-                      ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->stream), "rk4");
+                      ctx->d_desc.p, np, ctx->d_t.p, ctx->d_p.p, ctx->stream, ctx->nefields), "rk4");
   if (ctx->adj.armed) {
     // states every K steps of particle 0, taken from the raw knots before the division by c
     auto& A = ctx->adj;
@@ -867,8 +868,14 @@ void om_context_destroy(om_context* ctx) {
 const char* om_last_error(const om_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const om_particle* particles, int np) {
+  return om_trajectories_eb(ctx, fields, nfields, nullptr, 0, particles, np);
+}
+
+int om_trajectories_eb(om_context* ctx, const om_field* fields, int nfields, const om_field* efields, int nefields,
+                       const om_particle* particles, int np) {
   return guarded(ctx, [&] {
-    if (nfields < 0 || nfields > om::kMaxFields) throw std::out_of_range("too many fields");
+    if (nfields < 0 || nefields < 0 || nfields + nefields > om::kMaxFields) throw std::out_of_range("too many fields");
+    if (nefields > 0 && ctx->adj.armed) throw std::invalid_argument("the reverse pass supports magnetic fields only");
     if (np < 1) throw std::length_error("no particles");
     if ((int) ctx->levels.size() != np) ctx->free_levels();
     for (auto& lc : ctx->levels)
@@ -881,13 +888,15 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
     ctx->grid_bufs.reserve(nfields);
     std::vector<om::FieldDev> F;
     for (int i = 0; i < nfields; ++i) F.push_back(make_field(ctx, fields[i]));
+    for (int i = 0; i < nefields; ++i) F.push_back(make_field(ctx, efields[i]));
     ctx->batch.drop_graphs();
     ctx->batch.ready = false;
     ctx->adj.ready = false;
     ctx->fields_host = F;
     ctx->nfields = nfields;
-    ctx->d_fields.ensure(nfields > 0 ? nfields : 1);
-    if (nfields > 0) upload_staged(ctx, ctx->stage_fields, ctx->d_fields, F.data(), F.size());
+    ctx->nefields = nefields;
+    ctx->d_fields.ensure(F.size() > 0 ? F.size() : 1);
+    if (!F.empty()) upload_staged(ctx, ctx->stage_fields, ctx->d_fields, F.data(), F.size());
     ctx->np = 0;
     ctx->user_traj = false;
     ctx->desc.assign(np, om::TrajDesc());
@@ -913,6 +922,7 @@ int om_trajectories(om_context* ctx, const om_field* fields, int nfields, const 
       D.dt = DeltaT;
       D.t0_over_c = P.t0 / kC;
       D.qomg = P.qomg;
+      D.qom = P.q / P.m;
       for (int k = 0; k < 3; ++k) {
         D.x0[k] = P.x0[k];
         D.b0[k] = P.beta0[k];

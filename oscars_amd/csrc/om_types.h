@@ -42,6 +42,7 @@ struct TrajDesc {
   double x0[3], b0[3];
   double t0_over_c;  // T0 / c in seconds (reference stores T0 in metres)
   double qomg;       // Q / M / gamma
+  double qom;        // Q / M (equations of motion with electric fields)
   double dt;         // RK4 step [s]
   long nf, nb;       // forward / backward point counts
   long knot_offset;  // first knot in the packed knot arrays

@@ -228,6 +228,8 @@ struct RkCtx {
   double qomg;
   int err;
   GridZ g;
+  int nfe;     // electric fields, stored after the nf magnetic ones in F
+  double qom;  // Q / M
 };
 
 // Field-model variants of the equations of motion:
@@ -244,7 +246,8 @@ struct RkCtx {
 //   M = kGridZ          one 1-D grid along z without rotation, samples in LDS: the
 //                       grid branch of field_eval with the identity rotation and the
 //                       x/y range tests (n = 1 there) removed, which changes no value.
-constexpr int kFieldsGeneric = 0, kOneUndulator = 1, kAxisUndX = 2, kAxisUndY = 3, kAxisUndZ = 4, kGridZ = 5;
+constexpr int kFieldsGeneric = 0, kOneUndulator = 1, kAxisUndX = 2, kAxisUndY = 3, kAxisUndZ = 4, kGridZ = 5,
+              kFieldsEB = 6;
 
 __device__ __forceinline__ void grid_z_eval(const GridZ& g, double z, double B[3]) {
   double coord = z - g.trans;
@@ -287,6 +290,27 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
       d[3] = c.qomg * (-(x[1] * v));
       d[5] = 0.0;
     }
+    return;
+  }
+  if (M == kFieldsEB) {
+    // src/OSCARSSR.cc:1283-1318: magnetic and electric fields, the factor
+    // Q/M sqrt(1 - beta^2) taken at each evaluation
+    double bb = (x[1] * x[1] + x[3] * x[3] + x[5] * x[5]) / (kC * kC);
+    if (1. - bb <= 0) {
+      c.err = 1;
+      return;
+    }
+    double B[3], E[3];
+    get_b(c.F, c.nf, x[0], x[2], x[4], B);
+    get_b(c.F + c.nf, c.nfe, x[0], x[2], x[4], E);
+    double q = c.qom * sqrt(1. - bb);
+    double be = (x[1] * E[0] + x[3] * E[1] + x[5] * E[2]) / kC;
+    d[0] = x[1];
+    d[1] = q * (E[0] - x[5] * B[1] + x[3] * B[2] - x[1] * be / kC);
+    d[2] = x[3];
+    d[3] = q * (E[1] + x[5] * B[0] - x[1] * B[2] - x[3] * be / kC);
+    d[4] = x[5];
+    d[5] = q * (E[2] + x[1] * B[1] - x[3] * B[0] - x[5] * be / kC);
     return;
   }
   // |beta| >= 1 test of src/OSCARSSR.cc:1260-1264; only its sign is used, so

@@ -52,14 +52,14 @@ __global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p
 template <int M>
 __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const GridZ g, const FieldDev* __restrict__ F,
                                                  int nfields, TrajDesc* __restrict__ desc, int np,
-                                                 double* __restrict__ t, double* __restrict__ p) {
+                                                 double* __restrict__ t, double* __restrict__ p, int nefields) {
   extern __shared__ double lds_grid[];
   // Highest issue priority for these waves: the integration is one dependent
   // chain per particle, and when it runs beside other kernels (the previous
   // step's sums) it should not wait behind their waves on its SIMD.
   __builtin_amdgcn_s_setprio(3);
   int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  RkCtx c{F, nfields, 0.0, 0, g};
+  RkCtx c{F, nfields, 0.0, 0, g, nefields, 0.0};
   if (M == kGridZ) {
     // stage the map in LDS (all lanes), then integrate
     for (long i = threadIdx.x; i < 3 * g.n; i += blockDim.x) lds_grid[i] = g.data[i];
@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const GridZ
   if (ip >= np) return;
   TrajDesc D = desc[ip];
   c.qomg = D.qomg;
+  c.qom = D.qom;
   double* tk = t + D.knot_offset;
   double* pk = p + 9 * D.knot_offset;
   double x[6], dxdt[6];
@@ -355,7 +356,19 @@ __global__ void __launch_bounds__(256) beta_diff_kernel(const double* __restrict
 
 // ---------------------------------------------------------------- launchers (host)
 extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields,
-                                    om::TrajDesc* desc, int np, double* t, double* p, hipStream_t s) {
+                                    om::TrajDesc* desc, int np, double* t, double* p, hipStream_t s,
+                                    int nefields) {
+  if (nefields > 0) {
+    // electric fields (after the magnetic ones in F): the general equations of motion
+    om::UndParams u0;
+    memset(&u0, 0, sizeof(u0));
+    om::GridZ g0;
+    memset(&g0, 0, sizeof(g0));
+    int threads0 = 64, blocks0 = (np + threads0 - 1) / threads0;
+    hipLaunchKernelGGL(om::rk4_kernel<om::kFieldsEB>, dim3(blocks0), dim3(threads0), 0, s, u0, g0, F, nfields, desc,
+                       np, t, p, nefields);
+    return hipGetLastError();
+  }
   int threads = 64;
   int blocks = (np + threads - 1) / threads;
   om::UndParams u;
@@ -379,7 +392,7 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) grid_bytes);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(om::rk4_kernel<om::kGridZ>, dim3(blocks), dim3(threads), grid_bytes, s, u, g, F, nfields, desc,
-                       np, t, p);
+                       np, t, p, 0);
     return hipGetLastError();
   }
   if (nfields == 1 && (f.type == om::kUndulator || f.type == om::kEPU)) {
@@ -401,20 +414,20 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     if (f.type == om::kUndulator && axis_z && f.taper == 0 && nz == 1) {
       if (f.field[0] != 0)
         hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndX>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
-                           t, p);
+                           t, p, 0);
       else if (f.field[1] != 0)
         hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndY>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
-                           t, p);
+                           t, p, 0);
       else
         hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
-                           t, p);
+                           t, p, 0);
     } else {
       hipLaunchKernelGGL(om::rk4_kernel<om::kOneUndulator>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc,
-                         np, t, p);
+                         np, t, p, 0);
     }
   } else {
     hipLaunchKernelGGL(om::rk4_kernel<om::kFieldsGeneric>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
-                       t, p);
+                       t, p, 0);
   }
   return hipGetLastError();
 }

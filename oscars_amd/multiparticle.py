@@ -77,11 +77,11 @@ class BatchedFlux:
 
     def __init__(self, device, fields, particles, windows, requests, req_of, weights, mode="mixed",
                  polarization=(0.0,) * 6, precision=0.01, max_level=-2, max_level_extended=0, level_cap=0,
-                 waves_per_cu=0):
+                 waves_per_cu=0, efields=()):
         self.dev = device
         self.n_particles = len(particles)
         self.requests = requests
-        device.trajectories(fields, om_particles(particles, windows))
+        device.trajectories(fields, om_particles(particles, windows), efields=efields)
         conv = device.conv(precision, max_level, max_level_extended, 0, mode)
         device.batch_setup(requests, req_of, weights, list(polarization), conv, waves_per_cu, level_cap)
 

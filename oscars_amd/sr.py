@@ -80,6 +80,7 @@ class sr:
 
     def __init__(self, nthreads=2, gpu=0, device=0):
         self._fields = []        # list of dicts kept for om_field construction
+        self._efields = []       # electric fields (om_field)
         self._grids = []         # keep numpy arrays alive
         self._beam = None
         self._particle = None
@@ -203,6 +204,41 @@ class sr:
         f.width = float(width)
         self._fields.append(f)
         self._traj_valid = False
+
+    # ------------------------------------------------------------ electric fields
+    def _add_efield(self, f):
+        self._efields.append(f)
+        self._traj_valid = False
+
+    def add_efield_gaussian(self, efield, sigma, rotations=None, translation=None, frequency=0, frequency_phase=0,
+                            time_offset=0, name=""):
+        """Gaussian electric field [V/m] (the Gaussian model of add_bfield_gaussian)."""
+        E, S = _v3(efield, "efield"), _v3(sigma, "sigma")
+        f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, rotations, translation)
+        f.field[:] = E
+        f.shape[:] = S
+        self._add_efield(f)
+
+    def add_efield_uniform(self, efield, width=None, rotations=None, translation=None, frequency=0,
+                           frequency_phase=0, time_offset=0, name=""):
+        """Uniform electric field [V/m] inside a box (the model of add_bfield_uniform)."""
+        E = _v3(efield, "efield")
+        W = _v3(width, "width") if width is not None and len(width) != 0 else [0.0, 0.0, 0.0]
+        f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, rotations, translation)
+        f.field[:] = E
+        f.shape[:] = W
+        self._add_efield(f)
+
+    def add_efield_undulator(self, efield, period, nperiods, phase=0.0, translation=None, taper=0.0, frequency=0,
+                             frequency_phase=0, time_offset=0, name=""):
+        """Ideal-undulator electric field [V/m] (the model of add_bfield_undulator)."""
+        n0 = len(self._fields)
+        self.add_bfield_undulator(efield, period, nperiods, phase=phase, translation=translation, taper=taper,
+                                  name=name)
+        self._add_efield(self._fields.pop(n0))
+
+    def clear_efields(self):
+        self._efields = []
 
     def add_bfield_grid(self, start, step, npoints, samples, rotations=None, translation=None):
         """Regular 1-D field map from header values and a (n, 3) sample array
@@ -460,7 +496,7 @@ class sr:
             raise ValueError("start time is greater than T0 or stop time is less than T0")
         if self._ctstart >= self._ctstop:
             raise ValueError("start time is greater than stop time.  check that set_ctstartstop is set")
-        self._device().trajectories(self._fields, [self._om_particle(self._particle)])
+        self._device().trajectories(self._fields, [self._om_particle(self._particle)], efields=self._efields)
         self._traj_valid = True
 
     def calculate_trajectory(self, ofile="", bofile="", oformat=""):
@@ -599,7 +635,7 @@ class sr:
         w = [1.0 / len(plist)] * len(plist)
         bf = M.BatchedFlux(dev, self._fields, plist, windows, [(np.asarray(obs), np.asarray(energies))],
                            [0] * len(plist), w, mode=mode, polarization=pol, precision=precision,
-                           max_level=max_level, max_level_extended=max_level_extended)
+                           max_level=max_level, max_level_extended=max_level_extended, efields=self._efields)
         self._traj_valid = False
         outs, st = bf.run(recompute_trajectories=False)
         self.last_stats = st
@@ -613,7 +649,7 @@ class sr:
             self._ensure_trajectory()
             return fn(self._device(), 0, 1.0, False)
         dev = self._device()
-        dev.trajectories(self._fields, [self._om_particle(p) for p in plist])
+        dev.trajectories(self._fields, [self._om_particle(p) for p in plist], efields=self._efields)
         self._traj_valid = False
         w = 1.0 / float(len(plist))
         out = None

@@ -132,3 +132,17 @@ def undulator_like(x, y, z):
 
 
 import numpy as _np  # noqa: E402
+
+
+def with_efields(s, npoints=4001, magnetic=True):
+    """A planar undulator with electric fields added: a Gaussian kick, a
+    uniform box and an undulator-like E field (V/m); without the magnet when
+    magnetic is False."""
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    if magnetic:
+        s.add_bfield_undulator(**UNDULATOR)
+    s.add_efield_gaussian(efield=[2e6, -1e6, 3e5], sigma=[0, 0, 0.1], translation=[0, 0, -0.4])
+    s.add_efield_uniform(efield=[0, 5e5, 1e4], width=[0, 0, 0.3], translation=[0, 0, 0.5])
+    s.add_efield_undulator(efield=[0, 3e6, 0], period=[0, 0, 0.049], nperiods=31)
+    s.set_ctstartstop(0, 2)
+    s.set_npoints_trajectory(npoints)

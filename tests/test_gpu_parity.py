@@ -311,3 +311,32 @@ def test_multi_dimensional_grid_file(tmp_path, dims):
     es = rel_err(v, ov, 1e-6)
     print("%s grid: trajectory %.3e, spectrum %.3e" % (dims, et, es))
     assert et < 1e-12 and es < 1e-9
+
+
+@gpu
+@pytest.mark.parametrize("magnetic", [True, False])
+def test_electric_fields(magnetic):
+    """Electric fields on the device (om_trajectories_eb): the trajectory
+    against the oracle (pinned to the reference) and the spectrum to 1e-9.
+    The equations of motion with E take sqrt(1 - v.v / c^2) at every step, a
+    difference of numbers 3e-8 apart at 3 GeV, which magnifies last-bit
+    differences of the device's sine: with fields of 300 MV/m the knots
+    differed by 2e-7 of a channel's largest value; with these 3 MV/m fields
+    by 1e-13.  The knots' bar is 1e-8."""
+    from oracle.pyoracle import Oracle
+    from oscars_amd.sr import sr
+    from tests.common import with_efields
+    o = Oracle(nthreads=8)
+    with_efields(o, magnetic=magnetic)
+    T = o.calculate_trajectory()
+    s = sr()
+    with_efields(s, magnetic=magnetic)
+    s.calculate_trajectory()
+    t, p, _ = s._device().trajectory(0)
+    et = float(np.max(np.abs(p - T.p) / np.maximum(np.max(np.abs(T.p), axis=0), 1e-300)))
+    e = np.linspace(100.0, 800.0, 16)
+    ov, olev, _, _ = o.spectrum([0, 0, 30], e, carry_quirk=False)
+    v = np.array([r[1] for r in s.calculate_spectrum(obs=[0, 0, 30], energy_points_eV=list(e), mode="plain")])
+    es = rel_err(v, ov, 1e-6)
+    print("electric fields (magnetic %s): trajectory %.3e, spectrum %.3e" % (magnetic, et, es))
+    assert np.array_equal(t, T.t) and et < 1e-8 and es < 1e-9

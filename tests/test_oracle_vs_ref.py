@@ -260,3 +260,17 @@ def test_gaussian_uniform_quadrupole_fields():
     e, v, lev, conv, _ = o.calculate_spectrum([0, 0, 30], npoints=20, energy_range_eV=[10, 2000])
     re, rv, _ = r.calculate_spectrum([0, 0, 30], npoints=20, energy_range_eV=[10, 2000])
     assert same(v, rv)
+
+
+@pytest.mark.parametrize("magnetic", [True, False])
+def test_electric_fields(magnetic):
+    """Electric fields in the equations of motion (with and without magnetic
+    fields): trajectory and spectrum bit for bit."""
+    from tests.common import with_efields
+    o, r = pair(with_efields, magnetic=magnetic)
+    rt, rp, rypp = r.calculate_trajectory()
+    T = o.calculate_trajectory()
+    assert same(T.t, rt) and same(T.p, rp) and same(T.ypp, rypp)
+    e, v, lev, conv, _ = o.calculate_spectrum([0, 0, 30], npoints=8, energy_range_eV=[100, 800])
+    re, rv, _ = r.calculate_spectrum([0, 0, 30], npoints=8, energy_range_eV=[100, 800])
+    assert same(v, rv)
