@@ -237,6 +237,20 @@ class sr:
                                   name=name)
         self._add_efield(self._fields.pop(n0))
 
+    def add_efield_file(self, ifile, iformat="OSCARS", rotations=None, translation=None, scale=None, name=""):
+        """Electric-field map [V/m] from a file, in any format add_bfield_file reads."""
+        n0 = len(self._fields)
+        self.add_bfield_file(ifile, iformat, rotations, translation, scale, name)
+        self._add_efield(self._fields.pop(n0))
+
+    def add_efield_interpolated(self, mapping, iformat, parameter, rotations=None, translation=None, scale=None,
+                                frequency=0, frequency_phase=0, time_offset=0, name="", ofile=""):
+        """Electric-field map interpolated between maps, as add_bfield_interpolated."""
+        n0 = len(self._fields)
+        self.add_bfield_interpolated(mapping, iformat, parameter, rotations, translation, scale, frequency,
+                                     frequency_phase, time_offset, name, ofile)
+        self._add_efield(self._fields.pop(n0))
+
     def clear_efields(self):
         self._efields = []
 

@@ -152,3 +152,16 @@ def test_srw_and_spectra_maps(tmp_path, fmt):
     o.add_bfield_grid_exact(st, sp, stp, npts, data, **kw)
     r.add_bfield_file_format(path, fmt, **kw)
     _finish(o, r, POINTS)
+
+
+def test_efield_file_is_an_electric_field(tmp_path):
+    """add_efield_file keeps the map among the electric fields (the product's
+    list), not the magnetic ones; the map itself is read as add_bfield_file
+    reads it."""
+    path = str(tmp_path / "e.txt")
+    write_oscars_grid(path, [0, 0, -0.5], [0, 0, 0.0025], [1, 1, 401], lambda x, y, z: [1e5 * z, 0, 0])
+    s = SR.sr()
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1])
+    s.add_efield_file(path)
+    assert len(s._efields) == 1 and len(s._fields) == 0
+    assert int(s._efields[0].grid_n[2]) == 401
