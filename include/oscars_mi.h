@@ -156,6 +156,13 @@ int om_level_get(om_context* ctx, int i, int level, int extended, double* rows);
  * = E [V/m].  *n in = capacity, out = knot count; nothing is written when the
  * capacity is short. */
 int om_efield_time(om_context* ctx, int i, const double* obs, long* n, double* out_t, double* out_e3);
+/* Sum of the given fields at n points (xyz: n rows of x, y, z [m]) into
+ * out3 (n rows), evaluated on the device: get_bfield / get_efield
+ * (src/OSCARSSR_Python.cc:1927-1975, src/OSCARSSR.cc:201-218, 381-398,
+ * src/TFieldContainer.cc:85-107).  The fields are those of this call only; the
+ * context's trajectories and their fields are left as they were.  No fields
+ * gives zeros. */
+int om_field_points(om_context* ctx, const om_field* fields, int nfields, const double* xyz, long n, double* out3);
 /* Status of trajectory i: 0 ok, 1 |beta| >= 1 met during integration. */
 int om_trajectory_status(om_context* ctx, int i);
 

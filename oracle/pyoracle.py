@@ -268,6 +268,11 @@ class Oracle:
         self.L.orc_get_b(C.byref(self.F), _v3(x), 0.0, out.ctypes.data_as(_dp))
         return out
 
+    def get_efield(self, x):
+        out = np.zeros(3)
+        self.L.orc_get_b(C.byref(self.FE), _v3(x), 0.0, out.ctypes.data_as(_dp))
+        return out
+
     def set_particle_beam(self, energy_GeV, x0=(0, 0, 0), d0=(0, 0, 1), current=0.0, type="electron"):
         self.L.orc_particle_electron(C.byref(self.P), float(energy_GeV), _v3(x0), _v3(d0), float(current))
         self.set_ctstartstop(-3, 3)
@@ -525,6 +530,7 @@ def ref_lib():
         L.ref_set_ctstartstop.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.ref_set_npoints_trajectory.argtypes = [C.c_void_p, C.c_long]
         L.ref_get_bfield.argtypes = [C.c_void_p, _dp, _dp]
+        L.ref_get_efield.argtypes = [C.c_void_p, _dp, _dp]
         L.ref_calculate_trajectory.argtypes = [C.c_void_p]
         L.ref_trajectory_npoints.argtypes = [C.c_void_p]
         L.ref_trajectory_npoints.restype = C.c_long
@@ -687,6 +693,11 @@ class RefSim:
     def get_bfield(self, x):
         out = np.zeros(3)
         self.L.ref_get_bfield(self.h, _v3(x), out.ctypes.data_as(_dp))
+        return out
+
+    def get_efield(self, x):
+        out = np.zeros(3)
+        self.L.ref_get_efield(self.h, _v3(x), out.ctypes.data_as(_dp))
         return out
 
     def set_particle_beam(self, energy_GeV, x0=(0, 0, 0), d0=(0, 0, 1), current=0.0, type="electron"):

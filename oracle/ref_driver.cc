@@ -285,6 +285,12 @@ int ref_get_bfield(void* h, double const* x, double* out) {
   return 0;
 }
 
+int ref_get_efield(void* h, double const* x, double* out) {
+  TVector3D E = static_cast<OSCARSSR*>(h)->GetE(v3(x), 0);
+  out[0] = E.GetX(); out[1] = E.GetY(); out[2] = E.GetZ();
+  return 0;
+}
+
 int ref_calculate_trajectory(void* h) {
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);

@@ -60,7 +60,8 @@ class om_stats(C.Structure):
 
 EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context_destroy", "om_last_error",
            "om_trajectories", "om_trajectories_eb", "om_trajectory_set", "om_trajectory_get", "om_power_density_surface",
-           "om_total_power", "om_efield_time", "om_power_density_stl", "om_level_get", "om_trajectory_status", "om_flux",
+           "om_total_power", "om_efield_time", "om_field_points", "om_power_density_stl", "om_level_get",
+           "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
 
@@ -108,6 +109,7 @@ def lib():
                                            _dp, _ip, _up, C.POINTER(om_stats)]
     L.om_total_power.argtypes = [C.c_void_p, C.c_int, C.POINTER(om_conv), _dp, _dp, _ip, C.POINTER(om_stats)]
     L.om_efield_time.argtypes = [C.c_void_p, C.c_int, _dp, _lp, _dp, _dp]
+    L.om_field_points.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, _dp, C.c_long, _dp]
     L.om_power_density_stl.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, C.POINTER(om_conv), _dp, _ip, _up,
                                        C.POINTER(om_stats)]
     _vp = C.c_void_p  # device addresses
@@ -302,6 +304,14 @@ class Device:
         e = np.zeros((n.value, 3))
         self._chk(self.L.om_efield_time(self.h, int(particle), _ptr(o), C.byref(n), _ptr(t), _ptr(e)))
         return t, e
+
+    def field_points(self, fields, xyz):
+        """Sum of `fields` at the rows of xyz[n, 3], evaluated on the device: [n, 3]."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros_like(x)
+        fa = (om_field * max(1, len(fields)))(*fields)
+        self._chk(self.L.om_field_points(self.h, fa, len(fields), _ptr(x), len(x), _ptr(out)))
+        return out
 
     def total_power(self, particle, conv):
         """(power [W], precision, level, stats); level -1: not converged."""

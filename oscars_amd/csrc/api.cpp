@@ -31,6 +31,8 @@ hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, h
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s, int nefields);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
+hipError_t om_launch_field_points(const om::FieldDev* F, int nf, const double* xyz, long n, double* out,
+                                  hipStream_t s);
 hipError_t om_launch_efield_time(const double* p, long n, const double* obs, double c0, double dt, double* out_t,
                                  double* out_e, hipStream_t s);
 int om_tp_blocks(long npts);
@@ -219,7 +221,8 @@ struct om_context {
   DevBuf<unsigned char> d_outconv;
   DevBuf<unsigned long long> d_stats;
   DevBuf<double> d_bd;
-  DevBuf<double> d_normals, d_tp, d_tp_partial, d_et, d_stl;
+  DevBuf<double> d_normals, d_tp, d_tp_partial, d_et, d_stl, d_fp, d_fp_out;
+  DevBuf<om::FieldDev> d_fp_fields;
   std::vector<om::FieldDev> fields_host;
   int nfields = 0;
   int nefields = 0;  // electric fields, after the magnetic ones in d_fields
@@ -304,7 +307,7 @@ struct om_context {
     d_prec.release(); d_out.release(); d_outcomp.release(); d_oute.release(); d_level.release();
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
     d_list.release(); d_border.release();
-    d_normals.release(); d_tp.release(); d_tp_partial.release(); d_et.release(); d_stl.release();
+    d_normals.release(); d_tp.release(); d_tp_partial.release(); d_et.release(); d_stl.release(); d_fp.release(); d_fp_out.release(); d_fp_fields.release();
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) (void) hipStreamDestroy(stream);
@@ -1021,6 +1024,38 @@ int om_efield_time(om_context* ctx, int i, const double* obs, long* n, double* o
           "efield time");
     read_back(ctx, out_t, ctx->d_et.p, D.n * sizeof(double));
     read_back(ctx, out_e3, ctx->d_et.p + D.n, 3 * D.n * sizeof(double));
+  });
+}
+
+int om_field_points(om_context* ctx, const om_field* fields, int nfields, const double* xyz, long n, double* out3) {
+  return guarded(ctx, [&] {
+    if (nfields < 0 || nfields > om::kMaxFields) throw std::out_of_range("too many fields");
+    if (n < 0 || (n > 0 && (!xyz || !out3))) throw std::invalid_argument("missing inputs");
+    if (n == 0) return;
+    if (nfields == 0) {
+      std::memset(out3, 0, 3 * n * sizeof(double));
+      return;
+    }
+    // Grid samples for these fields go up after the trajectory's own and come
+    // down again before returning: the trajectory's fields stay as they are.
+    const size_t keep = ctx->grid_bufs.size();
+    try {
+      std::vector<om::FieldDev> F;
+      for (int i = 0; i < nfields; ++i) F.push_back(make_field(ctx, fields[i]));
+      upload_staged(ctx, ctx->stage_fields, ctx->d_fp_fields, F.data(), F.size());
+      upload_staged(ctx, ctx->stage_rb, ctx->d_fp, xyz, (size_t) (3 * n));
+      ctx->d_fp_out.ensure(3 * n);
+      check(om_launch_field_points(ctx->d_fp_fields.p, nfields, ctx->d_fp.p, n, ctx->d_fp_out.p, ctx->stream),
+            "field points");
+      read_back(ctx, out3, ctx->d_fp_out.p, 3 * n * sizeof(double));
+    } catch (...) {
+      check(hipStreamSynchronize(ctx->stream), "sync");
+      for (size_t k = keep; k < ctx->grid_bufs.size(); ++k) ctx->grid_bufs[k].release();
+      ctx->grid_bufs.resize(keep);
+      throw;
+    }
+    for (size_t k = keep; k < ctx->grid_bufs.size(); ++k) ctx->grid_bufs[k].release();
+    ctx->grid_bufs.resize(keep);
   });
 }
 

@@ -316,6 +316,21 @@ __global__ void __launch_bounds__(256) efield_time_kernel(const double* __restri
   out_t[i] = (double) i * dt + D / kC;
 }
 
+// ---------------------------------------------------------------- field values at points
+// Sum of the fields at n points (x, y, z rows): one point per lane.
+
+__global__ void __launch_bounds__(256) field_points_kernel(const FieldDev* __restrict__ F, int nf,
+                                                           const double* __restrict__ xyz, long n,
+                                                           double* __restrict__ out) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double B[3];
+  get_b(F, nf, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], B);
+  out[3 * i + 0] = B[0];
+  out[3 * i + 1] = B[1];
+  out[3 * i + 2] = B[2];
+}
+
 // ---------------------------------------------------------------- level samples
 
 __global__ void __launch_bounds__(256) level_kernel(LevelJob J) {
@@ -472,6 +487,13 @@ extern "C" hipError_t om_launch_efield_time(const double* p, long n, const doubl
   unsigned blocks = (unsigned) ((n + 255) / 256);
   hipLaunchKernelGGL(om::efield_time_kernel, dim3(blocks), dim3(256), 0, s, p, n, obs[0], obs[1], obs[2], c0, dt,
                      out_t, out_e);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_field_points(const om::FieldDev* F, int nf, const double* xyz, long n, double* out,
+                                             hipStream_t s) {
+  unsigned blocks = (unsigned) ((n + 255) / 256);
+  hipLaunchKernelGGL(om::field_points_kernel, dim3(blocks), dim3(256), 0, s, F, nf, xyz, n, out);
   return hipGetLastError();
 }
 

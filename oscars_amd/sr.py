@@ -52,6 +52,12 @@ def _warn(msg):
     sys.stderr.write(msg)
 
 
+def _named(f, name):
+    """Tag a field with its name (get_bfield / get_efield select by it)."""
+    f.name = name
+    return f
+
+
 def surface_points(points, normal=0, rotations=None, translation=None):
     """Observation points and unit normals of calculate_power_density: each
     [[x, y, z], [nx, ny, nz]] rotated, the normal reversed for normal = -1,
@@ -95,6 +101,9 @@ class sr:
         self._traj_valid = False
         self.last_stats = None
         self._stl = []           # (name, tri[n, 15], scale) per STL object
+        self._sum_spectrum = []  # add_to_spectrum: [[E, V], ...]
+        self._sum_flux = []      # add_to_flux: [[[x, y, z], V], ...]
+        self._sum_pd = []        # add_to_power_density: as add_to_flux
 
     # ------------------------------------------------------------ device
     def _device(self):
@@ -137,7 +146,7 @@ class sr:
         f.translation[:] = tr
         f.phase = float(phase)
         f.taper = float(taper)
-        self._fields.append(f)
+        self._fields.append(_named(f, name))
         self._traj_valid = False
 
     def add_bfield_epu(self, bfieldA, bfieldB, period, nperiods, phase=0.0, rotations=None, translation=None,
@@ -155,7 +164,7 @@ class sr:
         f.translation[:] = tr
         f.phase = float(phase)
         f.taper = float(taper)
-        self._fields.append(f)
+        self._fields.append(_named(f, name))
         self._traj_valid = False
 
     def _simple_field(self, ftype, name, frequency, rotations, translation):
@@ -179,7 +188,7 @@ class sr:
         f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, rotations, translation)
         f.field[:] = B
         f.shape[:] = S
-        self._fields.append(f)
+        self._fields.append(_named(f, name))
         self._traj_valid = False
 
     def add_bfield_uniform(self, bfield, width=None, rotations=None, translation=None, frequency=0,
@@ -192,7 +201,7 @@ class sr:
         f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, rotations, translation)
         f.field[:] = B
         f.shape[:] = W
-        self._fields.append(f)
+        self._fields.append(_named(f, name))
         self._traj_valid = False
 
     def add_bfield_quadrupole(self, K, width, rotations=None, translation=None, frequency=0, frequency_phase=0,
@@ -202,7 +211,7 @@ class sr:
         f = self._simple_field(_lib.OM_FIELD_QUADRUPOLE, name, frequency, rotations, translation)
         f.K = float(K)
         f.width = float(width)
-        self._fields.append(f)
+        self._fields.append(_named(f, name))
         self._traj_valid = False
 
     # ------------------------------------------------------------ electric fields
@@ -217,7 +226,7 @@ class sr:
         f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, rotations, translation)
         f.field[:] = E
         f.shape[:] = S
-        self._add_efield(f)
+        self._add_efield(_named(f, name))
 
     def add_efield_uniform(self, efield, width=None, rotations=None, translation=None, frequency=0,
                            frequency_phase=0, time_offset=0, name=""):
@@ -227,7 +236,7 @@ class sr:
         f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, rotations, translation)
         f.field[:] = E
         f.shape[:] = W
-        self._add_efield(f)
+        self._add_efield(_named(f, name))
 
     def add_efield_undulator(self, efield, period, nperiods, phase=0.0, translation=None, taper=0.0, frequency=0,
                              frequency_phase=0, time_offset=0, name=""):
@@ -293,6 +302,12 @@ class sr:
         [x, y, z step factors, Bx, By, Bz factors]) or 'OSCARS1D <columns>'
         (an irregular 1-D table resampled at 10000 points per metre)
         (src/TField3D_Grid.cc:353-873)."""
+        n0 = len(self._fields)
+        self._read_bfield_file(ifile, iformat, rotations, translation, scale)
+        if len(self._fields) > n0:
+            _named(self._fields[-1], name)
+
+    def _read_bfield_file(self, ifile, iformat, rotations, translation, scale):
         fmt = iformat.upper()
         sc = [float(v) for v in scale] if scale else []
         if fmt == "OSCARS":
@@ -352,6 +367,7 @@ class sr:
         data = spline_at([a for a, _ in pairs], np.stack(samples), float(parameter))
         g = field_grid_scaled(start, step, npts, data, sc[:3])
         self._add_grid_exact(g[0], g[1], g[2], npts, g[3], rotations, translation)
+        _named(self._fields[-1], name)
 
     def clear_bfields(self):
         # src/OSCARSSR.cc:186-196: the current trajectory stays until a field is added
@@ -993,6 +1009,93 @@ class sr:
             raise ValueError("total power calculation did not converge.  Try increasing max_level or decreasing "
                              "precision")
         return {"POWER": pw, "PRECISION": pr, "LEVEL": float(lv)}[Q]
+
+    # ------------------------------------------------------------ field values
+    def _field_at(self, fields, x, name):
+        try:
+            X = _v3(x, "x")
+        except ValueError:
+            raise ValueError("Incorrect format in input")
+        sel = [f for f in fields if name == "" or getattr(f, "name", "") == name]
+        return self._device().field_points(sel, [X])[0].tolist()
+
+    def get_bfield(self, x, t=0.0, name=""):
+        """Summed magnetic field [T] at x, or of the fields named `name` only
+        (src/OSCARSSR_Python.cc:1927-1975, src/TFieldContainer.cc:85-107).
+        The fields here do not depend on time, so t has no effect."""
+        return self._field_at(self._fields, x, name)
+
+    def get_efield(self, x, t=0.0, name=""):
+        """Summed electric field [V/m] at x, as get_bfield."""
+        return self._field_at(self._efields, x, name)
+
+    # ------------------------------------------------------------ accumulated results
+    @staticmethod
+    def _accumulate(store, new, weight):
+        # src/OSCARSSR.cc:2465-2530: the first call sets the points, later
+        # calls add weight * value to the values point by point
+        if len(store) == 0:
+            return [[p, v * weight] for p, v in new]
+        if len(store) != len(new):
+            raise ValueError("spectra dimensions do not match")
+        return [[p, v + w * weight] for (p, v), (_, w) in zip(store, new)]
+
+    def add_to_spectrum(self, spectrum, weight=1.0):
+        """Add weight * spectrum ([[E, V], ...]) to the stored spectrum
+        (src/OSCARSSR_Python.cc:8623-8656)."""
+        if len(spectrum) < 1:
+            raise ValueError("No points in spectrum.")
+        new = []
+        for p in spectrum:
+            if len(p) != 2:
+                raise ValueError("GetSpectrumFromList reporting not 2 points")
+            new.append((float(p[0]), float(p[1])))
+        self._sum_spectrum = self._accumulate(self._sum_spectrum, new, float(weight))
+
+    def get_spectrum(self):
+        """The stored spectrum [[E, V], ...] (src/OSCARSSR_Python.cc:8675-8680)."""
+        return [[e, v] for e, v in self._sum_spectrum]
+
+    def write_spectrum(self, ofile="", bofile=""):
+        """Write the stored spectrum as text and/or binary
+        (src/OSCARSSR_Python.cc:8381-8420)."""
+        e = [p for p, _ in self._sum_spectrum]
+        v = [w for _, w in self._sum_spectrum]
+        if ofile != "":
+            IO.write_spectrum_text(ofile, e, v)
+        if bofile != "":
+            IO.write_spectrum_binary(bofile, e, v)
+
+    @staticmethod
+    def _points_list(points, what):
+        if len(points) < 1:
+            raise ValueError("No points in %s." % what)
+        new = []
+        for p in points:
+            if len(p) != 2:
+                raise ValueError("GetT3DScalarContainerFromList reporting not 2 points")
+            if len(p[0]) != 3:
+                raise ValueError("number of elements not 3")
+            new.append(([float(c) for c in p[0]], float(p[1])))
+        return new
+
+    def add_to_flux(self, flux, weight=1.0):
+        """Add weight * flux ([[[x, y, z], V], ...]) to the stored flux
+        (src/OSCARSSR_Python.cc:8877-8910)."""
+        self._sum_flux = self._accumulate(self._sum_flux, self._points_list(flux, "flux"), float(weight))
+
+    def get_flux(self):
+        """The stored flux [[[x, y, z], V], ...] (src/OSCARSSR_Python.cc:8929-8934)."""
+        return [[list(p), v] for p, v in self._sum_flux]
+
+    def add_to_power_density(self, power_density, weight=1.0):
+        """Add weight * power density ([[[x, y, z], V], ...]) to the stored one
+        (src/OSCARSSR_Python.cc:8957-8990; its empty-input message says flux)."""
+        self._sum_pd = self._accumulate(self._sum_pd, self._points_list(power_density, "flux"), float(weight))
+
+    def get_power_density(self):
+        """The stored power density (src/OSCARSSR_Python.cc:9009-9014)."""
+        return [[list(p), v] for p, v in self._sum_pd]
 
     # ------------------------------------------------------------ averaging of saved results
     @staticmethod
