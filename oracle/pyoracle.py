@@ -531,6 +531,8 @@ def ref_lib():
         L.ref_set_npoints_trajectory.argtypes = [C.c_void_p, C.c_long]
         L.ref_get_bfield.argtypes = [C.c_void_p, _dp, _dp]
         L.ref_get_efield.argtypes = [C.c_void_p, _dp, _dp]
+        L.ref_set_beam_append.argtypes = [C.c_int, C.c_double]
+        L.ref_set_beam_append.restype = None
         L.ref_calculate_trajectory.argtypes = [C.c_void_p]
         L.ref_trajectory_npoints.argtypes = [C.c_void_p]
         L.ref_trajectory_npoints.restype = C.c_long
@@ -721,6 +723,14 @@ class RefSim:
                                            opt(beta, 2), opt(alpha, 2), opt(gamma, 2), opt(lattice_reference, 3),
                                            opt(eta, 2), None if distribution is None else distribution.encode(),
                                            None if beam is None else beam.encode()))
+
+    def add_particle_beam_full(self, weight=1.0, **kw):
+        """add_particle_beam: set_particle_beam_full keeping the beams there."""
+        self.L.ref_set_beam_append(1, float(weight))
+        try:
+            self.set_particle_beam_full(**kw)
+        finally:
+            self.L.ref_set_beam_append(0, 1.0)
 
     def set_seed(self, seed):
         self.L.ref_set_seed(self.h, int(seed))

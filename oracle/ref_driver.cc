@@ -184,6 +184,16 @@ int ref_set_particle(void* h, double const* x0, double const* beta0, double t0) 
 
 // sr.set_particle_beam with the beam-spread arguments, in the binding's order
 // (src/OSCARSSR_Python.cc:3873-4040).  A null pointer is an argument not given.
+// add_particle_beam in place of set_particle_beam for the next
+// ref_set_beam_full calls: keep the beams already there, give the new one this
+// weight and leave the current particle alone (src/OSCARSSR_Python.cc:3561-3580).
+static int g_append_beam = 0;
+static double g_beam_weight = 1;
+void ref_set_beam_append(int append, double weight) {
+  g_append_beam = append;
+  g_beam_weight = weight;
+}
+
 int ref_set_beam_full(void* h, double energy_GeV, double const* x0, double const* d0, double t0, double current,
                       double const* horizontal, double const* emittance, double sigma_energy_GeV,
                       double const* beta, double const* alpha, double const* gamma,
@@ -191,19 +201,20 @@ int ref_set_beam_full(void* h, double energy_GeV, double const* x0, double const
                       char const* predefined) {
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);
-    s->ClearParticleBeams();
+    if (!g_append_beam) s->ClearParticleBeams();
+    double const W = g_append_beam ? g_beam_weight : 1;
     TVector3D Direction = d0 ? v3(d0) : TVector3D(0, 0, 1);
     TVector3D Horizontal = horizontal ? v3(horizontal) : -Direction.Orthogonal();
     Horizontal = Horizontal.UnitVector();
     TParticleBeam* b = 0x0;
     if (predefined) {
       // a predefined beam (null d0: direction not given; energy < 0: not given)
-      b = &(s->AddParticleBeam(std::string(predefined), "", 1));
+      b = &(s->AddParticleBeam(std::string(predefined), "", W));
       if (energy_GeV >= 0) b->SetE0(energy_GeV);
       if (d0) b->SetU0(Direction);
       b->SetX0(v3(x0));
     } else {
-      b = &(s->AddParticleBeam("electron", "", v3(x0), Direction, energy_GeV, t0, current, 1));
+      b = &(s->AddParticleBeam("electron", "", v3(x0), Direction, energy_GeV, t0, current, W));
     }
     b->SetHorizontalDirection(Horizontal);
     if (emittance) {
@@ -232,7 +243,7 @@ int ref_set_beam_full(void* h, double energy_GeV, double const* x0, double const
     }
     b->SetEta(eta ? TVector2D(eta[0], eta[1]) : TVector2D(0, 0));
     s->SetCTStartStop(-3, 3);
-    s->SetNewParticle("", "ideal");
+    if (!g_append_beam) s->SetNewParticle("", "ideal");
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }

@@ -203,6 +203,23 @@ def distribution_of(emittance, distribution=""):
 
 
 # ---------------------------------------------------------------- sampling
+def _frame(beam, u0):
+    hin = beam.get("horizontal_unit")
+    if hin is None:
+        hd = beam.get("horizontal_direction")
+        hin = G.unit([float(v) for v in hd] if hd else [-v for v in G.orthogonal(beam["d0"])])
+    return beam_directions(u0, hin)
+
+
+def beam_frame(beam):
+    """(horizontal, vertical) unit directions of a beam description, the
+    reference's fHorizontalDirection / fVerticalDirection
+    (src/TParticleBeam.cc:374-379)."""
+    ideal = G.ideal_particle(beam.get("type", "electron"), beam["energy_GeV"], beam["x0"], beam["d0"],
+                             beam.get("t0", 0.0), beam.get("current", 0.0), beam["q"], beam["m"])
+    return _frame(beam, beam.get("u0") or ideal.u0)
+
+
 def sample_particles(beam, n, seed=None, rng=None):
     """n particles (geometry.Particle) drawn from a beam description (the dict
     kept by sr.set_particle_beam).  Draws come from ``rng`` (anything with
@@ -217,11 +234,7 @@ def sample_particles(beam, n, seed=None, rng=None):
                              beam.get("current", 0.0), q, m)
     E0 = ideal.e0
     u0 = beam.get("u0") or ideal.u0
-    hin = beam.get("horizontal_unit")
-    if hin is None:
-        hd = beam.get("horizontal_direction")
-        hin = G.unit([float(v) for v in hd] if hd else [-v for v in G.orthogonal(beam["d0"])])
-    hdir, vdir = beam_directions(u0, hin)
+    hdir, vdir = _frame(beam, u0)
     eps = [float(v) for v in beam["emittance"]] if beam.get("emittance") is not None else [0.0, 0.0]
     dist = beam.get("distribution") or distribution_of(beam.get("emittance"))
     tw = twiss_from(beam.get("beta"), beam.get("alpha"), beam.get("gamma"))

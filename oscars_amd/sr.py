@@ -88,7 +88,9 @@ class sr:
         self._fields = []        # list of dicts kept for om_field construction
         self._efields = []       # electric fields (om_field)
         self._grids = []         # keep numpy arrays alive
-        self._beam = None
+        self._beam = None        # the first beam's description
+        self._beams = []         # (name, description, weight) per beam
+        self._wsum = []          # running sums of the weights
         self._particle = None
         self._ctstart = 0.0
         self._ctstop = 0.0
@@ -375,11 +377,44 @@ class sr:
         self._grids = []
 
     # ------------------------------------------------------------ beam / trajectory window
-    def set_particle_beam(self, type="electron", name="", energy_GeV=-1, d0=None, x0=None, beam="",
+    def set_particle_beam(self, **kw):
+        """Clear the beams, add this one and make its ideal particle current
+        (src/OSCARSSR_Python.cc:3561-3580); keywords as add_particle_beam."""
+        self.clear_particle_beams()
+        self.add_particle_beam(**kw)
+
+    def clear_particle_beams(self):
+        """Remove every beam; the current particle stays (src/OSCARSSR.cc:540-547)."""
+        self._beams, self._wsum, self._beam = [], [], None
+
+    def _beam_index(self):
+        # src/TParticleBeamContainer.cc:135-166: one uniform draw picks a beam
+        # by weight when there is more than one
+        if len(self._beams) == 1:
+            return 0
+        r = B.stream().random() * self._wsum[-1]
+        for i, w in enumerate(self._wsum):
+            if r < w:
+                return i
+        raise ValueError("did not find a beam for this weight")
+
+    def _beam_by_name(self, name):
+        """A beam by name, or one drawn by weight for "" (src/TParticleBeamContainer.cc:107-121)."""
+        if not self._beams:
+            raise ValueError("No particle beam defined")
+        if name == "":
+            return self._beams[self._beam_index()][1]
+        for n, b, _ in self._beams:
+            if n == name:
+                return b
+        raise ValueError("beam name not in map")
+
+    def add_particle_beam(self, type="electron", name="", energy_GeV=-1, d0=None, x0=None, beam="",
                           sigma_energy_GeV=0, t0=0, current=0, weight=1, rotations=None, translation=None,
                           horizontal_direction=None, beta=None, alpha=None, gamma=None, emittance=None, eta=None,
                           lattice_reference=None, mass=0, charge=0, ctstartstop=None, distribution=""):
-        # src/OSCARSSR_Python.cc:3795-4040
+        """Add a beam; random particles come from the beams in proportion to
+        their weights (src/OSCARSSR_Python.cc:3699-4070)."""
         pre = None
         if beam:
             pre = B.predefined_beam(beam)
@@ -428,10 +463,20 @@ class sr:
                 beam_p.update(beta=pre["beta"], alpha=[0.0, 0.0], gamma=None)
         if distribution:
             dist = B.distribution_of(emittance, distribution)
-        self._beam = dict(type=type, name=name, energy_GeV=energy_GeV, x0=pos, d0=direction, t0=float(t0),
-                          current=float(current), sigma_energy_GeV=float(sigma_energy_GeV),
-                          horizontal_unit=hdir, distribution=dist, q=p.q, m=p.m, **beam_p)
-        self._particle = p
+        my = name if name != "" else "_beam%d" % len(self._beams)
+        if any(n == my for n, _, _ in self._beams):
+            raise ValueError("Error in predefined beam name / definition" if pre is not None else
+                             "invalid argument in adding particle beam.  possibly 'name' already exists")
+        b = dict(type=type, name=my, energy_GeV=energy_GeV, x0=pos, d0=direction, t0=float(t0),
+                 current=float(current), sigma_energy_GeV=float(sigma_energy_GeV),
+                 horizontal_unit=hdir, distribution=dist, q=p.q, m=p.m, **beam_p)
+        self._beams.append((my, b, float(weight)))
+        self._wsum.append(float(weight) + (self._wsum[-1] if self._wsum else 0.0))
+        if len(self._beams) == 1:
+            # the first beam: its ideal particle is current (set_particle_beam's
+            # SetNewParticle("", "ideal") draws nothing with one beam)
+            self._beam = b
+            self._particle = p
         if ctstartstop is not None:
             if len(ctstartstop) != 2:
                 raise ValueError("Incorrect format in 'ctstartstop'")
@@ -441,15 +486,17 @@ class sr:
         self._traj_valid = False
 
     def set_new_particle(self, beam="", particle=""):
+        """A new current particle from the named beam, or one drawn by weight:
+        ideal, or random for anything else (src/OSCARSSR.cc:506-520)."""
         if self._beam is None:
             raise ValueError("No particle beam defined")
+        b = self._beam_by_name(beam)
         if particle.lower() == "ideal":
-            b = self._beam
             self._particle = G.ideal_particle(b["type"], b["energy_GeV"], b["x0"], b["d0"], b["t0"], b["current"],
                                               b["q"], b["m"])
         else:
             # src/TParticleBeam.cc:667-689: anything but "ideal" draws a random particle
-            self._particle = B.sample_particles(self._beam, 1)[0]
+            self._particle = B.sample_particles(b, 1)[0]
         self._traj_valid = False
 
     def set_particle(self, x0, beta0, t0=0.0):
@@ -654,7 +701,43 @@ class sr:
         iteration of src/OSCARSSR.cc:1782-1790)."""
         if nparticles == 0:
             return None
-        return B.sample_particles(self._beam, nparticles)
+        if len(self._beams) == 1:
+            return B.sample_particles(self._beam, nparticles)
+        return [B.sample_particles(self._beam_by_name(""), 1)[0] for _ in range(nparticles)]
+
+    def get_emittance(self, name=""):
+        """[horizontal, vertical] emittance of a beam (src/OSCARSSR_Python.cc:4396-4410)."""
+        e = self._beam_by_name(name).get("emittance")
+        return [float(e[0]), float(e[1])] if e is not None else [0.0, 0.0]
+
+    def get_beam_horizontal_direction(self, name=""):
+        """Unit horizontal direction of a beam (src/OSCARSSR_Python.cc:4745-4760)."""
+        return list(B.beam_frame(self._beam_by_name(name))[0])
+
+    def get_beam_vertical_direction(self, name=""):
+        """Unit vertical direction of a beam, -unit(horizontal x u0)."""
+        return list(B.beam_frame(self._beam_by_name(name))[1])
+
+    def set_twiss_parameters(self, beam="", beta=None, alpha=None, gamma=None, lattice_reference=None):
+        """Twiss parameters of a beam, any two of beta, alpha, gamma or all three,
+        or beta alone with alpha 0 (src/OSCARSSR_Python.cc:4269-4360)."""
+        b = self._beam_by_name(beam)
+        two = lambda v, n: None if v is None else _v2(v, n)  # noqa: E731
+        tw = dict(beta=two(beta, "beta"), alpha=two(alpha, "alpha"), gamma=two(gamma, "gamma"))
+        ref = None if lattice_reference is None else _v3(lattice_reference, "lattice_reference")
+        if B.twiss_from(tw["beta"], tw["alpha"], tw["gamma"]) is None:
+            return
+        b.update(tw, lattice_reference=ref)
+        self._traj_valid = False
+
+    def set_particle_beam_size(self, sigma=None, sigmap=None, beam="", lattice_reference=None, eta=None):
+        """Checks its arguments and changes nothing, as the reference's binding
+        does (src/OSCARSSR_Python.cc:4105-4160)."""
+        for v, n in ((sigma, "sigma"), (sigmap, "sigmap")):
+            if v is not None and len(v) != 0:
+                _v2(v, n)
+        if lattice_reference is not None:
+            _v3(lattice_reference, "lattice_reference")
 
     def _batched(self, nparticles, obs, energies, pol, precision, max_level, max_level_extended, mode):
         """nparticles particles in one persistent launch (tiled / mixed kernels)."""

@@ -238,3 +238,66 @@ def test_nparticles_spectrum_matches_reference_build():
         o.calculate_trajectory()
         tot += o.spectrum([0, 0, 30], e)[0] * (1.0 / 3)
     assert np.max(np.abs(tot - vref) / np.abs(vref)) < 1e-14
+
+
+def test_several_beams_live_against_reference_build():
+    """Two beams of weights 1 and 3: each particle first draws its beam by
+    weight, then its own offsets, as the reference does draw for draw."""
+    P = _ref()
+    s = P.RefSim()
+    a, b = BEAM_CASES[0], BEAM_CASES[2]
+    s.set_particle_beam_full(**a)
+    s.add_particle_beam_full(weight=3.0, **b)
+    s.set_seed(11)
+    ref = []
+    for _ in range(600):
+        s.new_particle()
+        p = s.particle()
+        ref.append(p["x0"] + p["b0"])
+    o = sr()
+    o.set_particle_beam(**a)
+    o.add_particle_beam(weight=3.0, **b)
+    o.set_seed(11)
+    got = np.array([p.x0 + p.b0 for p in o._particles_for(600)])
+    assert np.array_equal(got, np.array(ref))
+    # the beams are picked about 1:3
+    o.set_seed(12)
+    picks = np.array([o._beam_index() for _ in range(4000)])
+    assert abs(np.mean(picks == 1) - 0.75) < 0.03
+
+
+def test_beam_accessors():
+    o = sr()
+    with pytest.raises(ValueError, match="No particle beam defined"):
+        o.get_emittance()
+    o.set_particle_beam(**BEAM_CASES[1])
+    o.add_particle_beam(name="second", weight=2.0, **BEAM_CASES[2])
+    assert o.get_emittance(name="_beam0") == BEAM_CASES[1]["emittance"]
+    assert o.get_emittance(name="second") == BEAM_CASES[2]["emittance"]
+    with pytest.raises(ValueError, match="beam name not in map"):
+        o.get_emittance(name="third")
+    with pytest.raises(ValueError, match="possibly 'name' already exists"):
+        o.add_particle_beam(name="second", **BEAM_CASES[2])
+    h = np.array(o.get_beam_horizontal_direction(name="_beam0"))
+    v = np.array(o.get_beam_vertical_direction(name="_beam0"))
+    u0 = np.array(BEAM_CASES[1]["d0"], dtype=float) / np.linalg.norm(BEAM_CASES[1]["d0"])
+    assert abs(np.linalg.norm(h) - 1) < 1e-15 and abs(np.linalg.norm(v) - 1) < 1e-15
+    assert abs(h @ v) < 1e-15 and abs(v @ u0) < 1e-15
+    assert np.allclose(v, -np.cross(h, u0) / np.linalg.norm(np.cross(h, u0)), rtol=0, atol=1e-16)
+    o.set_particle_beam_size(sigma=[1e-5, 1e-6], sigmap=[1e-6, 1e-7])  # accepted, no effect
+    o.clear_particle_beams()
+    with pytest.raises(ValueError, match="No particle beam defined"):
+        o.set_new_particle()
+
+
+def test_set_twiss_parameters_is_the_set_particle_beam_form():
+    """Twiss parameters given afterwards sample the same particles as the same
+    parameters given to set_particle_beam (pinned to the reference above)."""
+    kw = dict(BEAM_CASES[1])
+    tw = {k: kw.pop(k) for k in ("beta", "alpha", "lattice_reference")}
+    o = sr()
+    o.set_particle_beam(**kw, beta=[9.0, 9.0])
+    o.set_twiss_parameters(**tw)
+    o.set_seed(3)
+    got = np.array([p.x0 + p.b0 for p in o._particles_for(200)])
+    assert np.array_equal(got, _sr_particles(BEAM_CASES[1], 3, 200))
