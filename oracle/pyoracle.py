@@ -531,6 +531,8 @@ def ref_lib():
         L.ref_set_npoints_trajectory.argtypes = [C.c_void_p, C.c_long]
         L.ref_get_bfield.argtypes = [C.c_void_p, _dp, _dp]
         L.ref_get_efield.argtypes = [C.c_void_p, _dp, _dp]
+        L.ref_remove_efield.argtypes = [C.c_void_p, C.c_char_p]
+        L.ref_remove_efield.restype = None
         L.ref_set_beam_append.argtypes = [C.c_int, C.c_double]
         L.ref_set_beam_append.restype = None
         L.ref_calculate_trajectory.argtypes = [C.c_void_p]
@@ -701,6 +703,9 @@ class RefSim:
         out = np.zeros(3)
         self.L.ref_get_efield(self.h, _v3(x), out.ctypes.data_as(_dp))
         return out
+
+    def remove_efield(self, name=""):
+        self.L.ref_remove_efield(self.h, name.encode())
 
     def set_particle_beam(self, energy_GeV, x0=(0, 0, 0), d0=(0, 0, 1), current=0.0, type="electron"):
         self._chk(self.L.ref_set_beam(self.h, float(energy_GeV), _v3(x0), _v3(d0), float(current)))

@@ -296,6 +296,8 @@ int ref_get_bfield(void* h, double const* x, double* out) {
   return 0;
 }
 
+void ref_remove_efield(void* h, char const* name) { static_cast<OSCARSSR*>(h)->RemoveElectricField(name); }
+
 int ref_get_efield(void* h, double const* x, double* out) {
   TVector3D E = static_cast<OSCARSSR*>(h)->GetE(v3(x), 0);
   out[0] = E.GetX(); out[1] = E.GetY(); out[2] = E.GetZ();

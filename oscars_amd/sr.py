@@ -87,6 +87,7 @@ class sr:
     def __init__(self, nthreads=2, gpu=0, device=0):
         self._fields = []        # list of dicts kept for om_field construction
         self._efields = []       # electric fields (om_field)
+        self._stale_eb = None    # E+B equations kept after a removal (None: follow the fields)
         self._grids = []         # keep numpy arrays alive
         self._beam = None        # the first beam's description
         self._beams = []         # (name, description, weight) per beam
@@ -148,7 +149,7 @@ class sr:
         f.translation[:] = tr
         f.phase = float(phase)
         f.taper = float(taper)
-        self._fields.append(_named(f, name))
+        self._push_b(_named(f, name))
         self._traj_valid = False
 
     def add_bfield_epu(self, bfieldA, bfieldB, period, nperiods, phase=0.0, rotations=None, translation=None,
@@ -166,7 +167,7 @@ class sr:
         f.translation[:] = tr
         f.phase = float(phase)
         f.taper = float(taper)
-        self._fields.append(_named(f, name))
+        self._push_b(_named(f, name))
         self._traj_valid = False
 
     def _simple_field(self, ftype, name, frequency, rotations, translation):
@@ -190,7 +191,7 @@ class sr:
         f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, rotations, translation)
         f.field[:] = B
         f.shape[:] = S
-        self._fields.append(_named(f, name))
+        self._push_b(_named(f, name))
         self._traj_valid = False
 
     def add_bfield_uniform(self, bfield, width=None, rotations=None, translation=None, frequency=0,
@@ -203,7 +204,7 @@ class sr:
         f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, rotations, translation)
         f.field[:] = B
         f.shape[:] = W
-        self._fields.append(_named(f, name))
+        self._push_b(_named(f, name))
         self._traj_valid = False
 
     def add_bfield_quadrupole(self, K, width, rotations=None, translation=None, frequency=0, frequency_phase=0,
@@ -213,12 +214,13 @@ class sr:
         f = self._simple_field(_lib.OM_FIELD_QUADRUPOLE, name, frequency, rotations, translation)
         f.K = float(K)
         f.width = float(width)
-        self._fields.append(_named(f, name))
+        self._push_b(_named(f, name))
         self._traj_valid = False
 
     # ------------------------------------------------------------ electric fields
     def _add_efield(self, f):
         self._efields.append(f)
+        self._stale_eb = None
         self._traj_valid = False
 
     def add_efield_gaussian(self, efield, sigma, rotations=None, translation=None, frequency=0, frequency_phase=0,
@@ -262,8 +264,38 @@ class sr:
                                      frequency_phase, time_offset, name, ofile)
         self._add_efield(self._fields.pop(n0))
 
+    def _push_b(self, f):
+        self._fields.append(f)
+        self._stale_eb = None
+
+    def _efields_for_derivs(self):
+        # src/OSCARSSR.cc:1185-1200: the equations of motion are chosen when a
+        # field is added or the fields are cleared, not when one is removed; a
+        # removal that leaves magnetic fields only keeps the E+B equations,
+        # which a zero electric field reproduces here
+        if self._stale_eb and not self._efields and self._fields:
+            f = _lib.om_field()
+            f.type = _lib.OM_FIELD_UNIFORM
+            return [f]
+        return self._efields
+
+    def _remove(self, fields, name):
+        if self._stale_eb is None:
+            self._stale_eb = len(self._efields) > 0 or len(self._fields) == 0
+        return [f for f in fields if name == "" or getattr(f, "name", "") != name]
+
+    def remove_bfield(self, name):
+        """Remove the magnetic fields of this name (src/TFieldContainer.cc:52-70);
+        the current trajectory stays, as in the reference."""
+        self._fields = self._remove(self._fields, name)
+
+    def remove_efield(self, name):
+        """Remove the electric fields of this name."""
+        self._efields = self._remove(self._efields, name)
+
     def clear_efields(self):
         self._efields = []
+        self._stale_eb = None
 
     def add_bfield_grid(self, start, step, npoints, samples, rotations=None, translation=None):
         """Regular 1-D field map from header values and a (n, 3) sample array
@@ -280,7 +312,7 @@ class sr:
         f.rotations[:] = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
         f.translation[:] = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
         self._grids.append(s)
-        self._fields.append(f)
+        self._push_b(f)
         self._traj_valid = False
 
     def _add_grid_exact(self, start, stop, step, npoints, samples, rotations, translation):
@@ -296,7 +328,7 @@ class sr:
         f.rotations[:] = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
         f.translation[:] = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
         self._grids.append(s)
-        self._fields.append(f)
+        self._push_b(f)
         self._traj_valid = False
 
     def add_bfield_file(self, ifile, iformat="OSCARS", rotations=None, translation=None, scale=None, name=""):
@@ -375,6 +407,7 @@ class sr:
         # src/OSCARSSR.cc:186-196: the current trajectory stays until a field is added
         self._fields = []
         self._grids = []
+        self._stale_eb = None
 
     # ------------------------------------------------------------ beam / trajectory window
     def set_particle_beam(self, **kw):
@@ -573,7 +606,7 @@ class sr:
             raise ValueError("start time is greater than T0 or stop time is less than T0")
         if self._ctstart >= self._ctstop:
             raise ValueError("start time is greater than stop time.  check that set_ctstartstop is set")
-        self._device().trajectories(self._fields, [self._om_particle(self._particle)], efields=self._efields)
+        self._device().trajectories(self._fields, [self._om_particle(self._particle)], efields=self._efields_for_derivs())
         self._traj_valid = True
 
     def calculate_trajectory(self, ofile="", bofile="", oformat=""):
@@ -748,7 +781,7 @@ class sr:
         w = [1.0 / len(plist)] * len(plist)
         bf = M.BatchedFlux(dev, self._fields, plist, windows, [(np.asarray(obs), np.asarray(energies))],
                            [0] * len(plist), w, mode=mode, polarization=pol, precision=precision,
-                           max_level=max_level, max_level_extended=max_level_extended, efields=self._efields)
+                           max_level=max_level, max_level_extended=max_level_extended, efields=self._efields_for_derivs())
         self._traj_valid = False
         outs, st = bf.run(recompute_trajectories=False)
         self.last_stats = st
@@ -762,7 +795,7 @@ class sr:
             self._ensure_trajectory()
             return fn(self._device(), 0, 1.0, False)
         dev = self._device()
-        dev.trajectories(self._fields, [self._om_particle(p) for p in plist], efields=self._efields)
+        dev.trajectories(self._fields, [self._om_particle(p) for p in plist], efields=self._efields_for_derivs())
         self._traj_valid = False
         w = 1.0 / float(len(plist))
         out = None

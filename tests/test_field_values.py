@@ -175,3 +175,54 @@ def test_get_bfield_leaves_trajectory_fields_alone():
         other.get_bfield([0, 0, 0.01])
         b = s.calculate_spectrum(**kw)
     assert a == b
+
+
+def test_remove_fields_by_name():
+    """remove_bfield / remove_efield drop every field of that name and keep
+    the rest; "" removes nothing; a removal that leaves magnetic fields only
+    keeps the E+B equations of motion (a zero electric field here)."""
+    from oscars_amd import sr as SR
+    s = SR.sr()
+    s.add_bfield_undulator(**UNDULATOR, name="u")
+    s.add_bfield_gaussian(bfield=[0, 0.8, 0.05], sigma=[0, 0, 0.1], name="g")
+    s.add_bfield_uniform(bfield=[0.1, 0, 0], width=[0, 0, 0.2], name="g")
+    s.add_efield_uniform(efield=[1e5, 0, 0], width=[0, 0, 1], name="e")
+    s.remove_bfield("")
+    assert len(s._fields) == 3
+    s.remove_bfield("g")
+    assert [f.name for f in s._fields] == ["u"]
+    assert len(s._efields_for_derivs()) == 1
+    s.remove_efield("e")
+    assert s._efields == []
+    eb = s._efields_for_derivs()
+    assert len(eb) == 1 and list(eb[0].field) == [0.0, 0.0, 0.0] and list(eb[0].shape) == [0.0, 0.0, 0.0]
+    s.add_bfield_gaussian(bfield=[0, 0.1, 0], sigma=[0, 0, 0.1])  # an addition chooses again
+    assert s._efields_for_derivs() == []
+    t = SR.sr()
+    t.add_bfield_undulator(**UNDULATOR, name="u")
+    t.remove_bfield("u")
+    assert t._efields_for_derivs() == []
+
+
+@pytest.mark.gpu
+def test_removed_efields_keep_the_eb_equations_on_the_device():
+    """After its electric field is removed an undulator trajectory is the
+    oracle's E+B integration with a zero electric field (the reference's own,
+    in tests/test_oracle_vs_ref.py), knots within 1e-13."""
+    from oracle.pyoracle import Oracle
+    from oscars_amd import sr as SR
+    o, s = Oracle(nthreads=1), SR.sr()
+    for x in (o, s):
+        x.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+        x.add_bfield_undulator(**UNDULATOR)
+        x.set_ctstartstop(0, 2)
+        x.set_npoints_trajectory(4001)
+    s.add_efield_uniform(efield=[0, 5e5, 1e4], width=[0, 0, 0.3], translation=[0, 0, 0.5], name="e")
+    s.remove_efield("e")
+    o.add_efield_uniform(efield=[0, 0, 0])
+    T = o.calculate_trajectory()
+    s.calculate_trajectory()
+    t, p, _ = s._device().trajectory(0)
+    err = float(np.max(np.abs(p - T.p) / np.maximum(np.max(np.abs(T.p), axis=0), 1e-300)))
+    assert len(t) == len(T.t) and np.max(np.abs(t - T.t)) <= 1e-13 * np.max(np.abs(T.t))
+    assert err <= 1e-13, err

@@ -274,3 +274,31 @@ def test_electric_fields(magnetic):
     e, v, lev, conv, _ = o.calculate_spectrum([0, 0, 30], npoints=8, energy_range_eV=[100, 800])
     re, rv, _ = r.calculate_spectrum([0, 0, 30], npoints=8, energy_range_eV=[100, 800])
     assert same(v, rv)
+
+
+def test_removed_efields_keep_the_eb_equations():
+    """The reference picks its equations of motion when a field is added or
+    cleared: after its electric fields are removed an undulator is still
+    integrated with the E+B equations, which is the oracle's E+B integration
+    with one zero electric field."""
+    from tests.common import UNDULATOR
+    from oracle.pyoracle import RefSim
+    r = RefSim()
+    r.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    r.add_bfield_undulator(**UNDULATOR)
+    r.add_efield_uniform(efield=[0, 5e5, 1e4], width=[0, 0, 0.3], translation=[0, 0, 0.5])
+    r.remove_efield("")
+    r.set_ctstartstop(0, 2)
+    r.set_npoints_trajectory(4001)
+    rt, rp, _ = r.calculate_trajectory()
+    from oracle.pyoracle import Oracle
+    o = Oracle(nthreads=1)
+    o.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    o.add_bfield_undulator(**UNDULATOR)
+    o.set_ctstartstop(0, 2)
+    o.set_npoints_trajectory(4001)
+    plain = o.calculate_trajectory().p.copy()
+    o.add_efield_uniform(efield=[0, 0, 0])
+    T = o.calculate_trajectory()
+    assert same(T.t, rt) and same(T.p, rp)
+    assert not np.array_equal(plain, rp)  # the B-only equations differ in the last bits
