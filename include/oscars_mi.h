@@ -169,6 +169,12 @@ int om_trajectory_status(om_context* ctx, int i);
 /* Flux on n_obs observation points x n_e photon energies (point index
  * obs*n_e + e); n_e = 1 is the reference's flux map at one energy.
  * pol6: polarization vector (Re, Im) x 3, all zeros for 'all'.
+ * out_level: the last level summed per point (its converged level, or the
+ * level cap where out_conv is 0: the value is the sum at the cap, as in the
+ * reference); with quantity OM_Q_LEVEL out_value holds the converged level
+ * or -1, the reference's Result_Level (src/OSCARSSR_Cuda.cu:483-498).
+ * out_conv: 1 converged, 0 NotConverged (decided per point, as the
+ * reference's GPU kernel does).
  * out_e: complex E (6 doubles per point) after C0*dt and polarization; may be NULL. */
 int om_flux(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
             const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
@@ -260,6 +266,16 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
  * sums, replayed from a captured HIP graph; out[r] receives request r's grid
  * (n_obs x n_e, energy fastest). */
 int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out, om_stats* stats);
+/* One step of the batch launched stage by stage (no graph), with HIP events
+ * between the stages, for profiling: stage_ms[5] = ms of the trajectories
+ * (RK4 + spline; 0 without recompute), the level rows, the persistent kernel,
+ * the deep-level kernel and the FP64 recheck kernel; counts[7] = the device
+ * counters: units taken, unconverged points, evals, points flagged for the
+ * FP64 recheck, points handed to the deep-level kernel, amplitude
+ * evaluations and integrand evaluations inside the persistent kernel.  No
+ * counterpart in the reference. */
+int om_batch_profile(om_context* ctx, int recompute_trajectories, double* stage_ms, unsigned long long* counts,
+                     om_stats* stats);
 /* Times the persistent kernel on the first max(sample_items, 16 x CUs x the
  * largest candidate) work units for each candidate waves-per-CU value and
  * keeps the fastest (returned in *best). */

@@ -63,6 +63,7 @@ EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context
            "om_total_power", "om_efield_time", "om_field_points", "om_power_density_stl", "om_level_get",
            "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
+           "om_batch_profile",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
 
 _lib = None
@@ -123,6 +124,7 @@ def lib():
                                  C.POINTER(om_batch_opts)]
     L.om_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(om_stats)]
     L.om_batch_tune.argtypes = [C.c_void_p, _ip, C.c_int, C.c_long, _ip, _dp]
+    L.om_batch_profile.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_ulonglong), C.POINTER(om_stats)]
     L.om_adjoint_forward.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.c_int, C.POINTER(om_particle),
                                      _dp, C.c_long, _dp, C.c_long, C.c_int, _dp, C.POINTER(om_stats)]
     L.om_adjoint_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(om_stats)]
@@ -403,6 +405,18 @@ class Device:
         st = om_stats()
         self._chk(self.L.om_batch_run(self.h, int(bool(recompute_trajectories)), arr, C.byref(st)))
         return [o.reshape(n, m) for o, (n, m) in zip(outs, self._batch_shapes)], st.as_dict()
+
+    def batch_profile(self, recompute_trajectories=True):
+        """One batch step launched stage by stage with HIP events between the
+        stages: ms of (trajectories, level rows, persistent kernel, deep-level
+        kernel, FP64 recheck kernel) and the device counters."""
+        ms = np.zeros(5)
+        cnt = (C.c_ulonglong * 7)()
+        st = om_stats()
+        self._chk(self.L.om_batch_profile(self.h, int(bool(recompute_trajectories)), _ptr(ms), cnt, C.byref(st)))
+        names = ("units", "unconverged", "evals", "rechecked", "handed_over", "main_amplitudes", "main_evals")
+        stages = ("trajectories", "levels", "main", "deep", "fix")
+        return dict(zip(stages, ms.tolist())), {k: int(v) for k, v in zip(names, cnt)}, st.as_dict()
 
     def adjoint_forward(self, fields, grad_field, particle, obs_xyz, energies_eV, level):
         """Frozen-level flux (n_obs x n_e) of one particle; keeps what adjoint_backward needs."""

@@ -28,6 +28,7 @@ extern "C" {
 hipError_t om_launch_level_batch(const om::LevelJobBatch* J, hipStream_t s);
 hipError_t om_launch_batch_clear(double* out, long n, unsigned long long* counters, int nc, hipStream_t s);
 hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s);
+hipError_t om_launch_lw_batch_stages(const om::BatchArgs* A, int mixed, int n_groups, int stages, hipStream_t s);
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s, int nefields);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
@@ -1362,7 +1363,10 @@ int om_power_density_dev(om_context* ctx, int ip, const double* obs_xyz, long n_
 
 // batch counters: work-unit counter, unconverged points, evaluations, borderline points listed,
 // points handed to the one-wave-per-point kernel
-constexpr int kBatchCounters = 5;
+// counters: [0] next work unit, [1] unconverged, [2] evals, [3] flagged for the
+// FP64 recheck, [4] handed over, [5] amplitudes and [6] evals inside the
+// persistent kernel
+constexpr int kBatchCounters = 7;
 constexpr long kBatchFixCap = 1L << 27;
 constexpr long kBatchDeepCap = 1L << 22;
 constexpr int kBatchDeepLevel = 18;          // levels from here on: one wave per point
@@ -1495,6 +1499,8 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     A.evals = B.counters.p + 2;
     A.n_fix = B.counters.p + 3;
     A.n_deep = B.counters.p + 4;
+    A.main_amps = B.counters.p + 5;
+    A.main_evals = B.counters.p + 6;
     A.Ldeep = kBatchDeepLevel;
     A.deep_cap = total_points < kBatchDeepCap ? total_points : kBatchDeepCap;
     B.deep_code.ensure(A.deep_cap);
@@ -1574,6 +1580,56 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
                 "out");
     }
     if (stats) {
+      stats->evals = (double) cnt[2];
+      stats->kernel_ms = ms;
+      stats->total_ms = wall.ms();
+      stats->max_level = B.Lmax;
+      stats->n_unconverged = (long) cnt[1];
+      stats->n_rechecked = (long) cnt[3];
+    }
+  });
+}
+
+int om_batch_profile(om_context* ctx, int recompute_trajectories, double* stage_ms, unsigned long long* counts,
+                     om_stats* stats) {
+  return guarded(ctx, [&] {
+    Timer wall;
+    auto& B = ctx->batch;
+    if (!B.ready) throw std::logic_error("om_batch_setup has not been called for the current trajectories");
+    if (!stage_ms || !counts) throw std::invalid_argument("missing profile outputs");
+    hipEvent_t ev[6];
+    for (int i = 0; i < 6; ++i) check(hipEventCreate(&ev[i]), "event");
+    struct Guard {
+      hipEvent_t* e;
+      ~Guard() {
+        for (int i = 0; i < 6; ++i) (void) hipEventDestroy(e[i]);
+      }
+    } guard{ev};
+    // the sequence om_batch_run replays, launched one stage at a time
+    check(hipEventRecord(ev[0], ctx->stream), "event");
+    if (recompute_trajectories) launch_trajectories(ctx, B.np);
+    check(hipEventRecord(ev[1], ctx->stream), "event");
+    batch_levels(ctx);
+    check(om_launch_batch_clear(B.out.p, B.total_out, B.counters.p, kBatchCounters, ctx->stream), "clear");
+    check(hipEventRecord(ev[2], ctx->stream), "event");
+    check(om_launch_lw_batch_stages(&B.args, B.mixed, B.groups, 1, ctx->stream), "lw batch");
+    check(hipEventRecord(ev[3], ctx->stream), "event");
+    check(om_launch_lw_batch_stages(&B.args, B.mixed, B.groups, 2, ctx->stream), "lw batch deep");
+    check(hipEventRecord(ev[4], ctx->stream), "event");
+    if (B.mixed) check(om_launch_lw_batch_stages(&B.args, B.mixed, B.groups, 4, ctx->stream), "lw batch fix");
+    check(hipEventRecord(ev[5], ctx->stream), "event");
+    check(hipEventSynchronize(ev[5]), "sync");
+    for (int i = 0; i < 5; ++i) {
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "elapsed");
+      stage_ms[i] = ms;
+    }
+    unsigned long long cnt[kBatchCounters];
+    read_back(ctx, cnt, B.counters.p, sizeof(cnt));
+    for (int i = 0; i < kBatchCounters; ++i) counts[i] = cnt[i];
+    if (stats) {
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, ev[0], ev[5]), "elapsed");
       stats->evals = (double) cnt[2];
       stats->kernel_ms = ms;
       stats->total_ms = wall.ms();

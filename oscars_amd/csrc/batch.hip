@@ -115,6 +115,7 @@ __device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, int la
     w_flags[e] = 0;
   }
   __syncthreads();  // the state written above is read by other lanes
+  unsigned long long n_amp = 0, n_ev = 0;  // wave-uniform work counts of this unit
   for (int L = 0; L <= A.Lmax; ++L) {
     if (L == A.Ldeep) {
       // hand the energies still active to the one-wave-per-point kernel
@@ -156,6 +157,8 @@ __device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, int la
     const bool stored = L <= A.Lb;
     const double dt = T / (2.0 * (double) (1L << L) - 1);
     for (int g = 0; g < n_act; g += 64) {
+      n_amp += (unsigned long long) ns;
+      n_ev += (unsigned long long) ns * (unsigned long long) (n_act - g < 64 ? n_act - g : 64);
       const bool valid = g + lane < n_act;
       const int e = s_list[valid ? g + lane : g];  // spare lanes shadow the group's first energy
       const double negw = -R.omega[e0 + e];
@@ -277,6 +280,8 @@ __device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, int la
     atomicAdd(A.n_unconverged, lane == 0 ? (unsigned long long) __popcll(nc) : 0ULL);
     atomicAdd(A.evals, lane == 0 ? ev : 0ULL);
   }
+  atomicAdd(A.main_amps, lane == 0 ? n_amp : 0ULL);
+  atomicAdd(A.main_evals, lane == 0 ? n_ev : 0ULL);
   __syncthreads();  // the next unit overwrites the scratch
 }
 
@@ -431,18 +436,81 @@ __global__ void __launch_bounds__(64 * kPointWaves) lw_batch_deep_kernel(BatchAr
   }
 }
 
-// FP64 recomputation of the points the mixed kernels listed: one workgroup per
-// point from level 0 (a few points per unit: far cheaper than re-walking the
-// unit), so their levels and values are the FP64 path's.
-__global__ void __launch_bounds__(64 * kPointWaves) lw_batch_fix_kernel(BatchArgs A) {
-  __shared__ double sh[8][kPointWaves];
-  __shared__ unsigned long long sh_slot;
+// FP64 recomputation of the points the mixed kernels listed, from level 0, so
+// their levels and values are the FP64 path's.  One wave per point, its lanes
+// splitting each level's samples (a unit rarely holds more than a few such
+// points, so sharing the amplitude across them would leave most lanes idle):
+// no workgroup barrier, and a point's low levels occupy one wave, not eight.
+// Four waves per workgroup; the waves stride over the list independently.
+__global__ void __launch_bounds__(256) lw_batch_fix_kernel(BatchArgs A) {
+  const int lane = threadIdx.x & 63;
+  const long wave = (long) blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long n_waves = (long) gridDim.x * 4;
   long n = (long) *A.n_fix;
   if (n > A.fix_cap) n = A.fix_cap;
-  for (long k = blockIdx.x; k < n; k += gridDim.x) {
-    unsigned long long code = A.fix_list[k];
+  for (long k = wave; k < n; k += n_waves) {
+    const unsigned long long code = A.fix_list[k];
+    const long item = (long) (code / kBatchEB);
+    const long e_in = (long) (code - (unsigned long long) item * kBatchEB);
+    int lo = 0, hi = A.np;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (A.item_prefix[mid] <= item) lo = mid; else hi = mid;
+    }
+    const int ip = lo;
+    const long local = item - A.item_prefix[ip];
+    const ReqDev R = A.req[A.req_of[ip]];
+    const long io = local / R.n_blocks_e;
+    const long ie = (local - io * R.n_blocks_e) * kBatchEB + e_in;
+    const V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
+    const double ts = A.tspan[2 * ip], te = A.tspan[2 * ip + 1];
+    const double T = te - ts;
+    const double* prow = A.rows + 10 * (long) ip * A.stride;
+    const TrajDesc D = A.desc[ip];
+    const double negw = -R.omega[ie];
     double sum[6] = {0, 0, 0, 0, 0, 0};
-    batch_point<false>(A, code, 0, sum, -1.0, 0.0, 0.0, 0, 0, sh, &sh_slot);
+    double lastmag = -1;
+    int lev = 0;
+    bool c = false;
+    for (int L = 0; L <= A.Lmax && !c; ++L) {
+      const long ns = 1L << L;
+      double tot[6], maxd, an;
+      if (L <= A.Lb) {
+        slice_level_sum_rows<false>(prow + 10 * ((1L << L) - 1), 0, ns, obs, negw, lane, tot, maxd, an);
+      } else {
+        const double p2L = (double) (1L << L);
+        const double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
+        const double tfirst = ts + (te - ts) / (2.0 * p2L);
+        slice_level_sum<false>(
+            0, ns, negw, lane,
+            [&](long i) { return interpolated_amplitude(A, D, obs, tfirst + spacing * (double) i); }, tot, maxd,
+            an);
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) sum[q] += tot[q];
+      lev = L;
+      // convergence test, src/OSCARSSR.cc:4227-4239 (the batch kernel's)
+      double E[6];
+      const double dt = T / (2.0 * (double) (1L << L) - 1);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) E[q] = sum[q] * dt;
+      if (A.use_pol) pol_project(E, A.pol);
+      const double mag = cmag2(E);
+      const double prec = fabs(mag - lastmag) / lastmag;
+      c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == lastmag);
+      lastmag = mag;
+    }
+    // every lane holds the same totals (butterfly sums): lane 0 adds the point
+    const double f = A.C0 * (T / (2.0 * (double) (1L << lev) - 1));
+    double E[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) E[q] = sum[q] * f;
+    if (A.use_pol) pol_project(E, A.pol);
+    if (lane == 0) {
+      atomicAdd(&R.out[io * R.n_e + ie], A.C2 * cmag2(E) * A.weight[ip]);
+      if (!c) atomicAdd(A.n_unconverged, 1ULL);
+      atomicAdd(A.evals, (unsigned long long) ((2L << lev) - 1));
+    }
   }
 }
 
@@ -473,18 +541,27 @@ extern "C" hipError_t om_launch_batch_clear(double* out, long n, unsigned long l
   return hipGetLastError();
 }
 
-// The batch: the persistent kernel, the per-point kernel for the points it
-// handed over, and (mixed) the FP64 recomputation of borderline points.
-extern "C" hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s) {
+// The batch: the persistent kernel (stage bit 0), the per-point kernel for the
+// points it handed over (bit 1) and, mixed, the FP64 recomputation of
+// borderline points (bit 2); stages = 7 launches all three in order.
+extern "C" hipError_t om_launch_lw_batch_stages(const om::BatchArgs* A, int mixed, int n_groups, int stages,
+                                                hipStream_t s) {
   int pg = n_groups / om::kPointWaves > 0 ? n_groups / om::kPointWaves : 1;  // same wave count
   dim3 pb(64 * om::kPointWaves);
   if (mixed) {
-    hipLaunchKernelGGL(om::lw_batch_kernel<true>, dim3(n_groups), dim3(64), 0, s, *A);
-    hipLaunchKernelGGL(om::lw_batch_deep_kernel<true>, dim3(pg), pb, 0, s, *A);
-    hipLaunchKernelGGL(om::lw_batch_fix_kernel, dim3(pg), pb, 0, s, *A);
+    if (stages & 1) hipLaunchKernelGGL(om::lw_batch_kernel<true>, dim3(n_groups), dim3(64), 0, s, *A);
+    if (stages & 2) hipLaunchKernelGGL(om::lw_batch_deep_kernel<true>, dim3(pg), pb, 0, s, *A);
+    if (stages & 4) {
+      int fb = n_groups / 4 > 0 ? n_groups / 4 : 1;  // four single-wave points per workgroup
+      hipLaunchKernelGGL(om::lw_batch_fix_kernel, dim3(fb), dim3(256), 0, s, *A);
+    }
   } else {
-    hipLaunchKernelGGL(om::lw_batch_kernel<false>, dim3(n_groups), dim3(64), 0, s, *A);
-    hipLaunchKernelGGL(om::lw_batch_deep_kernel<false>, dim3(pg), pb, 0, s, *A);
+    if (stages & 1) hipLaunchKernelGGL(om::lw_batch_kernel<false>, dim3(n_groups), dim3(64), 0, s, *A);
+    if (stages & 2) hipLaunchKernelGGL(om::lw_batch_deep_kernel<false>, dim3(pg), pb, 0, s, *A);
   }
   return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s) {
+  return om_launch_lw_batch_stages(A, mixed, n_groups, 7, s);
 }

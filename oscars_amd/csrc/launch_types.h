@@ -178,6 +178,11 @@ struct BatchArgs {
   int* deep_flags;
   long deep_cap;
   unsigned long long* n_deep;  // points handed over (those beyond deep_cap stay with their unit)
+  // work done inside lw_batch_kernel itself (for its roofline): amplitude
+  // evaluations, one per (unit, energy group, sample), and integrand
+  // evaluations, one per (active energy, sample)
+  unsigned long long* main_amps;
+  unsigned long long* main_evals;
 };
 
 }  // namespace om

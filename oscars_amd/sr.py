@@ -88,7 +88,6 @@ class sr:
         self._fields = []        # list of dicts kept for om_field construction
         self._efields = []       # electric fields (om_field)
         self._stale_eb = None    # E+B equations kept after a removal (None: follow the fields)
-        self._grids = []         # keep numpy arrays alive
         self._beam = None        # the first beam's description
         self._beams = []         # (name, description, weight) per beam
         self._wsum = []          # running sums of the weights
@@ -282,7 +281,9 @@ class sr:
     def _remove(self, fields, name):
         if self._stale_eb is None:
             self._stale_eb = len(self._efields) > 0 or len(self._fields) == 0
-        return [f for f in fields if name == "" or getattr(f, "name", "") != name]
+        # src/TFieldContainer.cc:52-70: every field whose name equals `name`
+        # goes, so "" removes the unnamed ones
+        return [f for f in fields if getattr(f, "name", "") != name]
 
     def remove_bfield(self, name):
         """Remove the magnetic fields of this name (src/TFieldContainer.cc:52-70);
@@ -311,7 +312,7 @@ class sr:
         f.grid_data = s.ctypes.data_as(_lib._dp)
         f.rotations[:] = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
         f.translation[:] = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
-        self._grids.append(s)
+        f._keep = s  # grid_data points into s: the field owns its samples
         self._push_b(f)
         self._traj_valid = False
 
@@ -327,7 +328,7 @@ class sr:
         f.grid_data = s.ctypes.data_as(_lib._dp)
         f.rotations[:] = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
         f.translation[:] = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
-        self._grids.append(s)
+        f._keep = s  # grid_data points into s: the field owns its samples
         self._push_b(f)
         self._traj_valid = False
 
@@ -406,15 +407,14 @@ class sr:
     def clear_bfields(self):
         # src/OSCARSSR.cc:186-196: the current trajectory stays until a field is added
         self._fields = []
-        self._grids = []
         self._stale_eb = None
 
     # ------------------------------------------------------------ beam / trajectory window
-    def set_particle_beam(self, **kw):
+    def set_particle_beam(self, *args, **kw):
         """Clear the beams, add this one and make its ideal particle current
-        (src/OSCARSSR_Python.cc:3561-3580); keywords as add_particle_beam."""
+        (src/OSCARSSR_Python.cc:3561-3580); arguments as add_particle_beam."""
         self.clear_particle_beams()
-        self.add_particle_beam(**kw)
+        self.add_particle_beam(*args, **kw)
 
     def clear_particle_beams(self):
         """Remove every beam; the current particle stays (src/OSCARSSR.cc:540-547)."""
@@ -754,12 +754,14 @@ class sr:
     def set_twiss_parameters(self, beam="", beta=None, alpha=None, gamma=None, lattice_reference=None):
         """Twiss parameters of a beam, any two of beta, alpha, gamma or all three,
         or beta alone with alpha 0 (src/OSCARSSR_Python.cc:4269-4360)."""
-        b = self._beam_by_name(beam)
         two = lambda v, n: None if v is None else _v2(v, n)  # noqa: E731
         tw = dict(beta=two(beta, "beta"), alpha=two(alpha, "alpha"), gamma=two(gamma, "gamma"))
         ref = None if lattice_reference is None else _v3(lattice_reference, "lattice_reference")
         if B.twiss_from(tw["beta"], tw["alpha"], tw["gamma"]) is None:
             return
+        # the beam is looked up inside a matching branch only, as the
+        # reference does, so a call without effect draws nothing from the stream
+        b = self._beam_by_name(beam)
         b.update(tw, lattice_reference=ref)
         self._traj_valid = False
 

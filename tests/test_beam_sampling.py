@@ -301,3 +301,34 @@ def test_set_twiss_parameters_is_the_set_particle_beam_form():
     o.set_seed(3)
     got = np.array([p.x0 + p.b0 for p in o._particles_for(200)])
     assert np.array_equal(got, _sr_particles(BEAM_CASES[1], 3, 200))
+
+
+def test_set_particle_beam_positional_arguments():
+    """set_particle_beam takes add_particle_beam's arguments by position too
+    (the binding's 'type, name, energy_GeV, d0, x0, ...' order,
+    src/OSCARSSR_Python.cc:3743-3766)."""
+    from oscars_amd import sr as SR
+    a, b = SR.sr(), SR.sr()
+    a.set_particle_beam("electron", "b", 3.0, [0, 0, 1], [0, 0, -1], current=0.5)
+    b.set_particle_beam(type="electron", name="b", energy_GeV=3.0, d0=[0, 0, 1], x0=[0, 0, -1], current=0.5)
+    assert a._beams[0][0] == b._beams[0][0] == "b"
+    assert a._beams[0][1] == b._beams[0][1]
+
+
+def test_set_twiss_without_effect_draws_nothing():
+    """With several beams, a set_twiss_parameters call that names no Twiss
+    combination draws no uniform for the beam choice: the reference looks the
+    beam up inside a matching branch only."""
+    from oscars_amd import beam as Bm
+    from oscars_amd import sr as SR
+    s = SR.sr()
+    s.add_particle_beam(energy_GeV=3.0, x0=[0, 0, -1], name="a")
+    s.add_particle_beam(energy_GeV=3.0, x0=[0, 0, -1], name="b", weight=2)
+    s.set_seed(7)
+    s.set_twiss_parameters(alpha=[0.1, 0.2])  # alpha alone: no combination
+    x = Bm.stream().random()
+    s.set_seed(7)
+    assert Bm.stream().random() == x
+    s.set_seed(7)
+    s.set_twiss_parameters(beta=[1.0, 2.0])  # a combination: the beam is drawn
+    assert Bm.stream().random() != x

@@ -4,7 +4,9 @@ add_to_flux / add_to_power_density.
 
 The oracle's field sums are pinned to the reference's GetB / GetE here and in
 tests/test_field_files.py; the device sums are then held to the oracle."""
+import ctypes
 import os
+import sys
 import tempfile
 
 import numpy as np
@@ -179,16 +181,22 @@ def test_get_bfield_leaves_trajectory_fields_alone():
 
 def test_remove_fields_by_name():
     """remove_bfield / remove_efield drop every field of that name and keep
-    the rest; "" removes nothing; a removal that leaves magnetic fields only
+    the rest; "" removes the unnamed fields (the reference's own removal of an
+    unnamed field with "" is test_removed_efields_keep_the_eb_equations in
+    tests/test_oracle_vs_ref.py); a removal that leaves magnetic fields only
     keeps the E+B equations of motion (a zero electric field here)."""
     from oscars_amd import sr as SR
     s = SR.sr()
     s.add_bfield_undulator(**UNDULATOR, name="u")
     s.add_bfield_gaussian(bfield=[0, 0.8, 0.05], sigma=[0, 0, 0.1], name="g")
     s.add_bfield_uniform(bfield=[0.1, 0, 0], width=[0, 0, 0.2], name="g")
+    s.add_bfield_uniform(bfield=[0.2, 0, 0], width=[0, 0, 0.2])
     s.add_efield_uniform(efield=[1e5, 0, 0], width=[0, 0, 1], name="e")
+    s.add_efield_uniform(efield=[2e5, 0, 0], width=[0, 0, 1])
     s.remove_bfield("")
-    assert len(s._fields) == 3
+    assert [f.name for f in s._fields] == ["u", "g", "g"]
+    s.remove_efield("")
+    assert [f.name for f in s._efields] == ["e"]
     s.remove_bfield("g")
     assert [f.name for f in s._fields] == ["u"]
     assert len(s._efields_for_derivs()) == 1
@@ -202,6 +210,26 @@ def test_remove_fields_by_name():
     t.add_bfield_undulator(**UNDULATOR, name="u")
     t.remove_bfield("u")
     assert t._efields_for_derivs() == []
+
+
+def test_grid_samples_outlive_clear_bfields(tmp_path):
+    """An electric-field map keeps its samples when the magnetic fields are
+    cleared: the field itself owns the array its grid pointer reads."""
+    import gc
+    from oscars_amd import sr as SR
+    z = -1 + 0.02 * np.arange(101)
+    path = tmp_path / "e.dat"
+    write_oscars_grid(str(path), [0, 0, -1], [0, 0, 0.02], [1, 1, 101], lambda x, y, z: [1e5 * z, 2e5, -3e5 * z * z])
+    s = SR.sr()
+    s.add_efield_file(str(path))
+    s.add_bfield_undulator(**UNDULATOR)
+    f = s._efields[-1]
+    addr = ctypes.cast(f.grid_data, ctypes.c_void_p).value
+    s.clear_bfields()
+    gc.collect()
+    assert f._keep.ctypes.data == addr and sys.getrefcount(f._keep) >= 2
+    got = np.ctypeslib.as_array(f.grid_data, shape=(101 * 3,)).reshape(101, 3)
+    assert np.array_equal(got[:, 0], 1e5 * z) and np.all(got[:, 1] == 2e5)
 
 
 @pytest.mark.gpu
