@@ -58,7 +58,10 @@ enum { OM_Q_VALUE = 0, OM_Q_PRECISION = 1, OM_Q_LEVEL = 2 };
  * no fall-off along that axis), rotations.  Uniform: field, centre
  * (translation), full width per axis (shape; <= 0: unbounded), rotations.
  * Quadrupole: gradient K [T/m], width along the magnet axis, rotations,
- * translation.  Time-dependent fields are not supported. */
+ * translation.  Any model: frequency [Hz], frequency_phase and time_offset
+ * [s] multiply it by cos(2 pi frequency (t + time_offset) + frequency_phase)
+ * (src/TField3D_IdealUndulator.cc:147-151 and the same in every model; a grid
+ * carries the factor also at zero frequency, src/TField3D_Grid.cc:277). */
 typedef struct {
   int type;
   int nperiods;
@@ -80,6 +83,9 @@ typedef struct {
    * 1 = start and grid_stop taken as given (a resampled OSCARS1D table) */
   int grid_exact;
   double grid_stop[3];
+  double frequency;
+  double frequency_phase;
+  double time_offset;
 } om_field;
 
 /* One particle: initial position [m], beta, T0 [m] (c*t, as the reference),
@@ -162,8 +168,11 @@ int om_efield_time(om_context* ctx, int i, const double* obs, long* n, double* o
  * src/TFieldContainer.cc:85-107).  The fields are those of this call only; the
  * context's trajectories and their fields are left as they were.  No fields
  * gives zeros. */
-int om_field_points(om_context* ctx, const om_field* fields, int nfields, const double* xyz, long n, double* out3);
-/* Status of trajectory i: 0 ok, 1 |beta| >= 1 met during integration. */
+int om_field_points(om_context* ctx, const om_field* fields, int nfields, const double* xyz, long n, double t,
+                    double* out3);
+/* Status of trajectory i: 0 ok (steps that met |beta| >= 1 were halved as the
+ * reference halves them, src/OSCARSSR.cc:1362-1378), 1 a step still met
+ * |beta| >= 1 after 40 halvings (the reference recurses without limit). */
 int om_trajectory_status(om_context* ctx, int i);
 
 /* Flux on n_obs observation points x n_e photon energies (point index

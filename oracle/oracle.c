@@ -265,8 +265,17 @@ static int und_region(const orc_field* f, double D, double PhaseShift, double* a
   return 1;
 }
 
+/* cos(2 pi f (T + t0) + phase), src/TField3D_IdealUndulator.cc:151 */
+static double td_factor(const orc_field* f, double T) {
+  return cos(k_twopi() * f->freq * (T + f->toff) + f->fphase);
+}
+
+static void td_apply(const orc_field* f, double T, double* out) {
+  double c = td_factor(f, T);
+  out[0] = out[0] * c; out[1] = out[1] * c; out[2] = out[2] * c;
+}
+
 static void field_eval(const orc_field* f, const double* X, double T, double* out) {
-  (void) T;
   out[0] = out[1] = out[2] = 0;
   if (f->type == ORC_FIELD_UNDULATOR || f->type == ORC_FIELD_EPU) {
     double d[3] = {X[0] - f->center[0], X[1] - f->center[1], X[2] - f->center[2]};
@@ -287,6 +296,8 @@ static void field_eval(const orc_field* f, const double* X, double T, double* ou
       }
       double scale = 1;
       int r = und_region(f, D, PhaseShift, &scale);
+      /* outside the undulator the reference returns zero before the time factor */
+      if (r == 0 && f->type == ORC_FIELD_UNDULATOR) return;
       if (r == 0) continue;
       double s = sin(k_twopi() * (D - PhaseShift) / f->period_length);
       double v[3];
@@ -300,6 +311,7 @@ static void field_eval(const orc_field* f, const double* X, double T, double* ou
         out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
       }
     }
+    if (f->freq != 0) td_apply(f, T, out);
     return;
   }
   double x[3] = {X[0], X[1], X[2]};
@@ -313,12 +325,14 @@ static void field_eval(const orc_field* f, const double* X, double T, double* ou
           (!(f->shape[2] <= 0) && fabs(RX[2]) > fabs(f->shape[2] / 2.)))
         return;
       out[0] = f->field[0]; out[1] = f->field[1]; out[2] = f->field[2];
+      if (f->freq != 0) td_apply(f, T, out);
       return;
     }
     double Fraction = 1;
     for (int k = 0; k < 3; ++k)
       if (f->shape[k] > 0) Fraction *= exp(-pow((RX[k]) / f->shape[k], 2) / 2.);
     out[0] = Fraction * f->field[0]; out[1] = Fraction * f->field[1]; out[2] = Fraction * f->field[2];
+    if (f->freq != 0) td_apply(f, T, out);
     return;
   }
   if (f->type == ORC_FIELD_QUADRUPOLE) {
@@ -328,6 +342,7 @@ static void field_eval(const orc_field* f, const double* X, double T, double* ou
     double Ret[3] = {f->K * x[1], f->K * x[0], 0};
     v_rot_xyz(Ret, f->rot);
     out[0] = Ret[0]; out[1] = Ret[1]; out[2] = Ret[2];
+    if (f->freq != 0) td_apply(f, T, out);
     return;
   }
   /* grid, src/TField3D_Grid.cc:153-278 (1-D branches) */
@@ -369,7 +384,8 @@ static void field_eval(const orc_field* f, const double* X, double T, double* ou
     }
 #undef GI
 #undef LERP
-    for (int k = 0; k < 3; ++k) out[k] = F3[k] * cos(k_twopi() * 0 * (T + 0) + 0);
+    double c = td_factor(f, T);
+    for (int k = 0; k < 3; ++k) out[k] = F3[k] * c;
     return;
   }
   double coord, start, step;
@@ -382,7 +398,7 @@ static void field_eval(const orc_field* f, const double* X, double T, double* ou
   const double* b = f->data + 3 * (n0 + 1);
   for (int k = 0; k < 3; ++k) {
     double v = a[k] + (b[k] - a[k]) * dd / step;
-    out[k] = v * cos(k_twopi() * 0 * (T + 0) + 0);
+    out[k] = v * td_factor(f, T);
   }
 }
 
@@ -476,6 +492,12 @@ static void derivs_b(rk_ctx* c, double t, const double* x, double* dxdt) {
   dxdt[5] = q * (x[1] * B[1] - x[3] * B[0]);
 }
 
+/* steps split by the last orc_trajectory_rk4_eb call (test instrumentation) */
+static long g_halvings = 0;
+static int g_halving_depth = 0;
+long orc_halving_count(void) { return g_halvings; }
+int orc_halving_depth(void) { return g_halving_depth; }
+
 /* src/OSCARSSR.cc:1321-1386 (NR rk4 with gamma-error step halving) */
 static void rk4_step(rk_ctx* c, double* y, double* dydx, double x, double h, double* yout, int depth) {
   double dym[6], dyt[6], yt[6], yo[6];
@@ -495,6 +517,8 @@ static void rk4_step(rk_ctx* c, double* y, double* dydx, double x, double h, dou
   for (i = 0; i < 6; ++i) yo[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
   int this_err = 1. - (yo[1] * yo[1] + yo[3] * yo[3] + yo[5] * yo[5]) / (K_C * K_C) <= 0 ? 1 : 0;
   if (c->err_gamma || this_err) {
+    ++g_halvings;
+    if (depth + 1 > g_halving_depth) g_halving_depth = depth + 1;
     c->err_gamma = 0;
     double yh[6];
     derivs_b(c, x, y, dydx);
@@ -532,6 +556,8 @@ int orc_trajectory_rk4(const orc_fields* F, const orc_particle* P, double ctstar
 int orc_trajectory_rk4_eb(const orc_fields* F, const orc_fields* FE, const orc_particle* P, double ctstart,
                           double ctstop, long npoints, double* out_t, double* out_p) {
   rk_ctx c = {F, P, 0, FE};
+  g_halvings = 0;
+  g_halving_depth = 0;
   double DeltaT = ((ctstop - ctstart) / K_C / (double) (size_t) (npoints - 1));
   size_t NF = (size_t) (1 + (ctstop - P->t0) / K_C / DeltaT);
   size_t NB = (size_t) ((P->t0 - ctstart) / K_C / DeltaT);

@@ -39,6 +39,10 @@ typedef struct {
   /* Gaussian (sigma) / uniform box (width) per axis, quadrupole gradient and width */
   double shape[3];
   double K, width;
+  /* time dependence: the field is multiplied by
+   * cos(2 pi freq (T + toff) + fphase) (src/TField3D_IdealUndulator.cc:147-151
+   * and the same in every model; the grid always, src/TField3D_Grid.cc:277) */
+  double freq, fphase, toff;
 } orc_field;
 
 typedef struct {
@@ -79,6 +83,10 @@ int orc_add_grid_exact(orc_fields* F, const double* start, const double* stop, c
 int orc_add_grid(orc_fields* F, const double* start_in, const double* step_in, const long* npts,
                  const double* samples, const double* rotations, const double* translation);
 void orc_get_b(const orc_fields* F, const double* x, double t, double* out);
+/* RK4 steps (and sub-steps) the last trajectory integration split in two */
+long orc_halving_count(void);
+/* deepest split of that integration (1: a step halved once) */
+int orc_halving_depth(void);
 
 /* ---- particle ---- */
 void orc_particle_electron(orc_particle* P, double energy_GeV, const double* x0, const double* d0, double current);

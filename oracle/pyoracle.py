@@ -17,6 +17,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_ORACLE = os.path.join(HERE, "liboracle.so")
 LIB_REF = os.path.join(HERE, "_ref", "libref.so")
+# the reference core with its GPU boundary served by liboscars_mi.so (oracle/Makefile, ref_mi)
+LIB_REF_MI = os.path.join(HERE, "_ref", "libref_mi.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -52,6 +54,7 @@ class orc_field(C.Structure):
         ("rot", C.c_double * 3), ("trans", C.c_double * 3),
         ("data", _dp),
         ("shape", C.c_double * 3), ("K", C.c_double), ("width", C.c_double),
+        ("freq", C.c_double), ("fphase", C.c_double), ("toff", C.c_double),
     ]
 
 
@@ -203,41 +206,59 @@ class Oracle:
         self.nthreads = nthreads
         self.traj = None
         self._grids = []
+        self._td = (0.0, 0.0, 0.0)
+
+    def set_time_dependence(self, frequency=0.0, frequency_phase=0.0, time_offset=0.0):
+        """frequency, frequency_phase and time_offset of the fields added after
+        this call (as RefSim.set_time_dependence)."""
+        self._td = (float(frequency), float(frequency_phase), float(time_offset))
+
+    def _added(self, F):
+        f = F.f[F.n - 1]
+        f.freq, f.fphase, f.toff = self._td
 
     # -- setup (src/OSCARSSR_Python.cc kwarg semantics) --
     def add_bfield_undulator(self, bfield, period, nperiods, phase=0.0, translation=(0, 0, 0), taper=0.0):
         self.L.orc_add_undulator(C.byref(self.F), _v3(bfield), _v3(period), int(nperiods), _v3(translation),
                                  float(phase), float(taper))
+        self._added(self.F)
         self.traj = None
 
     def add_bfield_epu(self, bfieldA, bfieldB, period, nperiods, phase=0.0, translation=(0, 0, 0), taper=0.0):
         self.L.orc_add_epu(C.byref(self.F), _v3(bfieldA), _v3(bfieldB), _v3(period), int(nperiods),
                            _v3(translation), float(phase), float(taper))
+        self._added(self.F)
         self.traj = None
 
     def add_bfield_gaussian(self, bfield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
         self.L.orc_add_gaussian(C.byref(self.F), _v3(bfield), _v3(translation), _v3(sigma), _v3(rotations))
+        self._added(self.F)
         self.traj = None
 
     def add_efield_gaussian(self, efield, sigma, rotations=(0, 0, 0), translation=(0, 0, 0)):
         self.L.orc_add_gaussian(C.byref(self.FE), _v3(efield), _v3(translation), _v3(sigma), _v3(rotations))
+        self._added(self.FE)
         self.traj = None
 
     def add_efield_uniform(self, efield, width=(0, 0, 0), rotations=(0, 0, 0), translation=(0, 0, 0)):
         self.L.orc_add_uniform(C.byref(self.FE), _v3(efield), _v3(width), _v3(translation), _v3(rotations))
+        self._added(self.FE)
         self.traj = None
 
     def add_efield_undulator(self, efield, period, nperiods, translation=(0, 0, 0)):
         self.L.orc_add_undulator(C.byref(self.FE), _v3(efield), _v3(period), int(nperiods), _v3(translation), 0.0,
                                  0.0)
+        self._added(self.FE)
         self.traj = None
 
     def add_bfield_uniform(self, bfield, width=(0, 0, 0), rotations=(0, 0, 0), translation=(0, 0, 0)):
         self.L.orc_add_uniform(C.byref(self.F), _v3(bfield), _v3(width), _v3(translation), _v3(rotations))
+        self._added(self.F)
         self.traj = None
 
     def add_bfield_quadrupole(self, K, width, rotations=(0, 0, 0), translation=(0, 0, 0)):
         self.L.orc_add_quadrupole(C.byref(self.F), float(K), float(width), _v3(rotations), _v3(translation))
+        self._added(self.F)
         self.traj = None
 
     def add_bfield_grid(self, start, step, npts, samples, rotations=(0, 0, 0), translation=(0, 0, 0)):
@@ -247,6 +268,7 @@ class Oracle:
                                  s.ctypes.data_as(_dp), _v3(rotations), _v3(translation))
         if rc:
             raise ValueError("unsupported grid (rc=%d)" % rc)
+        self._added(self.F)
         self.traj = None
 
     def add_bfield_file(self, ifile, iformat="OSCARS", rotations=(0, 0, 0), translation=(0, 0, 0)):
@@ -261,16 +283,17 @@ class Oracle:
                                        s.ctypes.data_as(_dp), _v3(rotations), _v3(translation))
         if rc:
             raise ValueError("unsupported grid (rc=%d)" % rc)
+        self._added(self.F)
         self.traj = None
 
-    def get_bfield(self, x):
+    def get_bfield(self, x, t=0.0):
         out = np.zeros(3)
-        self.L.orc_get_b(C.byref(self.F), _v3(x), 0.0, out.ctypes.data_as(_dp))
+        self.L.orc_get_b(C.byref(self.F), _v3(x), float(t), out.ctypes.data_as(_dp))
         return out
 
-    def get_efield(self, x):
+    def get_efield(self, x, t=0.0):
         out = np.zeros(3)
-        self.L.orc_get_b(C.byref(self.FE), _v3(x), 0.0, out.ctypes.data_as(_dp))
+        self.L.orc_get_b(C.byref(self.FE), _v3(x), float(t), out.ctypes.data_as(_dp))
         return out
 
     def set_particle_beam(self, energy_GeV, x0=(0, 0, 0), d0=(0, 0, 1), current=0.0, type="electron"):
@@ -308,6 +331,11 @@ class Oracle:
                                      p.ctypes.data_as(_dp))
         self.traj = Trajectory(t, p)
         return self.traj
+
+    def halvings(self):
+        """(steps the last trajectory integration split in two, the deepest split)."""
+        self.L.orc_halving_count.restype = C.c_long
+        return int(self.L.orc_halving_count()), int(self.L.orc_halving_depth())
 
     def _traj(self):
         if self.traj is None:
@@ -507,10 +535,31 @@ def ref_available():
     return os.path.exists(LIB_REF)
 
 
-def ref_lib():
-    global _ref
+_ref_mi = None
+
+
+def ref_mi_available():
+    return os.path.exists(LIB_REF_MI)
+
+
+def ref_lib(with_gpu=False):
+    """The reference core (libref.so), or with_gpu the same core compiled with
+    its GPU boundary and linked against liboscars_mi.so (libref_mi.so; load
+    torch first, as oscars_amd._lib does)."""
+    global _ref, _ref_mi
+    if with_gpu:
+        if _ref_mi is None:
+            _ref_mi = _bind_ref(C.CDLL(LIB_REF_MI))
+            _ref_mi.ref_set_gpu.argtypes = [C.c_int]
+            _ref_mi.ref_set_gpu.restype = None
+        return _ref_mi
     if _ref is None:
-        L = C.CDLL(LIB_REF)
+        _ref = _bind_ref(C.CDLL(LIB_REF))
+    return _ref
+
+
+def _bind_ref(L):
+    if True:
         L.ref_last_error.restype = C.c_char_p
         L.ref_new.restype = C.c_void_p
         L.ref_delete.argtypes = [C.c_void_p]
@@ -529,8 +578,10 @@ def ref_lib():
         L.ref_get_particle.argtypes = [C.c_void_p, _dp]
         L.ref_set_ctstartstop.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.ref_set_npoints_trajectory.argtypes = [C.c_void_p, C.c_long]
-        L.ref_get_bfield.argtypes = [C.c_void_p, _dp, _dp]
-        L.ref_get_efield.argtypes = [C.c_void_p, _dp, _dp]
+        L.ref_get_bfield.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.ref_get_efield.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.ref_set_time_dependence.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.ref_set_time_dependence.restype = None
         L.ref_remove_efield.argtypes = [C.c_void_p, C.c_char_p]
         L.ref_remove_efield.restype = None
         L.ref_set_beam_append.argtypes = [C.c_int, C.c_double]
@@ -580,8 +631,7 @@ def ref_lib():
                                           C.POINTER(C.c_long)]
         L.ref_average_points.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_char_p, C.c_char_p,
                                          C.POINTER(C.c_long), _dp, _dp, C.c_long]
-        _ref = L
-    return _ref
+    return L
 
 
 def _ref_chk(L, rc):
@@ -633,10 +683,15 @@ _NOANGLE = 112188979912321.0
 class RefSim:
     """Scenario-level driver of the reference's compiled C++ core."""
 
-    def __init__(self, nthreads=1):
-        self.L = ref_lib()
+    def __init__(self, nthreads=1, gpu=False):
+        """gpu: the reference core whose GPU boundary is this repository's
+        liboscars_mi.so (libref_mi.so); its calculations then pass gpu=1."""
+        self.L = ref_lib(with_gpu=gpu)
         self.h = self.L.ref_new()
         self.nthreads = nthreads
+        self.L.ref_set_time_dependence(0.0, 0.0, 0.0)
+        if gpu:
+            self.L.ref_set_gpu(1)
 
     def __del__(self):
         try:
@@ -694,15 +749,20 @@ class RefSim:
     def add_bfield_file(self, ifile, iformat="OSCARS", rotations=(0, 0, 0), translation=(0, 0, 0)):
         self._chk(self.L.ref_add_grid_file(self.h, ifile.encode(), _v3(rotations), _v3(translation)))
 
-    def get_bfield(self, x):
+    def get_bfield(self, x, t=0.0):
         out = np.zeros(3)
-        self.L.ref_get_bfield(self.h, _v3(x), out.ctypes.data_as(_dp))
+        self.L.ref_get_bfield(self.h, _v3(x), float(t), out.ctypes.data_as(_dp))
         return out
 
-    def get_efield(self, x):
+    def get_efield(self, x, t=0.0):
         out = np.zeros(3)
-        self.L.ref_get_efield(self.h, _v3(x), out.ctypes.data_as(_dp))
+        self.L.ref_get_efield(self.h, _v3(x), float(t), out.ctypes.data_as(_dp))
         return out
+
+    def set_time_dependence(self, frequency=0.0, frequency_phase=0.0, time_offset=0.0):
+        """frequency, frequency_phase and time_offset of the fields added after
+        this call (the binding's keyword arguments of every add_*field)."""
+        self.L.ref_set_time_dependence(float(frequency), float(frequency_phase), float(time_offset))
 
     def remove_efield(self, name=""):
         self.L.ref_remove_efield(self.h, name.encode())

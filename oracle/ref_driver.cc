@@ -53,6 +53,22 @@ extern "C" {
 
 const char* ref_last_error() { return g_err.c_str(); }
 
+// GPU argument of the calculations below (src/OSCARSSR.cc:1713-1729): 1 sends
+// them through OSCARSSR::Calculate*GPU, i.e. through the GPU boundary the
+// library was linked with (oscars_amd/reference_shim/OSCARSSR_MI.cc in
+// _ref/libref_mi.so; the plain build has none and throws).
+static int g_gpu = 0;
+void ref_set_gpu(int gpu) { g_gpu = gpu; }
+
+// Time dependence (frequency [Hz], frequency_phase, time_offset [s]) given to
+// the field models added after this call (the binding's keyword arguments).
+static double g_freq = 0, g_fphase = 0, g_toff = 0;
+void ref_set_time_dependence(double frequency, double frequency_phase, double time_offset) {
+  g_freq = frequency;
+  g_fphase = frequency_phase;
+  g_toff = time_offset;
+}
+
 void* ref_new() { return new OSCARSSR(); }
 void ref_delete(void* h) { delete static_cast<OSCARSSR*>(h); }
 
@@ -61,7 +77,7 @@ int ref_add_undulator(void* h, double const* bfield, double const* period, int n
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);
     s->AddMagneticField((TField*) new TField3D_IdealUndulator(v3(bfield), v3(period), nperiods, v3(translation),
-                                                              phase, taper, 0, 0, 0, ""));
+                                                              phase, taper, g_freq, g_fphase, g_toff, ""));
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -71,7 +87,7 @@ int ref_add_epu(void* h, double const* bfieldA, double const* bfieldB, double co
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);
     s->AddMagneticField((TField*) new TField3D_IdealEPU(v3(bfieldA), v3(bfieldB), v3(period), nperiods,
-                                                        v3(translation), phase, taper, 0, 0, 0, ""));
+                                                        v3(translation), phase, taper, g_freq, g_fphase, g_toff, ""));
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -79,7 +95,8 @@ int ref_add_epu(void* h, double const* bfieldA, double const* bfieldB, double co
 int ref_add_grid_file(void* h, const char* path, double const* rotations, double const* translation) {
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);
-    s->AddMagneticField(path, "OSCARS", v3(rotations), v3(translation), std::vector<double>(), 0, 0, 0, "");
+    s->AddMagneticField(path, "OSCARS", v3(rotations), v3(translation), std::vector<double>(), g_freq, g_fphase,
+                        g_toff, "");
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -90,7 +107,7 @@ int ref_add_field_file(void* h, const char* path, const char* format, double con
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);
     std::vector<double> sc(scaling, scaling + nscale);
-    s->AddMagneticField(path, format, v3(rotations), v3(translation), sc, 0, 0, 0, "");
+    s->AddMagneticField(path, format, v3(rotations), v3(translation), sc, g_freq, g_fphase, g_toff, "");
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -101,14 +118,14 @@ int ref_add_efield(void* h, int type, double const* field, double const* shape, 
   try {
     OSCARSSR* s = static_cast<OSCARSSR*>(h);
     if (type == 3)
-      s->AddElectricField((TField*) new TField3D_Gaussian(v3(field), v3(translation), v3(shape), v3(rotations), 0, 0,
-                                                         0, ""));
+      s->AddElectricField((TField*) new TField3D_Gaussian(v3(field), v3(translation), v3(shape), v3(rotations), g_freq,
+                                                         g_fphase, g_toff, ""));
     else if (type == 4)
-      s->AddElectricField((TField*) new TField3D_UniformBox(v3(field), v3(shape), v3(translation), v3(rotations), 0,
-                                                           0, 0, ""));
+      s->AddElectricField((TField*) new TField3D_UniformBox(v3(field), v3(shape), v3(translation), v3(rotations),
+                                                           g_freq, g_fphase, g_toff, ""));
     else
       s->AddElectricField((TField*) new TField3D_IdealUndulator(v3(field), v3(period), nperiods, v3(translation), 0, 0,
-                                                               0, 0, 0, ""));
+                                                               g_freq, g_fphase, g_toff, ""));
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -122,7 +139,7 @@ int ref_add_field_interpolated(void* h, int n, double const* params, const char*
     for (int i = 0; i < n; ++i) M.push_back(std::make_pair(params[i], std::string(files[i])));
     std::vector<double> sc(scaling, scaling + nscale);
     static_cast<OSCARSSR*>(h)->AddMagneticFieldInterpolated(M, format, parameter, v3(rotations), v3(translation), sc,
-                                                            0, 0, 0, "", "");
+                                                            g_freq, g_fphase, g_toff, "", "");
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -134,7 +151,8 @@ int ref_add_gaussian(void* h, double const* bfield, double const* translation, d
                      double const* rotations) {
   try {
     static_cast<OSCARSSR*>(h)->AddMagneticField(
-        (TField*) new TField3D_Gaussian(v3(bfield), v3(translation), v3(sigma), v3(rotations), 0, 0, 0, ""));
+        (TField*) new TField3D_Gaussian(v3(bfield), v3(translation), v3(sigma), v3(rotations), g_freq, g_fphase,
+                                        g_toff, ""));
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -143,7 +161,8 @@ int ref_add_uniform(void* h, double const* bfield, double const* width, double c
                     double const* rotations) {
   try {
     static_cast<OSCARSSR*>(h)->AddMagneticField(
-        (TField*) new TField3D_UniformBox(v3(bfield), v3(width), v3(translation), v3(rotations), 0, 0, 0, ""));
+        (TField*) new TField3D_UniformBox(v3(bfield), v3(width), v3(translation), v3(rotations), g_freq, g_fphase,
+                                          g_toff, ""));
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -151,7 +170,7 @@ int ref_add_uniform(void* h, double const* bfield, double const* width, double c
 int ref_add_quadrupole(void* h, double K, double width, double const* rotations, double const* translation) {
   try {
     static_cast<OSCARSSR*>(h)->AddMagneticField(
-        (TField*) new TField3D_Quadrupole(K, width, v3(rotations), v3(translation), 0, 0, 0, ""));
+        (TField*) new TField3D_Quadrupole(K, width, v3(rotations), v3(translation), g_freq, g_fphase, g_toff, ""));
   } catch (std::exception const& e) { return fail(e); }
   return 0;
 }
@@ -290,16 +309,16 @@ int ref_get_particle(void* h, double* out /* x0[3] b0[3] t0 gamma q m qovermgamm
 void ref_set_ctstartstop(void* h, double a, double b) { static_cast<OSCARSSR*>(h)->SetCTStartStop(a, b); }
 void ref_set_npoints_trajectory(void* h, long n) { static_cast<OSCARSSR*>(h)->SetNPointsTrajectory((size_t) n); }
 
-int ref_get_bfield(void* h, double const* x, double* out) {
-  TVector3D B = static_cast<OSCARSSR*>(h)->GetB(v3(x), 0);
+int ref_get_bfield(void* h, double const* x, double t, double* out) {
+  TVector3D B = static_cast<OSCARSSR*>(h)->GetB(v3(x), t);
   out[0] = B.GetX(); out[1] = B.GetY(); out[2] = B.GetZ();
   return 0;
 }
 
 void ref_remove_efield(void* h, char const* name) { static_cast<OSCARSSR*>(h)->RemoveElectricField(name); }
 
-int ref_get_efield(void* h, double const* x, double* out) {
-  TVector3D E = static_cast<OSCARSSR*>(h)->GetE(v3(x), 0);
+int ref_get_efield(void* h, double const* x, double t, double* out) {
+  TVector3D E = static_cast<OSCARSSR*>(h)->GetE(v3(x), t);
   out[0] = E.GetX(); out[1] = E.GetY(); out[2] = E.GetZ();
   return 0;
 }
@@ -428,7 +447,7 @@ int ref_spectrum(void* h, double const* obs, int npoints, double e0, double e1, 
     }
     std::string P;
     make_pol(pol, angle, P);
-    s->CalculateSpectrum(v3(obs), C, P, angle, v3(hdir), v3(pdir), g_nparticles, nthreads, 0, -1, std::vector<int>(),
+    s->CalculateSpectrum(v3(obs), C, P, angle, v3(hdir), v3(pdir), g_nparticles, nthreads, g_gpu, -1, std::vector<int>(),
                          precision, max_level, max_level_extended, quantity);
     if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
     for (size_t i = 0; i != C.GetNPoints(); ++i) {
@@ -473,7 +492,7 @@ int ref_flux_rectangle(void* h, double energy_eV, const char* plane, int n1, int
     T3DScalarContainer C;
     std::string P;
     make_pol(pol, angle, P);
-    s->CalculateFlux(R, energy_eV, C, P, angle, v3(hdir), v3(pdir), g_nparticles, nthreads, 0, -1, std::vector<int>(),
+    s->CalculateFlux(R, energy_eV, C, P, angle, v3(hdir), v3(pdir), g_nparticles, nthreads, g_gpu, -1, std::vector<int>(),
                      precision, max_level, max_level_extended, dim, quantity);
     if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
     for (size_t i = 0; i != C.GetNPoints(); ++i) {
@@ -497,8 +516,8 @@ int ref_power_density_rectangle(void* h, const char* plane, int n1, int n2, doub
     make_rect(R, plane, n1, n2, w1, w2, rotations, translation, normal);
     T3DScalarContainer C;
     bool const Directional = normal == 0 ? false : true;
-    s->CalculatePowerDensity(R, C, dim, Directional, precision, max_level, max_level_extended, g_nparticles, nthreads, 0,
-                             -1, std::vector<int>(), quantity);
+    s->CalculatePowerDensity(R, C, dim, Directional, precision, max_level, max_level_extended, g_nparticles, nthreads,
+                             g_gpu, -1, std::vector<int>(), quantity);
     if (out_all_converged) *out_all_converged = C.AllConverged() ? 1 : 0;
     for (size_t i = 0; i != C.GetNPoints(); ++i) {
       T3DScalar const& p = C.GetPoint(i);

@@ -28,7 +28,8 @@ class om_field(C.Structure):
                 ("translation", C.c_double * 3), ("phase", C.c_double), ("taper", C.c_double),
                 ("grid_start", C.c_double * 3), ("grid_step", C.c_double * 3), ("grid_n", C.c_long * 3),
                 ("grid_data", _dp), ("rotations", C.c_double * 3), ("shape", C.c_double * 3), ("K", C.c_double),
-                ("width", C.c_double), ("grid_exact", C.c_int), ("grid_stop", C.c_double * 3)]
+                ("width", C.c_double), ("grid_exact", C.c_int), ("grid_stop", C.c_double * 3),
+                ("frequency", C.c_double), ("frequency_phase", C.c_double), ("time_offset", C.c_double)]
 
 
 class om_particle(C.Structure):
@@ -110,7 +111,7 @@ def lib():
                                            _dp, _ip, _up, C.POINTER(om_stats)]
     L.om_total_power.argtypes = [C.c_void_p, C.c_int, C.POINTER(om_conv), _dp, _dp, _ip, C.POINTER(om_stats)]
     L.om_efield_time.argtypes = [C.c_void_p, C.c_int, _dp, _lp, _dp, _dp]
-    L.om_field_points.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, _dp, C.c_long, _dp]
+    L.om_field_points.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, _dp, C.c_long, C.c_double, _dp]
     L.om_power_density_stl.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, C.POINTER(om_conv), _dp, _ip, _up,
                                        C.POINTER(om_stats)]
     _vp = C.c_void_p  # device addresses
@@ -307,12 +308,12 @@ class Device:
         self._chk(self.L.om_efield_time(self.h, int(particle), _ptr(o), C.byref(n), _ptr(t), _ptr(e)))
         return t, e
 
-    def field_points(self, fields, xyz):
-        """Sum of `fields` at the rows of xyz[n, 3], evaluated on the device: [n, 3]."""
+    def field_points(self, fields, xyz, t=0.0):
+        """Sum of `fields` at the rows of xyz[n, 3] and time t [s], evaluated on the device: [n, 3]."""
         x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         out = np.zeros_like(x)
         fa = (om_field * max(1, len(fields)))(*fields)
-        self._chk(self.L.om_field_points(self.h, fa, len(fields), _ptr(x), len(x), _ptr(out)))
+        self._chk(self.L.om_field_points(self.h, fa, len(fields), _ptr(x), len(x), float(t), _ptr(out)))
         return out
 
     def total_power(self, particle, conv):

@@ -491,9 +491,9 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
     for (int r = 0; r < 6; ++r)
       for (int q = 0; q < 6; ++q) M[r][q] = Mn[r][q];
     // next state, as the forward computes it
-    double d[6];
-    derivs<kFieldsGeneric>(rc, none, y, d);
-    rk4<kFieldsGeneric>(rc, none, y, d, A.dt);
+    double d[6];  // the reverse pass takes time-independent fields only: t does not enter
+    derivs<kFieldsGeneric>(rc, none, 0.0, y, d);
+    rk4<kFieldsGeneric>(rc, none, y, d, 0.0, A.dt);
   }
   double* o = A.segmap + 42 * s;
   for (int r = 0; r < 6; ++r)
@@ -535,8 +535,8 @@ __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
     for (int r = 0; r < 6; ++r) A.ybuf[6 * i + r] = y[r];
     if (i + 1 < i1) {
       double d[6];
-      derivs<kFieldsGeneric>(rc, none, y, d);
-      rk4<kFieldsGeneric>(rc, none, y, d, A.dt);
+      derivs<kFieldsGeneric>(rc, none, 0.0, y, d);
+      rk4<kFieldsGeneric>(rc, none, y, d, 0.0, A.dt);
     }
   }
   double lam[6];

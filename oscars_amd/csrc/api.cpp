@@ -32,7 +32,7 @@ hipError_t om_launch_lw_batch_stages(const om::BatchArgs* A, int mixed, int n_gr
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s, int nefields);
 hipError_t om_launch_knot_finish(double* p, long total, hipStream_t s);
-hipError_t om_launch_field_points(const om::FieldDev* F, int nf, const double* xyz, long n, double* out,
+hipError_t om_launch_field_points(const om::FieldDev* F, int nf, const double* xyz, long n, double t, double* out,
                                   hipStream_t s);
 hipError_t om_launch_efield_time(const double* p, long n, const double* obs, double c0, double dt, double* out_t,
                                  double* out_e, hipStream_t s);
@@ -94,12 +94,40 @@ void check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+
+// Nonzero while the allocation-free entry point runs on this thread: any
+// device or pinned allocation is then a programming error and throws.
+thread_local int g_no_alloc = 0;
+
+struct NoAllocScope {
+  NoAllocScope() { ++g_no_alloc; }
+  ~NoAllocScope() { --g_no_alloc; }
+};
+
+// Bump allocator over the static device scratch array that the
+// allocation-free entry point uses in place of the context's buffers.
+struct Arena {
+  unsigned char* base = nullptr;
+  size_t cap = 0, used = 0;
+  template <class T>
+  T* take(size_t count) {
+    size_t bytes = (count * sizeof(T) + 255) & ~(size_t) 255;
+    if (bytes > cap - used)
+      throw std::length_error("the request needs more device scratch than the allocation-free entry point has (" +
+                              std::to_string(cap >> 20) + " MiB)");
+    T* p = reinterpret_cast<T*>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
   void ensure(size_t count) {
     if (count <= n && p) return;
+    if (g_no_alloc) throw std::logic_error("device allocation inside the allocation-free entry point");
     if (p) check(hipFree(p), "hipFree");
     p = nullptr;
     n = 0;
@@ -125,6 +153,8 @@ struct PinnedStage {
   void* get(size_t bytes) {
     if (pending) check(hipEventSynchronize(done), "staging wait");
     pending = false;
+    if (g_no_alloc && (!done || bytes > n))
+      throw std::logic_error("host staging allocation inside the allocation-free entry point");
     if (!done) check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "event");
     if (bytes > n) {
       if (p) check(hipHostFree(p), "hipHostFree");
@@ -321,6 +351,11 @@ namespace {
 // for this stream only (a copy into pageable memory can also wait for work
 // queued on other streams).
 void read_back(om_context* ctx, void* dst, const void* dsrc, size_t bytes) {
+  if (g_no_alloc) {  // straight into the caller's host memory: nothing is allocated
+    check(hipMemcpyAsync(dst, dsrc, bytes, hipMemcpyDeviceToHost, ctx->stream), "read back");
+    check(hipStreamSynchronize(ctx->stream), "sync");
+    return;
+  }
   void* ph = ctx->stage_rb.get(bytes);
   check(hipMemcpyAsync(ph, dsrc, bytes, hipMemcpyDeviceToHost, ctx->stream), "read back");
   check(hipStreamSynchronize(ctx->stream), "sync");
@@ -354,6 +389,13 @@ om::FieldDev make_field(om_context* ctx, const om_field& f) {
   }
   d.phase = f.phase;
   d.taper = f.taper;
+  // time dependence (src/TField3D_IdealUndulator.cc:147-151 and the other
+  // models; a grid multiplies by the factor even at zero frequency,
+  // src/TField3D_Grid.cc:277, which is 1 unless frequency_phase is set)
+  d.freq = f.frequency;
+  d.fphase = f.frequency_phase;
+  d.toff = f.time_offset;
+  d.td = f.frequency != 0 || (f.type == OM_FIELD_GRID && f.frequency_phase != 0) ? 1 : 0;
   double pm = std::sqrt(f.period[0] * f.period[0] + f.period[1] * f.period[1] + f.period[2] * f.period[2]);
   d.period_length = pm;
   d.inv_period_length = 1.0 / pm;
@@ -460,15 +502,20 @@ double dt_inclusive(double ts, double te, int L) { return (te - ts) / (std::pow(
 
 // Make sure levels [0, Lb] of trajectory ip exist on the device with the right
 // (memory / extended) spacing.
-void ensure_levels(om_context* ctx, int ip, int Lb, int stop_mem) {
-  LevelCache& lc = ctx->levels[ip];
+void ensure_levels(om_context* ctx, int ip, int Lb, int stop_mem, LevelCache* lc_own = nullptr,
+                   Arena* ar = nullptr) {
+  LevelCache& lc = lc_own ? *lc_own : ctx->levels[ip];
   const om::TrajDesc& D = ctx->desc[ip];
   double ts = ctx->tstart[ip], te = ctx->tstop[ip];
   for (int L = 0; L <= Lb; ++L) {
     int ext = L > stop_mem ? 1 : 0;
     if (lc.rows[L] && lc.built[L] && lc.extended[L] == ext) continue;
     long np = 1L << L;
-    if (!lc.rows[L]) check(hipMalloc((void**) &lc.rows[L], (size_t) np * 10 * sizeof(double)), "hipMalloc level");
+    if (!lc.rows[L] && ar) lc.rows[L] = ar->take<double>((size_t) np * 10);
+    if (!lc.rows[L]) {
+      if (g_no_alloc) throw std::logic_error("device allocation inside the allocation-free entry point");
+      check(hipMalloc((void**) &lc.rows[L], (size_t) np * 10 * sizeof(double)), "hipMalloc level");
+    }
     lc.built[L] = true;
     lc.extended[L] = ext;
     lc.have_bd[L] = false;
@@ -510,9 +557,9 @@ void ensure_beta_diff(om_context* ctx, int ip, int Lb) {
   }
 }
 
-void fill_levels(om_context* ctx, int ip, int Lb, om::Levels& lv) {
+void fill_levels(om_context* ctx, int ip, int Lb, om::Levels& lv, LevelCache* lc_own = nullptr) {
   std::memset(&lv, 0, sizeof(lv));
-  LevelCache& lc = ctx->levels[ip];
+  LevelCache& lc = lc_own ? *lc_own : ctx->levels[ip];
   for (int L = 0; L <= om::kMaxLevel; ++L) {
     lv.rows[L] = L <= Lb ? lc.rows[L] : nullptr;
     lv.dt_incl[L] = dt_inclusive(ctx->tstart[ip], ctx->tstop[ip], L);
@@ -520,23 +567,23 @@ void fill_levels(om_context* ctx, int ip, int Lb, om::Levels& lv) {
   }
 }
 
-void state_for(om_context* ctx, long npts, bool mixed, om::State& st) {
-  ctx->d_sum.ensure(6 * npts);
-  ctx->d_lastmag.ensure(npts);
-  ctx->d_prec.ensure(npts);
-  ctx->d_level.ensure(npts);
-  ctx->d_conv.ensure(npts);
-  if (mixed) {
-    ctx->d_err.ensure(2 * npts);
-    ctx->d_border.ensure(npts);
-  }
-  st.sum = ctx->d_sum.p;
-  st.err = mixed ? ctx->d_err.p : nullptr;
-  st.border = mixed ? ctx->d_border.p : nullptr;
-  st.lastmag = ctx->d_lastmag.p;
-  st.prec = ctx->d_prec.p;
-  st.level = ctx->d_level.p;
-  st.conv = ctx->d_conv.p;
+// A device buffer of n elements: from the arena when one is given, else the
+// context's cached buffer b (grown as needed).
+template <class T>
+T* scratch_buf(DevBuf<T>& b, size_t n, Arena* ar) {
+  if (ar) return ar->take<T>(n);
+  b.ensure(n);
+  return b.p;
+}
+
+void state_for(om_context* ctx, long npts, bool mixed, om::State& st, Arena* ar = nullptr) {
+  st.sum = scratch_buf(ctx->d_sum, 6 * npts, ar);
+  st.lastmag = scratch_buf(ctx->d_lastmag, npts, ar);
+  st.prec = scratch_buf(ctx->d_prec, npts, ar);
+  st.level = scratch_buf(ctx->d_level, npts, ar);
+  st.conv = scratch_buf(ctx->d_conv, npts, ar);
+  st.err = mixed ? scratch_buf(ctx->d_err, 2 * npts, ar) : nullptr;
+  st.border = mixed ? scratch_buf(ctx->d_border, npts, ar) : nullptr;
   check(om_launch_state_reset(&st, npts, ctx->stream), "state reset");
 }
 
@@ -563,11 +610,15 @@ void sync_desc(om_context* ctx) {
   ctx->desc_pending = false;
 }
 
+// status 1: a step still met |beta| >= 1 after halving it om::kMaxHalving
+// times (the reference recurses without limit, src/OSCARSSR.cc:1362-1378)
+const char* const kHalvingLimitMsg =
+    "trajectory integration met |beta| >= 1 and halving the RK4 step 40 times did not resolve it";
+
 void check_particle(om_context* ctx, int ip) {
   if (ip < 0 || ip >= ctx->np) throw std::out_of_range("particle index out of range (no trajectory computed?)");
   sync_desc(ctx);
-  if (ctx->desc[ip].status != 0)
-    throw std::out_of_range("trajectory integration met |beta| >= 1; step halving is not supported on the device");
+  if (ctx->desc[ip].status != 0) throw std::out_of_range(kHalvingLimitMsg);
 }
 
 template <class T>
@@ -597,18 +648,16 @@ struct Timer {
 // with dev_io, device pointers written in place.  Statistics are reduced on
 // the device.
 void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_value, int* out_level,
-                    unsigned char* out_conv, double* out_e, om_stats* stats, const om_conv& conv, bool dev_io) {
-  ctx->d_outcomp.ensure(npts);
-  F.out_comp = ctx->d_outcomp.p;
+                    unsigned char* out_conv, double* out_e, om_stats* stats, const om_conv& conv, bool dev_io,
+                    Arena* ar = nullptr) {
+  F.out_comp = scratch_buf(ctx->d_outcomp, npts, ar);
   F.weight = conv.weight;
   F.accumulate = conv.accumulate;
   if (dev_io) {
-    ctx->d_outlevel.ensure(npts);
-    ctx->d_outconv.ensure(npts);
     F.out = out_value;  // a running sum (accumulate) is already there
     F.out_e = out_e;
-    F.out_level = out_level ? out_level : ctx->d_outlevel.p;
-    F.out_conv = out_conv ? out_conv : ctx->d_outconv.p;
+    F.out_level = out_level ? out_level : scratch_buf(ctx->d_outlevel, npts, ar);
+    F.out_conv = out_conv ? out_conv : scratch_buf(ctx->d_outconv, npts, ar);
   } else {
     ctx->d_out.ensure(npts);
     ctx->d_outlevel.ensure(npts);
@@ -627,10 +676,8 @@ void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_valu
   }
   check(om_launch_finalize(&F, ctx->stream), "finalize");
   unsigned long long st3[3] = {0, 0, 0};
-  if (stats) {
-    ctx->d_stats.ensure(3);
-    check(om_launch_level_stats(F.out_level, F.out_conv, npts, ctx->d_stats.p, ctx->stream), "level stats");
-  }
+  unsigned long long* d_stats = stats ? scratch_buf(ctx->d_stats, 3, ar) : nullptr;
+  if (stats) check(om_launch_level_stats(F.out_level, F.out_conv, npts, d_stats, ctx->stream), "level stats");
   if (!dev_io) {
     check(hipMemcpyAsync(out_value, ctx->d_out.p, npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
           "out");
@@ -642,7 +689,7 @@ void finish_outputs(om_context* ctx, om::FinArgs& F, long npts, double* out_valu
       check(hipMemcpyAsync(out_e, ctx->d_oute.p, 6 * npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
             "efield");
   }
-  if (stats) read_back(ctx, st3, ctx->d_stats.p, sizeof(st3));  // also waits for the copies above
+  if (stats) read_back(ctx, st3, d_stats, sizeof(st3));  // also waits for the copies above
   else check(hipStreamSynchronize(ctx->stream), "sync");
   if (stats) {
     stats->evals = (double) st3[0];
@@ -674,12 +721,15 @@ void launch_trajectories(om_context* ctx, int np) {
         "spline");
 }
 
+// ar: the allocation-free entry point's scratch (device pointers only, dev_io);
+// every buffer of the call then comes from it and nothing is allocated.
 int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
             const double* pol6, const om_conv* conv, double* out_value, int* out_level, unsigned char* out_conv,
-            double* out_e, om_stats* stats, bool spectrum, bool dev_io) {
+            double* out_e, om_stats* stats, bool spectrum, bool dev_io, Arena* ar = nullptr) {
   return guarded(ctx, [&] {
     Timer wall;
     ctx->timer.used = 0;
+    if (ar && !dev_io) throw std::logic_error("the scratch arena takes device-resident inputs and outputs");
     check_particle(ctx, ip);
     if (!conv) throw std::invalid_argument("conv is NULL");
     if (n_obs < 1 || n_e < 1) throw std::length_error("no points specified");
@@ -689,7 +739,13 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     level_stops(*conv, stop_mem, stop_ext);
     if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
     if (!obs_xyz || !energies_eV || !out_value) throw std::invalid_argument("missing inputs or outputs");
-    if (dev_io) {
+    const double* d_obs = nullptr;
+    double* d_omega = nullptr;
+    if (ar) {  // the caller's device grid is read in place
+      d_obs = obs_xyz;
+      d_omega = ar->take<double>(n_e);
+      check(om_launch_omega_from_ev(energies_eV, n_e, d_omega, kQe, kHbar, ctx->stream), "omega");
+    } else if (dev_io) {
       ctx->d_obs.ensure(3 * n_obs);
       ctx->d_omega.ensure(n_e);
       check(hipMemcpyAsync(ctx->d_obs.p, obs_xyz, 3 * n_obs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
@@ -705,10 +761,21 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     bool mixed_any = mixed || conv->mode == OM_MODE_POINT_MIXED;
     om::LwArgs A;
     std::memset(&A, 0, sizeof(A));
-    state_for(ctx, npts, mixed_any, A.st);
+    state_for(ctx, npts, mixed_any, A.st, ar);
     A.npts = npts;
-    A.obs = ctx->d_obs.p;
-    A.omega = ctx->d_omega.p;
+    A.obs = ar ? d_obs : ctx->d_obs.p;
+    A.omega = ar ? d_omega : ctx->d_omega.p;
+    // level samples: the context's cache, or rebuilt into the arena per call
+    LevelCache arena_levels;
+    LevelCache* lc = ar ? &arena_levels : nullptr;
+    const size_t list_n = (size_t) n_obs * ((n_e + 63) / 64) * 64;
+    int* d_list = nullptr;
+    int* d_count = nullptr;
+    if (ar) {
+      d_list = ar->take<int>(list_n);
+      d_count = ar->take<int>(1);
+      check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    }
     A.n_omega = n_e;
     double pm2 = 0;
     for (int k = 0; k < 3; ++k) {
@@ -730,11 +797,11 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     long n_recheck = 0;
     int Ldone = 0;  // deepest level summed so far (its samples exist)
     while (true) {
-      ensure_levels(ctx, ip, Lb, stop_mem);
-      fill_levels(ctx, ip, Lb, A.lv);
+      ensure_levels(ctx, ip, Lb, stop_mem, lc, ar);
+      fill_levels(ctx, ip, Lb, A.lv, lc);
       A.La = La;
       A.Lb = Lb;
-      ctx->timer.begin(ctx->stream);
+      if (!ar) ctx->timer.begin(ctx->stream);
       if (!tiled && A.list && A.n_list <= kWaveListMax) {
         // few points left: one wave per point
         check(om_launch_lw_wave(&A, conv->mode == OM_MODE_POINT_MIXED ? 1 : 0, ctx->stream), "lw_wave");
@@ -746,22 +813,24 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       } else {
         check(om_launch_lw_tile(&A, n_obs, mixed ? 1 : 0, ctx->split, ctx->stream), "lw_tile");
       }
-      ctx->timer.end(ctx->stream);
+      if (!ar) ctx->timer.end(ctx->stream);
       Ldone = Lb;
       if (Lb >= stop_ext) break;
       La = Lb + 1;
       Lb = La;
-      ctx->d_list.ensure((size_t) n_obs * ((n_e + 63) / 64) * 64);
-      ctx->d_count.ensure(1);
+      if (!ar) {
+        d_list = scratch_buf(ctx->d_list, list_n, nullptr);
+        d_count = scratch_buf(ctx->d_count, 1, nullptr);
+      }
       if (tiled)
-        check(om_launch_tile_list(&A.st, n_obs, n_e, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "tile list");
+        check(om_launch_tile_list(&A.st, n_obs, n_e, La, d_list, d_count, ctx->stream), "tile list");
       else
-        check(om_launch_point_list(&A.st, npts, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "point list");
+        check(om_launch_point_list(&A.st, npts, La, d_list, d_count, ctx->stream), "point list");
       int nt = 0;
-      read_back(ctx, &nt, ctx->d_count.p, sizeof(int));
+      read_back(ctx, &nt, d_count, sizeof(int));
       if (nt == 0) break;
       ntiles = nt;
-      A.list = ctx->d_list.p;
+      A.list = d_list;
       A.n_list = nt;
     }
     // Mixed path: points whose stop decision lay within the FP32 error bound are
@@ -769,35 +838,37 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     // reset when listed), so every point stops where the FP64 path stops: one
     // launch over the levels the main pass reached, then one per deeper level.
     if (mixed_any) {
-      ctx->d_list.ensure((size_t) n_obs * ((n_e + 63) / 64) * 64);
-      ctx->d_count.ensure(1);
-      check(om_launch_border_list(&A.st, npts, ctx->d_list.p, ctx->d_count.p, ctx->stream), "border list");
+      if (!ar) {
+        d_list = scratch_buf(ctx->d_list, list_n, nullptr);
+        d_count = scratch_buf(ctx->d_count, 1, nullptr);
+      }
+      check(om_launch_border_list(&A.st, npts, d_list, d_count, ctx->stream), "border list");
       int nb = 0;
-      read_back(ctx, &nb, ctx->d_count.p, sizeof(int));
+      read_back(ctx, &nb, d_count, sizeof(int));
       n_recheck = nb;
       if (nb > 0) {
         om::State st64 = A.st;
         st64.err = nullptr;
         st64.border = nullptr;
         A.st = st64;
-        A.list = ctx->d_list.p;
+        A.list = d_list;
         A.n_list = nb;
         La = 0;
         Lb = Ldone;
         while (true) {
-          ensure_levels(ctx, ip, Lb, stop_mem);
-          fill_levels(ctx, ip, Lb, A.lv);
+          ensure_levels(ctx, ip, Lb, stop_mem, lc, ar);
+          fill_levels(ctx, ip, Lb, A.lv, lc);
           A.La = La;
           A.Lb = Lb;
-          ctx->timer.begin(ctx->stream);
+          if (!ar) ctx->timer.begin(ctx->stream);
           check(om_launch_lw_wave(&A, 0, ctx->stream), "lw_wave recheck");
-          ctx->timer.end(ctx->stream);
+          if (!ar) ctx->timer.end(ctx->stream);
           if (Lb >= stop_ext) break;
           La = Lb + 1;
           Lb = La;
-          check(om_launch_point_list(&A.st, npts, La, ctx->d_list.p, ctx->d_count.p, ctx->stream), "point list");
+          check(om_launch_point_list(&A.st, npts, La, d_list, d_count, ctx->stream), "point list");
           int nt = 0;
-          read_back(ctx, &nt, ctx->d_count.p, sizeof(int));
+          read_back(ctx, &nt, d_count, sizeof(int));
           if (nt == 0) break;
           A.n_list = nt;
         }
@@ -817,8 +888,13 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     for (int L = 0; L <= om::kMaxLevel; ++L) F.dt_incl[L] = A.lv.dt_incl[L];
     std::memcpy(F.pol, A.pol, sizeof(F.pol));
     F.use_pol = A.use_pol;
-    finish_outputs(ctx, F, npts, out_value, out_level, out_conv, out_e, stats, *conv, dev_io);
-    kms = ctx->timer.take();  // the stream is synchronised
+    if (ar) check(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    finish_outputs(ctx, F, npts, out_value, out_level, out_conv, out_e, stats, *conv, dev_io, ar);
+    if (ar) {  // the whole call's device time (the stream is synchronised)
+      check(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1), "elapsed");
+    } else {
+      kms = ctx->timer.take();
+    }
     if (stats) {
       stats->kernel_ms = kms;
       stats->total_ms = wall.ms();
@@ -1028,7 +1104,8 @@ int om_efield_time(om_context* ctx, int i, const double* obs, long* n, double* o
   });
 }
 
-int om_field_points(om_context* ctx, const om_field* fields, int nfields, const double* xyz, long n, double* out3) {
+int om_field_points(om_context* ctx, const om_field* fields, int nfields, const double* xyz, long n, double t,
+                    double* out3) {
   return guarded(ctx, [&] {
     if (nfields < 0 || nfields > om::kMaxFields) throw std::out_of_range("too many fields");
     if (n < 0 || (n > 0 && (!xyz || !out3))) throw std::invalid_argument("missing inputs");
@@ -1046,7 +1123,7 @@ int om_field_points(om_context* ctx, const om_field* fields, int nfields, const 
       upload_staged(ctx, ctx->stage_fields, ctx->d_fp_fields, F.data(), F.size());
       upload_staged(ctx, ctx->stage_rb, ctx->d_fp, xyz, (size_t) (3 * n));
       ctx->d_fp_out.ensure(3 * n);
-      check(om_launch_field_points(ctx->d_fp_fields.p, nfields, ctx->d_fp.p, n, ctx->d_fp_out.p, ctx->stream),
+      check(om_launch_field_points(ctx->d_fp_fields.p, nfields, ctx->d_fp.p, n, t, ctx->d_fp_out.p, ctx->stream),
             "field points");
       read_back(ctx, out3, ctx->d_fp_out.p, 3 * n * sizeof(double));
     } catch (...) {
@@ -1115,6 +1192,7 @@ int om_flux_dev(om_context* ctx, int particle, const double* obs_xyz, long n_obs
   return lw_sums(ctx, particle, obs_xyz, n_obs, energies_eV, n_e, pol6, conv, out_value, out_level, out_conv, out_e,
                  stats, false, true);
 }
+
 
 int om_spectrum_dev(om_context* ctx, int particle, const double* obs, const double* energies_eV, long n_e,
                     const double* pol6, const om_conv* conv, double* out_value, int* out_level,
@@ -1397,7 +1475,7 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     if (ctx->user_traj) throw std::invalid_argument("batched sums need integrated trajectories (om_trajectories)");
     sync_desc(ctx);
     for (int i = 0; i < np; ++i) {
-      if (ctx->desc[i].status != 0) throw std::out_of_range("trajectory integration met |beta| >= 1");
+      if (ctx->desc[i].status != 0) throw std::out_of_range(kHalvingLimitMsg);
       if (req_of[i] < 0 || req_of[i] >= n_req) throw std::out_of_range("req_of out of range");
       if (ctx->parts[i].q != ctx->parts[0].q || ctx->parts[i].current != ctx->parts[0].current)
         throw std::invalid_argument("all particles of a batch must share charge and current");
@@ -1707,6 +1785,9 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
         throw std::invalid_argument("the reverse pass supports grid fields without rotations only");
     if (grad_field >= 0 && fields[grad_field].type != OM_FIELD_GRID)
       throw std::invalid_argument("grad_field must name a grid field");
+    for (int i = 0; i < nfields; ++i)
+      if (fields[i].frequency != 0 || (fields[i].type == OM_FIELD_GRID && fields[i].frequency_phase != 0))
+        throw std::invalid_argument("the reverse pass supports time-independent fields only");
     ctx->adj.armed = true;
   });
   if (rc) return rc;
@@ -1716,7 +1797,9 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
   std::vector<double> E(6 * n_obs * n_e);
   rc = guarded(ctx, [&] {
     sync_desc(ctx);
-    if (ctx->desc[0].status != 0) throw std::out_of_range("trajectory integration met |beta| >= 1");
+    if (ctx->desc[0].status != 0) throw std::out_of_range(kHalvingLimitMsg);
+    if (ctx->desc[0].halved != 0)
+      throw std::invalid_argument("the reverse pass does not follow halved RK4 steps (|beta| >= 1 was met)");
     if (ctx->desc[0].nb != 0)
       throw std::invalid_argument("the reverse pass needs ctstart == T0 (no backward integration)");
   });

@@ -104,6 +104,13 @@ __device__ __forceinline__ double sin64(double x) {
   return s;
 }
 
+// cos(x) for the time dependence of the field models, as sin64.
+__device__ __forceinline__ double cos64(double x) {
+  double s, c;
+  sincos_reduced64(reduce_pio2(x), &s, &c);
+  return c;
+}
+
 // FP32 polynomial sin/cos on the FP64-reduced argument (mixed-precision path).
 __device__ __forceinline__ void sincos_reduced32(Reduced a, float* s, float* c) {
   float r = (float) a.r;

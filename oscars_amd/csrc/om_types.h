@@ -33,7 +33,11 @@ struct FieldDev {
   double shape[3];
   double K, width;
   int dims;  // grid: axes with more than one sample, bit 0 x, bit 1 y, bit 2 z
-  int pad2;
+  // time dependence: the model's value is multiplied by
+  // cos(2 pi freq (t + toff) + fphase) where td is set (any model with a
+  // frequency; a grid also with a phase alone, src/TField3D_Grid.cc:277)
+  int td;
+  double freq, fphase, toff;
 };
 
 // One particle's trajectory request (all per particle so that a batch can have
@@ -47,8 +51,8 @@ struct TrajDesc {
   long nf, nb;       // forward / backward point counts
   long knot_offset;  // first knot in the packed knot arrays
   long n;            // nf + nb
-  int status;        // written by the kernel: 0 ok, 1 |beta| >= 1 encountered
-  int pad;
+  int status;        // written by the kernel: 0 ok, 1 the step halving met its depth limit
+  int halved;        // written by the kernel: number of steps that were halved (|beta| >= 1 met)
 };
 
 }  // namespace om

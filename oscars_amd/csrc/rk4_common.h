@@ -16,10 +16,10 @@ namespace om {
 // ---------------------------------------------------------------- fields
 // Written without pointers into the field description or run-time array
 // indices, so a single field held by value stays in registers.
-__device__ __forceinline__ void undulator_part(const FieldDev& f, double D, double PhaseShift, double a0, double a1,
+__device__ __forceinline__ bool undulator_part(const FieldDev& f, double D, double PhaseShift, double a0, double a1,
                                                double a2, double taper_corr, double out[3], bool add) {
   double L = f.und_length, PL = f.period_length;
-  if (D > L / 2. + PhaseShift || D < -L / 2. + PhaseShift) return;
+  if (D > L / 2. + PhaseShift || D < -L / 2. + PhaseShift) return false;
   double scale = 1;
   bool term = false;
   if (D < -L / 2. + PhaseShift + PL || D > L / 2. + PhaseShift - PL) {
@@ -34,6 +34,20 @@ __device__ __forceinline__ void undulator_part(const FieldDev& f, double D, doub
   out[0] = add ? out[0] + v0 : v0;
   out[1] = add ? out[1] + v1 : v1;
   out[2] = add ? out[2] + v2 : v2;
+  return true;
+}
+
+// cos(2 pi f (t + t0) + phase), src/TField3D_IdealUndulator.cc:151 (the same
+// expression in every model)
+__device__ __forceinline__ double td_factor(const FieldDev& f, double t) {
+  return cos64(2.0 * kPi * f.freq * (t + f.toff) + f.fphase);
+}
+
+__device__ __forceinline__ void td_apply(const FieldDev& f, double t, double out[3]) {
+  double c = td_factor(f, t);
+  out[0] = out[0] * c;
+  out[1] = out[1] * c;
+  out[2] = out[2] * c;
 }
 
 __device__ __forceinline__ void rot_xyz(double v[3], const FieldDev& f) {
@@ -99,13 +113,18 @@ __device__ __noinline__ void grid_multi(const FieldDev& f, const double* X, doub
   }
 }
 
-__device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y, double z, double out[3]) {
+// One model's value at (x, y, z) and time t [s].  The time factor is applied
+// where the reference applies it: not to the zero an undulator, a uniform
+// box, a quadrupole or a grid returns outside its region.
+__device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y, double z, double t,
+                                           double out[3]) {
   out[0] = out[1] = out[2] = 0;
   if (f.type == kUndulator || f.type == kEPU) {
     double D = (x - f.center[0]) * f.unit[0] + (y - f.center[1]) * f.unit[1] + (z - f.center[2]) * f.unit[2];
     double tc = 1 + D * f.taper;
-    undulator_part(f, D, f.phase_shift_a, f.field[0], f.field[1], f.field[2], tc, out, false);
+    bool in = undulator_part(f, D, f.phase_shift_a, f.field[0], f.field[1], f.field[2], tc, out, false);
     if (f.type == kEPU) undulator_part(f, D, f.phase_shift_b, f.fieldB[0], f.fieldB[1], f.fieldB[2], tc, out, true);
+    if (f.td && (in || f.type == kEPU)) td_apply(f, t, out);
     return;
   }
   double X[3] = {x, y, z};
@@ -119,6 +138,7 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
           (f.shape[2] > 0 && fabs(R2) > fabs(f.shape[2] / 2.)))
         return;
       out[0] = f.field[0]; out[1] = f.field[1]; out[2] = f.field[2];
+      if (f.td) td_apply(f, t, out);
       return;
     }
     double frac = 1;
@@ -126,6 +146,7 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
     if (f.shape[1] > 0) { double u = R1 / f.shape[1]; frac *= exp(-(u * u) / 2.); }
     if (f.shape[2] > 0) { double u = R2 / f.shape[2]; frac *= exp(-(u * u) / 2.); }
     out[0] = frac * f.field[0]; out[1] = frac * f.field[1]; out[2] = frac * f.field[2];
+    if (f.td) td_apply(f, t, out);
     return;
   }
   if (f.type == kQuadrupole) {
@@ -135,6 +156,7 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
     double V[3] = {f.K * X[1], f.K * X[0], 0};
     rot_xyz(V, f);
     out[0] = V[0]; out[1] = V[1]; out[2] = V[2];
+    if (f.td) td_apply(f, t, out);
     return;
   }
   X[0] -= f.trans[0]; X[1] -= f.trans[1]; X[2] -= f.trans[2];
@@ -143,6 +165,7 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
   if (f.n[2] > 1 && (X[2] <= f.start[2] || X[2] >= f.stop[2])) return;
   if (f.axis < 0) {
     grid_multi(f, X, out);
+    if (f.td) td_apply(f, t, out);
     return;
   }
   int a = f.axis;
@@ -156,17 +179,19 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
   double dd = (coord - start) - (double) n0 * step;
   const double* lo = f.data + 3 * n0;
   double l0 = lo[0], l1 = lo[1], l2 = lo[2], h0 = lo[3], h1 = lo[4], h2 = lo[5];
-  out[0] = (l0 + div_by((h0 - l0) * dd, step, istep)) * 1.0;
-  out[1] = (l1 + div_by((h1 - l1) * dd, step, istep)) * 1.0;
-  out[2] = (l2 + div_by((h2 - l2) * dd, step, istep)) * 1.0;
+  // src/TField3D_Grid.cc:277: the factor cos(2 pi f (t + t0) + phase), 1 without time dependence
+  const double c = f.td ? td_factor(f, t) : 1.0;
+  out[0] = (l0 + div_by((h0 - l0) * dd, step, istep)) * c;
+  out[1] = (l1 + div_by((h1 - l1) * dd, step, istep)) * c;
+  out[2] = (l2 + div_by((h2 - l2) * dd, step, istep)) * c;
 }
 
-__device__ __forceinline__ void get_b(const FieldDev* __restrict__ F, int nf, double x, double y, double z,
+__device__ __forceinline__ void get_b(const FieldDev* __restrict__ F, int nf, double x, double y, double z, double t,
                                       double B[3]) {
   double s0 = 0, s1 = 0, s2 = 0;
   for (int i = 0; i < nf; ++i) {
     double v[3];
-    field_eval(F[i], x, y, z, v);
+    field_eval(F[i], x, y, z, t, v);
     s0 += v[0]; s1 += v[1]; s2 += v[2];
   }
   B[0] = s0; B[1] = s1; B[2] = s2;
@@ -230,6 +255,7 @@ struct RkCtx {
   GridZ g;
   int nfe;     // electric fields, stored after the nf magnetic ones in F
   double qom;  // Q / M
+  int halved;  // steps halved so far (src/OSCARSSR.cc:1362-1378)
 };
 
 // Field-model variants of the equations of motion:
@@ -263,9 +289,31 @@ __device__ __forceinline__ void grid_z_eval(const GridZ& g, double z, double B[3
   for (int k = 0; k < 3; ++k) B[k] = lo[k] + div_by((lo[3 + k] - lo[k]) * dd, g.step, g.inv_step);
 }
 
+// |beta| >= 1 test of src/OSCARSSR.cc:1260-1264 and 1368,
+//   1. - (v1 v1 + v3 v3 + v5 v5) / (c c) <= 0,
+// decided exactly without the division: with q the same sum of squares and
+// b = RN(c c), RN(q / b) >= 1 exactly when q / b >= 1 - 2^-54 (the midpoint
+// below 1 rounds to 1), i.e. when q - b >= -b 2^-54, and q - b is exact
+// wherever it decides the outcome (q within a factor 2 of b).
+constexpr double kC2 = kC * kC;
+constexpr double kBetaOneMargin = -kC2 * 5.551115123125783e-17;  // -b 2^-54
+
+__device__ __forceinline__ bool beta_at_least_one(double v1, double v3, double v5) {
+  double q = v1 * v1 + v3 * v3 + v5 * v5;
+  return q - kC2 >= kBetaOneMargin;
+}
+
+// Equations of motion at time t [s] (the field models read t only when they
+// are time-dependent).  A state with |beta| >= 1 sets c.err and leaves d as it
+// was (src/OSCARSSR.cc:1260-1264); with electric fields the reference sets the
+// flag and still writes d.
 template <int M>
-__device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const double* x, double* d) {
+__device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, const double* x, double* d) {
   if (M >= kAxisUndX && M <= kAxisUndZ) {
+    if (beta_at_least_one(x[1], x[3], x[5])) {
+      c.err = 1;
+      return;
+    }
     constexpr int K = M - kAxisUndX;
     double D = x[4] - f0.c2;
     double sc = und_scale(f0, D, f0.psA);
@@ -295,14 +343,11 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
   if (M == kFieldsEB) {
     // src/OSCARSSR.cc:1283-1318: magnetic and electric fields, the factor
     // Q/M sqrt(1 - beta^2) taken at each evaluation
+    if (beta_at_least_one(x[1], x[3], x[5])) c.err = 1;
     double bb = (x[1] * x[1] + x[3] * x[3] + x[5] * x[5]) / (kC * kC);
-    if (1. - bb <= 0) {
-      c.err = 1;
-      return;
-    }
     double B[3], E[3];
-    get_b(c.F, c.nf, x[0], x[2], x[4], B);
-    get_b(c.F + c.nf, c.nfe, x[0], x[2], x[4], E);
+    get_b(c.F, c.nf, x[0], x[2], x[4], t, B);
+    get_b(c.F + c.nf, c.nfe, x[0], x[2], x[4], t, E);
     double q = c.qom * sqrt(1. - bb);
     double be = (x[1] * E[0] + x[3] * E[1] + x[5] * E[2]) / kC;
     d[0] = x[1];
@@ -313,11 +358,7 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
     d[5] = q * (E[2] + x[1] * B[1] - x[3] * B[0] - x[5] * be / kC);
     return;
   }
-  // |beta| >= 1 test of src/OSCARSSR.cc:1260-1264; only its sign is used, so
-  // the product with 1/c^2 stands in for the division without changing it
-  // except within one rounding of |beta| = 1.
-  double OneMinus = 1. - (x[1] * x[1] + x[3] * x[3] + x[5] * x[5]) * (1.0 / (kC * kC));
-  if (OneMinus <= 0) {
+  if (beta_at_least_one(x[1], x[3], x[5])) {
     c.err = 1;
     return;
   }
@@ -327,7 +368,7 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
   } else if (M == kOneUndulator) {
     und_eval(f0, x[0], x[2], x[4], B);
   } else {
-    get_b(c.F, c.nf, x[0], x[2], x[4], B);
+    get_b(c.F, c.nf, x[0], x[2], x[4], t, B);
   }
   d[0] = x[1];
   d[1] = c.qomg * (-x[5] * B[1] + x[3] * B[2]);
@@ -337,27 +378,112 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, const doub
   d[5] = c.qomg * (x[1] * B[1] - x[3] * B[0]);
 }
 
-// Classic RK4 step; returns false when |beta| >= 1 was met (the reference then
-// halves the step recursively; that branch never triggers for physical beams,
-// so the kernel reports it as an error instead of recursing).
+// One classic RK4 step from (t, y) with slope dydx into yo
+// (src/OSCARSSR.cc:1321-1360); returns true when |beta| >= 1 was met in a
+// stage or in the result (the step must then be halved).
 template <int M>
-__device__ __forceinline__ bool rk4(RkCtx& c, const UndParams& f0, double* y, const double* dydx, double h) {
+__device__ __forceinline__ bool rk4_try(RkCtx& c, const UndParams& f0, const double* y, const double* dydx,
+                                        double t, double h, double* yo) {
   double dym[6], dyt[6], yt[6];
   double hh = h * 0.5;
   double h6 = h / 6.0;
+  double xh = t + hh;
+  for (int i = 0; i < 6; ++i) dyt[i] = dydx[i];  // a stage with |beta| >= 1 keeps earlier values
   for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
-  derivs<M>(c, f0, yt, dyt);
+  derivs<M>(c, f0, xh, yt, dyt);
   for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
-  derivs<M>(c, f0, yt, dym);
+  for (int i = 0; i < 6; ++i) dym[i] = dyt[i];
+  derivs<M>(c, f0, xh, yt, dym);
   for (int i = 0; i < 6; ++i) {
     yt[i] = y[i] + h * dym[i];
     dym[i] += dyt[i];
   }
-  derivs<M>(c, f0, yt, dyt);
-  double yo[6];
+  derivs<M>(c, f0, t + h, yt, dyt);
   for (int i = 0; i < 6; ++i) yo[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
-  bool bad = 1. - (yo[1] * yo[1] + yo[3] * yo[3] + yo[5] * yo[5]) * (1.0 / (kC * kC)) <= 0;
-  if (c.err || bad) return false;
+  return c.err || beta_at_least_one(yo[1], yo[3], yo[5]);
+}
+
+// Deepest halving of one step (h / 2^kMaxHalving); the reference recurses
+// without limit, here the integration stops with an error status instead.
+constexpr int kMaxHalving = 40;
+
+// State and slope of one integration, passed by value to the halving walk so
+// that the integration loop never takes their address (they stay in registers).
+struct RkPair {
+  double y[6], d[6];
+  int err, halved, ok;
+};
+
+// src/OSCARSSR.cc:1362-1378: a step that met |beta| >= 1 is split in two
+// halves, recursively.  The recursion is walked with an explicit stack of the
+// sub-steps' start times and lengths: a failed sub-step recomputes the slope at
+// its start and descends into its first half; a completed first half
+// recomputes the slope at the midpoint and continues with the second half.
+// The slope ends as the last one computed, as the reference's does.  ok = 0
+// when the depth limit is reached.  Out of line: it runs only for steps that
+// failed, and everything it needs arrives by value.
+template <int M>
+__device__ __noinline__ RkPair rk4_halving(RkCtx c, const UndParams f0, RkPair s, double t, double h) {
+  double sx[kMaxHalving + 1], sh[kMaxHalving + 1];
+  bool second[kMaxHalving + 1];
+  sx[0] = t;
+  sh[0] = h;
+  second[0] = false;
+  int d = 0;  // the failed sub-step
+  s.ok = 1;
+  while (true) {
+    // split sub-step d: slope at its start, then its first half
+    c.err = 0;
+    derivs<M>(c, f0, sx[d], s.y, s.d);
+    if (d == kMaxHalving) {
+      s.ok = 0;
+      break;
+    }
+    sx[d + 1] = sx[d];
+    sh[d + 1] = sh[d] / 2.;
+    second[d + 1] = false;
+    ++d;
+    bool done = false;
+    while (true) {
+      double yo[6];
+      if (rk4_try<M>(c, f0, s.y, s.d, sx[d], sh[d], yo)) break;  // sub-step d fails: split it
+      for (int i = 0; i < 6; ++i) s.y[i] = yo[i];
+      while (d > 0 && second[d]) --d;  // a completed second half completes its parent
+      if (d == 0) {
+        done = true;
+        break;
+      }
+      // sub-step d was a first half: its sibling starts at parent start + parent length / 2
+      sx[d] = sx[d - 1] + sh[d - 1] / 2.;
+      second[d] = true;
+      derivs<M>(c, f0, sx[d], s.y, s.d);
+    }
+    if (done) break;
+  }
+  s.err = c.err;
+  return s;
+}
+
+// RK4 step with the reference's halving; returns false only when the halving
+// depth limit was reached.
+template <int M>
+__device__ __forceinline__ bool rk4(RkCtx& c, const UndParams& f0, double* y, double* dydx, double t, double h) {
+  double yo[6];
+  if (rk4_try<M>(c, f0, y, dydx, t, h, yo)) {
+    ++c.halved;
+    RkPair s;
+    for (int i = 0; i < 6; ++i) {
+      s.y[i] = y[i];
+      s.d[i] = dydx[i];
+    }
+    s = rk4_halving<M>(c, f0, s, t, h);
+    for (int i = 0; i < 6; ++i) {
+      y[i] = s.y[i];
+      dydx[i] = s.d[i];
+    }
+    c.err = s.err;
+    return s.ok != 0;
+  }
   for (int i = 0; i < 6; ++i) y[i] = yo[i];
   return true;
 }

@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const GridZ
   // step's sums) it should not wait behind their waves on its SIMD.
   __builtin_amdgcn_s_setprio(3);
   int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  RkCtx c{F, nfields, 0.0, 0, g, nefields, 0.0};
+  RkCtx c{F, nfields, 0.0, 0, g, nefields, 0.0, 0};
   if (M == kGridZ) {
     // stage the map in LDS (all lanes), then integrate
     for (long i = threadIdx.x; i < 3 * g.n; i += blockDim.x) lds_grid[i] = g.data[i];
@@ -77,26 +77,27 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const GridZ
   x[2] = D.x0[1]; x[3] = D.b0[1] * kC;
   x[4] = D.x0[2]; x[5] = D.b0[2] * kC;
   int status = 0;
-  derivs<M>(c, f0, x, dxdt);
+  derivs<M>(c, f0, D.t0_over_c, x, dxdt);
   for (long i = 0; i < D.nf; ++i) {
     double tt = D.t0_over_c + D.dt * (double) i;
-    if (i == 0) derivs<M>(c, f0, x, dxdt);
+    if (i == 0) derivs<M>(c, f0, tt, x, dxdt);
     store_knot(tk, pk, D.nb + i, tt, x, dxdt);
-    derivs<M>(c, f0, x, dxdt);
-    if (!rk4<M>(c, f0, x, dxdt, D.dt)) { status = 1; break; }
+    derivs<M>(c, f0, tt, x, dxdt);
+    if (!rk4<M>(c, f0, x, dxdt, tt, D.dt)) { status = 1; break; }
   }
   x[0] = D.x0[0]; x[1] = D.b0[0] * kC;
   x[2] = D.x0[1]; x[3] = D.b0[1] * kC;
   x[4] = D.x0[2]; x[5] = D.b0[2] * kC;
   double dtr = -D.dt;
   for (long i = 0; i < D.nb && status == 0; ++i) {
+    double t1 = D.t0_over_c + dtr * (double) (i);
     double t2 = D.t0_over_c + dtr * (double) (i + 1);
-    derivs<M>(c, f0, x, dxdt);
-    if (!rk4<M>(c, f0, x, dxdt, dtr)) { status = 1; break; }
+    derivs<M>(c, f0, t1, x, dxdt);
+    if (!rk4<M>(c, f0, x, dxdt, t1, dtr)) { status = 1; break; }
     store_knot(tk, pk, D.nb - 1 - i, t2, x, dxdt);
   }
-  if (c.err) status = 1;
   desc[ip].status = status;
+  desc[ip].halved = c.halved;
 }
 
 // ---------------------------------------------------------------- spline
@@ -320,12 +321,12 @@ __global__ void __launch_bounds__(256) efield_time_kernel(const double* __restri
 // Sum of the fields at n points (x, y, z rows): one point per lane.
 
 __global__ void __launch_bounds__(256) field_points_kernel(const FieldDev* __restrict__ F, int nf,
-                                                           const double* __restrict__ xyz, long n,
+                                                           const double* __restrict__ xyz, long n, double t,
                                                            double* __restrict__ out) {
   long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double B[3];
-  get_b(F, nf, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], B);
+  get_b(F, nf, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], t, B);
   out[3 * i + 0] = B[0];
   out[3 * i + 1] = B[1];
   out[3 * i + 2] = B[2];
@@ -393,7 +394,8 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
   const om::FieldDev& f = *F0_host;
   // one z grid without rotation whose samples fit in LDS: the staged-grid variant
   const size_t grid_bytes = (size_t) 3 * f.n[2] * sizeof(double);
-  if (nfields == 1 && f.type == om::kGrid && f.axis == 2 && f.rot_sin[0] == 0 && f.rot_sin[1] == 0 &&
+  // the specialised variants hold time-independent fields only
+  if (nfields == 1 && !f.td && f.type == om::kGrid && f.axis == 2 && f.rot_sin[0] == 0 && f.rot_sin[1] == 0 &&
       f.rot_sin[2] == 0 && f.rot_cos[0] == 1 && f.rot_cos[1] == 1 && f.rot_cos[2] == 1 &&
       grid_bytes <= 160 * 1024) {
     g.data = f.data;
@@ -410,7 +412,7 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
                        np, t, p, 0);
     return hipGetLastError();
   }
-  if (nfields == 1 && (f.type == om::kUndulator || f.type == om::kEPU)) {
+  if (nfields == 1 && !f.td && (f.type == om::kUndulator || f.type == om::kEPU)) {
     u.c0 = f.center[0]; u.c1 = f.center[1]; u.c2 = f.center[2];
     u.u0 = f.unit[0]; u.u1 = f.unit[1]; u.u2 = f.unit[2];
     u.taper = f.taper;
@@ -490,10 +492,10 @@ extern "C" hipError_t om_launch_efield_time(const double* p, long n, const doubl
   return hipGetLastError();
 }
 
-extern "C" hipError_t om_launch_field_points(const om::FieldDev* F, int nf, const double* xyz, long n, double* out,
-                                             hipStream_t s) {
+extern "C" hipError_t om_launch_field_points(const om::FieldDev* F, int nf, const double* xyz, long n, double t,
+                                             double* out, hipStream_t s) {
   unsigned blocks = (unsigned) ((n + 255) / 256);
-  hipLaunchKernelGGL(om::field_points_kernel, dim3(blocks), dim3(256), 0, s, F, nf, xyz, n, out);
+  hipLaunchKernelGGL(om::field_points_kernel, dim3(blocks), dim3(256), 0, s, F, nf, xyz, n, t, out);
   return hipGetLastError();
 }
 

@@ -134,52 +134,67 @@ class sr:
         self._nthreads = int(nthreads)
 
     # ------------------------------------------------------------ fields
-    def add_bfield_undulator(self, bfield, period, nperiods, phase=0.0, rotations=None, translation=None, taper=0.0,
-                             frequency=0.0, frequency_phase=0.0, time_offset=0.0, name=""):
-        if frequency != 0:
-            raise ValueError("time-dependent fields are not supported by this implementation")
-        rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
-        tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
-        f = _lib.om_field()
-        f.type = _lib.OM_FIELD_UNDULATOR
-        f.field[:] = G.rot_xyz(_v3(bfield, "bfield"), rot)
-        f.period[:] = G.rot_xyz(_v3(period, "period"), rot)
-        f.nperiods = int(nperiods)
-        f.translation[:] = tr
-        f.phase = float(phase)
-        f.taper = float(taper)
-        self._push_b(_named(f, name))
-        self._traj_valid = False
-
-    def add_bfield_epu(self, bfieldA, bfieldB, period, nperiods, phase=0.0, rotations=None, translation=None,
-                       taper=0.0, frequency=0.0, frequency_phase=0.0, time_offset=0.0, name=""):
-        if frequency != 0:
-            raise ValueError("time-dependent fields are not supported by this implementation")
-        rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
-        tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
-        f = _lib.om_field()
-        f.type = _lib.OM_FIELD_EPU
-        f.field[:] = G.rot_xyz(_v3(bfieldA, "bfieldA"), rot)
-        f.fieldB[:] = G.rot_xyz(_v3(bfieldB, "bfieldB"), rot)
-        f.period[:] = G.rot_xyz(_v3(period, "period"), rot)
-        f.nperiods = int(nperiods)
-        f.translation[:] = tr
-        f.phase = float(phase)
-        f.taper = float(taper)
-        self._push_b(_named(f, name))
-        self._traj_valid = False
-
-    def _simple_field(self, ftype, name, frequency, rotations, translation):
-        if frequency != 0:
-            raise ValueError("time-dependent fields (frequency != 0) are not supported by this implementation")
+    @staticmethod
+    def _check_name(name):
         if len(name) > 0 and name[0] == "_":
             raise ValueError("'name' cannot begin with '_'.  This is reserved for internal use.  Please pick a "
                              "different name")
+
+    @staticmethod
+    def _set_time(f, frequency, frequency_phase, time_offset):
+        """The factor cos(2 pi frequency (t + time_offset) + frequency_phase) of
+        every model (src/TField3D_IdealUndulator.cc:147-151, TField3D_IdealEPU.cc:165-169,
+        TField3D_Gaussian.cc:107-111, TField3D_UniformBox.cc:115-119,
+        TField3D_Quadrupole.cc:78-82, TField3D_Grid.cc:277)."""
+        f.frequency = float(frequency)
+        f.frequency_phase = float(frequency_phase)
+        f.time_offset = float(time_offset)
+        return f
+
+    def _undulator(self, ftype, fieldA, fieldB, period, nperiods, phase, rotations, translation, taper, frequency,
+                   frequency_phase, time_offset, name):
+        self._check_name(name)
+        rot = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
+        tr = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
+        f = _lib.om_field()
+        f.type = ftype
+        f.field[:] = G.rot_xyz(fieldA, rot)
+        f.fieldB[:] = G.rot_xyz(fieldB, rot)
+        f.period[:] = G.rot_xyz(_v3(period, "period"), rot)
+        f.nperiods = int(nperiods)
+        f.translation[:] = tr
+        f.phase = float(phase)
+        f.taper = float(taper)
+        return _named(self._set_time(f, frequency, frequency_phase, time_offset), name)
+
+    def add_bfield_undulator(self, bfield, period, nperiods, phase=0.0, rotations=None, translation=None, taper=0.0,
+                             frequency=0.0, frequency_phase=0.0, time_offset=0.0, name=""):
+        """Ideal undulator (src/OSCARSSR_Python.cc:1309-1478, src/TField3D_IdealUndulator.cc:63-152)."""
+        self._push_b(self._undulator(_lib.OM_FIELD_UNDULATOR, _v3(bfield, "bfield"), [0.0, 0.0, 0.0], period,
+                                     nperiods, phase, rotations, translation, taper, frequency, frequency_phase,
+                                     time_offset, name))
+        self._traj_valid = False
+
+    def add_bfield_epu(self, period, nperiods, bfieldA=None, bfieldB=None, phase=0.0, rotations=None,
+                       translation=None, taper=0.0, frequency=0.0, frequency_phase=0.0, time_offset=0.0, name=""):
+        """Ideal elliptically polarising undulator: two undulator fields with
+        phase shifts of +-phase/2 (src/OSCARSSR_Python.cc:1479-1617,
+        src/TField3D_IdealEPU.cc:64-170)."""
+        if not bfieldA and not bfieldB:
+            raise ValueError("Must define at least one of 'bfieldA', 'bfieldB'")
+        fa = _v3(bfieldA, "bfieldA") if bfieldA else [0.0, 0.0, 0.0]
+        fb = _v3(bfieldB, "bfieldB") if bfieldB else [0.0, 0.0, 0.0]
+        self._push_b(self._undulator(_lib.OM_FIELD_EPU, fa, fb, period, nperiods, phase, rotations, translation,
+                                     taper, frequency, frequency_phase, time_offset, name))
+        self._traj_valid = False
+
+    def _simple_field(self, ftype, name, frequency, frequency_phase, time_offset, rotations, translation):
+        self._check_name(name)
         f = _lib.om_field()
         f.type = ftype
         f.rotations[:] = _v3(rotations, "rotations") if rotations else [0.0, 0.0, 0.0]
         f.translation[:] = _v3(translation, "translation") if translation else [0.0, 0.0, 0.0]
-        return f
+        return self._set_time(f, frequency, frequency_phase, time_offset)
 
     def add_bfield_gaussian(self, bfield, sigma, rotations=None, translation=None, frequency=0, frequency_phase=0,
                             time_offset=0, name=""):
@@ -187,7 +202,8 @@ class sr:
         translation; sigma_k <= 0 leaves axis k flat (src/OSCARSSR_Python.cc,
         add_bfield_gaussian; src/TField3D_Gaussian.cc:27-112)."""
         B, S = _v3(bfield, "bfield"), _v3(sigma, "sigma")
-        f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, rotations, translation)
+        f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, frequency_phase, time_offset, rotations,
+                               translation)
         f.field[:] = B
         f.shape[:] = S
         self._push_b(_named(f, name))
@@ -200,7 +216,8 @@ class sr:
         (src/TField3D_UniformBox.cc:45-120)."""
         B = _v3(bfield, "bfield")
         W = _v3(width, "width") if width is not None and len(width) != 0 else [0.0, 0.0, 0.0]
-        f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, rotations, translation)
+        f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, frequency_phase, time_offset, rotations,
+                               translation)
         f.field[:] = B
         f.shape[:] = W
         self._push_b(_named(f, name))
@@ -210,11 +227,22 @@ class sr:
                               time_offset=0, name=""):
         """Quadrupole (K y, K x, 0) [T] within |z| <= width / 2 of its frame
         (src/TField3D_Quadrupole.cc:20-83)."""
-        f = self._simple_field(_lib.OM_FIELD_QUADRUPOLE, name, frequency, rotations, translation)
+        f = self._simple_field(_lib.OM_FIELD_QUADRUPOLE, name, frequency, frequency_phase, time_offset, rotations,
+                               translation)
         f.K = float(K)
         f.width = float(width)
         self._push_b(_named(f, name))
         self._traj_valid = False
+
+    def add_bfield_halbach(self, bfield, period, nperiods, gap=0.0, magnet_height=0.0, magnet_width=0.0,
+                           npoints_per_period=4, name=""):
+        """The reference's Halbach model returns a zero field everywhere
+        (src/TField3D_Halbach.cc:61-68) and its binding is not registered in
+        the module's method table (src/OSCARSSR_Python.cc:9385, 9518); a zero
+        contributes nothing to the field sum, so it is accepted and not sent to
+        the device."""
+        self._check_name(name)
+        _v3(bfield, "bfield") if not isinstance(bfield, (int, float)) else None
 
     # ------------------------------------------------------------ electric fields
     def _add_efield(self, f):
@@ -226,7 +254,8 @@ class sr:
                             time_offset=0, name=""):
         """Gaussian electric field [V/m] (the Gaussian model of add_bfield_gaussian)."""
         E, S = _v3(efield, "efield"), _v3(sigma, "sigma")
-        f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, rotations, translation)
+        f = self._simple_field(_lib.OM_FIELD_GAUSSIAN, name, frequency, frequency_phase, time_offset, rotations,
+                               translation)
         f.field[:] = E
         f.shape[:] = S
         self._add_efield(_named(f, name))
@@ -236,31 +265,36 @@ class sr:
         """Uniform electric field [V/m] inside a box (the model of add_bfield_uniform)."""
         E = _v3(efield, "efield")
         W = _v3(width, "width") if width is not None and len(width) != 0 else [0.0, 0.0, 0.0]
-        f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, rotations, translation)
+        f = self._simple_field(_lib.OM_FIELD_UNIFORM, name, frequency, frequency_phase, time_offset, rotations,
+                               translation)
         f.field[:] = E
         f.shape[:] = W
         self._add_efield(_named(f, name))
 
-    def add_efield_undulator(self, efield, period, nperiods, phase=0.0, translation=None, taper=0.0, frequency=0,
-                             frequency_phase=0, time_offset=0, name=""):
-        """Ideal-undulator electric field [V/m] (the model of add_bfield_undulator)."""
-        n0 = len(self._fields)
-        self.add_bfield_undulator(efield, period, nperiods, phase=phase, translation=translation, taper=taper,
-                                  name=name)
-        self._add_efield(self._fields.pop(n0))
+    def add_efield_undulator(self, efield, period, nperiods, phase=0.0, rotations=None, translation=None, taper=0.0,
+                             frequency=0, frequency_phase=0, time_offset=0, name=""):
+        """Ideal-undulator electric field [V/m] (the model of add_bfield_undulator,
+        src/OSCARSSR_Python.cc:2859-2990)."""
+        self._add_efield(self._undulator(_lib.OM_FIELD_UNDULATOR, _v3(efield, "efield"), [0.0, 0.0, 0.0], period,
+                                         nperiods, phase, rotations, translation, taper, frequency, frequency_phase,
+                                         time_offset, name))
 
-    def add_efield_file(self, ifile, iformat="OSCARS", rotations=None, translation=None, scale=None, name=""):
-        """Electric-field map [V/m] from a file, in any format add_bfield_file reads."""
+    def add_efield_file(self, ifile, iformat, rotations=None, translation=None, scale=None, frequency=0,
+                        frequency_phase=0, time_offset=0, name=""):
+        """Electric-field map [V/m] from a file, in any format add_bfield_file reads
+        (src/OSCARSSR_Python.cc:2102-2252)."""
         n0 = len(self._fields)
-        self.add_bfield_file(ifile, iformat, rotations, translation, scale, name)
+        self.add_bfield_file(ifile, iformat=iformat, rotations=rotations, translation=translation, scale=scale,
+                             frequency=frequency, frequency_phase=frequency_phase, time_offset=time_offset,
+                             name=name)
         self._add_efield(self._fields.pop(n0))
 
     def add_efield_interpolated(self, mapping, iformat, parameter, rotations=None, translation=None, scale=None,
-                                frequency=0, frequency_phase=0, time_offset=0, name="", ofile=""):
+                                frequency=0, frequency_phase=0, time_offset=0, name=""):
         """Electric-field map interpolated between maps, as add_bfield_interpolated."""
         n0 = len(self._fields)
         self.add_bfield_interpolated(mapping, iformat, parameter, rotations, translation, scale, frequency,
-                                     frequency_phase, time_offset, name, ofile)
+                                     frequency_phase, time_offset, name)
         self._add_efield(self._fields.pop(n0))
 
     def _push_b(self, f):
@@ -332,19 +366,36 @@ class sr:
         self._push_b(f)
         self._traj_valid = False
 
-    def add_bfield_file(self, ifile, iformat="OSCARS", rotations=None, translation=None, scale=None, name=""):
-        """Field map from a file: 'OSCARS' (regular 1-, 2- or 3-D grid; scale =
-        [x, y, z step factors, Bx, By, Bz factors]) or 'OSCARS1D <columns>'
-        (an irregular 1-D table resampled at 10000 points per metre)
-        (src/TField3D_Grid.cc:353-873)."""
+    def add_bfield_file(self, ifile="", bifile="", iformat="", rotations=None, translation=None, scale=None,
+                        frequency=0.0, frequency_phase=0.0, time_offset=0.0, name=""):
+        """Field map from a file (src/OSCARSSR_Python.cc:526-640,
+        src/TField3D_Grid.cc:36-75, 353-1694): 'OSCARS' (regular 1-, 2- or 3-D
+        grid; scale = [x, y, z step factors, Bx, By, Bz factors]),
+        'OSCARS1D <columns>' (an irregular 1-D table resampled at 10000 points
+        per metre), 'SRW', 'SPECTRA', or a binary OSCARS grid given as bifile."""
+        if len(ifile) != 0 and len(bifile) != 0:
+            raise ValueError("cannot specify both 'ifile' and 'bifile'")
+        if len(ifile) != 0 and len(iformat) == 0:
+            raise ValueError("'iformat' is blank")
+        rot = _v3(rotations, "rotations") if rotations else None
+        tr = _v3(translation, "translation") if translation else None
+        sc = [float(v) for v in scale] if scale else []
+        self._check_name(name)
         n0 = len(self._fields)
-        self._read_bfield_file(ifile, iformat, rotations, translation, scale)
+        try:
+            if len(bifile) != 0:
+                start, stop, step, npts, data = read_binary_field_file(bifile, sc)
+                self._add_grid_exact(start, stop, step, npts, data, rot, tr)
+            else:
+                self._read_bfield_file(ifile, iformat, rot, tr, sc)
+        except (OSError, ValueError, IndexError) as ex:
+            raise ValueError("Could not import magnetic field.  Check 'ifile' and 'iformat' are correct (%s)" % ex)
         if len(self._fields) > n0:
+            self._set_time(self._fields[-1], frequency, frequency_phase, time_offset)
             _named(self._fields[-1], name)
 
-    def _read_bfield_file(self, ifile, iformat, rotations, translation, scale):
+    def _read_bfield_file(self, ifile, iformat, rotations, translation, sc):
         fmt = iformat.upper()
-        sc = [float(v) for v in scale] if scale else []
         if fmt == "OSCARS":
             start, step, npts, data = read_oscars_field_file(ifile)
             if not sc:
@@ -361,7 +412,8 @@ class sr:
             start, stop, step, npts, data = (read_srw_field_file if fmt == "SRW" else read_spectra_field_file)(ifile)
             self._add_grid_exact(start, stop, step, npts, data, rotations, translation)
         elif fmt == "BINARY":
-            raise ValueError("the BINARY field-file format is not supported by this implementation")
+            start, stop, step, npts, data = read_binary_field_file(ifile, sc)
+            self._add_grid_exact(start, stop, step, npts, data, rotations, translation)
         else:
             raise ValueError("incorrect format given")
 
@@ -375,11 +427,7 @@ class sr:
             raise ValueError("'iformat' is blank")
         if iformat.upper() != "OSCARS":
             raise ValueError("interpolation of %s maps is not supported by this implementation" % iformat)
-        if frequency != 0:
-            raise ValueError("time-dependent fields (frequency != 0) are not supported by this implementation")
-        if len(name) > 0 and name[0] == "_":
-            raise ValueError("'name' cannot begin with '_'.  This is reserved for internal use.  Please pick a "
-                             "different name")
+        self._check_name(name)
         pairs = []
         for m in mapping:
             if len(m) != 2:
@@ -402,6 +450,7 @@ class sr:
         data = spline_at([a for a, _ in pairs], np.stack(samples), float(parameter))
         g = field_grid_scaled(start, step, npts, data, sc[:3])
         self._add_grid_exact(g[0], g[1], g[2], npts, g[3], rotations, translation)
+        self._set_time(self._fields[-1], frequency, frequency_phase, time_offset)
         _named(self._fields[-1], name)
 
     def clear_bfields(self):
@@ -1129,23 +1178,22 @@ class sr:
         return {"POWER": pw, "PRECISION": pr, "LEVEL": float(lv)}[Q]
 
     # ------------------------------------------------------------ field values
-    def _field_at(self, fields, x, name):
+    def _field_at(self, fields, x, t, name):
         try:
             X = _v3(x, "x")
         except ValueError:
             raise ValueError("Incorrect format in input")
         sel = [f for f in fields if name == "" or getattr(f, "name", "") == name]
-        return self._device().field_points(sel, [X])[0].tolist()
+        return self._device().field_points(sel, [X], float(t))[0].tolist()
 
     def get_bfield(self, x, t=0.0, name=""):
-        """Summed magnetic field [T] at x, or of the fields named `name` only
-        (src/OSCARSSR_Python.cc:1927-1975, src/TFieldContainer.cc:85-107).
-        The fields here do not depend on time, so t has no effect."""
-        return self._field_at(self._fields, x, name)
+        """Summed magnetic field [T] at x and time t [s], or of the fields named
+        `name` only (src/OSCARSSR_Python.cc:1927-1975, src/TFieldContainer.cc:85-107)."""
+        return self._field_at(self._fields, x, t, name)
 
     def get_efield(self, x, t=0.0, name=""):
-        """Summed electric field [V/m] at x, as get_bfield."""
-        return self._field_at(self._efields, x, name)
+        """Summed electric field [V/m] at x and time t [s], as get_bfield."""
+        return self._field_at(self._efields, x, t, name)
 
     # ------------------------------------------------------------ accumulated results
     @staticmethod
@@ -1414,6 +1462,67 @@ def read_spectra_field_file(path):
     if data.shape != (total, 3):
         raise ValueError("file stream failure")
     return start, _grid_limits(start, step, n), step, n, data
+
+
+def read_binary_field_file(path, scaling=()):
+    """A binary OSCARS grid (src/TField3D_Grid.cc:875-1085, format version 1,
+    format string 'OSCARS'): int32 comment length and comment, int32 version,
+    int32 format length and format; float32 start and step and int32 count per
+    axis (x, y, z); then float32 Bx By Bz per point, z fastest, then y, then x.
+    The start is re-centred about the middle as the reader computes it
+    (XStartIN + XStepIN * (NX - 1) / 2. with the product in float32), field
+    scale factors multiply the float32 values.  Returns start, stop, step,
+    npoints and the samples [n, 3]."""
+    with open(path, "rb") as f:
+        buf = f.read()
+    pos = 0
+
+    def i32():
+        nonlocal pos
+        if pos + 4 > len(buf):
+            raise ValueError("error reading file.  Check format")
+        v = int(np.frombuffer(buf, dtype="<i4", count=1, offset=pos)[0])
+        pos += 4
+        return v
+
+    def f32():
+        nonlocal pos
+        if pos + 4 > len(buf):
+            raise ValueError("error reading file.  Check format")
+        v = np.frombuffer(buf, dtype="<f4", count=1, offset=pos)[0]
+        pos += 4
+        return v
+
+    ncomment = i32()
+    pos += max(ncomment, 0)
+    version = i32()
+    nfmt = i32()
+    fmt = buf[pos:pos + nfmt].decode("latin-1")
+    pos += nfmt
+    if version != 1:
+        raise ValueError("File Version number incorrect")
+    if fmt != "OSCARS":
+        raise ValueError("binary field files of format '%s' are not supported by this implementation" % fmt)
+    hdr = [(f32(), f32(), i32()) for _ in range(3)]
+    npts = [h[2] for h in hdr]
+    if min(npts) < 1:
+        raise ValueError("invalid number of points in at least one dimension")
+    start, stop, step = [], [], []
+    for a, (s_in, d_in, n) in enumerate(hdr):
+        d = float(d_in) * scaling[a] if len(scaling) > a else float(d_in)
+        middle = float(s_in) + float(np.float32(d_in * np.float32(n - 1))) / 2.
+        s0 = middle - d * (n - 1) / 2.
+        start.append(s0)
+        stop.append(s0 + (n - 1) * d)
+        step.append(d)
+    n = npts[0] * npts[1] * npts[2]
+    if pos + 12 * n > len(buf):
+        raise ValueError("error reading file.  Check format")
+    raw = np.frombuffer(buf, dtype="<f4", count=3 * n, offset=pos).reshape(n, 3).copy()
+    for k in range(3):
+        if len(scaling) > 3 + k and scaling[3 + k] != 1:
+            raw[:, k] = (raw[:, k].astype(np.float64) * scaling[3 + k]).astype(np.float32)
+    return start, stop, step, npts, raw.astype(np.float64)
 
 
 def read_oscars_field_file(path):

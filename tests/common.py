@@ -146,3 +146,83 @@ def with_efields(s, npoints=4001, magnetic=True):
     s.add_efield_undulator(efield=[0, 3e6, 0], period=[0, 0, 0.049], nperiods=31)
     s.set_ctstartstop(0, 2)
     s.set_npoints_trajectory(npoints)
+
+
+def _with_td(s, frequency, frequency_phase, time_offset, add, **kw):
+    """add(**kw) with a time-dependent factor cos(2 pi f (t + t0) + phase): the
+    product takes it as keyword arguments, the oracle and the reference build
+    as a setting for the fields added next."""
+    if hasattr(s, "set_time_dependence"):
+        s.set_time_dependence(frequency, frequency_phase, time_offset)
+        add(**kw)
+        s.set_time_dependence(0.0, 0.0, 0.0)
+    else:
+        add(frequency=frequency, frequency_phase=frequency_phase, time_offset=time_offset, **kw)
+
+
+def time_dependent(s, npoints=4001, electric=True):
+    """Time-dependent fields over a pass of 6.7 ns (ct 0..2 m): the planar
+    undulator at 150 MHz (one cycle over the pass, phase 0.3, offset 1 ns), a
+    Gaussian kick at 80 MHz, a uniform box at 40 MHz and a quadrupole at
+    200 MHz; with an electric field (uniform box at 120 MHz) when electric."""
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+    _with_td(s, 1.5e8, 0.3, 1e-9, s.add_bfield_undulator, **UNDULATOR)
+    _with_td(s, 8e7, -1.1, 0.0, s.add_bfield_gaussian, bfield=[0.02, 0.3, 0], sigma=[0, 0, 0.08],
+             translation=[0, 0, -0.7])
+    _with_td(s, 4e7, 0.0, 2e-9, s.add_bfield_uniform, bfield=[0.05, 0, 0.01], width=[0, 0, 0.2],
+             translation=[0, 0, 0.75])
+    _with_td(s, 2e8, 0.5, 0.0, s.add_bfield_quadrupole, K=1.5, width=0.2, translation=[0.0005, 0, 0.9])
+    if electric:
+        _with_td(s, 1.2e8, 0.2, 0.0, s.add_efield_uniform, efield=[2e5, -1e5, 0], width=[0, 0, 0.5],
+                 translation=[0, 0, 0.1])
+    s.set_ctstartstop(0, 2)
+    s.set_npoints_trajectory(npoints)
+
+
+def halving_fields(s, backward=False):
+    """Steps that meet |beta| >= 1 and are halved (src/OSCARSSR.cc:1362-1378):
+    a 40 T Gaussian bump (sigma 0.3 m) crossed with 1 m steps, where RK4's
+    rotation per step (omega_c dt ~ 4 rad) grows |v| past c, and a 2 T uniform
+    box around it; 3 GeV electrons.  With backward, T0 sits inside the window."""
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -0.5 if backward else -4], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_gaussian(bfield=[0, 40.0, 0], sigma=[0, 0, 0.3], translation=[0, 0, 0.0])
+    s.add_bfield_uniform(bfield=[0.3, 2.0, 0], width=[0, 0, 3.0], translation=[0, 0, 0.2])
+    if backward:
+        s.set_ctstartstop(-3.5, 4.5)
+        s.set_npoints_trajectory(9)
+    else:
+        s.set_ctstartstop(0, 8)
+        s.set_npoints_trajectory(9)
+
+
+def write_binary_grid(path, start, step, n, func, comment=b"", fmt=b"OSCARS", version=1):
+    """A binary OSCARS field map (the layout src/TField3D_Grid.cc:875-1085
+    reads): comment and format strings with int32 lengths, int32 version,
+    float32 start / step and int32 count per axis, float32 Bx By Bz per point
+    (z fastest, then y, then x)."""
+    import numpy as np
+    with open(path, "wb") as f:
+        f.write(np.array([len(comment)], dtype="<i4").tobytes() + comment)
+        f.write(np.array([version, len(fmt)], dtype="<i4").tobytes() + fmt)
+        for a in range(3):
+            f.write(np.array([start[a], step[a]], dtype="<f4").tobytes() + np.array([n[a]], dtype="<i4").tobytes())
+        vals = []
+        for ix in range(n[0]):
+            for iy in range(n[1]):
+                for iz in range(n[2]):
+                    x = [float(np.float32(start[0])) + ix * float(np.float32(step[0])),
+                         float(np.float32(start[1])) + iy * float(np.float32(step[1])),
+                         float(np.float32(start[2])) + iz * float(np.float32(step[2]))]
+                    vals.append(func(*x))
+        f.write(np.array(vals, dtype="<f4").tobytes())
+
+
+def halving_box(s, backward=False):
+    """Step halving without transcendental functions: a uniform 10 T box
+    crossed with 5 m steps.  Every RK4 stage then leaves |v| above c unless the
+    rotation per sub-step is below ~3e-4 rad, so each step splits down to
+    depth 14, as in the reference."""
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -2 if backward else -15], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_uniform(bfield=[0.5, 10.0, 0], width=[0, 0, 20.0], translation=[0, 0, 0])
+    s.set_ctstartstop(-20, 20) if backward else s.set_ctstartstop(0, 40)
+    s.set_npoints_trajectory(9)

@@ -114,7 +114,7 @@ def test_interpolated_maps(tmp_path, parameter):
     s.add_bfield_interpolated(mapping, "OSCARS", parameter, scale=sc)
     f = s._fields[-1]
     n = [int(v) for v in f.grid_n]
-    o.add_bfield_grid_exact(list(f.grid_start), list(f.grid_stop), list(f.grid_step), n, s._grids[-1])
+    o.add_bfield_grid_exact(list(f.grid_start), list(f.grid_stop), list(f.grid_step), n, f._keep)
     r.add_bfield_interpolated(mapping, "OSCARS", parameter, scale=sc)
     _finish(o, r, POINTS)
 
@@ -162,6 +162,84 @@ def test_efield_file_is_an_electric_field(tmp_path):
     write_oscars_grid(path, [0, 0, -0.5], [0, 0, 0.0025], [1, 1, 401], lambda x, y, z: [1e5 * z, 0, 0])
     s = SR.sr()
     s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1])
-    s.add_efield_file(path)
+    s.add_efield_file(path, "OSCARS")
     assert len(s._efields) == 1 and len(s._fields) == 0
     assert int(s._efields[0].grid_n[2]) == 401
+
+
+def test_binary_grid(tmp_path):
+    """A binary OSCARS map (bifile=, or iformat='BINARY'): float32 header and
+    samples, the start re-centred about the middle with the product taken in
+    float32 (src/TField3D_Grid.cc:875-1085).  The reference build cannot read
+    such a file: it frees the format string before comparing it
+    (src/TField3D_Grid.cc:903-911), and under glibc the freed bytes no longer
+    spell 'OSCARS'.  So the product's reading is pinned through a text map of
+    the same float32 values on a grid whose float32 products are exact: fed to
+    the oracle it must equal the reference's reading of the text map bit for
+    bit (fields, trajectory, spectrum)."""
+    from tests.common import write_binary_grid
+    start, step, n = [-2.0 ** -8, -2.0 ** -9, -0.5], [2.0 ** -8, 2.0 ** -8, 2.0 ** -9], [3, 2, 513]
+
+    def field(x, y, z):
+        return [float(np.float32(v)) for v in undulator_like(x, y, z)]
+
+    bpath, tpath = str(tmp_path / "g.bin"), str(tmp_path / "g.txt")
+    write_binary_grid(bpath, start, step, n, field)
+    write_oscars_grid(tpath, start, step, n, field)
+    o, r = _pair()
+    st, stop, sp, npts, data = SR.read_binary_field_file(bpath)
+    kw = dict(rotations=[0.001, -0.002, 0.01], translation=[0.0001, 0, 0.002])
+    o.add_bfield_grid_exact(st, stop, sp, npts, data, **kw)
+    r.add_bfield_file_format(tpath, "OSCARS", **kw)
+    _finish(o, r, POINTS)
+    with pytest.raises(ValueError, match="This binary format is not supported"):
+        r.add_bfield_file_format(bpath, "BINARY")
+    s = SR.sr()
+    s.add_bfield_file(bifile=bpath, **kw)
+    f = s._fields[-1]
+    assert list(f.grid_start) == st and list(f.grid_stop) == stop and np.array_equal(f._keep, data)
+
+
+def test_binary_grid_scaled(tmp_path):
+    """Scale factors on a binary map: steps scaled in double about the float32
+    middle, field factors applied to the float32 samples and rounded back to
+    float32 (src/TField3D_Grid.cc:960-1035), restated with numpy."""
+    from tests.common import write_binary_grid
+    path = str(tmp_path / "g.bin")
+    start, step, n = [-0.004, -0.003, -0.5013], [0.002, 0.002, 0.0025], [3, 2, 41]
+    write_binary_grid(path, start, step, n, undulator_like)
+    sc = [1.5, 1, 0.8, 1, 1.2, 0.9]
+    st, stop, sp, npts, data = SR.read_binary_field_file(path, sc)
+    _, _, _, _, raw = SR.read_binary_field_file(path)
+    for a in range(3):
+        s_in, d_in = np.float32(start[a]), np.float32(step[a])
+        middle = float(s_in) + float(d_in * np.float32(n[a] - 1)) / 2.
+        d = float(d_in) * sc[a]
+        assert sp[a] == d and st[a] == middle - d * (n[a] - 1) / 2. and stop[a] == st[a] + (n[a] - 1) * d
+    for k in range(3):
+        want = raw[:, k] if sc[3 + k] == 1 else (raw[:, k] * sc[3 + k]).astype(np.float32).astype(np.float64)
+        assert np.array_equal(data[:, k], want)
+
+
+def test_field_file_arguments(tmp_path):
+    """add_bfield_file's argument checks and positional order
+    (src/OSCARSSR_Python.cc:549-585: ifile, bifile, iformat, ...)."""
+    path = str(tmp_path / "g.txt")
+    write_oscars_grid(path, [0, 0, -0.5], [0, 0, 0.0025], [1, 1, 401], undulator_like)
+    s = SR.sr()
+    with pytest.raises(ValueError, match="'iformat' is blank"):
+        s.add_bfield_file(path)
+    with pytest.raises(ValueError, match="cannot specify both 'ifile' and 'bifile'"):
+        s.add_bfield_file(path, path, "OSCARS")
+    with pytest.raises(ValueError, match="Could not import magnetic field"):
+        s.add_bfield_file(path, "", "NOSUCHFORMAT")
+    s.add_bfield_file(path, "", "OSCARS", None, None, None, 1e8, 0.5, 1e-9, "m")
+    f = s._fields[-1]
+    assert (f.frequency, f.frequency_phase, f.time_offset, f.name) == (1e8, 0.5, 1e-9, "m")
+    s.add_efield_file(path, "OSCARS", None, [0, 0, 0.1], None, 2e8)
+    assert s._efields[-1].frequency == 2e8 and s._efields[-1].translation[2] == 0.1
+    s.add_bfield_epu([0, 0, 0.057], 40, [0, 0.6, 0], None, 0.3)
+    f = s._fields[-1]
+    assert list(f.field) == [0, 0.6, 0] and list(f.fieldB) == [0, 0, 0] and f.phase == 0.3
+    with pytest.raises(ValueError, match="Must define at least one of 'bfieldA', 'bfieldB'"):
+        s.add_bfield_epu([0, 0, 0.057], 40)

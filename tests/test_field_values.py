@@ -102,7 +102,7 @@ def _fieldsets(tmp):
         from oscars_amd import sr as SR
         kw = dict(rotations=[0.001, -0.002, 0.01], translation=[0.0001, 0, 0.002])
         if isinstance(x, SR.sr):
-            x.add_bfield_file(path, **kw)
+            x.add_bfield_file(path, iformat="OSCARS", **kw)
         else:
             x.add_bfield_grid(*SR.read_oscars_field_file(path), **kw)
 
@@ -221,7 +221,7 @@ def test_grid_samples_outlive_clear_bfields(tmp_path):
     path = tmp_path / "e.dat"
     write_oscars_grid(str(path), [0, 0, -1], [0, 0, 0.02], [1, 1, 101], lambda x, y, z: [1e5 * z, 2e5, -3e5 * z * z])
     s = SR.sr()
-    s.add_efield_file(str(path))
+    s.add_efield_file(str(path), "OSCARS")
     s.add_bfield_undulator(**UNDULATOR)
     f = s._efields[-1]
     addr = ctypes.cast(f.grid_data, ctypes.c_void_p).value

@@ -245,3 +245,74 @@ def test_capped_power_density(quantity):
         assert np.array_equal(v, ov)
     else:
         assert rel_err(v, ov, 1e-6) < 1e-9
+
+
+@gpu
+@pytest.mark.parametrize("electric", [False, True])
+def test_time_dependent_fields(electric):
+    """Fields with frequency / frequency_phase / time_offset on the device:
+    the time factor is taken at every RK4 stage time (the oracle is pinned to
+    the reference build in tests/test_oracle_vs_ref.py::test_time_dependent_fields).
+    Field values at three times, knots (1e-12 of each channel; 1e-8 with the
+    electric field, as test_electric_fields) and a spectrum at 1e-9, levels
+    identical."""
+    from tests.common import time_dependent
+    o, s = _pair(lambda x: time_dependent(x, electric=electric))
+    for t in (0.0, 1.3e-9, 4.1e-9):
+        for x in ([0.0, 0.0, -0.7], [0.0004, 0.0, 0.02], [0.0, 0.001, 0.75], [0.001, 0.002, 0.9]):
+            ref = o.get_bfield(x, t)
+            got = np.array(s.get_bfield(x, t))
+            assert np.allclose(got, ref, rtol=1e-14, atol=1e-15 * np.max(np.abs(ref))), (x, t, got, ref)
+            refe = o.get_efield(x, t)
+            assert np.allclose(np.array(s.get_efield(x, t)), refe, rtol=1e-14, atol=1e-15 * np.max(np.abs(refe) + 1))
+    T = o.calculate_trajectory()
+    s.calculate_trajectory()
+    t, p, _ = s.get_trajectory_arrays()
+    assert np.array_equal(t, T.t)
+    ek = _knot_err(p, T.p)
+    e = np.linspace(100.0, 900.0, 16)
+    ov, olev, oconv, _ = o.spectrum([0, 0, 30], e, carry_quirk=False)
+    v = np.array([r[1] for r in s.calculate_spectrum(obs=[0, 0, 30], energy_points_eV=list(e), mode="plain")])
+    es = rel_err(v, ov, 1e-6)
+    print("time-dependent fields (electric %s): knots %.3e, spectrum %.3e" % (electric, ek, es))
+    assert ek < (1e-8 if electric else 1e-12) and es < 1e-9
+    _same_levels(s, olev, oconv)
+
+
+@gpu
+@pytest.mark.parametrize("backward", [False, True])
+def test_step_halving_uniform_box(backward):
+    """Step halving in a uniform box, where no transcendental function enters
+    the equations of motion: the device's sub-steps are the reference's
+    operations in the same order, so the knots are bit-identical to the
+    oracle's (pinned to the reference build)."""
+    from tests.common import halving_box
+    o, s = _pair(lambda x: halving_box(x, backward=backward))
+    T = o.calculate_trajectory()
+    n, depth = o.halvings()
+    s.calculate_trajectory()
+    t, p, ypp = s.get_trajectory_arrays()
+    print("halving box (backward %s): %d splits, depth %d; knots %.3e" % (backward, n, depth, _knot_err(p, T.p)))
+    assert n > 0 and np.array_equal(t, T.t) and np.array_equal(p, T.p)
+
+
+@gpu
+@pytest.mark.parametrize("backward", [False, True])
+def test_step_halving(backward):
+    """RK4 steps that meet |beta| >= 1 are halved on the device as the
+    reference halves them (src/OSCARSSR.cc:1362-1378; the oracle's restatement
+    is pinned in tests/test_oracle_vs_ref.py::test_step_halving): 8 steps of
+    1 m through a 40 T bump, thousands of sub-steps, 14 levels deep.  Knots
+    within 1e-9 of each channel's largest value: the bump turns the electron
+    through several radians, so last-bit differences of the device's exp and
+    the C library's grow along the trajectory."""
+    from tests.common import halving_fields
+    o, s = _pair(lambda x: halving_fields(x, backward=backward))
+    T = o.calculate_trajectory()
+    n, depth = o.halvings()
+    s.calculate_trajectory()
+    t, p, ypp = s.get_trajectory_arrays()
+    assert np.array_equal(t, T.t)
+    ek = _knot_err(p, T.p)
+    print("step halving (backward %s): %d splits, depth %d; knots %.3e" % (backward, n, depth, ek))
+    assert n > 0 and ek < 1e-9

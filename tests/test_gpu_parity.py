@@ -297,7 +297,7 @@ def test_multi_dimensional_grid_file(tmp_path, dims):
         x.set_particle_beam(energy_GeV=3, x0=[0.0002, -0.0001, -1], d0=[0, 0, 1], current=0.5)
     st, sp, npts, data = SR.read_oscars_field_file(path)
     o.add_bfield_grid(st, sp, npts, data, **kw)
-    s.add_bfield_file(path, **kw)
+    s.add_bfield_file(path, iformat="OSCARS", **kw)
     for x in (o, s):
         x.set_ctstartstop(0, 2)
         x.set_npoints_trajectory(4001)

@@ -302,3 +302,68 @@ def test_removed_efields_keep_the_eb_equations():
     T = o.calculate_trajectory()
     assert same(T.t, rt) and same(T.p, rp)
     assert not np.array_equal(plain, rp)  # the B-only equations differ in the last bits
+
+
+@pytest.mark.parametrize("electric", [False, True])
+def test_time_dependent_fields(electric):
+    """Fields with frequency / frequency_phase / time_offset
+    (src/TField3D_IdealUndulator.cc:147-151, TField3D_Gaussian.cc:107-111,
+    TField3D_UniformBox.cc:115-119, TField3D_Quadrupole.cc:78-82): the field
+    at points and times, the trajectory (whose derivatives see the RK4 stage
+    times) and a spectrum, bit for bit."""
+    from tests.common import time_dependent
+    o, r = pair(time_dependent, electric=electric)
+    for t in (0.0, 1.3e-9, 4.1e-9):
+        for x in ([0.0, 0.0, -0.7], [0.0004, 0.0, 0.02], [0.0, 0.001, 0.75], [0.001, 0.002, 0.9]):
+            assert same(o.get_bfield(x, t), r.get_bfield(x, t))
+            assert same(o.get_efield(x, t), r.get_efield(x, t))
+    assert not same(o.get_bfield([0, 0, 0.02], 0.0), o.get_bfield([0, 0, 0.02], 1.3e-9))
+    rt, rp, rypp = r.calculate_trajectory()
+    T = o.calculate_trajectory()
+    assert same(T.t, rt) and same(T.p, rp) and same(T.ypp, rypp)
+    e, v, lev, conv, _ = o.calculate_spectrum([0, 0, 30], npoints=12, energy_range_eV=[100, 900])
+    re, rv, _ = r.calculate_spectrum([0, 0, 30], npoints=12, energy_range_eV=[100, 900])
+    assert same(v, rv)
+
+
+def test_time_dependent_grid(tmp_path):
+    """A grid map always carries cos(2 pi f (t + t0) + phase)
+    (src/TField3D_Grid.cc:277), so frequency_phase alone scales it by
+    cos(phase); with a frequency it oscillates."""
+    from tests.common import undulator_like, write_oscars_grid
+    path = str(tmp_path / "g.txt")
+    write_oscars_grid(path, [0, 0, -0.5], [0, 0, 0.0025], [1, 1, 401], undulator_like)
+    for f, ph, t0 in ((0.0, 0.4, 0.0), (9e7, 0.4, 1e-9)):
+        from oracle.pyoracle import Oracle, RefSim
+        o, r = Oracle(nthreads=3), RefSim(nthreads=3)
+        for s in (o, r):
+            s.set_time_dependence(f, ph, t0)
+            s.add_bfield_file(path)
+            s.set_time_dependence(0.0, 0.0, 0.0)
+            s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
+            s.set_ctstartstop(0, 2)
+            s.set_npoints_trajectory(2001)
+        for t in (0.0, 2e-9):
+            for z in (-0.3, 0.01, 0.2):
+                assert same(o.get_bfield([0, 0, z], t), r.get_bfield([0, 0, z], t))
+        rt, rp, rypp = r.calculate_trajectory()
+        T = o.calculate_trajectory()
+        assert same(T.t, rt) and same(T.p, rp) and same(T.ypp, rypp)
+
+
+@pytest.mark.parametrize("backward", [False, True])
+@pytest.mark.parametrize("case", ["bump", "box"])
+def test_step_halving(backward, case):
+    """RK4 steps that meet |beta| >= 1 are halved recursively
+    (src/OSCARSSR.cc:1362-1378), the stored acceleration being the last slope
+    the halving computed: the trajectory bit for bit, and it differs from the
+    oracle's count of split steps is checked to be nonzero."""
+    from tests.common import halving_box, halving_fields
+    o, r = pair(halving_fields if case == "bump" else halving_box, backward=backward)
+    rt, rp, rypp = r.calculate_trajectory()
+    T = o.calculate_trajectory()
+    n, depth = o.halvings()
+    assert n > 0 and depth < 40
+    assert same(T.t, rt) and same(T.p, rp) and same(T.ypp, rypp)
+    beta = np.sqrt(np.sum(T.p[:, 3:6] ** 2, axis=1))
+    assert np.all(beta < 1)
