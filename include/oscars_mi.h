@@ -241,6 +241,23 @@ int om_power_density_dev(om_context* ctx, int particle, const double* obs_xyz, l
                          int directional, const om_conv* conv, double* out_value, int* out_level,
                          unsigned char* out_conv, om_stats* stats);
 
+/* om_flux_dev without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Inputs and outputs are the
+ * caller's device buffers as in om_flux_dev; the per-point state, the level
+ * sample rows (rebuilt at every call), lists and counters come from a static
+ * device array of 64 MiB per GPU (one copy per device, held until the process
+ * ends), so no hipMalloc, hipFree or pinned host allocation happens.  A
+ * request needing more scratch than 64 MiB (about 56 bytes per output point,
+ * 76 in the mixed modes, plus 160 bytes x 2^L rows for the deepest level L
+ * summed) fails with an error.  conv->accumulate must be 0.  The trajectory is
+ * the context's current one (om_trajectories before, which allocates).  Every
+ * call on a GPU shares that GPU's scratch: calls that overlap on one GPU, from
+ * two contexts or threads, race. */
+int om_flux_dev_noalloc(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* energies_eV,
+                        long n_e, const double* pol6, const om_conv* conv, double* out_value, int* out_level,
+                        unsigned char* out_conv, double* out_e, om_stats* stats);
+
 /* ---- batched multi-particle sums (one persistent launch) ----
  * Replaces the reference's multi-particle loop (src/OSCARSSR.cc:4016-4063,
  * 1778-1818 and the CUDA host pipeline src/OSCARSSR_Cuda.cu:974-1147), which

@@ -65,7 +65,8 @@ EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context
            "om_trajectory_status", "om_flux",
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_batch_profile",
-           "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev"]
+           "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev",
+           "om_flux_dev_noalloc"]
 
 _lib = None
 
@@ -116,6 +117,8 @@ def lib():
                                        C.POINTER(om_stats)]
     _vp = C.c_void_p  # device addresses
     L.om_flux_dev.argtypes = [C.c_void_p, C.c_int, _vp, C.c_long, _vp, C.c_long, _dp, C.POINTER(om_conv), _vp, _vp,
+                              _vp, _vp, C.POINTER(om_stats)]
+    L.om_flux_dev_noalloc.argtypes = [C.c_void_p, C.c_int, _vp, C.c_long, _vp, C.c_long, _dp, C.POINTER(om_conv), _vp, _vp,
                               _vp, _vp, C.POINTER(om_stats)]
     L.om_spectrum_dev.argtypes = [C.c_void_p, C.c_int, _vp, _vp, C.c_long, _dp, C.POINTER(om_conv), _vp, _vp, _vp,
                                   _vp, C.POINTER(om_stats)]
@@ -348,6 +351,22 @@ class Device:
                                      self._dev_ptr(lev_t, torch.int32, n, "levels"),
                                      self._dev_ptr(conv_t, torch.uint8, n, "converged"),
                                      self._dev_ptr(efield_t, torch.float64, 6 * n, "efield"), C.byref(st)))
+        return st.as_dict()
+
+    def flux_dev_noalloc(self, particle, obs_t, e_t, pol6, conv, out_t, lev_t=None, conv_t=None, efield_t=None):
+        """om_flux_dev_noalloc: flux_dev with the library's static device scratch,
+        allocating nothing.  Returns the stats."""
+        import torch
+        n_obs, n_e = obs_t.shape[0], e_t.numel()
+        n = n_obs * n_e
+        pol = np.ascontiguousarray(pol6, dtype=np.float64)
+        st = om_stats()
+        self._chk(self.L.om_flux_dev_noalloc(self.h, int(particle), self._dev_ptr(obs_t, torch.float64, 3 * n_obs, "obs"),
+                                             n_obs, self._dev_ptr(e_t, torch.float64, n_e, "energies"), n_e, _ptr(pol),
+                                             C.byref(conv), self._dev_ptr(out_t, torch.float64, n, "out"),
+                                             self._dev_ptr(lev_t, torch.int32, n, "levels"),
+                                             self._dev_ptr(conv_t, torch.uint8, n, "converged"),
+                                             self._dev_ptr(efield_t, torch.float64, 6 * n, "efield"), C.byref(st)))
         return st.as_dict()
 
     def spectrum_dev(self, particle, obs_t, e_t, pol6, conv, out_t, lev_t=None, conv_t=None, efield_t=None):

@@ -28,6 +28,7 @@ extern "C" {
 hipError_t om_launch_level_batch(const om::LevelJobBatch* J, hipStream_t s);
 hipError_t om_launch_batch_clear(double* out, long n, unsigned long long* counters, int nc, hipStream_t s);
 hipError_t om_launch_lw_batch(const om::BatchArgs* A, int mixed, int n_groups, hipStream_t s);
+hipError_t om_scratch_setup(int device, void** ptr, size_t* bytes);
 hipError_t om_launch_lw_batch_stages(const om::BatchArgs* A, int mixed, int n_groups, int stages, hipStream_t s);
 hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F0_host, int nfields, om::TrajDesc* desc,
                          int np, double* t, double* p, hipStream_t s, int nefields);
@@ -229,6 +230,8 @@ struct LevelCache {
 
 struct om_context {
   int device = 0;
+  void* scratch = nullptr;  // the device's static scratch (om_flux_dev_noalloc), looked up at creation
+  size_t scratch_bytes = 0;
   int split = 0;  // waves per workgroup for the LW kernels (0: chosen from the grid size)
   int batch_wpc = 0;  // persistent batch kernel: waves per CU chosen by om_batch_tune (0: default)
   hipStream_t stream = nullptr;
@@ -935,6 +938,10 @@ int om_context_create(int device, om_context** out) {
     delete c;
     return 4;
   }
+  if (om_scratch_setup(device, &c->scratch, &c->scratch_bytes) != hipSuccess) {
+    delete c;
+    return 5;
+  }
   *out = c;
   return 0;
 }
@@ -1193,6 +1200,22 @@ int om_flux_dev(om_context* ctx, int particle, const double* obs_xyz, long n_obs
                  stats, false, true);
 }
 
+int om_flux_dev_noalloc(om_context* ctx, int particle, const double* obs_xyz, long n_obs, const double* energies_eV,
+                        long n_e, const double* pol6, const om_conv* conv, double* out_value, int* out_level,
+                        unsigned char* out_conv, double* out_e, om_stats* stats) {
+  if (conv && conv->accumulate) {
+    ctx->err = "accumulate is not supported by the allocation-free entry point";
+    return 1;
+  }
+  if (!ctx->scratch) {
+    ctx->err = "no device scratch for the allocation-free entry point";
+    return 1;
+  }
+  NoAllocScope scope;
+  Arena ar{static_cast<unsigned char*>(ctx->scratch), ctx->scratch_bytes, 0};
+  return lw_sums(ctx, particle, obs_xyz, n_obs, energies_eV, n_e, pol6, conv, out_value, out_level, out_conv, out_e,
+                 stats, false, true, &ar);
+}
 
 int om_spectrum_dev(om_context* ctx, int particle, const double* obs, const double* energies_eV, long n_e,
                     const double* pol6, const om_conv* conv, double* out_value, int* out_level,

@@ -31,6 +31,8 @@
 //   pd_point_kernel   power density, one lane = one observation point
 //   lw_finalize_kernel  C0*dt scaling, polarization projection, flux / precision /
 //                     level output, optional weighted Kahan accumulation over particles
+#include <mutex>
+
 #include "lw_common.h"
 
 namespace om {
@@ -930,6 +932,36 @@ extern "C" hipError_t om_launch_omega_from_ev(const double* e, long n, double* o
 }
 
 // out[0] = sum of 2^(L+1)-1, out[1] = highest level + 1, out[2] = unconverged points.
+// ---------------------------------------------------------------- allocation-free scratch
+// Device scratch of om_flux_dev_noalloc: per-point state, level sample rows,
+// lists and counters of one call are carved from it instead of allocated.
+// Each GPU the library runs on holds its own copy for the life of the process;
+// calls that overlap on one GPU share it and race.
+namespace {
+constexpr size_t kScratchBytes = (size_t) 64 << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on `device`, which must be current: looked
+// up once per device with hipGetSymbolAddress (in host code the array's name
+// is not a device address) and kept.
+extern "C" hipError_t om_scratch_setup(int device, void** ptr, size_t* bytes) {
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**) &p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  *bytes = kScratchBytes;
+  return hipSuccess;
+}
+
 extern "C" hipError_t om_launch_level_stats(const int* level, const unsigned char* conv, long n,
                                             unsigned long long* out, hipStream_t s) {
   hipLaunchKernelGGL(om::u64_fill_kernel, dim3(1), dim3(64), 0, s, out, 3);
