@@ -268,11 +268,12 @@ int om_flux_dev_noalloc(om_context* ctx, int particle, const double* obs_xyz, lo
  * trajectory lengths.  Levels 0..opts->max_level of every particle are stored
  * as sample rows; deeper levels are interpolated in the kernel, so every point
  * follows the conv's level rule; points unconverged at its last level are
- * counted in stats->n_unconverged.  Points still active at level 18 are
- * finished one wave per point (their samples split over the wave's lanes).  MIXED mode: points whose stop decision lay
- * within the FP32 sin/cos error bound are recomputed in FP64 in the same run
- * (stats->n_rechecked counts the work units, of up to 64 energies, that held
- * such points), so stopping levels match the FP64 kernel. */
+ * counted in stats->n_unconverged.  With opts->deep_level set, points still
+ * active at that level are finished by a workgroup per point (their samples
+ * split over its waves).  MIXED mode: points whose stop decision lay within
+ * the reduced-accuracy sin/cos error bound are recomputed with full-accuracy
+ * sin/cos in the same run (stats->n_rechecked counts them), so stopping levels
+ * match the FP64 kernel. */
 typedef struct {
   long n_obs;
   const double* obs_xyz;     /* 3 per point */
@@ -282,8 +283,11 @@ typedef struct {
 
 typedef struct {
   int waves_per_cu; /* persistent grid = CUs x this; 0: the tuned value (8 before any tuning) */
-  int max_level;    /* highest level held as stored sample rows (0: up to 17 within 16 GiB);
-                       deeper levels, up to the conv's max_level rule, are interpolated */
+  int max_level;    /* highest level held as stored sample rows (0: up to 18 within 40% of the
+                       device memory); deeper levels, up to the conv's max_level rule, are
+                       interpolated */
+  int deep_level;   /* points still active at this level are finished one workgroup per point
+                       (0: none, every point finishes in its work unit) */
 } om_batch_opts;
 
 int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int* req_of, const double* weight,
@@ -295,11 +299,13 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
 /* One step of the batch launched stage by stage (no graph), with HIP events
  * between the stages, for profiling: stage_ms[5] = ms of the trajectories
  * (RK4 + spline; 0 without recompute), the level rows, the persistent kernel,
- * the deep-level kernel and the FP64 recheck kernel; counts[7] = the device
- * counters: units taken, unconverged points, evals, points flagged for the
- * FP64 recheck, points handed to the deep-level kernel, amplitude
- * evaluations and integrand evaluations inside the persistent kernel.  No
- * counterpart in the reference. */
+ * the deep-level kernel and the FP64 recheck kernel; counts[9] = the device
+ * counters: units taken, unconverged points, evals, handed-over points listed
+ * for the FP64 recheck kernel, points handed to the deep-level kernel,
+ * amplitude evaluations and integrand evaluations inside the persistent
+ * kernel, points rechecked in FP64 inside the persistent kernel, and its
+ * integrand lane slots (idle lanes included).  No counterpart in the
+ * reference. */
 int om_batch_profile(om_context* ctx, int recompute_trajectories, double* stage_ms, unsigned long long* counts,
                      om_stats* stats);
 /* Times the persistent kernel on the first max(sample_items, 16 x CUs x the

@@ -48,7 +48,7 @@ class om_request(C.Structure):
 
 
 class om_batch_opts(C.Structure):
-    _fields_ = [("waves_per_cu", C.c_int), ("max_level", C.c_int)]
+    _fields_ = [("waves_per_cu", C.c_int), ("max_level", C.c_int), ("deep_level", C.c_int)]
 
 
 class om_stats(C.Structure):
@@ -397,7 +397,7 @@ class Device:
         return st.as_dict()
 
     # ------------------------------------------------------------ batched multi-particle sums
-    def batch_setup(self, requests, req_of, weight, pol6, conv, waves_per_cu=0, max_level=0):
+    def batch_setup(self, requests, req_of, weight, pol6, conv, waves_per_cu=0, max_level=0, deep_level=0):
         """requests: list of (obs_xyz[n,3], energies[m]); particle i adds weight[i]*flux to requests[req_of[i]]."""
         self._batch_keep = []
         ra = (om_request * len(requests))()
@@ -414,7 +414,7 @@ class Device:
         ro = np.ascontiguousarray(req_of, dtype=np.int32)
         w = np.ascontiguousarray(weight, dtype=np.float64)
         pol = np.ascontiguousarray(pol6, dtype=np.float64)
-        opts = om_batch_opts(int(waves_per_cu), int(max_level))
+        opts = om_batch_opts(int(waves_per_cu), int(max_level), int(deep_level))
         self._chk(self.L.om_batch_setup(self.h, ra, len(requests), _ptr(ro, _ip), _ptr(w), _ptr(pol),
                                         C.byref(conv), C.byref(opts)))
         self._batch_shapes = shapes
@@ -431,10 +431,11 @@ class Device:
         stages: ms of (trajectories, level rows, persistent kernel, deep-level
         kernel, FP64 recheck kernel) and the device counters."""
         ms = np.zeros(5)
-        cnt = (C.c_ulonglong * 7)()
+        cnt = (C.c_ulonglong * 9)()
         st = om_stats()
         self._chk(self.L.om_batch_profile(self.h, int(bool(recompute_trajectories)), _ptr(ms), cnt, C.byref(st)))
-        names = ("units", "unconverged", "evals", "rechecked", "handed_over", "main_amplitudes", "main_evals")
+        names = ("units", "unconverged", "evals", "fix_listed", "handed_over", "main_amplitudes", "main_evals",
+                 "rechecked_in_unit", "main_slots")
         stages = ("trajectories", "levels", "main", "deep", "fix")
         return dict(zip(stages, ms.tolist())), {k: int(v) for k, v in zip(names, cnt)}, st.as_dict()
 

@@ -1462,23 +1462,22 @@ int om_power_density_dev(om_context* ctx, int ip, const double* obs_xyz, long n_
                        true);
 }
 
-// batch counters: work-unit counter, unconverged points, evaluations, borderline points listed,
-// points handed to the one-wave-per-point kernel
-// counters: [0] next work unit, [1] unconverged, [2] evals, [3] flagged for the
-// FP64 recheck, [4] handed over, [5] amplitudes and [6] evals inside the
-// persistent kernel
-constexpr int kBatchCounters = 7;
+// batch counters: [0] next work unit, [1] unconverged, [2] evals, [3] handed-over
+// points listed for the FP64 recheck kernel, [4] handed over, [5] amplitudes,
+// [6] evals and [8] integrand lane slots inside the persistent kernel, [7]
+// points rechecked in FP64 inside their unit
+constexpr int kBatchCounters = 9;
 constexpr long kBatchFixCap = 1L << 27;
 constexpr long kBatchDeepCap = 1L << 22;
-constexpr int kBatchDeepLevel = 18;          // levels from here on: one wave per point
+constexpr int kBatchNoHandOver = 99;         // default hand-over level: none (above every level)
 constexpr int kBatchRowsCap = 17;            // highest level stored as rows by default ...
-constexpr double kBatchRowsBytes = 16.0 * (1L << 30);  // ... within this much memory
+constexpr double kBatchRowsFraction = 0.4;   // ... within this fraction of the device memory
 
 namespace {
 // Per-wave scratch of the persistent batch kernel for a grid of `groups` waves.
 void batch_scratch(om_context* ctx, long groups, om::BatchArgs& A) {
   auto& B = ctx->batch;
-  B.ws.ensure((size_t) groups * om::kBatchWsD * om::kBatchEB);
+  B.ws.ensure((size_t) groups * om::kBatchWsTotal);
   B.wsi.ensure((size_t) groups * om::kBatchWsI * om::kBatchEB);
   A.ws = B.ws.p;
   A.wsi = B.wsi.p;
@@ -1508,8 +1507,12 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     int cap = (opts && opts->max_level > 0) ? opts->max_level : kBatchRowsCap;
     if (cap > 20) throw std::out_of_range("batch rows above level 20 are not supported");
     if (stop_ext < 0) throw std::out_of_range("max_level leaves no level to sum");
-    if (!(opts && opts->max_level > 0))
-      while (cap > 10 && (double) np * (double) ((2L << cap) - 1) * 80.0 > kBatchRowsBytes) --cap;
+    if (!(opts && opts->max_level > 0)) {
+      size_t free_b = 0, total_b = 0;
+      check(hipMemGetInfo(&free_b, &total_b), "memory info");
+      const double budget = kBatchRowsFraction * (double) total_b;
+      while (cap > 10 && (double) np * (double) ((2L << cap) - 1) * 80.0 > budget) --cap;
+    }
     B.Lb = stop_ext < cap ? stop_ext : cap;
     B.Lmax = stop_ext;
     B.stop_mem = stop_mem;
@@ -1602,7 +1605,10 @@ int om_batch_setup(om_context* ctx, const om_request* req, int n_req, const int*
     A.n_deep = B.counters.p + 4;
     A.main_amps = B.counters.p + 5;
     A.main_evals = B.counters.p + 6;
-    A.Ldeep = kBatchDeepLevel;
+    A.n_recheck = B.counters.p + 7;
+    A.main_slots = B.counters.p + 8;
+    A.Ldeep = (opts && opts->deep_level > 0) ? opts->deep_level : kBatchNoHandOver;
+    if (A.Ldeep < 9) throw std::out_of_range("deep_level must be at least 9");
     A.deep_cap = total_points < kBatchDeepCap ? total_points : kBatchDeepCap;
     B.deep_code.ensure(A.deep_cap);
     B.deep_state.ensure((size_t) om::kBatchWsD * A.deep_cap);
@@ -1686,7 +1692,7 @@ int om_batch_run(om_context* ctx, int recompute_trajectories, double* const* out
       stats->total_ms = wall.ms();
       stats->max_level = B.Lmax;
       stats->n_unconverged = (long) cnt[1];
-      stats->n_rechecked = (long) cnt[3];
+      stats->n_rechecked = (long) (cnt[3] + cnt[7]);
     }
   });
 }
@@ -1736,7 +1742,7 @@ int om_batch_profile(om_context* ctx, int recompute_trajectories, double* stage_
       stats->total_ms = wall.ms();
       stats->max_level = B.Lmax;
       stats->n_unconverged = (long) cnt[1];
-      stats->n_rechecked = (long) cnt[3];
+      stats->n_rechecked = (long) (cnt[3] + cnt[7]);
     }
   });
 }

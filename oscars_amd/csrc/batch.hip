@@ -68,42 +68,315 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return ((unsigned long long) hi << 32) | lo;
 }
 
-// One work unit (particle ip, observation point io, energies e0 .. e0+n_eb-1):
-// the reference's level loop for each energy, weighted flux added into the
-// request's output.  Level by level, the energies still active are packed into
-// groups of 64 lanes (so lanes do not idle once their energy has converged);
-// for each group, lane j computes the amplitude of sample base+j and leaves it
-// in LDS, then every lane walks those 64 samples at its own energy.  The
-// per-energy state lives in the wave's scratch (ws, wsi) between levels.
-// MIXED: energies whose stop decision lay within the FP32 error bound add
-// nothing here; they are listed for lw_batch_fix_kernel.
-template <bool MIXED>
-__device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, int lane, double* __restrict__ ws,
-                                           int* __restrict__ wsi, double* s_tau, double (*s_v)[64], int* s_list) {
+// LDS of one persistent wave: the amplitudes of the current super-block of
+// kSuper samples (one double of padding after every 16 samples, so replicas
+// that walk different parts of it read different banks) and the active list.
+constexpr int kSuper = 256;
+constexpr int kBlkPad = kSuper + kSuper / 16;
+__device__ __forceinline__ int blk(int k) { return k + (k >> 4); }
+
+struct BatchLds {
+  double tau[kBlkPad];
+  double vx[kBlkPad];
+  double vy[kBlkPad];
+  double vz[kBlkPad];
+  unsigned short list[kBatchEB];
+};
+
+// One work unit (particle ip, observation point io, energies e0 .. e0+n_eb-1).
+struct Unit {
+  long item;
+  int ip;
+  long io, e0;
+  int n_eb;
+  V3 obs;
+  double ts, te, T;
+  const double* prow;
+  TrajDesc D;
+  ReqDev R;
+};
+
+__device__ __forceinline__ Unit unit_of(const BatchArgs& A, long item) {
+  Unit U;
+  U.item = item;
   // particle of this work unit: last p with item_prefix[p] <= item
   int lo = 0, hi = A.np;
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
     if (A.item_prefix[mid] <= item) lo = mid; else hi = mid;
   }
-  const int ip = lo;
-  const long local = item - A.item_prefix[ip];
-  const ReqDev R = A.req[A.req_of[ip]];
-  const long io = local / R.n_blocks_e;
-  const long e0 = (local - io * R.n_blocks_e) * kBatchEB;
-  const int n_eb = (int) (R.n_e - e0 < kBatchEB ? R.n_e - e0 : kBatchEB);
-  const V3 obs = v3(R.obs[3 * io], R.obs[3 * io + 1], R.obs[3 * io + 2]);
-  const double ts = A.tspan[2 * ip], te = A.tspan[2 * ip + 1];
-  const double T = te - ts;
-  const double* prow = A.rows + 10 * (long) ip * A.stride;
-  const TrajDesc D = A.desc[ip];
+  U.ip = lo;
+  const long local = item - A.item_prefix[U.ip];
+  U.R = A.req[A.req_of[U.ip]];
+  U.io = local / U.R.n_blocks_e;
+  U.e0 = (local - U.io * U.R.n_blocks_e) * kBatchEB;
+  U.n_eb = (int) (U.R.n_e - U.e0 < kBatchEB ? U.R.n_e - U.e0 : kBatchEB);
+  U.obs = v3(U.R.obs[3 * U.io], U.R.obs[3 * U.io + 1], U.R.obs[3 * U.io + 2]);
+  U.ts = A.tspan[2 * U.ip];
+  U.te = A.tspan[2 * U.ip + 1];
+  U.T = U.te - U.ts;
+  U.prow = A.rows + 10 * (long) U.ip * A.stride;
+  U.D = A.desc[U.ip];
+  return U;
+}
+
+// Flags of an energy in the wave's scratch.
+constexpr int kFConverged = 1;
+constexpr int kFBorderline = 2;  // mixed pass: a stop decision lay within the FP32 error bound
+constexpr int kFHanded = 4;      // handed to lw_batch_deep_kernel
+constexpr int kFRecheck = 8;     // recomputed in FP64 inside the unit
+constexpr int kFIdle = 16;       // not part of the FP64 recheck pass
+
+// The reference's level loop for the energies of a unit whose flags carry none
+// of the bits in `skip`.  SC32: reduced-accuracy sin/cos (sincos_fast64) with
+// the borderline test, else full-accuracy sin/cos.  Level by level, the still
+// active energies are packed into groups: 64 energies, one per lane, and for
+// the last r < 64 of a level, m = 64 / r replicas of the r energies, replica q
+// walking the q-th contiguous part of every super-block.  The level's samples
+// go in super-blocks of kSuper: the lanes compute their amplitudes into LDS
+// once, then every group walks the super-block at its energies (a group's
+// partial sums wait in the wave's scratch while the others walk), so a
+// sample's amplitude is computed once per level whatever the number of
+// groups.  The replicas' partial sums are added in a fixed tree order.  The
+// largest phase step takes the sample before each part from LDS (or from the
+// previous super-block), so it is the same maximum over the same consecutive
+// pairs as a sequential walk.  HANDOVER: energies still active at level Ldeep
+// are handed to lw_batch_deep_kernel.
+template <bool SC32, bool HANDOVER>
+__device__ __forceinline__ void unit_levels(const BatchArgs& A, const Unit& U, int lane, double* __restrict__ ws,
+                                            int* __restrict__ wsi, BatchLds& S, int skip, unsigned long long& n_amp,
+                                            unsigned long long& n_ev, unsigned long long& n_slot) {
   double* w_sum = ws;                    // 6 x kBatchEB
   double* w_lastmag = ws + 6 * kBatchEB;
   double* w_prec = ws + 7 * kBatchEB;
   double* w_an = ws + 8 * kBatchEB;      // mixed: sum of |V|_1 over the samples so far
   double* w_dm = ws + 9 * kBatchEB;      // mixed: error bound of lastmag
+  double* w_grp = ws + kBatchWsD * kBatchEB;  // partial sums of the groups between super-blocks
   int* w_level = wsi;
-  int* w_flags = wsi + kBatchEB;         // bit 0 converged, bit 1 borderline (mixed), bit 2 handed over
+  int* w_flags = wsi + kBatchEB;
+  const int n_eb = U.n_eb;
+  for (int L = 0; L <= A.Lmax; ++L) {
+    if (HANDOVER && L == A.Ldeep) {
+      // hand the energies still active to the one-wave-per-point kernel
+      for (int t = 0; t < n_eb; t += 64) {
+        int e = t + lane;
+        int ec = e < n_eb ? e : 0;
+        bool a = e < n_eb && (w_flags[ec] & skip) == 0;
+        unsigned long long m = __ballot(a);
+        unsigned long long first = atomicAdd(A.n_deep, lane == 0 ? (unsigned long long) __popcll(m) : 0ULL);
+        unsigned long long j = uniform_u64(first) + (unsigned long long) __popcll(m & ((1ULL << lane) - 1ULL));
+        if (a && j < (unsigned long long) A.deep_cap) {
+          A.deep_code[j] = (unsigned long long) U.item * kBatchEB + (unsigned long long) e;
+#pragma unroll
+          for (int k = 0; k < kBatchWsD; ++k) A.deep_state[k * A.deep_cap + j] = ws[k * kBatchEB + e];
+          A.deep_level[j] = w_level[e];
+          A.deep_flags[j] = w_flags[e];
+          w_flags[e] |= kFHanded;
+        }
+      }
+      __syncthreads();
+    }
+    // pack the energies still active
+    int n_act = 0;
+    for (int t = 0; t < n_eb; t += 64) {
+      int e = t + lane;
+      bool a = e < n_eb && (w_flags[e < n_eb ? e : 0] & skip) == 0;
+      unsigned long long m = __ballot(a);
+      if (a) S.list[n_act + __popcll(m & ((1ULL << lane) - 1ULL))] = e;
+      n_act += __popcll(m);
+    }
+    __syncthreads();
+    if (n_act == 0) break;
+    const double* __restrict__ rows = U.prow + 10 * ((1L << L) - 1);
+    const long ns = 1L << L;
+    // sample times of level L (the expressions of level_batch_kernel)
+    const double p2L = (double) (1L << L);
+    const double spacing = L > A.stop_mem ? (U.te - U.ts) / (p2L - 1) : (U.te - U.ts) / p2L;
+    const double tfirst = U.ts + (U.te - U.ts) / (2.0 * p2L);
+    const bool stored = L <= A.Lb;
+    const double dt = U.T / (2.0 * (double) (1L << L) - 1);
+    const int n_groups = (n_act + 63) >> 6;  // 64 energies each, the last one r <= 64
+    const long n_sb = (ns + kSuper - 1) / kSuper;
+    n_amp += (unsigned long long) ns;
+    double anl = 0;   // mixed: sum of |V|_1 over the samples this lane computed
+    double tprev = 0; // tau of the last sample of the previous super-block
+    double ps[6] = {0, 0, 0, 0, 0, 0};
+    double maxd = 0;
+    for (long sb = 0; sb < n_sb; ++sb) {
+      const long base = sb * kSuper;
+      const int cnt = (int) ((ns - base) < kSuper ? (ns - base) : kSuper);
+#pragma unroll 1
+      for (int h = 0; h < kSuper; h += 64) {
+        const long si = base + h + lane;
+        if (si < ns) {
+          Amp a;
+          if (stored) {
+            a = amplitude(U.obs, rows + 10 * si);
+          } else {
+            a = interpolated_amplitude(A, U.D, U.obs, tfirst + spacing * (double) si);
+          }
+          S.tau[blk(h + lane)] = a.tau;
+          S.vx[blk(h + lane)] = a.V.x;
+          S.vy[blk(h + lane)] = a.V.y;
+          S.vz[blk(h + lane)] = a.V.z;
+          if (SC32) anl += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);
+        }
+      }
+      wave_sync();
+      const bool last_sb = sb == n_sb - 1;
+      if (SC32 && last_sb) {
+        // every sample of the level was computed by one lane: the |V|_1 sum is wave-uniform
+        for (int off = 32; off > 0; off >>= 1) anl += __shfl_xor(anl, off);
+      }
+      for (int grp = 0; grp < n_groups; ++grp) {
+        const int g = grp << 6;
+        const int W = n_act - g < 64 ? n_act - g : 64;  // energies in this group
+        const int m = 64 / W;                            // replicas of each
+        const int q = lane / W;                          // this lane's replica (q >= m: idle)
+        const int j = lane - q * W;
+        const bool act = q < m;
+        const int e = S.list[g + (act ? j : 0)];  // idle lanes shadow the group's first energy
+        const double negw = -U.R.omega[U.e0 + e];
+        if (sb == 0) {
+          n_ev += (unsigned long long) ns * (unsigned long long) W;
+          const long full = ns >> 6, part = ns & 63;
+          n_slot += 64ULL * ((unsigned long long) full * (unsigned long long) ((64 + m - 1) / m) +
+                             (unsigned long long) ((part + m - 1) / m));
+        }
+        if (n_groups > 1) {
+          // this group's partial sums from the previous super-block (lane-private)
+          double* gp = w_grp + grp * 7 * 64 + lane;
+          if (sb == 0) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ps[c] = 0;
+            maxd = 0;
+          } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ps[c] = gp[c * 64];
+            maxd = gp[6 * 64];
+          }
+        }
+        double lastphase;  // phase of the sample before the part being walked
+        auto walk = [&](int k0, int k1) {
+          int k = k0;
+          for (; k + 4 <= k1; k += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int b = blk(k + u);
+              double phi = negw * S.tau[b];
+              maxd = fmax(maxd, fabs(phi - lastphase));
+              lastphase = phi;
+              double sn, cs;
+              sincos_mode(SC32, phi, sn, cs);
+              double vx = S.vx[b], vy = S.vy[b], vz = S.vz[b];
+              ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+              ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+              ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+            }
+          }
+          for (; k < k1; ++k) {
+            const int b = blk(k);
+            double phi = negw * S.tau[b];
+            maxd = fmax(maxd, fabs(phi - lastphase));
+            lastphase = phi;
+            double sn, cs;
+            sincos_mode(SC32, phi, sn, cs);
+            double vx = S.vx[b], vy = S.vy[b], vz = S.vz[b];
+            ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
+            ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
+            ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
+          }
+        };
+        if (m == 1) {  // one energy per lane: the whole super-block, wave-uniform bounds
+          lastphase = negw * (base > 0 ? tprev : S.tau[0]);
+          walk(0, cnt);
+        } else {
+          const int k0 = act ? (q * cnt) / m : 0;
+          const int k1 = act ? ((q + 1) * cnt) / m : 0;
+          lastphase = negw * (k0 > 0 ? S.tau[blk(k0 - 1)] : (base > 0 ? tprev : S.tau[0]));
+          walk(k0, k1);
+        }
+        if (!last_sb) {
+          if (n_groups > 1) {
+            double* gp = w_grp + grp * 7 * 64 + lane;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) gp[c * 64] = ps[c];
+            gp[6 * 64] = maxd;
+          }
+          continue;
+        }
+        // the level is complete for this group
+        if (m > 1) {
+          // replicas q..q+2s-1 into replica q, for s = 1, 2, 4, ...: replica 0
+          // ends with the totals, in the same order for every group shape
+          for (int st = 1; st < m; st <<= 1) {
+            const int src = lane + st * W < 64 ? lane + st * W : lane;
+            const bool take = act && (q % (2 * st)) == 0 && q + st < m;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+              double o = __shfl(ps[c], src);
+              ps[c] = take ? ps[c] + o : ps[c];
+            }
+            double om = __shfl(maxd, src);
+            maxd = take ? fmax(maxd, om) : maxd;
+          }
+        }
+        if (q == 0) {
+          double sum[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) sum[c] = w_sum[c * kBatchEB + e] + ps[c];
+          // convergence test, src/OSCARSSR.cc:4227-4239
+          double E[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) E[c] = sum[c] * dt;
+          if (A.use_pol) pol_project(E, A.pol);
+          const double mag = cmag2(E);
+          const double mag_prev = w_lastmag[e];
+          const double prec = fabs(mag - mag_prev) / mag_prev;
+          const bool c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == mag_prev);
+          int flags = w_flags[e] | (c ? kFConverged : 0);
+          if (SC32) {
+            double an_run = w_an[e] + anl;
+            double dm_last = w_dm[e];
+            if (level_borderline(L, dt, A.precision, mag_prev, mag, prec, maxd, an_run, dm_last))
+              flags |= kFBorderline;
+            w_an[e] = an_run;
+            w_dm[e] = dm_last;
+          }
+#pragma unroll
+          for (int c = 0; c < 6; ++c) w_sum[c * kBatchEB + e] = sum[c];
+          w_lastmag[e] = mag;
+          w_prec[e] = prec;
+          w_level[e] = L;
+          w_flags[e] = flags;
+        }
+      }
+      tprev = S.tau[blk(cnt - 1)];
+      wave_sync();  // the next super-block overwrites the amplitudes
+    }
+    __syncthreads();  // this level's state is read by other lanes at the next one
+  }
+}
+
+// One work unit: the level loop for the block's energies, weighted flux added
+// into the request's output with FP64 atomics.  MIXED: a first pass with FP32
+// sin/cos; the energies whose stop decision lay within the FP32 error bound are
+// then recomputed from level 0 with FP64 sin/cos in a second pass of the same
+// unit (their amplitudes shared, the rows still in cache), so every energy
+// stops where the FP64 path stops.  The per-energy state lives in the wave's
+// scratch (ws, wsi) between levels.
+template <bool MIXED>
+__device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, int lane, double* __restrict__ ws,
+                                           int* __restrict__ wsi, BatchLds& S) {
+  const Unit U = unit_of(A, item);
+  const int n_eb = U.n_eb;
+  double* w_sum = ws;
+  double* w_lastmag = ws + 6 * kBatchEB;
+  double* w_prec = ws + 7 * kBatchEB;
+  double* w_an = ws + 8 * kBatchEB;
+  double* w_dm = ws + 9 * kBatchEB;
+  int* w_level = wsi;
+  int* w_flags = wsi + kBatchEB;
   for (int e = lane; e < n_eb; e += 64) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) w_sum[k * kBatchEB + e] = 0;
@@ -115,173 +388,59 @@ __device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, int la
     w_flags[e] = 0;
   }
   __syncthreads();  // the state written above is read by other lanes
-  unsigned long long n_amp = 0, n_ev = 0;  // wave-uniform work counts of this unit
-  for (int L = 0; L <= A.Lmax; ++L) {
-    if (L == A.Ldeep) {
-      // hand the energies still active to the one-wave-per-point kernel
-      for (int t = 0; t < n_eb; t += 64) {
-        int e = t + lane;
-        int ec = e < n_eb ? e : 0;
-        bool a = e < n_eb && (w_flags[ec] & 1) == 0;
-        unsigned long long m = __ballot(a);
-        unsigned long long first = atomicAdd(A.n_deep, lane == 0 ? (unsigned long long) __popcll(m) : 0ULL);
-        unsigned long long j = uniform_u64(first) + (unsigned long long) __popcll(m & ((1ULL << lane) - 1ULL));
-        if (a && j < (unsigned long long) A.deep_cap) {
-          A.deep_code[j] = (unsigned long long) item * kBatchEB + (unsigned long long) e;
-#pragma unroll
-          for (int k = 0; k < kBatchWsD; ++k) A.deep_state[k * A.deep_cap + j] = ws[k * kBatchEB + e];
-          A.deep_level[j] = w_level[e];
-          A.deep_flags[j] = w_flags[e];
-          w_flags[e] |= 4;
-        }
-      }
-      __syncthreads();
-    }
-    // pack the energies still active (not converged, not handed over)
-    int n_act = 0;
+  unsigned long long n_amp = 0, n_ev = 0, n_slot = 0;  // wave-uniform work counts of this unit
+  unit_levels<MIXED, true>(A, U, lane, ws, wsi, S, kFConverged | kFHanded, n_amp, n_ev, n_slot);
+  if (MIXED) {
+    // FP64 recheck: reset the borderline energies, park the others
+    int n_re = 0;
     for (int t = 0; t < n_eb; t += 64) {
-      int e = t + lane;
-      bool a = e < n_eb && (w_flags[e < n_eb ? e : 0] & 5) == 0;
-      unsigned long long m = __ballot(a);
-      if (a) s_list[n_act + __popcll(m & ((1ULL << lane) - 1ULL))] = e;
-      n_act += __popcll(m);
+      const int e = t + lane;
+      const int ec = e < n_eb ? e : 0;
+      const int flags = w_flags[ec];
+      const bool want = e < n_eb && (flags & (kFBorderline | kFHanded)) == kFBorderline;
+      n_re += __popcll(__ballot(want));
+      if (want) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w_sum[k * kBatchEB + e] = 0;
+        w_lastmag[e] = -1;
+        w_prec[e] = -1;
+        w_level[e] = -1;
+        w_flags[e] = kFRecheck;
+      } else if (e < n_eb) {
+        w_flags[e] = flags | kFIdle;
+      }
     }
     __syncthreads();
-    if (n_act == 0) break;
-    const double* __restrict__ rows = prow + 10 * ((1L << L) - 1);
-    const long ns = 1L << L;
-    // sample times of level L (the expressions of level_batch_kernel)
-    const double p2L = (double) (1L << L);
-    const double spacing = L > A.stop_mem ? (te - ts) / (p2L - 1) : (te - ts) / p2L;
-    const double tfirst = ts + (te - ts) / (2.0 * p2L);
-    const bool stored = L <= A.Lb;
-    const double dt = T / (2.0 * (double) (1L << L) - 1);
-    for (int g = 0; g < n_act; g += 64) {
-      n_amp += (unsigned long long) ns;
-      n_ev += (unsigned long long) ns * (unsigned long long) (n_act - g < 64 ? n_act - g : 64);
-      const bool valid = g + lane < n_act;
-      const int e = s_list[valid ? g + lane : g];  // spare lanes shadow the group's first energy
-      const double negw = -R.omega[e0 + e];
-      double ps[6] = {0, 0, 0, 0, 0, 0};
-      double maxd = 0, lastphase = 0;
-      double anl = 0;  // mixed path: sum of |V|_1 over the samples this lane computed
-      for (long base = 0; base < ns; base += 64) {
-        long si = base + lane;
-        if (si < ns) {
-          Amp a;
-          if (stored) {
-            a = amplitude(obs, rows + 10 * si);
-          } else {
-            a = interpolated_amplitude(A, D, obs, tfirst + spacing * (double) si);
-          }
-          s_tau[lane] = a.tau;
-          s_v[0][lane] = a.V.x;
-          s_v[1][lane] = a.V.y;
-          s_v[2][lane] = a.V.z;
-          if (MIXED) anl += fabs(a.V.x) + fabs(a.V.y) + fabs(a.V.z);
-        }
-        wave_sync();
-        int cnt = (int) ((ns - base) < 64 ? (ns - base) : 64);
-        if (base == 0) lastphase = negw * s_tau[0];
-        int k = 0;
-        for (; k + 4 <= cnt; k += 4) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            double phi = negw * s_tau[k + u];
-            maxd = fmax(maxd, fabs(phi - lastphase));
-            lastphase = phi;
-            double sn, cs;
-            sincos_mode(MIXED, phi, sn, cs);
-            double vx = s_v[0][k + u], vy = s_v[1][k + u], vz = s_v[2][k + u];
-            ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
-            ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
-            ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
-          }
-        }
-        for (; k < cnt; ++k) {
-          double phi = negw * s_tau[k];
-          maxd = fmax(maxd, fabs(phi - lastphase));
-          lastphase = phi;
-          double sn, cs;
-          sincos_mode(MIXED, phi, sn, cs);
-          double vx = s_v[0][k], vy = s_v[1][k], vz = s_v[2][k];
-          ps[0] = __builtin_fma(vx, cs, ps[0]); ps[1] = __builtin_fma(vx, sn, ps[1]);
-          ps[2] = __builtin_fma(vy, cs, ps[2]); ps[3] = __builtin_fma(vy, sn, ps[3]);
-          ps[4] = __builtin_fma(vz, cs, ps[4]); ps[5] = __builtin_fma(vz, sn, ps[5]);
-        }
-        wave_sync();
-      }
-      if (MIXED) {
-        // every lane walked the same samples: the level's |V|_1 sum is wave-uniform
-        for (int off = 32; off > 0; off >>= 1) anl += __shfl_xor(anl, off);
-      }
-      if (valid) {
-        double sum[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) sum[k] = w_sum[k * kBatchEB + e] + ps[k];
-        // convergence test, src/OSCARSSR.cc:4227-4239
-        double E[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) E[k] = sum[k] * dt;
-        if (A.use_pol) pol_project(E, A.pol);
-        const double mag = cmag2(E);
-        const double mag_prev = w_lastmag[e];
-        const double prec = fabs(mag - mag_prev) / mag_prev;
-        const bool c = (L > 8 && prec < A.precision && maxd < kPi) || (L > 8 && maxd < kPi && mag == mag_prev);
-        int flags = w_flags[e] | (c ? 1 : 0);
-        if (MIXED) {
-          double an_run = w_an[e] + anl;
-          double dm_last = w_dm[e];
-          if (level_borderline(L, dt, A.precision, mag_prev, mag, prec, maxd, an_run, dm_last)) flags |= 2;
-          w_an[e] = an_run;
-          w_dm[e] = dm_last;
-        }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) w_sum[k * kBatchEB + e] = sum[k];
-        w_lastmag[e] = mag;
-        w_prec[e] = prec;
-        w_level[e] = L;
-        w_flags[e] = flags;
-      }
+    if (n_re > 0) {
+      unit_levels<false, false>(A, U, lane, ws, wsi, S, kFConverged | kFHanded | kFIdle, n_amp, n_ev, n_slot);
     }
-    __syncthreads();  // this level's state is read by other lanes at the next one
+    atomicAdd(A.n_recheck, lane == 0 ? (unsigned long long) n_re : 0ULL);
   }
-  // outputs, fix-up list and bookkeeping for the block's energies (all lanes
-  // take part in the atomics; only lane 0 adds nonzero counts)
+  // outputs and bookkeeping for the block's energies (all lanes take part in
+  // the atomics; only lane 0 adds nonzero counts)
   for (int t = 0; t < n_eb; t += 64) {
     const int e = t + lane;
     const int ec = e < n_eb ? e : 0;
     const int lev = w_level[ec];
     const int flags = w_flags[ec];
-    const bool v = e < n_eb && (flags & 4) == 0;  // handed-over energies finish in the deep kernel
-    bool fixed = false;
-    if (MIXED) {
-      bool want = v && (flags & 2);
-      unsigned long long m = __ballot(want);
-      unsigned long long first = atomicAdd(A.n_fix, lane == 0 ? (unsigned long long) __popcll(m) : 0ULL);
-      unsigned long long j = uniform_u64(first) + (unsigned long long) __popcll(m & ((1ULL << lane) - 1ULL));
-      if (want && j < (unsigned long long) A.fix_cap) {
-        A.fix_list[j] = (unsigned long long) item * kBatchEB + (unsigned long long) e;
-        fixed = true;
-      }
-    }
-    if (v && !fixed) {
-      double f = A.C0 * (T / (2.0 * (double) (1L << lev) - 1));
+    const bool v = e < n_eb && (flags & kFHanded) == 0;  // handed-over energies finish in the deep kernel
+    if (v) {
+      double f = A.C0 * (U.T / (2.0 * (double) (1L << lev) - 1));
       double E[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) E[k] = w_sum[k * kBatchEB + e] * f;
       if (A.use_pol) pol_project(E, A.pol);
-      atomicAdd(&R.out[io * R.n_e + e0 + e], A.C2 * cmag2(E) * A.weight[ip]);
+      atomicAdd(&U.R.out[U.io * U.R.n_e + U.e0 + e], A.C2 * cmag2(E) * A.weight[U.ip]);
     }
-    const bool counted = v && !fixed;
-    unsigned long long nc = __ballot(counted && !(flags & 1));
-    unsigned long long ev = counted ? (unsigned long long) ((2L << lev) - 1) : 0ULL;
+    unsigned long long nc = __ballot(v && !(flags & kFConverged));
+    unsigned long long ev = v ? (unsigned long long) ((2L << lev) - 1) : 0ULL;
     for (int off = 32; off > 0; off >>= 1) ev += __shfl_down(ev, off);
     atomicAdd(A.n_unconverged, lane == 0 ? (unsigned long long) __popcll(nc) : 0ULL);
     atomicAdd(A.evals, lane == 0 ? ev : 0ULL);
   }
   atomicAdd(A.main_amps, lane == 0 ? n_amp : 0ULL);
   atomicAdd(A.main_evals, lane == 0 ? n_ev : 0ULL);
+  atomicAdd(A.main_slots, lane == 0 ? n_slot : 0ULL);
   __syncthreads();  // the next unit overwrites the scratch
 }
 
@@ -289,11 +448,9 @@ __device__ __forceinline__ void batch_unit(const BatchArgs& A, long item, int la
 // the rare interpolation path may spill.
 template <bool MIXED>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw_batch_kernel(BatchArgs A) {
-  __shared__ double s_tau[64];
-  __shared__ double s_v[3][64];
-  __shared__ int s_list[kBatchEB];
+  __shared__ BatchLds S;
   int lane = threadIdx.x;
-  double* ws = A.ws + (long) blockIdx.x * kBatchWsD * kBatchEB;
+  double* ws = A.ws + (long) blockIdx.x * kBatchWsTotal;
   int* wsi = A.wsi + (long) blockIdx.x * kBatchWsI * kBatchEB;
   while (true) {
     // Next work unit.  Every lane executes the atomic (lane 0 adds 1, the others
@@ -304,10 +461,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) lw
     unsigned long long mine = atomicAdd(A.counter, lane == 0 ? 1ULL : 0ULL);
     long item = (long) uniform_u64(mine);
     if (item >= A.total_items) break;
-    batch_unit<MIXED>(A, item, lane, ws, wsi, s_tau, s_v, s_list);
+    batch_unit<MIXED>(A, item, lane, ws, wsi, S);
   }
 }
-
 // Waves per point in the per-point kernels: each level's samples are split
 // into kPointWaves contiguous slices, one per wave, combined in LDS.
 constexpr int kPointWaves = 8;

@@ -129,6 +129,9 @@ struct LevelJobBatch {
 constexpr int kBatchEB = 512;
 constexpr int kBatchWsD = 10;
 constexpr int kBatchWsI = 2;
+// plus, per wave, 7 partial sums x 64 lanes for each of the kBatchEB / 64
+// energy groups of a level (kept between super-blocks of samples)
+constexpr int kBatchWsTotal = kBatchWsD * kBatchEB + (kBatchEB / 64) * 7 * 64;
 
 struct ReqDev {
   const double* obs;    // 3 per point
@@ -161,10 +164,14 @@ struct BatchArgs {
   long total_items;
   unsigned long long* n_unconverged;
   unsigned long long* evals;  // sum over points of 2^(L+1)-1 (as integer)
-  double* ws;                 // per persistent wave: kBatchWsD x kBatchEB doubles
+  double* ws;                 // per persistent wave: kBatchWsTotal doubles
   int* wsi;                   // per persistent wave: kBatchWsI x kBatchEB ints
-  // mixed path: points whose stop decision lay within the FP32 error bound, as
-  // item * kBatchEB + energy-in-block, recomputed in FP64 by lw_batch_fix_kernel
+  // mixed path: points the batch kernel recomputed in FP64 inside their unit
+  // (their stop decision lay within the FP32 error bound)
+  unsigned long long* n_recheck;
+  // mixed path: handed-over points whose stop decision lay within the FP32
+  // error bound, as item * kBatchEB + energy-in-block, recomputed in FP64 by
+  // lw_batch_fix_kernel
   unsigned long long* fix_list;
   long fix_cap;
   unsigned long long* n_fix;  // points flagged (those beyond fix_cap keep the mixed result)
@@ -179,10 +186,12 @@ struct BatchArgs {
   long deep_cap;
   unsigned long long* n_deep;  // points handed over (those beyond deep_cap stay with their unit)
   // work done inside lw_batch_kernel itself (for its roofline): amplitude
-  // evaluations, one per (unit, energy group, sample), and integrand
-  // evaluations, one per (active energy, sample)
+  // evaluations, one per (unit, energy group, sample), integrand evaluations,
+  // one per (active energy, sample), and lane slots of the integrand loop
+  // (idle lanes included)
   unsigned long long* main_amps;
   unsigned long long* main_evals;
+  unsigned long long* main_slots;
 };
 
 }  // namespace om

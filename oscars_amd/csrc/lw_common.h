@@ -44,12 +44,26 @@ __device__ __forceinline__ double cmag2(const double E[6]) {
 //   V = (1-|B|^2)(N-B)/(D^2 (1-N.B)^2) + N x ((N-B) x A)/(D (1-N.B)^2)
 // and tau = t + D/c.  tau and N = R/D follow the reference's operations
 // exactly (the phase, and 1 - N.B, which cancels to ~1/(2 gamma^2), are
-// sensitive to the last bit); the two final scalings of V multiply by
-// reciprocals instead of dividing each component (V moves by ~1e-16 relative).
+// sensitive to the last bit): D is the IEEE square root, and R/D and D/c are
+// the IEEE quotients, formed from the correctly rounded reciprocals 1/D (one
+// division) and 1/c (a constant) with Markstein's correction (div_by).  The
+// two final scalings of V use a Newton-refined hardware reciprocal of
+// D (1-N.B)^2 and the reciprocal of D (V moves by a few 1e-16 relative).
 struct Amp {
   double tau;
   V3 V;
 };
+
+constexpr double kInvC = 1.0 / kC;  // correctly rounded at compile time
+
+// 1/x to within about one ulp: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double recip_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.0);
+  return __builtin_fma(y, e, y);
+}
 
 __device__ __forceinline__ Amp amplitude(V3 obs, const double* __restrict__ s) {
   V3 X = v3(s[1], s[2], s[3]);
@@ -58,18 +72,17 @@ __device__ __forceinline__ Amp amplitude(V3 obs, const double* __restrict__ s) {
   V3 R = sub(obs, X);
   double D2 = mag2(R);
   double D = sqrt(D2);
-  V3 N = v3(R.x / D, R.y / D, R.z / D);
+  double invD = 1.0 / D;
+  V3 N = v3(div_by(R.x, D, invD), div_by(R.y, D, invD), div_by(R.z, D, invD));
   Amp a;
-  a.tau = s[0] + D / kC;
+  a.tau = s[0] + div_by(D, kC, kInvC);
   double s1 = 1 - mag2(B);
   V3 NmB = sub(N, B);
   double om1 = 1 - dot(N, B);
   double om2 = om1 * om1;
-  double den1 = D * D * om2;
-  double den2 = D * om2;
   V3 c2 = cross(N, cross(NmB, Ac));
-  double q1 = s1 / den1;
-  double q2 = 1.0 / den2;
+  double q2 = recip_nr(D * om2);
+  double q1 = s1 * q2 * invD;
   a.V = v3(NmB.x * q1 + c2.x * q2, NmB.y * q1 + c2.y * q2, NmB.z * q1 + c2.z * q2);
   return a;
 }
@@ -85,10 +98,7 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ void sincos_mode(bool mixed, double phi, double& sn, double& cs) {
   if (mixed) {
-    float sf, cf;
-    sincos_reduced32(reduce_pio2(phi), &sf, &cf);
-    sn = (double) sf;
-    cs = (double) cf;
+    sincos_fast64(phi, &sn, &cs);
   } else {
     sincos_huge64(phi, &sn, &cs);
   }
@@ -196,24 +206,24 @@ __device__ __forceinline__ void slice_level_sum_rows(const double* __restrict__ 
   an = al;
 }
 
-// Mixed path: bound of the error the FP32 sin/cos puts into a level's stop
-// decision.  The FP32 cos/sin of the exactly reduced phase is within
-// kSinCos32Err of the exact value (largest error over 1e8 reduced arguments in
-// [-pi/4, pi/4]: 9.35e-8, for cos near 0; the bound adds margin).  Each real
+// Mixed path: bound of the error the reduced-accuracy sin/cos (sincos_fast64)
+// puts into a level's stop decision.  Its cos/sin of the exactly reduced phase
+// are within OM_SINCOS_FAST_ERR of the exact values (the polynomials' largest
+// error on a dense grid, with a margin for FP64 rounding).  Each real
 // component of the sum, Re/Im of sum_i V_k,i e^{i phi}, is then off by at most
-// kSinCos32Err * sum_i |V_k,i|, so the complex vector E = sum * dt is off by
-// d <= sqrt(2) kSinCos32Err an dt, an = sum over the samples so far of |V|_1
-// (the polarization projection does not increase the norm), and |E|^2 by
+// OM_SINCOS_FAST_ERR * sum_i |V_k,i|, so the complex vector E = sum * dt is off
+// by d <= sqrt(2) OM_SINCOS_FAST_ERR an dt, an = sum over the samples so far of
+// |V|_1 (the polarization projection does not increase the norm), and |E|^2 by
 // dm <= 2 |E| d + 3 d^2 (|E| from the mixed sum, within d of the exact one).
 // prec = |mag - lastmag| / lastmag then moves by at most
 // (dm + dm_last (1 + prec)) / (lastmag - dm_last).  A decision whose margin
 // |prec - precision| is within 1.25 times that bound is "borderline" (the FP64
 // path might decide the other way); such points are recomputed in FP64.
-constexpr double kSinCos32Err = 1.2e-7;
+constexpr double kSinCosMixedErr = OM_SINCOS_FAST_ERR;
 
 __device__ __forceinline__ bool level_borderline(int L, double dt, double precision, double mag_prev, double mag,
                                                  double prec, double maxd, double an, double& dm_last) {
-  double d = 1.4142135623730951 * kSinCos32Err * an * dt;
+  double d = 1.4142135623730951 * kSinCosMixedErr * an * dt;
   double dm = 2.0 * sqrt(mag) * d + 3.0 * d * d;
   double den = mag_prev - dm_last;
   double b = 1.25 * (dm + dm_last * (1.0 + prec)) / den;

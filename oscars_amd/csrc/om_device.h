@@ -111,21 +111,37 @@ __device__ __forceinline__ double cos64(double x) {
   return c;
 }
 
-// FP32 polynomial sin/cos on the FP64-reduced argument (mixed-precision path).
-__device__ __forceinline__ void sincos_reduced32(Reduced a, float* s, float* c) {
-  float r = (float) a.r;
-  float u = r * r;
-  float ps = __builtin_fmaf(u, OM_S32_2, OM_S32_1);
-  ps = __builtin_fmaf(u, ps, OM_S32_0);
-  float sr = __builtin_fmaf(r * u, ps, r);
-  float pc = __builtin_fmaf(u, OM_C32_2, OM_C32_1);
-  pc = __builtin_fmaf(u, pc, OM_C32_0);
-  float cr = __builtin_fmaf(u * u, pc, __builtin_fmaf(-0.5f, u, 1.0f));
-  int q = a.q;
-  float ss = (q & 1) ? cr : sr;
-  float cc = (q & 1) ? sr : cr;
-  *s = (q & 2) ? -ss : ss;
-  *c = ((q + 1) & 2) ? -cc : cc;
+// Reduced-accuracy sin/cos of a huge phase, in FP64 (mixed path).  The phase
+// is reduced modulo pi with the same exact two-part scheme (k = nearest
+// integer to phi/pi, r = phi - k*P1 exact for |r| < 2, then r -= k*P2),
+// |r| <= pi/2 + 2e-5; sin(r) = r + r^3 SF(r^2) and cos(r) = 1 + r^2 QF(r^2)
+// with the low-degree polynomials of sincos_coeffs.h, whose absolute error,
+// FP64 rounding included, is at most OM_SINCOS_FAST_ERR; sin(phi) and cos(phi)
+// are those of r with the sign flipped for odd k (one shift and two xors
+// instead of a quadrant select).
+constexpr double kOneOverPi = 0.31830988618379067154;
+constexpr double kPiHi = 3.1415926535897931160;      // nearest double to pi
+constexpr double kPiLo = 1.2246467991473532072e-16;  // pi - kPiHi (rounded)
+
+__device__ __forceinline__ void sincos_fast64(double phi, double* s, double* c) {
+  double kk = __builtin_fma(phi, kOneOverPi, kRoundMagic);
+  unsigned q = (unsigned) (uint32_t) __double_as_longlong(kk);
+  double k = kk - kRoundMagic;
+  double r = __builtin_fma(-k, kPiHi, phi);
+  r = __builtin_fma(-k, kPiLo, r);
+  double u = r * r;
+  double ps = __builtin_fma(u, OM_SF_3, OM_SF_2);
+  ps = __builtin_fma(u, ps, OM_SF_1);
+  ps = __builtin_fma(u, ps, OM_SF_0);
+  double sr = __builtin_fma(r * u, ps, r);
+  double pq = __builtin_fma(u, OM_QF_4, OM_QF_3);
+  pq = __builtin_fma(u, pq, OM_QF_2);
+  pq = __builtin_fma(u, pq, OM_QF_1);
+  pq = __builtin_fma(u, pq, OM_QF_0);
+  double cr = __builtin_fma(u, pq, 1.0);
+  unsigned long long sg = (unsigned long long) (q << 31) << 32;  // bit 0 of k -> sign bit
+  *s = __longlong_as_double(__double_as_longlong(sr) ^ (long long) sg);
+  *c = __longlong_as_double(__double_as_longlong(cr) ^ (long long) sg);
 }
 
 }  // namespace om

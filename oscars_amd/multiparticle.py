@@ -77,13 +77,13 @@ class BatchedFlux:
 
     def __init__(self, device, fields, particles, windows, requests, req_of, weights, mode="mixed",
                  polarization=(0.0,) * 6, precision=0.01, max_level=-2, max_level_extended=0, level_cap=0,
-                 waves_per_cu=0, efields=()):
+                 waves_per_cu=0, efields=(), deep_level=0):
         self.dev = device
         self.n_particles = len(particles)
         self.requests = requests
         device.trajectories(fields, om_particles(particles, windows), efields=efields)
         conv = device.conv(precision, max_level, max_level_extended, 0, mode)
-        device.batch_setup(requests, req_of, weights, list(polarization), conv, waves_per_cu, level_cap)
+        device.batch_setup(requests, req_of, weights, list(polarization), conv, waves_per_cu, level_cap, deep_level)
 
     def tune(self, candidates=(2, 4, 8, 16), sample_items=20000):
         return self.dev.batch_tune(candidates, sample_items)

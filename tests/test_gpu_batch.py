@@ -121,10 +121,40 @@ def test_sr_nparticles_batched_matches_particle_loop():
 
 
 @gpu
-def test_batched_mixed_levels_match_fp64():
+@pytest.mark.parametrize("mode,tol", [("tiled", 1e-9), ("mixed", 1e-5)])
+def test_batched_hand_over_matches_oracle(batch_case, mode, tol):
+    """deep_level=10: every point still active at level 10 is finished by the
+    workgroup-per-point kernel from its handed-over state (and, mixed, its
+    borderline points by the FP64 recheck kernel) instead of inside its unit.
+    Same bar against the oracle, same evaluation count as the default."""
+    from oscars_amd import multiparticle as M
+    from oscars_amd.sr import sr
+    parts, windows, reqs, req_of, weights, ref = batch_case
+    s = sr()
+    s.add_bfield_undulator(**UNDULATOR)
+    dev = s._device()
+    base = M.BatchedFlux(dev, s._fields, parts, windows, reqs, req_of, weights, mode=mode)
+    _, st0 = base.run(recompute_trajectories=False)
+    bf = M.BatchedFlux(dev, s._fields, parts, windows, reqs, req_of, weights, mode=mode, deep_level=10)
+    outs, st = bf.run(recompute_trajectories=False)
+    _, cnt, _ = dev.batch_profile(False)
+    print("%s deep_level 10: handed over %d, listed for the recheck kernel %d, rechecked in unit %d"
+          % (mode, cnt["handed_over"], cnt["fix_listed"], cnt["rechecked_in_unit"]))
+    assert cnt["handed_over"] > 0
+    assert st["evals"] == st0["evals"] and st["n_unconverged"] == 0
+    for r in range(len(reqs)):
+        e = rel_err(outs[r], ref[r], 1e-6)
+        print("%s deep_level 10 request %d: max rel err %.3e" % (mode, r, e))
+        assert e < tol
+
+
+@gpu
+@pytest.mark.parametrize("deep_level", [0, 10])
+def test_batched_mixed_levels_match_fp64(deep_level):
     """Mixed batch against the FP64 batch on a denser case: identical levels
-    (equal evaluation counts: borderline lanes are recomputed in FP64 inside
-    the same graph) and per-point agreement to 1e-5."""
+    (equal evaluation counts: borderline points are recomputed in FP64 inside
+    the same graph, in their unit or, for points handed over at deep_level, by
+    the recheck kernel) and per-point agreement to 1e-5."""
     from oscars_amd import multiparticle as M
     from oscars_amd.sr import sr
     parts, windows, _, _, _ = scenario(n_particles=24, seed=7)
@@ -136,11 +166,11 @@ def test_batched_mixed_levels_match_fp64():
     dev = s._device()
     res = {}
     for mode in ("tiled", "mixed"):
-        bf = M.BatchedFlux(dev, s._fields, parts, windows, reqs, req_of, weights, mode=mode)
+        bf = M.BatchedFlux(dev, s._fields, parts, windows, reqs, req_of, weights, mode=mode, deep_level=deep_level)
         res[mode] = bf.run(recompute_trajectories=False)
     (o64, s64), (o32, s32) = res["tiled"], res["mixed"]
     errs = [rel_err(a, b, 1e-6) for a, b in zip(o32, o64)]
-    print("mixed batch: evals %g (fp64 %g), units rechecked %d, max rel err %.3e"
-          % (s32["evals"], s64["evals"], s32["n_rechecked"], max(errs)))
+    print("mixed batch (deep level %d): evals %g (fp64 %g), points rechecked %d, max rel err %.3e"
+          % (deep_level, s32["evals"], s64["evals"], s32["n_rechecked"], max(errs)))
     assert s32["evals"] == s64["evals"]
     assert max(errs) < 1e-5
