@@ -54,6 +54,9 @@ constexpr double kTwoOverPi = 0.63661977236758134308;
 constexpr double kPio2Hi = 1.5707963267948966192;      // nearest double to pi/2
 constexpr double kPio2Lo = 6.123233995736766036e-17;   // pi/2 - kPio2Hi (rounded)
 constexpr double kRoundMagic = 6755399441055744.0;     // 1.5 * 2^52
+constexpr double kOneOverPi = 0.31830988618379067154;
+constexpr double kPiHi = 3.1415926535897931160;      // nearest double to pi
+constexpr double kPiLo = 1.2246467991473532072e-16;  // pi - kPiHi (rounded)
 
 struct Reduced {
   double r;
@@ -96,15 +99,35 @@ __device__ __forceinline__ void sincos_huge64(double phi, double* s, double* c) 
   sincos_reduced64(reduce_pio2(phi), s, c);
 }
 
-// sin(x) for the field models (|x| up to ~1e5 rad): the same exact reduction
-// and polynomials, without a large-argument path; within about one ulp.
+// sin(x) for the field models (|x| up to ~1e5 rad), within two ulp.
+// The argument is reduced exactly modulo pi (k = nearest integer to x/pi,
+// r = x - k*P1 exact for |r| < 2, then r -= k*P2), sin(r) = r + r^3 SH(r^2)
+// on |r| <= pi/2 + 2e-5 with SH of degree 7 evaluated by Estrin's scheme, and
+// the sign flipped for odd k: one polynomial and no quadrant select, with a
+// short dependency chain (the RK4 integration is one serial chain of these).
 __device__ __forceinline__ double sin64(double x) {
-  double s, c;
-  sincos_reduced64(reduce_pio2(x), &s, &c);
-  return s;
+  double kk = __builtin_fma(x, kOneOverPi, kRoundMagic);
+  unsigned q = (unsigned) (uint32_t) __double_as_longlong(kk);
+  double k = kk - kRoundMagic;
+  double r = __builtin_fma(-k, kPiHi, x);
+  r = __builtin_fma(-k, kPiLo, r);
+  double u = r * r;
+  double u2 = u * u;
+  double u4 = u2 * u2;
+  double p01 = __builtin_fma(u, OM_SH_1, OM_SH_0);
+  double p23 = __builtin_fma(u, OM_SH_3, OM_SH_2);
+  double p45 = __builtin_fma(u, OM_SH_5, OM_SH_4);
+  double p67 = __builtin_fma(u, OM_SH_7, OM_SH_6);
+  double q0 = __builtin_fma(u2, p23, p01);
+  double q1 = __builtin_fma(u2, p67, p45);
+  double sh = __builtin_fma(u4, q1, q0);
+  double sr = __builtin_fma(r * u, sh, r);
+  unsigned long long sg = (unsigned long long) (q << 31) << 32;  // bit 0 of k -> sign bit
+  return __longlong_as_double(__double_as_longlong(sr) ^ (long long) sg);
 }
 
-// cos(x) for the time dependence of the field models, as sin64.
+// cos(x) for the time dependence of the field models: the reduction by pi/2
+// and polynomial pair of sincos_huge64, within about one ulp.
 __device__ __forceinline__ double cos64(double x) {
   double s, c;
   sincos_reduced64(reduce_pio2(x), &s, &c);
@@ -119,10 +142,6 @@ __device__ __forceinline__ double cos64(double x) {
 // FP64 rounding included, is at most OM_SINCOS_FAST_ERR; sin(phi) and cos(phi)
 // are those of r with the sign flipped for odd k (one shift and two xors
 // instead of a quadrant select).
-constexpr double kOneOverPi = 0.31830988618379067154;
-constexpr double kPiHi = 3.1415926535897931160;      // nearest double to pi
-constexpr double kPiLo = 1.2246467991473532072e-16;  // pi - kPiHi (rounded)
-
 __device__ __forceinline__ void sincos_fast64(double phi, double* s, double* c) {
   double kk = __builtin_fma(phi, kOneOverPi, kRoundMagic);
   unsigned q = (unsigned) (uint32_t) __double_as_longlong(kk);

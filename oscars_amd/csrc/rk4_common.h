@@ -310,34 +310,38 @@ __device__ __forceinline__ bool beta_at_least_one(double v1, double v3, double v
 template <int M>
 __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, const double* x, double* d) {
   if (M >= kAxisUndX && M <= kAxisUndZ) {
-    if (beta_at_least_one(x[1], x[3], x[5])) {
-      c.err = 1;
-      return;
-    }
+    // Branch-free, so the |beta| test and the magnet's extent are evaluated
+    // beside the sine rather than before it on the serial chain.  Outside the
+    // magnet the field is an exact zero (the reference returns zero there).
     constexpr int K = M - kAxisUndX;
+    const bool bad = beta_at_least_one(x[1], x[3], x[5]);
     double D = x[4] - f0.c2;
     double sc = und_scale(f0, D, f0.psA);
     double a = K == 0 ? f0.a0 : (K == 1 ? f0.a1 : f0.a2);
-    // outside the magnet the field is zero (a * 0 * s up to the sign of zero):
-    // the sine is skipped there
-    double v = 0.0;
-    if (sc != 0.0) v = a * sc * sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
-    d[0] = x[1];
-    d[2] = x[3];
-    d[4] = x[5];
+    double s = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
+    double v = sc != 0.0 ? a * sc * s : 0.0;
+    double n1, n3, n5;
     if (K == 0) {
-      d[1] = 0.0;
-      d[3] = c.qomg * (x[5] * v);
-      d[5] = c.qomg * (-(x[3] * v));
+      n1 = 0.0;
+      n3 = c.qomg * (x[5] * v);
+      n5 = c.qomg * (-(x[3] * v));
     } else if (K == 1) {
-      d[1] = c.qomg * (-(x[5] * v));
-      d[3] = 0.0;
-      d[5] = c.qomg * (x[1] * v);
+      n1 = c.qomg * (-(x[5] * v));
+      n3 = 0.0;
+      n5 = c.qomg * (x[1] * v);
     } else {
-      d[1] = c.qomg * (x[3] * v);
-      d[3] = c.qomg * (-(x[1] * v));
-      d[5] = 0.0;
+      n1 = c.qomg * (x[3] * v);
+      n3 = c.qomg * (-(x[1] * v));
+      n5 = 0.0;
     }
+    // a state with |beta| >= 1 sets the flag and leaves d as it was
+    d[0] = bad ? d[0] : x[1];
+    d[1] = bad ? d[1] : n1;
+    d[2] = bad ? d[2] : x[3];
+    d[3] = bad ? d[3] : n3;
+    d[4] = bad ? d[4] : x[5];
+    d[5] = bad ? d[5] : n5;
+    c.err = bad ? 1 : c.err;
     return;
   }
   if (M == kFieldsEB) {

@@ -98,3 +98,51 @@ def test_bound_catches_a_perturbed_polynomial():
     assert float(np.max(np.abs(s - np.sin(r)))) > _bound()
     _, c = poly(r, qf=Q[:2] + [Q[2] * (1 + 1e-3)] + Q[3:])
     assert float(np.max(np.abs(c - np.cos(r)))) > _bound()
+
+
+def _sh():
+    co = {m.group(1): float(m.group(2)) for m in re.finditer(r"#define (OM_SH_\d) \(([-0-9.e+]+)\)", _header())}
+    return [co["OM_SH_%d" % i] for i in range(8)]
+
+
+def _sin64(x, sh=None):
+    """sin64 (om_device.h) of one double: exact reduction modulo pi as the
+    kernel rounds it, then the Estrin evaluation (numpy roundings)."""
+    S = sh or _sh()
+    one_over_pi, pi_hi, pi_lo, magic = 0.31830988618379067154, 3.1415926535897931160, 1.2246467991473532072e-16, \
+        6755399441055744.0
+    kk = float(Fraction(x) * Fraction(one_over_pi) + Fraction(magic))
+    k = kk - magic
+    r = float(Fraction(x) - Fraction(k) * Fraction(pi_hi))
+    r = float(Fraction(r) - Fraction(k) * Fraction(pi_lo))
+    u = r * r
+    u2 = u * u
+    u4 = u2 * u2
+    q0 = (u2 * (u * S[3] + S[2])) + (u * S[1] + S[0])
+    q1 = (u2 * (u * S[7] + S[6])) + (u * S[5] + S[4])
+    s = (r * u) * (u4 * q1 + q0) + r
+    return -s if int(k) % 2 else s
+
+
+def test_field_sine_within_two_ulp():
+    """The field models' sine: within two ulp of the exact value (half an ulp
+    from the reduction's last rounding, the rest from the polynomial; numpy's
+    separate roundings here stay inside the same two), near its zeros too, for
+    arguments up to 1e5 rad."""
+    mp.mp.dps = 50
+    rng = np.random.default_rng(9)
+    xs = np.concatenate([rng.uniform(-1e5, 1e5, 1500), rng.uniform(-10, 10, 1500),
+                         np.pi * np.arange(-40, 41) + rng.uniform(-1e-6, 1e-6, 81), [1e-300, -1e-20, 0.0]])
+    worst = 0.0
+    for x in xs:
+        ref = float(mp.sin(mp.mpf(float(x))))
+        got = _sin64(float(x))
+        ulp = np.spacing(abs(ref)) if ref != 0 else 5e-324
+        worst = max(worst, abs(got - ref) / ulp)
+    print("field sine: worst error %.2f ulp" % worst)
+    assert worst <= 2.05
+    # a coefficient off by 1e-12 relative is already visible
+    S = _sh()
+    bad = max(abs(_sin64(float(x), [S[0]] + [S[1] * (1 + 1e-12)] + S[2:]) - float(mp.sin(mp.mpf(float(x))))) /
+              np.spacing(abs(float(mp.sin(mp.mpf(float(x)))))) for x in np.linspace(1.2, 1.5, 50))
+    assert bad > 2.0
