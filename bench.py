@@ -59,13 +59,17 @@ def log(msg):
     print("# " + msg, file=sys.stderr, flush=True)
 
 
+# the sources the persistent batch kernel and its launch are built from
+BATCH_SOURCES = ("batch.hip", "lw_common.h", "om_device.h", "om_types.h", "launch_types.h", "sincos_coeffs.h",
+                 "rk4_common.h", "api.cpp")
+
+
 def source_hash():
-    """Hash of the kernel and ABI sources (ties a committed counter profile to the code it measured)."""
+    """Hash of the batch kernel's sources (ties a committed counter profile to the code it measured)."""
     h = hashlib.sha256()
-    for name in sorted(os.listdir(CSRC)):
-        if name.endswith((".hip", ".h", ".cpp")):
-            with open(os.path.join(CSRC, name), "rb") as f:
-                h.update(name.encode() + f.read())
+    for name in BATCH_SOURCES:
+        with open(os.path.join(CSRC, name), "rb") as f:
+            h.update(name.encode() + f.read())
     return h.hexdigest()[:16]
 
 
@@ -199,7 +203,8 @@ def roofline(head, counters_file):
             tj = json.load(f)
         if tj.get("source_hash") == source_hash():
             traffic = tj.get("hbm_bytes_per_launch")
-            extra = {k: tj.get(k) for k in ("valu_issue_frac", "valu_wave_instr_per_eval", "hbm_gbs")}
+            extra = {k: tj.get(k) for k in ("valu_issue_frac", "valu_wave_instr_per_eval", "hbm_gbs", "clock_ghz",
+                                            "fp64_tflops_counted")}
             note = tj.get("source")
         else:
             note = "%s measured other kernel sources (hash %s, now %s): traffic not reported" % (
@@ -427,7 +432,7 @@ def main():
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--counters", action="store_true",
                     help="one eager batch step after setup, nothing timed (for rocprofv3 --pmc runs)")
-    ap.add_argument("--waves-per-cu", type=int, default=8, help="persistent grid of a --counters run")
+    ap.add_argument("--waves-per-cu", type=int, default=16, help="persistent grid of a --counters run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device-to-device copy and exit (reference for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
