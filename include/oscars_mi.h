@@ -42,8 +42,10 @@ enum {
 /* Kernel families: PLAIN one lane per output point, all FP64 (bit-identical
  * phase; the reference-faithful baseline); TILED one wave per observation
  * point x 64 energies sharing the per-sample amplitude, FP64; MIXED the tiled
- * kernel with FP32 sin/cos of the exactly reduced FP64 phase; POINT_MIXED the
- * per-point kernel with FP32 sin/cos (single-energy maps). */
+ * kernel with reduced-accuracy sin/cos (FP64 polynomials within 4.8e-9) of the
+ * exactly reduced FP64 phase, borderline stop decisions rechecked at full
+ * accuracy; POINT_MIXED the per-point kernel with the same sin/cos
+ * (single-energy maps). */
 enum { OM_MODE_PLAIN = 0, OM_MODE_TILED = 1, OM_MODE_MIXED = 2, OM_MODE_POINT_MIXED = 3 };
 enum { OM_Q_VALUE = 0, OM_Q_PRECISION = 1, OM_Q_LEVEL = 2 };
 
@@ -122,8 +124,8 @@ typedef struct {
   double total_ms;     /* whole call, host wall clock */
   int max_level;       /* highest level any point reached */
   long n_unconverged;
-  long n_rechecked;    /* mixed modes: points whose stop decision lay within the FP32 error
-                          bound and were recomputed in FP64 */
+  long n_rechecked;    /* mixed modes: points whose stop decision lay within the sin/cos error
+                          bound and were recomputed with full-accuracy sin/cos */
 } om_stats;
 
 int om_device_count(void);

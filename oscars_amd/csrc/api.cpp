@@ -836,7 +836,7 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
       A.list = d_list;
       A.n_list = nt;
     }
-    // Mixed path: points whose stop decision lay within the FP32 error bound are
+    // Mixed path: points whose stop decision lay within the sin/cos error bound are
     // recomputed from level 0 in FP64, one wave per point (their state was
     // reset when listed), so every point stops where the FP64 path stops: one
     // launch over the levels the main pass reached, then one per deeper level.

@@ -122,7 +122,7 @@ __device__ __forceinline__ Unit unit_of(const BatchArgs& A, long item) {
 
 // Flags of an energy in the wave's scratch.
 constexpr int kFConverged = 1;
-constexpr int kFBorderline = 2;  // mixed pass: a stop decision lay within the FP32 error bound
+constexpr int kFBorderline = 2;  // mixed pass: a stop decision lay within the sin/cos error bound
 constexpr int kFHanded = 4;      // handed to lw_batch_deep_kernel
 constexpr int kFRecheck = 8;     // recomputed in FP64 inside the unit
 constexpr int kFIdle = 16;       // not part of the FP64 recheck pass
@@ -359,8 +359,8 @@ __device__ __forceinline__ void unit_levels(const BatchArgs& A, const Unit& U, i
 }
 
 // One work unit: the level loop for the block's energies, weighted flux added
-// into the request's output with FP64 atomics.  MIXED: a first pass with FP32
-// sin/cos; the energies whose stop decision lay within the FP32 error bound are
+// into the request's output with FP64 atomics.  MIXED: a first pass with the
+// reduced-accuracy sin/cos; the energies whose stop decision lay within its error bound are
 // then recomputed from level 0 with FP64 sin/cos in a second pass of the same
 // unit (their amplitudes shared, the rows still in cache), so every energy
 // stops where the FP64 path stops.  The per-energy state lives in the wave's

@@ -167,10 +167,10 @@ struct BatchArgs {
   double* ws;                 // per persistent wave: kBatchWsTotal doubles
   int* wsi;                   // per persistent wave: kBatchWsI x kBatchEB ints
   // mixed path: points the batch kernel recomputed in FP64 inside their unit
-  // (their stop decision lay within the FP32 error bound)
+  // (their stop decision lay within the reduced-accuracy sin/cos error bound)
   unsigned long long* n_recheck;
-  // mixed path: handed-over points whose stop decision lay within the FP32
-  // error bound, as item * kBatchEB + energy-in-block, recomputed in FP64 by
+  // mixed path: handed-over points whose stop decision lay within the
+  // reduced-accuracy sin/cos error bound, as item * kBatchEB + energy-in-block, recomputed in FP64 by
   // lw_batch_fix_kernel
   unsigned long long* fix_list;
   long fix_cap;

@@ -22,12 +22,12 @@
 //                     and broadcast through LDS to the 64 energy lanes, which each
 //                     evaluate phi = -omega*tau, sin/cos and the complex MAC.
 //                     FP64 variant: bit-identical phase, FP64 sin/cos polynomial;
-//                     mixed variant: the same FP64 phase and exact FP64
-//                     reduction, then an FP32 sin/cos polynomial.  Amplitudes
-//                     and the complex accumulation stay FP64 in both: with
-//                     packed FP32 disabled, v_fma_f32 and v_fma_f64 issue at
-//                     the same rate on gfx950, so FP32 there would cost
-//                     accuracy and save nothing.
+//                     mixed variant: the same FP64 phase, reduced exactly
+//                     modulo pi, then low-degree FP64 sin/cos polynomials
+//                     (error <= 4.8e-9).  Amplitudes and the complex
+//                     accumulation stay FP64 in both: with packed FP32
+//                     disabled, v_fma_f32 and v_fma_f64 issue at the same rate
+//                     on gfx950, so FP32 would cost accuracy and save nothing.
 //   pd_point_kernel   power density, one lane = one observation point
 //   lw_finalize_kernel  C0*dt scaling, polarization projection, flux / precision /
 //                     level output, optional weighted Kahan accumulation over particles
@@ -101,7 +101,7 @@ __device__ __forceinline__ void slice_combine(SliceLds<S>& sh, int lane, long ns
 
 // ---------------------------------------------------------------- one lane per point
 // MIXED = false is the plain FP64 path; MIXED = true evaluates sin/cos with the
-// FP32 polynomial on the exactly reduced FP64 phase.  Every lane reads the same
+// reduced-accuracy polynomials on the exactly reduced FP64 phase.  Every lane reads the same
 // trajectory row (scalar loads) and computes its own amplitude and phase.
 template <bool MIXED, int S>
 __global__ void __launch_bounds__(64 * S) lw_point_kernel(LwArgs A) {
@@ -600,7 +600,7 @@ __global__ void __launch_bounds__(256) lw_finalize_kernel(FinArgs F) {
 // samples, levels La..Lb from the point's state.  Used where few points are
 // active (the last levels of a map, and the FP64 recomputation of the mixed
 // path's borderline points): a wave per point keeps their latency low.
-// MIXED: FP32 sin/cos with the error-bound bookkeeping of the per-point kernel.
+// MIXED: reduced-accuracy sin/cos with the error-bound bookkeeping of the per-point kernel.
 // Four waves (points) per workgroup.
 template <bool MIXED>
 __global__ void __launch_bounds__(256) lw_wave_kernel(LwArgs A) {

@@ -106,7 +106,7 @@ __device__ __forceinline__ void sincos_mode(bool mixed, double phi, double& sn, 
 
 // Samples [i0, i1) of one level of one point's sum, the 64 lanes of a wave
 // splitting them (lane j takes samples i0+j, i0+j+64, ...; amp_at(i) gives
-// sample i's amplitude).  MIXED: FP32 sin/cos of the exactly reduced phase, and
+// sample i's amplitude).  MIXED: reduced-accuracy sin/cos of the exactly reduced phase, and
 // `an` returns the slice's sum of |V|_1 (for the error bound); else FP64.  The
 // largest phase step between consecutive samples takes the previous sample's
 // phase from the neighbouring lane (for sample i0 > 0, from amp_at(i0 - 1)).

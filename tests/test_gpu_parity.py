@@ -101,7 +101,7 @@ def test_spectrum(oracle_planar, gpu_planar, mode, tol):
 @pytest.mark.parametrize("mode,ref", [("mixed", "tiled"), ("point_mixed", "plain")])
 def test_mixed_levels_match_fp64(gpu_planar, mode, ref):
     """Every stop decision of the mixed paths matches the FP64 path: decisions
-    within the FP32 error bound are recomputed in FP64 (counted in
+    within the sin/cos error bound are recomputed in FP64 (counted in
     n_rechecked)."""
     kw = dict(npoints=[41, 41], width=[0.02, 0.02], translation=[0, 0, 30])
     e = np.linspace(140.0, 460.0, 64) if mode == "mixed" else np.array([152.11])
