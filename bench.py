@@ -64,12 +64,36 @@ BATCH_SOURCES = ("batch.hip", "lw_common.h", "om_device.h", "om_types.h", "launc
                  "rk4_common.h", "api.cpp")
 
 
-def source_hash():
-    """Hash of the batch kernel's sources (ties a committed counter profile to the code it measured)."""
+def code_tokens(text):
+    """C/C++ source without comments and whitespace (string literals kept)."""
+    out, i, n = [], 0, len(text)
+    while i < n:
+        if text.startswith("//", i):
+            j = text.find("\n", i)
+            i = n if j < 0 else j
+        elif text.startswith("/*", i):
+            j = text.find("*/", i + 2)
+            i = n if j < 0 else j + 2
+        elif text[i] == '"':
+            j = i + 1
+            while j < n and text[j] != '"':
+                j += 2 if text[j] == "\\" else 1
+            out.append(text[i:j + 1])
+            i = j + 1
+        else:
+            if not text[i].isspace():
+                out.append(text[i])
+            i += 1
+    return "".join(out)
+
+
+def source_hash(read=None):
+    """Hash of the batch kernel's code, comments and layout aside (ties a
+    committed counter profile to the code it measured)."""
+    read = read or (lambda name: open(os.path.join(CSRC, name)).read())
     h = hashlib.sha256()
     for name in BATCH_SOURCES:
-        with open(os.path.join(CSRC, name), "rb") as f:
-            h.update(name.encode() + f.read())
+        h.update(name.encode() + code_tokens(read(name)).encode())
     return h.hexdigest()[:16]
 
 
