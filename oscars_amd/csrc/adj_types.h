@@ -21,7 +21,14 @@ struct AdjRkArgs {
   const FieldDev* F;
   int nfields;
   int gfield;
-  double* gbar;        // 3 per sample of the differentiated grid
+  // grid-sample gradient contributions: 10 records per knot (5 field
+  // evaluations, each adding to two neighbouring samples), record r = grid
+  // sample grec_key[r] (ngrid: none) and the 3 values grec_val[3r..3r+2];
+  // summed per sample in record order afterwards, so the gradient does not
+  // depend on scheduling
+  int* grec_key;       // null: no grid to differentiate
+  double* grec_val;
+  int ngrid;
   double qomg, dt;
   long nf, K, nseg;
   const double* ckpt;  // nseg x 6

@@ -20,6 +20,9 @@
 //   C: each segment recomputes its states again and sweeps back from its
 //      lambda_{i1}, scattering the field-sample gradients (FP64 atomics).
 // The sequential depth is about 3 K steps instead of 3 N.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
 #include "adj_types.h"
 #include "lw_common.h"
 #include "rk4_common.h"
@@ -380,9 +383,11 @@ __device__ __forceinline__ void fjac_vec(const FieldD& fd, double qomg, const do
   out[5] = qomg * ((dv[0] * fd.B[1] - dv[1] * fd.B[0]) + (v[0] * dB[1] - v[1] * dB[0]));
 }
 
-// adjoint: ybar += Jf(y)^T w, and the grid-sample gradient of w . f(y) into gbar
+// adjoint: ybar += Jf(y)^T w, and the grid-sample gradient of w . f(y) as the
+// two records rec, rec + 1 (the samples either side of the point; ngrid marks
+// no contribution)
 __device__ __forceinline__ void fjac_tvec(const FieldD& fd, double qomg, const double* y, const double* w,
-                                          double* ybar, double* gbar) {
+                                          double* ybar, const AdjRkArgs& A, long rec) {
   double wa[3] = {w[1], w[3], w[5]};
   double v[3] = {y[1], y[3], y[5]};
   // v-bar = w_pos + qomg (B x w_a);  B-bar = qomg (w_a x v)
@@ -392,21 +397,21 @@ __device__ __forceinline__ void fjac_tvec(const FieldD& fd, double qomg, const d
   double Bb[3] = {qomg * (wa[1] * v[2] - wa[2] * v[1]), qomg * (wa[2] * v[0] - wa[0] * v[2]),
                   qomg * (wa[0] * v[1] - wa[1] * v[0])};
   for (int j = 0; j < 3; ++j) ybar[2 * j] += Bb[0] * fd.J[0][j] + Bb[1] * fd.J[1][j] + Bb[2] * fd.J[2][j];
-  if (gbar && fd.n0 >= 0) {
-    double* g0 = gbar + 3 * fd.n0;
+  if (A.grec_key && rec >= 0) {
+    const bool on = fd.n0 >= 0;
+    A.grec_key[rec] = on ? fd.n0 : A.ngrid;
+    A.grec_key[rec + 1] = on ? fd.n0 + 1 : A.ngrid;
     for (int i = 0; i < 3; ++i) {
-      atomicAdd(g0 + i, (1 - fd.w) * Bb[i]);
-      atomicAdd(g0 + 3 + i, fd.w * Bb[i]);
+      A.grec_val[3 * rec + i] = on ? (1 - fd.w) * Bb[i] : 0.0;
+      A.grec_val[3 * (rec + 1) + i] = on ? fd.w * Bb[i] : 0.0;
     }
   }
 }
 
-
-
 // c_i: direct knot terms and the late-stored acceleration terms of state i.
 // Acceleration knot a_{i+1} = f_a(y_i)/c, and a_0 = f_a(y_0)/c.
 __device__ __forceinline__ void direct_terms(const AdjRkArgs& A, long i, const FieldD& fd, const double* y,
-                                             double* c, double* gbar) {
+                                             double* c, long rec) {
   const double* k = A.kb + 9 * i;
   for (int r = 0; r < 6; ++r) c[r] = 0;
   c[0] = k[0]; c[2] = k[1]; c[4] = k[2];
@@ -422,7 +427,7 @@ __device__ __forceinline__ void direct_terms(const AdjRkArgs& A, long i, const F
     w[1] += k[6] / kC; w[3] += k[7] / kC; w[5] += k[8] / kC;
     any = true;
   }
-  if (any) fjac_tvec(fd, A.qomg, y, w, c, gbar);
+  if (any) fjac_tvec(fd, A.qomg, y, w, c, A, rec);
 }
 
 // Phase A: per segment, lambda_{i0} = M lambda_{i1} + b.
@@ -444,7 +449,7 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
     FieldD f0;
     field_grad(A.F, A.nfields, A.gfield, y, f0);
     double c[6];
-    direct_terms(A, i, f0, y, c, nullptr);
+    direct_terms(A, i, f0, y, c, -1);
     for (int r = 0; r < 6; ++r) {
       double acc = b[r];
       for (int q = 0; q < 6; ++q) acc += M[r][q] * c[q];
@@ -572,27 +577,27 @@ __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
       }
       double sb[6];
       for (int r = 0; r < 6; ++r) sb[r] = 0;
-      fjac_tvec(f3, A.qomg, y3, k4b, sb, A.gbar);  // k4 = f(y3), y3 = y + h k3
+      fjac_tvec(f3, A.qomg, y3, k4b, sb, A, 10 * i);      // k4 = f(y3), y3 = y + h k3
       for (int r = 0; r < 6; ++r) {
         yb[r] += sb[r];
         k3b[r] += h * sb[r];
         sb[r] = 0;
       }
-      fjac_tvec(f2, A.qomg, y2, k3b, sb, A.gbar);  // k3 = f(y2), y2 = y + hh k2
+      fjac_tvec(f2, A.qomg, y2, k3b, sb, A, 10 * i + 2);  // k3 = f(y2), y2 = y + hh k2
       for (int r = 0; r < 6; ++r) {
         yb[r] += sb[r];
         k2b[r] += hh * sb[r];
         sb[r] = 0;
       }
-      fjac_tvec(f1, A.qomg, y1, k2b, sb, A.gbar);  // k2 = f(y1), y1 = y + hh k1
+      fjac_tvec(f1, A.qomg, y1, k2b, sb, A, 10 * i + 4);  // k2 = f(y1), y1 = y + hh k1
       for (int r = 0; r < 6; ++r) {
         yb[r] += sb[r];
         k1b[r] += hh * sb[r];
       }
-      fjac_tvec(f0, A.qomg, y, k1b, yb, A.gbar);  // k1 = f(y)
+      fjac_tvec(f0, A.qomg, y, k1b, yb, A, 10 * i + 6);   // k1 = f(y)
     }
     double c[6];
-    direct_terms(A, i, f0, y, c, A.gbar);
+    direct_terms(A, i, f0, y, c, 10 * i + 8);
     for (int r = 0; r < 6; ++r) lam[r] = yb[r] + c[r];
   }
   if (s == 0) {
@@ -603,6 +608,37 @@ __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
 __global__ void __launch_bounds__(256) adj_fill_kernel(double* __restrict__ p, long n) {
   long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
+}
+
+// Record keys start at `v` (no contribution).
+__global__ void __launch_bounds__(256) adj_key_fill_kernel(int* __restrict__ k, long n, int v) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) k[i] = v;
+}
+
+// gbar[3g + c]: the sum over the records of grid sample g, in record order (the
+// records sorted stably by sample; the first of sample g found by bisection).
+__global__ void __launch_bounds__(256) adj_grad_gather_kernel(const int* __restrict__ skey,
+                                                              const int* __restrict__ sidx, long n,
+                                                              const double* __restrict__ val, int ngrid,
+                                                              double* __restrict__ gbar) {
+  long g = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrid) return;
+  long lo = 0, hi = n;
+  while (lo < hi) {
+    long mid = (lo + hi) >> 1;
+    if (skey[mid] < g) lo = mid + 1; else hi = mid;
+  }
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (long r = lo; r < n && skey[r] == g; ++r) {
+    const long k = sidx[r];
+    s0 += val[3 * k];
+    s1 += val[3 * k + 1];
+    s2 += val[3 * k + 2];
+  }
+  gbar[3 * g] = s0;
+  gbar[3 * g + 1] = s1;
+  gbar[3 * g + 2] = s2;
 }
 
 }  // namespace om
@@ -654,5 +690,35 @@ extern "C" hipError_t om_launch_adj_fill(double* p, long n, hipStream_t s) {
   long blocks = (n + 255) / 256;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(om::adj_fill_kernel, dim3((unsigned) blocks), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_adj_key_fill(int* k, long n, int v, hipStream_t s) {
+  long blocks = (n + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(om::adj_key_fill_kernel, dim3((unsigned) blocks), dim3(256), 0, s, k, n, v);
+  return hipGetLastError();
+}
+
+// Bytes of temporary storage the stable sort of n records (keys below 2^end_bit) takes.
+extern "C" hipError_t om_adj_grad_sort_bytes(long n, int end_bit, size_t* bytes) {
+  return rocprim::radix_sort_pairs(nullptr, *bytes, (const int*) nullptr, (int*) nullptr,
+                                   rocprim::counting_iterator<int>(0), (int*) nullptr, (unsigned) n, 0,
+                                   (unsigned) end_bit);
+}
+
+// The records sorted stably by grid sample (rocPRIM's radix sort: records of
+// one sample keep their order), then summed per sample into gbar.
+extern "C" hipError_t om_launch_adj_grad_gather(const int* key, int* skey, int* sidx, const double* val, long n,
+                                                int ngrid, int end_bit, void* temp, size_t temp_bytes, double* gbar,
+                                                hipStream_t s) {
+  size_t bytes = temp_bytes;
+  hipError_t e = rocprim::radix_sort_pairs(temp, bytes, key, skey, rocprim::counting_iterator<int>(0), sidx,
+                                           (unsigned) n, 0, (unsigned) end_bit, s);
+  if (e != hipSuccess) return e;
+  long blocks = (ngrid + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(om::adj_grad_gather_kernel, dim3((unsigned) blocks), dim3(256), 0, s, skey, sidx, n, val, ngrid,
+                     gbar);
   return hipGetLastError();
 }

@@ -70,6 +70,10 @@ hipError_t om_launch_adj_spline(const double* rows, long ns, const double* sb, i
                                 double* cc, hipStream_t s);
 hipError_t om_launch_adj_rk4(const om::AdjRkArgs* A, hipStream_t s);
 hipError_t om_launch_adj_fill(double* p, long n, hipStream_t s);
+hipError_t om_launch_adj_key_fill(int* k, long n, int v, hipStream_t s);
+hipError_t om_adj_grad_sort_bytes(long n, int end_bit, size_t* bytes);
+hipError_t om_launch_adj_grad_gather(const int* key, int* skey, int* sidx, const double* val, long n, int ngrid,
+                                     int end_bit, void* temp, size_t temp_bytes, double* gbar, hipStream_t s);
 }
 
 namespace {
@@ -309,11 +313,15 @@ struct om_context {
     std::vector<double> E;  // 6 per output: C0 dt_L S (no polarization projection)
     DevBuf<double> ckpt, rows, obs, omega, g, partial, sb, yb, yppb, Pw, ppw, sgw, ub, cc, segmap, lam, ybuf, gbar;
     DevBuf<int> klo;
+    DevBuf<int> grec_key, grec_skey, grec_sidx;  // grid-gradient records and their stable sort
+    DevBuf<double> grec_val;
+    DevBuf<unsigned char> sort_tmp;
     void release() {
       ckpt.release(); rows.release(); obs.release(); omega.release(); g.release(); partial.release();
       sb.release(); yb.release(); yppb.release(); Pw.release(); ppw.release(); sgw.release(); ub.release();
       cc.release();
       segmap.release(); lam.release(); ybuf.release(); gbar.release(); klo.release();
+      grec_key.release(); grec_skey.release(); grec_sidx.release(); grec_val.release(); sort_tmp.release();
       ready = false;
     }
   } adj;
@@ -1935,12 +1943,27 @@ int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad
     A.lam.ensure(6 * (A.nseg + 1));
     A.ybuf.ensure(6 * A.nf);
     A.gbar.ensure(A.ngrid > 0 ? 3 * A.ngrid : 3);
-    check(om_launch_adj_fill(A.gbar.p, A.ngrid > 0 ? 3 * A.ngrid : 3, ctx->stream), "clear");
+    // grid-sample gradients: 10 records per knot, summed per sample in record order
+    const long nrec = 10 * A.nf;
+    int end_bit = 1;
+    while ((1L << end_bit) <= A.ngrid) ++end_bit;
+    if (A.ngrid > 0) {
+      A.grec_key.ensure(nrec);
+      A.grec_skey.ensure(nrec);
+      A.grec_sidx.ensure(nrec);
+      A.grec_val.ensure(3 * nrec);
+      size_t tmp = 0;
+      check(om_adj_grad_sort_bytes(nrec, end_bit, &tmp), "sort size");
+      A.sort_tmp.ensure(tmp > 0 ? tmp : 1);
+      check(om_launch_adj_key_fill(A.grec_key.p, nrec, (int) A.ngrid, ctx->stream), "clear");
+    }
     om::AdjRkArgs R;
     R.F = ctx->d_fields.p;
     R.nfields = ctx->nfields;
     R.gfield = A.gfield;
-    R.gbar = A.ngrid > 0 ? A.gbar.p : nullptr;
+    R.grec_key = A.ngrid > 0 ? A.grec_key.p : nullptr;
+    R.grec_val = A.ngrid > 0 ? A.grec_val.p : nullptr;
+    R.ngrid = (int) A.ngrid;
     R.qomg = D.qomg;
     R.dt = D.dt;
     R.nf = A.nf;
@@ -1952,6 +1975,10 @@ int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad
     R.lam = A.lam.p;
     R.ybuf = A.ybuf.p;
     check(om_launch_adj_rk4(&R, ctx->stream), "adjoint rk4");
+    if (A.ngrid > 0)
+      check(om_launch_adj_grad_gather(A.grec_key.p, A.grec_skey.p, A.grec_sidx.p, A.grec_val.p, nrec, (int) A.ngrid,
+                                      end_bit, A.sort_tmp.p, A.sort_tmp.n, A.gbar.p, ctx->stream),
+            "grid gradient");
     check(hipEventRecord(e1, ctx->stream), "event");
     double lam0[6];
     check(hipMemcpyAsync(lam0, A.lam.p, sizeof(lam0), hipMemcpyDeviceToHost, ctx->stream), "lambda");

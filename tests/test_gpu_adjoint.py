@@ -160,3 +160,18 @@ def test_autograd_function_matches_problem(problem):
     gf, ga, _ = problem.backward(2.0 * (F2 - f_ideal.numpy()))
     assert np.allclose(s.grad.numpy(), gf, rtol=1e-12, atol=0)
     assert np.allclose(a.grad.numpy(), ga, rtol=1e-12, atol=0)
+
+
+@gpu
+def test_adjoint_gradient_is_bitwise_reproducible(problem):
+    """The grid-sample gradient is summed per sample in a fixed record order
+    (no floating-point atomics), so repeated reverse passes agree bit for bit."""
+    err, ideal = maps()
+    f_ideal = problem.forward(ideal, (0.0, 0.0))
+    out = []
+    for _ in range(3):
+        F = problem.forward(err, (0.0, 0.0))
+        gf, ga, gx = problem.backward(2.0 * (F - f_ideal))
+        out.append((gf.copy(), np.array(ga), np.array(gx)))
+    for gf, ga, gx in out[1:]:
+        assert np.array_equal(gf, out[0][0]) and np.array_equal(ga, out[0][1]) and np.array_equal(gx, out[0][2])
