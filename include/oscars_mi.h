@@ -45,7 +45,10 @@ enum {
  * kernel with reduced-accuracy sin/cos (FP64 polynomials within 4.8e-9) of the
  * exactly reduced FP64 phase, borderline stop decisions rechecked at full
  * accuracy; POINT_MIXED the per-point kernel with the same sin/cos
- * (single-energy maps). */
+ * (single-energy maps).  Power density: PLAIN and TILED evaluate the
+ * reference's FP64 term; MIXED and POINT_MIXED a term built on a Newton-refined
+ * reciprocal square root and reciprocal (within about 1e-8 of the FP64 one),
+ * borderline stop decisions recomputed with the FP64 term. */
 enum { OM_MODE_PLAIN = 0, OM_MODE_TILED = 1, OM_MODE_MIXED = 2, OM_MODE_POINT_MIXED = 3 };
 enum { OM_Q_VALUE = 0, OM_Q_PRECISION = 1, OM_Q_LEVEL = 2 };
 

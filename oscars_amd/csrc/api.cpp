@@ -58,7 +58,7 @@ hipError_t om_launch_lw_wave(const om::LwArgs* A, int mixed, hipStream_t s);
 hipError_t om_launch_omega_from_ev(const double* e, long n, double* om, double qe, double hbar, hipStream_t s);
 hipError_t om_launch_level_stats(const int* level, const unsigned char* conv, long n, unsigned long long* out,
                                  hipStream_t s);
-hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s);
+hipError_t om_launch_pd_point(const om::PdArgs* A, int mixed, int split, hipStream_t s);
 hipError_t om_launch_finalize(const om::FinArgs* F, hipStream_t s);
 hipError_t om_launch_state_reset(const om::State* st, long npts, hipStream_t s);
 hipError_t om_launch_count_active(const om::State* st, long npts, int* out, hipStream_t s);
@@ -1255,9 +1255,12 @@ int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, co
     } else {
       upload(ctx, ctx->d_obs, obs_xyz, 3 * n_obs);
     }
+    // MIXED / POINT_MIXED: the reciprocal-based terms, borderline stop
+    // decisions recomputed from level 0 with the FP64 terms
+    bool mixed = conv->mode == OM_MODE_MIXED || conv->mode == OM_MODE_POINT_MIXED;
     om::PdArgs A;
     std::memset(&A, 0, sizeof(A));
-    state_for(ctx, n_obs, false, A.st);
+    state_for(ctx, n_obs, mixed, A.st);
     A.npts = n_obs;
     A.obs = ctx->d_obs.p;
     A.has_normal = normal || normals ? 1 : 0;
@@ -1274,21 +1277,51 @@ int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, co
     A.precision = conv->precision;
     A.two_over_gamma = 2. / (ctx->parts[ip].gamma);
     float kms = 0;
-    int La = 0;
-    int Lb = stop_ext < kFirstLevels ? stop_ext : kFirstLevels;
-    while (true) {
-      ensure_levels(ctx, ip, Lb, stop_mem);
-      ensure_beta_diff(ctx, ip, Lb);
-      fill_levels(ctx, ip, Lb, A.lv);
-      A.La = La;
-      A.Lb = Lb;
-      ctx->timer.begin(ctx->stream);
-      check(om_launch_pd_point(&A, ctx->stream), "pd_point");
-      ctx->timer.end(ctx->stream);
-      if (Lb >= stop_ext) break;
-      if (count_active(ctx, A.st, n_obs) == 0) break;
-      La = Lb + 1;
-      Lb = La;
+    // Levels 0..13 in one launch over every point, then one launch per level
+    // over the points still active (compacted list); the same for the mixed
+    // path's rechecked points, in FP64.
+    auto level_loop = [&](int La, int Lb, bool mix) {
+      int Ldone = La;
+      while (true) {
+        ensure_levels(ctx, ip, Lb, stop_mem);
+        ensure_beta_diff(ctx, ip, Lb);
+        fill_levels(ctx, ip, Lb, A.lv);
+        A.La = La;
+        A.Lb = Lb;
+        ctx->timer.begin(ctx->stream);
+        check(om_launch_pd_point(&A, mix ? 1 : 0, ctx->split, ctx->stream), "pd_point");
+        ctx->timer.end(ctx->stream);
+        Ldone = Lb;
+        if (Lb >= stop_ext) break;
+        La = Lb + 1;
+        Lb = La;
+        int* d_list = scratch_buf(ctx->d_list, n_obs, nullptr);
+        int* d_count = scratch_buf(ctx->d_count, 1, nullptr);
+        check(om_launch_point_list(&A.st, n_obs, La, d_list, d_count, ctx->stream), "point list");
+        int nt = 0;
+        read_back(ctx, &nt, d_count, sizeof(int));
+        if (nt == 0) break;
+        A.list = d_list;
+        A.n_list = nt;
+      }
+      return Ldone;
+    };
+    int Ldone = level_loop(0, stop_ext < kFirstLevels ? stop_ext : kFirstLevels, mixed);
+    long n_recheck = 0;
+    if (mixed) {
+      int* d_list = scratch_buf(ctx->d_list, n_obs, nullptr);
+      int* d_count = scratch_buf(ctx->d_count, 1, nullptr);
+      check(om_launch_border_list(&A.st, n_obs, d_list, d_count, ctx->stream), "border list");
+      int nb = 0;
+      read_back(ctx, &nb, d_count, sizeof(int));
+      n_recheck = nb;
+      if (nb > 0) {
+        A.st.err = nullptr;
+        A.st.border = nullptr;
+        A.list = d_list;
+        A.n_list = nb;
+        level_loop(0, Ldone, false);
+      }
     }
     const om_particle& P = ctx->parts[ip];
     om::FinArgs F;
@@ -1306,6 +1339,7 @@ int power_density(om_context* ctx, int ip, const double* obs_xyz, long n_obs, co
     if (stats) {
       stats->kernel_ms = kms;
       stats->total_ms = wall.ms();
+      stats->n_rechecked = n_recheck;
     }
   });
 }

@@ -52,6 +52,8 @@ struct PdArgs {
   int directional;
   double precision;
   double two_over_gamma;
+  const int* list;          // null: every point; else n_list point indices, one per lane
+  long n_list;
 };
 
 // Power density at the centres of STL triangles with occlusion by the others.

@@ -28,7 +28,13 @@
 //                     accumulation stay FP64 in both: with packed FP32
 //                     disabled, v_fma_f32 and v_fma_f64 issue at the same rate
 //                     on gfx950, so FP32 would cost accuracy and save nothing.
-//   pd_point_kernel   power density, one lane = one observation point
+//   pd_point_kernel   power density, one lane = one observation point, a level's
+//                     samples split over the S waves of a workgroup.  FP64
+//                     variant: the reference's operations; mixed variant:
+//                     reciprocal square root and reciprocal refined by Newton
+//                     steps instead of the square root and quotients, with an
+//                     error bound per point that sends borderline stop
+//                     decisions to the FP64 variant
 //   lw_finalize_kernel  C0*dt scaling, polarization projection, flux / precision /
 //                     level output, optional weighted Kahan accumulation over particles
 #include <mutex>
@@ -353,10 +359,121 @@ __global__ void __launch_bounds__(64 * S) lw_tile_kernel(LwArgs A, long n_tiles_
 }
 
 // ---------------------------------------------------------------- power density
+// One workgroup of S waves per 64 observation points (lane = point): wave w
+// sums the contiguous slice [ns*w/S, ns*(w+1)/S) of each level, the rows being
+// wave-uniform (scalar loads); the slices' partial sums are added in wave order
+// through LDS, so all waves hold the same totals and stop decisions.
+//
+// MIXED = false: the reference's operations for each term (IEEE square root
+// and quotients).  MIXED = true: N = R / |R| from a Newton-refined reciprocal
+// square root, one Newton-refined reciprocal of (1-N.B)^5 |R|^2 in place of
+// the square root and the five quotients, and
+// Num = N x ((N-B) x Ac) = (N-B)(N.Ac) - Ac (1-N.B)  (|N| = 1).
+// With x = 1 - N.B, T = |Num|^2 / (x^5 |R|^2) >= |term| and
+// Q = x |Ac|^2 / (x^5 |R|^2), a mixed term lies within
+// (Kx / x + K0) T + KQ Q of the FP64 term (u = 2^-53):
+//   - N differs by at most 7.5 u between the paths, x by 14 u, so x^5 by
+//     70 u / x relative;
+//   - Num, a function of N with slope at most 6 |Ac|, moves by 45 u |Ac|, and
+//     the two paths' roundings and x add 46 u |Ac|, so
+//     |d|Num|^2| <= 2 |Num| 91 u |Ac| <= 91 u (|Num|^2 / x + x |Ac|^2);
+//   - the powers, reciprocal, quotients and N.normal (a directional surface
+//     may skip a term the other path keeps when N.normal is within 14 u of 0)
+//     add 41 u of T.
+// Kx = 260 u, K0 = 64 u, KQ = 150 u keep a margin of 1.6 over those sums
+// (tests/test_pd_mixed_bound.py checks the bound on the CPU, for electrons of
+// 3 GeV, down to x = 1/(2 gamma^2)).  A level adds (Kx / min x + K0) sum T +
+// KQ sum Q to the running bound e, the sums taken over the terms kept and over
+// the skipped ones with N.normal > -16 u; both paths' summation roundings add at
+// most 2 n u sum T after n samples.
+constexpr double kPdU = 1.1102230246251565e-16;  // 2^-53
+constexpr double kPdKx = 260 * kPdU;
+constexpr double kPdK0 = 64 * kPdU;
+constexpr double kPdKQ = 150 * kPdU;
 
-__global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
-  long p = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = p < A.npts;
+// 1/sqrt(x) to within about two ulp: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  double e = __builtin_fma(-(h * y), y, 0.5);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-(h * y), y, 0.5);
+  return __builtin_fma(y, e, y);
+}
+
+__device__ __forceinline__ double mag2_fma(V3 a) {
+  return __builtin_fma(a.z, a.z, __builtin_fma(a.y, a.y, a.x * a.x));
+}
+
+// One term of the power-density sum, src/OSCARSSR.cc:2886-2921.  Adds it to
+// ps unless the surface is directional and N.normal <= 0; the mixed path adds
+// T and Q to tsum and qsum and keeps the smallest x.
+template <bool MIXED>
+__device__ __forceinline__ void pd_term(const PdArgs& A, V3 obs, V3 nrm, const double* __restrict__ s, double& ps,
+                                        double& tsum, double& qsum, double& xmin) {
+  V3 X = v3(s[1], s[2], s[3]);
+  V3 B = v3(s[4], s[5], s[6]);
+  V3 Ac = v3(s[7], s[8], s[9]);
+  V3 R = sub(obs, X);
+  if (!MIXED) {
+    double r2 = mag2(R);
+    double m = sqrt(r2);
+    V3 N1 = v3(R.x / m, R.y / m, R.z / m);
+    // src/OSCARSSR.cc:2908: a surface without normals takes N1.normal as 1
+    double ndn = A.has_normal ? dot(N1, nrm) : 1.0;
+    V3 Num = cross(N1, cross(sub(N1, B), Ac));
+    double x = 1 - dot(B, N1);
+    double x2 = x * x;
+    double den = x2 * x2 * x;
+    // (Num.N2)^2 + (Num.N3)^2 = |Num|^2 because Num is perpendicular to N1
+    double term = mag2(Num) / den / r2 * ndn;
+    if (!(A.directional && ndn <= 0)) ps += term;
+  } else {
+    double r2 = mag2_fma(R);
+    double im = rsqrt_nr(r2);
+    V3 N1 = v3(R.x * im, R.y * im, R.z * im);
+    double ndn = A.has_normal ? __builtin_fma(N1.z, nrm.z, __builtin_fma(N1.y, nrm.y, N1.x * nrm.x)) : 1.0;
+    V3 NmB = sub(N1, B);
+    double x = __builtin_fma(-B.x, N1.x, __builtin_fma(-B.y, N1.y, __builtin_fma(-B.z, N1.z, 1.0)));
+    double nac = __builtin_fma(N1.z, Ac.z, __builtin_fma(N1.y, Ac.y, N1.x * Ac.x));
+    V3 Num = v3(__builtin_fma(NmB.x, nac, -(Ac.x * x)), __builtin_fma(NmB.y, nac, -(Ac.y * x)),
+                __builtin_fma(NmB.z, nac, -(Ac.z * x)));
+    double x2 = x * x;
+    double inv = recip_nr((x2 * x2) * (x * r2));
+    double t = mag2_fma(Num) * inv;
+    bool keep = !(A.directional && ndn <= 0);
+    if (keep) ps = __builtin_fma(t, ndn, ps);
+    // a skipped term counts in the bound only where the FP64 path might keep it
+    bool counted = keep || ndn > -16 * kPdU;
+    tsum += counted ? t : 0.0;
+    qsum = __builtin_fma(counted ? x * mag2_fma(Ac) : 0.0, inv, qsum);
+    xmin = fmin(xmin, x);
+  }
+}
+
+template <int S>
+struct PdLds {
+  double ps[S][64];
+  double ts[S][64];
+  double qs[S][64];
+  double xm[S][64];
+};
+
+template <bool MIXED, int S>
+__global__ void __launch_bounds__(64 * S) pd_point_kernel(PdArgs A) {
+  __shared__ PdLds<S> sh;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  long g = (long) blockIdx.x * 64 + lane;
+  bool valid;
+  long p;
+  if (A.list) {  // compacted: the points still active at this level
+    valid = g < A.n_list;
+    p = valid ? (long) A.list[g] : 0;
+  } else {
+    valid = g < A.npts;
+    p = g;
+  }
   long pc = valid ? p : 0;
   V3 obs = v3(A.obs[3 * pc], A.obs[3 * pc + 1], A.obs[3 * pc + 2]);
   V3 nrm = A.normals ? v3(A.normals[3 * pc], A.normals[3 * pc + 1], A.normals[3 * pc + 2])
@@ -366,51 +483,93 @@ __global__ void __launch_bounds__(256) pd_point_kernel(PdArgs A) {
   double sum = A.st.sum[6 * pc];
   double lastsum = A.st.lastmag[pc];
   double prec = A.st.prec[pc];
+  double e_run = 0, b_run = 0;
+  bool border = false;
+  if (MIXED) {
+    e_run = A.st.err[2 * pc];
+    b_run = A.st.err[2 * pc + 1];
+  }
   for (int L = A.La; L <= A.Lb; ++L) {
-    if (!__any(active)) break;
+    if (S > 1 ? !__syncthreads_or(active) : !__any(active)) break;
     const double* __restrict__ rows = A.lv.rows[L];
     long ns = 1L << L;
-    for (long i = 0; i < ns; ++i) {
-      const double* s = rows + 10 * i;
-      V3 X = v3(s[1], s[2], s[3]);
-      V3 B = v3(s[4], s[5], s[6]);
-      V3 Ac = v3(s[7], s[8], s[9]);
-      V3 R = sub(obs, X);
-      double r2 = mag2(R);
-      double m = sqrt(r2);
-      V3 N1 = v3(R.x / m, R.y / m, R.z / m);
-      // src/OSCARSSR.cc:2908: a surface without normals takes N1.normal as 1
-      double ndn = A.has_normal ? dot(N1, nrm) : 1.0;
-      V3 Num = cross(N1, cross(sub(N1, B), Ac));
-      double x = 1 - dot(B, N1);
-      double x2 = x * x;
-      double den = x2 * x2 * x;
-      // (Num.N2)^2 + (Num.N3)^2 = |Num|^2 because Num is perpendicular to N1
-      double term = mag2(Num) / den / r2 * ndn;
-      bool skip = A.directional && ndn <= 0;
-      if (active && !skip) sum += term;
+    long i0 = ns * w / S, i1 = ns * (w + 1) / S;
+    double ps = 0, tsum = 0, qsum = 0, xmin = 2.0;
+    for (long i = i0; i < i1; ++i) pd_term<MIXED>(A, obs, nrm, rows + 10 * i, ps, tsum, qsum, xmin);
+    if (S > 1) {
+      sh.ps[w][lane] = ps;
+      if (MIXED) {
+        sh.ts[w][lane] = tsum;
+        sh.qs[w][lane] = qsum;
+        sh.xm[w][lane] = xmin;
+      }
+      __syncthreads();
+      ps = 0;
+      tsum = 0;
+      qsum = 0;
+      xmin = 2.0;
+      for (int v = 0; v < S; ++v) {
+        long j0 = ns * v / S, j1 = ns * (v + 1) / S;
+        if (j1 <= j0) continue;
+        ps += sh.ps[v][lane];
+        if (MIXED) {
+          tsum += sh.ts[v][lane];
+          qsum += sh.qs[v][lane];
+          xmin = fmin(xmin, sh.xm[v][lane]);
+        }
+      }
+      // the next level's __syncthreads_or keeps these reads ahead of its writes
     }
     if (active) {
+      sum += ps;
       lev = L;
-      double thisSum = sum * A.lv.dt_incl[L];
+      double dt = A.lv.dt_incl[L];
+      double thisSum = sum * dt;
+      double prev = lastsum;
       prec = fabs(thisSum - lastsum) / lastsum;
       double bd = A.lv.beta_diff[L];
+      // src/OSCARSSR.cc:2940-2953
       bool c = (L > 8 && prec < A.precision && bd < A.two_over_gamma) ||
                (L > 8 && bd < A.two_over_gamma && thisSum == lastsum);
       bool c2 = !c && L > 8 && thisSum == lastsum;
       if (c2) prec = 0;
       lastsum = thisSum;
-      if (c || c2) {
+      if (MIXED) {
+        // error bounds of this level's and the last level's sums (the last from
+        // the running bounds as they stood before this level)
+        double dlast = L > 0 ? (e_run + 2.0 * (double) ns * kPdU * b_run) * A.lv.dt_incl[L - 1] +
+                                   4 * kPdU * fabs(prev)
+                             : 0.0;
+        e_run += (kPdKx / xmin + kPdK0) * tsum + kPdKQ * qsum;
+        b_run += tsum;
+        double dthis = (e_run + 4.0 * (double) ns * kPdU * b_run) * dt + 4 * kPdU * fabs(thisSum);
+        // no error at all (every term so far skipped far from N.normal = 0):
+        // the sums are the FP64 path's, and so is the decision
+        if (L > 8 && dthis + dlast > 0.0) {
+          double tol = 1.25 * (dthis + dlast);
+          double den = prev - dlast;
+          bool amb_equal = fabs(thisSum - prev) <= tol;
+          bool amb_prec = bd < A.two_over_gamma &&
+                          (den <= 0.0 || fabs(prec - A.precision) <= 1.25 * (dthis + dlast * (1.0 + prec)) / den);
+          border = border || amb_equal || amb_prec;
+        }
+      }
+      if (c || c2 || border) {
         A.st.conv[pc] = 1;
         active = false;
       }
     }
   }
-  if (valid && lev >= A.La) {
+  if (w == 0 && valid && lev >= A.La) {
     A.st.sum[6 * pc] = sum;
     A.st.lastmag[pc] = lastsum;
     A.st.prec[pc] = prec;
     A.st.level[pc] = lev;
+    if (MIXED) {
+      A.st.err[2 * pc] = e_run;
+      A.st.err[2 * pc + 1] = b_run;
+      if (border) A.st.border[pc] = 1;
+    }
   }
 }
 
@@ -996,9 +1155,30 @@ extern "C" hipError_t om_launch_tile_list(const om::State* st, long n_obs, long 
   return hipGetLastError();
 }
 
-extern "C" hipError_t om_launch_pd_point(const om::PdArgs* A, hipStream_t s) {
-  long blocks = (A->npts + 255) / 256;
-  hipLaunchKernelGGL(om::pd_point_kernel, dim3((unsigned) blocks), dim3(256), 0, s, *A);
+// Power density: S waves per 64 points, chosen so that the grid holds about
+// 8192 waves (at least 8 per SIMD) unless split > 0 names S.
+extern "C" hipError_t om_launch_pd_point(const om::PdArgs* A, int mixed, int split, hipStream_t s) {
+  long groups = ((A->list ? A->n_list : A->npts) + 63) / 64;
+  if (split <= 0) {
+    split = 1;
+    while (split < 8 && groups * split < 8192) split *= 2;
+  }
+  dim3 g((unsigned) groups);
+  if (mixed) {
+    switch (split) {
+      case 1: hipLaunchKernelGGL((om::pd_point_kernel<true, 1>), g, dim3(64), 0, s, *A); break;
+      case 2: hipLaunchKernelGGL((om::pd_point_kernel<true, 2>), g, dim3(128), 0, s, *A); break;
+      case 4: hipLaunchKernelGGL((om::pd_point_kernel<true, 4>), g, dim3(256), 0, s, *A); break;
+      default: hipLaunchKernelGGL((om::pd_point_kernel<true, 8>), g, dim3(512), 0, s, *A); break;
+    }
+  } else {
+    switch (split) {
+      case 1: hipLaunchKernelGGL((om::pd_point_kernel<false, 1>), g, dim3(64), 0, s, *A); break;
+      case 2: hipLaunchKernelGGL((om::pd_point_kernel<false, 2>), g, dim3(128), 0, s, *A); break;
+      case 4: hipLaunchKernelGGL((om::pd_point_kernel<false, 4>), g, dim3(256), 0, s, *A); break;
+      default: hipLaunchKernelGGL((om::pd_point_kernel<false, 8>), g, dim3(512), 0, s, *A); break;
+    }
+  }
   return hipGetLastError();
 }
 

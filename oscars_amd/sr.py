@@ -52,6 +52,11 @@ def _warn(msg):
     sys.stderr.write(msg)
 
 
+def _check_pd_mode(mode):
+    if mode not in ("plain", "tiled", "mixed", "point_mixed"):
+        raise ValueError("mode must be 'plain', 'tiled', 'mixed' or 'point_mixed'")
+
+
 def _named(f, name):
     """Tag a field with its name (get_bfield / get_efield select by it)."""
     f.name = name
@@ -998,7 +1003,13 @@ class sr:
     def calculate_power_density_rectangle(self, npoints=None, plane="XY", normal=0, width=None, rotations=None,
                                           translation=None, x0x1x2=None, nparticles=0, gpu=-1, ngpu=None,
                                           nthreads=0, precision=0.01, max_level=-2, max_level_extended=0, dim=2,
-                                          quantity="power density", ofile="", bofile=""):
+                                          quantity="power density", ofile="", bofile="", mode="plain"):
+        """Power density [W/mm^2] on a rectangle (src/OSCARSSR_Python.cc:6471-6797).
+        Extension: mode "mixed" evaluates each term with reciprocal-based
+        division and square root (within about 1e-6 of the FP64 terms) and
+        recomputes in FP64 every point whose stop decision lies within that
+        error's bound, so the levels are the FP64 path's."""
+        _check_pd_mode(mode)
         if self._beam is None:
             raise ValueError("No particle beam defined")
         if npoints is None:
@@ -1012,7 +1023,7 @@ class sr:
         obs_a = np.array(obs)
 
         def run(dev, ip, w, acc, out=None):
-            conv = dev.conv(precision, max_level, max_level_extended, q, "plain", w, acc)
+            conv = dev.conv(precision, max_level, max_level_extended, q, mode, w, acc)
             v, lev, cv, st = dev.power_density(ip, obs_a, nv, directional, conv, out=out)
             self.last_stats = st
             self._last_levels = lev
@@ -1029,11 +1040,13 @@ class sr:
             IO.write_points_binary(bofile, cont, v, dim)
         return [[list(cont[i]), float(v[i])] for i in range(len(cont))]
 
-    def _pd_points(self, obs, normals, directional, nparticles, precision, max_level, max_level_extended, q):
+    def _pd_points(self, obs, normals, directional, nparticles, precision, max_level, max_level_extended, q,
+                   mode="plain"):
+        _check_pd_mode(mode)
         obs_a = np.asarray(obs, dtype=np.float64).reshape(-1, 3)
 
         def run(dev, ip, w, acc, out=None):
-            conv = dev.conv(precision, max_level, max_level_extended, q, "plain", w, acc)
+            conv = dev.conv(precision, max_level, max_level_extended, q, mode, w, acc)
             v, lev, cv, st = dev.power_density_surface(ip, obs_a, normals, directional, conv, out=out)
             self.last_stats = st
             self._last_levels = lev
@@ -1044,17 +1057,18 @@ class sr:
 
     def calculate_power_density(self, points, normal=0, rotations=None, translation=None, nparticles=0, gpu=-1,
                                 ngpu=None, nthreads=0, precision=0.01, max_level=-2, max_level_extended=0,
-                                quantity="power density", ofile="", bofile=""):
+                                quantity="power density", ofile="", bofile="", mode="plain"):
         """Power density [W/mm^2] at [[x, y, z], [nx, ny, nz]] points, rotated
         then translated (normals rotated only); returns [[[x, y, z], pd], ...]
-        in absolute coordinates (src/OSCARSSR_Python.cc:6105-6380)."""
+        in absolute coordinates (src/OSCARSSR_Python.cc:6105-6380).  Extension:
+        mode as in calculate_power_density_rectangle."""
         if self._beam is None:
             raise ValueError("No particle beam defined")
         obs, nrm = surface_points(points, normal, rotations, translation)
         self._check_common(nparticles, nthreads, 0 if gpu == 0 else gpu)
         q = self._quantity(quantity, pd=True)
         v = self._pd_points(obs, np.array(nrm).reshape(-1, 3), normal != 0, nparticles, precision, max_level,
-                            max_level_extended, q)
+                            max_level_extended, q, mode)
         if not self._last_conv.all():
             _warn("Not all points converged to desired precision.  Can try increasing 'max_level_extended'\n")
         if ofile != "":
@@ -1064,10 +1078,12 @@ class sr:
         return [[list(obs[i]), float(v[i])] for i in range(len(obs))]
 
     def calculate_power_density_line(self, x1, x2, npoints=0, ofile="", bofile="", normal=0, nparticles=0, gpu=-1,
-                                     nthreads=0, precision=0.01, max_level=-2, max_level_extended=0, dim=1):
+                                     nthreads=0, precision=0.01, max_level=-2, max_level_extended=0, dim=1,
+                                     mode="plain"):
         """Power density at npoints evenly spaced from x1 to x2 (a surface
         without normals); dim 1 returns the signed distance from the middle,
-        dim 3 the points (src/OSCARSSR_Python.cc:7355-7560)."""
+        dim 3 the points (src/OSCARSSR_Python.cc:7355-7560).  Extension: mode
+        as in calculate_power_density_rectangle."""
         if self._beam is None:
             raise ValueError("No particle beam defined")
         if dim not in (1, 3):
@@ -1079,7 +1095,7 @@ class sr:
         step = [(b[k] - a[k]) / (npoints - 1) for k in range(3)]
         obs = [[a[k] + i * step[k] for k in range(3)] for i in range(npoints)]
         self._check_common(nparticles, nthreads, 0 if gpu == 0 else gpu)
-        v = self._pd_points(obs, None, normal != 0, nparticles, precision, max_level, max_level_extended, 0)
+        v = self._pd_points(obs, None, normal != 0, nparticles, precision, max_level, max_level_extended, 0, mode)
         if dim == 3:
             pos = obs
         else:
