@@ -24,6 +24,8 @@ Reported beside the headline (one JSON line, rank 0, N=1):
                  energies from a 4001-sample By(z) map, forward and reverse
   maps           the single-electron flux maps (201 x 201 at 152.11 and 260 eV,
                  51 x 51 x 512 energies), mixed and plain FP64
+  power_density  power density on 401 x 401 points from an EPU trajectory,
+                 mixed and FP64
   roofline       the persistent kernel's algorithmic flops over its own time
                  (HIP events around it on the library stream, eager launch)
   cpu_baseline   the reference's C++ core on the host cores, on a bounded sample
@@ -429,6 +431,62 @@ def maps_leg(steps=4):
     return res
 
 
+# ---------------------------------------------------------------- power-density leg
+def epu_scenario(s):
+    """Elliptically polarising undulator: 0.6 T x 0.6 T, period 57 mm, 40 periods,
+    3 GeV from z = -1.5 m, 40001-point trajectory over ct 0..3 m."""
+    s.set_particle_beam(energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    s.add_bfield_epu(bfieldA=[0, 0.6, 0], bfieldB=[0.6, 0, 0], period=[0, 0, 0.057], nperiods=40,
+                     phase=np.pi / 2)
+    s.set_ctstartstop(0, 3)
+    s.set_npoints_trajectory(40001)
+
+
+def power_density_leg(steps=4):
+    """Power density on 401 x 401 points (+-40 mm at 20 m, normal +z) from the
+    EPU trajectory, device-resident inputs and outputs, mixed and FP64; every
+    step integrates its trajectory first."""
+    import torch
+    from oscars_amd import geometry as G
+    from oscars_amd.sr import sr
+    s = sr()
+    epu_scenario(s)
+    dev = s._device()
+    obs, _, _ = G.rectangle("XY", 401, 401, 0.08, 0.08, [0, 0, 0], [0, 0, 20.0], 0, 2)
+    t_obs = torch.tensor(np.array(obs), dtype=torch.float64, device="cuda")
+    n = t_obs.shape[0]
+    out = {m: (torch.zeros(n, dtype=torch.float64, device="cuda"), torch.zeros(n, dtype=torch.int32, device="cuda"),
+               torch.zeros(n, dtype=torch.uint8, device="cuda")) for m in ("mixed", "plain")}
+
+    def step(mode):
+        dev.trajectories(s._fields, [s._om_particle(s._particle)], check_status=False)
+        return dev.power_density_dev(0, t_obs, [0, 0, 1], 0, dev.conv(mode=mode), *out[mode])
+
+    res = {}
+    for mode in ("mixed", "plain"):
+        step(mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kms, ev = 0.0, 0.0
+        for _ in range(steps):
+            st = step(mode)
+            kms += st["kernel_ms"]
+            ev += st["evals"]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[mode] = {"ms_per_step": 1e3 * dt / steps, "kernel_ms": kms / steps, "evals_per_step": ev / steps,
+                     "kernel_evals_per_s": ev / (kms * 1e-3), "rechecked_fp64": st["n_rechecked"]}
+        log("power density %s: %.2f ms/step, power-density kernels %.3f ms" % (mode, 1e3 * dt / steps, kms / steps))
+    vm, lm = out["mixed"][0].cpu().numpy(), out["mixed"][1].cpu().numpy()
+    vp, lp = out["plain"][0].cpu().numpy(), out["plain"][1].cpu().numpy()
+    res["rel_err_mixed_vs_plain"] = rel_errors(vm, vp)
+    res["level_mismatches"] = int(np.sum(lm != lp))
+    res["kernel_ratio_mixed_vs_plain"] = res["plain"]["kernel_ms"] / res["mixed"]["kernel_ms"]
+    res["workload"] = ("EPU 0.6 T x 0.6 T, 40 x 57 mm, 3 GeV, 40001-point RK4; power density 401x401 "
+                       "(+-40 mm @20 m)")
+    return res
+
+
 # ---------------------------------------------------------------- counter runs
 def counters_run(args, cfg):
     """Setup, then one eager batch step (for rocprofv3 --pmc runs of the persistent kernel)."""
@@ -454,6 +512,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-maps", action="store_true")
+    ap.add_argument("--no-pd", action="store_true")
+    ap.add_argument("--power-density-only", action="store_true",
+                    help="run the power-density leg alone and print its JSON (for rocprofv3 runs)")
     ap.add_argument("--counters", action="store_true",
                     help="one eager batch step after setup, nothing timed (for rocprofv3 --pmc runs)")
     ap.add_argument("--waves-per-cu", type=int, default=16, help="persistent grid of a --counters run")
@@ -476,6 +537,9 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.power_density_only:
+        print(json.dumps(power_density_leg(), default=float))
+        return
     cfg = Config4(args.particles, args.seed)
     if args.counters:
         return counters_run(args, cfg)
@@ -531,6 +595,11 @@ def main():
             line["maps"] = maps_leg()
         except Exception as ex:
             line["maps"] = {"error": str(ex)}
+    if not args.no_pd:
+        try:
+            line["power_density"] = power_density_leg()
+        except Exception as ex:
+            line["power_density"] = {"error": str(ex)}
     print(json.dumps(line, default=float))
 
 

@@ -208,13 +208,45 @@ struct UndParams {
   double psA, psB;
   double a0, a1, a2, b0, b1, b2;  // field A, field B (zero for a planar undulator)
   int epu;
+  // The region thresholds of the two parts (phase shifts psA, psB), formed on
+  // the host in the reference's operation order (src/TField3D_IdealUndulator.cc:130-137):
+  // -L/2+PS, L/2+PS, (-L/2+PS)+PL, (L/2+PS)-PL, (-L/2+PS)+PL/2, (L/2+PS)-PL/2
+  double thA[6], thB[6];
 };
 
-__device__ __forceinline__ double und_scale(const UndParams& f, double D, double PS) {
-  double L2 = f.half_len;
-  bool out = D > L2 + PS || D < -L2 + PS;
-  bool term = D < -L2 + PS + f.PL || D > L2 + PS - f.PL;
-  bool quarter = D < -L2 + PS + f.half_PL || D > L2 + PS - f.half_PL;
+// A copy of a wave-uniform double held in vector registers.  The integration
+// is one lane's dependent chain with the vector registers mostly free, while
+// the scalar registers cannot hold every constant of the field: kept there, the
+// constants spill into vector lanes and each use costs two v_readlane.  The
+// identity DPP move makes the compiler treat the value as per-lane.
+__device__ __forceinline__ double in_vgpr(double x) {
+  long long b = __builtin_bit_cast(long long, x);
+  int lo = __builtin_amdgcn_update_dpp(0, (int) (b & 0xffffffff), 0xE4, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), 0xE4, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long) hi << 32) | (unsigned) lo);
+}
+
+__device__ __forceinline__ UndParams und_in_vgpr(const UndParams& f) {
+  UndParams v = f;
+  v.c0 = in_vgpr(f.c0); v.c1 = in_vgpr(f.c1); v.c2 = in_vgpr(f.c2);
+  v.u0 = in_vgpr(f.u0); v.u1 = in_vgpr(f.u1); v.u2 = in_vgpr(f.u2);
+  v.taper = in_vgpr(f.taper); v.half_len = in_vgpr(f.half_len); v.PL = in_vgpr(f.PL);
+  v.inv_PL = in_vgpr(f.inv_PL); v.half_PL = in_vgpr(f.half_PL);
+  v.psA = in_vgpr(f.psA); v.psB = in_vgpr(f.psB);
+  v.a0 = in_vgpr(f.a0); v.a1 = in_vgpr(f.a1); v.a2 = in_vgpr(f.a2);
+  v.b0 = in_vgpr(f.b0); v.b1 = in_vgpr(f.b1); v.b2 = in_vgpr(f.b2);
+  for (int k = 0; k < 6; ++k) {
+    v.thA[k] = in_vgpr(f.thA[k]);
+    v.thB[k] = in_vgpr(f.thB[k]);
+  }
+  return v;
+}
+
+// 0 outside the magnet, 1/4 and 3/4 on the terminating half periods, else 1.
+__device__ __forceinline__ double und_scale(const double th[6], double D) {
+  bool out = D > th[1] || D < th[0];
+  bool term = D < th[2] || D > th[3];
+  bool quarter = D < th[4] || D > th[5];
   double sc = term ? (quarter ? 0.25 : 0.75) : 1.0;
   return out ? 0.0 : sc;
 }
@@ -222,12 +254,12 @@ __device__ __forceinline__ double und_scale(const UndParams& f, double D, double
 __device__ __forceinline__ void und_eval(const UndParams& f, double x, double y, double z, double B[3]) {
   double D = (x - f.c0) * f.u0 + (y - f.c1) * f.u1 + (z - f.c2) * f.u2;
   double tc = 1 + D * f.taper;
-  double scA = und_scale(f, D, f.psA);
+  double scA = und_scale(f.thA, D);
   double sA = sin64(div_by(2.0 * kPi * (D - f.psA), f.PL, f.inv_PL));
   // terminated poles: (field*scale)*sin*taper; full poles: field*sin*taper (scale 1 is exact)
   double v0 = f.a0 * scA * sA * tc, v1 = f.a1 * scA * sA * tc, v2 = f.a2 * scA * sA * tc;
   if (f.epu) {
-    double scB = und_scale(f, D, f.psB);
+    double scB = und_scale(f.thB, D);
     double sB = sin64(div_by(2.0 * kPi * (D - f.psB), f.PL, f.inv_PL));
     v0 = v0 + f.b0 * scB * sB * tc;
     v1 = v1 + f.b1 * scB * sB * tc;
@@ -272,8 +304,13 @@ struct RkCtx {
 //   M = kGridZ          one 1-D grid along z without rotation, samples in LDS: the
 //                       grid branch of field_eval with the identity rotation and the
 //                       x/y range tests (n = 1 there) removed, which changes no value.
+//   M = kUndZ / kEpuZ   one undulator (kUndZ) or EPU (kEpuZ) with its period along
+//                       z and no taper, fields in any direction: und_eval with
+//                       D = z - c_z and the taper factor 1, both exact (the dot
+//                       product's other terms are zeros, 1 + D 0 = 1), branch-free
+//                       like the axis variants.
 constexpr int kFieldsGeneric = 0, kOneUndulator = 1, kAxisUndX = 2, kAxisUndY = 3, kAxisUndZ = 4, kGridZ = 5,
-              kFieldsEB = 6;
+              kFieldsEB = 6, kUndZ = 7, kEpuZ = 8;
 
 __device__ __forceinline__ void grid_z_eval(const GridZ& g, double z, double B[3]) {
   double coord = z - g.trans;
@@ -306,7 +343,9 @@ __device__ __forceinline__ bool beta_at_least_one(double v1, double v3, double v
 // Equations of motion at time t [s] (the field models read t only when they
 // are time-dependent).  A state with |beta| >= 1 sets c.err and leaves d as it
 // was (src/OSCARSSR.cc:1260-1264); with electric fields the reference sets the
-// flag and still writes d.
+// flag and still writes d.  The branch-free single-undulator variants set the
+// flag and write d: a flagged step is discarded whole, so no kept value
+// depends on d there.
 template <int M>
 __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, const double* x, double* d) {
   if (M >= kAxisUndX && M <= kAxisUndZ) {
@@ -316,7 +355,7 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, 
     constexpr int K = M - kAxisUndX;
     const bool bad = beta_at_least_one(x[1], x[3], x[5]);
     double D = x[4] - f0.c2;
-    double sc = und_scale(f0, D, f0.psA);
+    double sc = und_scale(f0.thA, D);
     double a = K == 0 ? f0.a0 : (K == 1 ? f0.a1 : f0.a2);
     double s = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
     double v = sc != 0.0 ? a * sc * s : 0.0;
@@ -334,14 +373,45 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, 
       n3 = c.qomg * (-(x[1] * v));
       n5 = 0.0;
     }
-    // a state with |beta| >= 1 sets the flag and leaves d as it was
-    d[0] = bad ? d[0] : x[1];
-    d[1] = bad ? d[1] : n1;
-    d[2] = bad ? d[2] : x[3];
-    d[3] = bad ? d[3] : n3;
-    d[4] = bad ? d[4] : x[5];
-    d[5] = bad ? d[5] : n5;
-    c.err = bad ? 1 : c.err;
+    // A state with |beta| >= 1 sets the flag, which discards the whole step
+    // (rk4 redoes it by halving from its start), so d need not keep its old
+    // values as the reference's derivatives do: no kept result depends on them.
+    d[0] = x[1];
+    d[1] = n1;
+    d[2] = x[3];
+    d[3] = n3;
+    d[4] = x[5];
+    d[5] = n5;
+    c.err |= bad ? 1 : 0;
+    return;
+  }
+  if (M == kUndZ || M == kEpuZ) {
+    const bool bad = beta_at_least_one(x[1], x[3], x[5]);
+    double D = x[4] - f0.c2;
+    double scA = und_scale(f0.thA, D);
+    double sA = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
+    double v0 = f0.a0 * scA * sA, v1 = f0.a1 * scA * sA, v2 = f0.a2 * scA * sA;
+    if (M == kEpuZ) {
+      double scB = und_scale(f0.thB, D);
+      double sB = sin64(div_by(2.0 * kPi * (D - f0.psB), f0.PL, f0.inv_PL));
+      v0 = v0 + f0.b0 * scB * sB;
+      v1 = v1 + f0.b1 * scB * sB;
+      v2 = v2 + f0.b2 * scB * sB;
+    }
+    double B0 = 0.0 + v0, B1 = 0.0 + v1, B2 = 0.0 + v2;
+    double n1 = c.qomg * (-x[5] * B1 + x[3] * B2);
+    double n3 = c.qomg * (x[5] * B0 - x[1] * B2);
+    double n5 = c.qomg * (x[1] * B1 - x[3] * B0);
+    // A state with |beta| >= 1 sets the flag, which discards the whole step
+    // (rk4 redoes it by halving from its start), so d need not keep its old
+    // values as the reference's derivatives do: no kept result depends on them.
+    d[0] = x[1];
+    d[1] = n1;
+    d[2] = x[3];
+    d[3] = n3;
+    d[4] = x[5];
+    d[5] = n5;
+    c.err |= bad ? 1 : 0;
     return;
   }
   if (M == kFieldsEB) {

@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p
 // ONE: a single undulator/EPU arrives by value in the kernel arguments (read
 // through the scalar cache into registers); otherwise fields are read from F.
 template <int M>
-__global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const GridZ g, const FieldDev* __restrict__ F,
+__global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const GridZ g, const FieldDev* __restrict__ F,
                                                  int nfields, TrajDesc* __restrict__ desc, int np,
                                                  double* __restrict__ t, double* __restrict__ p, int nefields) {
   extern __shared__ double lds_grid[];
@@ -67,6 +67,9 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f0, const GridZ
     c.g.data = lds_grid;
   }
   if (ip >= np) return;
+  // the single undulator's constants, in vector registers (see in_vgpr)
+  const UndParams f0 = (M == kOneUndulator || (M >= kAxisUndX && M <= kAxisUndZ) || M == kUndZ || M == kEpuZ)
+                           ? und_in_vgpr(f_arg) : f_arg;
   TrajDesc D = desc[ip];
   c.qomg = D.qomg;
   c.qom = D.qom;
@@ -425,6 +428,16 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     u.a0 = f.field[0]; u.a1 = f.field[1]; u.a2 = f.field[2];
     u.b0 = f.fieldB[0]; u.b1 = f.fieldB[1]; u.b2 = f.fieldB[2];
     u.epu = f.type == om::kEPU;
+    for (int k = 0; k < 2; ++k) {
+      double ps = k == 0 ? u.psA : u.psB;
+      double* th = k == 0 ? u.thA : u.thB;
+      th[0] = -u.half_len + ps;
+      th[1] = u.half_len + ps;
+      th[2] = -u.half_len + ps + u.PL;
+      th[3] = u.half_len + ps - u.PL;
+      th[4] = -u.half_len + ps + u.half_PL;
+      th[5] = u.half_len + ps - u.half_PL;
+    }
     // planar undulator along z, untapered, field along one axis: the reduced equations
     int nz = (f.field[0] != 0) + (f.field[1] != 0) + (f.field[2] != 0);
     bool axis_z = f.unit[0] == 0 && f.unit[1] == 0 && f.unit[2] == 1;
@@ -438,6 +451,12 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
       else
         hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
+    } else if (axis_z && f.taper == 0 && f.type == om::kEPU) {
+      hipLaunchKernelGGL(om::rk4_kernel<om::kEpuZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
+                         p, 0);
+    } else if (axis_z && f.taper == 0) {
+      hipLaunchKernelGGL(om::rk4_kernel<om::kUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
+                         p, 0);
     } else {
       hipLaunchKernelGGL(om::rk4_kernel<om::kOneUndulator>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc,
                          np, t, p, 0);
