@@ -451,6 +451,87 @@ __device__ __forceinline__ void pd_term(const PdArgs& A, V3 obs, V3 nrm, const d
   }
 }
 
+// Per-point state of the power-density level loop.
+struct PdPoint {
+  double sum, lastsum, prec, e_run, b_run;
+  int lev;
+  bool border;
+};
+
+template <bool MIXED>
+__device__ __forceinline__ PdPoint pd_load(const PdArgs& A, long pc) {
+  PdPoint q;
+  q.sum = A.st.sum[6 * pc];
+  q.lastsum = A.st.lastmag[pc];
+  q.prec = A.st.prec[pc];
+  q.lev = A.st.level[pc];
+  q.e_run = MIXED ? A.st.err[2 * pc] : 0.0;
+  q.b_run = MIXED ? A.st.err[2 * pc + 1] : 0.0;
+  q.border = false;
+  return q;
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void pd_store(const PdArgs& A, long pc, const PdPoint& q) {
+  A.st.sum[6 * pc] = q.sum;
+  A.st.lastmag[pc] = q.lastsum;
+  A.st.prec[pc] = q.prec;
+  A.st.level[pc] = q.lev;
+  if (MIXED) {
+    A.st.err[2 * pc] = q.e_run;
+    A.st.err[2 * pc + 1] = q.b_run;
+    if (q.border) A.st.border[pc] = 1;
+  }
+}
+
+// End of level L for an active point: adds the level's partial sum ps (and, in
+// the mixed path, its bound terms), then the reference's stop test,
+// src/OSCARSSR.cc:2940-2953.  Returns true when the point stops (converged, or
+// a mixed decision within the error bound, to be recomputed in FP64).
+template <bool MIXED>
+__device__ __forceinline__ bool pd_level_end(const PdArgs& A, int L, double ps, double tsum, double qsum, double xmin,
+                                             PdPoint& q) {
+  long ns = 1L << L;
+  q.sum += ps;
+  q.lev = L;
+  double dt = A.lv.dt_incl[L];
+  double thisSum = q.sum * dt;
+  double prev = q.lastsum;
+  q.prec = fabs(thisSum - q.lastsum) / q.lastsum;
+  double bd = A.lv.beta_diff[L];
+  bool c = (L > 8 && q.prec < A.precision && bd < A.two_over_gamma) ||
+           (L > 8 && bd < A.two_over_gamma && thisSum == q.lastsum);
+  bool c2 = !c && L > 8 && thisSum == q.lastsum;
+  if (c2) q.prec = 0;
+  q.lastsum = thisSum;
+  if (MIXED) {
+    // error bounds of this level's and the last level's sums (the last from
+    // the running bounds as they stood before this level)
+    double dlast = L > 0 ? (q.e_run + 2.0 * (double) ns * kPdU * q.b_run) * A.lv.dt_incl[L - 1] +
+                               4 * kPdU * fabs(prev)
+                         : 0.0;
+    q.e_run += (kPdKx / xmin + kPdK0) * tsum + kPdKQ * qsum;
+    q.b_run += tsum;
+    double dthis = (q.e_run + 4.0 * (double) ns * kPdU * q.b_run) * dt + 4 * kPdU * fabs(thisSum);
+    // no error at all (every term so far skipped far from N.normal = 0):
+    // the sums are the FP64 path's, and so is the decision
+    if (L > 8 && dthis + dlast > 0.0) {
+      double tol = 1.25 * (dthis + dlast);
+      double den = prev - dlast;
+      bool amb_equal = fabs(thisSum - prev) <= tol;
+      bool amb_prec = bd < A.two_over_gamma &&
+                      (den <= 0.0 || fabs(q.prec - A.precision) <= 1.25 * (dthis + dlast * (1.0 + q.prec)) / den);
+      q.border = q.border || amb_equal || amb_prec;
+    }
+  }
+  return c || c2 || q.border;
+}
+
+__device__ __forceinline__ V3 pd_normal(const PdArgs& A, long pc) {
+  return A.normals ? v3(A.normals[3 * pc], A.normals[3 * pc + 1], A.normals[3 * pc + 2])
+                   : v3(A.normal[0], A.normal[1], A.normal[2]);
+}
+
 template <int S>
 struct PdLds {
   double ps[S][64];
@@ -476,19 +557,9 @@ __global__ void __launch_bounds__(64 * S) pd_point_kernel(PdArgs A) {
   }
   long pc = valid ? p : 0;
   V3 obs = v3(A.obs[3 * pc], A.obs[3 * pc + 1], A.obs[3 * pc + 2]);
-  V3 nrm = A.normals ? v3(A.normals[3 * pc], A.normals[3 * pc + 1], A.normals[3 * pc + 2])
-                    : v3(A.normal[0], A.normal[1], A.normal[2]);
-  int lev = A.st.level[pc];
-  bool active = valid && !A.st.conv[pc] && lev == A.La - 1;
-  double sum = A.st.sum[6 * pc];
-  double lastsum = A.st.lastmag[pc];
-  double prec = A.st.prec[pc];
-  double e_run = 0, b_run = 0;
-  bool border = false;
-  if (MIXED) {
-    e_run = A.st.err[2 * pc];
-    b_run = A.st.err[2 * pc + 1];
-  }
+  V3 nrm = pd_normal(A, pc);
+  PdPoint q = pd_load<MIXED>(A, pc);
+  bool active = valid && !A.st.conv[pc] && q.lev == A.La - 1;
   for (int L = A.La; L <= A.Lb; ++L) {
     if (S > 1 ? !__syncthreads_or(active) : !__any(active)) break;
     const double* __restrict__ rows = A.lv.rows[L];
@@ -520,57 +591,45 @@ __global__ void __launch_bounds__(64 * S) pd_point_kernel(PdArgs A) {
       }
       // the next level's __syncthreads_or keeps these reads ahead of its writes
     }
-    if (active) {
-      sum += ps;
-      lev = L;
-      double dt = A.lv.dt_incl[L];
-      double thisSum = sum * dt;
-      double prev = lastsum;
-      prec = fabs(thisSum - lastsum) / lastsum;
-      double bd = A.lv.beta_diff[L];
-      // src/OSCARSSR.cc:2940-2953
-      bool c = (L > 8 && prec < A.precision && bd < A.two_over_gamma) ||
-               (L > 8 && bd < A.two_over_gamma && thisSum == lastsum);
-      bool c2 = !c && L > 8 && thisSum == lastsum;
-      if (c2) prec = 0;
-      lastsum = thisSum;
+    if (active && pd_level_end<MIXED>(A, L, ps, tsum, qsum, xmin, q)) {
+      A.st.conv[pc] = 1;
+      active = false;
+    }
+  }
+  if (w == 0 && valid && q.lev >= A.La) pd_store<MIXED>(A, pc, q);
+}
+
+// One wave per listed point (few points left, or the mixed path's FP64
+// recheck): lane j sums samples j, j + 64, ... of each level, and the lanes'
+// partial sums are combined by a butterfly in a fixed order, so every lane
+// holds the same total and takes the same decision.
+template <bool MIXED>
+__global__ void __launch_bounds__(64) pd_wave_kernel(PdArgs A) {
+  const int lane = threadIdx.x;
+  long pc = A.list[blockIdx.x];
+  V3 obs = v3(A.obs[3 * pc], A.obs[3 * pc + 1], A.obs[3 * pc + 2]);
+  V3 nrm = pd_normal(A, pc);
+  PdPoint q = pd_load<MIXED>(A, pc);
+  if (A.st.conv[pc] || q.lev != A.La - 1) return;
+  for (int L = A.La; L <= A.Lb; ++L) {
+    const double* __restrict__ rows = A.lv.rows[L];
+    long ns = 1L << L;
+    double ps = 0, tsum = 0, qsum = 0, xmin = 2.0;
+    for (long i = lane; i < ns; i += 64) pd_term<MIXED>(A, obs, nrm, rows + 10 * i, ps, tsum, qsum, xmin);
+    for (int off = 32; off > 0; off >>= 1) {
+      ps += __shfl_xor(ps, off);
       if (MIXED) {
-        // error bounds of this level's and the last level's sums (the last from
-        // the running bounds as they stood before this level)
-        double dlast = L > 0 ? (e_run + 2.0 * (double) ns * kPdU * b_run) * A.lv.dt_incl[L - 1] +
-                                   4 * kPdU * fabs(prev)
-                             : 0.0;
-        e_run += (kPdKx / xmin + kPdK0) * tsum + kPdKQ * qsum;
-        b_run += tsum;
-        double dthis = (e_run + 4.0 * (double) ns * kPdU * b_run) * dt + 4 * kPdU * fabs(thisSum);
-        // no error at all (every term so far skipped far from N.normal = 0):
-        // the sums are the FP64 path's, and so is the decision
-        if (L > 8 && dthis + dlast > 0.0) {
-          double tol = 1.25 * (dthis + dlast);
-          double den = prev - dlast;
-          bool amb_equal = fabs(thisSum - prev) <= tol;
-          bool amb_prec = bd < A.two_over_gamma &&
-                          (den <= 0.0 || fabs(prec - A.precision) <= 1.25 * (dthis + dlast * (1.0 + prec)) / den);
-          border = border || amb_equal || amb_prec;
-        }
-      }
-      if (c || c2 || border) {
-        A.st.conv[pc] = 1;
-        active = false;
+        tsum += __shfl_xor(tsum, off);
+        qsum += __shfl_xor(qsum, off);
+        xmin = fmin(xmin, __shfl_xor(xmin, off));
       }
     }
-  }
-  if (w == 0 && valid && lev >= A.La) {
-    A.st.sum[6 * pc] = sum;
-    A.st.lastmag[pc] = lastsum;
-    A.st.prec[pc] = prec;
-    A.st.level[pc] = lev;
-    if (MIXED) {
-      A.st.err[2 * pc] = e_run;
-      A.st.err[2 * pc + 1] = b_run;
-      if (border) A.st.border[pc] = 1;
+    if (pd_level_end<MIXED>(A, L, ps, tsum, qsum, xmin, q)) {
+      if (lane == 0) A.st.conv[pc] = 1;
+      break;
     }
   }
+  if (lane == 0 && q.lev >= A.La) pd_store<MIXED>(A, pc, q);
 }
 
 // ---------------------------------------------------------------- power density on STL triangles
@@ -1156,8 +1215,15 @@ extern "C" hipError_t om_launch_tile_list(const om::State* st, long n_obs, long 
 }
 
 // Power density: S waves per 64 points, chosen so that the grid holds about
-// 8192 waves (at least 8 per SIMD) unless split > 0 names S.
+// 8192 waves (at least 8 per SIMD) unless split > 0 names S; a list of at most
+// kPdWaveListMax points runs one wave per point.
+constexpr long kPdWaveListMax = 8192;
 extern "C" hipError_t om_launch_pd_point(const om::PdArgs* A, int mixed, int split, hipStream_t s) {
+  if (A->list && A->n_list <= kPdWaveListMax && split <= 0) {  // few points: one wave each
+    if (mixed) hipLaunchKernelGGL(om::pd_wave_kernel<true>, dim3((unsigned) A->n_list), dim3(64), 0, s, *A);
+    else hipLaunchKernelGGL(om::pd_wave_kernel<false>, dim3((unsigned) A->n_list), dim3(64), 0, s, *A);
+    return hipGetLastError();
+  }
   long groups = ((A->list ? A->n_list : A->npts) + 63) / 64;
   if (split <= 0) {
     split = 1;
