@@ -334,22 +334,31 @@ def adjoint_leg(dev, steps=2):
     dev.trajectories([f], M.om_particles([p0], [(0.0, 2.0, 20001)]))
     _, lev, _, _, _ = dev.flux(0, obs, energies, [0.0] * 6, dev.conv(mode="tiled"))
     L = int(lev.max())
+    import torch
     prob = A.FieldMapProblem(dev, [0, 0, z0], [0, 0, dz], [1, 1, n], 3.0, [0, 0, -1], 0.0, 2.0, 20001, obs, energies, L)
-    f_ideal = prob.forward(ideal)
+    # samples, flux and gradients resident on the device (om_adjoint_*_dev)
+    err_t = torch.tensor(err, dtype=torch.float64, device="cuda")
+    f_ideal = prob.forward_dev(torch.tensor(ideal, dtype=torch.float64, device="cuda")).clone()
     tf = tb = 0.0
     for k in range(steps + 1):
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        F = prob.forward(err)
+        F = prob.forward_dev(err_t)
+        torch.cuda.synchronize()
         t1 = time.perf_counter()
-        gf, ga, gx = prob.backward(2.0 * (F - f_ideal))
+        gf, ga, gx = prob.backward_dev(2.0 * (F - f_ideal))
+        torch.cuda.synchronize()
         t2 = time.perf_counter()
         if k > 0:  # the first pass is warm-up
             tf += t1 - t0
             tb += t2 - t1
     tf /= steps
     tb /= steps
+    gf = gf.cpu().numpy()
+    ga = ga.cpu().numpy()
     evals = float(len(obs) * len(energies) * ((2 << L) - 1))
-    out = {"workload": "51x51 at 30 m x 4 energies, 4001-sample By map, 20001-point RK4, level %d" % L,
+    out = {"workload": "51x51 at 30 m x 4 energies, 4001-sample By map, 20001-point RK4, level %d; samples, flux "
+                       "and gradients as device tensors" % L,
            "evals_per_pass": evals, "forward_ms": 1e3 * tf, "backward_ms": 1e3 * tb,
            "forward_kernel_ms": prob.last_stats["forward"]["kernel_ms"],
            "backward_kernel_ms": prob.last_stats["backward"]["kernel_ms"],

@@ -339,6 +339,19 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
 int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad_field, double* grad_x0,
                         double* grad_beta0, om_stats* stats);
 
+/* om_adjoint_forward / om_adjoint_backward on device memory (no counterpart in
+ * the reference): obs_xyz, energies_eV, out_flux, dloss_dflux and every
+ * gradient are device pointers, so a caller holding its tensors in HBM (an
+ * autograd function) moves nothing over PCIe.  grad_samples: the samples of
+ * fields[grad_field] on the device (3 per sample, unrotated), used in place of
+ * that om_field's host samples (its grid_data may be NULL); NULL: the host
+ * samples.  The calls return once the results are written. */
+int om_adjoint_forward_dev(om_context* ctx, const om_field* fields, int nfields, int grad_field,
+                           const double* grad_samples, const om_particle* particle, const double* obs_xyz, long n_obs,
+                           const double* energies_eV, long n_e, int level, double* out_flux, om_stats* stats);
+int om_adjoint_backward_dev(om_context* ctx, const double* dloss_dflux, double* grad_field, double* grad_x0,
+                            double* grad_beta0, om_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,7 +66,7 @@ EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_batch_profile",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev",
-           "om_flux_dev_noalloc"]
+           "om_flux_dev_noalloc", "om_adjoint_forward_dev", "om_adjoint_backward_dev"]
 
 _lib = None
 
@@ -132,6 +132,10 @@ def lib():
     L.om_adjoint_forward.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.c_int, C.POINTER(om_particle),
                                      _dp, C.c_long, _dp, C.c_long, C.c_int, _dp, C.POINTER(om_stats)]
     L.om_adjoint_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(om_stats)]
+    L.om_adjoint_forward_dev.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.c_int, _vp,
+                                         C.POINTER(om_particle), _vp, C.c_long, _vp, C.c_long, C.c_int, _vp,
+                                         C.POINTER(om_stats)]
+    L.om_adjoint_backward_dev.argtypes = [C.c_void_p, _vp, _vp, _vp, _vp, C.POINTER(om_stats)]
     _lib = L
     return L
 
@@ -463,6 +467,35 @@ class Device:
         self._chk(self.L.om_adjoint_backward(self.h, _ptr(g), _ptr(gf) if n_grid > 0 else None, _ptr(gx), _ptr(gb),
                                              C.byref(st)))
         return gf[:n_grid], gx, gb, st.as_dict()
+
+    def adjoint_forward_dev(self, fields, grad_field, samples_t, particle, obs_t, e_t, level, out_t):
+        """om_adjoint_forward_dev on GPU tensors: the grid field's samples samples_t (n, 3) float64 (None: the
+        om_field's host samples), obs_t (n_obs, 3), e_t (n_e), out_t (n_obs * n_e).  Returns the stats."""
+        import torch
+        fa = (om_field * max(1, len(fields)))(*fields)
+        n_obs, n_e = obs_t.shape[0], e_t.numel()
+        ns = samples_t.numel() if samples_t is not None else 0
+        st = om_stats()
+        self._chk(self.L.om_adjoint_forward_dev(
+            self.h, fa, len(fields), int(grad_field), self._dev_ptr(samples_t, torch.float64, ns, "samples"),
+            C.byref(particle), self._dev_ptr(obs_t, torch.float64, 3 * n_obs, "obs"), n_obs,
+            self._dev_ptr(e_t, torch.float64, n_e, "energies"), n_e, int(level),
+            self._dev_ptr(out_t, torch.float64, n_obs * n_e, "out"), C.byref(st)))
+        self._adj_shape = (n_obs, n_e)
+        return st.as_dict()
+
+    def adjoint_backward_dev(self, dflux_t, gf_t, gx_t, gb_t):
+        """om_adjoint_backward_dev on GPU tensors: dflux_t (n_obs * n_e), gf_t (n_grid, 3) or None, gx_t and gb_t
+        (3) or None.  Returns the stats."""
+        import torch
+        n = self._adj_shape[0] * self._adj_shape[1]
+        st = om_stats()
+        self._chk(self.L.om_adjoint_backward_dev(
+            self.h, self._dev_ptr(dflux_t, torch.float64, n, "dloss_dflux"),
+            self._dev_ptr(gf_t, torch.float64, gf_t.numel() if gf_t is not None else 0, "grad_field"),
+            self._dev_ptr(gx_t, torch.float64, 3, "grad_x0"), self._dev_ptr(gb_t, torch.float64, 3, "grad_beta0"),
+            C.byref(st)))
+        return st.as_dict()
 
     def batch_tune(self, candidates=(2, 4, 8, 16, 32), sample_items=20000):
         c = np.ascontiguousarray(candidates, dtype=np.int32)

@@ -107,6 +107,7 @@ class FieldMapProblem:
         self.level = int(level)
         self.other = list(other_fields)
         self._jac = None
+        self._dev_inputs = None
         self.last_stats = {}
 
     def _particle(self, angles):
@@ -140,6 +141,36 @@ class FieldMapProblem:
         self.last_stats["backward"] = st
         return gf, gb @ self._jac, gx
 
+    # device-resident variants: float64 torch tensors on the GPU in and out
+    def forward_dev(self, samples_t, angles=(0.0, 0.0)):
+        """forward() with the samples (n, 3) as a GPU tensor; returns the flux
+        (n_obs, n_e) as a GPU tensor.  The two beam angles are host numbers
+        (they set the particle the trajectory is launched with)."""
+        import torch
+        f, _ = grid_field(*self.grid, np.zeros((0, 3)))
+        f.grid_data = None
+        part, jac = self._particle(angles)
+        out = torch.empty(len(self.obs) * len(self.energies), dtype=torch.float64, device=samples_t.device)
+        if self._dev_inputs is None or self._dev_inputs[0].device != samples_t.device:
+            self._dev_inputs = (torch.as_tensor(self.obs, device=samples_t.device),
+                                torch.as_tensor(self.energies, device=samples_t.device))
+        st = self.dev.adjoint_forward_dev([f] + self.other, 0, samples_t.contiguous(), part, self._dev_inputs[0],
+                                          self._dev_inputs[1], self.level, out)
+        self._jac = jac
+        self.last_stats["forward"] = st
+        return out.view(len(self.obs), len(self.energies))
+
+    def backward_dev(self, dflux_t):
+        """backward() on GPU tensors: (dLoss/dsamples (n, 3), dLoss/d(x', y') (2), dLoss/dx0 (3))."""
+        import torch
+        dev = dflux_t.device
+        gf = torch.empty((self.n_grid, 3), dtype=torch.float64, device=dev)
+        gx = torch.empty(3, dtype=torch.float64, device=dev)
+        gb = torch.empty(3, dtype=torch.float64, device=dev)
+        st = self.dev.adjoint_backward_dev(dflux_t.contiguous().view(-1), gf, gx, gb)
+        self.last_stats["backward"] = st
+        return gf, gb @ torch.as_tensor(self._jac, device=dev), gx
+
 
 try:
     import torch
@@ -149,14 +180,21 @@ try:
 
         @staticmethod
         def forward(ctx, samples, angles, problem):
-            flux = problem.forward(samples.detach().cpu().numpy(), angles.detach().cpu().numpy())
             ctx.problem = problem
             ctx.dev_samples = samples.device
             ctx.dev_angles = angles.device
+            ctx.on_gpu = samples.is_cuda
+            if samples.is_cuda:  # samples, flux and gradients stay on the device
+                return problem.forward_dev(samples.detach().to(torch.float64),
+                                           [float(a) for a in angles.detach().cpu().tolist()])
+            flux = problem.forward(samples.detach().cpu().numpy(), angles.detach().cpu().numpy())
             return torch.from_numpy(flux).to(samples.device)
 
         @staticmethod
         def backward(ctx, gout):
+            if ctx.on_gpu:
+                gf, ga, _ = ctx.problem.backward_dev(gout.detach().to(torch.float64))
+                return gf, ga.to(ctx.dev_angles), None
             gf, ga, _ = ctx.problem.backward(gout.detach().cpu().numpy())
             return (torch.from_numpy(np.ascontiguousarray(gf)).to(ctx.dev_samples),
                     torch.from_numpy(np.ascontiguousarray(ga)).to(ctx.dev_angles), None)

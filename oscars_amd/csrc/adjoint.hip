@@ -18,7 +18,9 @@
 //      lambda_{i0} = M lambda_{i1} + b (tangents through the four RK4 stages),
 //   B: one thread chains the sqrt(N) segment maps from the end,
 //   C: each segment recomputes its states again and sweeps back from its
-//      lambda_{i1}, scattering the field-sample gradients (FP64 atomics).
+//      lambda_{i1}, writing each field evaluation's grid-sample gradient as
+//      records in fixed slots; the records are sorted stably by sample and
+//      summed per sample in record order (bit-for-bit reproducible).
 // The sequential depth is about 3 K steps instead of 3 N.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
@@ -610,6 +612,24 @@ __global__ void __launch_bounds__(256) adj_fill_kernel(double* __restrict__ p, l
   if (i < n) p[i] = 0.0;
 }
 
+// dLoss/dS from dLoss/dF, F = C2 |E|^2 with E = C0 dt_L S:
+// g[6p + k] = (dl[p] f) E[6p + k], f = 2 C2 C0 dt_L.
+__global__ void __launch_bounds__(256) adj_dloss_kernel(const double* __restrict__ dl, const double* __restrict__ E,
+                                                        double f, double* __restrict__ g, long npts) {
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * npts) g[i] = dl[i / 6] * f * E[i];
+}
+
+// Gradients with respect to the initial position and beta from the adjoint
+// state at t0 (state velocity v = beta c); null outputs are skipped.
+__global__ void __launch_bounds__(64) adj_lam_out_kernel(const double* __restrict__ lam, double* __restrict__ gx0,
+                                                         double* __restrict__ gb0, double c) {
+  int k = threadIdx.x;
+  if (k >= 3) return;
+  if (gx0) gx0[k] = lam[2 * k];
+  if (gb0) gb0[k] = lam[2 * k + 1] * c;
+}
+
 // Record keys start at `v` (no contribution).
 __global__ void __launch_bounds__(256) adj_key_fill_kernel(int* __restrict__ k, long n, int v) {
   long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
@@ -690,6 +710,18 @@ extern "C" hipError_t om_launch_adj_fill(double* p, long n, hipStream_t s) {
   long blocks = (n + 255) / 256;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(om::adj_fill_kernel, dim3((unsigned) blocks), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_adj_dloss(const double* dl, const double* E, double f, double* g, long npts,
+                                          hipStream_t s) {
+  long blocks = (6 * npts + 255) / 256;
+  hipLaunchKernelGGL(om::adj_dloss_kernel, dim3((unsigned) blocks), dim3(256), 0, s, dl, E, f, g, npts);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t om_launch_adj_lam_out(const double* lam, double* gx0, double* gb0, double c, hipStream_t s) {
+  hipLaunchKernelGGL(om::adj_lam_out_kernel, dim3(1), dim3(64), 0, s, lam, gx0, gb0, c);
   return hipGetLastError();
 }
 

@@ -71,6 +71,8 @@ hipError_t om_launch_adj_spline(const double* rows, long ns, const double* sb, i
 hipError_t om_launch_adj_rk4(const om::AdjRkArgs* A, hipStream_t s);
 hipError_t om_launch_adj_fill(double* p, long n, hipStream_t s);
 hipError_t om_launch_adj_key_fill(int* k, long n, int v, hipStream_t s);
+hipError_t om_launch_adj_dloss(const double* dl, const double* E, double f, double* g, long npts, hipStream_t s);
+hipError_t om_launch_adj_lam_out(const double* lam, double* gx0, double* gb0, double c, hipStream_t s);
 hipError_t om_adj_grad_sort_bytes(long n, int end_bit, size_t* bytes);
 hipError_t om_launch_adj_grad_gather(const int* key, int* skey, int* sidx, const double* val, long n, int ngrid,
                                      int end_bit, void* temp, size_t temp_bytes, double* gbar, hipStream_t s);
@@ -237,6 +239,8 @@ struct om_context {
   void* scratch = nullptr;  // the device's static scratch (om_flux_dev_noalloc), looked up at creation
   size_t scratch_bytes = 0;
   int split = 0;  // waves per workgroup for the LW kernels (0: chosen from the grid size)
+  // device samples of a grid field whose om_field carries no host samples (om_adjoint_forward_dev)
+  const double* dev_grid_samples = nullptr;
   int batch_wpc = 0;  // persistent batch kernel: waves per CU chosen by om_batch_tune (0: default)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -310,13 +314,15 @@ struct om_context {
     int gfield = -1, L = 0;
     long n_obs = 0, n_e = 0;
     double C0 = 0, C2 = 0, dt_L = 0;
-    std::vector<double> E;  // 6 per output: C0 dt_L S (no polarization projection)
+    DevBuf<double> E;  // 6 per output: C0 dt_L S (no polarization projection), on the device
+    DevBuf<double> dl;  // dLoss/dflux uploaded from the host
     DevBuf<double> ckpt, rows, obs, omega, g, partial, sb, yb, yppb, Pw, ppw, sgw, ub, cc, segmap, lam, ybuf, gbar;
     DevBuf<int> klo;
     DevBuf<int> grec_key, grec_skey, grec_sidx;  // grid-gradient records and their stable sort
     DevBuf<double> grec_val;
     DevBuf<unsigned char> sort_tmp;
     void release() {
+      E.release(); dl.release();
       ckpt.release(); rows.release(); obs.release(); omega.release(); g.release(); partial.release();
       sb.release(); yb.release(); yppb.release(); Pw.release(); ppw.release(); sgw.release(); ub.release();
       cc.release();
@@ -447,27 +453,36 @@ om::FieldDev make_field(om_context* ctx, const om_field& f) {
     }
     long n = (long) nx * ny * nz;
     if (n < 2) throw std::invalid_argument("field grid needs at least two samples");
-    std::vector<double> data(3 * n);
-    for (long i = 0; i < n; ++i) {
-      double v[3] = {f.grid_data[3 * i], f.grid_data[3 * i + 1], f.grid_data[3 * i + 2]};
-      // samples are rotated by the same angles as the point (src/TField3D_Grid.cc:536-538)
-      double s = std::sin(f.rotations[0]), c = std::cos(f.rotations[0]), yy = v[1];
-      v[1] = c * yy - s * v[2];
-      v[2] = s * yy + c * v[2];
-      s = std::sin(f.rotations[1]); c = std::cos(f.rotations[1]);
-      double zz = v[2];
-      v[2] = c * zz - s * v[0];
-      v[0] = s * zz + c * v[0];
-      s = std::sin(f.rotations[2]); c = std::cos(f.rotations[2]);
-      double xx = v[0];
-      v[0] = c * xx - s * v[1];
-      v[1] = s * xx + c * v[1];
-      data[3 * i] = v[0]; data[3 * i + 1] = v[1]; data[3 * i + 2] = v[2];
-    }
     ctx->grid_bufs.emplace_back();
     DevBuf<double>& g = ctx->grid_bufs.back();
-    g.ensure(data.size());
-    check(hipMemcpy(g.p, data.data(), data.size() * sizeof(double), hipMemcpyHostToDevice), "grid upload");
+    g.ensure(3 * n);
+    if (!f.grid_data) {
+      // samples already on the device (om_adjoint_forward_dev), unrotated
+      if (!ctx->dev_grid_samples) throw std::invalid_argument("grid field without samples");
+      if (f.rotations[0] != 0 || f.rotations[1] != 0 || f.rotations[2] != 0)
+        throw std::invalid_argument("device grid samples cannot be rotated");
+      check(hipMemcpyAsync(g.p, ctx->dev_grid_samples, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
+            "grid copy");
+    } else {
+      std::vector<double> data(3 * n);
+      for (long i = 0; i < n; ++i) {
+        double v[3] = {f.grid_data[3 * i], f.grid_data[3 * i + 1], f.grid_data[3 * i + 2]};
+        // samples are rotated by the same angles as the point (src/TField3D_Grid.cc:536-538)
+        double s = std::sin(f.rotations[0]), c = std::cos(f.rotations[0]), yy = v[1];
+        v[1] = c * yy - s * v[2];
+        v[2] = s * yy + c * v[2];
+        s = std::sin(f.rotations[1]); c = std::cos(f.rotations[1]);
+        double zz = v[2];
+        v[2] = c * zz - s * v[0];
+        v[0] = s * zz + c * v[0];
+        s = std::sin(f.rotations[2]); c = std::cos(f.rotations[2]);
+        double xx = v[0];
+        v[0] = c * xx - s * v[1];
+        v[1] = s * xx + c * v[1];
+        data[3 * i] = v[0]; data[3 * i + 1] = v[1]; data[3 * i + 2] = v[2];
+      }
+      check(hipMemcpy(g.p, data.data(), data.size() * sizeof(double), hipMemcpyHostToDevice), "grid upload");
+    }
     d.data = g.p;
     for (int k = 0; k < 3; ++k) {
       d.rot[k] = f.rotations[k];
@@ -1833,9 +1848,12 @@ int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_ite
 }
 
 
-int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int grad_field,
-                       const om_particle* particle, const double* obs_xyz, long n_obs, const double* energies_eV,
-                       long n_e, int level, double* out_flux, om_stats* stats) {
+namespace {
+// dev_io: obs_xyz, energies_eV and out_flux are device pointers, and the grid
+// field's samples come from dev_samples (its om_field holds none).
+int adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int grad_field, const double* dev_samples,
+                    const om_particle* particle, const double* obs_xyz, long n_obs, const double* energies_eV, long n_e,
+                    int level, double* out_flux, om_stats* stats, bool dev_io) {
   int rc = guarded(ctx, [&] {
     ctx->adj.ready = false;
     if (!particle || !obs_xyz || !energies_eV || !out_flux) throw std::invalid_argument("missing adjoint inputs");
@@ -1859,13 +1877,19 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
     for (int i = 0; i < nfields; ++i)
       if (fields[i].frequency != 0 || (fields[i].type == OM_FIELD_GRID && fields[i].frequency_phase != 0))
         throw std::invalid_argument("the reverse pass supports time-independent fields only");
+    if (dev_samples && grad_field < 0) throw std::invalid_argument("device samples need a grad_field");
     ctx->adj.armed = true;
   });
   if (rc) return rc;
-  rc = om_trajectories(ctx, fields, nfields, particle, 1);
+  std::vector<om_field> fl(fields, fields + nfields);
+  if (dev_samples) {
+    fl[grad_field].grid_data = nullptr;
+    ctx->dev_grid_samples = dev_samples;
+  }
+  rc = om_trajectories(ctx, fl.data(), nfields, particle, 1);
+  ctx->dev_grid_samples = nullptr;
   ctx->adj.armed = false;
   if (rc) return rc;
-  std::vector<double> E(6 * n_obs * n_e);
   rc = guarded(ctx, [&] {
     sync_desc(ctx);
     if (ctx->desc[0].status != 0) throw std::out_of_range(kHalvingLimitMsg);
@@ -1873,6 +1897,7 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
       throw std::invalid_argument("the reverse pass does not follow halved RK4 steps (|beta| >= 1 was met)");
     if (ctx->desc[0].nb != 0)
       throw std::invalid_argument("the reverse pass needs ctstart == T0 (no backward integration)");
+    ctx->adj.E.ensure(6 * n_obs * n_e);
   });
   if (rc) return rc;
   om_conv c;
@@ -1884,8 +1909,15 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
   c.mode = OM_MODE_TILED;
   c.weight = 1;
   c.accumulate = 0;
-  rc = lw_sums(ctx, 0, obs_xyz, n_obs, energies_eV, n_e, nullptr, &c, out_flux, nullptr, nullptr, E.data(), stats,
-               false, false);
+  if (dev_io) {
+    rc = lw_sums(ctx, 0, obs_xyz, n_obs, energies_eV, n_e, nullptr, &c, out_flux, nullptr, nullptr, ctx->adj.E.p,
+                 stats, false, true);
+  } else {
+    std::vector<double> E(6 * n_obs * n_e);
+    rc = lw_sums(ctx, 0, obs_xyz, n_obs, energies_eV, n_e, nullptr, &c, out_flux, nullptr, nullptr, E.data(), stats,
+                 false, false);
+    if (!rc) rc = guarded(ctx, [&] { upload(ctx, ctx->adj.E, E.data(), E.size()); });
+  }
   if (rc) return rc;
   return guarded(ctx, [&] {
     auto& A = ctx->adj;
@@ -1899,20 +1931,43 @@ int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int
     A.L = level;
     A.n_obs = n_obs;
     A.n_e = n_e;
-    A.E.swap(E);
     A.C0 = P.q / (four_pi() * kC * kEps0 * kSqrt2Pi);
     A.C2 = four_pi() * P.current / (kH * std::fabs(P.q) * kMu0 * kC) * 1e-6 * 0.001;
     A.dt_L = dt_inclusive(ctx->tstart[0], ctx->tstop[0], level);
-    std::vector<double> om(n_e);
-    for (long i = 0; i < n_e; ++i) om[i] = ev_to_omega(energies_eV[i]);
-    upload(ctx, A.obs, obs_xyz, 3 * n_obs);
-    upload(ctx, A.omega, om.data(), n_e);
+    A.obs.ensure(3 * n_obs);
+    A.omega.ensure(n_e);
+    if (dev_io) {
+      check(hipMemcpyAsync(A.obs.p, obs_xyz, 3 * n_obs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "obs");
+      check(om_launch_omega_from_ev(energies_eV, n_e, A.omega.p, kQe, kHbar, ctx->stream), "omega");
+    } else {
+      std::vector<double> om(n_e);
+      for (long i = 0; i < n_e; ++i) om[i] = ev_to_omega(energies_eV[i]);
+      upload(ctx, A.obs, obs_xyz, 3 * n_obs);
+      upload(ctx, A.omega, om.data(), n_e);
+    }
     A.ready = true;
   });
 }
+}  // namespace
 
-int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad_field, double* grad_x0,
-                        double* grad_beta0, om_stats* stats) {
+int om_adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int grad_field,
+                       const om_particle* particle, const double* obs_xyz, long n_obs, const double* energies_eV,
+                       long n_e, int level, double* out_flux, om_stats* stats) {
+  return adjoint_forward(ctx, fields, nfields, grad_field, nullptr, particle, obs_xyz, n_obs, energies_eV, n_e, level,
+                         out_flux, stats, false);
+}
+
+int om_adjoint_forward_dev(om_context* ctx, const om_field* fields, int nfields, int grad_field,
+                           const double* grad_samples, const om_particle* particle, const double* obs_xyz, long n_obs,
+                           const double* energies_eV, long n_e, int level, double* out_flux, om_stats* stats) {
+  return adjoint_forward(ctx, fields, nfields, grad_field, grad_samples, particle, obs_xyz, n_obs, energies_eV, n_e,
+                         level, out_flux, stats, true);
+}
+
+namespace {
+// dev_io: dloss_dflux and every output are device pointers.
+int adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad_field, double* grad_x0,
+                     double* grad_beta0, om_stats* stats, bool dev_io) {
   return guarded(ctx, [&] {
     Timer wall;
     auto& A = ctx->adj;
@@ -1921,11 +1976,13 @@ int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad
     const long npts = A.n_obs * A.n_e;
     const om::TrajDesc& D = ctx->desc[0];
     // dLoss/dS from dLoss/dF: F = C2 |E|^2, E = C0 dt_L S
-    std::vector<double> g(6 * npts);
-    double f = 2.0 * A.C2 * A.C0 * A.dt_L;
-    for (long p = 0; p < npts; ++p)
-      for (int k = 0; k < 6; ++k) g[6 * p + k] = dloss_dflux[p] * f * A.E[6 * p + k];
-    upload(ctx, A.g, g.data(), g.size());
+    const double* dl = dloss_dflux;
+    if (!dev_io) {
+      upload(ctx, A.dl, dloss_dflux, npts);
+      dl = A.dl.p;
+    }
+    A.g.ensure(6 * npts);
+    check(om_launch_adj_dloss(dl, A.E.p, 2.0 * A.C2 * A.C0 * A.dt_L, A.g.p, npts, ctx->stream), "dloss");
     // level samples 0..L of particle 0, concatenated
     int stop_mem = A.L;
     ensure_levels(ctx, 0, A.L, stop_mem);
@@ -2014,22 +2071,31 @@ int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad
                                       end_bit, A.sort_tmp.p, A.sort_tmp.n, A.gbar.p, ctx->stream),
             "grid gradient");
     check(hipEventRecord(e1, ctx->stream), "event");
-    double lam0[6];
-    check(hipMemcpyAsync(lam0, A.lam.p, sizeof(lam0), hipMemcpyDeviceToHost, ctx->stream), "lambda");
-    if (grad_field && A.ngrid > 0)
-      check(hipMemcpyAsync(grad_field, A.gbar.p, 3 * A.ngrid * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
-            "grad field");
-    check(hipStreamSynchronize(ctx->stream), "sync");
-    if (grad_x0) {
-      grad_x0[0] = lam0[0];
-      grad_x0[1] = lam0[2];
-      grad_x0[2] = lam0[4];
-    }
-    // state velocity v = beta c
-    if (grad_beta0) {
-      grad_beta0[0] = lam0[1] * kC;
-      grad_beta0[1] = lam0[3] * kC;
-      grad_beta0[2] = lam0[5] * kC;
+    if (dev_io) {
+      if (grad_field && A.ngrid > 0)
+        check(hipMemcpyAsync(grad_field, A.gbar.p, 3 * A.ngrid * sizeof(double), hipMemcpyDeviceToDevice,
+                             ctx->stream),
+              "grad field");
+      check(om_launch_adj_lam_out(A.lam.p, grad_x0, grad_beta0, kC, ctx->stream), "initial-state gradients");
+      check(hipStreamSynchronize(ctx->stream), "sync");
+    } else {
+      double lam0[6];
+      check(hipMemcpyAsync(lam0, A.lam.p, sizeof(lam0), hipMemcpyDeviceToHost, ctx->stream), "lambda");
+      if (grad_field && A.ngrid > 0)
+        check(hipMemcpyAsync(grad_field, A.gbar.p, 3 * A.ngrid * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+              "grad field");
+      check(hipStreamSynchronize(ctx->stream), "sync");
+      if (grad_x0) {
+        grad_x0[0] = lam0[0];
+        grad_x0[1] = lam0[2];
+        grad_x0[2] = lam0[4];
+      }
+      // state velocity v = beta c
+      if (grad_beta0) {
+        grad_beta0[0] = lam0[1] * kC;
+        grad_beta0[1] = lam0[3] * kC;
+        grad_beta0[2] = lam0[5] * kC;
+      }
     }
     if (stats) {
       float ms = 0;
@@ -2041,6 +2107,17 @@ int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad
       stats->n_unconverged = 0;
     }
   });
+}
+}  // namespace
+
+int om_adjoint_backward(om_context* ctx, const double* dloss_dflux, double* grad_field, double* grad_x0,
+                        double* grad_beta0, om_stats* stats) {
+  return adjoint_backward(ctx, dloss_dflux, grad_field, grad_x0, grad_beta0, stats, false);
+}
+
+int om_adjoint_backward_dev(om_context* ctx, const double* dloss_dflux, double* grad_field, double* grad_x0,
+                            double* grad_beta0, om_stats* stats) {
+  return adjoint_backward(ctx, dloss_dflux, grad_field, grad_x0, grad_beta0, stats, true);
 }
 
 }  // extern "C"

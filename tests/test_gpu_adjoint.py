@@ -175,3 +175,35 @@ def test_adjoint_gradient_is_bitwise_reproducible(problem):
         out.append((gf.copy(), np.array(ga), np.array(gx)))
     for gf, ga, gx in out[1:]:
         assert np.array_equal(gf, out[0][0]) and np.array_equal(ga, out[0][1]) and np.array_equal(gx, out[0][2])
+
+
+@gpu
+def test_device_resident_adjoint_equals_host_path(problem):
+    """om_adjoint_forward_dev / om_adjoint_backward_dev (samples, flux and
+    gradients as GPU tensors) give the host path's values bit for bit, and the
+    autograd function on GPU tensors leaves its gradients on the GPU."""
+    import torch
+    from oscars_amd import adjoint as A
+    err, ideal = maps()
+    f_ideal = problem.forward(ideal, (0.0, 0.0))
+    ang = (2e-6, -1e-6)
+    F = problem.forward(err, ang)
+    gf, ga, gx = problem.backward(2.0 * (F - f_ideal))
+    s_t = torch.tensor(err, dtype=torch.float64, device="cuda")
+    F_t = problem.forward_dev(s_t, ang)
+    assert F_t.is_cuda and np.array_equal(F_t.cpu().numpy(), F)
+    gf_t, ga_t, gx_t = problem.backward_dev(2.0 * (F_t - torch.as_tensor(f_ideal, device="cuda")))
+    assert gf_t.is_cuda and ga_t.is_cuda and gx_t.is_cuda
+    assert np.array_equal(gf_t.cpu().numpy(), gf)
+    assert np.array_equal(gx_t.cpu().numpy(), gx)
+    assert np.allclose(ga_t.cpu().numpy(), ga, rtol=1e-14, atol=0)
+    # autograd on GPU tensors
+    s = s_t.clone().requires_grad_(True)
+    a = torch.tensor(ang, dtype=torch.float64, device="cuda", requires_grad=True)
+    Fa = A.FieldMapFlux.apply(s, a, problem)
+    assert Fa.is_cuda
+    ((Fa - torch.as_tensor(f_ideal, device="cuda")) ** 2).sum().backward()
+    assert s.grad.is_cuda and a.grad.is_cuda
+    assert np.array_equal(s.grad.cpu().numpy(), gf)
+    assert np.allclose(a.grad.cpu().numpy(), ga, rtol=1e-14, atol=0)
+    print("device-resident adjoint: flux and grid gradients bit-identical to the host path")
