@@ -491,9 +491,37 @@ def power_density_leg(steps=4):
     res["rel_err_mixed_vs_plain"] = rel_errors(vm, vp)
     res["level_mismatches"] = int(np.sum(lm != lp))
     res["kernel_ratio_mixed_vs_plain"] = res["plain"]["kernel_ms"] / res["mixed"]["kernel_ms"]
+    res["stl"] = stl_leg()
     res["workload"] = ("EPU 0.6 T x 0.6 T, 40 x 57 mm, 3 GeV, 40001-point RK4; power density 401x401 "
                        "(+-40 mm @20 m)")
     return res
+
+
+def stl_leg(subdiv=4):
+    """Power density on the centres of a 5120-triangle sphere (radius 10 mm at
+    z = 5 m, the layout of the reference's CAD example) with occlusion, from
+    the planar undulator's trajectory: the bounding-volume hierarchy against
+    testing every triangle for every ray (the reference's loop)."""
+    from oscars_amd import geometry as G
+    from oscars_amd.sr import sr
+    s = sr()
+    map_scenario(s)
+    s._ensure_trajectory()
+    dev = s._device()
+    tri = G.stl_rows(G.icosphere(0.010, subdiv, (0.0, 0.0, 5.0)))
+    out = {}
+    for mode in ("tiled", "plain"):
+        dev.power_density_stl(0, tri, dev.conv(mode=mode))  # warm-up
+        v, lev, _, st = dev.power_density_stl(0, tri, dev.conv(mode=mode))
+        out[mode] = (v, lev, st)
+        log("stl %s: %d triangles, kernels %.2f ms" % (mode, len(tri), st["kernel_ms"]))
+    return {"triangles": len(tri), "bvh_kernel_ms": out["tiled"][2]["kernel_ms"],
+            "every_triangle_kernel_ms": out["plain"][2]["kernel_ms"],
+            "speedup": out["plain"][2]["kernel_ms"] / out["tiled"][2]["kernel_ms"],
+            "identical": bool(np.array_equal(out["tiled"][0], out["plain"][0]) and
+                              np.array_equal(out["tiled"][1], out["plain"][1])),
+            "workload": "planar undulator 1 T / 49 mm / 31 periods, 20001-point RK4; sphere of 10 mm at z = 5 m, "
+                        "%d triangles" % len(tri)}
 
 
 # ---------------------------------------------------------------- counter runs

@@ -10,10 +10,12 @@
 //   grid header handling src/TField3D_Grid.cc:413-437
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -264,6 +266,8 @@ struct om_context {
   DevBuf<unsigned long long> d_stats;
   DevBuf<double> d_bd;
   DevBuf<double> d_normals, d_tp, d_tp_partial, d_et, d_stl, d_fp, d_fp_out;
+  DevBuf<double> d_bvh_box;
+  DevBuf<int> d_bvh_meta, d_bvh_order;
   DevBuf<om::FieldDev> d_fp_fields;
   std::vector<om::FieldDev> fields_host;
   int nfields = 0;
@@ -356,6 +360,7 @@ struct om_context {
     d_conv.release(); d_count.release(); d_outlevel.release(); d_outconv.release(); d_bd.release();
     d_list.release(); d_border.release();
     d_normals.release(); d_tp.release(); d_tp_partial.release(); d_et.release(); d_stl.release(); d_fp.release(); d_fp_out.release(); d_fp_fields.release();
+    d_bvh_box.release(); d_bvh_meta.release(); d_bvh_order.release();
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) (void) hipStreamDestroy(stream);
@@ -1378,6 +1383,92 @@ int om_power_density_surface(om_context* ctx, int ip, const double* obs_xyz, lon
                        false, normals);
 }
 
+namespace {
+// Bounding-volume hierarchy over STL triangles for the occlusion test of
+// om_power_density_stl, in depth-first order: an inner node's first child
+// follows it, and meta[0] names the node to visit when a node is missed or
+// done.  Leaves hold at most kBvhLeaf triangles (listed in `order`).
+//
+// The boxes are inflated so that the hierarchy never drops a triangle the
+// reference's test (Moller and Trumbore, src/TTriangle3D.cc:159-215) would
+// accept.  That test accepts |det| >= 1e-12 m^2; its barycentric u is
+// tv.p / det with |tv| <= Rmax (the largest distance from a trajectory point
+// to the mesh) and |p| <= |e2|, so rounding moves u by at most
+// 3 u_r Rmax |e2| / 1e-12 (v likewise with |e1|), and the accepted point by at
+// most 6.6e-4 Rmax |e1| |e2| per m^2.  Each box is inflated by 4 times that,
+// plus 1e-9 of its coordinates, and the segment is extended at both ends by
+// the largest inflation.  The inflation costs only extra tests.
+constexpr int kBvhLeaf = 4;
+
+struct StlBvh {
+  std::vector<double> box;  // 6 per node
+  std::vector<int> meta;    // 3 per node
+  std::vector<int> order;
+  double pad_t = 0;
+};
+
+void stl_bvh_build(const double* tri, long n, double rmax, bool one_leaf, StlBvh& h) {
+  std::vector<double> lo(3 * n), hi(3 * n), mid(3 * n);
+  double pad_max = 0;
+  for (long i = 0; i < n; ++i) {
+    const double* t = tri + 12 * i;
+    double e1 = std::sqrt(t[3] * t[3] + t[4] * t[4] + t[5] * t[5]);
+    double e2 = std::sqrt(t[6] * t[6] + t[7] * t[7] + t[8] * t[8]);
+    double pad_rel = 4 * 6.6e-4 * rmax * e1 * e2;
+    for (int k = 0; k < 3; ++k) {
+      double a = t[k], b = t[k] + t[3 + k], c = t[k] + t[6 + k];
+      double mn = std::min(a, std::min(b, c)), mx = std::max(a, std::max(b, c));
+      double pad = pad_rel + 1e-9 * std::max(std::fabs(mn), std::fabs(mx)) + 1e-12;
+      pad_max = std::max(pad_max, pad);
+      lo[3 * i + k] = mn - pad;
+      hi[3 * i + k] = mx + pad;
+      mid[3 * i + k] = 0.5 * (mn + mx);
+    }
+  }
+  h.pad_t = pad_max + 1e-9 * rmax;
+  h.order.resize(n);
+  for (long i = 0; i < n; ++i) h.order[i] = (int) i;
+  h.box.clear();
+  h.meta.clear();
+  std::function<void(long, long)> build = [&](long first, long count) {
+    long node = (long) h.meta.size() / 3;
+    double bl[3] = {1e300, 1e300, 1e300}, bh[3] = {-1e300, -1e300, -1e300};
+    double cl[3] = {1e300, 1e300, 1e300}, ch[3] = {-1e300, -1e300, -1e300};
+    for (long k = first; k < first + count; ++k) {
+      int j = h.order[k];
+      for (int a = 0; a < 3; ++a) {
+        bl[a] = std::min(bl[a], lo[3 * j + a]);
+        bh[a] = std::max(bh[a], hi[3 * j + a]);
+        cl[a] = std::min(cl[a], mid[3 * j + a]);
+        ch[a] = std::max(ch[a], mid[3 * j + a]);
+      }
+    }
+    for (int a = 0; a < 3; ++a) h.box.push_back(bl[a]);
+    for (int a = 0; a < 3; ++a) h.box.push_back(bh[a]);
+    h.meta.push_back(0);
+    h.meta.push_back((int) first);
+    h.meta.push_back(0);
+    if (one_leaf || count <= kBvhLeaf) {
+      h.meta[3 * node + 2] = (int) count;
+    } else {
+      int axis = 0;
+      for (int a = 1; a < 3; ++a)
+        if (ch[a] - cl[a] > ch[axis] - cl[axis]) axis = a;
+      long half = count / 2;
+      std::nth_element(h.order.begin() + first, h.order.begin() + first + half, h.order.begin() + first + count,
+                       [&](int x, int y) {
+                         double cx = mid[3 * x + axis], cy = mid[3 * y + axis];
+                         return cx < cy || (cx == cy && x < y);
+                       });
+      build(first, half);
+      build(first + half, count - half);
+    }
+    h.meta[3 * node] = (int) (h.meta.size() / 3);  // the node after this subtree
+  };
+  build(0, n);
+}
+}  // namespace
+
 int om_power_density_stl(om_context* ctx, int ip, const double* tri15, long n, const om_conv* conv, double* out_value,
                          int* out_level, unsigned char* out_conv, om_stats* stats) {
   return guarded(ctx, [&] {
@@ -1403,12 +1494,41 @@ int om_power_density_stl(om_context* ctx, int ip, const double* tri15, long n, c
     }
     upload(ctx, ctx->d_stl, tri.data(), tri.size());
     upload(ctx, ctx->d_obs, centre.data(), centre.size());
+    // Rmax: a bound on the distance from any trajectory point (within
+    // |ctstop - ctstart| of x0, the speed being below c) to any mesh point
+    const om_particle& Pp = ctx->parts[ip];
+    double mlo[3] = {1e300, 1e300, 1e300}, mhi[3] = {-1e300, -1e300, -1e300};
+    for (long i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k)
+        for (int v = 0; v < 3; ++v) {
+          double c = tri15[15 * i + 3 * v + k];
+          mlo[k] = std::min(mlo[k], c);
+          mhi[k] = std::max(mhi[k], c);
+        }
+    double far2 = 0, diag2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      double d = std::max(std::fabs(Pp.x0[k] - mlo[k]), std::fabs(Pp.x0[k] - mhi[k]));
+      far2 += d * d;
+      diag2 += (mhi[k] - mlo[k]) * (mhi[k] - mlo[k]);
+    }
+    double rmax = std::sqrt(far2) + std::fabs(Pp.ctstop - Pp.ctstart) + std::sqrt(diag2);
+    // PLAIN: one leaf, every triangle tested for every ray as in the reference
+    StlBvh bvh;
+    stl_bvh_build(tri.data(), n, rmax, conv->mode == OM_MODE_PLAIN, bvh);
+    upload(ctx, ctx->d_bvh_box, bvh.box.data(), bvh.box.size());
+    upload(ctx, ctx->d_bvh_meta, bvh.meta.data(), bvh.meta.size());
+    upload(ctx, ctx->d_bvh_order, bvh.order.data(), bvh.order.size());
     om::StlArgs A;
     std::memset(&A, 0, sizeof(A));
     state_for(ctx, n, false, A.st);
     A.ntri = n;
     A.tri = ctx->d_stl.p;
     A.centre = ctx->d_obs.p;
+    A.bvh_box = ctx->d_bvh_box.p;
+    A.bvh_meta = ctx->d_bvh_meta.p;
+    A.bvh_order = ctx->d_bvh_order.p;
+    A.bvh_nodes = (int) (bvh.meta.size() / 3);
+    A.pad_t = bvh.pad_t;
     A.precision = conv->precision;
     A.two_over_gamma = 2. / (ctx->parts[ip].gamma);
     int La = 0;

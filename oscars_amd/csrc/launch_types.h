@@ -66,6 +66,13 @@ struct StlArgs {
   const double* centre;  // 3 per triangle
   double precision;
   double two_over_gamma;
+  // bounding-volume hierarchy in depth-first order (one leaf with every
+  // triangle: the reference's test of every triangle for every ray)
+  const double* bvh_box;  // 6 per node: low x y z, high x y z (inflated)
+  const int* bvh_meta;    // 3 per node: next node when missed or done, first listed triangle, count (0: inner)
+  const int* bvh_order;   // triangle indices in leaf order
+  int bvh_nodes;
+  double pad_t;           // slack of the ray segment at both ends [m]
 };
 
 // Total power: per-level sums of the integrand over the level's samples.

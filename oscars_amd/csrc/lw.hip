@@ -633,13 +633,16 @@ __global__ void __launch_bounds__(64) pd_wave_kernel(PdArgs A) {
 }
 
 // ---------------------------------------------------------------- power density on STL triangles
-// src/OSCARSSR.cc:3318-3527.  One wave per triangle centre; the wave walks the
-// level's samples in order.  For each sample the 64 lanes test the other
-// triangles, 64 at a time, for a hit between the sample and the centre
-// (src/TTriangle3D.cc:159-215, Moller and Trumbore, same operations); the
-// sample is skipped when any lane finds one.  Unblocked samples add the power
-// density integrand with |N1 . normal| and move the beta-step maximum, as in
-// the reference, so every lane holds the same sum and the lane 0 stores it.
+// src/OSCARSSR.cc:3318-3527.  One wave per triangle centre; its lanes take 64
+// consecutive samples of a level at a time.  Each lane asks whether any other
+// triangle lies between its sample and the centre (src/TTriangle3D.cc:159-215,
+// Moller and Trumbore, same operations), walking a bounding-volume hierarchy
+// whose boxes are inflated by more than the test's rounding can move an
+// accepted intersection, so it finds a hit exactly when the test of every
+// triangle would.  Unblocked samples add the power density integrand with
+// |N1 . normal| and move the beta-step maximum, the step taken from the last
+// unblocked sample before (the lane's ballot gives it), as in the reference;
+// the 64 terms are added by a butterfly in a fixed order.
 constexpr int kStlWaves = 4;
 
 __device__ __forceinline__ bool stl_blocks(const double* __restrict__ T, V3 O, V3 D, double dist) {
@@ -662,6 +665,59 @@ __device__ __forceinline__ bool stl_blocks(const double* __restrict__ T, V3 O, V
   return t > 0 && t < dist;
 }
 
+// Does the segment X + t D, t in [t0, t1], meet the box?  An axis the
+// direction does not move along is a plain containment test.
+__device__ __forceinline__ bool stl_box_hit(const double* __restrict__ b, V3 X, V3 D, V3 inv, double t0, double t1) {
+  double lo[3] = {b[0], b[1], b[2]}, hi[3] = {b[3], b[4], b[5]};
+  double x[3] = {X.x, X.y, X.z}, d[3] = {D.x, D.y, D.z}, iv[3] = {inv.x, inv.y, inv.z};
+  bool out = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double ta = (lo[a] - x[a]) * iv[a], tb = (hi[a] - x[a]) * iv[a];
+    bool flat = d[a] == 0.0;
+    out = out || (flat && (x[a] < lo[a] || x[a] > hi[a]));
+    t0 = flat ? t0 : fmax(t0, fmin(ta, tb));
+    t1 = flat ? t1 : fmin(t1, fmax(ta, tb));
+  }
+  return !out && t0 <= t1;
+}
+
+// Any triangle other than `self` between X and the point at distance dist
+// along the unit direction N1.
+__device__ __forceinline__ bool stl_any_hit(const StlArgs& A, V3 X, V3 N1, double dist, long self) {
+  V3 inv = v3(1.0 / N1.x, 1.0 / N1.y, 1.0 / N1.z);
+  int i = 0;
+  while (i < A.bvh_nodes) {
+    const int* m = A.bvh_meta + 3 * i;
+    int next = m[0];
+    if (!stl_box_hit(A.bvh_box + 6 * i, X, N1, inv, -A.pad_t, dist + A.pad_t)) {
+      i = next;
+      continue;
+    }
+    int cnt = m[2];
+    if (cnt == 0) {  // inner node: its first child follows it
+      ++i;
+      continue;
+    }
+    for (int k = 0; k < cnt; ++k) {
+      int j = A.bvh_order[m[1] + k];
+      if (j != self && stl_blocks(A.tri + 12 * j, X, N1, dist)) return true;
+    }
+    i = next;
+  }
+  return false;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  return v;
+}
+
 __global__ void __launch_bounds__(64 * kStlWaves) pd_stl_kernel(StlArgs A) {
   const int lane = threadIdx.x & 63;
   const long p = (long) blockIdx.x * kStlWaves + (threadIdx.x >> 6);
@@ -677,38 +733,44 @@ __global__ void __launch_bounds__(64 * kStlWaves) pd_stl_kernel(StlArgs A) {
   for (int L = A.La; L <= A.Lb && !done; ++L) {
     const double* __restrict__ rows = A.lv.rows[L];
     const long ns = 1L << L;
-    V3 lastB = v3(0, 0, 0);
+    V3 lastB = v3(0, 0, 0);  // the last unblocked sample's beta (none yet: zero)
     double bdmax = -1;
-    for (long i = 0; i < ns; ++i) {
-      const double* s = rows + 10 * i;
+    for (long base = 0; base < ns; base += 64) {
+      const long i = base + lane;
+      const bool valid = i < ns;
+      const double* s = rows + 10 * (valid ? i : 0);
       V3 X = v3(s[1], s[2], s[3]);
       V3 R = sub(obs, X);
       double r2 = mag2(R);
       double dist = sqrt(r2);
       V3 N1 = v3(R.x / dist, R.y / dist, R.z / dist);
-      bool blocked = false;
-      for (long base = 0; base < A.ntri; base += 64) {
-        long j = base + lane;
-        bool hit = j < A.ntri && j != p && stl_blocks(A.tri + 12 * j, X, N1, dist);
-        if (__any(hit)) {
-          blocked = true;
-          break;
-        }
-      }
-      if (blocked) continue;
+      const bool blocked = !valid || stl_any_hit(A, X, N1, dist, p);
+      const unsigned long long open = __ballot(!blocked);
       V3 B = v3(s[4], s[5], s[6]);
       V3 Ac = v3(s[7], s[8], s[9]);
-      V3 dB = sub(B, lastB);
-      double bd = sqrt(mag2(dB));
-      if (i > 0 && bd > bdmax) bdmax = bd;
-      lastB = B;
-      double ndn = fabs(dot(N1, nrm));
-      V3 Num = cross(N1, cross(sub(N1, B), Ac));
-      double x = 1 - dot(B, N1);
-      double x2 = x * x;
-      double den = x2 * x2 * x;
-      // (Num.N2)^2 + (Num.N3)^2 = |Num|^2 because Num is perpendicular to N1
-      sum += mag2(Num) / den / r2 * ndn;
+      // the last unblocked sample before this one: in this group, else carried
+      const unsigned long long below = lane ? open & ((1ull << lane) - 1) : 0ull;
+      const int src = below ? 63 - __clzll(below) : lane;
+      V3 pB = v3(__shfl(B.x, src), __shfl(B.y, src), __shfl(B.z, src));
+      V3 prevB = below ? pB : lastB;
+      double term = 0, bd = -1;
+      if (!blocked) {
+        V3 dB = sub(B, prevB);
+        bd = i > 0 ? sqrt(mag2(dB)) : -1.0;
+        double ndn = fabs(dot(N1, nrm));
+        V3 Num = cross(N1, cross(sub(N1, B), Ac));
+        double x = 1 - dot(B, N1);
+        double x2 = x * x;
+        double den = x2 * x2 * x;
+        // (Num.N2)^2 + (Num.N3)^2 = |Num|^2 because Num is perpendicular to N1
+        term = mag2(Num) / den / r2 * ndn;
+      }
+      sum += wave_sum(term);
+      bdmax = fmax(bdmax, wave_max(bd));
+      if (open) {
+        const int last = 63 - __clzll(open);
+        lastB = v3(__shfl(B.x, last), __shfl(B.y, last), __shfl(B.z, last));
+      }
     }
     lev = L;
     double thisSum = sum * A.lv.dt_incl[L];

@@ -300,3 +300,50 @@ def polarization_vector(polarization, angle, hdir, pdir):
             out[2 * k] = re * r2
             out[2 * k + 1] = im * r2
     return out
+
+
+def icosphere(radius, subdiv, centre=(0.0, 0.0, 0.0)):
+    """Triangles (normal, a, b, c) of an icosahedron subdivided `subdiv` times
+    and pushed out to a sphere of `radius` (20 * 4^subdiv triangles, outward
+    unit normals): a synthetic closed mesh for calculate_power_density_stl."""
+    import numpy as np
+    t = (1 + 5 ** 0.5) / 2
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    v = [np.array(p, dtype=float) / np.linalg.norm(p) for p in v]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+         [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+         [8, 6, 7], [9, 8, 1]]
+    for _ in range(subdiv):
+        cache = {}
+
+        def middle(i, j):
+            k = (min(i, j), max(i, j))
+            if k not in cache:
+                m = (v[i] + v[j]) / 2
+                v.append(m / np.linalg.norm(m))
+                cache[k] = len(v) - 1
+            return cache[k]
+
+        nf = []
+        for a, b, c in f:
+            ab, bc, ca = middle(a, b), middle(b, c), middle(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    P = np.array(v) * radius + np.asarray(centre, dtype=float)
+    out = []
+    for a, b, c in f:
+        n = np.cross(P[b] - P[a], P[c] - P[a])
+        out.append((list(n / np.linalg.norm(n)), list(P[a]), list(P[b]), list(P[c])))
+    return out
+
+
+def stl_rows(triangles):
+    """tri[n, 15] (A, B, C, normal, centre (A + B + C) / 3) of (normal, a, b, c)
+    triangles, the layout read_stl gives, for Device.power_density_stl."""
+    import numpy as np
+    rows = []
+    for n, a, b, c in triangles:
+        a, b, c = np.asarray(a, float), np.asarray(b, float), np.asarray(c, float)
+        rows.append(np.concatenate([a, b, c, np.asarray(n, float), (a + b + c) / 3.]))
+    return np.array(rows).reshape(-1, 15)

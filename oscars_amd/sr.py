@@ -1132,12 +1132,16 @@ class sr:
 
     def calculate_power_density_stl(self, farfield_origin=None, ofile="", bofile="", stlofile="", nparticles=0,
                                     gpu=-1, ngpu=None, nthreads=0, precision=0.01, max_level=-2,
-                                    max_level_extended=0, quantity="power density"):
+                                    max_level_extended=0, quantity="power density", mode="tiled"):
         """Power density [W/mm^2] at the centre of every STL triangle, with the
         samples whose path to the centre meets another triangle left out;
         returns [[[A, B, C], pd], ...] (src/OSCARSSR_Python.cc:6993-7213).
         ``farfield_origin`` is checked and unused, and ``ofile`` / ``bofile``
-        write nothing, as in the reference; ``stlofile`` writes the triangles."""
+        write nothing, as in the reference; ``stlofile`` writes the triangles.
+        Extension: mode "plain" tests every triangle for every ray, as the
+        reference does; any other mode (default) walks a bounding-volume
+        hierarchy that finds the same hits, so the results are the same."""
+        _check_pd_mode(mode)
         if self._beam is None:
             raise ValueError("No particle beam defined")
         if farfield_origin is not None:
@@ -1149,7 +1153,7 @@ class sr:
         tri = np.concatenate([t for _, t, _ in self._stl])
 
         def run(dev, ip, w, acc, out=None):
-            conv = dev.conv(precision, max_level, max_level_extended, q, "plain", w, acc)
+            conv = dev.conv(precision, max_level, max_level_extended, q, mode, w, acc)
             v, lev, cv, st = dev.power_density_stl(ip, tri, conv, out=out)
             self.last_stats = st
             self._last_levels = lev

@@ -88,6 +88,12 @@ def write_stl_triangles(path, tris, header=b"test mesh"):
         f.write(rec.tobytes())
 
 
+def stl_icosphere(radius, subdiv, centre=(0.0, 0.0, 0.0)):
+    """Triangles (normal, a, b, c) of a subdivided icosahedron on a sphere."""
+    from oscars_amd import geometry as G
+    return G.icosphere(radius, subdiv, centre)
+
+
 def stl_scene(dirname):
     """A screen of 36 triangles at 20 m with a 4-triangle blocker at 10 m in
     front of part of it (coordinates in mm, scale 1e-3).  Returns the two

@@ -172,3 +172,45 @@ def test_three_corner_rectangles():
     e2 = rel_err(np.array([r[1] for r in res]), fv, 1e-6)
     print("three-corner surfaces: power density %.3e, flux %.3e" % (e, e2))
     assert e < 1e-9 and e2 < 1e-9
+
+
+@gpu
+@pytest.mark.parametrize("scene", ["sphere", "grazing"])
+def test_power_density_stl_hierarchy_equals_every_triangle(tmp_path, scene):
+    """The bounding-volume hierarchy finds the same hits as testing every
+    triangle for every ray (mode 'plain', the reference's loop): values and
+    levels bit for bit.  'sphere': a 1280-triangle sphere of 10 mm at 5 m
+    (the layout of the reference's CAD example), its far side shadowed by its
+    near side; 'grazing': a plate in the beam's plane beside a screen, its
+    triangles nearly parallel to the rays that pass them."""
+    import time
+    from oscars_amd.sr import sr
+    from tests.common import stl_icosphere, stl_plate, write_stl_triangles
+    s = sr()
+    planar(s, npoints=4001)
+    if scene == "sphere":
+        path = str(tmp_path / "sphere.stl")
+        write_stl_triangles(path, stl_icosphere(10.0, 3))
+        s.add_stl(path, scale=1e-3, translation=[0, 0, 5])
+    else:
+        side = [(n, [a[0], a[2], a[1]], [b[0], b[2], b[1]], [c[0], c[2], c[1]])
+                for n, a, b, c in stl_plate(-4, 4, 1000, 20000, 0.3, 8, 40)]  # the plane y = 0.3 mm, along z
+        screen = stl_plate(-6, 6, -3, 3, 20000, 12, 6)
+        path = str(tmp_path / "grazing.stl")
+        write_stl_triangles(path, side + screen)
+        s.add_stl(path, scale=1e-3)
+    out = {}
+    for mode in ("plain", "tiled"):
+        t0 = time.perf_counter()
+        res = s.calculate_power_density_stl(max_level=11, mode=mode)
+        dt = time.perf_counter() - t0
+        out[mode] = (np.array([r[1] for r in res]), s._last_levels.copy(), s.last_stats["kernel_ms"])
+        print("%s, %d triangles, %s: %.1f ms (kernels %.1f ms)" % (scene, len(res), mode, 1e3 * dt,
+                                                                    out[mode][2]))
+    v0, l0, _ = out["plain"]
+    v1, l1, _ = out["tiled"]
+    shadowed = int(np.sum(v0 == 0))
+    print("%s: %d of %d centres see no unblocked sample" % (scene, shadowed, len(v0)))
+    assert np.array_equal(l0, l1)
+    assert np.array_equal(v0, v1)
+    assert 0 < np.count_nonzero(v0) < len(v0) or scene == "grazing"
