@@ -149,7 +149,10 @@ def batched_headline(args, cfg):
     # the persistent grid is tuned on the whole workload: a prefix of the work
     # units is not representative of the mix of request sizes
     best, tt = bf.tune(candidates=(8, 16), sample_items=1 << 40)
-    log("setup + tuner %.1f s: %d waves/CU %s" % (time.perf_counter() - t0, best, tt))
+    # then the level from which the points still running are handed over to
+    # one workgroup each (0: none), on the whole launch
+    deep, td = bf.tune_deep((0, 13, 15))
+    log("setup + tuner %.1f s: %d waves/CU %s, hand-over level %d %s" % (time.perf_counter() - t0, best, tt, deep, td))
     for k in range(args.warmup):
         _, st = bf.run(recompute_trajectories=True)
         log("warmup step %d: %.1f ms" % (k, st["kernel_ms"]))
@@ -165,7 +168,8 @@ def batched_headline(args, cfg):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     res = dict(value=evals / dt, ms=1e3 * dt / args.steps, evals_per_step=evals / args.steps, outs=outs,
-               unconverged=st["n_unconverged"], rechecked=st["n_rechecked"], waves_per_cu=best)
+               unconverged=st["n_unconverged"], rechecked=st["n_rechecked"], waves_per_cu=best,
+               tuning={"waves_per_cu_ms": tt, "hand_over_level_ms": td, "hand_over_level": deep})
     # one more step launched stage by stage with HIP events around each kernel
     ms, cnt, pst = dev.batch_profile(True)
     res["profile"] = dict(stage_ms=ms, counts=cnt)
@@ -600,7 +604,8 @@ def main():
         "dtype": "fp64 (sin/cos to 4.8e-9, borderline points rechecked at full accuracy)",
         "data": "synthetic",
         "config": {"workload": cfg.describe(), "mode": "mixed", "parallelism": "single GPU",
-                   "waves_per_cu": head["waves_per_cu"], "evals_per_step": head["evals_per_step"],
+                   "waves_per_cu": head["waves_per_cu"], "tuning": head["tuning"],
+                   "evals_per_step": head["evals_per_step"],
                    "unconverged": head["unconverged"], "rechecked_fp64": head["rechecked"]},
         "roofline": roofline(head, os.path.join(ROOT, "profiles", "lw_batch_counters.json")),
         "cpu_baseline": None,

@@ -317,6 +317,12 @@ int om_batch_profile(om_context* ctx, int recompute_trajectories, double* stage_
  * largest candidate) work units for each candidate waves-per-CU value and
  * keeps the fastest (returned in *best). */
 int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_items, int* best, double* ms_out);
+/* Times the whole batch launch (persistent kernel, deep-level and recheck
+ * kernels) on the full workload for each candidate hand-over level (0: none;
+ * else at least 9, as om_batch_opts.deep_level) and keeps the fastest
+ * (returned in *best; the times in ms_out[n] when not NULL).  No counterpart
+ * in the reference. */
+int om_batch_tune_deep(om_context* ctx, const int* deep_levels, int n, int* best, double* ms_out);
 
 /* ---- reverse pass (adjoint) ------------------------------------------------
  * The reference has no derivative path; these entry points differentiate its

@@ -66,7 +66,7 @@ EXPORTS = ["om_device_count", "om_device_info", "om_context_create", "om_context
            "om_spectrum", "om_power_density", "om_batch_setup", "om_batch_run", "om_batch_tune",
            "om_batch_profile",
            "om_adjoint_forward", "om_adjoint_backward", "om_flux_dev", "om_spectrum_dev", "om_power_density_dev",
-           "om_flux_dev_noalloc", "om_adjoint_forward_dev", "om_adjoint_backward_dev"]
+           "om_flux_dev_noalloc", "om_adjoint_forward_dev", "om_adjoint_backward_dev", "om_batch_tune_deep"]
 
 _lib = None
 
@@ -128,6 +128,7 @@ def lib():
                                  C.POINTER(om_batch_opts)]
     L.om_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(om_stats)]
     L.om_batch_tune.argtypes = [C.c_void_p, _ip, C.c_int, C.c_long, _ip, _dp]
+    L.om_batch_tune_deep.argtypes = [C.c_void_p, _ip, C.c_int, _ip, _dp]
     L.om_batch_profile.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_ulonglong), C.POINTER(om_stats)]
     L.om_adjoint_forward.argtypes = [C.c_void_p, C.POINTER(om_field), C.c_int, C.c_int, C.POINTER(om_particle),
                                      _dp, C.c_long, _dp, C.c_long, C.c_int, _dp, C.POINTER(om_stats)]
@@ -496,6 +497,14 @@ class Device:
             self._dev_ptr(gx_t, torch.float64, 3, "grad_x0"), self._dev_ptr(gb_t, torch.float64, 3, "grad_beta0"),
             C.byref(st)))
         return st.as_dict()
+
+    def batch_tune_deep(self, deep_levels=(0, 13, 15)):
+        """om_batch_tune_deep: (best hand-over level, {level: ms})."""
+        c = np.ascontiguousarray(deep_levels, dtype=np.int32)
+        ms = np.zeros(len(c))
+        best = C.c_int(0)
+        self._chk(self.L.om_batch_tune_deep(self.h, _ptr(c, _ip), len(c), C.byref(best), _ptr(ms)))
+        return best.value, dict(zip([int(x) for x in c], ms.tolist()))
 
     def batch_tune(self, candidates=(2, 4, 8, 16, 32), sample_items=20000):
         c = np.ascontiguousarray(candidates, dtype=np.int32)

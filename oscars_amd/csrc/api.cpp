@@ -1967,6 +1967,37 @@ int om_batch_tune(om_context* ctx, const int* candidates, int n, long sample_ite
   });
 }
 
+int om_batch_tune_deep(om_context* ctx, const int* deep_levels, int n, int* best, double* ms_out) {
+  return guarded(ctx, [&] {
+    auto& B = ctx->batch;
+    if (!B.ready) throw std::logic_error("om_batch_setup has not been called");
+    if (n < 1 || !deep_levels) throw std::invalid_argument("no candidates");
+    for (int i = 0; i < n; ++i)
+      if (deep_levels[i] != 0 && deep_levels[i] < 9) throw std::out_of_range("deep_level must be 0 or at least 9");
+    batch_levels(ctx);
+    double best_ms = 1e300;
+    int best_d = deep_levels[0];
+    for (int i = 0; i < n; ++i) {
+      B.args.Ldeep = deep_levels[i] > 0 ? deep_levels[i] : kBatchNoHandOver;
+      check(om_launch_batch_clear(B.out.p, B.total_out, B.counters.p, kBatchCounters, ctx->stream), "clear");
+      check(hipEventRecord(ctx->ev0, ctx->stream), "event");
+      check(om_launch_lw_batch(&B.args, B.mixed, B.groups, ctx->stream), "lw batch");
+      check(hipEventRecord(ctx->ev1, ctx->stream), "event");
+      check(hipEventSynchronize(ctx->ev1), "sync");
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "elapsed");
+      if (ms_out) ms_out[i] = ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_d = deep_levels[i];
+      }
+    }
+    B.args.Ldeep = best_d > 0 ? best_d : kBatchNoHandOver;
+    B.drop_graphs();
+    if (best) *best = best_d;
+  });
+}
+
 
 namespace {
 // dev_io: obs_xyz, energies_eV and out_flux are device pointers, and the grid

@@ -88,6 +88,10 @@ class BatchedFlux:
     def tune(self, candidates=(2, 4, 8, 16), sample_items=20000):
         return self.dev.batch_tune(candidates, sample_items)
 
+    def tune_deep(self, deep_levels=(0, 13, 15)):
+        """Hand-over level (0: none) timed on the whole workload; the fastest is kept."""
+        return self.dev.batch_tune_deep(deep_levels)
+
     def run(self, recompute_trajectories=True):
         return self.dev.batch_run(recompute_trajectories)
 

@@ -97,6 +97,29 @@ def test_batch_tuner_keeps_results(batch_case):
 
 
 @gpu
+@pytest.mark.parametrize("mode,tol", [("tiled", 1e-9), ("mixed", 1e-5)])
+def test_batch_hand_over_tuner_keeps_results(batch_case, mode, tol):
+    """The hand-over level tuner times every candidate on the whole launch and
+    keeps the fastest; whichever it keeps, the sums stay within the bar."""
+    from oscars_amd import multiparticle as M
+    from oscars_amd.sr import sr
+    parts, windows, reqs, req_of, weights, ref = batch_case
+    s = sr()
+    s.add_bfield_undulator(**UNDULATOR)
+    dev = s._device()
+    bf = M.BatchedFlux(dev, s._fields, parts, windows, reqs, req_of, weights, mode=mode)
+    best, times = bf.tune_deep((0, 10, 12))
+    print("%s hand-over tuner: %s -> %d" % (mode, times, best))
+    assert best in (0, 10, 12) and len(times) == 3
+    outs, st = bf.run(recompute_trajectories=False)
+    assert st["n_unconverged"] == 0
+    for r in range(len(reqs)):
+        assert rel_err(outs[r], ref[r], 1e-6) < tol
+    with pytest.raises(Exception):
+        bf.tune_deep((5,))
+
+
+@gpu
 def test_sr_nparticles_batched_matches_particle_loop():
     """sr.calculate_spectrum(nparticles=N): the batched tiled path against the
     per-particle plain loop over the same sampled particles."""
