@@ -344,8 +344,8 @@ __device__ __forceinline__ void field_grad(const FieldDev* __restrict__ F, int n
     int a = f.axis;
     double coord = X[a], start = f.start[a], step = f.step[a], istep = f.inv_step[a];
     long na = f.n[a];
-    long n0 = (long) (size_t) div_by(coord - start, step, istep);
-    if (n0 > na - 2) n0 = na - 2;
+    int n0 = cell_of(div_by(coord - start, step, istep));
+    if (n0 > na - 2) n0 = (int) (na - 2);
     double dd = (coord - start) - (double) n0 * step;
     const double* lo = f.data + 3 * n0;
     for (int i = 0; i < 3; ++i) {

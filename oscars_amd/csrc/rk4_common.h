@@ -65,6 +65,12 @@ __device__ __forceinline__ void rot_xyz(double v[3], const FieldDev& f) {
   v[1] = s * xx + c * v[1];
 }
 
+// Cell index (size_t)((x - start) / step) of src/TField3D_Grid.cc:178-180 for a
+// coordinate strictly inside the map: the quotient is positive and below the
+// sample count, where truncation to int gives the same index in one
+// conversion instead of the unsigned 64-bit sequence.
+__device__ __forceinline__ int cell_of(double q) { return (int) q; }
+
 // Two- and three-dimensional grids (src/TField3D_Grid.cc:174-268): the cell
 // index per axis, then linear steps along x, y and z in the reference's order,
 // each ((d * (hi - lo)) / step).  The index is kept inside the map.
@@ -73,8 +79,8 @@ __device__ __forceinline__ long grid_cell(const FieldDev& f, int a, double c, do
     d = 0;
     return 0;
   }
-  long i = (long) (size_t) div_by(c - f.start[a], f.step[a], f.inv_step[a]);
-  if (i > f.n[a] - 2) i = f.n[a] - 2;
+  int i = cell_of(div_by(c - f.start[a], f.step[a], f.inv_step[a]));
+  if (i > f.n[a] - 2) i = (int) (f.n[a] - 2);
   d = (c - f.start[a]) - (double) i * f.step[a];
   return i;
 }
@@ -174,8 +180,8 @@ __device__ __forceinline__ void field_eval(const FieldDev& f, double x, double y
   double step = a == 0 ? f.step[0] : (a == 1 ? f.step[1] : f.step[2]);
   double istep = a == 0 ? f.inv_step[0] : (a == 1 ? f.inv_step[1] : f.inv_step[2]);
   long na = a == 0 ? f.n[0] : (a == 1 ? f.n[1] : f.n[2]);
-  long n0 = (long) (size_t) div_by(coord - start, step, istep);
-  if (n0 > na - 2) n0 = na - 2;  // keeps the n0+1 read inside the map
+  int n0 = cell_of(div_by(coord - start, step, istep));
+  if (n0 > na - 2) n0 = (int) (na - 2);  // keeps the n0+1 read inside the map
   double dd = (coord - start) - (double) n0 * step;
   const double* lo = f.data + 3 * n0;
   double l0 = lo[0], l1 = lo[1], l2 = lo[2], h0 = lo[3], h1 = lo[4], h2 = lo[5];
@@ -318,8 +324,8 @@ __device__ __forceinline__ void grid_z_eval(const GridZ& g, double z, double B[3
     B[0] = B[1] = B[2] = 0;
     return;
   }
-  long n0 = (long) (size_t) div_by(coord - g.start, g.step, g.inv_step);
-  if (n0 > g.n - 2) n0 = g.n - 2;
+  int n0 = cell_of(div_by(coord - g.start, g.step, g.inv_step));
+  if (n0 > g.n - 2) n0 = (int) (g.n - 2);
   double dd = (coord - g.start) - (double) n0 * g.step;
   const double* lo = g.data + 3 * n0;
 #pragma unroll
@@ -343,9 +349,9 @@ __device__ __forceinline__ bool beta_at_least_one(double v1, double v3, double v
 // Equations of motion at time t [s] (the field models read t only when they
 // are time-dependent).  A state with |beta| >= 1 sets c.err and leaves d as it
 // was (src/OSCARSSR.cc:1260-1264); with electric fields the reference sets the
-// flag and still writes d.  The branch-free single-undulator variants set the
-// flag and write d: a flagged step is discarded whole, so no kept value
-// depends on d there.
+// flag and still writes d.  Here every variant sets the flag and writes d: a
+// flagged step is discarded whole (rk4 redoes it by halving from its start,
+// with the slope recomputed there), so no kept value depends on d then.
 template <int M>
 __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, const double* x, double* d) {
   if (M >= kAxisUndX && M <= kAxisUndZ) {
@@ -432,10 +438,8 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, 
     d[5] = q * (E[2] + x[1] * B[1] - x[3] * B[0] - x[5] * be / kC);
     return;
   }
-  if (beta_at_least_one(x[1], x[3], x[5])) {
-    c.err = 1;
-    return;
-  }
+  // the flag discards the whole step (see above), so d is written regardless
+  const bool bad = beta_at_least_one(x[1], x[3], x[5]);
   double B[3];
   if (M == kGridZ) {
     grid_z_eval(c.g, x[4], B);
@@ -450,6 +454,7 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, 
   d[3] = c.qomg * (x[5] * B[0] - x[1] * B[2]);
   d[4] = x[5];
   d[5] = c.qomg * (x[1] * B[1] - x[3] * B[0]);
+  c.err |= bad ? 1 : 0;
 }
 
 // One classic RK4 step from (t, y) with slope dydx into yo
