@@ -134,3 +134,41 @@ def test_beam_sampling_statistics():
     assert np.std(xp) == pytest.approx(math.sqrt(0.9e-9 / 1.5), rel=0.05)
     assert np.std(e) == pytest.approx(0.0027, rel=0.05)
     assert abs(np.corrcoef(x, xp)[0, 1]) < 0.05
+
+
+def test_mode_keywords_and_halbach():
+    """The power-density extensions reject unknown modes before any GPU work;
+    add_bfield_halbach (the reference's model returns zero everywhere,
+    src/TField3D_Halbach.cc:58-65) adds nothing to the field list."""
+    from oscars_amd.sr import sr
+    s = sr()
+    planar(s)
+    n = len(s._fields)
+    s.add_bfield_halbach(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=10)
+    assert len(s._fields) == n
+    with pytest.raises(ValueError, match="mode must be"):
+        s.calculate_power_density_rectangle(npoints=[3, 3], width=[0.01, 0.01], mode="fast")
+    with pytest.raises(ValueError, match="mode must be"):
+        s.calculate_power_density_line([0, 0, 20], [0.01, 0, 20], npoints=3, mode="fast")
+    with pytest.raises(ValueError, match="mode must be"):
+        s.calculate_power_density_stl(mode="fast")
+
+
+def test_icosphere_is_closed_and_outward():
+    """The synthetic STL sphere: 20 * 4^k triangles, every edge shared by two
+    triangles, normals pointing away from the centre, rows as read_stl lays them out."""
+    from oscars_amd import geometry as G
+    tris = G.icosphere(2.0, 2, (1.0, 0.0, 5.0))
+    assert len(tris) == 320
+    edges = {}
+    for _, a, b, c in tris:
+        for p, q in ((a, b), (b, c), (c, a)):
+            k = tuple(sorted((tuple(np.round(p, 12)), tuple(np.round(q, 12)))))
+            edges[k] = edges.get(k, 0) + 1
+    assert set(edges.values()) == {2}
+    rows = G.stl_rows(tris)
+    assert rows.shape == (320, 15)
+    centre = np.array([1.0, 0.0, 5.0])
+    assert np.all(np.einsum("ij,ij->i", rows[:, 9:12], rows[:, 12:15] - centre) > 0)
+    assert np.allclose(np.linalg.norm(rows[:, 0:3] - centre, axis=1), 2.0)
+    assert np.allclose(rows[:, 12:15], (rows[:, 0:3] + rows[:, 3:6] + rows[:, 6:9]) / 3)
