@@ -294,6 +294,10 @@ struct RkCtx {
   int nfe;     // electric fields, stored after the nf magnetic ones in F
   double qom;  // Q / M
   int halved;  // steps halved so far (src/OSCARSSR.cc:1362-1378)
+  // kGridZ: the samples of the last cell read from LDS (the cell changes every
+  // few steps, so most stages skip the LDS round trip on their serial chain)
+  int gc_n0 = -1;
+  double gc[6] = {0, 0, 0, 0, 0, 0};
 };
 
 // Field-model variants of the equations of motion:
@@ -318,7 +322,7 @@ struct RkCtx {
 constexpr int kFieldsGeneric = 0, kOneUndulator = 1, kAxisUndX = 2, kAxisUndY = 3, kAxisUndZ = 4, kGridZ = 5,
               kFieldsEB = 6, kUndZ = 7, kEpuZ = 8;
 
-__device__ __forceinline__ void grid_z_eval(const GridZ& g, double z, double B[3]) {
+__device__ __forceinline__ void grid_z_eval(const GridZ& g, int& cn0, double cs[6], double z, double B[3]) {
   double coord = z - g.trans;
   if (coord <= g.start || coord >= g.stop) {
     B[0] = B[1] = B[2] = 0;
@@ -326,10 +330,15 @@ __device__ __forceinline__ void grid_z_eval(const GridZ& g, double z, double B[3
   }
   int n0 = cell_of(div_by(coord - g.start, g.step, g.inv_step));
   if (n0 > g.n - 2) n0 = (int) (g.n - 2);
-  double dd = (coord - g.start) - (double) n0 * g.step;
-  const double* lo = g.data + 3 * n0;
+  if (n0 != cn0) {
+    const double* lo = g.data + 3 * n0;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) B[k] = lo[k] + div_by((lo[3 + k] - lo[k]) * dd, g.step, g.inv_step);
+    for (int k = 0; k < 6; ++k) cs[k] = lo[k];
+    cn0 = n0;
+  }
+  double dd = (coord - g.start) - (double) n0 * g.step;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) B[k] = cs[k] + div_by((cs[3 + k] - cs[k]) * dd, g.step, g.inv_step);
 }
 
 // |beta| >= 1 test of src/OSCARSSR.cc:1260-1264 and 1368,
@@ -442,7 +451,7 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, 
   const bool bad = beta_at_least_one(x[1], x[3], x[5]);
   double B[3];
   if (M == kGridZ) {
-    grid_z_eval(c.g, x[4], B);
+    grid_z_eval(c.g, c.gc_n0, c.gc, x[4], B);
   } else if (M == kOneUndulator) {
     und_eval(f0, x[0], x[2], x[4], B);
   } else {

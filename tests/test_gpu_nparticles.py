@@ -83,3 +83,10 @@ def test_nparticles_power_density():
     err = rel_err(v, ref, 1e-6)
     print("nparticles power density: max rel err %.3e" % err)
     assert err < 1e-9
+    s.set_seed(SEED)
+    res = s.calculate_power_density_rectangle(npoints=[11, 11], width=[0.02, 0.02], translation=[0, 0, 30],
+                                              nparticles=NP, mode="mixed")
+    vm = np.array([r[1] for r in res])
+    em = rel_err(vm, ref, 1e-6)
+    print("nparticles power density, mixed: max rel err %.3e" % em)
+    assert em < 1e-5

@@ -113,3 +113,22 @@ def test_mixed_power_density_capped(epu_dev):
             assert np.max(np.abs(vm - v64)) < 1e-6
         else:
             assert rel_err(vm, v64, 1e-6) < 1e-5
+
+
+@gpu
+def test_mixed_power_density_device_io(epu_dev):
+    """om_power_density_dev (device tensors) gives the host entry point's
+    values, levels and flags bit for bit, in both modes."""
+    import torch
+    g, dev = epu_dev
+    obs = _rect(31, 0.04, 20.0)
+    t_obs = torch.tensor(obs, dtype=torch.float64, device="cuda")
+    for mode in ("plain", "mixed"):
+        v, lev, cv, st = dev.power_density(0, obs, [0, 0, 1], 0, dev.conv(mode=mode))
+        out = torch.zeros(len(obs), dtype=torch.float64, device="cuda")
+        lt = torch.zeros(len(obs), dtype=torch.int32, device="cuda")
+        ct = torch.zeros(len(obs), dtype=torch.uint8, device="cuda")
+        st2 = dev.power_density_dev(0, t_obs, [0, 0, 1], 0, dev.conv(mode=mode), out, lt, ct)
+        assert np.array_equal(out.cpu().numpy(), v)
+        assert np.array_equal(lt.cpu().numpy(), lev) and np.array_equal(ct.cpu().numpy(), cv)
+        assert st2["n_rechecked"] == st["n_rechecked"]
