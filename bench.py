@@ -360,6 +360,25 @@ def adjoint_leg(dev, steps=2):
     tb /= steps
     gf = gf.cpu().numpy()
     ga = ga.cpu().numpy()
+    # for scale: the plain FP64 forward of the same problem (RK4 of the error
+    # map's trajectory + one lane per point summing levels 0..L), device tensors
+    fe, keep_e = A.grid_field([0, 0, z0], [0, 0, dz], [1, 1, n], err)
+    parts = M.om_particles([p0], [(0.0, 2.0, 20001)])
+    obs_t = torch.tensor(obs, dtype=torch.float64, device="cuda")
+    e_t = torch.tensor(energies, dtype=torch.float64, device="cuda")
+    out_t = torch.zeros(len(obs) * len(energies), dtype=torch.float64, device="cuda")
+    conv = dev.conv(precision=0.0, max_level=L, mode="plain")
+    tp = 0.0
+    for k in range(steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.trajectories([fe], parts)
+        dev.flux_dev(0, obs_t, e_t, [0.0] * 6, conv, out_t)
+        torch.cuda.synchronize()
+        if k > 0:
+            tp += time.perf_counter() - t0
+    tp /= steps
+    plain_err = rel_errors(out_t.cpu().numpy(), F.reshape(-1).cpu().numpy())
     evals = float(len(obs) * len(energies) * ((2 << L) - 1))
     out = {"workload": "51x51 at 30 m x 4 energies, 4001-sample By map, 20001-point RK4, level %d; samples, flux "
                        "and gradients as device tensors" % L,
@@ -368,9 +387,13 @@ def adjoint_leg(dev, steps=2):
            "backward_kernel_ms": prob.last_stats["backward"]["kernel_ms"],
            "forward_evals_per_s": evals / tf, "adjoint_evals_per_s": evals / tb,
            "fwd_plus_adjoint_evals_per_s": 2 * evals / (tf + tb),
+           "plain_forward_ms": 1e3 * tp, "plain_forward_evals_per_s": evals / tp,
+           "fwd_plus_adjoint_vs_plain_forward": (2 * evals / (tf + tb)) / (evals / tp),
+           "plain_forward_vs_adjoint_forward_rel_err": plain_err,
            "grad_norm_field": float(np.linalg.norm(gf)), "grad_angles": [float(v) for v in ga],
            "fd_check": "tests/test_gpu_adjoint.py (long-double central differences, 1e-4)"}
-    log("adjoint: forward %.1f ms, backward %.1f ms" % (1e3 * tf, 1e3 * tb))
+    log("adjoint: forward %.1f ms, backward %.1f ms; plain FP64 forward of the same problem %.1f ms" %
+        (1e3 * tf, 1e3 * tb, 1e3 * tp))
     return out
 
 
