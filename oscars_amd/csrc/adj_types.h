@@ -36,6 +36,7 @@ struct AdjRkArgs {
   double* segmap;      // nseg x 42: M (6x6 row-major), b (6)
   double* lam;         // (nseg + 1) x 6: lambda at segment starts
   double* ybuf;        // nf x 6 recomputed states
+  long lds_cap;        // doubles of dynamic LDS the segment kernels may stage the fields in (set by the launcher)
 };
 
 }  // namespace om

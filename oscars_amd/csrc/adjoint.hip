@@ -336,27 +336,113 @@ __device__ __forceinline__ void field_grad(const FieldDev* __restrict__ F, int n
       }
       continue;
     }
-    // grid without rotation (the host rejects rotated grids for the reverse pass)
-    double X[3] = {x - f.trans[0], yy - f.trans[1], z - f.trans[2]};
-    if (f.n[0] > 1 && (X[0] <= f.start[0] || X[0] >= f.stop[0])) continue;
-    if (f.n[1] > 1 && (X[1] <= f.start[1] || X[1] >= f.stop[1])) continue;
-    if (f.n[2] > 1 && (X[2] <= f.start[2] || X[2] >= f.stop[2])) continue;
+    // grid without rotation (the host rejects rotated grids for the reverse pass).
+    // The axis is a run-time value: everything it selects is picked with
+    // selects, so no local array is indexed at run time (that would put the
+    // arrays, and the field derivatives with them, in scratch memory).
+    double X0 = x - f.trans[0], X1 = yy - f.trans[1], X2 = z - f.trans[2];
+    if (f.n[0] > 1 && (X0 <= f.start[0] || X0 >= f.stop[0])) continue;
+    if (f.n[1] > 1 && (X1 <= f.start[1] || X1 >= f.stop[1])) continue;
+    if (f.n[2] > 1 && (X2 <= f.start[2] || X2 >= f.stop[2])) continue;
     int a = f.axis;
-    double coord = X[a], start = f.start[a], step = f.step[a], istep = f.inv_step[a];
-    long na = f.n[a];
+    double coord = a == 0 ? X0 : (a == 1 ? X1 : X2);
+    double start = a == 0 ? f.start[0] : (a == 1 ? f.start[1] : f.start[2]);
+    double step = a == 0 ? f.step[0] : (a == 1 ? f.step[1] : f.step[2]);
+    double istep = a == 0 ? f.inv_step[0] : (a == 1 ? f.inv_step[1] : f.inv_step[2]);
+    long na = a == 0 ? f.n[0] : (a == 1 ? f.n[1] : f.n[2]);
     int n0 = cell_of(div_by(coord - start, step, istep));
     if (n0 > na - 2) n0 = (int) (na - 2);
     double dd = (coord - start) - (double) n0 * step;
     const double* lo = f.data + 3 * n0;
     for (int i = 0; i < 3; ++i) {
       r.B[i] += lo[i] + div_by((lo[3 + i] - lo[i]) * dd, step, istep);
-      r.J[i][a] += (lo[3 + i] - lo[i]) / step;
+      double g = (lo[3 + i] - lo[i]) / step;
+      r.J[i][0] = a == 0 ? r.J[i][0] + g : r.J[i][0];
+      r.J[i][1] = a == 1 ? r.J[i][1] + g : r.J[i][1];
+      r.J[i][2] = a == 2 ? r.J[i][2] + g : r.J[i][2];
     }
     if (fi == gfield) {
       r.n0 = n0;
       r.w = dd / step;
     }
   }
+}
+
+// The forward's field sum (get_b / field_eval, rk4_common.h) for the fields
+// the reverse pass takes: undulators and EPUs, and one-dimensional grids
+// without rotation or time dependence.  Same operations in the same order
+// (the zero rotation is applied as field_eval applies it), without the other
+// models' branches and the out-of-line multi-dimensional grid call, which
+// would cost register traffic on every step of the segment kernels.
+__device__ __forceinline__ void adj_get_b(const FieldDev* __restrict__ F, int nf, double x, double y, double z,
+                                          double B[3]) {
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int i = 0; i < nf; ++i) {
+    const FieldDev& f = F[i];
+    double v[3] = {0, 0, 0};
+    if (f.type == kUndulator || f.type == kEPU) {
+      double D = (x - f.center[0]) * f.unit[0] + (y - f.center[1]) * f.unit[1] + (z - f.center[2]) * f.unit[2];
+      double tc = 1 + D * f.taper;
+      undulator_part(f, D, f.phase_shift_a, f.field[0], f.field[1], f.field[2], tc, v, false);
+      if (f.type == kEPU) undulator_part(f, D, f.phase_shift_b, f.fieldB[0], f.fieldB[1], f.fieldB[2], tc, v, true);
+    } else {
+      double X[3] = {x, y, z};
+      rot_xyz(X, f);
+      X[0] -= f.trans[0]; X[1] -= f.trans[1]; X[2] -= f.trans[2];
+      bool out = (f.n[0] > 1 && (X[0] <= f.start[0] || X[0] >= f.stop[0])) ||
+                 (f.n[1] > 1 && (X[1] <= f.start[1] || X[1] >= f.stop[1])) ||
+                 (f.n[2] > 1 && (X[2] <= f.start[2] || X[2] >= f.stop[2]));
+      if (!out) {
+        int a = f.axis;
+        double coord = a == 0 ? X[0] : (a == 1 ? X[1] : X[2]);
+        double start = a == 0 ? f.start[0] : (a == 1 ? f.start[1] : f.start[2]);
+        double step = a == 0 ? f.step[0] : (a == 1 ? f.step[1] : f.step[2]);
+        double istep = a == 0 ? f.inv_step[0] : (a == 1 ? f.inv_step[1] : f.inv_step[2]);
+        long na = a == 0 ? f.n[0] : (a == 1 ? f.n[1] : f.n[2]);
+        int n0 = cell_of(div_by(coord - start, step, istep));
+        if (n0 > na - 2) n0 = (int) (na - 2);
+        double dd = (coord - start) - (double) n0 * step;
+        const double* lo = f.data + 3 * n0;
+        v[0] = (lo[0] + div_by((lo[3] - lo[0]) * dd, step, istep)) * 1.0;
+        v[1] = (lo[1] + div_by((lo[4] - lo[1]) * dd, step, istep)) * 1.0;
+        v[2] = (lo[2] + div_by((lo[5] - lo[2]) * dd, step, istep)) * 1.0;
+      }
+    }
+    s0 += v[0]; s1 += v[1]; s2 += v[2];
+  }
+  B[0] = s0; B[1] = s1; B[2] = s2;
+}
+
+// The forward's equations of motion and RK4 step (derivs<kFieldsGeneric>,
+// rk4_try, rk4_common.h) over adj_get_b.  The forward refuses trajectories
+// whose steps were halved, so the plain step is the forward's.
+__device__ __forceinline__ void adj_derivs(const FieldDev* __restrict__ F, int nf, double qomg, const double* x,
+                                           double* d) {
+  double B[3];
+  adj_get_b(F, nf, x[0], x[2], x[4], B);
+  d[0] = x[1];
+  d[1] = qomg * (-x[5] * B[1] + x[3] * B[2]);
+  d[2] = x[3];
+  d[3] = qomg * (x[5] * B[0] - x[1] * B[2]);
+  d[4] = x[5];
+  d[5] = qomg * (x[1] * B[1] - x[3] * B[0]);
+}
+
+__device__ __forceinline__ void adj_step(const FieldDev* __restrict__ F, int nf, double qomg, double h, double* y) {
+  double dydx[6], dym[6], dyt[6], yt[6];
+  adj_derivs(F, nf, qomg, y, dydx);
+  double hh = h * 0.5;
+  double h6 = h / 6.0;
+  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
+  adj_derivs(F, nf, qomg, yt, dyt);
+  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
+  adj_derivs(F, nf, qomg, yt, dym);
+  for (int i = 0; i < 6; ++i) {
+    yt[i] = y[i] + h * dym[i];
+    dym[i] += dyt[i];
+  }
+  adj_derivs(F, nf, qomg, yt, dyt);
+  for (int i = 0; i < 6; ++i) y[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
 }
 
 // f(y) = (v, qomg v x B(x)) in state order (x, vx, y, vy, z, vz)
@@ -432,29 +518,85 @@ __device__ __forceinline__ void direct_terms(const AdjRkArgs& A, long i, const F
   if (any) fjac_tvec(fd, A.qomg, y, w, c, A, rec);
 }
 
+// The segment kernels evaluate the fields at every stage of every step on
+// one lane's serial chain: the field descriptions and the grid samples are
+// copied into LDS first when they fit (a 4001-sample map takes 96 KB), so the
+// chain waits on LDS instead of L2.  The copies are bit for bit; returns the
+// descriptions to use (the global ones when they do not fit).
+extern __shared__ double adj_lds[];
+
+__device__ const FieldDev* stage_fields(const FieldDev* __restrict__ F, int nf, long cap) {
+  constexpr long kWords = (long) (sizeof(FieldDev) / sizeof(double));
+  long need = nf * kWords;
+  for (int i = 0; i < nf; ++i)
+    if (F[i].type == kGrid) need += 3 * F[i].n[0] * F[i].n[1] * F[i].n[2];
+  if (need > cap) return F;
+  const double* src = reinterpret_cast<const double*>(F);
+  for (long k = threadIdx.x; k < nf * kWords; k += blockDim.x) adj_lds[k] = src[k];
+  long off = nf * kWords;
+  for (int i = 0; i < nf; ++i) {
+    if (F[i].type != kGrid) continue;
+    long n = 3 * F[i].n[0] * F[i].n[1] * F[i].n[2];
+    for (long k = threadIdx.x; k < n; k += blockDim.x) adj_lds[off + k] = F[i].data[k];
+    off += n;
+  }
+  __syncthreads();
+  FieldDev* Fl = reinterpret_cast<FieldDev*>(adj_lds);
+  if (threadIdx.x == 0) {
+    long o = nf * kWords;
+    for (int i = 0; i < nf; ++i) {
+      if (Fl[i].type != kGrid) continue;
+      Fl[i].data = adj_lds + o;
+      o += 3 * Fl[i].n[0] * Fl[i].n[1] * Fl[i].n[2];
+    }
+  }
+  __syncthreads();
+  return Fl;
+}
+
 // Phase A: per segment, lambda_{i0} = M lambda_{i1} + b.
+// Eight lanes per segment: every lane of the group integrates the same states
+// and field derivatives; lane q < 6 forms column q of the step Jacobian
+// (J e_q through the four stages) and column q of M J^T, the full matrices
+// passing through LDS.  Each product is summed in the same order as a single
+// lane would, so the map is the same to the bit.
+constexpr int kSegLanes = 8;
+
+struct SegLds {
+  double J[64 / kSegLanes][kSegLanes][6];  // per group: column q of J at [q]
+  double M[64 / kSegLanes][6][kSegLanes];  // per group: M[r][q]
+};
+
 __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
-  long s = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= A.nseg) return;
-  long i0 = s * A.K, i1 = i0 + A.K < A.nf ? i0 + A.K : A.nf;
+  __shared__ SegLds sh;
+  const int grp = threadIdx.x / kSegLanes;
+  const int q = threadIdx.x % kSegLanes;
+  const int col = q < 6 ? q : 5;  // lanes 6 and 7 repeat column 5 and write nothing that is read
+  const long s = (long) blockIdx.x * (64 / kSegLanes) + grp;
+  const bool live = s < A.nseg;
+  const long ss = live ? s : A.nseg - 1;
+  const FieldDev* __restrict__ F = stage_fields(A.F, A.nfields, A.lds_cap);
+  long i0 = ss * A.K, i1 = i0 + A.K < A.nf ? i0 + A.K : A.nf;
   double y[6];
-  for (int r = 0; r < 6; ++r) y[r] = A.ckpt[6 * s + r];
+  for (int r = 0; r < 6; ++r) y[r] = A.ckpt[6 * ss + r];
   double M[6][6], b[6];
+#pragma unroll
   for (int r = 0; r < 6; ++r) {
     b[r] = 0;
-    for (int q = 0; q < 6; ++q) M[r][q] = r == q ? 1.0 : 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) M[r][m] = r == m ? 1.0 : 0.0;
   }
-  RkCtx rc{A.F, A.nfields, A.qomg, 0};
-  UndParams none{};
   double h = A.dt, hh = h * 0.5, h6 = h / 6.0;
   for (long i = i0; i < i1; ++i) {
     FieldD f0;
-    field_grad(A.F, A.nfields, A.gfield, y, f0);
+    field_grad(F, A.nfields, A.gfield, y, f0);
     double c[6];
     direct_terms(A, i, f0, y, c, -1);
+#pragma unroll
     for (int r = 0; r < 6; ++r) {
       double acc = b[r];
-      for (int q = 0; q < 6; ++q) acc += M[r][q] * c[q];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) acc += M[r][m] * c[m];
       b[r] = acc;
     }
     if (i == A.nf - 1) break;
@@ -463,20 +605,18 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
     fval(f0, A.qomg, y, k1);
     for (int r = 0; r < 6; ++r) y1[r] = y[r] + hh * k1[r];
     FieldD f1;
-    field_grad(A.F, A.nfields, A.gfield, y1, f1);
+    field_grad(F, A.nfields, A.gfield, y1, f1);
     fval(f1, A.qomg, y1, k2);
     for (int r = 0; r < 6; ++r) y2[r] = y[r] + hh * k2[r];
     FieldD f2;
-    field_grad(A.F, A.nfields, A.gfield, y2, f2);
+    field_grad(F, A.nfields, A.gfield, y2, f2);
     fval(f2, A.qomg, y2, k3);
     for (int r = 0; r < 6; ++r) y3[r] = y[r] + h * k3[r];
     FieldD f3;
-    field_grad(A.F, A.nfields, A.gfield, y3, f3);
-    // J column by column: J e_q
-    double J[6][6];
-    for (int q = 0; q < 6; ++q) {
-      double e[6] = {0, 0, 0, 0, 0, 0};
-      e[q] = 1.0;
+    field_grad(F, A.nfields, A.gfield, y3, f3);
+    {  // column `col` of J: J e_col
+      double e[6];
+      for (int r = 0; r < 6; ++r) e[r] = r == col ? 1.0 : 0.0;
       double d1[6], d2[6], d3[6], d4[6], t[6];
       fjac_vec(f0, A.qomg, y, e, d1);
       for (int r = 0; r < 6; ++r) t[r] = e[r] + hh * d1[r];
@@ -485,27 +625,37 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
       fjac_vec(f2, A.qomg, y2, t, d3);
       for (int r = 0; r < 6; ++r) t[r] = e[r] + h * d3[r];
       fjac_vec(f3, A.qomg, y3, t, d4);
-      for (int r = 0; r < 6; ++r) J[r][q] = e[r] + h6 * (d1[r] + d4[r] + 2.0 * (d2[r] + d3[r]));
+      for (int r = 0; r < 6; ++r) sh.J[grp][q][r] = e[r] + h6 * (d1[r] + d4[r] + 2.0 * (d2[r] + d3[r]));
     }
-    // M <- M J^T
-    double Mn[6][6];
+    wave_sync();
+    // column `col` of M J^T: (M J^T)[r][col] = sum_m M[r][m] J[col][m], J[col][m] in column m's entry col
+    double Jrow[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) Jrow[m] = sh.J[grp][m][col];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      double acc = 0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) acc += M[r][m] * Jrow[m];
+      sh.M[grp][r][q] = acc;
+    }
+    wave_sync();
+#pragma unroll
     for (int r = 0; r < 6; ++r)
-      for (int q = 0; q < 6; ++q) {
-        double acc = 0;
-        for (int m = 0; m < 6; ++m) acc += M[r][m] * J[q][m];
-        Mn[r][q] = acc;
-      }
-    for (int r = 0; r < 6; ++r)
-      for (int q = 0; q < 6; ++q) M[r][q] = Mn[r][q];
-    // next state, as the forward computes it
-    double d[6];  // the reverse pass takes time-independent fields only: t does not enter
-    derivs<kFieldsGeneric>(rc, none, 0.0, y, d);
-    rk4<kFieldsGeneric>(rc, none, y, d, 0.0, A.dt);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) M[r][m] = sh.M[grp][r][m];
+    wave_sync();  // every lane has read J and M before the next step writes them
+    adj_step(F, A.nfields, A.qomg, A.dt, y);  // the next state, as the forward computes it
   }
+  if (!live) return;
   double* o = A.segmap + 42 * s;
-  for (int r = 0; r < 6; ++r)
-    for (int q = 0; q < 6; ++q) o[6 * r + q] = M[r][q];
-  for (int r = 0; r < 6; ++r) o[36 + r] = b[r];
+  if (q < 6)
+#pragma unroll
+    for (int r = 0; r < 6; ++r)  // M[r][q] by selects (no run-time index into the register array)
+      o[6 * r + q] = q == 0 ? M[r][0] : q == 1 ? M[r][1] : q == 2 ? M[r][2] : q == 3 ? M[r][3] : q == 4 ? M[r][4]
+                                                                                                   : M[r][5];
+  if (q == 0)
+    for (int r = 0; r < 6; ++r) o[36 + r] = b[r];
 }
 
 // Phase B: chain the segment maps from the end (lambda after the last knot is 0).
@@ -531,20 +681,15 @@ __global__ void __launch_bounds__(64) adj_chain_kernel(AdjRkArgs A) {
 // Phase C: per segment, recompute the states and sweep back from lambda_{i1},
 // scattering the grid-sample gradients.
 __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
+  const FieldDev* __restrict__ F = stage_fields(A.F, A.nfields, A.lds_cap);
   long s = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= A.nseg) return;
   long i0 = s * A.K, i1 = i0 + A.K < A.nf ? i0 + A.K : A.nf;
-  RkCtx rc{A.F, A.nfields, A.qomg, 0};
-  UndParams none{};
   double y[6];
   for (int r = 0; r < 6; ++r) y[r] = A.ckpt[6 * s + r];
   for (long i = i0; i < i1; ++i) {
     for (int r = 0; r < 6; ++r) A.ybuf[6 * i + r] = y[r];
-    if (i + 1 < i1) {
-      double d[6];
-      derivs<kFieldsGeneric>(rc, none, 0.0, y, d);
-      rk4<kFieldsGeneric>(rc, none, y, d, 0.0, A.dt);
-    }
+    if (i + 1 < i1) adj_step(F, A.nfields, A.qomg, A.dt, y);
   }
   double lam[6];
   for (int r = 0; r < 6; ++r) lam[r] = A.lam[6 * (s + 1) + r];
@@ -552,22 +697,22 @@ __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
   for (long i = i1 - 1; i >= i0; --i) {
     for (int r = 0; r < 6; ++r) y[r] = A.ybuf[6 * i + r];
     FieldD f0;
-    field_grad(A.F, A.nfields, A.gfield, y, f0);
+    field_grad(F, A.nfields, A.gfield, y, f0);
     double yb[6] = {0, 0, 0, 0, 0, 0};
     if (i < A.nf - 1) {
       double k1[6], y1[6], k2[6], y2[6], k3[6], y3[6];
       fval(f0, A.qomg, y, k1);
       for (int r = 0; r < 6; ++r) y1[r] = y[r] + hh * k1[r];
       FieldD f1;
-      field_grad(A.F, A.nfields, A.gfield, y1, f1);
+      field_grad(F, A.nfields, A.gfield, y1, f1);
       fval(f1, A.qomg, y1, k2);
       for (int r = 0; r < 6; ++r) y2[r] = y[r] + hh * k2[r];
       FieldD f2;
-      field_grad(A.F, A.nfields, A.gfield, y2, f2);
+      field_grad(F, A.nfields, A.gfield, y2, f2);
       fval(f2, A.qomg, y2, k3);
       for (int r = 0; r < 6; ++r) y3[r] = y[r] + h * k3[r];
       FieldD f3;
-      field_grad(A.F, A.nfields, A.gfield, y3, f3);
+      field_grad(F, A.nfields, A.gfield, y3, f3);
       // yo = y + h6 (k1 + k4 + 2 (k2 + k3))
       double k1b[6], k2b[6], k3b[6], k4b[6];
       for (int r = 0; r < 6; ++r) {
@@ -700,9 +845,25 @@ extern "C" hipError_t om_launch_adj_spline(const double* rows, long ns, const do
 
 extern "C" hipError_t om_launch_adj_rk4(const om::AdjRkArgs* A, hipStream_t s) {
   long blocks = (A->nseg + 63) / 64;
-  hipLaunchKernelGGL(om::adj_seg_map_kernel, dim3((unsigned) blocks), dim3(64), 0, s, *A);
-  hipLaunchKernelGGL(om::adj_chain_kernel, dim3(1), dim3(64), 0, s, *A);
-  hipLaunchKernelGGL(om::adj_seg_reverse_kernel, dim3((unsigned) blocks), dim3(64), 0, s, *A);
+  long map_blocks = (A->nseg + 64 / om::kSegLanes - 1) / (64 / om::kSegLanes);
+  // dynamic LDS for the staged fields: what the 160 KB of a CU leaves beside
+  // the static arrays (one workgroup per CU; these kernels run few waves)
+  constexpr size_t kMapBytes = 160 * 1024 - sizeof(om::SegLds) - 1024;
+  constexpr size_t kRevBytes = 160 * 1024 - 1024;
+  om::AdjRkArgs R = *A;
+  size_t map_bytes = kMapBytes, rev_bytes = kRevBytes;
+  if (hipFuncSetAttribute((const void*) om::adj_seg_map_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int) map_bytes) != hipSuccess)
+    map_bytes = 0;
+  if (hipFuncSetAttribute((const void*) om::adj_seg_reverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int) rev_bytes) != hipSuccess)
+    rev_bytes = 0;
+  (void) hipGetLastError();  // a refused attribute leaves the fields in global memory
+  R.lds_cap = (long) (map_bytes / sizeof(double));
+  hipLaunchKernelGGL(om::adj_seg_map_kernel, dim3((unsigned) map_blocks), dim3(64), map_bytes, s, R);
+  hipLaunchKernelGGL(om::adj_chain_kernel, dim3(1), dim3(64), 0, s, R);
+  R.lds_cap = (long) (rev_bytes / sizeof(double));
+  hipLaunchKernelGGL(om::adj_seg_reverse_kernel, dim3((unsigned) blocks), dim3(64), rev_bytes, s, R);
   return hipGetLastError();
 }
 
