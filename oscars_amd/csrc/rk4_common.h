@@ -232,6 +232,15 @@ __device__ __forceinline__ double in_vgpr(double x) {
   return __builtin_bit_cast(double, ((long long) hi << 32) | (unsigned) lo);
 }
 
+// The value of x in the neighbouring lane (lanes 2k and 2k+1 exchange), by a
+// DPP quad permutation of both halves: a move, exact.
+__device__ __forceinline__ double lane_pair_other(double x) {
+  long long b = __builtin_bit_cast(long long, x);
+  int lo = __builtin_amdgcn_update_dpp(0, (int) (b & 0xffffffff), 0xB1, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), 0xB1, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long) hi << 32) | (unsigned) lo);
+}
+
 __device__ __forceinline__ UndParams und_in_vgpr(const UndParams& f) {
   UndParams v = f;
   v.c0 = in_vgpr(f.c0); v.c1 = in_vgpr(f.c1); v.c2 = in_vgpr(f.c2);
@@ -318,7 +327,8 @@ struct RkCtx {
 //                       z and no taper, fields in any direction: und_eval with
 //                       D = z - c_z and the taper factor 1, both exact (the dot
 //                       product's other terms are zeros, 1 + D 0 = 1), branch-free
-//                       like the axis variants.
+//                       like the axis variants.  kEpuZ runs in lane pairs, each
+//                       lane evaluating one of the two parts (rk4_kernel).
 constexpr int kFieldsGeneric = 0, kOneUndulator = 1, kAxisUndX = 2, kAxisUndY = 3, kAxisUndZ = 4, kGridZ = 5,
               kFieldsEB = 6, kUndZ = 7, kEpuZ = 8;
 
@@ -407,11 +417,13 @@ __device__ __forceinline__ void derivs(RkCtx& c, const UndParams& f0, double t, 
     double sA = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
     double v0 = f0.a0 * scA * sA, v1 = f0.a1 * scA * sA, v2 = f0.a2 * scA * sA;
     if (M == kEpuZ) {
-      double scB = und_scale(f0.thB, D);
-      double sB = sin64(div_by(2.0 * kPi * (D - f0.psB), f0.PL, f0.inv_PL));
-      v0 = v0 + f0.b0 * scB * sB;
-      v1 = v1 + f0.b1 * scB * sB;
-      v2 = v2 + f0.b2 * scB * sB;
+      // Two lanes integrate the same state: one holds part A in f0's A slots,
+      // the other part B there (rk4_kernel swaps them).  Each adds the other's
+      // part to its own, A + B in one lane and B + A in the other, the same
+      // double (IEEE addition commutes), which is the reference's A + B.
+      v0 = v0 + lane_pair_other(v0);
+      v1 = v1 + lane_pair_other(v1);
+      v2 = v2 + lane_pair_other(v2);
     }
     double B0 = 0.0 + v0, B1 = 0.0 + v1, B2 = 0.0 + v2;
     double n1 = c.qomg * (-x[5] * B1 + x[3] * B2);

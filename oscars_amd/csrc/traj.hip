@@ -44,9 +44,33 @@ __global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p
   p[9 * k + c] = p[9 * k + c] / kC;
 }
 
-// One thread per particle.  Forward points go to knot nb+i, backward points to
-// knot nb-1-i, which is the order the reference reaches after its two array
-// reversals.
+// Lanes per particle.  The forward and the backward integration both start
+// from the state at T0 and never meet (src/OSCARSSR.cc:943-1006), so they run
+// in two lanes of one wave: a window with T0 inside costs the longer of the two
+// instead of their sum.  kEpuZ pairs lanes once more: the two lanes of a
+// direction evaluate the EPU's parts A and B one each (see derivs).
+template <int M>
+constexpr int rk4_lanes() {
+  return M == kEpuZ ? 4 : 2;
+}
+
+// Value of x in the other lane of a quad given by the DPP quad permutation Q
+// (0xB1: lanes 0<->1 and 2<->3; 0x4E: lanes 0<->2 and 1<->3).
+template <int Q>
+__device__ __forceinline__ int quad_other(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, Q, 0xF, 0xF, false);
+}
+
+// Forward points go to knot nb+i, backward points to knot nb-1-i, which is the
+// order the reference reaches after its two array reversals.  The loop is the
+// same for both directions (step h = +-dt, time T0/c + h i): the forward lane
+// stores the initial knot first and each later knot after the step that
+// reaches it, like the reference's store before the next step; its last step
+// is taken and not stored, as there (after the loop, so that every step of the
+// loop stores).  The fErrorGamma flag (c.err) that the
+// reference carries from the forward into the backward loop is always clear
+// after a completed RK4 step, so it reaches the backward loop only when there
+// is no forward step, which the backward lane reproduces.
 // ONE: a single undulator/EPU arrives by value in the kernel arguments (read
 // through the scalar cache into registers); otherwise fields are read from F.
 template <int M>
@@ -58,7 +82,11 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
   // chain per particle, and when it runs beside other kernels (the previous
   // step's sums) it should not wait behind their waves on its SIMD.
   __builtin_amdgcn_s_setprio(3);
-  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int P = rk4_lanes<M>();
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ip = gid / P;
+  const int sub = gid % P;
+  const int dir = M == kEpuZ ? sub >> 1 : sub;  // 0 forward, 1 backward
   RkCtx c{F, nfields, 0.0, 0, g, nefields, 0.0, 0};
   if (M == kGridZ) {
     // stage the map in LDS (all lanes), then integrate
@@ -66,10 +94,16 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
     __syncthreads();
     c.g.data = lds_grid;
   }
-  if (ip >= np) return;
+  if (ip >= np) return;  // all lanes of a particle together
   // the single undulator's constants, in vector registers (see in_vgpr)
-  const UndParams f0 = (M == kOneUndulator || (M >= kAxisUndX && M <= kAxisUndZ) || M == kUndZ || M == kEpuZ)
-                           ? und_in_vgpr(f_arg) : f_arg;
+  UndParams f0 = (M == kOneUndulator || (M >= kAxisUndX && M <= kAxisUndZ) || M == kUndZ || M == kEpuZ)
+                     ? und_in_vgpr(f_arg) : f_arg;
+  if (M == kEpuZ && (sub & 1)) {
+    // this lane evaluates part B in the place of part A
+    f0.psA = f0.psB;
+    f0.a0 = f0.b0; f0.a1 = f0.b1; f0.a2 = f0.b2;
+    for (int k = 0; k < 6; ++k) f0.thA[k] = f0.thB[k];
+  }
   TrajDesc D = desc[ip];
   c.qomg = D.qomg;
   c.qom = D.qom;
@@ -79,28 +113,37 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
   x[0] = D.x0[0]; x[1] = D.b0[0] * kC;
   x[2] = D.x0[1]; x[3] = D.b0[1] * kC;
   x[4] = D.x0[2]; x[5] = D.b0[2] * kC;
+  const bool fwd = dir == 0 && D.nf > 0;
+  // steps whose result is stored: nb backward, nf - 1 forward
+  const long n = dir ? D.nb : (D.nf > 0 ? D.nf - 1 : 0);
+  const double h = dir ? -D.dt : D.dt;
   int status = 0;
-  derivs<M>(c, f0, D.t0_over_c, x, dxdt);
-  for (long i = 0; i < D.nf; ++i) {
-    double tt = D.t0_over_c + D.dt * (double) i;
-    if (i == 0) derivs<M>(c, f0, tt, x, dxdt);
-    store_knot(tk, pk, D.nb + i, tt, x, dxdt);
-    derivs<M>(c, f0, tt, x, dxdt);
-    if (!rk4<M>(c, f0, x, dxdt, tt, D.dt)) { status = 1; break; }
+  if (dir == 0 || D.nf == 0) derivs<M>(c, f0, D.t0_over_c, x, dxdt);
+  if (fwd) {
+    derivs<M>(c, f0, D.t0_over_c + h * 0.0, x, dxdt);
+    store_knot(tk, pk, D.nb, D.t0_over_c + h * 0.0, x, dxdt);
   }
-  x[0] = D.x0[0]; x[1] = D.b0[0] * kC;
-  x[2] = D.x0[1]; x[3] = D.b0[1] * kC;
-  x[4] = D.x0[2]; x[5] = D.b0[2] * kC;
-  double dtr = -D.dt;
-  for (long i = 0; i < D.nb && status == 0; ++i) {
-    double t1 = D.t0_over_c + dtr * (double) (i);
-    double t2 = D.t0_over_c + dtr * (double) (i + 1);
+  for (long i = 0; i < n; ++i) {
+    double t1 = D.t0_over_c + h * (double) i;
     derivs<M>(c, f0, t1, x, dxdt);
-    if (!rk4<M>(c, f0, x, dxdt, t1, dtr)) { status = 1; break; }
-    store_knot(tk, pk, D.nb - 1 - i, t2, x, dxdt);
+    if (!rk4<M>(c, f0, x, dxdt, t1, h)) { status = 1; break; }
+    store_knot(tk, pk, dir ? D.nb - 1 - i : D.nb + 1 + i, D.t0_over_c + h * (double) (i + 1), x, dxdt);
   }
-  desc[ip].status = status;
-  desc[ip].halved = c.halved;
+  if (fwd && status == 0) {
+    // the forward's last step: taken (its halving can fail) and not stored
+    double t1 = D.t0_over_c + h * (double) (D.nf - 1);
+    derivs<M>(c, f0, t1, x, dxdt);
+    if (!rk4<M>(c, f0, x, dxdt, t1, h)) status = 1;
+  }
+  // the two directions' outcomes, combined in the lanes of the particle
+  int halved = c.halved;
+  constexpr int Q = M == kEpuZ ? 0x4E : 0xB1;
+  status |= quad_other<Q>(status);
+  halved += quad_other<Q>(halved);
+  if (sub == 0) {
+    desc[ip].status = status;
+    desc[ip].halved = halved;
+  }
 }
 
 // ---------------------------------------------------------------- spline
@@ -383,13 +426,15 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     memset(&u0, 0, sizeof(u0));
     om::GridZ g0;
     memset(&g0, 0, sizeof(g0));
-    int threads0 = 64, blocks0 = (np + threads0 - 1) / threads0;
+    int threads0 = 64, blocks0 = (np * om::rk4_lanes<om::kFieldsEB>() + threads0 - 1) / threads0;
     hipLaunchKernelGGL(om::rk4_kernel<om::kFieldsEB>, dim3(blocks0), dim3(threads0), 0, s, u0, g0, F, nfields, desc,
                        np, t, p, nefields);
     return hipGetLastError();
   }
   int threads = 64;
-  int blocks = (np + threads - 1) / threads;
+  // rk4_lanes lanes per particle: 2 (forward, backward), 4 for kEpuZ
+  int blocks = (np * om::rk4_lanes<om::kAxisUndY>() + threads - 1) / threads;
+  int blocks_epu = (np * om::rk4_lanes<om::kEpuZ>() + threads - 1) / threads;
   om::UndParams u;
   memset(&u, 0, sizeof(u));
   om::GridZ g;
@@ -452,7 +497,7 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
         hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
     } else if (axis_z && f.taper == 0 && f.type == om::kEPU) {
-      hipLaunchKernelGGL(om::rk4_kernel<om::kEpuZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
+      hipLaunchKernelGGL(om::rk4_kernel<om::kEpuZ>, dim3(blocks_epu), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
                          p, 0);
     } else if (axis_z && f.taper == 0) {
       hipLaunchKernelGGL(om::rk4_kernel<om::kUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,

@@ -134,6 +134,43 @@ def test_backward_flux_map_and_power_density():
     assert ef < 1e-9 and ep < 1e-9
 
 
+@gpu
+@pytest.mark.parametrize("field", ["undulator", "epu"])
+def test_windows_of_their_own_in_one_launch(field):
+    """Particles with windows of their own in one RK4 launch: forward only,
+    backward only, and both ways with different lengths.  A particle's forward
+    and backward integrations run in two lanes of one wave (four lanes for the
+    EPU, whose two parts take a lane each), so lanes of a wave step in opposite
+    directions and stop after different counts; each particle's knots must be
+    the oracle's integration of its own window."""
+    from oracle.pyoracle import Oracle
+    from oscars_amd.sr import sr
+    windows = [(0.0, 0.6, 3001), (-0.5, 0.0, 2501), (-0.3, 0.7, 5001), (-0.8, 0.2, 4001), (-0.1, 0.9, 2001),
+               (-0.6, 0.6, 6001), (0.0, 0.3, 1501)]
+
+    def setup(x):
+        x.set_particle_beam(energy_GeV=3, x0=[0, 0, 0], d0=[0, 0, 1], current=0.5)
+        if field == "epu":
+            x.add_bfield_epu(**EPU)
+        else:
+            x.add_bfield_undulator(**UNDULATOR)
+
+    s = sr()
+    setup(s)
+    dev = s._device()
+    dev.trajectories(s._fields, [s._om_particle(s._particle, ctstart=a, ctstop=b, npoints=n) for a, b, n in windows])
+    for j, (a, b, n) in enumerate(windows):
+        o = Oracle(nthreads=8)
+        setup(o)
+        o.set_ctstartstop(a, b)
+        o.set_npoints_trajectory(n)
+        T = o.calculate_trajectory()
+        t, p, ypp = dev.trajectory(j)
+        ek = _knot_err(p, T.p)
+        print("%s window (%g, %g), %d knots: %.3e" % (field, a, b, len(t), ek), flush=True)
+        assert np.array_equal(t, T.t) and ek < 1e-12
+
+
 def _planar8001(x):
     x.set_particle_beam(energy_GeV=3, x0=[0, 0, -1], d0=[0, 0, 1], current=0.5)
     x.add_bfield_undulator(**UNDULATOR)
