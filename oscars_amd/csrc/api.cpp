@@ -824,6 +824,9 @@ int lw_sums(om_context* ctx, int ip, const double* obs_xyz, long n_obs, const do
     // one launch per level over the points still active, compacted into full
     // waves (tiles of one observation point for the tiled kernels).
     int Lb = stop_ext < kListFrom - 1 ? stop_ext : kListFrom - 1;
+    // precision <= 0: no point stops before the cap (prec < precision never
+    // holds), so the dense launch takes every level and nothing is compacted
+    if (conv->precision <= 0 && stop_ext == stop_mem) Lb = stop_ext;
     long ntiles = 0;
     long n_recheck = 0;
     int Ldone = 0;  // deepest level summed so far (its samples exist)
@@ -2057,7 +2060,9 @@ int adjoint_forward(om_context* ctx, const om_field* fields, int nfields, int gr
   c.max_level = level;
   c.max_level_extended = 0;
   c.quantity = 0;
-  c.mode = OM_MODE_TILED;
+  // one lane per (point, energy) for a few energies: a tile's 64 energy lanes
+  // would be mostly idle
+  c.mode = n_e < 64 ? OM_MODE_PLAIN : OM_MODE_TILED;
   c.weight = 1;
   c.accumulate = 0;
   if (dev_io) {
