@@ -44,14 +44,150 @@ __global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p
   p[9 * k + c] = p[9 * k + c] / kC;
 }
 
+// ---------------------------------------------------------------- paired stages
+// Variants whose field depends on z alone.  In an RK4 step the z of stage 2 is
+// y4 + (h/2) k1[4] with k1[4] = y5, and once k1 and k2 are known the z of
+// stages 3 and 4 are y4 + (h/2) k2[4] and y4 + h k3[4], where k2[4] and k3[4]
+// are the vz of the stage-2 and stage-3 inputs, known before their fields.  So
+// the step's four field evaluations pair up, (1, 2) and then (3, 4), and two
+// lanes evaluate a pair at once and exchange the results; everything else runs
+// in both lanes in the order of derivs and rk4_try, so the doubles are theirs.
+template <int M>
+constexpr bool paired_z() {
+  return (M >= kAxisUndX && M <= kAxisUndZ) || M == kUndZ || M == kGridZ;
+}
+
+struct ZField {
+  double b0, b1, b2;  // kAxisUnd*: b0 is the field component, b1 = b2 = 0
+};
+
+// The field part of derivs<M> at z.
+template <int M>
+__device__ __forceinline__ ZField field_z(RkCtx& c, const UndParams& f0, double z) {
+  ZField F;
+  if (M >= kAxisUndX && M <= kAxisUndZ) {
+    constexpr int K = M - kAxisUndX;
+    double D = z - f0.c2;
+    double sc = und_scale(f0.thA, D);
+    double a = K == 0 ? f0.a0 : (K == 1 ? f0.a1 : f0.a2);
+    double s = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
+    F.b0 = sc != 0.0 ? a * sc * s : 0.0;
+    F.b1 = F.b2 = 0.0;
+  } else if (M == kUndZ) {
+    double D = z - f0.c2;
+    double scA = und_scale(f0.thA, D);
+    double sA = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
+    double v0 = f0.a0 * scA * sA, v1 = f0.a1 * scA * sA, v2 = f0.a2 * scA * sA;
+    F.b0 = 0.0 + v0;
+    F.b1 = 0.0 + v1;
+    F.b2 = 0.0 + v2;
+  } else {
+    double B[3];
+    grid_z_eval(c.g, c.gc_n0, c.gc, z, B);
+    F.b0 = B[0];
+    F.b1 = B[1];
+    F.b2 = B[2];
+  }
+  return F;
+}
+
+// The rest of derivs<M>, given the field at x[4].
+template <int M>
+__device__ __forceinline__ void derivs_z(RkCtx& c, const ZField& F, const double* x, double* d) {
+  const bool bad = beta_at_least_one(x[1], x[3], x[5]);
+  if (M >= kAxisUndX && M <= kAxisUndZ) {
+    constexpr int K = M - kAxisUndX;
+    double v = F.b0;
+    if (K == 0) {
+      d[1] = 0.0;
+      d[3] = c.qomg * (x[5] * v);
+      d[5] = c.qomg * (-(x[3] * v));
+    } else if (K == 1) {
+      d[1] = c.qomg * (-(x[5] * v));
+      d[3] = 0.0;
+      d[5] = c.qomg * (x[1] * v);
+    } else {
+      d[1] = c.qomg * (x[3] * v);
+      d[3] = c.qomg * (-(x[1] * v));
+      d[5] = 0.0;
+    }
+  } else {
+    d[1] = c.qomg * (-x[5] * F.b1 + x[3] * F.b2);
+    d[3] = c.qomg * (x[5] * F.b0 - x[1] * F.b2);
+    d[5] = c.qomg * (x[1] * F.b1 - x[3] * F.b0);
+  }
+  d[0] = x[1];
+  d[2] = x[3];
+  d[4] = x[5];
+  c.err |= bad ? 1 : 0;
+}
+
+// The fields at zA (evaluated by the slot-0 lane) and zB (slot 1), both
+// returned in both lanes.
+template <int M>
+__device__ __forceinline__ void field_pair(RkCtx& c, const UndParams& f0, int slot, double zA, double zB, ZField& FA,
+                                           ZField& FB) {
+  ZField m = field_z<M>(c, f0, slot ? zB : zA);
+  ZField o;
+  o.b0 = lane_pair_other(m.b0);
+  if (M >= kAxisUndX && M <= kAxisUndZ) {
+    o.b1 = o.b2 = 0.0;
+  } else {
+    o.b1 = lane_pair_other(m.b1);
+    o.b2 = lane_pair_other(m.b2);
+  }
+  FA = slot ? o : m;
+  FB = slot ? m : o;
+}
+
+// derivs<M> at (t, y) into dydx followed by rk4<M>(y, dydx, t, h), with the
+// step's fields evaluated in lane pairs.  A step that meets |beta| >= 1 is
+// redone by the halving walk from y with the slope dydx, as in rk4.
+template <int M>
+__device__ __forceinline__ bool rk4_paired(RkCtx& c, const UndParams& f0, int slot, double* y, double* dydx, double t,
+                                           double h) {
+  double hh = h * 0.5;
+  double h6 = h / 6.0;
+  double k2[6], k3[6], k4[6], yt[6], y3[6], yo[6];
+  ZField F1, F2, F3, F4;
+  field_pair<M>(c, f0, slot, y[4], y[4] + hh * y[5], F1, F2);
+  derivs_z<M>(c, F1, y, dydx);
+  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];  // yt[4] is the second z above (dydx[4] = y[5])
+  derivs_z<M>(c, F2, yt, k2);
+  for (int i = 0; i < 6; ++i) y3[i] = y[i] + hh * k2[i];
+  field_pair<M>(c, f0, slot, y3[4], y[4] + h * y3[5], F3, F4);
+  derivs_z<M>(c, F3, y3, k3);
+  for (int i = 0; i < 6; ++i) yt[i] = y[i] + h * k3[i];  // yt[4] is the fourth z (k3[4] = y3[5])
+  derivs_z<M>(c, F4, yt, k4);
+  for (int i = 0; i < 6; ++i) yo[i] = y[i] + h6 * (dydx[i] + k4[i] + 2.0 * (k3[i] + k2[i]));
+  if (c.err || beta_at_least_one(yo[1], yo[3], yo[5])) {
+    ++c.halved;
+    RkPair s;
+    for (int i = 0; i < 6; ++i) {
+      s.y[i] = y[i];
+      s.d[i] = dydx[i];
+    }
+    s = rk4_halving<M>(c, f0, s, t, h);
+    for (int i = 0; i < 6; ++i) {
+      y[i] = s.y[i];
+      dydx[i] = s.d[i];
+    }
+    c.err = s.err;
+    return s.ok != 0;
+  }
+  for (int i = 0; i < 6; ++i) y[i] = yo[i];
+  return true;
+}
+
 // Lanes per particle.  The forward and the backward integration both start
 // from the state at T0 and never meet (src/OSCARSSR.cc:943-1006), so they run
 // in two lanes of one wave: a window with T0 inside costs the longer of the two
 // instead of their sum.  kEpuZ pairs lanes once more: the two lanes of a
-// direction evaluate the EPU's parts A and B one each (see derivs).
+// direction evaluate the EPU's parts A and B one each (see derivs); the
+// paired_z variants pair them to evaluate two stages' fields at once.
 template <int M>
 constexpr int rk4_lanes() {
-  return M == kEpuZ ? 4 : 2;
+  return (M == kEpuZ || paired_z<M>()) ? 4 : 2;
 }
 
 // Value of x in the other lane of a quad given by the DPP quad permutation Q
@@ -86,7 +222,8 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int ip = gid / P;
   const int sub = gid % P;
-  const int dir = M == kEpuZ ? sub >> 1 : sub;  // 0 forward, 1 backward
+  const int dir = P == 4 ? sub >> 1 : sub;  // 0 forward, 1 backward
+  const int slot = sub & 1;                  // paired_z: the stage of a field pair this lane evaluates
   RkCtx c{F, nfields, 0.0, 0, g, nefields, 0.0, 0};
   if (M == kGridZ) {
     // stage the map in LDS (all lanes), then integrate
@@ -125,19 +262,27 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
   }
   for (long i = 0; i < n; ++i) {
     double t1 = D.t0_over_c + h * (double) i;
-    derivs<M>(c, f0, t1, x, dxdt);
-    if (!rk4<M>(c, f0, x, dxdt, t1, h)) { status = 1; break; }
+    if (paired_z<M>()) {
+      if (!rk4_paired<M>(c, f0, slot, x, dxdt, t1, h)) { status = 1; break; }
+    } else {
+      derivs<M>(c, f0, t1, x, dxdt);
+      if (!rk4<M>(c, f0, x, dxdt, t1, h)) { status = 1; break; }
+    }
     store_knot(tk, pk, dir ? D.nb - 1 - i : D.nb + 1 + i, D.t0_over_c + h * (double) (i + 1), x, dxdt);
   }
   if (fwd && status == 0) {
     // the forward's last step: taken (its halving can fail) and not stored
     double t1 = D.t0_over_c + h * (double) (D.nf - 1);
-    derivs<M>(c, f0, t1, x, dxdt);
-    if (!rk4<M>(c, f0, x, dxdt, t1, h)) status = 1;
+    if (paired_z<M>()) {
+      if (!rk4_paired<M>(c, f0, slot, x, dxdt, t1, h)) status = 1;
+    } else {
+      derivs<M>(c, f0, t1, x, dxdt);
+      if (!rk4<M>(c, f0, x, dxdt, t1, h)) status = 1;
+    }
   }
   // the two directions' outcomes, combined in the lanes of the particle
   int halved = c.halved;
-  constexpr int Q = M == kEpuZ ? 0x4E : 0xB1;
+  constexpr int Q = P == 4 ? 0x4E : 0xB1;
   status |= quad_other<Q>(status);
   halved += quad_other<Q>(halved);
   if (sub == 0) {
@@ -432,9 +577,12 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     return hipGetLastError();
   }
   int threads = 64;
-  // rk4_lanes lanes per particle: 2 (forward, backward), 4 for kEpuZ
-  int blocks = (np * om::rk4_lanes<om::kAxisUndY>() + threads - 1) / threads;
-  int blocks_epu = (np * om::rk4_lanes<om::kEpuZ>() + threads - 1) / threads;
+  // rk4_lanes lanes per particle: 2 (forward, backward), 4 for kEpuZ and the paired_z variants
+  int blocks = (np * om::rk4_lanes<om::kOneUndulator>() + threads - 1) / threads;
+  int blocks4 = (np * 4 + threads - 1) / threads;  // kEpuZ and the paired_z variants
+  static_assert(om::rk4_lanes<om::kEpuZ>() == 4 && om::rk4_lanes<om::kAxisUndY>() == 4 &&
+                    om::rk4_lanes<om::kUndZ>() == 4 && om::rk4_lanes<om::kGridZ>() == 4,
+                "lanes per particle");
   om::UndParams u;
   memset(&u, 0, sizeof(u));
   om::GridZ g;
@@ -456,7 +604,7 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     hipError_t e = hipFuncSetAttribute((const void*) om::rk4_kernel<om::kGridZ>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) grid_bytes);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(om::rk4_kernel<om::kGridZ>, dim3(blocks), dim3(threads), grid_bytes, s, u, g, F, nfields, desc,
+    hipLaunchKernelGGL(om::rk4_kernel<om::kGridZ>, dim3(blocks4), dim3(threads), grid_bytes, s, u, g, F, nfields, desc,
                        np, t, p, 0);
     return hipGetLastError();
   }
@@ -488,19 +636,19 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     bool axis_z = f.unit[0] == 0 && f.unit[1] == 0 && f.unit[2] == 1;
     if (f.type == om::kUndulator && axis_z && f.taper == 0 && nz == 1) {
       if (f.field[0] != 0)
-        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndX>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndX>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
       else if (f.field[1] != 0)
-        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndY>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndY>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
       else
-        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
     } else if (axis_z && f.taper == 0 && f.type == om::kEPU) {
-      hipLaunchKernelGGL(om::rk4_kernel<om::kEpuZ>, dim3(blocks_epu), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
+      hipLaunchKernelGGL(om::rk4_kernel<om::kEpuZ>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
                          p, 0);
     } else if (axis_z && f.taper == 0) {
-      hipLaunchKernelGGL(om::rk4_kernel<om::kUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
+      hipLaunchKernelGGL(om::rk4_kernel<om::kUndZ>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
                          p, 0);
     } else {
       hipLaunchKernelGGL(om::rk4_kernel<om::kOneUndulator>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc,
