@@ -54,7 +54,19 @@ __global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p
 // in both lanes in the order of derivs and rk4_try, so the doubles are theirs.
 template <int M>
 constexpr bool paired_z() {
-  return (M >= kAxisUndX && M <= kAxisUndZ) || M == kUndZ || M == kGridZ;
+  return (M >= kAxisUndX && M <= kAxisUndZ) || M == kUndZ || M == kEpuZ || M == kGridZ;
+}
+
+// x in the lane of the other stage of a field pair: the neighbouring lane, or
+// for kEpuZ (whose neighbouring lanes hold the EPU's two parts) the lane two
+// away, by a DPP quad permutation of both halves: exact.
+template <int M>
+__device__ __forceinline__ double stage_other(double x) {
+  constexpr int Q = M == kEpuZ ? 0x4E : 0xB1;
+  long long b = __builtin_bit_cast(long long, x);
+  int lo = __builtin_amdgcn_update_dpp(0, (int) (b & 0xffffffff), Q, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), Q, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long) hi << 32) | (unsigned) lo);
 }
 
 struct ZField {
@@ -73,11 +85,17 @@ __device__ __forceinline__ ZField field_z(RkCtx& c, const UndParams& f0, double 
     double s = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
     F.b0 = sc != 0.0 ? a * sc * s : 0.0;
     F.b1 = F.b2 = 0.0;
-  } else if (M == kUndZ) {
+  } else if (M == kUndZ || M == kEpuZ) {
     double D = z - f0.c2;
     double scA = und_scale(f0.thA, D);
     double sA = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
     double v0 = f0.a0 * scA * sA, v1 = f0.a1 * scA * sA, v2 = f0.a2 * scA * sA;
+    if (M == kEpuZ) {
+      // this lane's part (A or B, see rk4_kernel) plus the neighbour's, as in derivs
+      v0 = v0 + lane_pair_other(v0);
+      v1 = v1 + lane_pair_other(v1);
+      v2 = v2 + lane_pair_other(v2);
+    }
     F.b0 = 0.0 + v0;
     F.b1 = 0.0 + v1;
     F.b2 = 0.0 + v2;
@@ -129,12 +147,12 @@ __device__ __forceinline__ void field_pair(RkCtx& c, const UndParams& f0, int sl
                                            ZField& FB) {
   ZField m = field_z<M>(c, f0, slot ? zB : zA);
   ZField o;
-  o.b0 = lane_pair_other(m.b0);
+  o.b0 = stage_other<M>(m.b0);
   if (M >= kAxisUndX && M <= kAxisUndZ) {
     o.b1 = o.b2 = 0.0;
   } else {
-    o.b1 = lane_pair_other(m.b1);
-    o.b2 = lane_pair_other(m.b2);
+    o.b1 = stage_other<M>(m.b1);
+    o.b2 = stage_other<M>(m.b2);
   }
   FA = slot ? o : m;
   FB = slot ? m : o;
@@ -181,20 +199,13 @@ __device__ __forceinline__ bool rk4_paired(RkCtx& c, const UndParams& f0, int sl
 
 // Lanes per particle.  The forward and the backward integration both start
 // from the state at T0 and never meet (src/OSCARSSR.cc:943-1006), so they run
-// in two lanes of one wave: a window with T0 inside costs the longer of the two
-// instead of their sum.  kEpuZ pairs lanes once more: the two lanes of a
-// direction evaluate the EPU's parts A and B one each (see derivs); the
-// paired_z variants pair them to evaluate two stages' fields at once.
+// in two halves of a particle's lanes: a window with T0 inside costs the longer
+// of the two instead of their sum.  The paired_z variants take two lanes per
+// direction, one per stage of a field pair; kEpuZ two more, the EPU's parts A
+// and B (lane bit 0: part, bit 1: stage, bit 2: direction).
 template <int M>
 constexpr int rk4_lanes() {
-  return (M == kEpuZ || paired_z<M>()) ? 4 : 2;
-}
-
-// Value of x in the other lane of a quad given by the DPP quad permutation Q
-// (0xB1: lanes 0<->1 and 2<->3; 0x4E: lanes 0<->2 and 1<->3).
-template <int Q>
-__device__ __forceinline__ int quad_other(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, Q, 0xF, 0xF, false);
+  return M == kEpuZ ? 8 : (paired_z<M>() ? 4 : 2);
 }
 
 // Forward points go to knot nb+i, backward points to knot nb-1-i, which is the
@@ -222,8 +233,8 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int ip = gid / P;
   const int sub = gid % P;
-  const int dir = P == 4 ? sub >> 1 : sub;  // 0 forward, 1 backward
-  const int slot = sub & 1;                  // paired_z: the stage of a field pair this lane evaluates
+  const int dir = sub / (P / 2);                   // 0 forward, 1 backward
+  const int slot = M == kEpuZ ? (sub >> 1) & 1 : sub & 1;  // paired_z: the stage of a field pair this lane evaluates
   RkCtx c{F, nfields, 0.0, 0, g, nefields, 0.0, 0};
   if (M == kGridZ) {
     // stage the map in LDS (all lanes), then integrate
@@ -282,9 +293,8 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
   }
   // the two directions' outcomes, combined in the lanes of the particle
   int halved = c.halved;
-  constexpr int Q = P == 4 ? 0x4E : 0xB1;
-  status |= quad_other<Q>(status);
-  halved += quad_other<Q>(halved);
+  status |= __shfl_xor(status, P / 2);
+  halved += __shfl_xor(halved, P / 2);
   if (sub == 0) {
     desc[ip].status = status;
     desc[ip].halved = halved;
@@ -577,10 +587,11 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     return hipGetLastError();
   }
   int threads = 64;
-  // rk4_lanes lanes per particle: 2 (forward, backward), 4 for kEpuZ and the paired_z variants
+  // rk4_lanes lanes per particle: 2 (forward, backward), 4 for the paired_z variants, 8 for kEpuZ
   int blocks = (np * om::rk4_lanes<om::kOneUndulator>() + threads - 1) / threads;
-  int blocks4 = (np * 4 + threads - 1) / threads;  // kEpuZ and the paired_z variants
-  static_assert(om::rk4_lanes<om::kEpuZ>() == 4 && om::rk4_lanes<om::kAxisUndY>() == 4 &&
+  int blocks4 = (np * 4 + threads - 1) / threads;  // the paired_z variants
+  int blocks8 = (np * 8 + threads - 1) / threads;  // kEpuZ
+  static_assert(om::rk4_lanes<om::kEpuZ>() == 8 && om::rk4_lanes<om::kAxisUndY>() == 4 &&
                     om::rk4_lanes<om::kUndZ>() == 4 && om::rk4_lanes<om::kGridZ>() == 4,
                 "lanes per particle");
   om::UndParams u;
@@ -645,7 +656,7 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
         hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
     } else if (axis_z && f.taper == 0 && f.type == om::kEPU) {
-      hipLaunchKernelGGL(om::rk4_kernel<om::kEpuZ>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
+      hipLaunchKernelGGL(om::rk4_kernel<om::kEpuZ>, dim3(blocks8), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
                          p, 0);
     } else if (axis_z && f.taper == 0) {
       hipLaunchKernelGGL(om::rk4_kernel<om::kUndZ>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
