@@ -45,17 +45,22 @@ __global__ void __launch_bounds__(256) knot_finish_kernel(double* __restrict__ p
 }
 
 // ---------------------------------------------------------------- paired stages
-// Variants whose field depends on z alone.  In an RK4 step the z of stage 2 is
-// y4 + (h/2) k1[4] with k1[4] = y5, and once k1 and k2 are known the z of
-// stages 3 and 4 are y4 + (h/2) k2[4] and y4 + h k3[4], where k2[4] and k3[4]
-// are the vz of the stage-2 and stage-3 inputs, known before their fields.  So
-// the step's four field evaluations pair up, (1, 2) and then (3, 4), and two
-// lanes evaluate a pair at once and exchange the results; everything else runs
-// in both lanes in the order of derivs and rk4_try, so the doubles are theirs.
-template <int M>
-constexpr bool paired_z() {
-  return (M >= kAxisUndX && M <= kAxisUndZ) || M == kUndZ || M == kEpuZ || M == kGridZ;
-}
+// The fields depend on position and time, never on the velocity.  In an RK4
+// step from (t, y) with h2 = h/2 the four stages take their fields at the
+// positions y_p, y_p + h2 k1_p, y_p + h2 k2_p and y_p + h k3_p (times t, t + h2,
+// t + h2, t + h), and the position rows of a slope are the stage input's
+// velocity: k1_p = y_v, k2_p = (y + h2 k1)_v, k3_p = (y + h2 k2)_v.  So the
+// fields of stages 1 and 2 follow from y alone and those of stages 3 and 4 once
+// k1 and k2 are known: the step's four field evaluations pair up, (1, 2) and
+// then (3, 4), and two lanes evaluate a pair at once and exchange the results.
+// Everything else runs in both lanes in the order of derivs and rk4_try, so the
+// doubles are theirs.
+// Scalar members (an array member would make the lane selects go through
+// scratch memory).
+struct StageField {
+  double b0, b1, b2;  // kAxisUnd*: b0 is the field component, b1 = b2 = 0
+  double e0, e1, e2;  // kFieldsEB: the electric field
+};
 
 // x in the lane of the other stage of a field pair: the neighbouring lane, or
 // for kEpuZ (whose neighbouring lanes hold the EPU's two parts) the lane two
@@ -69,24 +74,24 @@ __device__ __forceinline__ double stage_other(double x) {
   return __builtin_bit_cast(double, ((long long) hi << 32) | (unsigned) lo);
 }
 
-struct ZField {
-  double b0, b1, b2;  // kAxisUnd*: b0 is the field component, b1 = b2 = 0
-};
-
-// The field part of derivs<M> at z.
+// The field part of derivs<M> at (px, py, pz, t).
 template <int M>
-__device__ __forceinline__ ZField field_z(RkCtx& c, const UndParams& f0, double z) {
-  ZField F;
+__device__ __forceinline__ StageField field_at(RkCtx& c, const UndParams& f0, double px, double py, double pz,
+                                               double t) {
+  StageField F;
+  F.e0 = F.e1 = F.e2 = 0.0;
+  double B[3] = {0.0, 0.0, 0.0}, E[3] = {0.0, 0.0, 0.0};
   if (M >= kAxisUndX && M <= kAxisUndZ) {
     constexpr int K = M - kAxisUndX;
-    double D = z - f0.c2;
+    double D = pz - f0.c2;
     double sc = und_scale(f0.thA, D);
     double a = K == 0 ? f0.a0 : (K == 1 ? f0.a1 : f0.a2);
     double s = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
     F.b0 = sc != 0.0 ? a * sc * s : 0.0;
     F.b1 = F.b2 = 0.0;
+    return F;
   } else if (M == kUndZ || M == kEpuZ) {
-    double D = z - f0.c2;
+    double D = pz - f0.c2;
     double scA = und_scale(f0.thA, D);
     double sA = sin64(div_by(2.0 * kPi * (D - f0.psA), f0.PL, f0.inv_PL));
     double v0 = f0.a0 * scA * sA, v1 = f0.a1 * scA * sA, v2 = f0.a2 * scA * sA;
@@ -99,19 +104,42 @@ __device__ __forceinline__ ZField field_z(RkCtx& c, const UndParams& f0, double 
     F.b0 = 0.0 + v0;
     F.b1 = 0.0 + v1;
     F.b2 = 0.0 + v2;
+    return F;
+  } else if (M == kGridZ) {
+    grid_z_eval(c.g, c.gc_n0, c.gc, pz, B);
+  } else if (M == kOneUndulator) {
+    und_eval(f0, px, py, pz, B);
+  } else if (M == kFieldsEB) {
+    get_b(c.F, c.nf, px, py, pz, t, B);
+    get_b(c.F + c.nf, c.nfe, px, py, pz, t, E);
   } else {
-    double B[3];
-    grid_z_eval(c.g, c.gc_n0, c.gc, z, B);
-    F.b0 = B[0];
-    F.b1 = B[1];
-    F.b2 = B[2];
+    get_b(c.F, c.nf, px, py, pz, t, B);
   }
+  F.b0 = B[0];
+  F.b1 = B[1];
+  F.b2 = B[2];
+  F.e0 = E[0];
+  F.e1 = E[1];
+  F.e2 = E[2];
   return F;
 }
 
-// The rest of derivs<M>, given the field at x[4].
+// The rest of derivs<M>, given the fields at the position of x.
 template <int M>
-__device__ __forceinline__ void derivs_z(RkCtx& c, const ZField& F, const double* x, double* d) {
+__device__ __forceinline__ void derivs_with(RkCtx& c, const StageField& F, const double* x, double* d) {
+  if (M == kFieldsEB) {
+    if (beta_at_least_one(x[1], x[3], x[5])) c.err = 1;
+    double bb = (x[1] * x[1] + x[3] * x[3] + x[5] * x[5]) / (kC * kC);
+    double q = c.qom * sqrt(1. - bb);
+    double be = (x[1] * F.e0 + x[3] * F.e1 + x[5] * F.e2) / kC;
+    d[0] = x[1];
+    d[1] = q * (F.e0 - x[5] * F.b1 + x[3] * F.b2 - x[1] * be / kC);
+    d[2] = x[3];
+    d[3] = q * (F.e1 + x[5] * F.b0 - x[1] * F.b2 - x[3] * be / kC);
+    d[4] = x[5];
+    d[5] = q * (F.e2 + x[1] * F.b1 - x[3] * F.b0 - x[5] * be / kC);
+    return;
+  }
   const bool bad = beta_at_least_one(x[1], x[3], x[5]);
   if (M >= kAxisUndX && M <= kAxisUndZ) {
     constexpr int K = M - kAxisUndX;
@@ -140,22 +168,33 @@ __device__ __forceinline__ void derivs_z(RkCtx& c, const ZField& F, const double
   c.err |= bad ? 1 : 0;
 }
 
-// The fields at zA (evaluated by the slot-0 lane) and zB (slot 1), both
+// The fields at A (evaluated by the slot-0 lane) and B (slot 1), both
 // returned in both lanes.
 template <int M>
-__device__ __forceinline__ void field_pair(RkCtx& c, const UndParams& f0, int slot, double zA, double zB, ZField& FA,
-                                           ZField& FB) {
-  ZField m = field_z<M>(c, f0, slot ? zB : zA);
-  ZField o;
+__device__ __forceinline__ void field_pair(RkCtx& c, const UndParams& f0, int slot, double xa, double ya, double za,
+                                           double ta, double xb, double yb, double zb, double tb, StageField& FA,
+                                           StageField& FB) {
+  StageField m = field_at<M>(c, f0, slot ? xb : xa, slot ? yb : ya, slot ? zb : za, slot ? tb : ta);
+  constexpr bool one = M >= kAxisUndX && M <= kAxisUndZ;
+  StageField o;
   o.b0 = stage_other<M>(m.b0);
-  if (M >= kAxisUndX && M <= kAxisUndZ) {
-    o.b1 = o.b2 = 0.0;
-  } else {
-    o.b1 = stage_other<M>(m.b1);
-    o.b2 = stage_other<M>(m.b2);
-  }
-  FA = slot ? o : m;
-  FB = slot ? m : o;
+  o.b1 = one ? 0.0 : stage_other<M>(m.b1);
+  o.b2 = one ? 0.0 : stage_other<M>(m.b2);
+  o.e0 = M == kFieldsEB ? stage_other<M>(m.e0) : 0.0;
+  o.e1 = M == kFieldsEB ? stage_other<M>(m.e1) : 0.0;
+  o.e2 = M == kFieldsEB ? stage_other<M>(m.e2) : 0.0;
+  FA.b0 = slot ? o.b0 : m.b0;
+  FA.b1 = slot ? o.b1 : m.b1;
+  FA.b2 = slot ? o.b2 : m.b2;
+  FA.e0 = slot ? o.e0 : m.e0;
+  FA.e1 = slot ? o.e1 : m.e1;
+  FA.e2 = slot ? o.e2 : m.e2;
+  FB.b0 = slot ? m.b0 : o.b0;
+  FB.b1 = slot ? m.b1 : o.b1;
+  FB.b2 = slot ? m.b2 : o.b2;
+  FB.e0 = slot ? m.e0 : o.e0;
+  FB.e1 = slot ? m.e1 : o.e1;
+  FB.e2 = slot ? m.e2 : o.e2;
 }
 
 // derivs<M> at (t, y) into dydx followed by rk4<M>(y, dydx, t, h), with the
@@ -166,17 +205,19 @@ __device__ __forceinline__ bool rk4_paired(RkCtx& c, const UndParams& f0, int sl
                                            double h) {
   double hh = h * 0.5;
   double h6 = h / 6.0;
+  double xh = t + hh;
   double k2[6], k3[6], k4[6], yt[6], y3[6], yo[6];
-  ZField F1, F2, F3, F4;
-  field_pair<M>(c, f0, slot, y[4], y[4] + hh * y[5], F1, F2);
-  derivs_z<M>(c, F1, y, dydx);
-  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];  // yt[4] is the second z above (dydx[4] = y[5])
-  derivs_z<M>(c, F2, yt, k2);
+  StageField F1, F2, F3, F4;
+  field_pair<M>(c, f0, slot, y[0], y[2], y[4], t, y[0] + hh * y[1], y[2] + hh * y[3], y[4] + hh * y[5], xh, F1, F2);
+  derivs_with<M>(c, F1, y, dydx);
+  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];  // its positions are stage 2's above
+  derivs_with<M>(c, F2, yt, k2);
   for (int i = 0; i < 6; ++i) y3[i] = y[i] + hh * k2[i];
-  field_pair<M>(c, f0, slot, y3[4], y[4] + h * y3[5], F3, F4);
-  derivs_z<M>(c, F3, y3, k3);
-  for (int i = 0; i < 6; ++i) yt[i] = y[i] + h * k3[i];  // yt[4] is the fourth z (k3[4] = y3[5])
-  derivs_z<M>(c, F4, yt, k4);
+  field_pair<M>(c, f0, slot, y3[0], y3[2], y3[4], xh, y[0] + h * y3[1], y[2] + h * y3[3], y[4] + h * y3[5], t + h,
+                F3, F4);
+  derivs_with<M>(c, F3, y3, k3);
+  for (int i = 0; i < 6; ++i) yt[i] = y[i] + h * k3[i];  // its positions are stage 4's above
+  derivs_with<M>(c, F4, yt, k4);
   for (int i = 0; i < 6; ++i) yo[i] = y[i] + h6 * (dydx[i] + k4[i] + 2.0 * (k3[i] + k2[i]));
   if (c.err || beta_at_least_one(yo[1], yo[3], yo[5])) {
     ++c.halved;
@@ -200,12 +241,12 @@ __device__ __forceinline__ bool rk4_paired(RkCtx& c, const UndParams& f0, int sl
 // Lanes per particle.  The forward and the backward integration both start
 // from the state at T0 and never meet (src/OSCARSSR.cc:943-1006), so they run
 // in two halves of a particle's lanes: a window with T0 inside costs the longer
-// of the two instead of their sum.  The paired_z variants take two lanes per
-// direction, one per stage of a field pair; kEpuZ two more, the EPU's parts A
-// and B (lane bit 0: part, bit 1: stage, bit 2: direction).
+// of the two instead of their sum.  Each direction takes two lanes, one per
+// stage of a field pair (bit 0: stage, bit 1: direction); kEpuZ two more, the
+// EPU's parts A and B (bit 0: part, bit 1: stage, bit 2: direction).
 template <int M>
 constexpr int rk4_lanes() {
-  return M == kEpuZ ? 8 : (paired_z<M>() ? 4 : 2);
+  return M == kEpuZ ? 8 : 4;
 }
 
 // Forward points go to knot nb+i, backward points to knot nb-1-i, which is the
@@ -234,7 +275,7 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
   const int ip = gid / P;
   const int sub = gid % P;
   const int dir = sub / (P / 2);                   // 0 forward, 1 backward
-  const int slot = M == kEpuZ ? (sub >> 1) & 1 : sub & 1;  // paired_z: the stage of a field pair this lane evaluates
+  const int slot = M == kEpuZ ? (sub >> 1) & 1 : sub & 1;  // the stage of a field pair this lane evaluates
   RkCtx c{F, nfields, 0.0, 0, g, nefields, 0.0, 0};
   if (M == kGridZ) {
     // stage the map in LDS (all lanes), then integrate
@@ -273,23 +314,13 @@ __global__ void __launch_bounds__(64) rk4_kernel(const UndParams f_arg, const Gr
   }
   for (long i = 0; i < n; ++i) {
     double t1 = D.t0_over_c + h * (double) i;
-    if (paired_z<M>()) {
-      if (!rk4_paired<M>(c, f0, slot, x, dxdt, t1, h)) { status = 1; break; }
-    } else {
-      derivs<M>(c, f0, t1, x, dxdt);
-      if (!rk4<M>(c, f0, x, dxdt, t1, h)) { status = 1; break; }
-    }
+    if (!rk4_paired<M>(c, f0, slot, x, dxdt, t1, h)) { status = 1; break; }
     store_knot(tk, pk, dir ? D.nb - 1 - i : D.nb + 1 + i, D.t0_over_c + h * (double) (i + 1), x, dxdt);
   }
   if (fwd && status == 0) {
     // the forward's last step: taken (its halving can fail) and not stored
     double t1 = D.t0_over_c + h * (double) (D.nf - 1);
-    if (paired_z<M>()) {
-      if (!rk4_paired<M>(c, f0, slot, x, dxdt, t1, h)) status = 1;
-    } else {
-      derivs<M>(c, f0, t1, x, dxdt);
-      if (!rk4<M>(c, f0, x, dxdt, t1, h)) status = 1;
-    }
+    if (!rk4_paired<M>(c, f0, slot, x, dxdt, t1, h)) status = 1;
   }
   // the two directions' outcomes, combined in the lanes of the particle
   int halved = c.halved;
@@ -587,12 +618,11 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     return hipGetLastError();
   }
   int threads = 64;
-  // rk4_lanes lanes per particle: 2 (forward, backward), 4 for the paired_z variants, 8 for kEpuZ
-  int blocks = (np * om::rk4_lanes<om::kOneUndulator>() + threads - 1) / threads;
-  int blocks4 = (np * 4 + threads - 1) / threads;  // the paired_z variants
+  // rk4_lanes lanes per particle: 4 (two per direction), 8 for kEpuZ
+  int blocks = (np * 4 + threads - 1) / threads;   // every variant but kEpuZ
   int blocks8 = (np * 8 + threads - 1) / threads;  // kEpuZ
   static_assert(om::rk4_lanes<om::kEpuZ>() == 8 && om::rk4_lanes<om::kAxisUndY>() == 4 &&
-                    om::rk4_lanes<om::kUndZ>() == 4 && om::rk4_lanes<om::kGridZ>() == 4,
+                    om::rk4_lanes<om::kOneUndulator>() == 4 && om::rk4_lanes<om::kFieldsGeneric>() == 4,
                 "lanes per particle");
   om::UndParams u;
   memset(&u, 0, sizeof(u));
@@ -615,7 +645,7 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     hipError_t e = hipFuncSetAttribute((const void*) om::rk4_kernel<om::kGridZ>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) grid_bytes);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(om::rk4_kernel<om::kGridZ>, dim3(blocks4), dim3(threads), grid_bytes, s, u, g, F, nfields, desc,
+    hipLaunchKernelGGL(om::rk4_kernel<om::kGridZ>, dim3(blocks), dim3(threads), grid_bytes, s, u, g, F, nfields, desc,
                        np, t, p, 0);
     return hipGetLastError();
   }
@@ -647,19 +677,19 @@ extern "C" hipError_t om_launch_rk4(const om::FieldDev* F, const om::FieldDev* F
     bool axis_z = f.unit[0] == 0 && f.unit[1] == 0 && f.unit[2] == 1;
     if (f.type == om::kUndulator && axis_z && f.taper == 0 && nz == 1) {
       if (f.field[0] != 0)
-        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndX>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np,
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndX>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
       else if (f.field[1] != 0)
-        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndY>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np,
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndY>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
       else
-        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np,
+        hipLaunchKernelGGL(om::rk4_kernel<om::kAxisUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np,
                            t, p, 0);
     } else if (axis_z && f.taper == 0 && f.type == om::kEPU) {
       hipLaunchKernelGGL(om::rk4_kernel<om::kEpuZ>, dim3(blocks8), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
                          p, 0);
     } else if (axis_z && f.taper == 0) {
-      hipLaunchKernelGGL(om::rk4_kernel<om::kUndZ>, dim3(blocks4), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
+      hipLaunchKernelGGL(om::rk4_kernel<om::kUndZ>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc, np, t,
                          p, 0);
     } else {
       hipLaunchKernelGGL(om::rk4_kernel<om::kOneUndulator>, dim3(blocks), dim3(threads), 0, s, u, g, F, nfields, desc,
