@@ -135,25 +135,36 @@ def test_backward_flux_map_and_power_density():
 
 
 @gpu
-@pytest.mark.parametrize("field", ["undulator", "epu"])
+@pytest.mark.parametrize("field", ["undulator", "epu", "grid", "two_fields"])
 def test_windows_of_their_own_in_one_launch(field):
     """Particles with windows of their own in one RK4 launch: forward only,
     backward only, and both ways with different lengths.  A particle's forward
-    and backward integrations run in two lanes of one wave (four lanes for the
-    EPU, whose two parts take a lane each), so lanes of a wave step in opposite
-    directions and stop after different counts; each particle's knots must be
-    the oracle's integration of its own window."""
+    and backward integrations run side by side in one wave, each on two lanes
+    that evaluate a step's fields two stages at a time (and for the EPU on two
+    more, one per part), so lanes of a wave step in opposite directions and
+    stop after different counts; each particle's knots must be the oracle's
+    integration of its own window.  Fields: the planar undulator, the EPU, a
+    By(z) map with pole errors and an undulator plus a Gaussian kick (the
+    general variant)."""
     from oracle.pyoracle import Oracle
     from oscars_amd.sr import sr
     windows = [(0.0, 0.6, 3001), (-0.5, 0.0, 2501), (-0.3, 0.7, 5001), (-0.8, 0.2, 4001), (-0.1, 0.9, 2001),
                (-0.6, 0.6, 6001), (0.0, 0.3, 1501)]
+    samples = None
+    if field == "grid":
+        from oscars_amd import adjoint as A
+        samples = A.undulator_pole_map(4001, -1.0, 5e-4, sigma_amp=0.005, sigma_z=20e-6, seed=3)
 
     def setup(x):
         x.set_particle_beam(energy_GeV=3, x0=[0, 0, 0], d0=[0, 0, 1], current=0.5)
         if field == "epu":
             x.add_bfield_epu(**EPU)
+        elif field == "grid":
+            x.add_bfield_grid([0, 0, -1.0], [0, 0, 5e-4], [1, 1, 4001], samples)
         else:
             x.add_bfield_undulator(**UNDULATOR)
+            if field == "two_fields":
+                x.add_bfield_gaussian(bfield=[0.05, 0, 0], sigma=[0, 0, 0.2], translation=[0, 0, 0.3])
 
     s = sr()
     setup(s)
