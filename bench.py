@@ -428,6 +428,8 @@ def maps_leg(steps=4):
         outs[k] = (torch.zeros(n, dtype=torch.float64, **cuda), torch.zeros(n, dtype=torch.int32, **cuda),
                    torch.zeros(n, dtype=torch.uint8, **cuda))
 
+    calls = {}
+
     def step(mode):
         dev.trajectories(s._fields, [s._om_particle(s._particle)], check_status=False)
         ev, kms = 0.0, {}
@@ -436,9 +438,11 @@ def maps_leg(steps=4):
             st = dev.flux_dev(0, t_map, t_em[e], [0.0] * 6, dev.conv(mode="plain"), *outs[k])
             ev += st["evals"]
             kms[k] = st["kernel_ms"]
+            calls[k] = st["evals"]
         st = dev.flux_dev(0, t_sweep, t_es, [0.0] * 6, dev.conv(mode=mode), *outs["sweep"])
         ev += st["evals"]
         kms["sweep"] = st["kernel_ms"]
+        calls["sweep"] = st["evals"]
         return ev, kms
 
     res = {}
@@ -453,7 +457,8 @@ def maps_leg(steps=4):
             ev += e
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        res[mode] = {"ms_per_step": 1e3 * dt / steps, "evals_per_s": ev / dt, "kernel_ms": kms}
+        res[mode] = {"ms_per_step": 1e3 * dt / steps, "evals_per_s": ev / dt, "kernel_ms": kms,
+                     "evals_per_call": dict(calls)}
         vals[mode] = {k: (v[0].cpu().numpy(), v[1].cpu().numpy()) for k, v in outs.items()}
         log("maps %s: %.2f ms/step, %.4g evals/s" % (mode, 1e3 * dt / steps, ev / dt))
     errs = {}
@@ -579,6 +584,8 @@ def main():
     ap.add_argument("--no-pd", action="store_true")
     ap.add_argument("--power-density-only", action="store_true",
                     help="run the power-density leg alone and print its JSON (for rocprofv3 runs)")
+    ap.add_argument("--maps-only", action="store_true",
+                    help="run the single-electron maps leg alone and print its JSON (for rocprofv3 runs)")
     ap.add_argument("--counters", action="store_true",
                     help="one eager batch step after setup, nothing timed (for rocprofv3 --pmc runs)")
     ap.add_argument("--waves-per-cu", type=int, default=16, help="persistent grid of a --counters run")
@@ -603,6 +610,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.power_density_only:
         print(json.dumps(power_density_leg(), default=float))
+        return
+    if args.maps_only:
+        print(json.dumps(maps_leg(), default=float))
         return
     cfg = Config4(args.particles, args.seed)
     if args.counters:
