@@ -576,6 +576,11 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
   const bool live = s < A.nseg;
   const long ss = live ? s : A.nseg - 1;
   const FieldDev* __restrict__ F = stage_fields(A.F, A.nfields, A.lds_cap);
+  // With grid fields only, the fields at the four stages (field_grad) are the
+  // ones adj_step would evaluate again, to the bit (the same interpolation,
+  // summed from zero in field order), so the next state is formed from them.
+  bool all_grid = true;
+  for (int fi = 0; fi < A.nfields; ++fi) all_grid = all_grid && F[fi].type == kGrid;
   long i0 = ss * A.K, i1 = i0 + A.K < A.nf ? i0 + A.K : A.nf;
   double y[6];
   for (int r = 0; r < 6; ++r) y[r] = A.ckpt[6 * ss + r];
@@ -588,6 +593,9 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
   }
   double h = A.dt, hh = h * 0.5, h6 = h / 6.0;
   for (long i = i0; i < i1; ++i) {
+    // the recomputed state, for the reverse sweep
+    if (live && q == 0)
+      for (int r = 0; r < 6; ++r) A.ybuf[6 * i + r] = y[r];
     FieldD f0;
     field_grad(F, A.nfields, A.gfield, y, f0);
     double c[6];
@@ -645,7 +653,14 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
 #pragma unroll
       for (int m = 0; m < 6; ++m) M[r][m] = sh.M[grp][r][m];
     wave_sync();  // every lane has read J and M before the next step writes them
-    adj_step(F, A.nfields, A.qomg, A.dt, y);  // the next state, as the forward computes it
+    // the next state, as the forward computes it
+    if (all_grid) {
+      double k4[6];
+      fval(f3, A.qomg, y3, k4);
+      for (int r = 0; r < 6; ++r) y[r] = y[r] + h6 * (k1[r] + k4[r] + 2.0 * (k3[r] + k2[r]));
+    } else {
+      adj_step(F, A.nfields, A.qomg, A.dt, y);
+    }
   }
   if (!live) return;
   double* o = A.segmap + 42 * s;
@@ -686,11 +701,7 @@ __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
   if (s >= A.nseg) return;
   long i0 = s * A.K, i1 = i0 + A.K < A.nf ? i0 + A.K : A.nf;
   double y[6];
-  for (int r = 0; r < 6; ++r) y[r] = A.ckpt[6 * s + r];
-  for (long i = i0; i < i1; ++i) {
-    for (int r = 0; r < 6; ++r) A.ybuf[6 * i + r] = y[r];
-    if (i + 1 < i1) adj_step(F, A.nfields, A.qomg, A.dt, y);
-  }
+  // the segment's states, recomputed from its checkpoint by adj_seg_map_kernel
   double lam[6];
   for (int r = 0; r < 6; ++r) lam[r] = A.lam[6 * (s + 1) + r];
   double h = A.dt, hh = h * 0.5, h6 = h / 6.0;
