@@ -356,14 +356,14 @@ __device__ __forceinline__ void field_grad(const FieldDev* __restrict__ F, int n
     const double* lo = f.data + 3 * n0;
     for (int i = 0; i < 3; ++i) {
       r.B[i] += lo[i] + div_by((lo[3 + i] - lo[i]) * dd, step, istep);
-      double g = (lo[3 + i] - lo[i]) / step;
+      double g = div_by(lo[3 + i] - lo[i], step, istep);  // the IEEE quotient
       r.J[i][0] = a == 0 ? r.J[i][0] + g : r.J[i][0];
       r.J[i][1] = a == 1 ? r.J[i][1] + g : r.J[i][1];
       r.J[i][2] = a == 2 ? r.J[i][2] + g : r.J[i][2];
     }
     if (fi == gfield) {
       r.n0 = n0;
-      r.w = dd / step;
+      r.w = div_by(dd, step, istep);
     }
   }
 }
@@ -498,16 +498,16 @@ __device__ __forceinline__ void fjac_tvec(const FieldD& fd, double qomg, const d
 
 // c_i: direct knot terms and the late-stored acceleration terms of state i.
 // Acceleration knot a_{i+1} = f_a(y_i)/c, and a_0 = f_a(y_0)/c.
-__device__ __forceinline__ void direct_terms(const AdjRkArgs& A, long i, const FieldD& fd, const double* y,
-                                             double* c, long rec) {
-  const double* k = A.kb + 9 * i;
+// k: knot i's row of kb, k1: knot i+1's (read ahead by the segment kernels, so
+// the loads are in flight a step before their use).
+__device__ __forceinline__ void direct_terms(const AdjRkArgs& A, long i, const double* k, const double* k1,
+                                             const FieldD& fd, const double* y, double* c, long rec) {
   for (int r = 0; r < 6; ++r) c[r] = 0;
   c[0] = k[0]; c[2] = k[1]; c[4] = k[2];
   c[1] = k[3] / kC; c[3] = k[4] / kC; c[5] = k[5] / kC;
   double w[6] = {0, 0, 0, 0, 0, 0};
   bool any = false;
   if (i + 1 < A.nf) {
-    const double* k1 = A.kb + 9 * (i + 1);
     w[1] += k1[6] / kC; w[3] += k1[7] / kC; w[5] += k1[8] / kC;
     any = true;
   }
@@ -592,14 +592,26 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
     for (int m = 0; m < 6; ++m) M[r][m] = r == m ? 1.0 : 0.0;
   }
   double h = A.dt, hh = h * 0.5, h6 = h / 6.0;
+  // kb rows of knots i and i + 1, read one step ahead
+  double kc[9], kn[9];
+  for (int r = 0; r < 9; ++r) {
+    kc[r] = A.kb[9 * i0 + r];
+    kn[r] = i0 + 1 < A.nf ? A.kb[9 * (i0 + 1) + r] : 0.0;
+  }
   for (long i = i0; i < i1; ++i) {
+    double kf[9];
+    for (int r = 0; r < 9; ++r) kf[r] = i + 2 < A.nf ? A.kb[9 * (i + 2) + r] : 0.0;
     // the recomputed state, for the reverse sweep
     if (live && q == 0)
       for (int r = 0; r < 6; ++r) A.ybuf[6 * i + r] = y[r];
     FieldD f0;
     field_grad(F, A.nfields, A.gfield, y, f0);
     double c[6];
-    direct_terms(A, i, f0, y, c, -1);
+    direct_terms(A, i, kc, kn, f0, y, c, -1);
+    for (int r = 0; r < 9; ++r) {
+      kc[r] = kn[r];
+      kn[r] = kf[r];
+    }
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
       double acc = b[r];
@@ -705,7 +717,15 @@ __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
   double lam[6];
   for (int r = 0; r < 6; ++r) lam[r] = A.lam[6 * (s + 1) + r];
   double h = A.dt, hh = h * 0.5, h6 = h / 6.0;
+  // kb rows of knots i and i + 1, the next one down read one step ahead
+  double kc[9], kn[9];
+  for (int r = 0; r < 9; ++r) {
+    kc[r] = A.kb[9 * (i1 - 1) + r];
+    kn[r] = i1 < A.nf ? A.kb[9 * i1 + r] : 0.0;
+  }
   for (long i = i1 - 1; i >= i0; --i) {
+    double kf[9];
+    for (int r = 0; r < 9; ++r) kf[r] = i > i0 ? A.kb[9 * (i - 1) + r] : 0.0;
     for (int r = 0; r < 6; ++r) y[r] = A.ybuf[6 * i + r];
     FieldD f0;
     field_grad(F, A.nfields, A.gfield, y, f0);
@@ -755,8 +775,14 @@ __global__ void __launch_bounds__(64) adj_seg_reverse_kernel(AdjRkArgs A) {
       fjac_tvec(f0, A.qomg, y, k1b, yb, A, 10 * i + 6);   // k1 = f(y)
     }
     double c[6];
-    direct_terms(A, i, f0, y, c, 10 * i + 8);
-    for (int r = 0; r < 6; ++r) lam[r] = yb[r] + c[r];
+    direct_terms(A, i, kc, kn, f0, y, c, 10 * i + 8);
+    for (int r = 0; r < 6; ++r) {
+      lam[r] = yb[r] + c[r];
+    }
+    for (int r = 0; r < 9; ++r) {
+      kn[r] = kc[r];
+      kc[r] = kf[r];
+    }
   }
   if (s == 0) {
     for (int r = 0; r < 6; ++r) A.lam[r] = lam[r];
