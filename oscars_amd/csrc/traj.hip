@@ -74,6 +74,27 @@ __device__ __forceinline__ double stage_other(double x) {
   return __builtin_bit_cast(double, ((long long) hi << 32) | (unsigned) lo);
 }
 
+// grid_z_eval without the cached cell and without branches: the lanes of a
+// pair evaluate different stages, so a cache miss in one lane would run the
+// reload with the other masked off, and the test itself sits on the chain.
+// The cell index is clamped into the map (the value is then replaced by the
+// zero outside it), and the samples are read from LDS every time; the doubles
+// are grid_z_eval's.
+__device__ __forceinline__ void grid_z_direct(const GridZ& g, double z, double B[3]) {
+  double coord = z - g.trans;
+  bool out = coord <= g.start || coord >= g.stop;
+  int n0 = cell_of(div_by(coord - g.start, g.step, g.inv_step));
+  n0 = n0 > (int) (g.n - 2) ? (int) (g.n - 2) : n0;
+  n0 = n0 < 0 ? 0 : n0;
+  const double* lo = g.data + 3 * n0;
+  double dd = (coord - g.start) - (double) n0 * g.step;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double v = lo[k] + div_by((lo[3 + k] - lo[k]) * dd, g.step, g.inv_step);
+    B[k] = out ? 0.0 : v;
+  }
+}
+
 // The field part of derivs<M> at (px, py, pz, t).
 template <int M>
 __device__ __forceinline__ StageField field_at(RkCtx& c, const UndParams& f0, double px, double py, double pz,
@@ -106,7 +127,7 @@ __device__ __forceinline__ StageField field_at(RkCtx& c, const UndParams& f0, do
     F.b2 = 0.0 + v2;
     return F;
   } else if (M == kGridZ) {
-    grid_z_eval(c.g, c.gc_n0, c.gc, pz, B);
+    grid_z_direct(c.g, pz, B);
   } else if (M == kOneUndulator) {
     und_eval(f0, px, py, pz, B);
   } else if (M == kFieldsEB) {
