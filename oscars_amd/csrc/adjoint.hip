@@ -554,24 +554,20 @@ __device__ const FieldDev* stage_fields(const FieldDev* __restrict__ F, int nf, 
   return Fl;
 }
 
-// Phase A: per segment, lambda_{i0} = M lambda_{i1} + b.
-// Eight lanes per segment: every lane of the group integrates the same states
-// and field derivatives; lane q < 6 forms column q of the step Jacobian
-// (J e_q through the four stages) and column q of M J^T, the full matrices
-// passing through LDS.  Each product is summed in the same order as a single
-// lane would, so the map is the same to the bit.
+// Phase A: per segment, lambda_{i0} = M lambda_{i1} + b with
+// M = J_{i0}^T J_{i0+1}^T ... J_{i1-1}^T (J_i the RK4 step's Jacobian) and
+// b = sum_i (J_{i0}^T ... J_{i-1}^T) c_i (c_i the knot's direct terms).  Row q of
+// M is then the tangent e_q carried forward through the segment's steps:
+// row q <- J_i row q.  Eight lanes per segment: every lane of the group
+// integrates the same states and field derivatives, and lane q < 6 carries row
+// q of M through each step's four stages and adds its share of b, so nothing
+// passes between the lanes.
 constexpr int kSegLanes = 8;
 
-struct SegLds {
-  double J[64 / kSegLanes][kSegLanes][6];  // per group: column q of J at [q]
-  double M[64 / kSegLanes][6][kSegLanes];  // per group: M[r][q]
-};
-
 __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
-  __shared__ SegLds sh;
   const int grp = threadIdx.x / kSegLanes;
   const int q = threadIdx.x % kSegLanes;
-  const int col = q < 6 ? q : 5;  // lanes 6 and 7 repeat column 5 and write nothing that is read
+  const int row = q < 6 ? q : 5;  // lanes 6 and 7 repeat row 5 and write nothing
   const long s = (long) blockIdx.x * (64 / kSegLanes) + grp;
   const bool live = s < A.nseg;
   const long ss = live ? s : A.nseg - 1;
@@ -584,13 +580,9 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
   long i0 = ss * A.K, i1 = i0 + A.K < A.nf ? i0 + A.K : A.nf;
   double y[6];
   for (int r = 0; r < 6; ++r) y[r] = A.ckpt[6 * ss + r];
-  double M[6][6], b[6];
-#pragma unroll
-  for (int r = 0; r < 6; ++r) {
-    b[r] = 0;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) M[r][m] = r == m ? 1.0 : 0.0;
-  }
+  double t[6];  // row `row` of M
+  for (int r = 0; r < 6; ++r) t[r] = r == row ? 1.0 : 0.0;
+  double bq = 0;  // b[row]
   double h = A.dt, hh = h * 0.5, h6 = h / 6.0;
   // kb rows of knots i and i + 1, read one step ahead
   double kc[9], kn[9];
@@ -612,12 +604,11 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
       kc[r] = kn[r];
       kn[r] = kf[r];
     }
+    {
+      double acc = bq;
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      double acc = b[r];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) acc += M[r][m] * c[m];
-      b[r] = acc;
+      for (int m = 0; m < 6; ++m) acc += t[m] * c[m];
+      bq = acc;
     }
     if (i == A.nf - 1) break;
     // stage states and their field derivatives
@@ -634,37 +625,17 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
     for (int r = 0; r < 6; ++r) y3[r] = y[r] + h * k3[r];
     FieldD f3;
     field_grad(F, A.nfields, A.gfield, y3, f3);
-    {  // column `col` of J: J e_col
-      double e[6];
-      for (int r = 0; r < 6; ++r) e[r] = r == col ? 1.0 : 0.0;
-      double d1[6], d2[6], d3[6], d4[6], t[6];
-      fjac_vec(f0, A.qomg, y, e, d1);
-      for (int r = 0; r < 6; ++r) t[r] = e[r] + hh * d1[r];
-      fjac_vec(f1, A.qomg, y1, t, d2);
-      for (int r = 0; r < 6; ++r) t[r] = e[r] + hh * d2[r];
-      fjac_vec(f2, A.qomg, y2, t, d3);
-      for (int r = 0; r < 6; ++r) t[r] = e[r] + h * d3[r];
-      fjac_vec(f3, A.qomg, y3, t, d4);
-      for (int r = 0; r < 6; ++r) sh.J[grp][q][r] = e[r] + h6 * (d1[r] + d4[r] + 2.0 * (d2[r] + d3[r]));
+    {  // row <- J row, through the four stages
+      double d1[6], d2[6], d3[6], d4[6], u[6];
+      fjac_vec(f0, A.qomg, y, t, d1);
+      for (int r = 0; r < 6; ++r) u[r] = t[r] + hh * d1[r];
+      fjac_vec(f1, A.qomg, y1, u, d2);
+      for (int r = 0; r < 6; ++r) u[r] = t[r] + hh * d2[r];
+      fjac_vec(f2, A.qomg, y2, u, d3);
+      for (int r = 0; r < 6; ++r) u[r] = t[r] + h * d3[r];
+      fjac_vec(f3, A.qomg, y3, u, d4);
+      for (int r = 0; r < 6; ++r) t[r] = t[r] + h6 * (d1[r] + d4[r] + 2.0 * (d2[r] + d3[r]));
     }
-    wave_sync();
-    // column `col` of M J^T: (M J^T)[r][col] = sum_m M[r][m] J[col][m], J[col][m] in column m's entry col
-    double Jrow[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) Jrow[m] = sh.J[grp][m][col];
-#pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      double acc = 0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) acc += M[r][m] * Jrow[m];
-      sh.M[grp][r][q] = acc;
-    }
-    wave_sync();
-#pragma unroll
-    for (int r = 0; r < 6; ++r)
-#pragma unroll
-      for (int m = 0; m < 6; ++m) M[r][m] = sh.M[grp][r][m];
-    wave_sync();  // every lane has read J and M before the next step writes them
     // the next state, as the forward computes it
     if (all_grid) {
       double k4[6];
@@ -674,15 +645,10 @@ __global__ void __launch_bounds__(64) adj_seg_map_kernel(AdjRkArgs A) {
       adj_step(F, A.nfields, A.qomg, A.dt, y);
     }
   }
-  if (!live) return;
+  if (!live || q >= 6) return;
   double* o = A.segmap + 42 * s;
-  if (q < 6)
-#pragma unroll
-    for (int r = 0; r < 6; ++r)  // M[r][q] by selects (no run-time index into the register array)
-      o[6 * r + q] = q == 0 ? M[r][0] : q == 1 ? M[r][1] : q == 2 ? M[r][2] : q == 3 ? M[r][3] : q == 4 ? M[r][4]
-                                                                                                   : M[r][5];
-  if (q == 0)
-    for (int r = 0; r < 6; ++r) o[36 + r] = b[r];
+  for (int m = 0; m < 6; ++m) o[6 * q + m] = t[m];
+  o[36 + q] = bq;
 }
 
 // Phase B: chain the segment maps from the end (lambda after the last knot is 0).
@@ -885,7 +851,7 @@ extern "C" hipError_t om_launch_adj_rk4(const om::AdjRkArgs* A, hipStream_t s) {
   long map_blocks = (A->nseg + 64 / om::kSegLanes - 1) / (64 / om::kSegLanes);
   // dynamic LDS for the staged fields: what the 160 KB of a CU leaves beside
   // the static arrays (one workgroup per CU; these kernels run few waves)
-  constexpr size_t kMapBytes = 160 * 1024 - sizeof(om::SegLds) - 1024;
+  constexpr size_t kMapBytes = 160 * 1024 - 1024;
   constexpr size_t kRevBytes = 160 * 1024 - 1024;
   om::AdjRkArgs R = *A;
   size_t map_bytes = kMapBytes, rev_bytes = kRevBytes;
